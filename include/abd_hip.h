@@ -1,0 +1,166 @@
+/*
+ * abd_hip.h — C ABI of the MI355X affine-body-dynamics (ABD) Newton step with IPC barriers.
+ *
+ * The reference path is the Python/Warp solver class `NewtonAbd` (abd.py:583-757) with its
+ * mixins `AbdComplex` (abd_simple.py:52-78), `ContactSolverBase` (contact.py:203-300) and
+ * `LineSearchInterface` (gauss_newton.py:87-144).  Each entry point below replaces one method or
+ * kernel launch of that class; the reference file:line is given beside it.  The Python class
+ * `colb_amd.abd.NewtonAbd` binds these through ctypes (see colb_amd/_lib.py).
+ *
+ * Conventions
+ *  - Every function returns 0 on success or a negative AbdStatus code; abd_last_error() gives
+ *    the message.  Calls return with the context's stream synchronised.
+ *  - Plain pointers are HOST pointers unless the name ends in `_dev`.
+ *  - Body DOF vector (12 doubles per body): [c(3), a0(3), a1(3), a2(3)], where a_k is row k of
+ *    the reference's q = A^T (quat_util.py:40-47), so a world vertex is x = c + sum_k xr_k a_k
+ *    (abd_simple.py:26-35).  Velocities use the same layout ([v, qdot rows]).
+ *  - A context owns its device buffers and one HIP stream; it is not thread-safe.
+ *  - Scenes: bodies carry a scene index (non-decreasing).  Scenes never interact; every
+ *    reduction (energy, residual, PCG scalars, line-search step) is per scene, so one context
+ *    solves many independent scenes in the same launches.
+ */
+#ifndef ABD_HIP_H
+#define ABD_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum AbdStatus {
+    ABD_OK = 0,
+    ABD_ERR_HIP = -1,          /* a HIP runtime call failed */
+    ABD_ERR_ARG = -2,          /* invalid argument or inconsistent scene */
+    ABD_ERR_CAPACITY = -3,     /* a caller buffer or contact capacity was too small */
+    ABD_ERR_NAN = -4,          /* NaN/Inf detected in state, gradient or step */
+    ABD_ERR_STATE = -5,        /* call order violated (e.g. solve before assemble) */
+};
+
+enum AbdPairKind { ABD_PT = 0, ABD_EE = 1, ABD_PG = 2 };
+
+/* Distance sub-type after closest-feature classification of a pair. */
+enum AbdDistType { ABD_D_PP = 0, ABD_D_PE = 1, ABD_D_PT = 2, ABD_D_EE = 3, ABD_D_PG = 4 };
+
+typedef struct AbdParams {
+    double dt;                 /* time step (links.py:16) */
+    double gravity[3];         /* gravity acceleration vector (abd.py:28, 166) */
+    double dhat;               /* barrier activation distance; s_hat = dhat^2 (barrier.py:4-5) */
+    double kappa;              /* orthogonality stiffness (ortho.py:5) */
+    double k_contact;          /* contact stiffness multiplying dt^2 * B(d^2) (abd.py:30) */
+    double newton_tol;         /* stop when ||grad||_2 < tol (abd.py:699) */
+    int    max_newton;         /* Newton iteration cap (abd.py:679) */
+    int    cg_max_iter;        /* PCG iteration cap */
+    double cg_tol;             /* PCG stop: ||r||_2 <= cg_tol * ||rhs||_2 */
+    int    ground_enabled;     /* static plane n.x = ground_offset */
+    int    ccd_enabled;        /* filter the line search by additive CCD */
+    double ground_normal[3];
+    double ground_offset;
+    double ccd_gap_ratio;      /* ACCD s: the returned TOI keeps >= s * initial gap */
+    int    ccd_max_iter;       /* ACCD iteration cap (conservative stop) */
+    int    precision;          /* 0 = all FP64; 1 = FP32 assembly/PCG, FP64 near-contact/CCD/energy */
+    double ls_floor;           /* line-search floor: stop halving when 0.5*alpha < ls_floor*alpha0
+                                  (gauss_newton.py:110, 1e-2) */
+    double pinned_diag;        /* Hessian diagonal of pinned bodies (abd.py:146, 100) */
+    int    max_candidates;     /* capacity of the candidate-pair list (grown on demand) */
+    int    reserved0;
+} AbdParams;
+
+typedef struct AbdSceneDesc {
+    int n_bodies, n_verts, n_edges, n_tris, n_scenes;
+    const double* rest_x;      /* [n_verts*3] body-frame rest positions (geometry.py:53-54) */
+    const int*    vert_body;   /* [n_verts] owning body (geometry.py:43-46) */
+    const int*    edges;       /* [n_edges*2] global vertex ids (geometry.py:45, 58) */
+    const int*    tris;        /* [n_tris*3] global vertex ids (geometry.py:47-48, 57) */
+    const int*    body_scene;  /* [n_bodies] scene index, non-decreasing */
+    const int*    body_pinned; /* [n_bodies] 1 = kinematically fixed (reference: mass <= 0) */
+    const double* mass4;       /* [n_bodies*16] row-major 4x4 mass matrix; the 12x12 inertia
+                                  block is mass4 (x) I3.  Reference: diag(m, J, J, J)
+                                  (rbd_simple.py:42-48, abd.py:77-85) */
+} AbdSceneDesc;
+
+typedef struct AbdStepStats {
+    int    newton_iters;       /* solves performed in the last step */
+    int    converged;          /* residual fell below newton_tol */
+    int    cg_iters;           /* PCG iterations summed over the step */
+    int    ls_floor_hits;      /* line searches that ended at the floor without decrease */
+    int    n_active;           /* active contacts (d < dhat) at the last assembly */
+    int    n_candidates;       /* candidate pairs of the last broad phase */
+    double residual_first;     /* ||grad|| at the first iterate */
+    double residual_last;      /* ||grad|| at the last assembled iterate */
+    double alpha_min;          /* smallest accepted line-search step */
+    double energy;             /* incremental potential at the end of the step */
+    double ccd_alpha_min;      /* smallest CCD bound seen */
+    double reserved;
+} AbdStepStats;
+
+typedef struct AbdCtx AbdCtx;
+
+/* NewtonAbd.__init__ (abd.py:584-599) + AbdComplex/AffineBodyBase setup (abd_simple.py:10-22,
+ * 53-62): uploads the rest geometry and mass data, allocates device state. */
+int abd_create(const AbdSceneDesc* scene, const AbdParams* params, AbdCtx** out);
+void abd_destroy(AbdCtx* ctx);
+const char* abd_last_error(const AbdCtx* ctx);
+int abd_get_params(const AbdCtx* ctx, AbdParams* out);
+int abd_set_params(AbdCtx* ctx, const AbdParams* params);
+
+/* AbdComplex.reset / init kernel (abd_simple.py:37-49, 64-69): sets the state.  q is the current
+ * iterate (reference BDFAffine.nxt), q_now and qdot the start-of-step state (BDFAffine.now). */
+int abd_set_state(AbdCtx* ctx, const double* q, const double* q_now, const double* qdot);
+int abd_get_state(const AbdCtx* ctx, double* q, double* q_now, double* qdot);
+
+/* NewtonAbd.step (abd.py:674-702): nsteps full time steps, device resident.  stats (may be
+ * NULL) receives n_scenes entries for the last step. */
+int abd_step(AbdCtx* ctx, int nsteps, AbdStepStats* stats);
+
+/* Profiller arrays for the last step (abd.py:695-696, utils/data_collector.py:8-9):
+ * residual[s*max_newton + it], alpha[...], cg_iters[...]; unused entries are 0. */
+int abd_get_iter_log(const AbdCtx* ctx, double* residual, double* alpha, int* cg_iters);
+
+/* ContactSolverBase.detect_collision (contact.py:270-279): broad + narrow phase at the current
+ * iterate.  Writes the active pairs (d < dhat) as sorted (kind, i, j) triples and their d^2.
+ * cap = capacity in pairs; *n receives the count even when it exceeds cap (then
+ * ABD_ERR_CAPACITY). */
+int abd_detect(AbdCtx* ctx, int* pairs, double* d2, int cap, int* n);
+
+/* inertia_grad_hess + contact_hessian_ee launches and the BSR build (abd.py:682-688, 634):
+ * assembles gradient and Hessian at the current iterate.  Outputs (any may be NULL):
+ * grad[n_bodies*12]; diag[n_bodies*144] row-major 12x12 blocks; pair_ij[2*cap] and
+ * offdiag[cap*144] for the upper blocks (i < j); *n_pairs; energy[n_scenes]. */
+int abd_assemble(AbdCtx* ctx, double* grad, double* diag, int* pair_ij, double* offdiag,
+                 int cap, int* n_pairs, double* energy);
+
+/* compute_du (abd.py:631-658): solves H du = grad with block-Jacobi PCG on the last assembled
+ * system.  du[n_bodies*12] (may be NULL), residual[n_scenes] = ||grad||_2, cg_iters[n_scenes]. */
+int abd_solve(AbdCtx* ctx, double* du, double* residual, int* cg_iters);
+
+/* compute_g (abd.py:725-735): incremental potential at q (host, [n_bodies*12]; NULL = current
+ * iterate), energy[n_scenes]. */
+int abd_energy(AbdCtx* ctx, const double* q, double* energy);
+
+/* Additive CCD bound of the step q -> q - du over swept candidates: alpha[n_scenes] in (0, 1]. */
+int abd_ccd(AbdCtx* ctx, const double* du, double* alpha);
+
+/* line_search_iterative (gauss_newton.py:98-115) with the CCD filter: uses the last solve's du,
+ * applies q -= alpha*du per scene, returns alpha[n_scenes]. */
+int abd_line_search(AbdCtx* ctx, double* alpha);
+
+/* forward_states (abd.py:574-581, 670-672): qdot = (q - q_now)/dt, q_now = q. */
+int abd_forward(AbdCtx* ctx);
+
+/* compute_V (abd_simple.py:26-35, 71-76): world vertex positions xyz[n_verts*3]. */
+int abd_get_vertices(AbdCtx* ctx, double* xyz);
+
+/* get_contact_points (abd.py:748-766): midpoints[cap*3] of the closest points and distances
+ * dist[cap] of the active pairs, in abd_detect order. */
+int abd_get_contact_points(AbdCtx* ctx, double* midpoints, double* dist, int cap, int* n);
+
+/* Per-kernel timing of the last abd_step (HIP events on the context stream): names are
+ * '\n'-separated into names[name_cap]; ms[k] = summed milliseconds, calls[k], bytes[k] =
+ * algorithmic bytes moved; *n = number of kernels. */
+int abd_get_kernel_times(const AbdCtx* ctx, char* names, int name_cap, double* ms, int* calls,
+                         double* bytes, int cap, int* n);
+int abd_enable_timing(AbdCtx* ctx, int on);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ABD_HIP_H */
