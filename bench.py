@@ -1,0 +1,136 @@
+"""Benchmark: simulated milliseconds per wall-clock millisecond of the ABD Newton step on one
+MI355X, on the 32x32 torus chain net (1024 links of 288 verts / 576 tris, four corners pinned,
+FP64, dt 8 ms, dhat 1e-3, kappa 1e7, 16 Newton iterations, tolerance 1e-5).
+
+Prints one JSON line (see the `result` dict).  The CPU baseline is the project's FP64 oracle
+(oracle/, "port") timed on this host on a bounded sample of the same workload: it starts from the
+GPU's state at the beginning of the timed region and runs whole steps until ~20 s have passed.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "sim-ms per wall-ms + Newton iters to 1e-5 residual, learned vs plain ABD"
+
+
+def workload(name):
+    from colb_amd import scene as S
+    from colb_amd.params import north_star
+    if name == "chain_net_32x32":
+        sc = S.chain_net(n=32)
+        p = north_star(dt=8e-3, dhat=1e-3, kappa=1e7, k_contact=1e20, max_newton=16, newton_tol=1e-5)
+        return sc, p, {"workload": "chain_net_32x32", "links": 1024, "verts_per_link": 288,
+                       "tris_per_link": 576, "pinned": 4, "dt_ms": 8.0, "dhat": 1e-3, "kappa": 1e7,
+                       "k_contact": 1e20, "max_newton": 16, "newton_tol": 1e-5, "precision": "fp64"}
+    if name == "cubes_8":
+        sc = S.cubes_on_ground()
+        p = north_star(dt=8e-3, ground_enabled=True, dhat=1e-3, kappa=1e7, k_contact=1e20)
+        return sc, p, {"workload": "cubes_8_on_ground", "bodies": 8, "dt_ms": 8.0, "dhat": 1e-3, "precision": "fp64"}
+    raise SystemExit(f"unknown workload {name}")
+
+
+def cpu_baseline(sc, p, state, budget_s, threads):
+    from oracle import oracle as O
+    O.build(force=False)
+    orc = O.Oracle(sc, p, threads=threads)
+    orc.set_state(*state)
+    t0 = time.perf_counter()
+    steps, iters = 0, 0
+    while True:
+        st = orc.step()[0]
+        steps += 1
+        iters += st["newton_iters"]
+        if time.perf_counter() - t0 > budget_s:
+            break
+    wall = time.perf_counter() - t0
+    return {"value": steps * p.dt * 1e3 / (wall * 1e3), "unit": "sim-ms/wall-ms", "cores": threads,
+            "kind": "port", "wall_s": round(wall, 3), "newton_iters_per_step": iters / steps,
+            "sample": f"{steps} step(s) of the same workload from the GPU's state at the start of the timed "
+                      f"region, FP64 oracle, OpenMP over {threads} threads"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="chain_net_32x32")
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--profile-steps", type=int, default=2)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build targets one GPU; --gpus must be 1 (multi-GPU sharding is design-only)"}))
+        sys.exit(2)
+
+    from colb_amd.abd import NewtonAbd
+    sc, p, cfg = workload(args.workload)
+    print(f"# workload {cfg['workload']}: {sc.n_bodies} bodies, {sc.n_verts} verts, {sc.n_tris} tris, "
+          f"{sc.n_edges} edges", file=sys.stderr, flush=True)
+    solver = NewtonAbd(p.dt, sc, params=p)
+    for k in range(args.warmup):
+        st = solver.step()[0]
+        print(f"# warmup {k}: {st}", file=sys.stderr, flush=True)
+    state0 = solver.get_state()
+    iters, cg = [], []
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        st = solver.step()[0]
+        iters.append(st["newton_iters"])
+        cg.append(st["cg_iters"])
+    t1 = time.perf_counter()
+    wall_ms = (t1 - t0) * 1e3
+    ms_per_step = wall_ms / args.steps
+    sim_per_wall = args.steps * p.dt * 1e3 / wall_ms
+    print(f"# timed: {ms_per_step:.2f} ms/step, newton {np.mean(iters):.2f}, cg {np.mean(cg):.1f}, "
+          f"last {st}", file=sys.stderr, flush=True)
+
+    # per-kernel HIP-event timing over a few extra steps (kept out of the timed region)
+    solver.enable_timing(True)
+    solver.step(args.profile_steps)
+    kt = solver.kernel_times()
+    solver.enable_timing(False)
+    single = ["pcg_spmv", "pcg_update", "narrow_phase", "contact_hess", "offblock_sum", "body_assemble",
+              "ccd", "energy"]
+    dom = max((k for k in kt if k in single), key=lambda k: kt[k]["ms"], default=None)
+    roof = None
+    if dom:
+        d = kt[dom]
+        avg_ms = d["ms"] / d["calls"]
+        bytes_per = d["bytes"] / d["calls"]
+        ach = bytes_per / (avg_ms * 1e-3) / 1e9
+        roof = {"bound": "hbm", "kernel": dom, "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK_GBS, 5), "traffic": None, "avg_us": round(avg_ms * 1e3, 3),
+                "bytes_per_launch": bytes_per}
+    for k, v in kt.items():
+        print(f"# kernel {k:20s} {v['ms']:9.3f} ms {v['calls']:6d} calls {v['bytes'] / max(v['ms'], 1e-9) / 1e6:8.1f} GB/s",
+              file=sys.stderr, flush=True)
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = len(os.sched_getaffinity(0))
+        cpu = cpu_baseline(sc, p, state0, args.cpu_budget, threads)
+        print(f"# cpu baseline: {cpu}", file=sys.stderr, flush=True)
+
+    result = {
+        "metric": METRIC, "value": round(sim_per_wall, 6), "unit": "sim-ms/wall-ms", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (procedural scene)", "config": cfg,
+        "newton_iters_per_step": round(float(np.mean(iters)), 3), "cg_iters_per_step": round(float(np.mean(cg)), 1),
+        "method": "plain",
+        "roofline": roof, "cpu_baseline": cpu,
+    }
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

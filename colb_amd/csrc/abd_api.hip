@@ -1,0 +1,1161 @@
+// abd_api.hip — host side of the MI355X ABD Newton step: the context, the device-resident
+// Newton loop and the C ABI declared in include/abd_hip.h.
+//
+// One Newton iteration (NewtonAbd.step, abd.py:680-699):
+//   assemble  narrow phase over the candidate list -> active contacts -> per-contact 12x12 blocks
+//             -> body-pair blocks (sorted keys, fixed summation order) -> per-body diagonal blocks,
+//             gradient, block-Jacobi inverse and energy
+//   solve     tiled block-Jacobi PCG, per-scene scalars
+//   CCD       swept LBVH broad phase q -> q - du (also the candidate list of the next iteration)
+//             + additive CCD -> per-scene step bound
+//   line search  halving from the CCD bound until the energy decreases (floor 1/64)
+// Host synchronisations per iteration: counts that size the sorts, PCG convergence polls and the
+// line-search/Newton flags.
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+
+#include <hip/hip_runtime.h>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "abd_internal.hpp"
+
+using namespace abd;
+
+namespace {
+
+struct Timer {
+    bool on = false;
+    struct Rec {
+        std::string name;
+        hipEvent_t a, b;
+        double bytes;
+    };
+    std::vector<Rec> open;
+    std::vector<hipEvent_t> pool;
+    std::map<std::string, double> ms, bytes;
+    std::map<std::string, int> calls;
+    std::vector<std::string> order;
+    hipEvent_t get() {
+        if (!pool.empty()) {
+            hipEvent_t e = pool.back();
+            pool.pop_back();
+            return e;
+        }
+        hipEvent_t e;
+        (void)hipEventCreate(&e);
+        return e;
+    }
+};
+
+}  // namespace
+
+struct AbdCtx {
+    AbdParams P;
+    int nb = 0, nv = 0, ne = 0, nt = 0, ns = 0;
+    hipStream_t st = nullptr;
+    std::string err;
+    std::vector<void*> allocs;
+    Geo G;
+    // host copies of the scene tables
+    std::vector<int> h_scene_b0, h_scene_v0, h_scene_e0, h_scene_t0, h_tile_b0, h_scene_tile0;
+    int ntiles = 0;
+    // static tables
+    double4* rest = nullptr;
+    int2* edges = nullptr;
+    int4* tris = nullptr;
+    double* mass4 = nullptr;
+    int *pinned = nullptr, *bscene = nullptr, *scene_b0 = nullptr, *scene_v0 = nullptr, *scene_e0 = nullptr,
+        *scene_t0 = nullptr;
+    int *tile_b0 = nullptr, *tile_b1 = nullptr, *tile_scene = nullptr, *scene_tile0 = nullptr;
+    // state
+    double *q = nullptr, *qnow = nullptr, *qdot = nullptr, *qtilde = nullptr, *qtmp = nullptr;
+    double *du = nullptr, *grad = nullptr, *diag = nullptr, *pinv = nullptr, *e_body = nullptr;
+    double4 *X0 = nullptr, *X1 = nullptr, *Xt = nullptr;
+    float* bounds = nullptr;
+    // LBVH (0 = triangles, 1 = edges)
+    struct Tree {
+        int n = 0;
+        float4* boxes = nullptr;
+        uint64_t *keys = nullptr, *skeys = nullptr;
+        int *idx = nullptr, *sidx = nullptr;
+        BvhNode* nodes = nullptr;
+        int *parent = nullptr, *leaf_parent = nullptr, *flags = nullptr, *root = nullptr;
+    } tree[2];
+    // candidates
+    int cand_cap = 0, n_cand = 0;
+    uint64_t *cand = nullptr, *cand_s = nullptr, *act = nullptr;
+    int *flag = nullptr, *pos = nullptr, *cand_lo = nullptr, *cand_hi = nullptr;
+    double* e_cand = nullptr;
+    bool cand_valid = false;
+    // active contacts
+    int act_cap = 0, n_act = 0;
+    int2* act_body = nullptr;
+    double *act_g = nullptr, *act_H = nullptr;
+    uint64_t *inc = nullptr, *inc_s = nullptr, *pk = nullptr, *pk_s = nullptr;
+    int *head = nullptr, *hpos = nullptr, *pair_b = nullptr, *run_start = nullptr;
+    int n_inc = 0, n_pk = 0, n_pairs = 0;
+    // BSR
+    int pair_cap = 0;
+    double *off = nullptr, *offT = nullptr;
+    uint64_t *ent_k = nullptr, *ent_ks = nullptr;
+    int *ent_v = nullptr, *ent_vs = nullptr, *row_ptr = nullptr;
+    // PCG
+    double *r = nullptr, *z = nullptr, *p0 = nullptr, *p1 = nullptr, *Ap = nullptr, *part = nullptr, *bnorm = nullptr;
+    int* cg_iters = nullptr;
+    int* cg_done = nullptr;
+    // per-scene control
+    SceneCtl C;
+    int* nact_s = nullptr;
+    int* act_before = nullptr;
+    double* report = nullptr;
+    int* cnt = nullptr;       // device counters
+    int* h_cnt = nullptr;     // pinned host mirror
+    double* h_report = nullptr;
+    // sort scratch
+    void* sort_tmp = nullptr;
+    size_t sort_bytes = 0;
+    // logs
+    std::vector<double> log_res, log_alpha;
+    std::vector<int> log_cg;
+    std::vector<AbdStepStats> stats;
+    bool assembled = false, solved = false;
+    Timer T;
+};
+
+namespace {
+
+enum { C_COUNT = 0, C_OVERFLOW = 1, C_NINC = 2, C_NPK = 3, C_NUNIQ = 4, C_ANY = 5, C_PENDING = 6, C_NACT = 7, C_ERR = 8,
+       C_N = 9 };
+enum { R_RES = 0, R_ALPHA, R_ALPHA0, R_FLOOR, R_ACTIVE, R_CG, R_E1, R_NACT, R_NCAND, R_N };
+
+#define HIPCHK(expr)                                                                               \
+    do {                                                                                           \
+        hipError_t _e = (expr);                                                                    \
+        if (_e != hipSuccess) {                                                                    \
+            ctx->err = std::string(#expr) + ": " + hipGetErrorName(_e) + " " + hipGetErrorString(_e); \
+            return ABD_ERR_HIP;                                                                    \
+        }                                                                                          \
+    } while (0)
+
+#define CHK(expr)                                                                                  \
+    do {                                                                                           \
+        int _rc = (expr);                                                                          \
+        if (_rc != ABD_OK) return _rc;                                                             \
+    } while (0)
+
+template <class T>
+int dalloc(AbdCtx* ctx, T** p, size_t n) {
+    n = std::max<size_t>(n, 1);
+    HIPCHK(hipMalloc((void**)p, n * sizeof(T)));
+    ctx->allocs.push_back((void*)*p);
+    return ABD_OK;
+}
+
+template <class T>
+int dfree(AbdCtx* ctx, T*& p) {
+    if (p) {
+        auto it = std::find(ctx->allocs.begin(), ctx->allocs.end(), (void*)p);
+        if (it != ctx->allocs.end()) ctx->allocs.erase(it);
+        HIPCHK(hipFree(p));
+        p = nullptr;
+    }
+    return ABD_OK;
+}
+
+void tic(AbdCtx* ctx, const char* name, double bytes = 0.0) {
+    if (!ctx->T.on) return;
+    Timer::Rec r{name, ctx->T.get(), ctx->T.get(), bytes};
+    (void)hipEventRecord(r.a, ctx->st);
+    ctx->T.open.push_back(r);
+}
+
+void toc(AbdCtx* ctx) {
+    if (!ctx->T.on || ctx->T.open.empty()) return;
+    (void)hipEventRecord(ctx->T.open.back().b, ctx->st);
+}
+
+void timer_collect(AbdCtx* ctx) {
+    if (!ctx->T.on) return;
+    (void)hipStreamSynchronize(ctx->st);
+    for (auto& r : ctx->T.open) {
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, r.a, r.b);
+        if (!ctx->T.calls.count(r.name)) ctx->T.order.push_back(r.name);
+        ctx->T.ms[r.name] += ms;
+        ctx->T.calls[r.name] += 1;
+        ctx->T.bytes[r.name] += r.bytes;
+        ctx->T.pool.push_back(r.a);
+        ctx->T.pool.push_back(r.b);
+    }
+    ctx->T.open.clear();
+}
+
+// --------------------------------------------------------------------------------------------
+// small helper kernels
+// --------------------------------------------------------------------------------------------
+__global__ void k_fill_i32(int n, int* p, int v) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+__global__ void k_fill_u64(int n, unsigned long long* p, unsigned long long v) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+__global__ void k_nact_scene(int ns, const int* lo, const int* hi, const int* flag, const int* pos, int* out) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= ns) return;
+    int a = lo[s], b = hi[s];
+    out[s] = (b > a) ? (pos[b - 1] + flag[b - 1] - pos[a]) : 0;
+}
+__global__ void k_report(int ns, SceneCtl c, const int* cg, const int* nact, const int* lo, const int* hi,
+                         const int* active_before, double* rep) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= ns) return;
+    double* R = rep + (size_t)s * R_N;
+    R[R_RES] = c.residual[s];
+    R[R_ALPHA] = c.alpha[s];
+    R[R_ALPHA0] = c.alpha0[s];
+    R[R_FLOOR] = c.ls_floor[s];
+    R[R_ACTIVE] = active_before[s];
+    R[R_CG] = cg[s];
+    R[R_E1] = c.E1[s];
+    R[R_NACT] = nact[s];
+    R[R_NCAND] = hi[s] - lo[s];
+}
+__global__ void k_copy(int n, const double* a, double* b) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = a[i];
+}
+
+inline int cdiv(long a, int b) { return (int)((a + b - 1) / b); }
+
+// Debug mode (AbdParams.reserved0 bit 0): synchronise after every launch group and report the
+// first failing one by name, with the device error word.
+int dbg(AbdCtx* ctx, const char* what) {
+    if (!(ctx->P.reserved0 & 1)) return ABD_OK;
+    hipError_t e = hipStreamSynchronize(ctx->st);
+    if (e == hipSuccess) e = hipGetLastError();
+    if (e != hipSuccess) {
+        ctx->err = std::string("after ") + what + ": " + hipGetErrorName(e) + " " + hipGetErrorString(e);
+        return ABD_ERR_HIP;
+    }
+    int w = 0;
+    if (hipMemcpy(&w, ctx->cnt + C_ERR, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) return ABD_ERR_HIP;
+    if (w) {
+        char buf[160];
+        std::snprintf(buf, sizeof buf, "after %s: device index check failed (error bits 0x%x)", what, w);
+        ctx->err = buf;
+        return ABD_ERR_STATE;
+    }
+    if (ctx->P.reserved0 & 2) { std::fprintf(stderr, "[abd] ok %s\n", what); std::fflush(stderr); }
+    return ABD_OK;
+}
+
+int fill_i32(AbdCtx* ctx, int* p, int n, int v) {
+    if (n <= 0) return ABD_OK;
+    hipLaunchKernelGGL(k_fill_i32, dim3(cdiv(n, 256)), dim3(256), 0, ctx->st, n, p, v);
+    CHK(dbg(ctx, "k_fill_i32"));
+    HIPCHK(hipGetLastError());
+    return ABD_OK;
+}
+
+// --------------------------------------------------------------------------------------------
+// sort / scan (abd_sort.hip) with scratch grown on demand
+// --------------------------------------------------------------------------------------------
+int ensure_tmp(AbdCtx* ctx, size_t bytes) {
+    if (bytes <= ctx->sort_bytes) return ABD_OK;
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    CHK(dfree(ctx, ctx->sort_tmp));
+    size_t nb = bytes + bytes / 4 + 4096;
+    HIPCHK(hipMalloc(&ctx->sort_tmp, nb));
+    ctx->allocs.push_back(ctx->sort_tmp);
+    ctx->sort_bytes = nb;
+    return ABD_OK;
+}
+
+int sort_keys(AbdCtx* ctx, const uint64_t* in, uint64_t* out, int n, int end_bit = 64) {
+    if (n <= 0) return ABD_OK;
+    CHK(ensure_tmp(ctx, radix_scratch_bytes(n)));
+    HIPCHK(radix_sort64(ctx->st, in, out, nullptr, nullptr, n, end_bit, ctx->sort_tmp));
+    return dbg(ctx, "radix_sort64(keys)");
+}
+
+int sort_pairs(AbdCtx* ctx, const uint64_t* kin, uint64_t* kout, const int* vin, int* vout, int n, int end_bit = 64) {
+    if (n <= 0) return ABD_OK;
+    CHK(ensure_tmp(ctx, radix_scratch_bytes(n)));
+    HIPCHK(radix_sort64(ctx->st, kin, kout, vin, vout, n, end_bit, ctx->sort_tmp));
+    return dbg(ctx, "radix_sort64(pairs)");
+}
+
+int excl_scan(AbdCtx* ctx, const int* in, int* out, int n) {
+    if (n <= 0) return ABD_OK;
+    CHK(ensure_tmp(ctx, sizeof(int) * (size_t)scan_scratch_ints(n)));
+    HIPCHK(excl_scan_int(ctx->st, in, out, n, (int*)ctx->sort_tmp));
+    return dbg(ctx, "excl_scan_int");
+}
+
+int sync_counters(AbdCtx* ctx) {
+    HIPCHK(hipMemcpyAsync(ctx->h_cnt, ctx->cnt, C_N * sizeof(int), hipMemcpyDeviceToHost, ctx->st));
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    if (ctx->h_cnt[C_ERR]) {
+        char buf[128];
+        std::snprintf(buf, sizeof buf, "device index check failed (error bits 0x%x)", ctx->h_cnt[C_ERR]);
+        ctx->err = buf;
+        return ABD_ERR_STATE;
+    }
+    return ABD_OK;
+}
+
+int bits_for(int n) {
+    int b = 1;
+    while ((1 << b) < n) ++b;
+    return b;
+}
+
+// --------------------------------------------------------------------------------------------
+// capacities
+// --------------------------------------------------------------------------------------------
+int grow_candidates(AbdCtx* ctx, int cap) {
+    if (cap <= ctx->cand_cap) return ABD_OK;
+    cap = std::max(cap, ctx->cand_cap * 2);
+    CHK(dfree(ctx, ctx->cand)); CHK(dfree(ctx, ctx->cand_s)); CHK(dfree(ctx, ctx->act));
+    CHK(dfree(ctx, ctx->flag)); CHK(dfree(ctx, ctx->pos)); CHK(dfree(ctx, ctx->e_cand));
+    CHK(dalloc(ctx, &ctx->cand, cap)); CHK(dalloc(ctx, &ctx->cand_s, cap)); CHK(dalloc(ctx, &ctx->act, cap));
+    CHK(dalloc(ctx, &ctx->flag, cap)); CHK(dalloc(ctx, &ctx->pos, cap)); CHK(dalloc(ctx, &ctx->e_cand, cap));
+    ctx->cand_cap = cap;
+    return ABD_OK;
+}
+
+int grow_active(AbdCtx* ctx, int cap) {
+    if (cap <= ctx->act_cap) return ABD_OK;
+    cap = std::max(cap, ctx->act_cap * 2);
+    if (cap > (1 << 24)) {
+        ctx->err = "more than 2^24 active contacts";
+        return ABD_ERR_CAPACITY;
+    }
+    CHK(dfree(ctx, ctx->act_body)); CHK(dfree(ctx, ctx->act_g)); CHK(dfree(ctx, ctx->act_H));
+    CHK(dfree(ctx, ctx->inc)); CHK(dfree(ctx, ctx->inc_s)); CHK(dfree(ctx, ctx->pk)); CHK(dfree(ctx, ctx->pk_s));
+    CHK(dfree(ctx, ctx->head)); CHK(dfree(ctx, ctx->hpos)); CHK(dfree(ctx, ctx->pair_b)); CHK(dfree(ctx, ctx->run_start));
+    CHK(dalloc(ctx, &ctx->act_body, cap)); CHK(dalloc(ctx, &ctx->act_g, (size_t)cap * 24));
+    CHK(dalloc(ctx, &ctx->act_H, (size_t)cap * 432));
+    CHK(dalloc(ctx, &ctx->inc, 2 * (size_t)cap)); CHK(dalloc(ctx, &ctx->inc_s, 2 * (size_t)cap));
+    CHK(dalloc(ctx, &ctx->pk, cap)); CHK(dalloc(ctx, &ctx->pk_s, cap));
+    CHK(dalloc(ctx, &ctx->head, cap)); CHK(dalloc(ctx, &ctx->hpos, cap));
+    CHK(dalloc(ctx, &ctx->pair_b, 2 * (size_t)cap)); CHK(dalloc(ctx, &ctx->run_start, (size_t)cap + 1));
+    ctx->act_cap = cap;
+    return ABD_OK;
+}
+
+int grow_pairs(AbdCtx* ctx, int cap) {
+    if (cap <= ctx->pair_cap) return ABD_OK;
+    cap = std::max(cap, ctx->pair_cap * 2);
+    CHK(dfree(ctx, ctx->off)); CHK(dfree(ctx, ctx->offT)); CHK(dfree(ctx, ctx->ent_k)); CHK(dfree(ctx, ctx->ent_ks));
+    CHK(dfree(ctx, ctx->ent_v)); CHK(dfree(ctx, ctx->ent_vs));
+    CHK(dalloc(ctx, &ctx->off, (size_t)cap * 144)); CHK(dalloc(ctx, &ctx->offT, (size_t)cap * 144));
+    CHK(dalloc(ctx, &ctx->ent_k, 2 * (size_t)cap)); CHK(dalloc(ctx, &ctx->ent_ks, 2 * (size_t)cap));
+    CHK(dalloc(ctx, &ctx->ent_v, 2 * (size_t)cap)); CHK(dalloc(ctx, &ctx->ent_vs, 2 * (size_t)cap));
+    ctx->pair_cap = cap;
+    return ABD_OK;
+}
+
+// --------------------------------------------------------------------------------------------
+// broad phase: LBVH over triangles and edges of the swept positions X0 -> X1
+// (ContactSolverBase.update_bvh + edge_edge_collision, contact.py:244-279)
+// --------------------------------------------------------------------------------------------
+int build_tree(AbdCtx* ctx, int which, const double4* X0, const double4* X1, double infl) {
+    AbdCtx::Tree& T = ctx->tree[which];
+    if (T.n == 0) return ABD_OK;
+    launch_prim_boxes(ctx->st, ctx->G, which == 0 ? 1 : 0, X0, X1, infl, ctx->bounds, T.boxes, T.keys, T.idx);
+    CHK(dbg(ctx, "launch_prim_boxes"));
+    int end_bit = 32 + bits_for(ctx->ns + 1);
+    CHK(sort_pairs(ctx, T.keys, T.skeys, T.idx, T.sidx, T.n, end_bit));
+    CHK(fill_i32(ctx, T.root, ctx->ns, INT_MIN));
+    launch_bvh_build(ctx->st, T.n, ctx->ns, T.skeys, T.nodes, T.parent, T.leaf_parent, T.root);
+    CHK(dbg(ctx, "launch_bvh_build"));
+    if (T.n > 1) HIPCHK(hipMemsetAsync(T.flags, 0, sizeof(int) * (T.n - 1), ctx->st));
+    launch_bvh_refit(ctx->st, T.n, T.sidx, T.boxes, T.nodes, T.parent, T.leaf_parent, T.flags, ctx->cnt + C_ERR);
+    CHK(dbg(ctx, "launch_bvh_refit"));
+    HIPCHK(hipGetLastError());
+    return ABD_OK;
+}
+
+int broad_phase(AbdCtx* ctx, const double4* X0, const double4* X1) {
+    const double infl = ctx->P.dhat;
+    tic(ctx, "broad_phase_build");
+    launch_scene_bounds(ctx->st, ctx->G, X0, X1, ctx->bounds);
+    CHK(dbg(ctx, "launch_scene_bounds"));
+    CHK(build_tree(ctx, 0, X0, X1, 0.0));
+    CHK(build_tree(ctx, 1, X0, X1, 0.5 * infl));
+    toc(ctx);
+    for (int attempt = 0; attempt < 8; ++attempt) {
+        tic(ctx, "broad_phase_query");
+        HIPCHK(hipMemsetAsync(ctx->cnt + C_COUNT, 0, 2 * sizeof(int), ctx->st));
+        launch_query_pt(ctx->st, ctx->G, X0, X1, infl, ctx->P, ctx->tree[0].nodes, ctx->tree[0].sidx,
+                        ctx->tree[0].root, ctx->tree[0].boxes, ctx->cand, ctx->cand_cap, ctx->cnt + C_COUNT,
+                        ctx->cnt + C_OVERFLOW);
+        CHK(dbg(ctx, "launch_query_pt"));
+        launch_query_ee(ctx->st, ctx->G, ctx->tree[1].nodes, ctx->tree[1].sidx, ctx->tree[1].root,
+                        ctx->tree[1].boxes, ctx->cand, ctx->cand_cap, ctx->cnt + C_COUNT, ctx->cnt + C_OVERFLOW);
+        CHK(dbg(ctx, "launch_query_ee"));
+        HIPCHK(hipGetLastError());
+        toc(ctx);
+        CHK(sync_counters(ctx));
+        if (ctx->h_cnt[C_OVERFLOW] == 2) {
+            ctx->err = "LBVH traversal stack overflow";
+            return ABD_ERR_CAPACITY;
+        }
+        int n = ctx->h_cnt[C_COUNT];
+        if (n <= ctx->cand_cap) {
+            ctx->n_cand = n;
+            break;
+        }
+        CHK(grow_candidates(ctx, n + n / 2));
+    }
+    tic(ctx, "candidate_sort");
+    CHK(sort_keys(ctx, ctx->cand, ctx->cand_s, ctx->n_cand, 48 + bits_for(ctx->ns + 1)));
+    HIPCHK(hipMemsetAsync(ctx->cand_lo, 0, sizeof(int) * ctx->ns, ctx->st));
+    HIPCHK(hipMemsetAsync(ctx->cand_hi, 0, sizeof(int) * ctx->ns, ctx->st));
+    launch_cand_ranges(ctx->st, ctx->n_cand, ctx->cand_s, ctx->cand_lo, ctx->cand_hi);
+    CHK(dbg(ctx, "launch_cand_ranges"));
+    HIPCHK(hipGetLastError());
+    toc(ctx);
+    return ABD_OK;
+}
+
+int static_broad_phase(AbdCtx* ctx) {
+    launch_xform(ctx->st, ctx->G, ctx->q, nullptr, nullptr, ctx->X0);
+    CHK(dbg(ctx, "launch_xform"));
+    CHK(broad_phase(ctx, ctx->X0, ctx->X0));
+    ctx->cand_valid = true;
+    return ABD_OK;
+}
+
+// --------------------------------------------------------------------------------------------
+// assembly at the current iterate over the current candidate list
+// --------------------------------------------------------------------------------------------
+int assemble(AbdCtx* ctx) {
+    const int nc = ctx->n_cand;
+    launch_xform(ctx->st, ctx->G, ctx->q, nullptr, nullptr, ctx->X0);
+    CHK(dbg(ctx, "launch_xform"));
+    tic(ctx, "narrow_phase", (double)nc * (8 + 4 * 32 + 4 + 8));
+    launch_cand_eval(ctx->st, ctx->G, nc, ctx->cand_s, ctx->X0, ctx->P, ctx->flag, ctx->e_cand);
+    CHK(dbg(ctx, "launch_cand_eval"));
+    toc(ctx);
+    CHK(excl_scan(ctx, ctx->flag, ctx->pos, nc));
+    if (nc > 0) {
+        HIPCHK(hipMemcpyAsync(ctx->cnt + C_NACT, ctx->pos + nc - 1, sizeof(int), hipMemcpyDeviceToDevice, ctx->st));
+        HIPCHK(hipMemcpyAsync(ctx->cnt + C_ANY, ctx->flag + nc - 1, sizeof(int), hipMemcpyDeviceToDevice, ctx->st));
+    }
+    hipLaunchKernelGGL(k_nact_scene, dim3(cdiv(ctx->ns, 128)), dim3(128), 0, ctx->st, ctx->ns, ctx->cand_lo,
+                       ctx->cand_hi, ctx->flag, ctx->pos, ctx->nact_s);
+    CHK(dbg(ctx, "k_nact_scene"));
+    CHK(sync_counters(ctx));
+    int na = nc > 0 ? ctx->h_cnt[C_NACT] + ctx->h_cnt[C_ANY] : 0;
+    ctx->n_act = na;
+    CHK(grow_active(ctx, na));
+    launch_compact(ctx->st, nc, ctx->flag, ctx->pos, ctx->cand_s, ctx->act);
+    CHK(dbg(ctx, "launch_compact"));
+    tic(ctx, "contact_hess", (double)na * (8 + 4 * 32 + 432 * 8 + 24 * 8 + 8));
+    launch_contact_hess(ctx->st, ctx->G, na, ctx->act, ctx->X0, ctx->P, ctx->act_body, ctx->act_g, ctx->act_H);
+    CHK(dbg(ctx, "launch_contact_hess"));
+    toc(ctx);
+    HIPCHK(hipMemsetAsync(ctx->cnt + C_NINC, 0, 2 * sizeof(int), ctx->st));
+    launch_incidence(ctx->st, ctx->G, na, ctx->act_body, ctx->inc, ctx->cnt + C_NINC, ctx->pk, ctx->cnt + C_NPK);
+    CHK(dbg(ctx, "launch_incidence"));
+    HIPCHK(hipGetLastError());
+    CHK(sync_counters(ctx));
+    ctx->n_inc = ctx->h_cnt[C_NINC];
+    ctx->n_pk = ctx->h_cnt[C_NPK];
+    tic(ctx, "block_sort");
+    CHK(sort_keys(ctx, ctx->inc, ctx->inc_s, ctx->n_inc));
+    CHK(sort_keys(ctx, ctx->pk, ctx->pk_s, ctx->n_pk));
+    toc(ctx);
+    ctx->n_pairs = 0;
+    if (ctx->n_pk > 0) {
+        launch_run_heads(ctx->st, ctx->n_pk, ctx->pk_s, 24, ctx->head);
+        CHK(dbg(ctx, "launch_run_heads"));
+        CHK(excl_scan(ctx, ctx->head, ctx->hpos, ctx->n_pk));
+        launch_pair_unique(ctx->st, ctx->n_pk, ctx->pk_s, ctx->head, ctx->hpos, ctx->pair_b, ctx->run_start,
+                           ctx->cnt + C_NUNIQ);
+        CHK(dbg(ctx, "launch_pair_unique"));
+        CHK(sync_counters(ctx));
+        ctx->n_pairs = ctx->h_cnt[C_NUNIQ];
+        CHK(grow_pairs(ctx, ctx->n_pairs));
+        tic(ctx, "offblock_sum", (double)ctx->n_pk * 144 * 8 + (double)ctx->n_pairs * 2 * 144 * 8);
+        launch_offblock_sum(ctx->st, ctx->n_pairs, ctx->pair_b, ctx->run_start, ctx->n_pk, ctx->pk_s,
+                            ctx->act_body, ctx->act_H, ctx->off, ctx->offT, ctx->n_act, ctx->cnt + C_ERR);
+        CHK(dbg(ctx, "launch_offblock_sum"));
+        toc(ctx);
+        launch_csr_keys(ctx->st, ctx->n_pairs, ctx->pair_b, ctx->ent_k, ctx->ent_v);
+        CHK(dbg(ctx, "launch_csr_keys"));
+        CHK(sort_pairs(ctx, ctx->ent_k, ctx->ent_ks, ctx->ent_v, ctx->ent_vs, 2 * ctx->n_pairs));
+    }
+    launch_row_ptr(ctx->st, ctx->nb, 2 * ctx->n_pairs, ctx->ent_ks, ctx->row_ptr);
+    CHK(dbg(ctx, "launch_row_ptr"));
+    tic(ctx, "body_assemble", (double)ctx->nb * (3 * 96 + 128 + 3 * 1152 + 96 + 8) + (double)ctx->n_inc * (1152 + 96));
+    launch_body_assemble(ctx->st, ctx->G, ctx->P, ctx->q, ctx->qtilde, ctx->n_inc, ctx->inc_s, ctx->act_body,
+                         ctx->act_g, ctx->act_H, ctx->C.newton_active, ctx->grad, ctx->diag, ctx->pinv, ctx->e_body,
+                         ctx->n_act);
+    CHK(dbg(ctx, "launch_body_assemble"));
+    toc(ctx);
+    HIPCHK(hipGetLastError());
+    ctx->assembled = true;
+    return ABD_OK;
+}
+
+// --------------------------------------------------------------------------------------------
+// block-Jacobi PCG (compute_du, abd.py:631-658); du <- solution, residual <- ||grad||
+// --------------------------------------------------------------------------------------------
+int solve(AbdCtx* ctx) {
+    const int nt = ctx->ntiles;
+    launch_pcg_init(ctx->st, nt, ctx->tile_b0, ctx->tile_b1, ctx->grad, ctx->pinv, ctx->du, ctx->r, ctx->z, ctx->p1,
+                    ctx->part);
+    CHK(dbg(ctx, "launch_pcg_init"));
+    launch_pcg_tile_sum(ctx->st, ctx->ns, ctx->scene_tile0, ctx->part, nt, 2 * 2 + 0, ctx->bnorm, 1);
+    CHK(dbg(ctx, "launch_pcg_tile_sum"));
+    hipLaunchKernelGGL(k_copy, dim3(cdiv(ctx->ns, 64)), dim3(64), 0, ctx->st, ctx->ns, ctx->bnorm, ctx->C.residual);
+    CHK(dbg(ctx, "k_copy"));
+    HIPCHK(hipMemsetAsync(ctx->cg_iters, 0, sizeof(int) * ctx->ns, ctx->st));
+    HIPCHK(hipMemsetAsync(ctx->cg_done, 0, sizeof(int) * ctx->ns, ctx->st));
+    const double tol = ctx->P.cg_tol;
+    const int nblk = ctx->nb + 2 * ctx->n_pairs;
+    const double bytesA = (double)nblk * (1152 + 192) + (double)ctx->nb * 192;
+    const double bytesB = (double)ctx->nb * (1152 + 96 * 7);
+    int poll = 4;
+    for (int it = 0; it < ctx->P.cg_max_iter; ++it) {
+        double* pold = (it & 1) ? ctx->p0 : ctx->p1;
+        double* pnew = (it & 1) ? ctx->p1 : ctx->p0;
+        tic(ctx, "pcg_spmv", bytesA);
+        launch_pcg_A(ctx->st, it, nt, ctx->tile_b0, ctx->tile_b1, ctx->tile_scene, ctx->scene_tile0, ctx->diag,
+                     ctx->row_ptr, ctx->ent_ks, ctx->ent_vs, ctx->off, ctx->offT, ctx->z, pold, pnew, ctx->Ap,
+                     ctx->part, ctx->bnorm, tol, ctx->cg_iters, ctx->cg_done, ctx->nb, ctx->n_pairs,
+                     ctx->cnt + C_ERR);
+        toc(ctx);
+        tic(ctx, "pcg_update", bytesB);
+        launch_pcg_B(ctx->st, it, nt, ctx->tile_b0, ctx->tile_b1, ctx->tile_scene, ctx->scene_tile0, ctx->pinv, pnew,
+                     ctx->Ap, ctx->du, ctx->r, ctx->z, ctx->part, ctx->bnorm, tol, ctx->cg_done);
+        toc(ctx);
+        if (it + 1 == poll || it + 1 == ctx->P.cg_max_iter) {
+            HIPCHK(hipMemsetAsync(ctx->cnt + C_ANY, 0, sizeof(int), ctx->st));
+            launch_pcg_any(ctx->st, ctx->ns, nt, ctx->scene_tile0, ctx->part, it + 1, ctx->bnorm, tol, ctx->cg_done,
+                           ctx->cnt + C_ANY);
+            CHK(dbg(ctx, "launch_pcg_any"));
+            CHK(sync_counters(ctx));
+            if (!ctx->h_cnt[C_ANY]) break;
+            poll += std::min(32, std::max(4, poll / 2));
+        }
+    }
+    HIPCHK(hipGetLastError());
+    ctx->solved = true;
+    return ABD_OK;
+}
+
+// --------------------------------------------------------------------------------------------
+// CCD bound + backtracking line search (line_search_iterative, gauss_newton.py:98-115)
+// --------------------------------------------------------------------------------------------
+int ccd_bound(AbdCtx* ctx) {
+    launch_xform(ctx->st, ctx->G, ctx->q, nullptr, nullptr, ctx->X0);
+    CHK(dbg(ctx, "launch_xform"));
+    launch_xform(ctx->st, ctx->G, ctx->q, ctx->du, nullptr, ctx->X1);
+    CHK(dbg(ctx, "launch_xform"));
+    CHK(broad_phase(ctx, ctx->X0, ctx->X1));
+    hipLaunchKernelGGL(k_fill_u64, dim3(cdiv(ctx->ns, 128)), dim3(128), 0, ctx->st, ctx->ns, ctx->C.amax_bits,
+                       (unsigned long long)0x3FF0000000000000ull);
+    CHK(dbg(ctx, "k_fill_u64"));
+    if (ctx->P.ccd_enabled) {
+        tic(ctx, "ccd", (double)ctx->n_cand * (8 + 8 * 32));
+        launch_ccd(ctx->st, ctx->G, ctx->n_cand, ctx->cand_s, ctx->X0, ctx->X1, ctx->P, ctx->C.amax_bits);
+        CHK(dbg(ctx, "launch_ccd"));
+        toc(ctx);
+    }
+    HIPCHK(hipGetLastError());
+    return ABD_OK;
+}
+
+int line_search(AbdCtx* ctx) {
+    // E0 at q over the swept candidate list
+    launch_body_energy(ctx->st, ctx->G, ctx->P, ctx->q, nullptr, nullptr, ctx->qtilde, ctx->e_body);
+    CHK(dbg(ctx, "launch_body_energy"));
+    tic(ctx, "energy", (double)ctx->n_cand * (8 + 4 * 32 + 8));
+    launch_cand_eval(ctx->st, ctx->G, ctx->n_cand, ctx->cand_s, ctx->X0, ctx->P, nullptr, ctx->e_cand);
+    CHK(dbg(ctx, "launch_cand_eval"));
+    toc(ctx);
+    launch_scene_sum(ctx->st, ctx->G, ctx->e_body, ctx->e_cand, ctx->cand_lo, ctx->cand_hi, ctx->C.E0);
+    CHK(dbg(ctx, "launch_scene_sum"));
+    launch_ls_init(ctx->st, ctx->ns, ctx->P, ctx->C.newton_active, ctx->C);
+    CHK(dbg(ctx, "launch_ls_init"));
+    for (int trial = 0; trial < 64; ++trial) {
+        launch_xform(ctx->st, ctx->G, ctx->q, ctx->du, ctx->C.alpha, ctx->Xt);
+        CHK(dbg(ctx, "launch_xform"));
+        tic(ctx, "energy", (double)ctx->n_cand * (8 + 4 * 32 + 8));
+        launch_cand_eval(ctx->st, ctx->G, ctx->n_cand, ctx->cand_s, ctx->Xt, ctx->P, nullptr, ctx->e_cand);
+        CHK(dbg(ctx, "launch_cand_eval"));
+        toc(ctx);
+        launch_body_energy(ctx->st, ctx->G, ctx->P, ctx->q, ctx->du, ctx->C.alpha, ctx->qtilde, ctx->qtmp);
+        CHK(dbg(ctx, "launch_body_energy"));
+        launch_scene_sum(ctx->st, ctx->G, ctx->qtmp, ctx->e_cand, ctx->cand_lo, ctx->cand_hi, ctx->C.E1);
+        CHK(dbg(ctx, "launch_scene_sum"));
+        HIPCHK(hipMemsetAsync(ctx->cnt + C_PENDING, 0, sizeof(int), ctx->st));
+        launch_ls_decide(ctx->st, ctx->ns, ctx->P, ctx->C, ctx->cnt + C_PENDING);
+        CHK(dbg(ctx, "launch_ls_decide"));
+        HIPCHK(hipGetLastError());
+        CHK(sync_counters(ctx));
+        if (!ctx->h_cnt[C_PENDING]) break;
+    }
+    launch_apply(ctx->st, ctx->G, ctx->q, ctx->du, ctx->C.alpha, ctx->C.newton_active);
+    CHK(dbg(ctx, "launch_apply"));
+    HIPCHK(hipGetLastError());
+    // the accepted iterate lies inside the swept boxes: the candidates stay valid
+    ctx->cand_valid = true;
+    return ABD_OK;
+}
+
+int step_once(AbdCtx* ctx) {
+    const int ns = ctx->ns, mi = ctx->P.max_newton;
+    launch_qtilde(ctx->st, ctx->nb, ctx->qnow, ctx->qdot, ctx->P.dt, ctx->P.gravity, ctx->qtilde);
+    CHK(dbg(ctx, "launch_qtilde"));
+    CHK(fill_i32(ctx, ctx->C.newton_active, ns, 1));
+    ctx->log_res.assign((size_t)ns * mi, 0.0);
+    ctx->log_alpha.assign((size_t)ns * mi, 0.0);
+    ctx->log_cg.assign((size_t)ns * mi, 0);
+    ctx->stats.assign(ns, AbdStepStats{});
+    for (auto& s : ctx->stats) { s.alpha_min = 1.0; s.ccd_alpha_min = 1.0; }
+    std::vector<int> active(ns, 1);
+    for (int it = 0; it < mi; ++it) {
+        if (!ctx->cand_valid) CHK(static_broad_phase(ctx));
+        CHK(assemble(ctx));
+        CHK(solve(ctx));
+        CHK(ccd_bound(ctx));
+        CHK(line_search(ctx));
+        // keep the pre-check flags for the report, then stop converged scenes (abd.py:699)
+        HIPCHK(hipMemcpyAsync(ctx->act_before, ctx->C.newton_active, sizeof(int) * ns, hipMemcpyDeviceToDevice, ctx->st));
+        launch_newton_check(ctx->st, ns, ctx->P, ctx->C);
+        CHK(dbg(ctx, "launch_newton_check"));
+        hipLaunchKernelGGL(k_report, dim3(cdiv(ns, 128)), dim3(128), 0, ctx->st, ns, ctx->C, ctx->cg_iters,
+                           ctx->nact_s, ctx->cand_lo, ctx->cand_hi, ctx->act_before, ctx->report);
+        CHK(dbg(ctx, "k_report"));
+        HIPCHK(hipMemcpyAsync(ctx->h_report, ctx->report, sizeof(double) * R_N * ns, hipMemcpyDeviceToHost, ctx->st));
+        std::vector<int> now(ns);
+        HIPCHK(hipMemcpyAsync(now.data(), ctx->C.newton_active, sizeof(int) * ns, hipMemcpyDeviceToHost, ctx->st));
+        HIPCHK(hipStreamSynchronize(ctx->st));
+        bool any = false;
+        for (int s = 0; s < ns; ++s) {
+            const double* R = ctx->h_report + (size_t)s * R_N;
+            if (R[R_ACTIVE] != 0.0) {
+                AbdStepStats& S = ctx->stats[s];
+                ctx->log_res[(size_t)s * mi + it] = R[R_RES];
+                ctx->log_alpha[(size_t)s * mi + it] = R[R_ALPHA];
+                ctx->log_cg[(size_t)s * mi + it] = (int)R[R_CG];
+                if (it == 0) S.residual_first = R[R_RES];
+                S.residual_last = R[R_RES];
+                S.newton_iters = it + 1;
+                S.cg_iters += (int)R[R_CG];
+                S.ls_floor_hits += (int)R[R_FLOOR];
+                S.n_active = (int)R[R_NACT];
+                S.n_candidates = (int)R[R_NCAND];
+                S.alpha_min = std::min(S.alpha_min, R[R_ALPHA]);
+                S.ccd_alpha_min = std::min(S.ccd_alpha_min, R[R_ALPHA0]);
+                S.energy = R[R_E1];
+                if (now[s] == 0) S.converged = 1;
+            }
+            active[s] = now[s];
+            any |= now[s] != 0;
+        }
+        if (!any) break;
+    }
+    launch_forward(ctx->st, ctx->nb, ctx->P.dt, ctx->q, ctx->qnow, ctx->qdot);
+    CHK(dbg(ctx, "launch_forward"));
+    HIPCHK(hipGetLastError());
+    return ABD_OK;
+}
+
+int check_ctx(AbdCtx* ctx) { return ctx ? ABD_OK : ABD_ERR_ARG; }
+
+}  // namespace
+
+// =============================================================================================
+// C ABI
+// =============================================================================================
+extern "C" {
+
+int abd_create(const AbdSceneDesc* d, const AbdParams* p, AbdCtx** out) {
+    if (!d || !p || !out) return ABD_ERR_ARG;
+    *out = nullptr;
+    AbdCtx* ctx = new AbdCtx();
+    auto fail = [&](int rc) {
+        // keep the context so the caller can read the message, but mark it unusable
+        *out = ctx;
+        return rc;
+    };
+    ctx->P = *p;
+    {
+        double* n = ctx->P.ground_normal;
+        double l = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+        if (l > 0) { n[0] /= l; n[1] /= l; n[2] /= l; }
+    }
+    ctx->nb = d->n_bodies; ctx->nv = d->n_verts; ctx->ne = d->n_edges; ctx->nt = d->n_tris; ctx->ns = d->n_scenes;
+    const int nb = ctx->nb, nv = ctx->nv, ne = ctx->ne, nt = ctx->nt, ns = ctx->ns;
+    if (nb <= 0 || nv <= 0 || ns <= 0 || ns > 65535 || nb >= (1 << 20)) {
+        ctx->err = "invalid scene sizes";
+        return fail(ABD_ERR_ARG);
+    }
+    // host validation and scene tables
+    std::vector<int> sb(ns + 1, 0), sv(ns + 1, 0), se(ns + 1, 0), stt(ns + 1, 0);
+    for (int b = 0; b < nb; ++b) {
+        int s = d->body_scene[b];
+        if (s < 0 || s >= ns || (b > 0 && s < d->body_scene[b - 1])) {
+            ctx->err = "body_scene must be non-decreasing in [0, n_scenes)";
+            return fail(ABD_ERR_ARG);
+        }
+        sb[s + 1]++;
+    }
+    std::vector<double4> rest(nv);
+    for (int v = 0; v < nv; ++v) {
+        int b = d->vert_body[v];
+        if (b < 0 || b >= nb || (v > 0 && b < d->vert_body[v - 1])) {
+            ctx->err = "vert_body must be non-decreasing in [0, n_bodies)";
+            return fail(ABD_ERR_ARG);
+        }
+        rest[v] = make_double4(d->rest_x[3 * v], d->rest_x[3 * v + 1], d->rest_x[3 * v + 2], (double)b);
+        sv[d->body_scene[b] + 1]++;
+    }
+    std::vector<int2> edges(std::max(ne, 1));
+    for (int e = 0; e < ne; ++e) {
+        int a = d->edges[2 * e], b = d->edges[2 * e + 1];
+        if (a < 0 || a >= nv || b < 0 || b >= nv || d->vert_body[a] != d->vert_body[b]) {
+            ctx->err = "edge indices out of range or spanning two bodies";
+            return fail(ABD_ERR_ARG);
+        }
+        int s = d->body_scene[d->vert_body[a]];
+        if (e > 0 && s < d->body_scene[d->vert_body[d->edges[2 * e - 2]]]) {
+            ctx->err = "edges must be grouped by scene";
+            return fail(ABD_ERR_ARG);
+        }
+        edges[e] = make_int2(a, b);
+        se[s + 1]++;
+    }
+    std::vector<int4> tris(std::max(nt, 1));
+    for (int t = 0; t < nt; ++t) {
+        int a = d->tris[3 * t], b = d->tris[3 * t + 1], c = d->tris[3 * t + 2];
+        if (a < 0 || a >= nv || b < 0 || b >= nv || c < 0 || c >= nv || d->vert_body[a] != d->vert_body[b] ||
+            d->vert_body[a] != d->vert_body[c]) {
+            ctx->err = "triangle indices out of range or spanning two bodies";
+            return fail(ABD_ERR_ARG);
+        }
+        int s = d->body_scene[d->vert_body[a]];
+        if (t > 0 && s < d->body_scene[d->vert_body[d->tris[3 * t - 3]]]) {
+            ctx->err = "triangles must be grouped by scene";
+            return fail(ABD_ERR_ARG);
+        }
+        tris[t] = make_int4(a, b, c, d->vert_body[a]);
+        stt[s + 1]++;
+    }
+    for (int s = 0; s < ns; ++s) {
+        sb[s + 1] += sb[s]; sv[s + 1] += sv[s]; se[s + 1] += se[s]; stt[s + 1] += stt[s];
+        if (sv[s + 1] - sv[s] > (int)KEY_IMASK || se[s + 1] - se[s] > (int)KEY_IMASK || stt[s + 1] - stt[s] > (int)KEY_IMASK) {
+            ctx->err = "a scene exceeds 2^23 primitives";
+            return fail(ABD_ERR_ARG);
+        }
+    }
+    ctx->h_scene_b0 = sb; ctx->h_scene_v0 = sv; ctx->h_scene_e0 = se; ctx->h_scene_t0 = stt;
+    // tiles of at most 5 bodies inside one scene
+    std::vector<int> tb0, tb1, tsc, st0(ns + 1, 0);
+    for (int s = 0; s < ns; ++s) {
+        st0[s] = (int)tb0.size();
+        for (int b = sb[s]; b < sb[s + 1]; b += 5) {
+            tb0.push_back(b);
+            tb1.push_back(std::min(b + 5, sb[s + 1]));
+            tsc.push_back(s);
+        }
+    }
+    st0[ns] = (int)tb0.size();
+    ctx->ntiles = (int)tb0.size();
+    ctx->h_tile_b0 = tb0;
+    ctx->h_scene_tile0 = st0;
+
+    {
+        hipError_t e = hipStreamCreateWithFlags(&ctx->st, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            ctx->st = nullptr;
+            ctx->err = std::string("hipStreamCreateWithFlags: ") + hipGetErrorName(e) + ": " + hipGetErrorString(e);
+            return fail(ABD_ERR_HIP);
+        }
+    }
+    int rc = ABD_OK;
+#define A(ptr, n) if ((rc = dalloc(ctx, &ctx->ptr, (n))) != ABD_OK) return fail(rc)
+#define U(ptr, src, n) if (hipMemcpy(ctx->ptr, (src), sizeof(*ctx->ptr) * (size_t)(n), hipMemcpyHostToDevice) != hipSuccess) { ctx->err = "upload failed"; return fail(ABD_ERR_HIP); }
+    A(rest, nv); U(rest, rest.data(), nv);
+    A(edges, std::max(ne, 1)); U(edges, edges.data(), std::max(ne, 1));
+    A(tris, std::max(nt, 1)); U(tris, tris.data(), std::max(nt, 1));
+    A(mass4, 16 * nb); U(mass4, d->mass4, 16 * nb);
+    A(pinned, nb); U(pinned, d->body_pinned, nb);
+    A(bscene, nb); U(bscene, d->body_scene, nb);
+    A(scene_b0, ns + 1); U(scene_b0, sb.data(), ns + 1);
+    A(scene_v0, ns + 1); U(scene_v0, sv.data(), ns + 1);
+    A(scene_e0, ns + 1); U(scene_e0, se.data(), ns + 1);
+    A(scene_t0, ns + 1); U(scene_t0, stt.data(), ns + 1);
+    A(tile_b0, ctx->ntiles); U(tile_b0, tb0.data(), ctx->ntiles);
+    A(tile_b1, ctx->ntiles); U(tile_b1, tb1.data(), ctx->ntiles);
+    A(tile_scene, ctx->ntiles); U(tile_scene, tsc.data(), ctx->ntiles);
+    A(scene_tile0, ns + 1); U(scene_tile0, st0.data(), ns + 1);
+    A(q, 12 * nb); A(qnow, 12 * nb); A(qdot, 12 * nb); A(qtilde, 12 * nb); A(qtmp, 12 * nb);
+    A(du, 12 * nb); A(grad, 12 * nb); A(diag, 144 * nb); A(pinv, 144 * nb); A(e_body, nb);
+    A(X0, nv); A(X1, nv); A(Xt, nv); A(bounds, 6 * ns);
+    for (int w = 0; w < 2; ++w) {
+        AbdCtx::Tree& T = ctx->tree[w];
+        T.n = (w == 0) ? nt : ne;
+        int n = std::max(T.n, 2);
+        A(tree[w].boxes, 2 * n); A(tree[w].keys, n); A(tree[w].skeys, n); A(tree[w].idx, n); A(tree[w].sidx, n);
+        A(tree[w].nodes, n); A(tree[w].parent, n); A(tree[w].leaf_parent, n); A(tree[w].flags, n); A(tree[w].root, ns);
+    }
+    A(cand_lo, ns); A(cand_hi, ns);
+    A(r, 12 * nb); A(z, 12 * nb); A(p0, 12 * nb); A(p1, 12 * nb); A(Ap, 12 * nb);
+    A(part, 6 * ctx->ntiles); A(bnorm, ns); A(cg_iters, ns); A(cg_done, ns);
+    A(C.newton_active, ns); A(C.residual, ns); A(C.alpha, ns); A(C.alpha0, ns); A(C.amax_bits, ns);
+    A(C.ls_done, ns); A(C.ls_floor, ns); A(C.E0, ns); A(C.E1, ns);
+    A(nact_s, ns); A(act_before, ns); A(report, R_N * ns); A(cnt, C_N); A(row_ptr, nb + 1);
+#undef A
+#undef U
+    if (hipHostMalloc((void**)&ctx->h_cnt, C_N * sizeof(int), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&ctx->h_report, R_N * ns * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+        ctx->err = "hipHostMalloc failed";
+        return fail(ABD_ERR_HIP);
+    }
+    if ((rc = grow_candidates(ctx, std::max(1024, std::min(p->max_candidates > 0 ? p->max_candidates : (1 << 20), 1 << 26)))) != ABD_OK) return fail(rc);
+    if ((rc = grow_active(ctx, 1024)) != ABD_OK) return fail(rc);
+    if ((rc = grow_pairs(ctx, 1024)) != ABD_OK) return fail(rc);
+    ctx->G = Geo{nb, nv, ne, nt, ns, ctx->rest, ctx->edges, ctx->tris, ctx->mass4, ctx->pinned, ctx->bscene,
+                 ctx->scene_b0, ctx->scene_v0, ctx->scene_e0, ctx->scene_t0, ctx->cnt + C_ERR};
+    if (hipMemset(ctx->cnt, 0, C_N * sizeof(int)) != hipSuccess) {
+        ctx->err = "hipMemset failed";
+        return fail(ABD_ERR_HIP);
+    }
+    // identity initial state
+    std::vector<double> q0(12 * (size_t)nb, 0.0);
+    for (int b = 0; b < nb; ++b) q0[12 * b + 3] = q0[12 * b + 7] = q0[12 * b + 11] = 1.0;
+    std::vector<double> zero(12 * (size_t)nb, 0.0);
+    if ((rc = abd_set_state(ctx, q0.data(), q0.data(), zero.data())) != ABD_OK) return fail(rc);
+    *out = ctx;
+    return ABD_OK;
+}
+
+void abd_destroy(AbdCtx* ctx) {
+    if (!ctx) return;
+    if (ctx->st) (void)hipStreamSynchronize(ctx->st);
+    for (void* p : ctx->allocs) (void)hipFree(p);
+    for (auto e : ctx->T.pool) (void)hipEventDestroy(e);
+    for (auto& r : ctx->T.open) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
+    if (ctx->h_cnt) (void)hipHostFree(ctx->h_cnt);
+    if (ctx->h_report) (void)hipHostFree(ctx->h_report);
+    if (ctx->st) (void)hipStreamDestroy(ctx->st);
+    delete ctx;
+}
+
+const char* abd_last_error(const AbdCtx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int abd_get_params(const AbdCtx* ctx, AbdParams* out) {
+    if (!ctx || !out) return ABD_ERR_ARG;
+    *out = ctx->P;
+    return ABD_OK;
+}
+
+int abd_set_params(AbdCtx* ctx, const AbdParams* p) {
+    if (!ctx || !p) return ABD_ERR_ARG;
+    ctx->P = *p;
+    double* n = ctx->P.ground_normal;
+    double l = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    if (l > 0) { n[0] /= l; n[1] /= l; n[2] /= l; }
+    ctx->cand_valid = false;
+    ctx->assembled = ctx->solved = false;
+    return ABD_OK;
+}
+
+int abd_set_state(AbdCtx* ctx, const double* q, const double* q_now, const double* qdot) {
+    CHK(check_ctx(ctx));
+    size_t n = 12 * (size_t)ctx->nb * sizeof(double);
+    if (q) HIPCHK(hipMemcpyAsync(ctx->q, q, n, hipMemcpyHostToDevice, ctx->st));
+    if (q_now) HIPCHK(hipMemcpyAsync(ctx->qnow, q_now, n, hipMemcpyHostToDevice, ctx->st));
+    if (qdot) HIPCHK(hipMemcpyAsync(ctx->qdot, qdot, n, hipMemcpyHostToDevice, ctx->st));
+    launch_qtilde(ctx->st, ctx->nb, ctx->qnow, ctx->qdot, ctx->P.dt, ctx->P.gravity, ctx->qtilde);
+    CHK(dbg(ctx, "launch_qtilde"));
+    CHK(fill_i32(ctx, ctx->C.newton_active, ctx->ns, 1));
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    ctx->cand_valid = false;
+    ctx->assembled = ctx->solved = false;
+    return ABD_OK;
+}
+
+int abd_get_state(const AbdCtx* cctx, double* q, double* q_now, double* qdot) {
+    AbdCtx* ctx = const_cast<AbdCtx*>(cctx);
+    CHK(check_ctx(ctx));
+    size_t n = 12 * (size_t)ctx->nb * sizeof(double);
+    if (q) HIPCHK(hipMemcpyAsync(q, ctx->q, n, hipMemcpyDeviceToHost, ctx->st));
+    if (q_now) HIPCHK(hipMemcpyAsync(q_now, ctx->qnow, n, hipMemcpyDeviceToHost, ctx->st));
+    if (qdot) HIPCHK(hipMemcpyAsync(qdot, ctx->qdot, n, hipMemcpyDeviceToHost, ctx->st));
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    return ABD_OK;
+}
+
+int abd_step(AbdCtx* ctx, int nsteps, AbdStepStats* stats) {
+    CHK(check_ctx(ctx));
+    if (nsteps < 0) return ABD_ERR_ARG;
+    for (int k = 0; k < nsteps; ++k) {
+        CHK(step_once(ctx));
+        timer_collect(ctx);
+    }
+    CHK(sync_counters(ctx));
+    if (stats)
+        for (int s = 0; s < ctx->ns && s < (int)ctx->stats.size(); ++s) stats[s] = ctx->stats[s];
+    for (auto& s : ctx->stats)
+        if (!std::isfinite(s.residual_last) || !std::isfinite(s.energy)) {
+            ctx->err = "non-finite residual or energy";
+            return ABD_ERR_NAN;
+        }
+    return ABD_OK;
+}
+
+int abd_get_iter_log(const AbdCtx* ctx, double* residual, double* alpha, int* cg) {
+    if (!ctx) return ABD_ERR_ARG;
+    if (residual) std::copy(ctx->log_res.begin(), ctx->log_res.end(), residual);
+    if (alpha) std::copy(ctx->log_alpha.begin(), ctx->log_alpha.end(), alpha);
+    if (cg) std::copy(ctx->log_cg.begin(), ctx->log_cg.end(), cg);
+    return ABD_OK;
+}
+
+int abd_detect(AbdCtx* ctx, int* pairs, double* d2, int cap, int* n) {
+    CHK(check_ctx(ctx));
+    CHK(static_broad_phase(ctx));
+    const int nc = ctx->n_cand;
+    std::vector<uint64_t> keys(nc);
+    std::vector<int> flag(nc);
+    std::vector<double> dd(nc);
+    if (nc > 0) {
+        launch_cand_eval(ctx->st, ctx->G, nc, ctx->cand_s, ctx->X0, ctx->P, ctx->flag, nullptr);
+        CHK(dbg(ctx, "launch_cand_eval"));
+        launch_pair_d2(ctx->st, ctx->G, nc, ctx->cand_s, ctx->X0, ctx->P, ctx->e_cand);
+        CHK(dbg(ctx, "launch_pair_d2"));
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(keys.data(), ctx->cand_s, nc * sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->st));
+        HIPCHK(hipMemcpyAsync(flag.data(), ctx->flag, nc * sizeof(int), hipMemcpyDeviceToHost, ctx->st));
+        HIPCHK(hipMemcpyAsync(dd.data(), ctx->e_cand, nc * sizeof(double), hipMemcpyDeviceToHost, ctx->st));
+    }
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    int na = 0;
+    for (int i = 0; i < nc; ++i) na += flag[i];
+    *n = na;
+    if (na > cap) {
+        ctx->err = "abd_detect: capacity too small";
+        return ABD_ERR_CAPACITY;
+    }
+    int k = 0;
+    for (int c = 0; c < nc; ++c) {
+        if (!flag[c]) continue;
+        uint64_t key = keys[c];
+        int s = key_scene(key), kind = key_kind(key), il = key_il(key), jl = key_jl(key);
+        int i = (kind == ABD_EE ? ctx->h_scene_e0[s] : ctx->h_scene_v0[s]) + il;
+        int j = (kind == ABD_PT ? ctx->h_scene_t0[s] + jl : (kind == ABD_EE ? ctx->h_scene_e0[s] + jl : 0));
+        pairs[3 * k] = kind;
+        pairs[3 * k + 1] = i;
+        pairs[3 * k + 2] = j;
+        if (d2) d2[k] = dd[c];
+        ++k;
+    }
+    return ABD_OK;
+}
+
+int abd_assemble(AbdCtx* ctx, double* grad, double* diag, int* pair_ij, double* offdiag, int cap, int* n_pairs,
+                 double* energy) {
+    CHK(check_ctx(ctx));
+    CHK(fill_i32(ctx, ctx->C.newton_active, ctx->ns, 1));
+    CHK(static_broad_phase(ctx));
+    CHK(assemble(ctx));
+    const int nb = ctx->nb;
+    if (grad) HIPCHK(hipMemcpyAsync(grad, ctx->grad, 12 * nb * sizeof(double), hipMemcpyDeviceToHost, ctx->st));
+    if (diag) HIPCHK(hipMemcpyAsync(diag, ctx->diag, 144 * (size_t)nb * sizeof(double), hipMemcpyDeviceToHost, ctx->st));
+    if (n_pairs) *n_pairs = ctx->n_pairs;
+    if (ctx->n_pairs > cap && (pair_ij || offdiag)) {
+        ctx->err = "abd_assemble: capacity too small";
+        HIPCHK(hipStreamSynchronize(ctx->st));
+        return ABD_ERR_CAPACITY;
+    }
+    if (pair_ij && ctx->n_pairs)
+        HIPCHK(hipMemcpyAsync(pair_ij, ctx->pair_b, 2 * ctx->n_pairs * sizeof(int), hipMemcpyDeviceToHost, ctx->st));
+    if (offdiag && ctx->n_pairs)
+        HIPCHK(hipMemcpyAsync(offdiag, ctx->off, 144 * (size_t)ctx->n_pairs * sizeof(double), hipMemcpyDeviceToHost, ctx->st));
+    if (energy) {
+        launch_scene_sum(ctx->st, ctx->G, ctx->e_body, ctx->e_cand, ctx->cand_lo, ctx->cand_hi, ctx->C.E0);
+        CHK(dbg(ctx, "launch_scene_sum"));
+        HIPCHK(hipMemcpyAsync(energy, ctx->C.E0, ctx->ns * sizeof(double), hipMemcpyDeviceToHost, ctx->st));
+    }
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    return ABD_OK;
+}
+
+int abd_solve(AbdCtx* ctx, double* du, double* residual, int* cg) {
+    CHK(check_ctx(ctx));
+    if (!ctx->assembled) {
+        ctx->err = "abd_solve before abd_assemble";
+        return ABD_ERR_STATE;
+    }
+    CHK(solve(ctx));
+    if (du) HIPCHK(hipMemcpyAsync(du, ctx->du, 12 * ctx->nb * sizeof(double), hipMemcpyDeviceToHost, ctx->st));
+    if (residual) HIPCHK(hipMemcpyAsync(residual, ctx->bnorm, ctx->ns * sizeof(double), hipMemcpyDeviceToHost, ctx->st));
+    if (cg) HIPCHK(hipMemcpyAsync(cg, ctx->cg_iters, ctx->ns * sizeof(int), hipMemcpyDeviceToHost, ctx->st));
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    return ABD_OK;
+}
+
+int abd_energy(AbdCtx* ctx, const double* q, double* energy) {
+    CHK(check_ctx(ctx));
+    if (q) {
+        // evaluate at q without disturbing the iterate: swap buffers
+        HIPCHK(hipMemcpyAsync(ctx->qtmp, ctx->q, 12 * ctx->nb * sizeof(double), hipMemcpyDeviceToDevice, ctx->st));
+        HIPCHK(hipMemcpyAsync(ctx->q, q, 12 * ctx->nb * sizeof(double), hipMemcpyHostToDevice, ctx->st));
+    }
+    int rc = static_broad_phase(ctx);
+    if (rc == ABD_OK) {
+        launch_cand_eval(ctx->st, ctx->G, ctx->n_cand, ctx->cand_s, ctx->X0, ctx->P, nullptr, ctx->e_cand);
+        CHK(dbg(ctx, "launch_cand_eval"));
+        launch_body_energy(ctx->st, ctx->G, ctx->P, ctx->q, nullptr, nullptr, ctx->qtilde, ctx->e_body);
+        CHK(dbg(ctx, "launch_body_energy"));
+        launch_scene_sum(ctx->st, ctx->G, ctx->e_body, ctx->e_cand, ctx->cand_lo, ctx->cand_hi, ctx->C.E0);
+        CHK(dbg(ctx, "launch_scene_sum"));
+        HIPCHK(hipMemcpyAsync(energy, ctx->C.E0, ctx->ns * sizeof(double), hipMemcpyDeviceToHost, ctx->st));
+    }
+    if (q) {
+        HIPCHK(hipMemcpyAsync(ctx->q, ctx->qtmp, 12 * ctx->nb * sizeof(double), hipMemcpyDeviceToDevice, ctx->st));
+        ctx->cand_valid = false;
+    }
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    ctx->assembled = ctx->solved = false;
+    return rc;
+}
+
+int abd_ccd(AbdCtx* ctx, const double* du, double* alpha) {
+    CHK(check_ctx(ctx));
+    if (du) HIPCHK(hipMemcpyAsync(ctx->du, du, 12 * ctx->nb * sizeof(double), hipMemcpyHostToDevice, ctx->st));
+    CHK(ccd_bound(ctx));
+    ctx->cand_valid = false;  // swept list, not re-validated by a line search
+    std::vector<unsigned long long> bits(ctx->ns);
+    HIPCHK(hipMemcpyAsync(bits.data(), ctx->C.amax_bits, ctx->ns * sizeof(unsigned long long), hipMemcpyDeviceToHost, ctx->st));
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    for (int s = 0; s < ctx->ns; ++s) {
+        double a;
+        std::memcpy(&a, &bits[s], sizeof(double));
+        alpha[s] = ctx->P.ccd_enabled ? std::min(1.0, a) : 1.0;
+    }
+    return ABD_OK;
+}
+
+int abd_line_search(AbdCtx* ctx, double* alpha) {
+    CHK(check_ctx(ctx));
+    if (!ctx->solved) {
+        ctx->err = "abd_line_search before abd_solve";
+        return ABD_ERR_STATE;
+    }
+    CHK(ccd_bound(ctx));
+    CHK(line_search(ctx));
+    if (alpha) HIPCHK(hipMemcpyAsync(alpha, ctx->C.alpha, ctx->ns * sizeof(double), hipMemcpyDeviceToHost, ctx->st));
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    ctx->assembled = ctx->solved = false;
+    return ABD_OK;
+}
+
+int abd_forward(AbdCtx* ctx) {
+    CHK(check_ctx(ctx));
+    launch_forward(ctx->st, ctx->nb, ctx->P.dt, ctx->q, ctx->qnow, ctx->qdot);
+    CHK(dbg(ctx, "launch_forward"));
+    launch_qtilde(ctx->st, ctx->nb, ctx->qnow, ctx->qdot, ctx->P.dt, ctx->P.gravity, ctx->qtilde);
+    CHK(dbg(ctx, "launch_qtilde"));
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    return ABD_OK;
+}
+
+int abd_get_vertices(AbdCtx* ctx, double* xyz) {
+    CHK(check_ctx(ctx));
+    launch_xform(ctx->st, ctx->G, ctx->q, nullptr, nullptr, ctx->Xt);
+    CHK(dbg(ctx, "launch_xform"));
+    std::vector<double4> X(ctx->nv);
+    HIPCHK(hipMemcpyAsync(X.data(), ctx->Xt, ctx->nv * sizeof(double4), hipMemcpyDeviceToHost, ctx->st));
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    for (int v = 0; v < ctx->nv; ++v) {
+        xyz[3 * v] = X[v].x; xyz[3 * v + 1] = X[v].y; xyz[3 * v + 2] = X[v].z;
+    }
+    return ABD_OK;
+}
+
+int abd_get_contact_points(AbdCtx* ctx, double* midpoints, double* dist, int cap, int* n) {
+    CHK(check_ctx(ctx));
+    CHK(static_broad_phase(ctx));
+    const int nc = ctx->n_cand;
+    launch_cand_eval(ctx->st, ctx->G, nc, ctx->cand_s, ctx->X0, ctx->P, ctx->flag, nullptr);
+    CHK(dbg(ctx, "launch_cand_eval"));
+    CHK(excl_scan(ctx, ctx->flag, ctx->pos, nc));
+    if (nc > 0) {
+        HIPCHK(hipMemcpyAsync(ctx->cnt + C_NACT, ctx->pos + nc - 1, sizeof(int), hipMemcpyDeviceToDevice, ctx->st));
+        HIPCHK(hipMemcpyAsync(ctx->cnt + C_ANY, ctx->flag + nc - 1, sizeof(int), hipMemcpyDeviceToDevice, ctx->st));
+    }
+    CHK(sync_counters(ctx));
+    int na = nc > 0 ? ctx->h_cnt[C_NACT] + ctx->h_cnt[C_ANY] : 0;
+    *n = na;
+    if (na > cap) {
+        ctx->err = "abd_get_contact_points: capacity too small";
+        return ABD_ERR_CAPACITY;
+    }
+    if (na == 0) return ABD_OK;
+    CHK(grow_active(ctx, na));
+    launch_compact(ctx->st, nc, ctx->flag, ctx->pos, ctx->cand_s, ctx->act);
+    CHK(dbg(ctx, "launch_compact"));
+    double *dm = nullptr, *dd = nullptr;
+    CHK(dalloc(ctx, &dm, 3 * na));
+    CHK(dalloc(ctx, &dd, na));
+    launch_contact_points(ctx->st, ctx->G, na, ctx->act, ctx->X0, ctx->P, dm, dd);
+    CHK(dbg(ctx, "launch_contact_points"));
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(midpoints, dm, 3 * na * sizeof(double), hipMemcpyDeviceToHost, ctx->st));
+    HIPCHK(hipMemcpyAsync(dist, dd, na * sizeof(double), hipMemcpyDeviceToHost, ctx->st));
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    CHK(dfree(ctx, dm));
+    CHK(dfree(ctx, dd));
+    return ABD_OK;
+}
+
+int abd_enable_timing(AbdCtx* ctx, int on) {
+    CHK(check_ctx(ctx));
+    ctx->T.on = on != 0;
+    ctx->T.ms.clear(); ctx->T.calls.clear(); ctx->T.bytes.clear(); ctx->T.order.clear();
+    return ABD_OK;
+}
+
+int abd_get_kernel_times(const AbdCtx* ctx, char* names, int name_cap, double* ms, int* calls, double* bytes, int cap,
+                         int* n) {
+    if (!ctx) return ABD_ERR_ARG;
+    std::string all;
+    int k = 0;
+    for (auto& name : ctx->T.order) {
+        if (k < cap) {
+            ms[k] = ctx->T.ms.at(name);
+            calls[k] = ctx->T.calls.at(name);
+            bytes[k] = ctx->T.bytes.at(name);
+        }
+        all += name;
+        all += "\n";
+        ++k;
+    }
+    *n = k;
+    if (names && name_cap > 0) {
+        std::strncpy(names, all.c_str(), name_cap - 1);
+        names[name_cap - 1] = 0;
+    }
+    return k > cap ? ABD_ERR_CAPACITY : ABD_OK;
+}
+
+}  // extern "C"

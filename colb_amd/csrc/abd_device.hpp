@@ -1,0 +1,313 @@
+// abd_device.hpp — device-side FP64 math of the ABD Newton step on gfx950.
+//
+// Contact distances follow the IPC distance types (Li et al. 2020): after closest-feature
+// classification a pair's squared distance is one of
+//   PP  |y0 - y1|^2
+//   PE  |(e0 - p) x (e1 - p)|^2 / |e1 - e0|^2
+//   PT / EE  (a.n)^2 / |n|^2,  n = u x v, with a, u, v linear in the stencil points.
+// The reference obtains these derivatives from the (un-vendored) ipctk_wp module
+// (abd.py:15-17, 247, 376, 485) and the closest points from Warp's closest_point_edge_edge
+// (abd.py:190).  Here each Hessian entry is an explicit function of a few shared
+// intermediates, so the 64 lanes of a wave evaluate the 12x12 entries in parallel.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace abd {
+
+struct D3 {
+    double x, y, z;
+};
+__device__ __forceinline__ D3 mk(double x, double y, double z) { return {x, y, z}; }
+__device__ __forceinline__ D3 operator+(D3 a, D3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ D3 operator-(D3 a, D3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ D3 operator*(double s, D3 a) { return {s * a.x, s * a.y, s * a.z}; }
+__device__ __forceinline__ double dot(D3 a, D3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ D3 cross(D3 a, D3 b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ double cmp(D3 a, int k) { return k == 0 ? a.x : (k == 1 ? a.y : a.z); }
+// ([w]x)_{rc}
+__device__ __forceinline__ double skew_rc(D3 w, int r, int c) {
+    if (r == c) return 0.0;
+    if (r == 0) return c == 1 ? -w.z : w.y;
+    if (r == 1) return c == 0 ? w.z : -w.x;
+    return c == 0 ? -w.y : w.x;
+}
+
+enum { K_PT = 0, K_EE = 1, K_PG = 2 };
+enum { T_PP = 0, T_PE = 1, T_PT = 2, T_EE = 3, T_PG = 4 };
+
+// barrier.py:47-66: cubic barrier in s = d^2
+__device__ __forceinline__ void barrier3(double s, double shat, double& B, double& B1, double& B2) {
+    if (s < shat) {
+        double d = s - shat;
+        B = -d * d * d;
+        B1 = -3.0 * d * d;
+        B2 = -6.0 * d;
+    } else {
+        B = B1 = B2 = 0.0;
+    }
+}
+
+// ortho.py:9-16: kappa * sum_ij (a_i.a_j - delta_ij)^2, a = rows 1..3 of the DOF vector
+__device__ __forceinline__ double ortho_energy(const double* q, double kappa) {
+    const double* a = q + 3;
+    double e = 0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            double t = a[3 * i] * a[3 * j] + a[3 * i + 1] * a[3 * j + 1] + a[3 * i + 2] * a[3 * j + 2] - (i == j ? 1.0 : 0.0);
+            e += t * t;
+        }
+    return kappa * e;
+}
+
+// ---------------------------------------------------------------------------------------------
+// closest-feature classification (Ericson, Real-Time Collision Detection 5.1.5 and 5.1.9)
+// ---------------------------------------------------------------------------------------------
+struct Stencil {
+    int type, ns;
+    int st[4];
+};
+
+__device__ __forceinline__ Stencil classify_pt(const D3* x) {
+    Stencil S;
+    D3 p = x[0], A = x[1], B = x[2], C = x[3];
+    D3 ab = B - A, ac = C - A, ap = p - A;
+    double d1 = dot(ab, ap), d2 = dot(ac, ap);
+    if (d1 <= 0 && d2 <= 0) { S.type = T_PP; S.ns = 2; S.st[0] = 0; S.st[1] = 1; return S; }
+    D3 bp = p - B;
+    double d3 = dot(ab, bp), d4 = dot(ac, bp);
+    if (d3 >= 0 && d4 <= d3) { S.type = T_PP; S.ns = 2; S.st[0] = 0; S.st[1] = 2; return S; }
+    double vc = d1 * d4 - d3 * d2;
+    if (vc <= 0 && d1 >= 0 && d3 <= 0) { S.type = T_PE; S.ns = 3; S.st[0] = 0; S.st[1] = 1; S.st[2] = 2; return S; }
+    D3 cp = p - C;
+    double d5 = dot(ab, cp), d6 = dot(ac, cp);
+    if (d6 >= 0 && d5 <= d6) { S.type = T_PP; S.ns = 2; S.st[0] = 0; S.st[1] = 3; return S; }
+    double vb = d5 * d2 - d1 * d6;
+    if (vb <= 0 && d2 >= 0 && d6 <= 0) { S.type = T_PE; S.ns = 3; S.st[0] = 0; S.st[1] = 1; S.st[2] = 3; return S; }
+    double va = d3 * d6 - d5 * d4;
+    if (va <= 0 && (d4 - d3) >= 0 && (d5 - d6) >= 0) { S.type = T_PE; S.ns = 3; S.st[0] = 0; S.st[1] = 2; S.st[2] = 3; return S; }
+    S.type = T_PT; S.ns = 4; S.st[0] = 0; S.st[1] = 1; S.st[2] = 2; S.st[3] = 3;
+    return S;
+}
+
+#define ABD_PARALLEL_EPS 1e-10
+
+__device__ __forceinline__ Stencil classify_ee(const D3* x) {
+    Stencil S;
+    D3 d1 = x[1] - x[0], d2 = x[3] - x[2], r = x[0] - x[2];
+    double a = dot(d1, d1), e = dot(d2, d2), f = dot(d2, r);
+    double c = dot(d1, r), b = dot(d1, d2);
+    double denom = a * e - b * b;
+    double s, t;
+    if (denom > ABD_PARALLEL_EPS * a * e) s = fmin(1.0, fmax(0.0, (b * f - c * e) / denom));
+    else s = 0.0;
+    t = (b * s + f) / e;
+    if (t < 0.0) { t = 0.0; s = fmin(1.0, fmax(0.0, -c / a)); }
+    else if (t > 1.0) { t = 1.0; s = fmin(1.0, fmax(0.0, (b - c) / a)); }
+    bool s_end = (s == 0.0 || s == 1.0), t_end = (t == 0.0 || t == 1.0);
+    int sa = (s == 1.0) ? 1 : 0, tb = (t == 1.0) ? 3 : 2;
+    if (s_end && t_end) { S.type = T_PP; S.ns = 2; S.st[0] = sa; S.st[1] = tb; return S; }
+    if (s_end) { S.type = T_PE; S.ns = 3; S.st[0] = sa; S.st[1] = 2; S.st[2] = 3; return S; }
+    if (t_end) { S.type = T_PE; S.ns = 3; S.st[0] = tb; S.st[1] = 0; S.st[2] = 1; return S; }
+    S.type = T_EE; S.ns = 4; S.st[0] = 0; S.st[1] = 1; S.st[2] = 2; S.st[3] = 3;
+    return S;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Distance evaluation: shared intermediates + per-entry gradient/Hessian
+// ---------------------------------------------------------------------------------------------
+struct DistEval {
+    Stencil S;
+    D3 y[4];
+    // plane form (PT/EE): d2 = s^2/N ; PE: d2 = N/L
+    double al[4], mu[4], nu[4], om[4];
+    D3 a, u, v, n, w;
+    double s, N, L, d2;
+    D3 vxa, axu;
+};
+
+__device__ __forceinline__ void set_coeffs(DistEval& E) {
+    for (int k = 0; k < 4; ++k) E.al[k] = E.mu[k] = E.nu[k] = E.om[k] = 0.0;
+    if (E.S.type == T_PT) {
+        E.al[0] = 1; E.al[1] = -1; E.mu[1] = -1; E.mu[2] = 1; E.nu[1] = -1; E.nu[3] = 1;
+    } else if (E.S.type == T_EE) {
+        E.al[0] = 1; E.al[2] = -1; E.mu[0] = -1; E.mu[1] = 1; E.nu[2] = -1; E.nu[3] = 1;
+    } else if (E.S.type == T_PE) {
+        E.mu[0] = -1; E.mu[1] = 1; E.nu[0] = -1; E.nu[2] = 1; E.om[1] = -1; E.om[2] = 1;
+    }
+}
+
+// x: the pair's 4 slot positions (PT: p,t0,t1,t2; EE: a0,a1,b0,b1)
+__device__ __forceinline__ void dist_eval(int kind, const D3* x, DistEval& E) {
+    E.S = (kind == K_PT) ? classify_pt(x) : classify_ee(x);
+    for (int k = 0; k < E.S.ns; ++k) E.y[k] = x[E.S.st[k]];
+    set_coeffs(E);
+    const int ns = E.S.ns;
+    if (E.S.type == T_PP) {
+        D3 d = E.y[0] - E.y[1];
+        E.d2 = dot(d, d);
+        E.a = d;
+        return;
+    }
+    E.a = mk(0, 0, 0); E.u = mk(0, 0, 0); E.v = mk(0, 0, 0); E.w = mk(0, 0, 0);
+    for (int k = 0; k < ns; ++k) {
+        E.a = E.a + E.al[k] * E.y[k];
+        E.u = E.u + E.mu[k] * E.y[k];
+        E.v = E.v + E.nu[k] * E.y[k];
+        E.w = E.w + E.om[k] * E.y[k];
+    }
+    E.n = cross(E.u, E.v);
+    E.N = dot(E.n, E.n);
+    if (E.S.type == T_PE) {
+        E.L = dot(E.w, E.w);
+        E.d2 = E.N / E.L;
+    } else {
+        E.s = dot(E.a, E.n);
+        E.d2 = E.s * E.s / E.N;
+        E.vxa = cross(E.v, E.a);
+        E.axu = cross(E.a, E.u);
+    }
+}
+
+// distance only (for CCD and energies)
+__device__ __forceinline__ double pair_d2(int kind, const D3* x) {
+    DistEval E;
+    dist_eval(kind, x, E);
+    return E.d2;
+}
+
+// (Dn_i)_{kr} = -mu_i [v]x_{kr} + nu_i [u]x_{kr}
+__device__ __forceinline__ double Dn(const DistEval& E, int i, int k, int r) {
+    return -E.mu[i] * skew_rc(E.v, k, r) + E.nu[i] * skew_rc(E.u, k, r);
+}
+// d s / d y_{i,r}
+__device__ __forceinline__ double gs_at(const DistEval& E, int R) {
+    int i = R / 3, r = R % 3;
+    return E.al[i] * cmp(E.n, r) + E.mu[i] * cmp(E.vxa, r) + E.nu[i] * cmp(E.axu, r);
+}
+// d N / d y_{i,r} = 2 (Dn_i^T n)_r
+__device__ __forceinline__ double gN_at(const DistEval& E, int R) {
+    int i = R / 3, r = R % 3;
+    return 2.0 * (Dn(E, i, 0, r) * E.n.x + Dn(E, i, 1, r) * E.n.y + Dn(E, i, 2, r) * E.n.z);
+}
+__device__ __forceinline__ double gL_at(const DistEval& E, int R) { return 2.0 * E.om[R / 3] * cmp(E.w, R % 3); }
+
+// gradient of d^2 w.r.t. stencil coordinate R (0 <= R < 3 ns)
+__device__ __forceinline__ double d2_grad(const DistEval& E, int R) {
+    if (E.S.type == T_PP) {
+        double g = 2.0 * cmp(E.a, R % 3);
+        return R < 3 ? g : -g;
+    }
+    if (E.S.type == T_PE) return gN_at(E, R) / E.L - E.N / (E.L * E.L) * gL_at(E, R);
+    return 2.0 * E.s / E.N * gs_at(E, R) - E.s * E.s / (E.N * E.N) * gN_at(E, R);
+}
+
+__device__ __forceinline__ double hN_at(const DistEval& E, int R, int C) {
+    int i = R / 3, r = R % 3, j = C / 3, c = C % 3;
+    double t = 0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) t += Dn(E, i, k, r) * Dn(E, j, k, c);
+    return 2.0 * t + 2.0 * (-E.mu[i] * E.nu[j] + E.nu[i] * E.mu[j]) * skew_rc(E.n, r, c);
+}
+
+// Hessian of d^2 w.r.t. stencil coordinates (R, C)
+__device__ __forceinline__ double d2_hess(const DistEval& E, int R, int C) {
+    int i = R / 3, r = R % 3, j = C / 3, c = C % 3;
+    if (E.S.type == T_PP) {
+        if (r != c) return 0.0;
+        return (i == j) ? 2.0 : -2.0;
+    }
+    if (E.S.type == T_PE) {
+        double hL = (r == c) ? 2.0 * E.om[i] * E.om[j] : 0.0;
+        double gNR = gN_at(E, R), gNC = gN_at(E, C), gLR = gL_at(E, R), gLC = gL_at(E, C);
+        double L = E.L, N = E.N;
+        return hN_at(E, R, C) / L - (gNR * gLC + gLR * gNC) / (L * L) + 2.0 * N / (L * L * L) * gLR * gLC -
+               N / (L * L) * hL;
+    }
+    double hs = E.al[i] * E.mu[j] * (-skew_rc(E.v, r, c)) + E.al[i] * E.nu[j] * skew_rc(E.u, r, c) +
+                E.mu[i] * E.al[j] * skew_rc(E.v, r, c) + E.mu[i] * E.nu[j] * (-skew_rc(E.a, r, c)) +
+                E.nu[i] * E.al[j] * (-skew_rc(E.u, r, c)) + E.nu[i] * E.mu[j] * skew_rc(E.a, r, c);
+    double gsR = gs_at(E, R), gsC = gs_at(E, C), gNR = gN_at(E, R), gNC = gN_at(E, C);
+    double s = E.s, N = E.N;
+    return 2.0 / N * gsR * gsC + 2.0 * s / N * hs - 2.0 * s / (N * N) * (gsR * gNC + gNR * gsC) +
+           2.0 * s * s / (N * N * N) * gNR * gNC - s * s / (N * N) * hN_at(E, R, C);
+}
+
+// ---------------------------------------------------------------------------------------------
+// wave-level helpers (workgroups of exactly one 64-lane wave)
+// ---------------------------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Cyclic Jacobi PSD projection of the symmetric m x m matrix A (LDS, row-major), by one wave:
+// A <- V max(Lambda, 0) V^T.  Same rotation order and stopping rule as the oracle's psd_project
+// (Golub & Van Loan 8.5).  V is LDS scratch of m*m doubles.
+__device__ void wave_psd_project(double* A, double* V, int m, int lane) {
+    const int mm = m * m;
+    for (int e = lane; e < mm; e += 64) V[e] = (e / m == e % m) ? 1.0 : 0.0;
+    double f = 0;
+    for (int e = lane; e < mm; e += 64) f += A[e] * A[e];
+    double fro = wave_sum(f);
+    __syncthreads();
+    if (fro == 0.0) return;
+    for (int sweep = 0; sweep < 32; ++sweep) {
+        double o = 0;
+        for (int e = lane; e < mm; e += 64) {
+            int p = e / m, q = e % m;
+            if (q > p) o += A[e] * A[e];
+        }
+        double off = wave_sum(o);
+        if (off <= 1e-32 * fro) break;
+        for (int p = 0; p < m - 1; ++p)
+            for (int q = p + 1; q < m; ++q) {
+                double apq = A[p * m + q];
+                if (apq == 0.0) continue;
+                double app = A[p * m + p], aqq = A[q * m + q];
+                double theta = (aqq - app) / (2.0 * apq);
+                double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+                double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+                __syncthreads();
+                if (lane < m) {
+                    int k = lane;
+                    double akp = A[k * m + p], akq = A[k * m + q];
+                    A[k * m + p] = c * akp - s * akq;
+                    A[k * m + q] = s * akp + c * akq;
+                } else if (lane < 2 * m) {
+                    int k = lane - m;
+                    double vkp = V[k * m + p], vkq = V[k * m + q];
+                    V[k * m + p] = c * vkp - s * vkq;
+                    V[k * m + q] = s * vkp + c * vkq;
+                }
+                __syncthreads();
+                if (lane < m) {
+                    int k = lane;
+                    double apk = A[p * m + k], aqk = A[q * m + k];
+                    A[p * m + k] = c * apk - s * aqk;
+                    A[q * m + k] = s * apk + c * aqk;
+                }
+                __syncthreads();
+            }
+    }
+    // reconstruct with clamped eigenvalues (diagonal of A)
+    double out[3];
+    int cnt = 0;
+    for (int e = lane; e < mm; e += 64) {
+        int r = e / m, c = e % m;
+        double h = 0;
+        for (int k = 0; k < m; ++k) h += V[r * m + k] * fmax(A[k * m + k], 0.0) * V[c * m + k];
+        out[cnt++] = h;
+    }
+    __syncthreads();
+    cnt = 0;
+    for (int e = lane; e < mm; e += 64) A[e] = out[cnt++];
+    __syncthreads();
+}
+
+}  // namespace abd

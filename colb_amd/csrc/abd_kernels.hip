@@ -1,0 +1,1277 @@
+// abd_kernels.hip — CDNA4 (gfx950) kernels of the ABD Newton step with IPC barriers.
+//
+// Data layout in HBM (structure of arrays across bodies and primitives):
+//   rest   double4[nv]  body-frame rest position + owning body (w)
+//   X      double4[nv]  world positions x = c + sum_k xr_k a_k (abd_simple.py:26-35)
+//   q      double[nb*12] DOF vectors [c, a0, a1, a2]; grad/du/x/r/z/p the same layout
+//   diag   double[nb*144] 12x12 diagonal blocks; off/offT double[npairs*144] off-diagonal blocks
+//   cand   uint64[ncand] sorted candidate-pair keys (scene | kind | i | j)
+// Every reduction is per scene, over fixed ranges in a fixed order, so results do not depend
+// on scheduling (the reference's wp.atomic_add sums, abd.py:64, 75, 301-303, do).
+#include <hip/hip_runtime.h>
+#include <climits>
+#include <float.h>
+#include <stdint.h>
+
+#include "abd_device.hpp"
+#include "abd_internal.hpp"
+
+namespace abd {
+
+static inline int cdiv(long a, int b) { return (int)((a + b - 1) / b); }
+
+// ---------------------------------------------------------------------------------------------
+// state kernels
+// ---------------------------------------------------------------------------------------------
+// tildeA / tildep (abd.py:160-166): q~ = q_now + dt*qdot + dt^2*(g, 0, 0, 0)
+__global__ void k_qtilde(int n, const double* __restrict__ qnow, const double* __restrict__ qdot, double dt,
+                         double g0, double g1, double g2, double* __restrict__ qt) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int k = i % 12;
+    double v = qnow[i] + dt * qdot[i];
+    if (k < 3) v += dt * dt * (k == 0 ? g0 : (k == 1 ? g1 : g2));
+    qt[i] = v;
+}
+
+void launch_qtilde(hipStream_t s, int nb, const double* qnow, const double* qdot, double dt, const double* g, double* qt) {
+    int n = nb * 12;
+    hipLaunchKernelGGL(k_qtilde, dim3(cdiv(n, 256)), dim3(256), 0, s, n, qnow, qdot, dt, g[0], g[1], g[2], qt);
+}
+
+// compute_V (abd_simple.py:26-35) at q - alpha_s * du
+__global__ void k_xform(Geo G, const double* __restrict__ q, const double* __restrict__ du,
+                        const double* __restrict__ alpha, double4* __restrict__ X) {
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= G.nv) return;
+    double4 r = G.rest[v];
+    int b = (int)r.w;
+    const double* qb = q + 12 * b;
+    double a = 0.0;
+    if (du) a = alpha ? alpha[G.bscene[b]] : 1.0;
+    double c[12];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) c[k] = du ? qb[k] - a * du[12 * b + k] : qb[k];
+    X[v] = make_double4(c[0] + r.x * c[3] + r.y * c[6] + r.z * c[9], c[1] + r.x * c[4] + r.y * c[7] + r.z * c[10],
+                        c[2] + r.x * c[5] + r.y * c[8] + r.z * c[11], 0.0);
+}
+
+void launch_xform(hipStream_t s, const Geo& G, const double* q, const double* du, const double* alpha, double4* X) {
+    hipLaunchKernelGGL(k_xform, dim3(cdiv(G.nv, 256)), dim3(256), 0, s, G, q, du, alpha, X);
+}
+
+// ---------------------------------------------------------------------------------------------
+// LBVH broad phase (Karras 2012): Morton keys prefixed by the scene index, so each scene's
+// primitives form one subtree whose root is recorded per scene.
+// ---------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_scene_bounds(Geo G, const double4* __restrict__ X0, const double4* __restrict__ X1,
+                                                      float* __restrict__ bounds) {
+    int s = blockIdx.x;
+    int v0 = G.scene_v0[s], v1 = G.scene_v0[s + 1];
+    float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+    for (int v = v0 + threadIdx.x; v < v1; v += blockDim.x) {
+        double4 a = X0[v], b = X1[v];
+        double pa[3] = {a.x, a.y, a.z}, pb[3] = {b.x, b.y, b.z};
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = fminf(lo[k], (float)fmin(pa[k], pb[k]));
+            hi[k] = fmaxf(hi[k], (float)fmax(pa[k], pb[k]));
+        }
+    }
+    __shared__ float sh[6][256];
+    for (int k = 0; k < 3; ++k) { sh[k][threadIdx.x] = lo[k]; sh[3 + k][threadIdx.x] = hi[k]; }
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o)
+            for (int k = 0; k < 3; ++k) {
+                sh[k][threadIdx.x] = fminf(sh[k][threadIdx.x], sh[k][threadIdx.x + o]);
+                sh[3 + k][threadIdx.x] = fmaxf(sh[3 + k][threadIdx.x], sh[3 + k][threadIdx.x + o]);
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) bounds[6 * s + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+void launch_scene_bounds(hipStream_t s, const Geo& G, const double4* X0, const double4* X1, float* bounds) {
+    hipLaunchKernelGGL(k_scene_bounds, dim3(G.ns), dim3(256), 0, s, G, X0, X1, bounds);
+}
+
+__device__ __forceinline__ uint32_t expand_bits10(uint32_t v) {
+    v = (v * 0x00010001u) & 0xFF0000FFu;
+    v = (v * 0x00000101u) & 0x0F00F00Fu;
+    v = (v * 0x00000011u) & 0xC30C30C3u;
+    v = (v * 0x00000005u) & 0x49249249u;
+    return v;
+}
+
+__device__ __forceinline__ uint32_t morton30(float x, float y, float z) {
+    uint32_t ix = (uint32_t)fminf(fmaxf(x * 1024.0f, 0.0f), 1023.0f);
+    uint32_t iy = (uint32_t)fminf(fmaxf(y * 1024.0f, 0.0f), 1023.0f);
+    uint32_t iz = (uint32_t)fminf(fmaxf(z * 1024.0f, 0.0f), 1023.0f);
+    return (expand_bits10(ix) << 2) | (expand_bits10(iy) << 1) | expand_bits10(iz);
+}
+
+// tri_mode 1: triangles (box of the swept triangle); 0: edges (inflated by infl)
+__global__ void k_prim_boxes(Geo G, int tri_mode, const double4* __restrict__ X0, const double4* __restrict__ X1,
+                             double infl, const float* __restrict__ bounds, float4* __restrict__ boxes,
+                             uint64_t* __restrict__ keys, int* __restrict__ idx) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int n = tri_mode ? G.nt : G.ne;
+    if (i >= n) return;
+    int vs[3], nvp;
+    if (tri_mode) {
+        int4 t = G.tris[i];
+        vs[0] = t.x; vs[1] = t.y; vs[2] = t.z; nvp = 3;
+    } else {
+        int2 e = G.edges[i];
+        vs[0] = e.x; vs[1] = e.y; vs[2] = e.x; nvp = 2;
+    }
+    double lo[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, hi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+    for (int k = 0; k < nvp; ++k) {
+        double4 a = X0[vs[k]], b = X1[vs[k]];
+        double pa[3] = {a.x, a.y, a.z}, pb[3] = {b.x, b.y, b.z};
+        for (int c = 0; c < 3; ++c) {
+            lo[c] = fmin(lo[c], fmin(pa[c], pb[c]));
+            hi[c] = fmax(hi[c], fmax(pa[c], pb[c]));
+        }
+    }
+    float flo[3], fhi[3];
+    for (int c = 0; c < 3; ++c) {
+        flo[c] = __double2float_rd(lo[c] - infl);
+        fhi[c] = __double2float_ru(hi[c] + infl);
+    }
+    boxes[2 * i] = make_float4(flo[0], flo[1], flo[2], 0.f);
+    boxes[2 * i + 1] = make_float4(fhi[0], fhi[1], fhi[2], 0.f);
+    int s = G.bscene[(int)G.rest[vs[0]].w];
+    const float* B = bounds + 6 * s;
+    float c3[3];
+    for (int c = 0; c < 3; ++c) {
+        float ext = fmaxf(B[3 + c] - B[c], 1e-12f);
+        c3[c] = (0.5f * (flo[c] + fhi[c]) - B[c]) / ext;
+    }
+    keys[i] = ((uint64_t)s << 32) | morton30(c3[0], c3[1], c3[2]);
+    idx[i] = i;
+}
+
+void launch_prim_boxes(hipStream_t s, const Geo& G, int tri_mode, const double4* X0, const double4* X1, double infl,
+                       const float* bounds, float4* boxes, uint64_t* keys, int* idx) {
+    int n = tri_mode ? G.nt : G.ne;
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_prim_boxes, dim3(cdiv(n, 256)), dim3(256), 0, s, G, tri_mode, X0, X1, infl, bounds, boxes, keys, idx);
+}
+
+__device__ __forceinline__ int lcp_delta(const uint64_t* __restrict__ k, int n, int i, int j) {
+    if (j < 0 || j >= n) return -1;
+    uint64_t a = k[i], b = k[j];
+    if (a == b) return 64 + __clz((unsigned)(i ^ j));
+    return __clzll(a ^ b);
+}
+
+__global__ void k_bvh_build(int n, int ns, const uint64_t* __restrict__ k, BvhNode* __restrict__ nodes,
+                            int* __restrict__ parent, int* __restrict__ leaf_parent, int* __restrict__ scene_root) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {  // a scene with a single primitive is rooted at its leaf
+        int s = (int)(k[i] >> 32);
+        bool first = (i == 0) || (int)(k[i - 1] >> 32) != s;
+        bool last = (i == n - 1) || (int)(k[i + 1] >> 32) != s;
+        if (first && last) scene_root[s] = ~i;
+    }
+    if (i >= n - 1) return;
+    int d = (lcp_delta(k, n, i, i + 1) - lcp_delta(k, n, i, i - 1)) >= 0 ? 1 : -1;
+    int dmin = lcp_delta(k, n, i, i - d);
+    int lmax = 2;
+    while (lcp_delta(k, n, i, i + lmax * d) > dmin) lmax *= 2;
+    int l = 0;
+    for (int t = lmax / 2; t >= 1; t /= 2)
+        if (lcp_delta(k, n, i, i + (l + t) * d) > dmin) l += t;
+    int j = i + l * d;
+    int dnode = lcp_delta(k, n, i, j);
+    int s = 0, t = l;
+    do {
+        t = (t + 1) >> 1;
+        if (lcp_delta(k, n, i, i + (s + t) * d) > dnode) s += t;
+    } while (t > 1);
+    int gamma = i + s * d + min(d, 0);
+    int lo = min(i, j), hi = max(i, j);
+    int left = (lo == gamma) ? ~gamma : gamma;
+    int right = (hi == gamma + 1) ? ~(gamma + 1) : gamma + 1;
+    nodes[i].left = left;
+    nodes[i].right = right;
+    if (left < 0) leaf_parent[~left] = i; else parent[left] = i;
+    if (right < 0) leaf_parent[~right] = i; else parent[right] = i;
+    if (i == 0) parent[0] = -1;
+    int sc = (int)(k[lo] >> 32);
+    if ((int)(k[hi] >> 32) == sc && (lo == 0 || (int)(k[lo - 1] >> 32) != sc) &&
+        (hi == n - 1 || (int)(k[hi + 1] >> 32) != sc))
+        scene_root[sc] = i;
+}
+
+void launch_bvh_build(hipStream_t s, int n, int ns, const uint64_t* skeys, BvhNode* nodes, int* parent, int* leaf_parent,
+                      int* scene_root) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_bvh_build, dim3(cdiv(n, 256)), dim3(256), 0, s, n, ns, skeys, nodes, parent, leaf_parent, scene_root);
+}
+
+// bottom-up refit: the second child to arrive at a node writes the union upwards
+__global__ void k_bvh_refit(int n, const int* __restrict__ sidx, const float4* __restrict__ boxes,
+                            BvhNode* nodes, const int* __restrict__ parent, const int* __restrict__ leaf_parent,
+                            int* flags, int* err) {
+    int leaf = blockIdx.x * blockDim.x + threadIdx.x;
+    if (leaf >= n || n < 2) return;
+    int prim = sidx[leaf];
+    if (prim < 0 || prim >= n) { atomicOr(err, ERR_BVH); return; }
+    float4 blo = boxes[2 * prim], bhi = boxes[2 * prim + 1];
+    float lo[3] = {blo.x, blo.y, blo.z}, hi[3] = {bhi.x, bhi.y, bhi.z};
+    int child = ~leaf;
+    int node = leaf_parent[leaf];
+    int guard = 0;
+    while (node >= 0) {
+        if (node >= n - 1 || ++guard > 2 * n) { atomicOr(err, ERR_BVH); return; }
+        volatile BvhNode* nd = nodes + node;
+        bool is_left = (nd->left == child);
+        for (int c = 0; c < 3; ++c) {
+            if (is_left) { nd->lo_l[c] = lo[c]; nd->hi_l[c] = hi[c]; }
+            else { nd->lo_r[c] = lo[c]; nd->hi_r[c] = hi[c]; }
+        }
+        __threadfence();
+        int old = atomicAdd(&flags[node], 1);
+        if (old == 0) return;
+        __threadfence();
+        for (int c = 0; c < 3; ++c) {
+            lo[c] = fminf(nd->lo_l[c], nd->lo_r[c]);
+            hi[c] = fmaxf(nd->hi_l[c], nd->hi_r[c]);
+        }
+        child = node;
+        node = parent[node];
+    }
+}
+
+void launch_bvh_refit(hipStream_t s, int n, const int* sidx, const float4* boxes, BvhNode* nodes, const int* parent,
+                      const int* leaf_parent, int* flags, int* err) {
+    if (n < 2) return;
+    hipLaunchKernelGGL(k_bvh_refit, dim3(cdiv(n, 256)), dim3(256), 0, s, n, sidx, boxes, nodes, parent, leaf_parent, flags,
+                       err);
+}
+
+__device__ __forceinline__ bool ov(const float* qlo, const float* qhi, const float* lo, const float* hi) {
+    return !(qlo[0] > hi[0] || lo[0] > qhi[0] || qlo[1] > hi[1] || lo[1] > qhi[1] || qlo[2] > hi[2] || lo[2] > qhi[2]);
+}
+
+#define BVH_STACK 64
+
+// Calls f(leaf_sorted_index) for every leaf whose box overlaps the query box.
+template <class F>
+__device__ __forceinline__ bool bvh_query(const BvhNode* __restrict__ nodes, const float4* __restrict__ boxes,
+                                          const int* __restrict__ sidx, int n, int root, const float* qlo,
+                                          const float* qhi, F&& f) {
+    if (root == INT_MIN) return true;
+    if (root < 0) {
+        if (~root >= n) return false;
+        int prim = sidx[~root];
+        if (prim < 0 || prim >= n) return false;
+        float4 a = boxes[2 * prim], b = boxes[2 * prim + 1];
+        float lo[3] = {a.x, a.y, a.z}, hi[3] = {b.x, b.y, b.z};
+        if (ov(qlo, qhi, lo, hi)) f(~root);
+        return true;
+    }
+    int stack[BVH_STACK];
+    int sp = 0;
+    stack[sp++] = root;
+    while (sp > 0) {
+        int ni = stack[--sp];
+        if (ni < 0 || ni >= n - 1) return false;
+        const BvhNode nd = nodes[ni];
+        bool l = ov(qlo, qhi, nd.lo_l, nd.hi_l), r = ov(qlo, qhi, nd.lo_r, nd.hi_r);
+        if ((nd.left < 0 && ~nd.left >= n) || (nd.right < 0 && ~nd.right >= n)) return false;
+        if (l) {
+            if (nd.left < 0) f(~nd.left);
+            else if (sp < BVH_STACK) stack[sp++] = nd.left;
+            else return false;
+        }
+        if (r) {
+            if (nd.right < 0) f(~nd.right);
+            else if (sp < BVH_STACK) stack[sp++] = nd.right;
+            else return false;
+        }
+    }
+    return true;
+}
+
+__device__ __forceinline__ void emit(uint64_t key, uint64_t* cand, int cap, int* count, int* overflow) {
+    int slot = atomicAdd(count, 1);
+    if (slot < cap) cand[slot] = key;
+    else *overflow = 1;
+}
+
+// point-triangle candidates (one thread per vertex) + point-ground candidates
+__global__ void k_query_pt(Geo G, const double4* __restrict__ X0, const double4* __restrict__ X1, double infl,
+                           int ground, double gn0, double gn1, double gn2, double goff,
+                           const BvhNode* __restrict__ nodes, const int* __restrict__ sidx,
+                           const int* __restrict__ scene_root, const float4* __restrict__ boxes, uint64_t* cand,
+                           int cap, int* count, int* overflow) {
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= G.nv) return;
+    int b = (int)G.rest[v].w;
+    int s = G.bscene[b];
+    int pin = G.pinned[b];
+    double4 a = X0[v], c = X1[v];
+    float qlo[3] = {__double2float_rd(fmin(a.x, c.x) - infl), __double2float_rd(fmin(a.y, c.y) - infl),
+                    __double2float_rd(fmin(a.z, c.z) - infl)};
+    float qhi[3] = {__double2float_ru(fmax(a.x, c.x) + infl), __double2float_ru(fmax(a.y, c.y) + infl),
+                    __double2float_ru(fmax(a.z, c.z) + infl)};
+    int vl = v - G.scene_v0[s];
+    int t0 = G.scene_t0[s];
+    if (G.nt > 0) {
+        bool ok = bvh_query(nodes, boxes, sidx, G.nt, scene_root[s], qlo, qhi, [&](int leaf) {
+            int t = sidx[leaf];
+            if (t < 0 || t >= G.nt) { atomicOr(G.err, ERR_BVH); return; }
+            int tb = G.tris[t].w;
+            if (tb == b || (pin && G.pinned[tb])) return;
+            emit(make_key(s, K_PT, vl, t - t0), cand, cap, count, overflow);
+        });
+        if (!ok) { *overflow = 2; atomicOr(G.err, ERR_BVH); }
+    }
+    if (ground && !pin) {
+        double d0 = gn0 * a.x + gn1 * a.y + gn2 * a.z - goff, d1 = gn0 * c.x + gn1 * c.y + gn2 * c.z - goff;
+        if (fmin(d0, d1) < infl) emit(make_key(s, K_PG, vl, 0), cand, cap, count, overflow);
+    }
+}
+
+void launch_query_pt(hipStream_t s, const Geo& G, const double4* X0, const double4* X1, double infl, const AbdParams& P,
+                     const BvhNode* nodes, const int* sidx, const int* scene_root, const float4* boxes,
+                     uint64_t* cand, int cap, int* count, int* overflow) {
+    hipLaunchKernelGGL(k_query_pt, dim3(cdiv(G.nv, 128)), dim3(128), 0, s, G, X0, X1, infl, P.ground_enabled,
+                       P.ground_normal[0], P.ground_normal[1], P.ground_normal[2], P.ground_offset, nodes, sidx,
+                       scene_root, boxes, cand, cap, count, overflow);
+}
+
+// edge-edge candidates (one thread per edge, partners with a larger index; contact.py:160)
+__global__ void k_query_ee(Geo G, const BvhNode* __restrict__ nodes, const int* __restrict__ sidx,
+                           const int* __restrict__ scene_root, const float4* __restrict__ boxes, uint64_t* cand,
+                           int cap, int* count, int* overflow) {
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= G.ne) return;
+    int2 ev = G.edges[e];
+    int b = (int)G.rest[ev.x].w;
+    int s = G.bscene[b];
+    int pin = G.pinned[b];
+    float4 lo4 = boxes[2 * e], hi4 = boxes[2 * e + 1];
+    float qlo[3] = {lo4.x, lo4.y, lo4.z}, qhi[3] = {hi4.x, hi4.y, hi4.z};
+    int e0 = G.scene_e0[s];
+    bool ok = bvh_query(nodes, boxes, sidx, G.ne, scene_root[s], qlo, qhi, [&](int leaf) {
+        int f = sidx[leaf];
+        if (f < 0 || f >= G.ne) { atomicOr(G.err, ERR_BVH); return; }
+        if (f <= e) return;
+        int fb = (int)G.rest[G.edges[f].x].w;
+        if (fb == b || (pin && G.pinned[fb])) return;
+        emit(make_key(s, K_EE, e - e0, f - e0), cand, cap, count, overflow);
+    });
+    if (!ok) { *overflow = 2; atomicOr(G.err, ERR_BVH); }
+}
+
+void launch_query_ee(hipStream_t s, const Geo& G, const BvhNode* nodes, const int* sidx, const int* scene_root,
+                     const float4* boxes, uint64_t* cand, int cap, int* count, int* overflow) {
+    if (G.ne == 0) return;
+    hipLaunchKernelGGL(k_query_ee, dim3(cdiv(G.ne, 128)), dim3(128), 0, s, G, nodes, sidx, scene_root, boxes, cand,
+                       cap, count, overflow);
+}
+
+__global__ void k_cand_ranges(int n, const uint64_t* __restrict__ k, int* lo, int* hi) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int s = key_scene(k[i]);
+    if (i == 0 || key_scene(k[i - 1]) != s) lo[s] = i;
+    if (i == n - 1 || key_scene(k[i + 1]) != s) hi[s] = i + 1;
+}
+
+void launch_cand_ranges(hipStream_t s, int n, const uint64_t* keys, int* lo, int* hi) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_cand_ranges, dim3(cdiv(n, 256)), dim3(256), 0, s, n, keys, lo, hi);
+}
+
+// ---------------------------------------------------------------------------------------------
+// narrow phase
+// ---------------------------------------------------------------------------------------------
+struct PairV {
+    int kind, scene, nvp;
+    int v[4];
+};
+
+// Decodes a candidate key into its vertices.  Every index is range-checked against the scene's
+// primitive ranges; a key that fails sets ERR_DECODE and decodes to kind -1 (callers skip it).
+__device__ __forceinline__ PairV decode(const Geo& G, uint64_t key) {
+    PairV p;
+    p.kind = key_kind(key);
+    p.scene = key_scene(key);
+    p.nvp = 0;
+    p.v[0] = p.v[1] = p.v[2] = p.v[3] = 0;
+    int il = key_il(key), jl = key_jl(key);
+    if (p.scene >= G.ns) { atomicOr(G.err, ERR_DECODE); p.kind = -1; p.scene = 0; return p; }
+    const int s = p.scene;
+    if (p.kind == K_PT) {
+        int v = G.scene_v0[s] + il, t = G.scene_t0[s] + jl;
+        if (v >= G.scene_v0[s + 1] || t >= G.scene_t0[s + 1]) { atomicOr(G.err, ERR_DECODE); p.kind = -1; return p; }
+        p.v[0] = v;
+        int4 tr = G.tris[t];
+        p.v[1] = tr.x; p.v[2] = tr.y; p.v[3] = tr.z;
+        p.nvp = 4;
+    } else if (p.kind == K_EE) {
+        int e0 = G.scene_e0[s];
+        if (e0 + il >= G.scene_e0[s + 1] || e0 + jl >= G.scene_e0[s + 1]) { atomicOr(G.err, ERR_DECODE); p.kind = -1; return p; }
+        int2 a = G.edges[e0 + il], b = G.edges[e0 + jl];
+        p.v[0] = a.x; p.v[1] = a.y; p.v[2] = b.x; p.v[3] = b.y;
+        p.nvp = 4;
+    } else if (p.kind == K_PG) {
+        int v = G.scene_v0[s] + il;
+        if (v >= G.scene_v0[s + 1]) { atomicOr(G.err, ERR_DECODE); p.kind = -1; return p; }
+        p.v[0] = v;
+        p.v[1] = p.v[2] = p.v[3] = v;
+        p.nvp = 1;
+    } else {
+        atomicOr(G.err, ERR_DECODE);
+        p.kind = -1;
+    }
+    return p;
+}
+
+__device__ __forceinline__ D3 ld3(const double4* X, int v) {
+    double4 a = X[v];
+    return mk(a.x, a.y, a.z);
+}
+
+__device__ __forceinline__ double cand_d2(const Geo& G, const PairV& p, const double4* X, const AbdParams& P) {
+    if (p.kind == K_PG) {
+        D3 x = ld3(X, p.v[0]);
+        double d = P.ground_normal[0] * x.x + P.ground_normal[1] * x.y + P.ground_normal[2] * x.z - P.ground_offset;
+        return d * d;
+    }
+    D3 x[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) x[k] = ld3(X, p.v[k]);
+    return pair_d2(p.kind, x);
+}
+
+// energy_contact (abd.py:66-75): dt^2 k B(d^2) for active pairs (d < dhat)
+__global__ void k_cand_eval(Geo G, int n, const uint64_t* __restrict__ keys, const double4* __restrict__ X, AbdParams P,
+                            int* __restrict__ flag, double* __restrict__ energy) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    PairV p = decode(G, keys[i]);
+    if (p.kind < 0) {
+        if (flag) flag[i] = 0;
+        if (energy) energy[i] = 0.0;
+        return;
+    }
+    double d2 = cand_d2(G, p, X, P);
+    double shat = P.dhat * P.dhat;
+    double B, B1, B2;
+    barrier3(d2, shat, B, B1, B2);
+    if (flag) flag[i] = d2 < shat ? 1 : 0;
+    if (energy) energy[i] = P.dt * P.dt * P.k_contact * B;
+}
+
+void launch_cand_eval(hipStream_t s, const Geo& G, int n, const uint64_t* keys, const double4* X, const AbdParams& P,
+                      int* flag, double* energy) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_cand_eval, dim3(cdiv(n, 256)), dim3(256), 0, s, G, n, keys, X, P, flag, energy);
+}
+
+__global__ void k_pair_d2(Geo G, int n, const uint64_t* __restrict__ keys, const double4* __restrict__ X, AbdParams P,
+                          double* __restrict__ d2) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    PairV p = decode(G, keys[i]);
+    d2[i] = (p.kind < 0) ? 0.0 : cand_d2(G, p, X, P);
+}
+
+void launch_pair_d2(hipStream_t s, const Geo& G, int n, const uint64_t* keys, const double4* X, const AbdParams& P,
+                    double* d2) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_pair_d2, dim3(cdiv(n, 256)), dim3(256), 0, s, G, n, keys, X, P, d2);
+}
+
+__global__ void k_compact(int n, const int* __restrict__ flag, const int* __restrict__ pos,
+                          const uint64_t* __restrict__ keys, uint64_t* __restrict__ out) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && flag[i]) out[pos[i]] = keys[i];
+}
+
+void launch_compact(hipStream_t s, int n, const int* flag, const int* pos, const uint64_t* keys, uint64_t* out) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_compact, dim3(cdiv(n, 256)), dim3(256), 0, s, n, flag, pos, keys, out);
+}
+
+// contact_hessian_ee (abd.py:198-571), one wave per active contact: d^2 gradient/Hessian,
+// barrier chain rule, PSD projection of the whole barrier Hessian, then the affine Jacobian
+// J = [1, xr^T] (x) I3 (abd.py:256-263) maps it to the three 12x12 body blocks
+// (A,A), (B,B), (A,B) and two 12-vectors.
+__global__ __launch_bounds__(64) void k_contact_hess(Geo G, int n, const uint64_t* __restrict__ act,
+                                                      const double4* __restrict__ X, AbdParams P,
+                                                      int2* __restrict__ body, double* __restrict__ g24,
+                                                      double* __restrict__ H432) {
+    __shared__ double A[144], V[144], gx[12], jv[4][4];
+    __shared__ int side[4];
+    const int c = blockIdx.x;
+    const int lane = threadIdx.x;
+    if (c >= n) return;
+    PairV p = decode(G, act[c]);
+    if (p.kind < 0) {
+        // invalid key (already flagged): contribute nothing
+        for (int e = lane; e < 432; e += 64) H432[(size_t)c * 432 + e] = 0.0;
+        if (lane < 24) g24[(size_t)c * 24 + lane] = 0.0;
+        if (lane == 0) body[c] = make_int2(-1, -1);
+        return;
+    }
+    const double kd = P.k_contact * P.dt * P.dt;
+    const double shat = P.dhat * P.dhat;
+    int m, ns;
+    int st[4] = {0, 1, 2, 3};
+    if (p.kind == K_PG) {
+        D3 x = ld3(X, p.v[0]);
+        D3 nrm = mk(P.ground_normal[0], P.ground_normal[1], P.ground_normal[2]);
+        double d = dot(nrm, x) - P.ground_offset;
+        double B, B1, B2;
+        barrier3(d * d, shat, B, B1, B2);
+        double lam = fmax(0.0, kd * (2.0 * B1 + 4.0 * B2 * d * d));
+        m = 3; ns = 1;
+        if (lane < 9) A[lane] = lam * cmp(nrm, lane / 3) * cmp(nrm, lane % 3);
+        if (lane < 3) gx[lane] = kd * B1 * 2.0 * d * cmp(nrm, lane);
+        __syncthreads();
+    } else {
+        D3 x[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) x[k] = ld3(X, p.v[k]);
+        DistEval E;
+        dist_eval(p.kind, x, E);
+        ns = E.S.ns;
+        m = 3 * ns;
+        for (int k = 0; k < 4; ++k) st[k] = E.S.st[k];
+        double B, B1, B2;
+        barrier3(E.d2, shat, B, B1, B2);
+        for (int e = lane; e < m * m; e += 64) {
+            int R = e / m, C = e % m;
+            A[e] = kd * (B1 * d2_hess(E, R, C) + B2 * d2_grad(E, R) * d2_grad(E, C));
+        }
+        if (lane < m) gx[lane] = kd * B1 * d2_grad(E, lane);
+        __syncthreads();
+        wave_psd_project(A, V, m, lane);
+    }
+    // stencil point k -> vertex -> body side and Jacobian row [1, xr]
+    const int bA = (int)G.rest[p.v[0]].w;
+    const int bB = (p.kind == K_PG) ? -1 : (int)G.rest[p.v[3]].w;
+    if (lane < ns) {
+        int vid = p.v[st[lane]];
+        double4 r = G.rest[vid];
+        side[lane] = ((int)r.w == bA) ? 0 : 1;
+        jv[lane][0] = 1.0; jv[lane][1] = r.x; jv[lane][2] = r.y; jv[lane][3] = r.z;
+    }
+    __syncthreads();
+    double* Hout = H432 + (size_t)c * 432;
+    for (int e = lane; e < 432; e += 64) {
+        int blk = e / 144, rc = e % 144;
+        int R = rc / 12, C = rc % 12;
+        int i = R / 3, r = R % 3, j = C / 3, cc = C % 3;
+        int sa = (blk == 1) ? 1 : 0, sb = (blk == 0) ? 0 : 1;
+        double h = 0.0;
+        for (int k = 0; k < ns; ++k) {
+            if (side[k] != sa) continue;
+            for (int l = 0; l < ns; ++l) {
+                if (side[l] != sb) continue;
+                h += jv[k][i] * jv[l][j] * A[(3 * k + r) * m + 3 * l + cc];
+            }
+        }
+        Hout[e] = h;
+    }
+    if (lane < 24) {
+        int sd = lane / 12, i = (lane % 12) / 3, r = lane % 3;
+        double g = 0.0;
+        for (int k = 0; k < ns; ++k)
+            if (side[k] == sd) g += jv[k][i] * gx[3 * k + r];
+        g24[(size_t)c * 24 + lane] = g;
+    }
+    if (lane == 0) body[c] = make_int2(bA, bB);
+}
+
+void launch_contact_hess(hipStream_t s, const Geo& G, int n, const uint64_t* act, const double4* X, const AbdParams& P,
+                         int2* body, double* g24, double* H432) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_contact_hess, dim3(n), dim3(64), 0, s, G, n, act, X, P, body, g24, H432);
+}
+
+// incidence keys (body << 32 | contact << 1 | role) and body-pair keys (min(20) | max(20) | contact(24))
+__global__ void k_incidence(Geo G, int n, const int2* __restrict__ body, uint64_t* inc, int* n_inc, uint64_t* pairk,
+                            int* n_pair) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    int2 b = body[c];
+    if (b.x < 0 || b.x >= G.nb || b.y >= G.nb) {
+        if (b.x != -1) atomicOr(G.err, ERR_BODY);
+        return;
+    }
+    bool dA = !G.pinned[b.x];
+    bool dB = b.y >= 0 && !G.pinned[b.y];
+    if (dA) inc[atomicAdd(n_inc, 1)] = ((uint64_t)b.x << 32) | ((uint64_t)c << 1);
+    if (dB) inc[atomicAdd(n_inc, 1)] = ((uint64_t)b.y << 32) | ((uint64_t)c << 1) | 1ull;
+    if (dA && dB) {
+        uint64_t lo = min(b.x, b.y), hi = max(b.x, b.y);
+        pairk[atomicAdd(n_pair, 1)] = (lo << 44) | (hi << 24) | (uint64_t)c;
+    }
+}
+
+void launch_incidence(hipStream_t s, const Geo& G, int n, const int2* body, uint64_t* inc, int* n_inc, uint64_t* pairk,
+                      int* n_pair) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_incidence, dim3(cdiv(n, 256)), dim3(256), 0, s, G, n, body, inc, n_inc, pairk, n_pair);
+}
+
+__global__ void k_run_heads(int n, const uint64_t* __restrict__ k, int shift, int* head) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    head[i] = (i == 0 || (k[i] >> shift) != (k[i - 1] >> shift)) ? 1 : 0;
+}
+
+void launch_run_heads(hipStream_t s, int n, const uint64_t* sorted, int shift, int* head) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_run_heads, dim3(cdiv(n, 256)), dim3(256), 0, s, n, sorted, shift, head);
+}
+
+__global__ void k_pair_unique(int n, const uint64_t* __restrict__ k, const int* __restrict__ head,
+                              const int* __restrict__ pos, int* pair_b, int* run_start, int* n_unique) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (head[i]) {
+        int u = pos[i];
+        pair_b[2 * u] = (int)(k[i] >> 44);
+        pair_b[2 * u + 1] = (int)((k[i] >> 24) & 0xFFFFF);
+        run_start[u] = i;
+    }
+    if (i == n - 1) {
+        int total = pos[i] + head[i];
+        run_start[total] = n;
+        *n_unique = total;
+    }
+}
+
+void launch_pair_unique(hipStream_t s, int n, const uint64_t* sorted, const int* head, const int* pos, int* pair_b,
+                        int* run_start, int* n_unique) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_pair_unique, dim3(cdiv(n, 256)), dim3(256), 0, s, n, sorted, head, pos, pair_b, run_start,
+                       n_unique);
+}
+
+// off-diagonal block (bmin, bmax) = sum over its contacts of H_AB (or H_AB^T when the contact's
+// first body is bmax), in sorted contact order
+__global__ void k_offblock_sum(int n_pairs, const int* __restrict__ pair_b, const int* __restrict__ run_start,
+                               const uint64_t* __restrict__ k, const int2* __restrict__ body,
+                               const double* __restrict__ H432, double* __restrict__ off, double* __restrict__ offT,
+                               int n_act, int* err) {
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)n_pairs * 144) return;
+    int u = (int)(t / 144), e = (int)(t % 144);
+    int R = e / 12, C = e % 12;
+    int bmin = pair_b[2 * u];
+    double h = 0.0;
+    for (int m = run_start[u]; m < run_start[u + 1]; ++m) {
+        int c = (int)(k[m] & 0xFFFFFF);
+        if (c >= n_act) { atomicOr(err, ERR_PAIR); break; }
+        const double* Hc = H432 + (size_t)c * 432 + 288;
+        h += (body[c].x == bmin) ? Hc[R * 12 + C] : Hc[C * 12 + R];
+    }
+    off[(size_t)u * 144 + e] = h;
+    offT[(size_t)u * 144 + C * 12 + R] = h;
+}
+
+void launch_offblock_sum(hipStream_t s, int n_pairs, const int* pair_b, const int* run_start, int n_keys,
+                         const uint64_t* sorted, const int2* body, const double* H432, double* off, double* offT,
+                         int n_act, int* err) {
+    if (n_pairs <= 0) return;
+    long n = (long)n_pairs * 144;
+    hipLaunchKernelGGL(k_offblock_sum, dim3(cdiv(n, 256)), dim3(256), 0, s, n_pairs, pair_b, run_start, sorted, body,
+                       H432, off, offT, n_act, err);
+}
+
+__global__ void k_csr_keys(int n_pairs, const int* __restrict__ pair_b, uint64_t* keys, int* vals) {
+    int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= n_pairs) return;
+    uint64_t a = (uint64_t)pair_b[2 * u], b = (uint64_t)pair_b[2 * u + 1];
+    keys[2 * u] = (a << 32) | b;
+    vals[2 * u] = u << 1;
+    keys[2 * u + 1] = (b << 32) | a;
+    vals[2 * u + 1] = (u << 1) | 1;
+}
+
+void launch_csr_keys(hipStream_t s, int n_pairs, const int* pair_b, uint64_t* keys, int* vals) {
+    if (n_pairs <= 0) return;
+    hipLaunchKernelGGL(k_csr_keys, dim3(cdiv(n_pairs, 256)), dim3(256), 0, s, n_pairs, pair_b, keys, vals);
+}
+
+__device__ __forceinline__ int lower_bound64(const uint64_t* a, int n, uint64_t x) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (a[mid] < x) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+__global__ void k_row_ptr(int nb, int n_ent, const uint64_t* __restrict__ k, int* row_ptr) {
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b > nb) return;
+    row_ptr[b] = lower_bound64(k, n_ent, (uint64_t)b << 32);
+}
+
+void launch_row_ptr(hipStream_t s, int nb, int n_ent, const uint64_t* sorted, int* row_ptr) {
+    hipLaunchKernelGGL(k_row_ptr, dim3(cdiv(nb + 1, 256)), dim3(256), 0, s, nb, n_ent, sorted, row_ptr);
+}
+
+// ---------------------------------------------------------------------------------------------
+// per-body assembly: inertia_grad_hess (abd.py:108-149) with a 4x4 mass matrix, PSD-projected
+// orthogonality Hessian, the body's contact incidences, the block-Jacobi inverse and the body
+// energy (energy_inertia, abd.py:53-64).  One wave per body.
+// ---------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_body_assemble(Geo G, AbdParams P, const double* __restrict__ q,
+                                                       const double* __restrict__ qtilde, int n_inc,
+                                                       const uint64_t* __restrict__ inc,
+                                                       const double* __restrict__ g24,
+                                                       const double* __restrict__ H432,
+                                                       const int* __restrict__ newton_active,
+                                                       double* __restrict__ grad, double* __restrict__ diag,
+                                                       double* __restrict__ pinv, double* __restrict__ e_body,
+                                                       int n_act) {
+    __shared__ double A[81], V[81], H[144], W[12 * 24], gq[12], fac[12], qb[12], dq[12];
+    const int b = blockIdx.x;
+    const int lane = threadIdx.x;
+    const double pd = P.pinned_diag;
+    if (G.pinned[b]) {
+        for (int e = lane; e < 144; e += 64) {
+            int r = e / 12, c = e % 12;
+            diag[(size_t)b * 144 + e] = (r == c) ? pd : 0.0;
+            pinv[(size_t)b * 144 + e] = (r == c) ? 1.0 / pd : 0.0;
+        }
+        if (lane < 12) grad[12 * b + lane] = 0.0;
+        if (lane == 0) e_body[b] = 0.0;
+        return;
+    }
+    const double dt2 = P.dt * P.dt;
+    const double kap = P.kappa;
+    const double* M = G.mass4 + 16 * b;
+    if (lane < 12) {
+        qb[lane] = q[12 * b + lane];
+        dq[lane] = qb[lane] - qtilde[12 * b + lane];
+    }
+    __syncthreads();
+    const double* a = qb + 3;
+    // ortho gradient / Hessian (ortho.py:19-41)
+    if (lane < 12) {
+        int i = lane / 3, r = lane % 3;
+        double g = 0.0;
+        for (int j = 0; j < 4; ++j) g += M[4 * i + j] * dq[3 * j + r];
+        if (i > 0) {
+            int ii = i - 1;
+            double s = -a[3 * ii + r];
+            for (int j = 0; j < 3; ++j)
+                s += (a[3 * ii] * a[3 * j] + a[3 * ii + 1] * a[3 * j + 1] + a[3 * ii + 2] * a[3 * j + 2]) * a[3 * j + r];
+            g += dt2 * 4.0 * kap * s;
+        }
+        gq[lane] = g;
+    }
+    for (int e = lane; e < 81; e += 64) {
+        int R = e / 9, C = e % 9;
+        int i = R / 3, r = R % 3, j = C / 3, c = C % 3;
+        double Gij = a[3 * i] * a[3 * j] + a[3 * i + 1] * a[3 * j + 1] + a[3 * i + 2] * a[3 * j + 2];
+        double h;
+        if (i == j) {
+            h = a[3 * i + r] * a[3 * i + c] + (r == c ? Gij - 1.0 : 0.0);
+            for (int k = 0; k < 3; ++k) h += a[3 * k + r] * a[3 * k + c];
+        } else {
+            h = a[3 * j + r] * a[3 * i + c] + (r == c ? Gij : 0.0);
+        }
+        A[e] = 4.0 * kap * h;
+    }
+    __syncthreads();
+    wave_psd_project(A, V, 9, lane);
+    for (int e = lane; e < 144; e += 64) {
+        int R = e / 12, C = e % 12;
+        int i = R / 3, r = R % 3, j = C / 3, c = C % 3;
+        double h = (r == c) ? M[4 * i + j] : 0.0;
+        if (i > 0 && j > 0) h += dt2 * A[(3 * (i - 1) + r) * 9 + 3 * (j - 1) + c];
+        H[e] = h;
+    }
+    __syncthreads();
+    // contact incidences of this body, in sorted (contact) order
+    int lo = lower_bound64(inc, n_inc, (uint64_t)b << 32);
+    int hi = lower_bound64(inc, n_inc, (uint64_t)(b + 1) << 32);
+    for (int m = lo; m < hi; ++m) {
+        uint32_t w = (uint32_t)inc[m];
+        int c = (int)(w >> 1), role = (int)(w & 1);
+        if (c >= n_act) {
+            if (lane == 0) atomicOr(G.err, ERR_INCIDENCE);
+            break;
+        }
+        const double* Hc = H432 + (size_t)c * 432 + role * 144;
+        for (int e = lane; e < 144; e += 64) H[e] += Hc[e];
+        if (lane < 12) gq[lane] += g24[(size_t)c * 24 + role * 12 + lane];
+    }
+    __syncthreads();
+    const bool active = newton_active[G.bscene[b]] != 0;
+    if (lane < 12) grad[12 * b + lane] = active ? gq[lane] : 0.0;
+    for (int e = lane; e < 144; e += 64) diag[(size_t)b * 144 + e] = H[e];
+    if (lane == 0) {
+        double ein = 0.0;
+        for (int i = 0; i < 4; ++i)
+            for (int j = 0; j < 4; ++j)
+                ein += M[4 * i + j] * (dq[3 * i] * dq[3 * j] + dq[3 * i + 1] * dq[3 * j + 1] + dq[3 * i + 2] * dq[3 * j + 2]);
+        e_body[b] = 0.5 * ein + dt2 * ortho_energy(qb, kap);
+    }
+    // block-Jacobi inverse: Gauss-Jordan on [H | I] (SPD, no pivoting)
+    for (int e = lane; e < 288; e += 64) {
+        int r = e / 24, c = e % 24;
+        W[e] = (c < 12) ? H[r * 12 + c] : ((c - 12 == r) ? 1.0 : 0.0);
+    }
+    __syncthreads();
+    for (int k = 0; k < 12; ++k) {
+        double piv = W[k * 24 + k];
+        __syncthreads();
+        if (lane < 24) W[k * 24 + lane] /= piv;
+        __syncthreads();
+        if (lane < 12) fac[lane] = W[lane * 24 + k];
+        __syncthreads();
+        for (int e = lane; e < 288; e += 64) {
+            int r = e / 24, c = e % 24;
+            if (r != k) W[e] -= fac[r] * W[k * 24 + c];
+        }
+        __syncthreads();
+    }
+    for (int e = lane; e < 144; e += 64) pinv[(size_t)b * 144 + e] = W[(e / 12) * 24 + 12 + e % 12];
+}
+
+void launch_body_assemble(hipStream_t s, const Geo& G, const AbdParams& P, const double* q, const double* qtilde,
+                          int n_inc, const uint64_t* inc_sorted, const int2* body, const double* g24,
+                          const double* H432, const int* newton_active, double* grad, double* diag, double* pinv,
+                          double* e_body, int n_act) {
+    hipLaunchKernelGGL(k_body_assemble, dim3(G.nb), dim3(64), 0, s, G, P, q, qtilde, n_inc, inc_sorted, g24, H432,
+                       newton_active, grad, diag, pinv, e_body, n_act);
+}
+
+// per-scene sum of body energies and candidate energies (fixed tree order)
+__global__ __launch_bounds__(256) void k_scene_sum(Geo G, const double* __restrict__ e_body,
+                                                    const double* __restrict__ e_cand, const int* __restrict__ lo,
+                                                    const int* __restrict__ hi, double* __restrict__ out) {
+    int s = blockIdx.x;
+    double acc = 0.0;
+    for (int b = G.scene_b0[s] + threadIdx.x; b < G.scene_b0[s + 1]; b += 256) acc += e_body[b];
+    if (e_cand)
+        for (int i = lo[s] + threadIdx.x; i < hi[s]; i += 256) acc += e_cand[i];
+    __shared__ double sh[256];
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[s] = sh[0];
+}
+
+void launch_scene_sum(hipStream_t s, const Geo& G, const double* e_body, const double* e_cand, const int* lo,
+                      const int* hi, double* out) {
+    hipLaunchKernelGGL(k_scene_sum, dim3(G.ns), dim3(256), 0, s, G, e_body, e_cand, lo, hi, out);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Block-Jacobi PCG (replaces warp.optim.linear.bicgstab, abd.py:636), tiled: one wave per tile
+// of at most 5 bodies (60 rows) of one scene.  Partial dot products per tile; each scene's
+// scalars are the sum of its tiles' partials, reduced by every wave in the same fixed order.
+// part layout: [quantity(3: pAp, rz, rr)][parity(2)][tile]
+// ---------------------------------------------------------------------------------------------
+#define TILE_BODIES 5
+
+__device__ __forceinline__ double scene_partial(const double* part, int ntiles, int qty, int par, int t0, int t1,
+                                                int lane) {
+    const double* p = part + (size_t)(qty * 2 + par) * ntiles;
+    double acc = 0.0;
+    for (int t = t0 + lane; t < t1; t += 64) acc += p[t];
+    return wave_sum(acc);
+}
+
+__global__ __launch_bounds__(64) void k_pcg_init(int ntiles, const int* __restrict__ tb0, const int* __restrict__ tb1,
+                                                  const double* __restrict__ g, const double* __restrict__ pinv,
+                                                  double* x, double* r, double* z, double* p, double* part) {
+    __shared__ double rs[60];
+    int t = blockIdx.x, lane = threadIdx.x;
+    int b0 = tb0[t], nbt = tb1[t] - b0;
+    int lb = lane / 12, row = lane % 12;
+    bool on = lb < nbt;
+    int idx = 12 * (b0 + lb) + row;
+    double rv = on ? g[idx] : 0.0;
+    if (lane < 60) rs[lane] = rv;
+    __syncthreads();
+    double zv = 0.0;
+    if (on) {
+        const double* Pb = pinv + (size_t)(b0 + lb) * 144 + row * 12;
+        for (int c = 0; c < 12; ++c) zv += Pb[c] * rs[lb * 12 + c];
+        x[idx] = 0.0;
+        r[idx] = rv;
+        z[idx] = zv;
+        p[idx] = 0.0;
+    }
+    double rz = wave_sum(on ? rv * zv : 0.0), rr = wave_sum(on ? rv * rv : 0.0);
+    if (lane == 0) {
+        part[(size_t)(1 * 2 + 0) * ntiles + t] = rz;
+        part[(size_t)(2 * 2 + 0) * ntiles + t] = rr;
+    }
+}
+
+void launch_pcg_init(hipStream_t s, int ntiles, const int* tile_b0, const int* tile_b1, const double* g,
+                     const double* pinv, double* x, double* r, double* z, double* p, double* part) {
+    hipLaunchKernelGGL(k_pcg_init, dim3(ntiles), dim3(64), 0, s, ntiles, tile_b0, tile_b1, g, pinv, x, r, z, p, part);
+}
+
+// per-scene sum of one partial (which = qty*2+par); optional sqrt (residual norms)
+__global__ __launch_bounds__(64) void k_pcg_tile_sum(int ns, const int* __restrict__ st0, const double* __restrict__ part,
+                                                      int stride, int which, double* out, int do_sqrt) {
+    int s = blockIdx.x;
+    double v = 0.0;
+    for (int t = st0[s] + threadIdx.x; t < st0[s + 1]; t += 64) v += part[(size_t)which * stride + t];
+    v = wave_sum(v);
+    if (threadIdx.x == 0) out[s] = do_sqrt ? sqrt(v) : v;
+}
+
+void launch_pcg_tile_sum(hipStream_t s, int ns, const int* scene_tile0, const double* part, int stride, int which,
+                         double* out, int do_sqrt) {
+    hipLaunchKernelGGL(k_pcg_tile_sum, dim3(ns), dim3(64), 0, s, ns, scene_tile0, part, stride, which, out, do_sqrt);
+}
+
+// A scene is done once its residual after `it` iterations (parity it&1) is small enough; the
+// flag makes that sticky, because later launches see the other parity's (older) partials.
+__device__ __forceinline__ bool pcg_done(const double* part, int ntiles, int par, int t0, int t1, int lane, double bn,
+                                         double tol, const int* done_flag) {
+    if (bn == 0.0 || *done_flag) return true;
+    double rr = scene_partial(part, ntiles, 2, par, t0, t1, lane);
+    return sqrt(rr) <= tol * bn;
+}
+
+// p = z + beta p_old (own rows stored; neighbours recomputed on the fly), Ap = H p, partial p.Ap
+__global__ __launch_bounds__(64) void k_pcg_A(int it, int ntiles, const int* __restrict__ tb0, const int* __restrict__ tb1,
+                                               const int* __restrict__ tscene, const int* __restrict__ st0,
+                                               const double* __restrict__ diag, const int* __restrict__ row_ptr,
+                                               const uint64_t* __restrict__ ent_key, const int* __restrict__ ent_val,
+                                               const double* __restrict__ off, const double* __restrict__ offT,
+                                               const double* __restrict__ z, const double* __restrict__ p_old,
+                                               double* __restrict__ p_new, double* __restrict__ Ap, double* part,
+                                               const double* __restrict__ bnorm, double tol, int* cg_iters,
+                                               int* cg_done, int nb, int n_pairs, int* err) {
+    int t = blockIdx.x, lane = threadIdx.x;
+    int s = tscene[t];
+    int t0 = st0[s], t1 = st0[s + 1];
+    const int par = it & 1;
+    if (pcg_done(part, ntiles, par, t0, t1, lane, bnorm[s], tol, cg_done + s)) {
+        if (t == t0 && lane == 0) cg_done[s] = 1;
+        return;
+    }
+    double beta = 0.0;
+    if (it > 0) {
+        double rz_new = scene_partial(part, ntiles, 1, par, t0, t1, lane);
+        double rz_old = scene_partial(part, ntiles, 1, par ^ 1, t0, t1, lane);
+        beta = rz_new / rz_old;
+    }
+    if (t == t0 && lane == 0) cg_iters[s] = it + 1;
+    int b0 = tb0[t], nbt = tb1[t] - b0;
+    int lb = lane / 12, row = lane % 12;
+    bool on = lb < nbt;
+    double apv = 0.0, pv = 0.0;
+    if (on) {
+        int b = b0 + lb;
+        int idx = 12 * b + row;
+        pv = z[idx] + beta * p_old[idx];
+        p_new[idx] = pv;
+        const double* D = diag + (size_t)b * 144 + row * 12;
+        for (int c = 0; c < 12; ++c) apv += D[c] * (z[12 * b + c] + beta * p_old[12 * b + c]);
+        for (int e = row_ptr[b]; e < row_ptr[b + 1]; ++e) {
+            int col = (int)(ent_key[e] & 0xFFFFFFFFu);
+            int v = ent_val[e];
+            if (col < 0 || col >= nb || (v >> 1) >= n_pairs || v < 0) {
+                atomicOr(err, ERR_CSR);
+                break;
+            }
+            const double* Bk = ((v & 1) ? offT : off) + (size_t)(v >> 1) * 144 + row * 12;
+            for (int c = 0; c < 12; ++c) apv += Bk[c] * (z[12 * col + c] + beta * p_old[12 * col + c]);
+        }
+        Ap[idx] = apv;
+    }
+    double pAp = wave_sum(on ? pv * apv : 0.0);
+    if (lane == 0) part[(size_t)(0 * 2 + par) * ntiles + t] = pAp;
+}
+
+void launch_pcg_A(hipStream_t s, int it, int ntiles, const int* tile_b0, const int* tile_b1, const int* tile_scene,
+                  const int* scene_tile0, const double* diag, const int* row_ptr, const uint64_t* ent_key,
+                  const int* ent_val, const double* off, const double* offT, const double* z, const double* p_old,
+                  double* p_new, double* Ap, double* part, const double* bnorm, double tol, int* cg_iters,
+                  int* cg_done, int nb, int n_pairs, int* err) {
+    hipLaunchKernelGGL(k_pcg_A, dim3(ntiles), dim3(64), 0, s, it, ntiles, tile_b0, tile_b1, tile_scene, scene_tile0,
+                       diag, row_ptr, ent_key, ent_val, off, offT, z, p_old, p_new, Ap, part, bnorm,
+                       tol, cg_iters, cg_done, nb, n_pairs, err);
+}
+
+// alpha = rz/pAp; x += alpha p; r -= alpha Ap; z = P r; partial r.z and r.r
+__global__ __launch_bounds__(64) void k_pcg_B(int it, int ntiles, const int* __restrict__ tb0, const int* __restrict__ tb1,
+                                               const int* __restrict__ tscene, const int* __restrict__ st0,
+                                               const double* __restrict__ pinv, const double* __restrict__ p,
+                                               const double* __restrict__ Ap, double* x, double* r, double* z,
+                                               double* part, const double* __restrict__ bnorm, double tol,
+                                               const int* __restrict__ cg_done) {
+    __shared__ double rs[60];
+    int t = blockIdx.x, lane = threadIdx.x;
+    int s = tscene[t];
+    int t0 = st0[s], t1 = st0[s + 1];
+    const int par = it & 1;
+    if (pcg_done(part, ntiles, par, t0, t1, lane, bnorm[s], tol, cg_done + s)) return;
+    double rz = scene_partial(part, ntiles, 1, par, t0, t1, lane);
+    double pAp = scene_partial(part, ntiles, 0, par, t0, t1, lane);
+    double alpha = rz / pAp;
+    int b0 = tb0[t], nbt = tb1[t] - b0;
+    int lb = lane / 12, row = lane % 12;
+    bool on = lb < nbt;
+    int idx = 12 * (b0 + lb) + row;
+    double rv = 0.0;
+    if (on) {
+        x[idx] += alpha * p[idx];
+        rv = r[idx] - alpha * Ap[idx];
+        r[idx] = rv;
+    }
+    if (lane < 60) rs[lane] = rv;
+    __syncthreads();
+    double zv = 0.0;
+    if (on) {
+        const double* Pb = pinv + (size_t)(b0 + lb) * 144 + row * 12;
+        for (int c = 0; c < 12; ++c) zv += Pb[c] * rs[lb * 12 + c];
+        z[idx] = zv;
+    }
+    double prz = wave_sum(on ? rv * zv : 0.0), prr = wave_sum(on ? rv * rv : 0.0);
+    if (lane == 0) {
+        part[(size_t)(1 * 2 + (par ^ 1)) * ntiles + t] = prz;
+        part[(size_t)(2 * 2 + (par ^ 1)) * ntiles + t] = prr;
+    }
+}
+
+void launch_pcg_B(hipStream_t s, int it, int ntiles, const int* tile_b0, const int* tile_b1, const int* tile_scene,
+                  const int* scene_tile0, const double* pinv, const double* p, const double* Ap, double* x, double* r,
+                  double* z, double* part, const double* bnorm, double tol, const int* cg_done) {
+    hipLaunchKernelGGL(k_pcg_B, dim3(ntiles), dim3(64), 0, s, it, ntiles, tile_b0, tile_b1, tile_scene, scene_tile0,
+                       pinv, p, Ap, x, r, z, part, bnorm, tol, cg_done);
+}
+
+// any scene still iterating after `it` completed iterations (residual of parity it&1)
+__global__ __launch_bounds__(64) void k_pcg_any(int ns, int ntiles, const int* __restrict__ st0,
+                                                 const double* __restrict__ part, int it,
+                                                 const double* __restrict__ bnorm, double tol,
+                                                 const int* __restrict__ cg_done, int* any) {
+    int s = blockIdx.x;
+    if (!pcg_done(part, ntiles, it & 1, st0[s], st0[s + 1], threadIdx.x, bnorm[s], tol, cg_done + s))
+        if (threadIdx.x == 0) atomicOr(any, 1);
+}
+
+void launch_pcg_any(hipStream_t s, int ns, int ntiles, const int* scene_tile0, const double* part, int it,
+                    const double* bnorm, double tol, const int* cg_done, int* any) {
+    hipLaunchKernelGGL(k_pcg_any, dim3(ns), dim3(64), 0, s, ns, ntiles, scene_tile0, part, it, bnorm, tol, cg_done,
+                       any);
+}
+
+// ---------------------------------------------------------------------------------------------
+// CCD bound (additive CCD, Li, Kaufman, Jiang 2021) per candidate -> per-scene minimum
+// ---------------------------------------------------------------------------------------------
+__device__ double accd_dev(int kind, const D3* x0, const D3* x1, double s, double tmax, int max_iter) {
+    D3 p[4], x[4];
+    D3 mean = mk(0, 0, 0);
+    for (int i = 0; i < 4; ++i) { p[i] = x1[i] - x0[i]; mean = mean + 0.25 * p[i]; }
+    for (int i = 0; i < 4; ++i) { p[i] = p[i] - mean; x[i] = x0[i]; }
+    double n0 = sqrt(dot(p[0], p[0])), n1 = sqrt(dot(p[1], p[1])), n2 = sqrt(dot(p[2], p[2])),
+           n3 = sqrt(dot(p[3], p[3]));
+    double lp = (kind == K_PT) ? n0 + fmax(n1, fmax(n2, n3)) : fmax(n0, n1) + fmax(n2, n3);
+    if (lp == 0.0) return tmax;
+    double d = sqrt(pair_d2(kind, x));
+    double g = s * d;
+    double t = 0.0;
+    double tl = (1.0 - s) * d / lp;
+    for (int it = 0; it < max_iter; ++it) {
+        for (int i = 0; i < 4; ++i) x[i] = x[i] + tl * p[i];
+        d = sqrt(pair_d2(kind, x));
+        if (t > 0.0 && d < g) return t;
+        t += tl;
+        if (t > tmax) return tmax;
+        tl = 0.9 * d / lp;
+    }
+    return t;
+}
+
+__global__ void k_ccd(Geo G, int n, const uint64_t* __restrict__ keys, const double4* __restrict__ X0,
+                      const double4* __restrict__ X1, AbdParams P, unsigned long long* amax_bits) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    PairV p = decode(G, keys[i]);
+    if (p.kind < 0) return;
+    double toi;
+    if (p.kind == K_PG) {
+        D3 a = ld3(X0, p.v[0]), b = ld3(X1, p.v[0]);
+        D3 nrm = mk(P.ground_normal[0], P.ground_normal[1], P.ground_normal[2]);
+        double d0 = dot(nrm, a) - P.ground_offset;
+        double v = -(dot(nrm, b) - dot(nrm, a));
+        toi = (v <= 0.0 || d0 <= 0.0) ? 1.0 : fmin(1.0, (1.0 - P.ccd_gap_ratio) * d0 / v);
+    } else {
+        D3 a[4], b[4];
+        for (int k = 0; k < 4; ++k) { a[k] = ld3(X0, p.v[k]); b[k] = ld3(X1, p.v[k]); }
+        toi = accd_dev(p.kind, a, b, P.ccd_gap_ratio, 1.0, P.ccd_max_iter);
+    }
+    if (toi < 1.0) atomicMin(&amax_bits[p.scene], (unsigned long long)__double_as_longlong(toi));
+}
+
+void launch_ccd(hipStream_t s, const Geo& G, int n, const uint64_t* keys, const double4* X0, const double4* X1,
+                const AbdParams& P, unsigned long long* amax_bits) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_ccd, dim3(cdiv(n, 128)), dim3(128), 0, s, G, n, keys, X0, X1, P, amax_bits);
+}
+
+// energy_inertia (abd.py:53-64) at q - alpha_s du
+__global__ void k_body_energy(Geo G, AbdParams P, const double* __restrict__ q, const double* __restrict__ du,
+                              const double* __restrict__ alpha, const double* __restrict__ qtilde,
+                              double* __restrict__ e_body) {
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= G.nb) return;
+    if (G.pinned[b]) { e_body[b] = 0.0; return; }
+    double a = alpha ? alpha[G.bscene[b]] : 0.0;
+    double qb[12], dq[12];
+    for (int k = 0; k < 12; ++k) {
+        qb[k] = du ? q[12 * b + k] - a * du[12 * b + k] : q[12 * b + k];
+        dq[k] = qb[k] - qtilde[12 * b + k];
+    }
+    const double* M = G.mass4 + 16 * b;
+    double ein = 0.0;
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j)
+            ein += M[4 * i + j] * (dq[3 * i] * dq[3 * j] + dq[3 * i + 1] * dq[3 * j + 1] + dq[3 * i + 2] * dq[3 * j + 2]);
+    e_body[b] = 0.5 * ein + P.dt * P.dt * ortho_energy(qb, P.kappa);
+}
+
+void launch_body_energy(hipStream_t s, const Geo& G, const AbdParams& P, const double* q, const double* du,
+                        const double* alpha, const double* qtilde, double* e_body) {
+    hipLaunchKernelGGL(k_body_energy, dim3(cdiv(G.nb, 128)), dim3(128), 0, s, G, P, q, du, alpha, qtilde, e_body);
+}
+
+// line_search_iterative (gauss_newton.py:98-115) per scene
+__global__ void k_ls_init(int ns, AbdParams P, const int* __restrict__ active, SceneCtl c) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= ns) return;
+    double amax = P.ccd_enabled ? fmin(1.0, __longlong_as_double((long long)c.amax_bits[s])) : 1.0;
+    c.alpha0[s] = amax;
+    c.alpha[s] = active[s] ? amax : 0.0;
+    c.ls_done[s] = active[s] ? 0 : 1;
+    c.ls_floor[s] = 0;
+}
+
+void launch_ls_init(hipStream_t s, int ns, const AbdParams& P, const int* newton_active, SceneCtl c) {
+    hipLaunchKernelGGL(k_ls_init, dim3(cdiv(ns, 128)), dim3(128), 0, s, ns, P, newton_active, c);
+}
+
+__global__ void k_ls_decide(int ns, AbdParams P, SceneCtl c, int* pending) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= ns || c.ls_done[s]) return;
+    if (c.E1[s] < c.E0[s]) {
+        c.ls_done[s] = 1;
+    } else if (c.alpha[s] * 0.5 < P.ls_floor * c.alpha0[s]) {
+        c.ls_done[s] = 1;
+        c.ls_floor[s] = 1;
+    } else {
+        c.alpha[s] *= 0.5;
+        atomicOr(pending, 1);
+    }
+}
+
+void launch_ls_decide(hipStream_t s, int ns, const AbdParams& P, SceneCtl c, int* pending) {
+    hipLaunchKernelGGL(k_ls_decide, dim3(cdiv(ns, 128)), dim3(128), 0, s, ns, P, c, pending);
+}
+
+// add_du_kernel (abd.py:780-785): q -= alpha du
+__global__ void k_apply(Geo G, double* q, const double* __restrict__ du, const double* __restrict__ alpha,
+                        const int* __restrict__ active) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 12 * G.nb) return;
+    int s = G.bscene[i / 12];
+    if (active[s]) q[i] -= alpha[s] * du[i];
+}
+
+void launch_apply(hipStream_t s, const Geo& G, double* q, const double* du, const double* alpha, const int* active) {
+    hipLaunchKernelGGL(k_apply, dim3(cdiv(12 * G.nb, 256)), dim3(256), 0, s, G, q, du, alpha, active);
+}
+
+// forward_states (abd.py:574-581)
+__global__ void k_forward(int n, double dt, const double* __restrict__ q, double* qnow, double* qdot) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    qdot[i] = (q[i] - qnow[i]) / dt;
+    qnow[i] = q[i];
+}
+
+void launch_forward(hipStream_t s, int nb, double dt, const double* q, double* qnow, double* qdot) {
+    hipLaunchKernelGGL(k_forward, dim3(cdiv(12 * nb, 256)), dim3(256), 0, s, 12 * nb, dt, q, qnow, qdot);
+}
+
+// abd.py:699: stop once the residual of the pre-update iterate is below tol
+__global__ void k_newton_check(int ns, AbdParams P, SceneCtl c) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= ns) return;
+    if (c.newton_active[s] && c.residual[s] < P.newton_tol) c.newton_active[s] = 0;
+}
+
+void launch_newton_check(hipStream_t s, int ns, const AbdParams& P, SceneCtl c) {
+    hipLaunchKernelGGL(k_newton_check, dim3(cdiv(ns, 128)), dim3(128), 0, s, ns, P, c);
+}
+
+// get_contact_points (abd.py:759-766): midpoint of the closest points and the distance
+__global__ void k_contact_points(Geo G, int n, const uint64_t* __restrict__ keys, const double4* __restrict__ X,
+                                 AbdParams P, double* mid, double* dist) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    PairV p = decode(G, keys[i]);
+    if (p.kind < 0) { mid[3 * i] = mid[3 * i + 1] = mid[3 * i + 2] = 0.0; dist[i] = 0.0; return; }
+    D3 c0, c1;
+    if (p.kind == K_PG) {
+        D3 x = ld3(X, p.v[0]);
+        D3 nrm = mk(P.ground_normal[0], P.ground_normal[1], P.ground_normal[2]);
+        double d = dot(nrm, x) - P.ground_offset;
+        c0 = x;
+        c1 = x - d * nrm;
+    } else {
+        D3 x[4];
+        for (int k = 0; k < 4; ++k) x[k] = ld3(X, p.v[k]);
+        DistEval E;
+        dist_eval(p.kind, x, E);
+        // closest point on the far feature of the stencil: y0 vs. the rest
+        if (E.S.type == T_PP) { c0 = E.y[0]; c1 = E.y[1]; }
+        else if (E.S.type == T_PE) {
+            D3 e = E.y[2] - E.y[1];
+            double t = dot(E.y[0] - E.y[1], e) / dot(e, e);
+            c0 = E.y[0]; c1 = E.y[1] + t * e;
+        } else if (E.S.type == T_PT) {
+            D3 nn = (1.0 / sqrt(E.N)) * E.n;
+            c0 = E.y[0];
+            c1 = E.y[0] - dot(E.y[0] - E.y[1], nn) * nn;
+        } else {
+            D3 d1 = E.y[1] - E.y[0], d2 = E.y[3] - E.y[2], r = E.y[0] - E.y[2];
+            double a = dot(d1, d1), e = dot(d2, d2), f = dot(d2, r), c = dot(d1, r), b = dot(d1, d2);
+            double den = a * e - b * b;
+            double s = (b * f - c * e) / den, t = (b * s + f) / e;
+            c0 = E.y[0] + s * d1; c1 = E.y[2] + t * d2;
+        }
+    }
+    D3 m = 0.5 * (c0 + c1);
+    mid[3 * i] = m.x; mid[3 * i + 1] = m.y; mid[3 * i + 2] = m.z;
+    D3 dd = c0 - c1;
+    dist[i] = sqrt(dot(dd, dd));
+}
+
+void launch_contact_points(hipStream_t s, const Geo& G, int n, const uint64_t* keys, const double4* X,
+                           const AbdParams& P, double* mid, double* dist) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_contact_points, dim3(cdiv(n, 256)), dim3(256), 0, s, G, n, keys, X, P, mid, dist);
+}
+
+}  // namespace abd

@@ -1,0 +1,204 @@
+// abd_sort.hip — stable LSD radix sort of 64-bit keys (optionally carrying int values) and an
+// exclusive prefix sum of ints, for the candidate, incidence and body-pair lists.
+//
+// Radix sort: 4-bit digits.  Per pass, each 256-thread block owns a tile of 2048 consecutive
+// elements (8 per thread, in order); it counts digits (k_radix_hist), the digit-major histogram
+// [digit][block] is prefix-summed, and the block re-reads its tile, ranks every element among
+// the equal digits of earlier threads and earlier items (a block-wide scan over a [digit][thread]
+// count matrix in LDS) and scatters it.  Each pass is stable, so the sort is stable.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace abd {
+
+#define SORT_T 256
+#define SORT_I 8
+#define SORT_TILE (SORT_T * SORT_I)
+#define RADIX 16
+
+static inline int cdivs(long a, int b) { return (int)((a + b - 1) / b); }
+
+// --------------------------------------------------------------------------------------------
+// exclusive scan of int (n <= SORT_TILE * SORT_TILE)
+// --------------------------------------------------------------------------------------------
+__device__ __forceinline__ int block_excl_scan256(int v, int* sh) {
+    const int t = threadIdx.x;
+    sh[t] = v;
+    __syncthreads();
+    for (int o = 1; o < SORT_T; o <<= 1) {
+        int a = (t >= o) ? sh[t - o] : 0;
+        __syncthreads();
+        sh[t] += a;
+        __syncthreads();
+    }
+    int incl = sh[t];
+    __syncthreads();
+    return incl - v;
+}
+
+__global__ __launch_bounds__(SORT_T) void k_scan_tile_sums(int n, const int* __restrict__ in, int* __restrict__ bsum) {
+    __shared__ int sh[SORT_T];
+    const int base = blockIdx.x * SORT_TILE + threadIdx.x * SORT_I;
+    int s = 0;
+    for (int i = 0; i < SORT_I; ++i)
+        if (base + i < n) s += in[base + i];
+    int ex = block_excl_scan256(s, sh);
+    if (threadIdx.x == SORT_T - 1) bsum[blockIdx.x] = ex + s;
+}
+
+// single block: exclusive scan of nb <= SORT_TILE block sums, in place
+__global__ __launch_bounds__(SORT_T) void k_scan_block_sums(int nb, int* bsum) {
+    __shared__ int sh[SORT_T];
+    const int base = threadIdx.x * SORT_I;
+    int v[SORT_I];
+    int s = 0;
+    for (int i = 0; i < SORT_I; ++i) {
+        v[i] = (base + i < nb) ? bsum[base + i] : 0;
+        s += v[i];
+    }
+    int ex = block_excl_scan256(s, sh);
+    for (int i = 0; i < SORT_I; ++i) {
+        if (base + i < nb) bsum[base + i] = ex;
+        ex += v[i];
+    }
+}
+
+__global__ __launch_bounds__(SORT_T) void k_scan_apply(int n, const int* __restrict__ in, const int* __restrict__ bsum,
+                                                        int* __restrict__ out) {
+    __shared__ int sh[SORT_T];
+    const int base = blockIdx.x * SORT_TILE + threadIdx.x * SORT_I;
+    int v[SORT_I];
+    int s = 0;
+    for (int i = 0; i < SORT_I; ++i) {
+        v[i] = (base + i < n) ? in[base + i] : 0;
+        s += v[i];
+    }
+    int ex = block_excl_scan256(s, sh) + bsum[blockIdx.x];
+    for (int i = 0; i < SORT_I; ++i) {
+        if (base + i < n) out[base + i] = ex;
+        ex += v[i];
+    }
+}
+
+// scratch: bsum needs cdiv(n, SORT_TILE) ints
+int scan_scratch_ints(long n) { return cdivs(n, SORT_TILE) + 1; }
+
+hipError_t excl_scan_int(hipStream_t s, const int* in, int* out, int n, int* bsum) {
+    if (n <= 0) return hipSuccess;
+    int nb = cdivs(n, SORT_TILE);
+    if (nb > SORT_TILE) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_scan_tile_sums, dim3(nb), dim3(SORT_T), 0, s, n, in, bsum);
+    hipLaunchKernelGGL(k_scan_block_sums, dim3(1), dim3(SORT_T), 0, s, nb, bsum);
+    hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(SORT_T), 0, s, n, in, bsum, out);
+    return hipGetLastError();
+}
+
+// --------------------------------------------------------------------------------------------
+// radix sort
+// --------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(SORT_T) void k_radix_hist(int n, const uint64_t* __restrict__ keys, int shift, int nblk,
+                                                        int* __restrict__ hist) {
+    __shared__ int cnt[RADIX];
+    if (threadIdx.x < RADIX) cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const int base = blockIdx.x * SORT_TILE + threadIdx.x * SORT_I;
+    for (int i = 0; i < SORT_I; ++i)
+        if (base + i < n) atomicAdd(&cnt[(int)((keys[base + i] >> shift) & (RADIX - 1))], 1);
+    __syncthreads();
+    if (threadIdx.x < RADIX) hist[threadIdx.x * nblk + blockIdx.x] = cnt[threadIdx.x];
+}
+
+__global__ __launch_bounds__(SORT_T) void k_radix_scatter(int n, const uint64_t* __restrict__ kin,
+                                                           const int* __restrict__ vin, uint64_t* __restrict__ kout,
+                                                           int* __restrict__ vout, int shift, int nblk,
+                                                           const int* __restrict__ hoff) {
+    __shared__ int m[RADIX * SORT_T];
+    __shared__ int sh[SORT_T];
+    const int t = threadIdx.x;
+    const int base = blockIdx.x * SORT_TILE + t * SORT_I;
+    uint64_t k[SORT_I];
+    int dg[SORT_I];
+    int c[RADIX];
+    for (int d = 0; d < RADIX; ++d) c[d] = 0;
+    for (int i = 0; i < SORT_I; ++i) {
+        if (base + i < n) {
+            k[i] = kin[base + i];
+            dg[i] = (int)((k[i] >> shift) & (RADIX - 1));
+            c[dg[i]]++;
+        } else {
+            dg[i] = -1;
+        }
+    }
+    for (int d = 0; d < RADIX; ++d) m[d * SORT_T + t] = c[d];
+    __syncthreads();
+    // block exclusive scan over the digit-major matrix: thread t owns entries [16t, 16t+16)
+    int run[RADIX];
+    int s = 0;
+    for (int j = 0; j < RADIX; ++j) {
+        run[j] = m[t * RADIX + j];
+        s += run[j];
+    }
+    int ex = block_excl_scan256(s, sh);
+    for (int j = 0; j < RADIX; ++j) {
+        m[t * RADIX + j] = ex;
+        ex += run[j];
+    }
+    __syncthreads();
+    // m[d*T + t] = items of the block with digit < d + items with digit d in threads < t
+    int seen[RADIX];
+    for (int d = 0; d < RADIX; ++d) seen[d] = 0;
+    for (int i = 0; i < SORT_I; ++i) {
+        int d = dg[i];
+        if (d < 0) continue;
+        int rank = m[d * SORT_T + t] - m[d * SORT_T] + seen[d];
+        seen[d]++;
+        int pos = hoff[d * nblk + blockIdx.x] + rank;
+        kout[pos] = k[i];
+        if (vin) vout[pos] = vin[base + i];
+    }
+}
+
+// scratch for radix_sort64: tmp keys (n), tmp vals (n, if values), hist 2*RADIX*nblk ints + scan
+size_t radix_scratch_bytes(long n) {
+    long nblk = cdivs(n, SORT_TILE);
+    return (size_t)n * 8 + (size_t)n * 4 + (size_t)(2 * RADIX * nblk + scan_scratch_ints(RADIX * nblk) + 64) * 4 + 1024;
+}
+
+hipError_t radix_sort64(hipStream_t s, const uint64_t* kin, uint64_t* kout, const int* vin, int* vout, int n,
+                        int end_bit, void* scratch) {
+    if (n <= 0) return hipSuccess;
+    const int nblk = cdivs(n, SORT_TILE);
+    if ((long)RADIX * nblk > (long)SORT_TILE * SORT_TILE) return hipErrorInvalidValue;
+    char* p = (char*)scratch;
+    uint64_t* ktmp = (uint64_t*)p;
+    p += (size_t)n * 8;
+    int* vtmp = (int*)p;
+    p += (size_t)n * 4;
+    int* hist = (int*)p;
+    int* hoff = hist + RADIX * nblk;
+    int* bsum = hoff + RADIX * nblk;
+    const int passes = (end_bit + 3) / 4;
+    if (passes == 0) {
+        hipMemcpyAsync(kout, kin, (size_t)n * 8, hipMemcpyDeviceToDevice, s);
+        if (vin) hipMemcpyAsync(vout, vin, (size_t)n * 4, hipMemcpyDeviceToDevice, s);
+        return hipGetLastError();
+    }
+    const uint64_t* ks = kin;
+    const int* vs = vin;
+    for (int pass = 0; pass < passes; ++pass) {
+        // the last pass writes the output buffers
+        bool to_out = ((passes - 1 - pass) % 2) == 0;
+        uint64_t* kd = to_out ? kout : ktmp;
+        int* vd = vin ? (to_out ? vout : vtmp) : nullptr;
+        int shift = 4 * pass;
+        hipLaunchKernelGGL(k_radix_hist, dim3(nblk), dim3(SORT_T), 0, s, n, ks, shift, nblk, hist);
+        hipError_t e = excl_scan_int(s, hist, hoff, RADIX * nblk, bsum);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_radix_scatter, dim3(nblk), dim3(SORT_T), 0, s, n, ks, vs, kd, vd, shift, nblk, hoff);
+        ks = kd;
+        vs = vd;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace abd

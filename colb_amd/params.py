@@ -35,6 +35,7 @@ class Params:
     ls_floor: float = 1e-2
     pinned_diag: float = 100.0
     max_candidates: int = 1 << 20
+    debug: int = 0  # bit 0: synchronise and check after every launch; bit 1: also log them
 
     def to_c(self) -> AbdParams:
         p = AbdParams()
@@ -58,6 +59,7 @@ class Params:
         p.ls_floor = self.ls_floor
         p.pinned_diag = self.pinned_diag
         p.max_candidates = self.max_candidates
+        p.reserved0 = self.debug
         return p
 
     def replace(self, **kw):

@@ -420,11 +420,12 @@ def read_obj(path):
     return V, edges_of_tris(F), F
 
 
-def load_json_scene(path, base_dir=None, length=0.4):
+def load_json_scene(path, base_dir=None, length=0.4, reference_init=True):
     """JSONComplex (utils/scene.py:65-85) + reference inertia (abd_simple.py:18-22,
     rbd_simple.py:42-48): per object v = V R^T * 2 with R = euler 'xyz' degrees; c = p;
-    m = mass/1000, J = 0.4 m L^2 (L = 0.4); mass <= 0 means pinned.  Initial velocities are
-    zero (the reference ignores the JSON velocities, abd_simple.py:37-49)."""
+    m = mass/1000, J = 0.4 m L^2 (L = 0.4); mass <= 0 means pinned.  The reference ignores the
+    JSON velocities and starts every body with v = 0 and qdot = skew((3, 3, 0))
+    (abd_simple.py:37-49); ``reference_init=False`` starts from rest instead."""
     base_dir = base_dir or os.path.dirname(os.path.dirname(os.path.abspath(path)))
     with open(path) as f:
         objs = json.load(f)
@@ -444,5 +445,9 @@ def load_json_scene(path, base_dir=None, length=0.4):
         J = 0.4 * m * length * length
         pinned = m <= 0
         M = np.diag([m, J, J, J]) if not pinned else np.diag([1.0, 1.0, 1.0, 1.0])
-        bld.add(v, F, E, M, q_from_pose(o.get("p", [0, 0, 0]), np.eye(3)), pinned=pinned)
+        qd = np.zeros(12)
+        if reference_init:
+            # rows of skew(w), w = (3, 3, 0): [[0, -wz, wy], [wz, 0, -wx], [-wy, wx, 0]]
+            qd[3:] = [0.0, 0.0, 3.0, 0.0, 0.0, -3.0, -3.0, 3.0, 0.0]
+        bld.add(v, F, E, M, q_from_pose(o.get("p", [0, 0, 0]), np.eye(3)), qd, pinned=pinned)
     return bld.build(os.path.basename(path))

@@ -1,0 +1,38 @@
+"""CPU build of colb_amd/csrc against the HIP emulation in tests/emu (g++, AddressSanitizer).
+TEST INFRASTRUCTURE ONLY: the result is loaded by tests that check kernel indexing on the CPU."""
+import os
+import subprocess
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+CSRC = os.path.join(ROOT, "colb_amd", "csrc")
+OUT = os.path.join(HERE, "_build")
+LIB = os.path.join(OUT, "libabd_emu.so")
+FLAGS = ["-std=c++20", "-O1", "-g", "-fPIC", "-fno-omit-frame-pointer", "-fsanitize=address",
+         "-I", os.path.join(HERE, "include"), "-I", os.path.join(ROOT, "include"), "-pthread",
+         "-Wno-unused-result", "-Wno-attributes"]
+
+
+def build(asan=True):
+    os.makedirs(OUT, exist_ok=True)
+    flags = FLAGS if asan else [f for f in FLAGS if f != "-fsanitize=address"]
+    objs = []
+    for src in ("abd_kernels.hip", "abd_sort.hip", "abd_api.hip"):
+        obj = os.path.join(OUT, src.replace(".hip", ".o"))
+        r = subprocess.run(["g++", *flags, "-x", "c++", "-c", os.path.join(CSRC, src), "-o", obj],
+                           capture_output=True, text=True)
+        if r.returncode:
+            raise RuntimeError(r.stderr[-5000:])
+        objs.append(obj)
+    r = subprocess.run(["g++", *flags, "-shared", "-o", LIB, *objs], capture_output=True, text=True)
+    if r.returncode:
+        raise RuntimeError(r.stderr[-3000:])
+    return LIB
+
+
+def asan_runtime():
+    return subprocess.run(["g++", "-print-file-name=libasan.so"], capture_output=True, text=True).stdout.strip()
+
+
+if __name__ == "__main__":
+    print(build())

@@ -1,0 +1,215 @@
+// emu_hip.hpp — CPU emulation of the HIP runtime subset used by colb_amd/csrc, for bounds-checked
+// (AddressSanitizer) host builds of the kernels.  TEST INFRASTRUCTURE ONLY.
+//
+// Kernels run block by block; each GPU thread of a block is an OS thread.  __shared__ variables
+// are function statics (blocks run one at a time), __syncthreads is a std::barrier over the
+// block, __shfl_xor goes through a block-wide exchange buffer, and threads that return early
+// drop out of the barrier (as exited waves do on the GPU).
+#pragma once
+#include <atomic>
+#include <ucontext.h>
+#include <cfloat>
+#include <climits>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <functional>
+#include <thread>
+#include <vector>
+
+#define __global__
+#define __device__
+#define __host__
+#define __forceinline__ inline
+#define __launch_bounds__(...)
+#define __shared__ static
+
+struct dim3 {
+    unsigned x = 1, y = 1, z = 1;
+    dim3() = default;
+    dim3(unsigned a, unsigned b = 1, unsigned c = 1) : x(a), y(b), z(c) {}
+};
+struct double4 { double x, y, z, w; };
+struct float4 { float x, y, z, w; };
+struct int2 { int x, y; };
+struct int4 { int x, y, z, w; };
+inline double4 make_double4(double a, double b, double c, double d) { return {a, b, c, d}; }
+inline float4 make_float4(float a, float b, float c, float d) { return {a, b, c, d}; }
+inline int2 make_int2(int a, int b) { return {a, b}; }
+inline int4 make_int4(int a, int b, int c, int d) { return {a, b, c, d}; }
+
+inline dim3 threadIdx, blockIdx, blockDim, gridDim;
+
+// ---- cooperative fibers: one per GPU thread of the running block -----------------------------
+struct EmuFiber {
+    ucontext_t ctx;
+    std::vector<char> stack;
+    bool done = false;
+};
+struct EmuSched {
+    ucontext_t main;
+    std::vector<EmuFiber> fib;
+    unsigned cur = 0;
+    std::function<void()> body;
+    std::vector<double> xbuf;
+    const void* main_bottom = nullptr;
+    size_t main_size = 0;
+};
+inline EmuSched* emu_sched = nullptr;
+
+#if defined(__SANITIZE_ADDRESS__)
+#include <sanitizer/common_interface_defs.h>
+#define EMU_START_SWITCH(fake, bottom, size) __sanitizer_start_switch_fiber(fake, bottom, size)
+#define EMU_FINISH_SWITCH(fake, bottom, size) __sanitizer_finish_switch_fiber(fake, bottom, size)
+#else
+#define EMU_START_SWITCH(fake, bottom, size) ((void)0)
+#define EMU_FINISH_SWITCH(fake, bottom, size) ((void)0)
+#endif
+
+inline void emu_yield() {
+    EmuSched* S = emu_sched;
+    EmuFiber& f = S->fib[S->cur];
+    void* fake = nullptr;
+    EMU_START_SWITCH(&fake, S->main_bottom, S->main_size);
+    swapcontext(&f.ctx, &S->main);
+    EMU_FINISH_SWITCH(fake, &S->main_bottom, &S->main_size);
+}
+
+inline void emu_fiber_entry() {
+    EmuSched* S = emu_sched;
+    EMU_FINISH_SWITCH(nullptr, &S->main_bottom, &S->main_size);
+    S->body();
+    S->fib[S->cur].done = true;
+    EMU_START_SWITCH(nullptr, S->main_bottom, S->main_size);
+}
+
+inline void __syncthreads() { emu_yield(); }
+inline void __threadfence() {}
+
+template <class T>
+inline T __shfl_xor(T v, int mask, int width = 64) {
+    (void)width;
+    static_assert(sizeof(T) <= sizeof(double), "shfl of wide type");
+    double w = 0;
+    std::memcpy(&w, &v, sizeof(T));
+    unsigned lane = threadIdx.x;
+    emu_sched->xbuf[lane] = w;
+    emu_yield();
+    double o = emu_sched->xbuf[lane ^ (unsigned)mask];
+    emu_yield();
+    T r;
+    std::memcpy(&r, &o, sizeof(T));
+    return r;
+}
+
+inline int atomicAdd(int* p, int v) { return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST); }
+inline int atomicOr(int* p, int v) { return __atomic_fetch_or(p, v, __ATOMIC_SEQ_CST); }
+inline unsigned long long atomicMin(unsigned long long* p, unsigned long long v) {
+    unsigned long long old = __atomic_load_n(p, __ATOMIC_SEQ_CST);
+    while (v < old && !__atomic_compare_exchange_n(p, &old, v, false, __ATOMIC_SEQ_CST, __ATOMIC_SEQ_CST)) {
+    }
+    return old;
+}
+inline int __clz(int x) { return x == 0 ? 32 : __builtin_clz((unsigned)x); }
+inline int __clzll(long long x) { return x == 0 ? 64 : __builtin_clzll((unsigned long long)x); }
+inline float __double2float_rd(double x) {
+    float f = (float)x;
+    if ((double)f > x) f = std::nextafter(f, -INFINITY);
+    return f;
+}
+inline float __double2float_ru(double x) {
+    float f = (float)x;
+    if ((double)f < x) f = std::nextafter(f, INFINITY);
+    return f;
+}
+inline long long __double_as_longlong(double d) { long long r; std::memcpy(&r, &d, 8); return r; }
+inline double __longlong_as_double(long long v) { double r; std::memcpy(&r, &v, 8); return r; }
+using std::fmin; using std::fmax; using std::sqrt; using std::fabs;
+inline int min(int a, int b) { return a < b ? a : b; }
+inline int max(int a, int b) { return a > b ? a : b; }
+inline unsigned long long min(unsigned long long a, unsigned long long b) { return a < b ? a : b; }
+inline unsigned long long max(unsigned long long a, unsigned long long b) { return a > b ? a : b; }
+
+// ---- runtime API ----------------------------------------------------------------------------
+typedef int hipError_t;
+enum { hipSuccess = 0, hipErrorInvalidValue = 1, hipErrorInvalidConfiguration = 9 };
+typedef void* hipStream_t;
+typedef void* hipEvent_t;
+enum hipMemcpyKind { hipMemcpyHostToHost, hipMemcpyHostToDevice, hipMemcpyDeviceToHost, hipMemcpyDeviceToDevice, hipMemcpyDefault };
+#define hipStreamNonBlocking 1
+#define hipHostMallocDefault 0
+inline thread_local hipError_t emu_last_error = hipSuccess;
+inline const char* hipGetErrorString(hipError_t e) { return e ? "emulated error" : "no error"; }
+inline const char* hipGetErrorName(hipError_t e) { return e ? "hipErrorEmulated" : "hipSuccess"; }
+inline hipError_t hipGetLastError() { hipError_t e = emu_last_error; emu_last_error = hipSuccess; return e; }
+inline hipError_t hipMalloc(void** p, size_t n) { *p = std::malloc(n); if (*p) std::memset(*p, 0xA5, n); return *p ? hipSuccess : 2; }
+template <class T> inline hipError_t hipMalloc(T** p, size_t n) { return hipMalloc((void**)p, n); }
+inline hipError_t hipFree(void* p) { std::free(p); return hipSuccess; }
+inline hipError_t hipHostMalloc(void** p, size_t n, int) { *p = std::malloc(n); return *p ? hipSuccess : 2; }
+inline hipError_t hipHostFree(void* p) { std::free(p); return hipSuccess; }
+inline hipError_t hipMemcpy(void* d, const void* s, size_t n, hipMemcpyKind) { if (n) std::memcpy(d, s, n); return hipSuccess; }
+inline hipError_t hipMemcpyAsync(void* d, const void* s, size_t n, hipMemcpyKind, hipStream_t) { if (n) std::memcpy(d, s, n); return hipSuccess; }
+inline hipError_t hipMemsetAsync(void* d, int v, size_t n, hipStream_t) { if (n) std::memset(d, v, n); return hipSuccess; }
+inline hipError_t hipMemset(void* d, int v, size_t n) { if (n) std::memset(d, v, n); return hipSuccess; }
+inline hipError_t hipStreamCreateWithFlags(hipStream_t* s, unsigned) { *s = (void*)0x1; return hipSuccess; }
+inline hipError_t hipStreamDestroy(hipStream_t) { return hipSuccess; }
+inline hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
+inline hipError_t hipEventCreate(hipEvent_t* e) { *e = (void*)0x1; return hipSuccess; }
+inline hipError_t hipEventRecord(hipEvent_t, hipStream_t) { return hipSuccess; }
+inline hipError_t hipEventElapsedTime(float* ms, hipEvent_t, hipEvent_t) { *ms = 0.f; return hipSuccess; }
+inline hipError_t hipEventDestroy(hipEvent_t) { return hipSuccess; }
+
+template <class K, class... Args>
+void emu_launch(const char* name, K kernel, dim3 grid, dim3 block, Args... args) {
+    const unsigned nt = block.x * block.y * block.z;
+    const unsigned nblk = grid.x * grid.y * grid.z;
+    if (nt == 0 || nblk == 0 || nt > 1024) { emu_last_error = hipErrorInvalidConfiguration; return; }
+    static const bool trace = std::getenv("EMU_TRACE") != nullptr;
+    if (trace) std::fprintf(stderr, "launch %s grid %u block %u\n", name, nblk, nt);
+    EmuSched S;
+    S.xbuf.assign(nt, 0.0);
+    S.body = [&]() { kernel(args...); };
+    EmuSched* prev = emu_sched;
+    emu_sched = &S;
+    blockDim = block;
+    gridDim = grid;
+    S.fib.resize(nt);
+    static std::vector<std::vector<char>> pool;  // fiber stacks reused across launches
+    if (pool.size() < nt) pool.resize(nt);
+    for (unsigned t = 0; t < nt; ++t) {
+        if (pool[t].empty()) pool[t].resize(128 * 1024);
+        S.fib[t].stack.swap(pool[t]);
+    }
+    for (unsigned b = 0; b < nblk; ++b) {
+        blockIdx = dim3(b % grid.x, 1, 1);
+        for (unsigned t = 0; t < nt; ++t) {
+            EmuFiber& f = S.fib[t];
+            f.done = false;
+            getcontext(&f.ctx);
+            f.ctx.uc_stack.ss_sp = f.stack.data();
+            f.ctx.uc_stack.ss_size = f.stack.size();
+            f.ctx.uc_link = &S.main;
+            makecontext(&f.ctx, emu_fiber_entry, 0);
+        }
+        unsigned live = nt;
+        while (live > 0) {
+            live = 0;
+            for (unsigned t = 0; t < nt; ++t) {
+                EmuFiber& f = S.fib[t];
+                if (f.done) continue;
+                S.cur = t;
+                threadIdx = dim3(t % block.x, 1, 1);
+                void* fake = nullptr;
+                EMU_START_SWITCH(&fake, f.stack.data(), f.stack.size());
+                swapcontext(&S.main, &f.ctx);
+                EMU_FINISH_SWITCH(fake, nullptr, nullptr);
+                if (!f.done) ++live;
+            }
+        }
+    }
+    for (unsigned t = 0; t < nt; ++t) S.fib[t].stack.swap(pool[t]);
+    emu_sched = prev;
+}
+#define hipLaunchKernelGGL(kernel, grid, block, shmem, stream, ...) emu_launch(#kernel, kernel, dim3(grid), dim3(block), __VA_ARGS__)

@@ -1,0 +1,186 @@
+"""Parity of the HIP path (through the C ABI) with the CPU FP64 oracle on the same seeded inputs.
+
+Tolerances: contact pair sets are compared exactly; d^2, energies, gradients and Hessian blocks
+within 1e-9 relative to the largest magnitude (FP64 with a different summation order and FMA
+contraction); PCG solutions (solved to cg_tol 1e-12) within 1e-7; trajectories within 1e-6
+relative when both sides converge Newton to 1e-9.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from colb_amd import scene as S
+from colb_amd.params import north_star, reference_compat
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def O():
+    from oracle import oracle as O
+    O.build(force=False)
+    return O
+
+
+def solver(sc, p):
+    from colb_amd.abd import NewtonAbd
+    return NewtonAbd(p.dt, sc, params=p)
+
+
+def two_cubes(gap=0.0005):
+    V, F = S.cube_mesh()
+    E = S.edges_of_tris(F)
+    M = S.mass4_from_mesh(V, F, 1e3)
+    b = S._Builder()
+    b.add(V, F, E, M, S.q_from_pose([0, 0.5, 0], np.eye(3)))
+    th = 0.3
+    Rm = np.array([[np.cos(th), 0, np.sin(th)], [0, 1, 0], [-np.sin(th), 0, np.cos(th)]])
+    b.add(V, F, E, M, S.q_from_pose([0.1, 1.5 + gap, 0.05], Rm))
+    return b.build("two_cubes")
+
+
+def contact_params(**kw):
+    base = dict(dhat=0.05, k_contact=1e12, kappa=1e5, ground_enabled=False, cg_tol=1e-12, cg_max_iter=500)
+    base.update(kw)
+    return north_star(**base)
+
+
+def perturbed(sc, seed, amp=1e-3):
+    rng = np.random.default_rng(seed)
+    return sc.q0 + amp * rng.normal(size=sc.q0.shape), 0.1 * rng.normal(size=sc.q0.shape)
+
+
+def rel(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+CASES = {
+    "two_cubes": lambda: (two_cubes(), contact_params()),
+    "cubes_ground": lambda: (S.cubes_on_ground(lift=0.02), north_star(ground_enabled=True, dhat=0.05, k_contact=1e14,
+                                                                        cg_tol=1e-12)),
+    "chain3": lambda: (S.chain_net(n=3), north_star(dhat=2e-3, k_contact=1e19, cg_tol=1e-12)),
+    "ref_chain": lambda: (S.load_json_scene(_ref_chain()), reference_compat(cg_tol=1e-12)),
+}
+
+
+def _ref_chain():
+    import os
+    return os.path.join(os.path.dirname(__file__), "golden", "ref_assets", "chains.json")
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_detect_energy_assemble_match_oracle(O, case):
+    say("case", case)
+    sc, p = CASES[case]()
+    q, qd = perturbed(sc, 1, amp=1e-4)
+    orc = O.Oracle(sc, p)
+    orc.set_state(q, sc.q0, qd)
+    gpu = solver(sc, p)
+    gpu.set_state(q, sc.q0, qd)
+    # contact pair sets: exact
+    po, d2o = orc.detect()
+    pg, d2g = gpu.detect_collision()
+    so = {tuple(r): d for r, d in zip(po.tolist(), d2o)}
+    sg = {tuple(r): d for r, d in zip(pg.tolist(), d2g)}
+    say("  active pairs", len(so), len(sg))
+    assert set(so) == set(sg)
+    for k in so:
+        assert sg[k] == pytest.approx(so[k], rel=1e-10, abs=1e-18)
+    # energy
+    Eo, Eg = orc.energy(), gpu.compute_g()
+    assert Eg == pytest.approx(Eo[0], rel=1e-11)
+    # gradient and Hessian blocks
+    go, Do, pijo, offo = orc.assemble()
+    gg, Dg, pijg, offg, Eg2 = gpu.assemble()
+    assert Eg2[0] == pytest.approx(Eo[0], rel=1e-11)
+    assert rel(gg, go) < 1e-9
+    assert rel(Dg, Do) < 1e-9
+    assert sorted(map(tuple, pijo.tolist())) == sorted(map(tuple, pijg.tolist()))
+    og = {tuple(k): B for k, B in zip(pijg.tolist(), offg)}
+    for k, B in zip(pijo.tolist(), offo):
+        assert rel(og[tuple(k)], B) < 1e-9 or np.abs(og[tuple(k)] - B).max() < 1e-9 * np.abs(Do).max()
+
+
+@pytest.mark.parametrize("case", ["two_cubes", "cubes_ground", "chain3"])
+def test_solve_and_ccd_match_oracle(O, case):
+    say("case", case)
+    sc, p = CASES[case]()
+    q, qd = perturbed(sc, 2, amp=1e-4)
+    orc = O.Oracle(sc, p)
+    orc.set_state(q, sc.q0, qd)
+    gpu = solver(sc, p)
+    gpu.set_state(q, sc.q0, qd)
+    duo, ito, reso = orc.solve()
+    resg = gpu.compute_du()
+    dug = gpu._du
+    say("  residual", reso[0], resg, "cg", ito[0], gpu.last_cg_iters[0])
+    assert resg == pytest.approx(reso[0], rel=1e-10)
+    assert rel(dug, duo) < 1e-7
+    ao = orc.ccd(duo)
+    ag = gpu.ccd(duo)
+    say("  ccd", ao, ag)
+    assert ag[0] == pytest.approx(ao[0], rel=1e-9)
+
+
+@pytest.mark.parametrize("case,nsteps", [("cubes_ground", 25), ("two_cubes", 10), ("chain3", 5)])
+def test_trajectory_matches_oracle(O, case, nsteps):
+    say("case", case)
+    sc, p = CASES[case]()
+    p = p.replace(newton_tol=1e-9, cg_tol=1e-12, cg_max_iter=2000)
+    orc = O.Oracle(sc, p)
+    gpu = solver(sc, p)
+    for k in range(nsteps):
+        so = orc.step()[0]
+        sg = gpu.step()[0]
+        qo = orc.get_state()[0]
+        qg = gpu.get_state()[0]
+        err = rel(qg, qo)
+        say(f"  step {k} newton {so['newton_iters']}/{sg['newton_iters']} active {so['n_active']}/{sg['n_active']} "
+            f"err {err:.2e}")
+        assert err < 1e-6
+        assert sg["n_active"] == so["n_active"]
+
+
+def test_multi_scene_batch_equals_single_scenes():
+    a = S.cubes_on_ground(seed=1, lift=0.02)
+    b = S.cubes_on_ground(seed=5, lift=0.02)
+    p = north_star(ground_enabled=True, dhat=0.05, k_contact=1e14, newton_tol=1e-9, cg_tol=1e-12)
+    ab = solver(S.concat_scenes([a, b]), p)
+    ga, gb = solver(a, p), solver(b, p)
+    for k in range(8):
+        ab.step(), ga.step(), gb.step()
+    q = ab.get_state()[0]
+    assert rel(q[:8], ga.get_state()[0]) < 1e-12
+    assert rel(q[8:], gb.get_state()[0]) < 1e-12
+
+
+def test_pinned_bodies_stay_fixed_and_no_penetration():
+    sc = S.chain_net(n=4)
+    p = north_star(dhat=1e-3, k_contact=1e20)
+    g = solver(sc, p)
+    q0 = sc.q0.copy()
+    for k in range(3):
+        st = g.step()
+        say("  step", k, st[0])
+    q = g.get_state()[0]
+    pin = sc.body_pinned.astype(bool)
+    np.testing.assert_array_equal(q[pin], q0[pin])
+    assert np.all(q[~pin, 1] < q0[~pin, 1])
+    pairs, d2 = g.detect_collision()
+    assert len(d2) == 0 or d2.min() > 0
+
+
+def test_deterministic_repeat():
+    sc = S.cubes_on_ground(lift=0.02)
+    p = north_star(ground_enabled=True, dhat=0.05, k_contact=1e14)
+    a, b = solver(sc, p), solver(sc, p)
+    for k in range(10):
+        a.step(), b.step()
+    np.testing.assert_array_equal(a.get_state()[0], b.get_state()[0])

@@ -70,17 +70,17 @@ def test_ortho_closed_form_and_fd(oracle_mod):
 # distances: classification, value (brute force), gradient and Hessian (finite differences)
 # ---------------------------------------------------------------------------------------------
 def brute_pt(x):
+    """min over the triangle via the square -> triangle map (u, v) = (s (1 - t), s t), so every
+    iterate is feasible; multi-start L-BFGS-B."""
     p, a, b, c = x.reshape(4, 3)
-    def f(uv):
-        u, v = uv
-        y = a + u * (b - a) + v * (c - a)
+    def f(st):
+        s, t = st
+        y = a + s * (1 - t) * (b - a) + s * t * (c - a)
         return np.sum((p - y) ** 2)
     best = None
-    for u0 in np.linspace(0, 1, 21):
-        for v0 in np.linspace(0, 1 - u0, max(2, int(21 * (1 - u0)))):
-            r = minimize(f, [u0, v0], method="SLSQP", bounds=[(0, 1), (0, 1)],
-                         constraints=[{"type": "ineq", "fun": lambda uv: 1 - uv[0] - uv[1]}],
-                         options={"ftol": 1e-16, "maxiter": 200})
+    for s0 in np.linspace(0, 1, 7):
+        for t0 in np.linspace(0, 1, 7):
+            r = minimize(f, [s0, t0], method="L-BFGS-B", bounds=[(0, 1), (0, 1)], options={"ftol": 1e-18, "gtol": 1e-14})
             if best is None or r.fun < best:
                 best = r.fun
     return best
