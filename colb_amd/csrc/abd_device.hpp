@@ -15,6 +15,20 @@
 
 namespace abd {
 
+// Optimisation barrier for bit-field extractions that feed address arithmetic.  hipcc (ROCm 7.2)
+// miscompiles `(size_t)(x & 0xFFFFFF) * C`: the mask is kept for comparisons but dropped from the
+// operand of the v_mad_u64_u32 that forms the address, so the unmasked register is used.  An
+// empty asm with a read-write register operand makes the masked value opaque and forces its use.
+#if defined(ABD_EMULATION)
+#define ABD_OPAQUE(v) asm volatile("" : "+r"(v))
+#else
+#define ABD_OPAQUE(v) asm volatile("" : "+v"(v))
+#endif
+__device__ __forceinline__ int opaque(int v) {
+    ABD_OPAQUE(v);
+    return v;
+}
+
 struct D3 {
     double x, y, z;
 };
@@ -249,7 +263,7 @@ __device__ __forceinline__ double wave_sum(double v) {
 // Cyclic Jacobi PSD projection of the symmetric m x m matrix A (LDS, row-major), by one wave:
 // A <- V max(Lambda, 0) V^T.  Same rotation order and stopping rule as the oracle's psd_project
 // (Golub & Van Loan 8.5).  V is LDS scratch of m*m doubles.
-__device__ void wave_psd_project(double* A, double* V, int m, int lane) {
+__device__ inline void wave_psd_project(double* A, double* V, int m, int lane) {
     const int mm = m * m;
     for (int e = lane; e < mm; e += 64) V[e] = (e / m == e % m) ? 1.0 : 0.0;
     double f = 0;

@@ -20,6 +20,12 @@ namespace abd {
 
 static inline int cdiv(long a, int b) { return (int)((a + b - 1) / b); }
 
+// device-side key field extraction through the optimisation barrier (see opaque())
+__device__ __forceinline__ int dkey_scene(uint64_t k) { return opaque((int)(k >> 48)); }
+__device__ __forceinline__ int dkey_kind(uint64_t k) { return opaque((int)((k >> 46) & 3)); }
+__device__ __forceinline__ int dkey_il(uint64_t k) { return opaque((int)((k >> 23) & KEY_IMASK)); }
+__device__ __forceinline__ int dkey_jl(uint64_t k) { return opaque((int)(k & KEY_IMASK)); }
+
 // ---------------------------------------------------------------------------------------------
 // state kernels
 // ---------------------------------------------------------------------------------------------
@@ -379,9 +385,9 @@ void launch_query_ee(hipStream_t s, const Geo& G, const BvhNode* nodes, const in
 __global__ void k_cand_ranges(int n, const uint64_t* __restrict__ k, int* lo, int* hi) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    int s = key_scene(k[i]);
-    if (i == 0 || key_scene(k[i - 1]) != s) lo[s] = i;
-    if (i == n - 1 || key_scene(k[i + 1]) != s) hi[s] = i + 1;
+    int s = dkey_scene(k[i]);
+    if (i == 0 || dkey_scene(k[i - 1]) != s) lo[s] = i;
+    if (i == n - 1 || dkey_scene(k[i + 1]) != s) hi[s] = i + 1;
 }
 
 void launch_cand_ranges(hipStream_t s, int n, const uint64_t* keys, int* lo, int* hi) {
@@ -401,11 +407,11 @@ struct PairV {
 // primitive ranges; a key that fails sets ERR_DECODE and decodes to kind -1 (callers skip it).
 __device__ __forceinline__ PairV decode(const Geo& G, uint64_t key) {
     PairV p;
-    p.kind = key_kind(key);
-    p.scene = key_scene(key);
+    p.kind = dkey_kind(key);
+    p.scene = dkey_scene(key);
     p.nvp = 0;
     p.v[0] = p.v[1] = p.v[2] = p.v[3] = 0;
-    int il = key_il(key), jl = key_jl(key);
+    int il = dkey_il(key), jl = dkey_jl(key);
     if (p.scene >= G.ns) { atomicOr(G.err, ERR_DECODE); p.kind = -1; p.scene = 0; return p; }
     const int s = p.scene;
     if (p.kind == K_PT) {
@@ -642,7 +648,7 @@ __global__ void k_pair_unique(int n, const uint64_t* __restrict__ k, const int* 
     if (head[i]) {
         int u = pos[i];
         pair_b[2 * u] = (int)(k[i] >> 44);
-        pair_b[2 * u + 1] = (int)((k[i] >> 24) & 0xFFFFF);
+        pair_b[2 * u + 1] = opaque((int)((k[i] >> 24) & 0xFFFFF));
         run_start[u] = i;
     }
     if (i == n - 1) {
@@ -672,7 +678,7 @@ __global__ void k_offblock_sum(int n_pairs, const int* __restrict__ pair_b, cons
     int bmin = pair_b[2 * u];
     double h = 0.0;
     for (int m = run_start[u]; m < run_start[u + 1]; ++m) {
-        int c = (int)(k[m] & 0xFFFFFF);
+        int c = opaque((int)(k[m] & 0xFFFFFF));
         if (c >= n_act) { atomicOr(err, ERR_PAIR); break; }
         const double* Hc = H432 + (size_t)c * 432 + 288;
         h += (body[c].x == bmin) ? Hc[R * 12 + C] : Hc[C * 12 + R];
@@ -803,7 +809,7 @@ __global__ __launch_bounds__(64) void k_body_assemble(Geo G, AbdParams P, const 
     int hi = lower_bound64(inc, n_inc, (uint64_t)(b + 1) << 32);
     for (int m = lo; m < hi; ++m) {
         uint32_t w = (uint32_t)inc[m];
-        int c = (int)(w >> 1), role = (int)(w & 1);
+        int c = opaque((int)(w >> 1)), role = opaque((int)(w & 1));
         if (c >= n_act) {
             if (lane == 0) atomicOr(G.err, ERR_INCIDENCE);
             break;
@@ -987,13 +993,13 @@ __global__ __launch_bounds__(64) void k_pcg_A(int it, int ntiles, const int* __r
         const double* D = diag + (size_t)b * 144 + row * 12;
         for (int c = 0; c < 12; ++c) apv += D[c] * (z[12 * b + c] + beta * p_old[12 * b + c]);
         for (int e = row_ptr[b]; e < row_ptr[b + 1]; ++e) {
-            int col = (int)(ent_key[e] & 0xFFFFFFFFu);
+            int col = opaque((int)(ent_key[e] & 0xFFFFFFFFu));
             int v = ent_val[e];
             if (col < 0 || col >= nb || (v >> 1) >= n_pairs || v < 0) {
                 atomicOr(err, ERR_CSR);
                 break;
             }
-            const double* Bk = ((v & 1) ? offT : off) + (size_t)(v >> 1) * 144 + row * 12;
+            const double* Bk = (opaque(v & 1) ? offT : off) + (size_t)opaque(v >> 1) * 144 + row * 12;
             for (int c = 0; c < 12; ++c) apv += Bk[c] * (z[12 * col + c] + beta * p_old[12 * col + c]);
         }
         Ap[idx] = apv;

@@ -9,6 +9,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "abd_device.hpp"
+
 namespace abd {
 
 #define SORT_T 256
@@ -103,7 +105,7 @@ __global__ __launch_bounds__(SORT_T) void k_radix_hist(int n, const uint64_t* __
     __syncthreads();
     const int base = blockIdx.x * SORT_TILE + threadIdx.x * SORT_I;
     for (int i = 0; i < SORT_I; ++i)
-        if (base + i < n) atomicAdd(&cnt[(int)((keys[base + i] >> shift) & (RADIX - 1))], 1);
+        if (base + i < n) atomicAdd(&cnt[opaque((int)((keys[base + i] >> shift) & (RADIX - 1)))], 1);
     __syncthreads();
     if (threadIdx.x < RADIX) hist[threadIdx.x * nblk + blockIdx.x] = cnt[threadIdx.x];
 }
@@ -123,7 +125,7 @@ __global__ __launch_bounds__(SORT_T) void k_radix_scatter(int n, const uint64_t*
     for (int i = 0; i < SORT_I; ++i) {
         if (base + i < n) {
             k[i] = kin[base + i];
-            dg[i] = (int)((k[i] >> shift) & (RADIX - 1));
+            dg[i] = opaque((int)((k[i] >> shift) & (RADIX - 1)));
             c[dg[i]]++;
         } else {
             dg[i] = -1;

@@ -8,7 +8,7 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 CSRC = os.path.join(ROOT, "colb_amd", "csrc")
 OUT = os.path.join(HERE, "_build")
 LIB = os.path.join(OUT, "libabd_emu.so")
-FLAGS = ["-std=c++20", "-O1", "-g", "-fPIC", "-fno-omit-frame-pointer", "-fsanitize=address",
+FLAGS = ["-std=c++20", "-DABD_EMULATION", "-O1", "-g", "-fPIC", "-fno-omit-frame-pointer", "-fsanitize=address",
          "-I", os.path.join(HERE, "include"), "-I", os.path.join(ROOT, "include"), "-pthread",
          "-Wno-unused-result", "-Wno-attributes"]
 

@@ -194,9 +194,20 @@ void emu_launch(const char* name, K kernel, dim3 grid, dim3 block, Args... args)
             makecontext(&f.ctx, emu_fiber_entry, 0);
         }
         unsigned live = nt;
+        static const bool shuffle = std::getenv("EMU_SHUFFLE") != nullptr;
+        std::vector<unsigned> order(nt);
+        for (unsigned t = 0; t < nt; ++t) order[t] = t;
+        static unsigned long long rng = 88172645463325252ull;
         while (live > 0) {
             live = 0;
-            for (unsigned t = 0; t < nt; ++t) {
+            if (shuffle)
+                for (unsigned t = nt - 1; t > 0; --t) {
+                    rng ^= rng << 13; rng ^= rng >> 7; rng ^= rng << 17;
+                    unsigned j = (unsigned)(rng % (t + 1));
+                    unsigned tmp = order[t]; order[t] = order[j]; order[j] = tmp;
+                }
+            for (unsigned oi = 0; oi < nt; ++oi) {
+                unsigned t = order[oi];
                 EmuFiber& f = S.fib[t];
                 if (f.done) continue;
                 S.cur = t;
