@@ -78,16 +78,17 @@ struct AbdCtx {
     double *q = nullptr, *qnow = nullptr, *qdot = nullptr, *qtilde = nullptr, *qtmp = nullptr;
     double *du = nullptr, *grad = nullptr, *diag = nullptr, *pinv = nullptr, *e_body = nullptr;
     double4 *X0 = nullptr, *X1 = nullptr, *Xt = nullptr;
-    float* bounds = nullptr;
-    // LBVH (0 = triangles, 1 = edges)
-    struct Tree {
-        int n = 0;
-        float4* boxes = nullptr;
-        uint64_t *keys = nullptr, *skeys = nullptr;
-        int *idx = nullptr, *sidx = nullptr;
-        BvhNode* nodes = nullptr;
-        int *parent = nullptr, *leaf_parent = nullptr, *flags = nullptr, *root = nullptr;
-    } tree[2];
+    // broad phase: per-body ranges, rest-frame BVHs of the distinct meshes, body pairs
+    BodyMesh M{};
+    int *body_v0 = nullptr, *body_e0 = nullptr, *body_t0 = nullptr, *body_mesh = nullptr;
+    int *mesh_tri_root = nullptr, *mesh_edge_root = nullptr;
+    RNode *tri_nodes = nullptr, *edge_nodes = nullptr;
+    int n_meshes = 0;
+    long long *pair_off = nullptr;
+    long long total_body_pairs = 0;
+    float* body_box = nullptr;
+    int2* bpairs = nullptr;
+    int bpair_cap = 0, n_bpairs = 0;
     // candidates
     int cand_cap = 0, n_cand = 0;
     uint64_t *cand = nullptr, *cand_s = nullptr, *act = nullptr;
@@ -366,55 +367,67 @@ int grow_pairs(AbdCtx* ctx, int cap) {
 }
 
 // --------------------------------------------------------------------------------------------
-// broad phase: LBVH over triangles and edges of the swept positions X0 -> X1
-// (ContactSolverBase.update_bvh + edge_edge_collision, contact.py:244-279)
+// broad phase over the linear path q -> q - du (du == nullptr: the static set at q), with
+// X0 / X1 the vertex positions at both ends (ContactSolverBase.update_bvh + the collision
+// queries, contact.py:143-169, 244-279; the rule itself is stated in abd_broad.hip)
 // --------------------------------------------------------------------------------------------
-int build_tree(AbdCtx* ctx, int which, const double4* X0, const double4* X1, double infl) {
-    AbdCtx::Tree& T = ctx->tree[which];
-    if (T.n == 0) return ABD_OK;
-    launch_prim_boxes(ctx->st, ctx->G, which == 0 ? 1 : 0, X0, X1, infl, ctx->bounds, T.boxes, T.keys, T.idx);
-    CHK(dbg(ctx, "launch_prim_boxes"));
-    int end_bit = 32 + bits_for(ctx->ns + 1);
-    CHK(sort_pairs(ctx, T.keys, T.skeys, T.idx, T.sidx, T.n, end_bit));
-    CHK(fill_i32(ctx, T.root, ctx->ns, INT_MIN));
-    launch_bvh_build(ctx->st, T.n, ctx->ns, T.skeys, T.nodes, T.parent, T.leaf_parent, T.root);
-    CHK(dbg(ctx, "launch_bvh_build"));
-    if (T.n > 1) HIPCHK(hipMemsetAsync(T.flags, 0, sizeof(int) * (T.n - 1), ctx->st));
-    launch_bvh_refit(ctx->st, T.n, T.sidx, T.boxes, T.nodes, T.parent, T.leaf_parent, T.flags, ctx->cnt + C_ERR);
-    CHK(dbg(ctx, "launch_bvh_refit"));
-    HIPCHK(hipGetLastError());
+int grow_body_pairs(AbdCtx* ctx, int cap) {
+    if (cap <= ctx->bpair_cap) return ABD_OK;
+    cap = std::max(cap, ctx->bpair_cap * 2);
+    CHK(dfree(ctx, ctx->bpairs));
+    CHK(dalloc(ctx, &ctx->bpairs, cap));
+    ctx->bpair_cap = cap;
     return ABD_OK;
 }
 
-int broad_phase(AbdCtx* ctx, const double4* X0, const double4* X1) {
+int broad_phase(AbdCtx* ctx, const double4* X0, const double4* X1, const double* du) {
     const double infl = ctx->P.dhat;
-    tic(ctx, "broad_phase_build");
-    launch_scene_bounds(ctx->st, ctx->G, X0, X1, ctx->bounds);
-    CHK(dbg(ctx, "launch_scene_bounds"));
-    CHK(build_tree(ctx, 0, X0, X1, 0.0));
-    CHK(build_tree(ctx, 1, X0, X1, 0.5 * infl));
+    tic(ctx, "broad_phase_pairs");
+    launch_body_boxes(ctx->st, ctx->G, ctx->M, X0, X1, 0.5 * infl, ctx->body_box);
+    CHK(dbg(ctx, "launch_body_boxes"));
+    for (int attempt = 0; attempt < 8; ++attempt) {
+        HIPCHK(hipMemsetAsync(ctx->cnt + C_COUNT, 0, 2 * sizeof(int), ctx->st));
+        launch_body_pairs(ctx->st, ctx->G, ctx->pair_off, ctx->total_body_pairs, ctx->body_box, ctx->bpairs,
+                          ctx->bpair_cap, ctx->cnt + C_COUNT, ctx->cnt + C_OVERFLOW);
+        CHK(dbg(ctx, "launch_body_pairs"));
+        HIPCHK(hipGetLastError());
+        CHK(sync_counters(ctx));
+        int n = ctx->h_cnt[C_COUNT];
+        if (n <= ctx->bpair_cap) {
+            ctx->n_bpairs = n;
+            break;
+        }
+        if (attempt == 7) {
+            ctx->err = "body-pair list did not fit";
+            return ABD_ERR_CAPACITY;
+        }
+        CHK(grow_body_pairs(ctx, n + n / 2));
+    }
     toc(ctx);
     for (int attempt = 0; attempt < 8; ++attempt) {
         tic(ctx, "broad_phase_query");
         HIPCHK(hipMemsetAsync(ctx->cnt + C_COUNT, 0, 2 * sizeof(int), ctx->st));
-        launch_query_pt(ctx->st, ctx->G, X0, X1, infl, ctx->P, ctx->tree[0].nodes, ctx->tree[0].sidx,
-                        ctx->tree[0].root, ctx->tree[0].boxes, ctx->cand, ctx->cand_cap, ctx->cnt + C_COUNT,
-                        ctx->cnt + C_OVERFLOW);
-        CHK(dbg(ctx, "launch_query_pt"));
-        launch_query_ee(ctx->st, ctx->G, ctx->tree[1].nodes, ctx->tree[1].sidx, ctx->tree[1].root,
-                        ctx->tree[1].boxes, ctx->cand, ctx->cand_cap, ctx->cnt + C_COUNT, ctx->cnt + C_OVERFLOW);
-        CHK(dbg(ctx, "launch_query_ee"));
+        launch_pair_queries(ctx->st, ctx->G, ctx->M, ctx->n_bpairs, ctx->bpairs, X0, X1, ctx->q, du, ctx->body_box,
+                            infl, ctx->cand, ctx->cand_cap, ctx->cnt + C_COUNT, ctx->cnt + C_OVERFLOW);
+        CHK(dbg(ctx, "launch_pair_queries"));
+        launch_ground_queries(ctx->st, ctx->G, X0, X1, ctx->P, infl, ctx->cand, ctx->cand_cap, ctx->cnt + C_COUNT,
+                              ctx->cnt + C_OVERFLOW);
+        CHK(dbg(ctx, "launch_ground_queries"));
         HIPCHK(hipGetLastError());
         toc(ctx);
         CHK(sync_counters(ctx));
         if (ctx->h_cnt[C_OVERFLOW] == 2) {
-            ctx->err = "LBVH traversal stack overflow";
+            ctx->err = "rest-frame BVH traversal failed (malformed tree or stack overflow)";
             return ABD_ERR_CAPACITY;
         }
         int n = ctx->h_cnt[C_COUNT];
         if (n <= ctx->cand_cap) {
             ctx->n_cand = n;
             break;
+        }
+        if (attempt == 7) {
+            ctx->err = "candidate list did not fit";
+            return ABD_ERR_CAPACITY;
         }
         CHK(grow_candidates(ctx, n + n / 2));
     }
@@ -432,7 +445,7 @@ int broad_phase(AbdCtx* ctx, const double4* X0, const double4* X1) {
 int static_broad_phase(AbdCtx* ctx) {
     launch_xform(ctx->st, ctx->G, ctx->q, nullptr, nullptr, ctx->X0);
     CHK(dbg(ctx, "launch_xform"));
-    CHK(broad_phase(ctx, ctx->X0, ctx->X0));
+    CHK(broad_phase(ctx, ctx->X0, ctx->X0, nullptr));
     ctx->cand_valid = true;
     return ABD_OK;
 }
@@ -565,7 +578,7 @@ int ccd_bound(AbdCtx* ctx) {
     CHK(dbg(ctx, "launch_xform"));
     launch_xform(ctx->st, ctx->G, ctx->q, ctx->du, nullptr, ctx->X1);
     CHK(dbg(ctx, "launch_xform"));
-    CHK(broad_phase(ctx, ctx->X0, ctx->X1));
+    CHK(broad_phase(ctx, ctx->X0, ctx->X1, ctx->du));
     hipLaunchKernelGGL(k_fill_u64, dim3(cdiv(ctx->ns, 128)), dim3(128), 0, ctx->st, ctx->ns, ctx->C.amax_bits,
                        (unsigned long long)0x3FF0000000000000ull);
     CHK(dbg(ctx, "k_fill_u64"));
@@ -678,6 +691,87 @@ int step_once(AbdCtx* ctx) {
 
 int check_ctx(AbdCtx* ctx) { return ctx ? ABD_OK : ABD_ERR_ARG; }
 
+// --------------------------------------------------------------------------------------------
+// rest-frame BVHs (host, once per distinct mesh)
+// --------------------------------------------------------------------------------------------
+float f_rd(double x) {
+    float f = (float)x;
+    if ((double)f > x) f = std::nextafter(f, -HUGE_VALF);
+    return f;
+}
+float f_ru(double x) {
+    float f = (float)x;
+    if ((double)f < x) f = std::nextafter(f, HUGE_VALF);
+    return f;
+}
+
+struct PrimBox {
+    double lo[3], hi[3], c[3];
+};
+
+// Top-down build over [lo, hi) of `order`: median split of the centroids along the longest axis
+// of their box (depth ceil(log2 n) + 1).  Nodes are appended in preorder; returns the node index.
+int bvh_build_range(const std::vector<PrimBox>& pb, std::vector<int>& order, int lo, int hi, std::vector<RNode>& out) {
+    const int me = (int)out.size();
+    out.push_back(RNode{});
+    double blo[3] = {HUGE_VAL, HUGE_VAL, HUGE_VAL}, bhi[3] = {-HUGE_VAL, -HUGE_VAL, -HUGE_VAL};
+    double clo[3] = {HUGE_VAL, HUGE_VAL, HUGE_VAL}, chi[3] = {-HUGE_VAL, -HUGE_VAL, -HUGE_VAL};
+    for (int i = lo; i < hi; ++i) {
+        const PrimBox& b = pb[order[i]];
+        for (int k = 0; k < 3; ++k) {
+            blo[k] = std::min(blo[k], b.lo[k]); bhi[k] = std::max(bhi[k], b.hi[k]);
+            clo[k] = std::min(clo[k], b.c[k]); chi[k] = std::max(chi[k], b.c[k]);
+        }
+    }
+    RNode nd;
+    for (int k = 0; k < 3; ++k) { nd.lo[k] = f_rd(blo[k]); nd.hi[k] = f_ru(bhi[k]); }
+    if (hi - lo == 1) {
+        nd.left = ~order[lo];
+        nd.right = -1;
+    } else {
+        int ax = 0;
+        for (int k = 1; k < 3; ++k)
+            if (chi[k] - clo[k] > chi[ax] - clo[ax]) ax = k;
+        const int mid = lo + (hi - lo) / 2;
+        std::nth_element(order.begin() + lo, order.begin() + mid, order.begin() + hi, [&](int a, int b) {
+            if (pb[a].c[ax] != pb[b].c[ax]) return pb[a].c[ax] < pb[b].c[ax];
+            return a < b;
+        });
+        nd.left = bvh_build_range(pb, order, lo, mid, out);
+        nd.right = bvh_build_range(pb, order, mid, hi, out);
+    }
+    out[me] = nd;
+    return me;
+}
+
+// BVH over primitives given as vertex-index tuples (arity 2 or 3) into `xr` (rest coords, xyz);
+// returns the root (INT_MIN if there are no primitives), nodes appended to `out`.
+int bvh_build(const double* xr, const int* prim, int arity, int n, std::vector<RNode>& out) {
+    if (n <= 0) return INT_MIN;
+    std::vector<PrimBox> pb(n);
+    for (int i = 0; i < n; ++i) {
+        PrimBox& b = pb[i];
+        for (int k = 0; k < 3; ++k) { b.lo[k] = HUGE_VAL; b.hi[k] = -HUGE_VAL; b.c[k] = 0.0; }
+        for (int a = 0; a < arity; ++a) {
+            const double* x = xr + 3 * (size_t)prim[arity * i + a];
+            for (int k = 0; k < 3; ++k) {
+                b.lo[k] = std::min(b.lo[k], x[k]);
+                b.hi[k] = std::max(b.hi[k], x[k]);
+                b.c[k] += x[k] / arity;
+            }
+        }
+    }
+    std::vector<int> order(n);
+    for (int i = 0; i < n; ++i) order[i] = i;
+    return bvh_build_range(pb, order, 0, n, out);
+}
+
+uint64_t fnv1a(uint64_t h, const void* p, size_t n) {
+    const unsigned char* c = (const unsigned char*)p;
+    for (size_t i = 0; i < n; ++i) { h ^= c[i]; h *= 1099511628211ull; }
+    return h;
+}
+
 }  // namespace
 
 // =============================================================================================
@@ -765,6 +859,74 @@ int abd_create(const AbdSceneDesc* d, const AbdParams* p, AbdCtx** out) {
         }
     }
     ctx->h_scene_b0 = sb; ctx->h_scene_v0 = sv; ctx->h_scene_e0 = se; ctx->h_scene_t0 = stt;
+    // per-body primitive ranges: edges and triangles must be grouped by body like the vertices
+    std::vector<int> bv0(nb + 1, 0), be0(nb + 1, 0), bt0(nb + 1, 0);
+    for (int v = 0; v < nv; ++v) bv0[d->vert_body[v] + 1]++;
+    for (int e = 0; e < ne; ++e) {
+        int b = d->vert_body[d->edges[2 * e]];
+        if (e > 0 && b < d->vert_body[d->edges[2 * e - 2]]) {
+            ctx->err = "edges must be grouped by body";
+            return fail(ABD_ERR_ARG);
+        }
+        be0[b + 1]++;
+    }
+    for (int t = 0; t < nt; ++t) {
+        int b = d->vert_body[d->tris[3 * t]];
+        if (t > 0 && b < d->vert_body[d->tris[3 * t - 3]]) {
+            ctx->err = "triangles must be grouped by body";
+            return fail(ABD_ERR_ARG);
+        }
+        bt0[b + 1]++;
+    }
+    for (int b = 0; b < nb; ++b) { bv0[b + 1] += bv0[b]; be0[b + 1] += be0[b]; bt0[b + 1] += bt0[b]; }
+    // distinct local meshes (same rest coordinates and local connectivity) share one BVH pair
+    std::vector<int> bmesh(nb), mesh_rep, mtr, mer;
+    std::vector<RNode> tnodes, enodes;
+    {
+        auto local = [&](int b, int arity) {
+            const int* src = arity == 3 ? d->tris : d->edges;
+            const int p0 = arity == 3 ? bt0[b] : be0[b], p1 = arity == 3 ? bt0[b + 1] : be0[b + 1];
+            std::vector<int> out((size_t)(p1 - p0) * arity);
+            for (size_t i = 0; i < out.size(); ++i) out[i] = src[(size_t)p0 * arity + i] - bv0[b];
+            return out;
+        };
+        std::map<uint64_t, std::vector<int>> by_hash;
+        for (int b = 0; b < nb; ++b) {
+            const int cnts[3] = {bv0[b + 1] - bv0[b], be0[b + 1] - be0[b], bt0[b + 1] - bt0[b]};
+            std::vector<int> lt = local(b, 3), le = local(b, 2);
+            const double* xr = d->rest_x + 3 * (size_t)bv0[b];
+            uint64_t h = 1469598103934665603ull;
+            h = fnv1a(h, cnts, sizeof cnts);
+            h = fnv1a(h, xr, sizeof(double) * 3 * (size_t)cnts[0]);
+            h = fnv1a(h, lt.data(), sizeof(int) * lt.size());
+            h = fnv1a(h, le.data(), sizeof(int) * le.size());
+            int found = -1;
+            for (int m : by_hash[h]) {
+                const int r = mesh_rep[m];
+                if (bv0[r + 1] - bv0[r] != cnts[0] || be0[r + 1] - be0[r] != cnts[1] || bt0[r + 1] - bt0[r] != cnts[2])
+                    continue;
+                if (std::memcmp(d->rest_x + 3 * (size_t)bv0[r], xr, sizeof(double) * 3 * (size_t)cnts[0]) != 0) continue;
+                if (local(r, 3) != lt || local(r, 2) != le) continue;
+                found = m;
+                break;
+            }
+            if (found < 0) {
+                found = (int)mesh_rep.size();
+                mesh_rep.push_back(b);
+                by_hash[h].push_back(found);
+                mtr.push_back(bvh_build(xr, lt.data(), 3, cnts[2], tnodes));
+                mer.push_back(bvh_build(xr, le.data(), 2, cnts[1], enodes));
+            }
+            bmesh[b] = found;
+        }
+    }
+    ctx->n_meshes = (int)mesh_rep.size();
+    std::vector<long long> poff(ns + 1, 0);
+    for (int s = 0; s < ns; ++s) {
+        long long n = sb[s + 1] - sb[s];
+        poff[s + 1] = poff[s] + n * (n - 1) / 2;
+    }
+    ctx->total_body_pairs = poff[ns];
     // tiles of at most 5 bodies inside one scene
     std::vector<int> tb0, tb1, tsc, st0(ns + 1, 0);
     for (int s = 0; s < ns; ++s) {
@@ -807,14 +969,19 @@ int abd_create(const AbdSceneDesc* d, const AbdParams* p, AbdCtx** out) {
     A(scene_tile0, ns + 1); U(scene_tile0, st0.data(), ns + 1);
     A(q, 12 * nb); A(qnow, 12 * nb); A(qdot, 12 * nb); A(qtilde, 12 * nb); A(qtmp, 12 * nb);
     A(du, 12 * nb); A(grad, 12 * nb); A(diag, 144 * nb); A(pinv, 144 * nb); A(e_body, nb);
-    A(X0, nv); A(X1, nv); A(Xt, nv); A(bounds, 6 * ns);
-    for (int w = 0; w < 2; ++w) {
-        AbdCtx::Tree& T = ctx->tree[w];
-        T.n = (w == 0) ? nt : ne;
-        int n = std::max(T.n, 2);
-        A(tree[w].boxes, 2 * n); A(tree[w].keys, n); A(tree[w].skeys, n); A(tree[w].idx, n); A(tree[w].sidx, n);
-        A(tree[w].nodes, n); A(tree[w].parent, n); A(tree[w].leaf_parent, n); A(tree[w].flags, n); A(tree[w].root, ns);
-    }
+    A(X0, nv); A(X1, nv); A(Xt, nv);
+    A(body_v0, nb + 1); U(body_v0, bv0.data(), nb + 1);
+    A(body_e0, nb + 1); U(body_e0, be0.data(), nb + 1);
+    A(body_t0, nb + 1); U(body_t0, bt0.data(), nb + 1);
+    A(body_mesh, nb); U(body_mesh, bmesh.data(), nb);
+    A(mesh_tri_root, ctx->n_meshes); U(mesh_tri_root, mtr.data(), ctx->n_meshes);
+    A(mesh_edge_root, ctx->n_meshes); U(mesh_edge_root, mer.data(), ctx->n_meshes);
+    A(tri_nodes, std::max<size_t>(tnodes.size(), 1)); if (!tnodes.empty()) { U(tri_nodes, tnodes.data(), tnodes.size()); }
+    A(edge_nodes, std::max<size_t>(enodes.size(), 1)); if (!enodes.empty()) { U(edge_nodes, enodes.data(), enodes.size()); }
+    A(pair_off, ns + 1); U(pair_off, poff.data(), ns + 1);
+    A(body_box, 6 * (size_t)nb);
+    ctx->M = BodyMesh{ctx->body_v0, ctx->body_e0, ctx->body_t0, ctx->body_mesh, ctx->mesh_tri_root, ctx->mesh_edge_root,
+                      ctx->tri_nodes, ctx->edge_nodes, (int)tnodes.size(), (int)enodes.size()};
     A(cand_lo, ns); A(cand_hi, ns);
     A(r, 12 * nb); A(z, 12 * nb); A(p0, 12 * nb); A(p1, 12 * nb); A(Ap, 12 * nb);
     A(part, 6 * ctx->ntiles); A(bnorm, ns); A(cg_iters, ns); A(cg_done, ns);
@@ -831,6 +998,8 @@ int abd_create(const AbdSceneDesc* d, const AbdParams* p, AbdCtx** out) {
     if ((rc = grow_candidates(ctx, std::max(1024, std::min(p->max_candidates > 0 ? p->max_candidates : (1 << 20), 1 << 26)))) != ABD_OK) return fail(rc);
     if ((rc = grow_active(ctx, 1024)) != ABD_OK) return fail(rc);
     if ((rc = grow_pairs(ctx, 1024)) != ABD_OK) return fail(rc);
+    if ((rc = grow_body_pairs(ctx, (int)std::max<long long>(1024, std::min<long long>(ctx->total_body_pairs, 16LL * nb)))) != ABD_OK)
+        return fail(rc);
     ctx->G = Geo{nb, nv, ne, nt, ns, ctx->rest, ctx->edges, ctx->tris, ctx->mass4, ctx->pinned, ctx->bscene,
                  ctx->scene_b0, ctx->scene_v0, ctx->scene_e0, ctx->scene_t0, ctx->cnt + C_ERR};
     if (hipMemset(ctx->cnt, 0, C_N * sizeof(int)) != hipSuccess) {

@@ -194,6 +194,30 @@ __device__ __forceinline__ double pair_d2(int kind, const D3* x) {
     return E.d2;
 }
 
+// Bound on how far the pair's distance can change along x0 -> x1 (additive CCD, Li, Kaufman,
+// Jiang 2021): displacements minus their mean (a common translation leaves the distance
+// unchanged), then the largest point motion on each primitive, summed.  p receives the
+// centred displacements.
+__device__ __forceinline__ double motion_bound(int kind, const D3* x0, const D3* x1, D3* p) {
+    D3 mean = mk(0, 0, 0);
+    for (int i = 0; i < 4; ++i) { p[i] = x1[i] - x0[i]; mean = mean + 0.25 * p[i]; }
+    for (int i = 0; i < 4; ++i) p[i] = p[i] - mean;
+    double n0 = sqrt(dot(p[0], p[0])), n1 = sqrt(dot(p[1], p[1])), n2 = sqrt(dot(p[2], p[2])),
+           n3 = sqrt(dot(p[3], p[3]));
+    return (kind == K_PT) ? n0 + fmax(n1, fmax(n2, n3)) : fmax(n0, n1) + fmax(n2, n3);
+}
+
+// Candidate rule of the broad phase: the pair can come closer than `infl` somewhere on the
+// linear path x0 -> x1 (distance at x0 minus the motion bound; relative slack 1e-9 so a pair
+// that is active at any point of the path is never dropped by rounding).
+__device__ __forceinline__ bool may_reach(int kind, const D3* x0, const D3* x1, double infl) {
+    D3 p[4];
+    double lp = motion_bound(kind, x0, x1, p);
+    double d2 = pair_d2(kind, x0);
+    double lim = infl * (1.0 + 1e-9);
+    return lp == 0.0 ? d2 < lim * lim : sqrt(d2) - lp < lim;
+}
+
 // (Dn_i)_{kr} = -mu_i [v]x_{kr} + nu_i [u]x_{kr}
 __device__ __forceinline__ double Dn(const DistEval& E, int i, int k, int r) {
     return -E.mu[i] * skew_rc(E.v, k, r) + E.nu[i] * skew_rc(E.u, k, r);

@@ -23,13 +23,28 @@ __host__ __device__ inline int key_kind(uint64_t k) { return (int)((k >> 46) & 3
 __host__ __device__ inline int key_il(uint64_t k) { return (int)((k >> 23) & KEY_IMASK); }
 __host__ __device__ inline int key_jl(uint64_t k) { return (int)(k & KEY_IMASK); }
 
-// LBVH node: children (>= 0 internal, < 0 leaf ~index in sorted order) and their boxes
-struct BvhNode {
+// Rest-frame BVH node of one mesh (built once on the host): box in the mesh's rest frame,
+// rounded outward to float; left >= 0 and right are children, left < 0 marks a leaf holding the
+// mesh-local primitive ~left.
+struct RNode {
+    float lo[3], hi[3];
     int left, right;
-    float lo_l[3], hi_l[3], lo_r[3], hi_r[3];
-    int pad[2];
 };
-static_assert(sizeof(BvhNode) == 64, "BvhNode is one 64-byte line");
+static_assert(sizeof(RNode) == 32, "RNode is 32 bytes");
+
+// Per-body primitive ranges and the rest-frame BVHs of the distinct meshes.  Vertices, edges and
+// triangles are grouped by body; bodies with identical local meshes share one mesh entry.
+struct BodyMesh {
+    const int* body_v0;  // [nb+1]
+    const int* body_e0;  // [nb+1]
+    const int* body_t0;  // [nb+1]
+    const int* body_mesh;
+    const int* mesh_tri_root;   // INT_MIN: no triangles
+    const int* mesh_edge_root;  // INT_MIN: no edges
+    const RNode* tri_nodes;
+    const RNode* edge_nodes;
+    int n_tri_nodes, n_edge_nodes;
+};
 
 // Static scene tables on the device (read-only during a step).
 struct Geo {
@@ -80,19 +95,17 @@ hipError_t radix_sort64(hipStream_t s, const uint64_t* kin, uint64_t* kout, cons
 
 void launch_qtilde(hipStream_t s, int nb, const double* qnow, const double* qdot, double dt, const double* g, double* qtilde);
 void launch_xform(hipStream_t s, const Geo& G, const double* q, const double* du, const double* alpha_scene, double4* X);
-void launch_scene_bounds(hipStream_t s, const Geo& G, const double4* X0, const double4* X1, float* bounds);
-void launch_prim_boxes(hipStream_t s, const Geo& G, int tri_mode, const double4* X0, const double4* X1, double infl,
-                       const float* bounds, float4* boxes, uint64_t* keys, int* idx);
-void launch_bvh_build(hipStream_t s, int n, int ns, const uint64_t* skeys, BvhNode* nodes, int* parent, int* leaf_parent,
-                      int* scene_root);
-void launch_bvh_refit(hipStream_t s, int n, const int* sidx, const float4* boxes, BvhNode* nodes, const int* parent,
-                      const int* leaf_parent, int* flags, int* err);
-void launch_query_pt(hipStream_t s, const Geo& G, const double4* X0, const double4* X1, double infl, const AbdParams& P,
-                     const BvhNode* nodes, const int* sidx, const int* scene_root, const float4* boxes,
-                     uint64_t* cand, int cap, int* count, int* overflow);
-void launch_query_ee(hipStream_t s, const Geo& G, const BvhNode* nodes, const int* sidx, const int* scene_root,
-                     const float4* boxes, uint64_t* cand, int cap, int* count, int* overflow);
-// tree sizes for traversal checks are passed through Geo::nt / Geo::ne
+// abd_broad.hip
+void launch_body_boxes(hipStream_t s, const Geo& G, const BodyMesh& M, const double4* X0, const double4* X1,
+                       double half_infl, float* box);
+void launch_body_pairs(hipStream_t s, const Geo& G, const long long* pair_off, long long total, const float* box,
+                       int2* out, int cap, int* count, int* overflow);
+void launch_pair_queries(hipStream_t s, const Geo& G, const BodyMesh& M, int npairs, const int2* pairs,
+                         const double4* X0, const double4* X1, const double* q, const double* du, const float* bbox,
+                         double infl, uint64_t* cand, int cap, int* count, int* overflow);
+void launch_ground_queries(hipStream_t s, const Geo& G, const double4* X0, const double4* X1, const AbdParams& P,
+                           double infl, uint64_t* cand, int cap, int* count, int* overflow);
+// abd_kernels.hip
 void launch_cand_ranges(hipStream_t s, int n, const uint64_t* keys, int* lo, int* hi);
 void launch_cand_eval(hipStream_t s, const Geo& G, int n, const uint64_t* keys, const double4* X, const AbdParams& P,
                       int* flag, double* energy);

@@ -67,321 +67,8 @@ void launch_xform(hipStream_t s, const Geo& G, const double* q, const double* du
 }
 
 // ---------------------------------------------------------------------------------------------
-// LBVH broad phase (Karras 2012): Morton keys prefixed by the scene index, so each scene's
-// primitives form one subtree whose root is recorded per scene.
+// per-scene ranges of the sorted candidate keys
 // ---------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_scene_bounds(Geo G, const double4* __restrict__ X0, const double4* __restrict__ X1,
-                                                      float* __restrict__ bounds) {
-    int s = blockIdx.x;
-    int v0 = G.scene_v0[s], v1 = G.scene_v0[s + 1];
-    float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
-    for (int v = v0 + threadIdx.x; v < v1; v += blockDim.x) {
-        double4 a = X0[v], b = X1[v];
-        double pa[3] = {a.x, a.y, a.z}, pb[3] = {b.x, b.y, b.z};
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            lo[k] = fminf(lo[k], (float)fmin(pa[k], pb[k]));
-            hi[k] = fmaxf(hi[k], (float)fmax(pa[k], pb[k]));
-        }
-    }
-    __shared__ float sh[6][256];
-    for (int k = 0; k < 3; ++k) { sh[k][threadIdx.x] = lo[k]; sh[3 + k][threadIdx.x] = hi[k]; }
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if (threadIdx.x < o)
-            for (int k = 0; k < 3; ++k) {
-                sh[k][threadIdx.x] = fminf(sh[k][threadIdx.x], sh[k][threadIdx.x + o]);
-                sh[3 + k][threadIdx.x] = fmaxf(sh[3 + k][threadIdx.x], sh[3 + k][threadIdx.x + o]);
-            }
-        __syncthreads();
-    }
-    if (threadIdx.x < 6) bounds[6 * s + threadIdx.x] = sh[threadIdx.x][0];
-}
-
-void launch_scene_bounds(hipStream_t s, const Geo& G, const double4* X0, const double4* X1, float* bounds) {
-    hipLaunchKernelGGL(k_scene_bounds, dim3(G.ns), dim3(256), 0, s, G, X0, X1, bounds);
-}
-
-__device__ __forceinline__ uint32_t expand_bits10(uint32_t v) {
-    v = (v * 0x00010001u) & 0xFF0000FFu;
-    v = (v * 0x00000101u) & 0x0F00F00Fu;
-    v = (v * 0x00000011u) & 0xC30C30C3u;
-    v = (v * 0x00000005u) & 0x49249249u;
-    return v;
-}
-
-__device__ __forceinline__ uint32_t morton30(float x, float y, float z) {
-    uint32_t ix = (uint32_t)fminf(fmaxf(x * 1024.0f, 0.0f), 1023.0f);
-    uint32_t iy = (uint32_t)fminf(fmaxf(y * 1024.0f, 0.0f), 1023.0f);
-    uint32_t iz = (uint32_t)fminf(fmaxf(z * 1024.0f, 0.0f), 1023.0f);
-    return (expand_bits10(ix) << 2) | (expand_bits10(iy) << 1) | expand_bits10(iz);
-}
-
-// tri_mode 1: triangles (box of the swept triangle); 0: edges (inflated by infl)
-__global__ void k_prim_boxes(Geo G, int tri_mode, const double4* __restrict__ X0, const double4* __restrict__ X1,
-                             double infl, const float* __restrict__ bounds, float4* __restrict__ boxes,
-                             uint64_t* __restrict__ keys, int* __restrict__ idx) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    int n = tri_mode ? G.nt : G.ne;
-    if (i >= n) return;
-    int vs[3], nvp;
-    if (tri_mode) {
-        int4 t = G.tris[i];
-        vs[0] = t.x; vs[1] = t.y; vs[2] = t.z; nvp = 3;
-    } else {
-        int2 e = G.edges[i];
-        vs[0] = e.x; vs[1] = e.y; vs[2] = e.x; nvp = 2;
-    }
-    double lo[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, hi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
-    for (int k = 0; k < nvp; ++k) {
-        double4 a = X0[vs[k]], b = X1[vs[k]];
-        double pa[3] = {a.x, a.y, a.z}, pb[3] = {b.x, b.y, b.z};
-        for (int c = 0; c < 3; ++c) {
-            lo[c] = fmin(lo[c], fmin(pa[c], pb[c]));
-            hi[c] = fmax(hi[c], fmax(pa[c], pb[c]));
-        }
-    }
-    float flo[3], fhi[3];
-    for (int c = 0; c < 3; ++c) {
-        flo[c] = __double2float_rd(lo[c] - infl);
-        fhi[c] = __double2float_ru(hi[c] + infl);
-    }
-    boxes[2 * i] = make_float4(flo[0], flo[1], flo[2], 0.f);
-    boxes[2 * i + 1] = make_float4(fhi[0], fhi[1], fhi[2], 0.f);
-    int s = G.bscene[(int)G.rest[vs[0]].w];
-    const float* B = bounds + 6 * s;
-    float c3[3];
-    for (int c = 0; c < 3; ++c) {
-        float ext = fmaxf(B[3 + c] - B[c], 1e-12f);
-        c3[c] = (0.5f * (flo[c] + fhi[c]) - B[c]) / ext;
-    }
-    keys[i] = ((uint64_t)s << 32) | morton30(c3[0], c3[1], c3[2]);
-    idx[i] = i;
-}
-
-void launch_prim_boxes(hipStream_t s, const Geo& G, int tri_mode, const double4* X0, const double4* X1, double infl,
-                       const float* bounds, float4* boxes, uint64_t* keys, int* idx) {
-    int n = tri_mode ? G.nt : G.ne;
-    if (n == 0) return;
-    hipLaunchKernelGGL(k_prim_boxes, dim3(cdiv(n, 256)), dim3(256), 0, s, G, tri_mode, X0, X1, infl, bounds, boxes, keys, idx);
-}
-
-__device__ __forceinline__ int lcp_delta(const uint64_t* __restrict__ k, int n, int i, int j) {
-    if (j < 0 || j >= n) return -1;
-    uint64_t a = k[i], b = k[j];
-    if (a == b) return 64 + __clz((unsigned)(i ^ j));
-    return __clzll(a ^ b);
-}
-
-__global__ void k_bvh_build(int n, int ns, const uint64_t* __restrict__ k, BvhNode* __restrict__ nodes,
-                            int* __restrict__ parent, int* __restrict__ leaf_parent, int* __restrict__ scene_root) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) {  // a scene with a single primitive is rooted at its leaf
-        int s = (int)(k[i] >> 32);
-        bool first = (i == 0) || (int)(k[i - 1] >> 32) != s;
-        bool last = (i == n - 1) || (int)(k[i + 1] >> 32) != s;
-        if (first && last) scene_root[s] = ~i;
-    }
-    if (i >= n - 1) return;
-    int d = (lcp_delta(k, n, i, i + 1) - lcp_delta(k, n, i, i - 1)) >= 0 ? 1 : -1;
-    int dmin = lcp_delta(k, n, i, i - d);
-    int lmax = 2;
-    while (lcp_delta(k, n, i, i + lmax * d) > dmin) lmax *= 2;
-    int l = 0;
-    for (int t = lmax / 2; t >= 1; t /= 2)
-        if (lcp_delta(k, n, i, i + (l + t) * d) > dmin) l += t;
-    int j = i + l * d;
-    int dnode = lcp_delta(k, n, i, j);
-    int s = 0, t = l;
-    do {
-        t = (t + 1) >> 1;
-        if (lcp_delta(k, n, i, i + (s + t) * d) > dnode) s += t;
-    } while (t > 1);
-    int gamma = i + s * d + min(d, 0);
-    int lo = min(i, j), hi = max(i, j);
-    int left = (lo == gamma) ? ~gamma : gamma;
-    int right = (hi == gamma + 1) ? ~(gamma + 1) : gamma + 1;
-    nodes[i].left = left;
-    nodes[i].right = right;
-    if (left < 0) leaf_parent[~left] = i; else parent[left] = i;
-    if (right < 0) leaf_parent[~right] = i; else parent[right] = i;
-    if (i == 0) parent[0] = -1;
-    int sc = (int)(k[lo] >> 32);
-    if ((int)(k[hi] >> 32) == sc && (lo == 0 || (int)(k[lo - 1] >> 32) != sc) &&
-        (hi == n - 1 || (int)(k[hi + 1] >> 32) != sc))
-        scene_root[sc] = i;
-}
-
-void launch_bvh_build(hipStream_t s, int n, int ns, const uint64_t* skeys, BvhNode* nodes, int* parent, int* leaf_parent,
-                      int* scene_root) {
-    if (n <= 0) return;
-    hipLaunchKernelGGL(k_bvh_build, dim3(cdiv(n, 256)), dim3(256), 0, s, n, ns, skeys, nodes, parent, leaf_parent, scene_root);
-}
-
-// bottom-up refit: the second child to arrive at a node writes the union upwards
-__global__ void k_bvh_refit(int n, const int* __restrict__ sidx, const float4* __restrict__ boxes,
-                            BvhNode* nodes, const int* __restrict__ parent, const int* __restrict__ leaf_parent,
-                            int* flags, int* err) {
-    int leaf = blockIdx.x * blockDim.x + threadIdx.x;
-    if (leaf >= n || n < 2) return;
-    int prim = sidx[leaf];
-    if (prim < 0 || prim >= n) { atomicOr(err, ERR_BVH); return; }
-    float4 blo = boxes[2 * prim], bhi = boxes[2 * prim + 1];
-    float lo[3] = {blo.x, blo.y, blo.z}, hi[3] = {bhi.x, bhi.y, bhi.z};
-    int child = ~leaf;
-    int node = leaf_parent[leaf];
-    int guard = 0;
-    while (node >= 0) {
-        if (node >= n - 1 || ++guard > 2 * n) { atomicOr(err, ERR_BVH); return; }
-        volatile BvhNode* nd = nodes + node;
-        bool is_left = (nd->left == child);
-        for (int c = 0; c < 3; ++c) {
-            if (is_left) { nd->lo_l[c] = lo[c]; nd->hi_l[c] = hi[c]; }
-            else { nd->lo_r[c] = lo[c]; nd->hi_r[c] = hi[c]; }
-        }
-        __threadfence();
-        int old = atomicAdd(&flags[node], 1);
-        if (old == 0) return;
-        __threadfence();
-        for (int c = 0; c < 3; ++c) {
-            lo[c] = fminf(nd->lo_l[c], nd->lo_r[c]);
-            hi[c] = fmaxf(nd->hi_l[c], nd->hi_r[c]);
-        }
-        child = node;
-        node = parent[node];
-    }
-}
-
-void launch_bvh_refit(hipStream_t s, int n, const int* sidx, const float4* boxes, BvhNode* nodes, const int* parent,
-                      const int* leaf_parent, int* flags, int* err) {
-    if (n < 2) return;
-    hipLaunchKernelGGL(k_bvh_refit, dim3(cdiv(n, 256)), dim3(256), 0, s, n, sidx, boxes, nodes, parent, leaf_parent, flags,
-                       err);
-}
-
-__device__ __forceinline__ bool ov(const float* qlo, const float* qhi, const float* lo, const float* hi) {
-    return !(qlo[0] > hi[0] || lo[0] > qhi[0] || qlo[1] > hi[1] || lo[1] > qhi[1] || qlo[2] > hi[2] || lo[2] > qhi[2]);
-}
-
-#define BVH_STACK 64
-
-// Calls f(leaf_sorted_index) for every leaf whose box overlaps the query box.
-template <class F>
-__device__ __forceinline__ bool bvh_query(const BvhNode* __restrict__ nodes, const float4* __restrict__ boxes,
-                                          const int* __restrict__ sidx, int n, int root, const float* qlo,
-                                          const float* qhi, F&& f) {
-    if (root == INT_MIN) return true;
-    if (root < 0) {
-        if (~root >= n) return false;
-        int prim = sidx[~root];
-        if (prim < 0 || prim >= n) return false;
-        float4 a = boxes[2 * prim], b = boxes[2 * prim + 1];
-        float lo[3] = {a.x, a.y, a.z}, hi[3] = {b.x, b.y, b.z};
-        if (ov(qlo, qhi, lo, hi)) f(~root);
-        return true;
-    }
-    int stack[BVH_STACK];
-    int sp = 0;
-    stack[sp++] = root;
-    while (sp > 0) {
-        int ni = stack[--sp];
-        if (ni < 0 || ni >= n - 1) return false;
-        const BvhNode nd = nodes[ni];
-        bool l = ov(qlo, qhi, nd.lo_l, nd.hi_l), r = ov(qlo, qhi, nd.lo_r, nd.hi_r);
-        if ((nd.left < 0 && ~nd.left >= n) || (nd.right < 0 && ~nd.right >= n)) return false;
-        if (l) {
-            if (nd.left < 0) f(~nd.left);
-            else if (sp < BVH_STACK) stack[sp++] = nd.left;
-            else return false;
-        }
-        if (r) {
-            if (nd.right < 0) f(~nd.right);
-            else if (sp < BVH_STACK) stack[sp++] = nd.right;
-            else return false;
-        }
-    }
-    return true;
-}
-
-__device__ __forceinline__ void emit(uint64_t key, uint64_t* cand, int cap, int* count, int* overflow) {
-    int slot = atomicAdd(count, 1);
-    if (slot < cap) cand[slot] = key;
-    else *overflow = 1;
-}
-
-// point-triangle candidates (one thread per vertex) + point-ground candidates
-__global__ void k_query_pt(Geo G, const double4* __restrict__ X0, const double4* __restrict__ X1, double infl,
-                           int ground, double gn0, double gn1, double gn2, double goff,
-                           const BvhNode* __restrict__ nodes, const int* __restrict__ sidx,
-                           const int* __restrict__ scene_root, const float4* __restrict__ boxes, uint64_t* cand,
-                           int cap, int* count, int* overflow) {
-    int v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= G.nv) return;
-    int b = (int)G.rest[v].w;
-    int s = G.bscene[b];
-    int pin = G.pinned[b];
-    double4 a = X0[v], c = X1[v];
-    float qlo[3] = {__double2float_rd(fmin(a.x, c.x) - infl), __double2float_rd(fmin(a.y, c.y) - infl),
-                    __double2float_rd(fmin(a.z, c.z) - infl)};
-    float qhi[3] = {__double2float_ru(fmax(a.x, c.x) + infl), __double2float_ru(fmax(a.y, c.y) + infl),
-                    __double2float_ru(fmax(a.z, c.z) + infl)};
-    int vl = v - G.scene_v0[s];
-    int t0 = G.scene_t0[s];
-    if (G.nt > 0) {
-        bool ok = bvh_query(nodes, boxes, sidx, G.nt, scene_root[s], qlo, qhi, [&](int leaf) {
-            int t = sidx[leaf];
-            if (t < 0 || t >= G.nt) { atomicOr(G.err, ERR_BVH); return; }
-            int tb = G.tris[t].w;
-            if (tb == b || (pin && G.pinned[tb])) return;
-            emit(make_key(s, K_PT, vl, t - t0), cand, cap, count, overflow);
-        });
-        if (!ok) { *overflow = 2; atomicOr(G.err, ERR_BVH); }
-    }
-    if (ground && !pin) {
-        double d0 = gn0 * a.x + gn1 * a.y + gn2 * a.z - goff, d1 = gn0 * c.x + gn1 * c.y + gn2 * c.z - goff;
-        if (fmin(d0, d1) < infl) emit(make_key(s, K_PG, vl, 0), cand, cap, count, overflow);
-    }
-}
-
-void launch_query_pt(hipStream_t s, const Geo& G, const double4* X0, const double4* X1, double infl, const AbdParams& P,
-                     const BvhNode* nodes, const int* sidx, const int* scene_root, const float4* boxes,
-                     uint64_t* cand, int cap, int* count, int* overflow) {
-    hipLaunchKernelGGL(k_query_pt, dim3(cdiv(G.nv, 128)), dim3(128), 0, s, G, X0, X1, infl, P.ground_enabled,
-                       P.ground_normal[0], P.ground_normal[1], P.ground_normal[2], P.ground_offset, nodes, sidx,
-                       scene_root, boxes, cand, cap, count, overflow);
-}
-
-// edge-edge candidates (one thread per edge, partners with a larger index; contact.py:160)
-__global__ void k_query_ee(Geo G, const BvhNode* __restrict__ nodes, const int* __restrict__ sidx,
-                           const int* __restrict__ scene_root, const float4* __restrict__ boxes, uint64_t* cand,
-                           int cap, int* count, int* overflow) {
-    int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= G.ne) return;
-    int2 ev = G.edges[e];
-    int b = (int)G.rest[ev.x].w;
-    int s = G.bscene[b];
-    int pin = G.pinned[b];
-    float4 lo4 = boxes[2 * e], hi4 = boxes[2 * e + 1];
-    float qlo[3] = {lo4.x, lo4.y, lo4.z}, qhi[3] = {hi4.x, hi4.y, hi4.z};
-    int e0 = G.scene_e0[s];
-    bool ok = bvh_query(nodes, boxes, sidx, G.ne, scene_root[s], qlo, qhi, [&](int leaf) {
-        int f = sidx[leaf];
-        if (f < 0 || f >= G.ne) { atomicOr(G.err, ERR_BVH); return; }
-        if (f <= e) return;
-        int fb = (int)G.rest[G.edges[f].x].w;
-        if (fb == b || (pin && G.pinned[fb])) return;
-        emit(make_key(s, K_EE, e - e0, f - e0), cand, cap, count, overflow);
-    });
-    if (!ok) { *overflow = 2; atomicOr(G.err, ERR_BVH); }
-}
-
-void launch_query_ee(hipStream_t s, const Geo& G, const BvhNode* nodes, const int* sidx, const int* scene_root,
-                     const float4* boxes, uint64_t* cand, int cap, int* count, int* overflow) {
-    if (G.ne == 0) return;
-    hipLaunchKernelGGL(k_query_ee, dim3(cdiv(G.ne, 128)), dim3(128), 0, s, G, nodes, sidx, scene_root, boxes, cand,
-                       cap, count, overflow);
-}
-
 __global__ void k_cand_ranges(int n, const uint64_t* __restrict__ k, int* lo, int* hi) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -1087,12 +774,8 @@ void launch_pcg_any(hipStream_t s, int ns, int ntiles, const int* scene_tile0, c
 // ---------------------------------------------------------------------------------------------
 __device__ double accd_dev(int kind, const D3* x0, const D3* x1, double s, double tmax, int max_iter) {
     D3 p[4], x[4];
-    D3 mean = mk(0, 0, 0);
-    for (int i = 0; i < 4; ++i) { p[i] = x1[i] - x0[i]; mean = mean + 0.25 * p[i]; }
-    for (int i = 0; i < 4; ++i) { p[i] = p[i] - mean; x[i] = x0[i]; }
-    double n0 = sqrt(dot(p[0], p[0])), n1 = sqrt(dot(p[1], p[1])), n2 = sqrt(dot(p[2], p[2])),
-           n3 = sqrt(dot(p[3], p[3]));
-    double lp = (kind == K_PT) ? n0 + fmax(n1, fmax(n2, n3)) : fmax(n0, n1) + fmax(n2, n3);
+    double lp = motion_bound(kind, x0, x1, p);
+    for (int i = 0; i < 4; ++i) x[i] = x0[i];
     if (lp == 0.0) return tmax;
     double d = sqrt(pair_d2(kind, x));
     double g = s * d;

@@ -371,17 +371,31 @@ void psd_project(int n, double* H) {
 // conservative time of impact: at the returned t the gap is >= s * (initial gap).  Vertex
 // trajectories are exactly linear in the line-search step because x = J q is linear in q.
 // ---------------------------------------------------------------------------------------------
-double accd(int kind, const V3* x0, const V3* x1, double s, double tmax, int max_iter) {
-    V3 p[4], x[4];
+// Bound on the change of the pair distance along x0 -> x1: the displacements minus their mean
+// (a common translation leaves the distance unchanged), largest point motion per primitive, summed.
+double motion_bound(int kind, const V3* x0, const V3* x1, V3* p) {
     V3 mean{0, 0, 0};
     for (int i = 0; i < 4; ++i) { p[i] = x1[i] - x0[i]; mean = mean + 0.25 * p[i]; }
-    for (int i = 0; i < 4; ++i) { p[i] = p[i] - mean; x[i] = x0[i]; }
-    double lp;
+    for (int i = 0; i < 4; ++i) p[i] = p[i] - mean;
     auto nrm = [](V3 a) { return std::sqrt(dot(a, a)); };
-    if (kind == ABD_PT)
-        lp = nrm(p[0]) + std::max(nrm(p[1]), std::max(nrm(p[2]), nrm(p[3])));
-    else
-        lp = std::max(nrm(p[0]), nrm(p[1])) + std::max(nrm(p[2]), nrm(p[3]));
+    if (kind == ABD_PT) return nrm(p[0]) + std::max(nrm(p[1]), std::max(nrm(p[2]), nrm(p[3])));
+    return std::max(nrm(p[0]), nrm(p[1])) + std::max(nrm(p[2]), nrm(p[3]));
+}
+
+// Candidate rule: the pair can come closer than `infl` on the path x0 -> x1 (distance at x0
+// minus the motion bound, relative slack 1e-9 so rounding never drops a pair active on the path).
+bool may_reach(int kind, const V3* x0, const V3* x1, double infl) {
+    V3 p[4];
+    double lp = motion_bound(kind, x0, x1, p);
+    double d2 = pair_dist2(kind, x0, nullptr, nullptr, nullptr);
+    double lim = infl * (1.0 + 1e-9);
+    return lp == 0.0 ? d2 < lim * lim : std::sqrt(d2) - lp < lim;
+}
+
+double accd(int kind, const V3* x0, const V3* x1, double s, double tmax, int max_iter) {
+    V3 p[4], x[4];
+    double lp = motion_bound(kind, x0, x1, p);
+    for (int i = 0; i < 4; ++i) x[i] = x0[i];
     if (lp == 0.0) return tmax;
     double d = std::sqrt(pair_dist2(kind, x, nullptr, nullptr, nullptr));
     double g = s * d;
@@ -513,9 +527,10 @@ int pair_scene(const Ctx& C, int kind, int i) {
 
 bool pair_dynamic(const Ctx& C, int b0, int b1) { return !(C.pinned[b0] && C.pinned[b1]); }
 
-// Candidate pairs whose primitives' boxes over X0 -> X1 (swept; X1 may equal X0), inflated by
-// `infl`, overlap.  Same-body pairs, pairs sharing a vertex (contact.py:158-160) and pairs
-// between two pinned bodies are skipped.  EE keeps i < j (contact.py:160).
+// Candidate pairs over the linear path X0 -> X1 (X1 may equal X0): the primitives' swept boxes
+// overlap (point box inflated by `infl`, edge boxes by infl/2 each) and may_reach holds.
+// Same-body pairs, pairs sharing a vertex (contact.py:158-160) and pairs between two pinned
+// bodies are skipped.  EE keeps i < j (contact.py:160).
 void broad_phase(const Ctx& C, const std::vector<V3>& X0, const std::vector<V3>& X1, double infl,
                  std::vector<Pair>& out) {
     out.clear();
@@ -559,6 +574,10 @@ void broad_phase(const Ctx& C, const std::vector<V3>& X0, const std::vector<V3>&
                 if (bt == bv || !pair_dynamic(C, bv, bt)) continue;
                 if (C.tris[3 * t] == v || C.tris[3 * t + 1] == v || C.tris[3 * t + 2] == v) continue;
                 if (!overlap(vbox[v], tbox[t])) continue;
+                const int* tv = &C.tris[3 * t];
+                V3 a[4] = {X0[v], X0[tv[0]], X0[tv[1]], X0[tv[2]]};
+                V3 b[4] = {X1[v], X1[tv[0]], X1[tv[1]], X1[tv[2]]};
+                if (!may_reach(ABD_PT, a, b, infl)) continue;
                 local[k].push_back({ABD_PT, v, t});
             }
         }
@@ -576,6 +595,9 @@ void broad_phase(const Ctx& C, const std::vector<V3>& X0, const std::vector<V3>&
                 if (C.vbody[a0] == C.vbody[b0] || !pair_dynamic(C, C.vbody[a0], C.vbody[b0])) continue;
                 if (a0 == b0 || a0 == b1 || a1 == b0 || a1 == b1) continue;
                 if (!overlap(ebox[e], ebox[f])) continue;
+                V3 a[4] = {X0[a0], X0[a1], X0[b0], X0[b1]};
+                V3 b[4] = {X1[a0], X1[a1], X1[b0], X1[b1]};
+                if (!may_reach(ABD_EE, a, b, infl)) continue;
                 local[sv.size() + k].push_back({ABD_EE, e, f});
             }
         }
