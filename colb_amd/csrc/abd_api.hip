@@ -81,8 +81,8 @@ struct AbdCtx {
     // broad phase: per-body ranges, rest-frame BVHs of the distinct meshes, body pairs
     BodyMesh M{};
     int *body_v0 = nullptr, *body_e0 = nullptr, *body_t0 = nullptr, *body_mesh = nullptr;
-    int *mesh_tri_root = nullptr, *mesh_edge_root = nullptr;
-    RNode *tri_nodes = nullptr, *edge_nodes = nullptr;
+    int *mesh_tn0 = nullptr, *mesh_en0 = nullptr, *body_tw0 = nullptr, *body_ew0 = nullptr;
+    BNode *tri_nodes = nullptr, *edge_nodes = nullptr, *tri_world = nullptr, *edge_world = nullptr;
     int n_meshes = 0;
     long long *pair_off = nullptr;
     long long total_body_pairs = 0;
@@ -404,11 +404,15 @@ int broad_phase(AbdCtx* ctx, const double4* X0, const double4* X1, const double*
         CHK(grow_body_pairs(ctx, n + n / 2));
     }
     toc(ctx);
+    tic(ctx, "broad_phase_nodes");
+    launch_world_nodes(ctx->st, ctx->G, ctx->M, ctx->q, du);
+    CHK(dbg(ctx, "launch_world_nodes"));
+    toc(ctx);
     for (int attempt = 0; attempt < 8; ++attempt) {
         tic(ctx, "broad_phase_query");
         HIPCHK(hipMemsetAsync(ctx->cnt + C_COUNT, 0, 2 * sizeof(int), ctx->st));
-        launch_pair_queries(ctx->st, ctx->G, ctx->M, ctx->n_bpairs, ctx->bpairs, X0, X1, ctx->q, du, ctx->body_box,
-                            infl, ctx->cand, ctx->cand_cap, ctx->cnt + C_COUNT, ctx->cnt + C_OVERFLOW);
+        launch_pair_queries(ctx->st, ctx->G, ctx->M, ctx->n_bpairs, ctx->bpairs, X0, X1, ctx->body_box, infl,
+                            ctx->cand, ctx->cand_cap, ctx->cnt + C_COUNT, ctx->cnt + C_OVERFLOW);
         CHK(dbg(ctx, "launch_pair_queries"));
         launch_ground_queries(ctx->st, ctx->G, X0, X1, ctx->P, infl, ctx->cand, ctx->cand_cap, ctx->cnt + C_COUNT,
                               ctx->cnt + C_OVERFLOW);
@@ -710,10 +714,14 @@ struct PrimBox {
 };
 
 // Top-down build over [lo, hi) of `order`: median split of the centroids along the longest axis
-// of their box (depth ceil(log2 n) + 1).  Nodes are appended in preorder; returns the node index.
-int bvh_build_range(const std::vector<PrimBox>& pb, std::vector<int>& order, int lo, int hi, std::vector<RNode>& out) {
+// of their box (depth ceil(log2 n) + 1).  Nodes are appended to `out` in preorder, so an internal
+// node's first child is the next node; `skip` is the mesh-local index one past the node's subtree
+// (where a traversal continues after rejecting or finishing the node).  `base` is the index of the
+// mesh's root in `out`.
+void bvh_build_range(const std::vector<PrimBox>& pb, std::vector<int>& order, int lo, int hi, int base,
+                     std::vector<BNode>& out) {
     const int me = (int)out.size();
-    out.push_back(RNode{});
+    out.push_back(BNode{});
     double blo[3] = {HUGE_VAL, HUGE_VAL, HUGE_VAL}, bhi[3] = {-HUGE_VAL, -HUGE_VAL, -HUGE_VAL};
     double clo[3] = {HUGE_VAL, HUGE_VAL, HUGE_VAL}, chi[3] = {-HUGE_VAL, -HUGE_VAL, -HUGE_VAL};
     for (int i = lo; i < hi; ++i) {
@@ -723,11 +731,10 @@ int bvh_build_range(const std::vector<PrimBox>& pb, std::vector<int>& order, int
             clo[k] = std::min(clo[k], b.c[k]); chi[k] = std::max(chi[k], b.c[k]);
         }
     }
-    RNode nd;
+    BNode nd;
     for (int k = 0; k < 3; ++k) { nd.lo[k] = f_rd(blo[k]); nd.hi[k] = f_ru(bhi[k]); }
     if (hi - lo == 1) {
-        nd.left = ~order[lo];
-        nd.right = -1;
+        nd.prim = order[lo];
     } else {
         int ax = 0;
         for (int k = 1; k < 3; ++k)
@@ -737,17 +744,18 @@ int bvh_build_range(const std::vector<PrimBox>& pb, std::vector<int>& order, int
             if (pb[a].c[ax] != pb[b].c[ax]) return pb[a].c[ax] < pb[b].c[ax];
             return a < b;
         });
-        nd.left = bvh_build_range(pb, order, lo, mid, out);
-        nd.right = bvh_build_range(pb, order, mid, hi, out);
+        nd.prim = -1;
+        bvh_build_range(pb, order, lo, mid, base, out);
+        bvh_build_range(pb, order, mid, hi, base, out);
     }
+    nd.skip = (int)out.size() - base;
     out[me] = nd;
-    return me;
 }
 
-// BVH over primitives given as vertex-index tuples (arity 2 or 3) into `xr` (rest coords, xyz);
-// returns the root (INT_MIN if there are no primitives), nodes appended to `out`.
-int bvh_build(const double* xr, const int* prim, int arity, int n, std::vector<RNode>& out) {
-    if (n <= 0) return INT_MIN;
+// BVH over primitives given as vertex-index tuples (arity 2 or 3) into `xr` (rest coords, xyz),
+// appended to `out` (2n - 1 nodes; none if n = 0).
+void bvh_build(const double* xr, const int* prim, int arity, int n, std::vector<BNode>& out) {
+    if (n <= 0) return;
     std::vector<PrimBox> pb(n);
     for (int i = 0; i < n; ++i) {
         PrimBox& b = pb[i];
@@ -763,7 +771,7 @@ int bvh_build(const double* xr, const int* prim, int arity, int n, std::vector<R
     }
     std::vector<int> order(n);
     for (int i = 0; i < n; ++i) order[i] = i;
-    return bvh_build_range(pb, order, 0, n, out);
+    bvh_build_range(pb, order, 0, n, (int)out.size(), out);
 }
 
 uint64_t fnv1a(uint64_t h, const void* p, size_t n) {
@@ -880,8 +888,8 @@ int abd_create(const AbdSceneDesc* d, const AbdParams* p, AbdCtx** out) {
     }
     for (int b = 0; b < nb; ++b) { bv0[b + 1] += bv0[b]; be0[b + 1] += be0[b]; bt0[b + 1] += bt0[b]; }
     // distinct local meshes (same rest coordinates and local connectivity) share one BVH pair
-    std::vector<int> bmesh(nb), mesh_rep, mtr, mer;
-    std::vector<RNode> tnodes, enodes;
+    std::vector<int> bmesh(nb), mesh_rep, mtn0(1, 0), men0(1, 0);
+    std::vector<BNode> tnodes, enodes;
     {
         auto local = [&](int b, int arity) {
             const int* src = arity == 3 ? d->tris : d->edges;
@@ -914,8 +922,10 @@ int abd_create(const AbdSceneDesc* d, const AbdParams* p, AbdCtx** out) {
                 found = (int)mesh_rep.size();
                 mesh_rep.push_back(b);
                 by_hash[h].push_back(found);
-                mtr.push_back(bvh_build(xr, lt.data(), 3, cnts[2], tnodes));
-                mer.push_back(bvh_build(xr, le.data(), 2, cnts[1], enodes));
+                bvh_build(xr, lt.data(), 3, cnts[2], tnodes);
+                bvh_build(xr, le.data(), 2, cnts[1], enodes);
+                mtn0.push_back((int)tnodes.size());
+                men0.push_back((int)enodes.size());
             }
             bmesh[b] = found;
         }
@@ -974,14 +984,32 @@ int abd_create(const AbdSceneDesc* d, const AbdParams* p, AbdCtx** out) {
     A(body_e0, nb + 1); U(body_e0, be0.data(), nb + 1);
     A(body_t0, nb + 1); U(body_t0, bt0.data(), nb + 1);
     A(body_mesh, nb); U(body_mesh, bmesh.data(), nb);
-    A(mesh_tri_root, ctx->n_meshes); U(mesh_tri_root, mtr.data(), ctx->n_meshes);
-    A(mesh_edge_root, ctx->n_meshes); U(mesh_edge_root, mer.data(), ctx->n_meshes);
+    A(mesh_tn0, ctx->n_meshes + 1); U(mesh_tn0, mtn0.data(), ctx->n_meshes + 1);
+    A(mesh_en0, ctx->n_meshes + 1); U(mesh_en0, men0.data(), ctx->n_meshes + 1);
     A(tri_nodes, std::max<size_t>(tnodes.size(), 1)); if (!tnodes.empty()) { U(tri_nodes, tnodes.data(), tnodes.size()); }
     A(edge_nodes, std::max<size_t>(enodes.size(), 1)); if (!enodes.empty()) { U(edge_nodes, enodes.data(), enodes.size()); }
+    {
+        // world-space copies of every body's node boxes, refreshed by each detection
+        std::vector<long long> tw(nb + 1, 0), ew(nb + 1, 0);
+        for (int b = 0; b < nb; ++b) {
+            const int m = bmesh[b];
+            tw[b + 1] = tw[b] + (mtn0[m + 1] - mtn0[m]);
+            ew[b + 1] = ew[b] + (men0[m + 1] - men0[m]);
+        }
+        if (tw[nb] > INT_MAX || ew[nb] > INT_MAX) {
+            ctx->err = "more than 2^31 BVH nodes over all bodies";
+            return fail(ABD_ERR_ARG);
+        }
+        std::vector<int> tw32(tw.begin(), tw.end()), ew32(ew.begin(), ew.end());
+        A(body_tw0, nb + 1); U(body_tw0, tw32.data(), nb + 1);
+        A(body_ew0, nb + 1); U(body_ew0, ew32.data(), nb + 1);
+        A(tri_world, std::max<long long>(tw[nb], 1));
+        A(edge_world, std::max<long long>(ew[nb], 1));
+    }
     A(pair_off, ns + 1); U(pair_off, poff.data(), ns + 1);
     A(body_box, 6 * (size_t)nb);
-    ctx->M = BodyMesh{ctx->body_v0, ctx->body_e0, ctx->body_t0, ctx->body_mesh, ctx->mesh_tri_root, ctx->mesh_edge_root,
-                      ctx->tri_nodes, ctx->edge_nodes, (int)tnodes.size(), (int)enodes.size()};
+    ctx->M = BodyMesh{ctx->body_v0, ctx->body_e0, ctx->body_t0, ctx->body_mesh, ctx->mesh_tn0, ctx->mesh_en0,
+                      ctx->body_tw0, ctx->body_ew0, ctx->tri_nodes, ctx->edge_nodes, ctx->tri_world, ctx->edge_world};
     A(cand_lo, ns); A(cand_hi, ns);
     A(r, 12 * nb); A(z, 12 * nb); A(p0, 12 * nb); A(p1, 12 * nb); A(Ap, 12 * nb);
     A(part, 6 * ctx->ntiles); A(bnorm, ns); A(cg_iters, ns); A(cg_done, ns);

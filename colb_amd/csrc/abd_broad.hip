@@ -7,11 +7,13 @@
 //   1. world boxes of the bodies over X0 and X1, inflated by dhat/2 (k_body_boxes);
 //   2. the body pairs of each scene whose boxes overlap and which are not both pinned
 //      (k_body_pairs);
-//   3. one workgroup per body pair: every vertex of either body against the other body's
-//      triangle BVH, every edge of the lower body against the upper body's edge BVH
-//      (k_pair_queries).  A rest-frame node box maps to world space through the affine state
-//      x = c + A xr at both ends of the path (positions are linear in the path parameter, so the
-//      swept image of the box lies in the union of its two world AABBs).
+//   3. world-space boxes of every body's BVH nodes: the rest-frame node box mapped through the
+//      affine state x = c + A xr at both ends of the path (positions are linear in the path
+//      parameter, so the swept image of the box lies in the union of its two world AABBs)
+//      (k_world_nodes);
+//   4. one workgroup per body pair: every vertex of either body against the other body's
+//      triangle BVH, every edge of the lower body against the upper body's edge BVH, by a
+//      stackless preorder traversal (k_pair_queries).
 // Leaves apply the exact candidate rule in FP64, the same as the oracle's broad phase:
 //   * the swept primitive boxes overlap (point box inflated by dhat, edge boxes by dhat/2 each,
 //     triangle boxes not inflated), and
@@ -125,7 +127,7 @@ void launch_body_pairs(hipStream_t s, const Geo& G, const long long* pair_off, l
 }
 
 // ---------------------------------------------------------------------------------------------
-// 3. per body pair traversal of the rest-frame BVHs
+// 3. world-space node boxes and 4. per body pair traversal
 // ---------------------------------------------------------------------------------------------
 // affine state of a body: x = c + A xr with A_ik = a_k[i] (q12 = [c, a0, a1, a2])
 struct Aff {
@@ -172,40 +174,74 @@ __device__ __forceinline__ void map_box(const Aff& f, const float* rlo, const fl
     }
 }
 
-#define RB_STACK 64
+__device__ __forceinline__ bool node_ov(const float* q, const BNode& n) {
+    return !(q[0] > n.hi[0] || n.lo[0] > q[3] || q[1] > n.hi[1] || n.lo[1] > q[4] || q[2] > n.hi[2] ||
+             n.lo[2] > q[5]);
+}
 
-// Traverse a rest-frame BVH from `root` against the world box qb; f(local primitive) at leaves.
-// Returns false on a malformed tree (index out of range or stack overflow).
-template <class F>
-__device__ __forceinline__ bool rest_query(const RNode* __restrict__ nodes, int nnodes, int root, const Aff& s0,
-                                           const Aff& s1, bool swept, const float* qb, F&& f) {
-    if (root == INT_MIN) return true;
-    int stack[RB_STACK];
-    int sp = 0;
-    stack[sp++] = root;
-    while (sp > 0) {
-        const int ni = stack[--sp];
-        if (ni < 0 || ni >= nnodes) return false;
-        const RNode nd = nodes[ni];
+// World-space node boxes of every body over the path q -> q - du (du == nullptr: at q).  One
+// 256-thread workgroup per (body, triangle / edge tree).
+__global__ __launch_bounds__(256) void k_world_nodes(Geo G, BodyMesh M, const double* __restrict__ q,
+                                                      const double* __restrict__ du) {
+    const int b = blockIdx.x;
+    const bool tri = blockIdx.y == 0;
+    const int m = M.body_mesh[b];
+    const int r0 = tri ? M.mesh_tn0[m] : M.mesh_en0[m];
+    const int n = (tri ? M.mesh_tn0[m + 1] : M.mesh_en0[m + 1]) - r0;
+    const int w0 = tri ? M.body_tw0[b] : M.body_ew0[b];
+    if ((tri ? M.body_tw0[b + 1] : M.body_ew0[b + 1]) - w0 != n) {
+        if (threadIdx.x == 0) atomicOr(G.err, ERR_BVH);
+        return;
+    }
+    const BNode* __restrict__ R = (tri ? M.tri_nodes : M.edge_nodes) + r0;
+    BNode* __restrict__ W = (tri ? M.tri_world : M.edge_world) + w0;
+    const Aff s0 = load_aff(q, nullptr, b);
+    const Aff s1 = du ? load_aff(q, du, b) : s0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const BNode r = R[i];
         float w[6];
-        map_box(s0, nd.lo, nd.hi, w);
-        if (swept) {
+        map_box(s0, r.lo, r.hi, w);
+        if (du) {
             float w1[6];
-            map_box(s1, nd.lo, nd.hi, w1);
+            map_box(s1, r.lo, r.hi, w1);
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
                 w[k] = fminf(w[k], w1[k]);
                 w[3 + k] = fmaxf(w[3 + k], w1[3 + k]);
             }
         }
-        if (!fbox_ov(qb, w)) continue;
-        if (nd.left < 0) {
-            f(~nd.left);
-        } else {
-            if (sp + 2 > RB_STACK) return false;
-            stack[sp++] = nd.right;
-            stack[sp++] = nd.left;
+        BNode o;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) { o.lo[k] = w[k]; o.hi[k] = w[3 + k]; }
+        o.prim = r.prim;
+        o.skip = r.skip;
+        W[i] = o;
+    }
+}
+
+void launch_world_nodes(hipStream_t s, const Geo& G, const BodyMesh& M, const double* q, const double* du) {
+    hipLaunchKernelGGL(k_world_nodes, dim3(G.nb, 2), dim3(256), 0, s, G, M, q, du);
+}
+
+// Stackless traversal of one body's world-space BVH (n nodes, nprim primitives) against the box
+// qb; f(local primitive) at overlapping leaves.  The index strictly increases at every step, so the
+// loop ends within n steps; returns false on a malformed node.
+template <class F>
+__device__ __forceinline__ bool world_query(const BNode* __restrict__ W, int n, int nprim, const float* qb, F&& f) {
+    int i = 0;
+    while (i < n) {
+        const BNode nd = W[i];
+        int next = nd.skip;
+        if (node_ov(qb, nd)) {
+            if (nd.prim >= 0) {
+                if (nd.prim >= nprim) return false;
+                f(nd.prim);
+            } else {
+                next = i + 1;
+            }
         }
+        if (next <= i || next > n) return false;
+        i = next;
     }
     return true;
 }
@@ -260,7 +296,6 @@ __device__ __forceinline__ void emit_key(const QueryCtx& Q, uint64_t key) {
 
 // One 256-thread workgroup per body pair (A < B, same scene).
 __global__ __launch_bounds__(256) void k_pair_queries(QueryCtx Q, const int2* __restrict__ pairs,
-                                                       const double* __restrict__ q, const double* __restrict__ du,
                                                        const float* __restrict__ bbox) {
     const int2 pr = pairs[blockIdx.x];
     const Geo& G = Q.G;
@@ -270,9 +305,6 @@ __global__ __launch_bounds__(256) void k_pair_queries(QueryCtx Q, const int2* __
         if (threadIdx.x == 0) atomicOr(G.err, ERR_PAIR);
         return;
     }
-    const bool swept = du != nullptr;
-    const Aff a0 = load_aff(q, nullptr, A), b0 = load_aff(q, nullptr, B);
-    const Aff a1 = swept ? load_aff(q, du, A) : a0, b1 = swept ? load_aff(q, du, B) : b0;
     const int s = G.bscene[A];
     const int vA = M.body_v0[A], vB = M.body_v0[B];
     const int nvA = M.body_v0[A + 1] - vA, nvB = M.body_v0[B + 1] - vB;
@@ -295,12 +327,10 @@ __global__ __launch_bounds__(256) void k_pair_queries(QueryCtx Q, const int2* __
             float qb[6];
             dbox_to_f(vb, 0.0, qb);
             if (!fbox_ov(qb, bbox + 6 * O)) continue;
-            const int mesh = M.body_mesh[O];
             const int t0 = M.body_t0[O], t1 = M.body_t0[O + 1];
-            bool ok = rest_query(M.tri_nodes, M.n_tri_nodes, M.mesh_tri_root[mesh], fromA ? b0 : a0,
-                                 fromA ? b1 : a1, swept, qb, [&](int lt) {
+            const int w0 = M.body_tw0[O];
+            bool ok = world_query(M.tri_world + w0, M.body_tw0[O + 1] - w0, t1 - t0, qb, [&](int lt) {
                 const int t = t0 + lt;
-                if (t >= t1) { atomicOr(G.err, ERR_BVH); return; }
                 const int4 tr = G.tris[t];
                 const double4 p0 = Q.X0[tr.x], p1 = Q.X0[tr.y], p2 = Q.X0[tr.z];
                 const double4 r0 = Q.X1[tr.x], r1 = Q.X1[tr.y], r2 = Q.X1[tr.z];
@@ -328,11 +358,10 @@ __global__ __launch_bounds__(256) void k_pair_queries(QueryCtx Q, const int2* __
             if (!fbox_ov(qb, bbox + 6 * B)) continue;
             // the other edge's box is inflated by h as well; node boxes are not
             dbox_to_f(eb, h, qb);
-            const int mesh = M.body_mesh[B];
             const int f0 = M.body_e0[B], f1 = M.body_e0[B + 1];
-            bool ok = rest_query(M.edge_nodes, M.n_edge_nodes, M.mesh_edge_root[mesh], b0, b1, swept, qb, [&](int lf) {
+            const int w0 = M.body_ew0[B];
+            bool ok = world_query(M.edge_world + w0, M.body_ew0[B + 1] - w0, f1 - f0, qb, [&](int lf) {
                 const int f = f0 + lf;
-                if (f >= f1) { atomicOr(G.err, ERR_BVH); return; }
                 const int2 fv = G.edges[f];
                 const double4 g0 = Q.X0[fv.x], g1 = Q.X0[fv.y], k0 = Q.X1[fv.x], k1 = Q.X1[fv.y];
                 DBox fb;
@@ -350,11 +379,11 @@ __global__ __launch_bounds__(256) void k_pair_queries(QueryCtx Q, const int2* __
 }
 
 void launch_pair_queries(hipStream_t s, const Geo& G, const BodyMesh& M, int npairs, const int2* pairs,
-                         const double4* X0, const double4* X1, const double* q, const double* du, const float* bbox,
-                         double infl, uint64_t* cand, int cap, int* count, int* overflow) {
+                         const double4* X0, const double4* X1, const float* bbox, double infl, uint64_t* cand,
+                         int cap, int* count, int* overflow) {
     if (npairs <= 0) return;
     QueryCtx Q{G, M, X0, X1, infl, cand, cap, count, overflow};
-    hipLaunchKernelGGL(k_pair_queries, dim3(npairs), dim3(256), 0, s, Q, pairs, q, du, bbox);
+    hipLaunchKernelGGL(k_pair_queries, dim3(npairs), dim3(256), 0, s, Q, pairs, bbox);
 }
 
 // point-ground candidates (one thread per vertex): min(d(X0), d(X1)) < dhat

@@ -284,9 +284,10 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-// Cyclic Jacobi PSD projection of the symmetric m x m matrix A (LDS, row-major), by one wave:
-// A <- V max(Lambda, 0) V^T.  Same rotation order and stopping rule as the oracle's psd_project
-// (Golub & Van Loan 8.5).  V is LDS scratch of m*m doubles.
+// Jacobi PSD projection of the symmetric m x m matrix A (LDS, row-major, m <= 12), by one wave:
+// A <- V max(Lambda, 0) V^T (Golub & Van Loan 8.5, parallel ordering 8.5.3).  The projection is a
+// unique matrix function, so it agrees with the oracle's cyclic-order psd_project to rounding;
+// same stopping rule.  V is LDS scratch of m*m doubles.
 __device__ inline void wave_psd_project(double* A, double* V, int m, int lane) {
     const int mm = m * m;
     for (int e = lane; e < mm; e += 64) V[e] = (e / m == e % m) ? 1.0 : 0.0;
@@ -295,6 +296,11 @@ __device__ inline void wave_psd_project(double* A, double* V, int m, int lane) {
     double fro = wave_sum(f);
     __syncthreads();
     if (fro == 0.0) return;
+    // Parallel (round-robin) Jacobi: a sweep is npl - 1 rounds of npl / 2 disjoint rotations
+    // (circle method over npl = m rounded up to even players; a pair with the extra player is
+    // skipped).  Lane task t = (pair k, index i): the k-th pair's column rotation of row i and row
+    // rotation of column i.
+    const int npl = m + (m & 1), half = npl / 2, ntask = half * m;
     for (int sweep = 0; sweep < 32; ++sweep) {
         double o = 0;
         for (int e = lane; e < mm; e += 64) {
@@ -303,35 +309,53 @@ __device__ inline void wave_psd_project(double* A, double* V, int m, int lane) {
         }
         double off = wave_sum(o);
         if (off <= 1e-32 * fro) break;
-        for (int p = 0; p < m - 1; ++p)
-            for (int q = p + 1; q < m; ++q) {
-                double apq = A[p * m + q];
-                if (apq == 0.0) continue;
-                double app = A[p * m + p], aqq = A[q * m + q];
-                double theta = (aqq - app) / (2.0 * apq);
-                double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
-                double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
-                __syncthreads();
-                if (lane < m) {
-                    int k = lane;
-                    double akp = A[k * m + p], akq = A[k * m + q];
-                    A[k * m + p] = c * akp - s * akq;
-                    A[k * m + q] = s * akp + c * akq;
-                } else if (lane < 2 * m) {
-                    int k = lane - m;
-                    double vkp = V[k * m + p], vkq = V[k * m + q];
-                    V[k * m + p] = c * vkp - s * vkq;
-                    V[k * m + q] = s * vkp + c * vkq;
+        for (int r = 0; r < npl - 1; ++r) {
+            int tp[2], tq[2], ti[2];
+            double tc[2], ts[2];
+            for (int u = 0; u < 2; ++u) {
+                const int t = lane + 64 * u;
+                tp[u] = -1;
+                if (t >= ntask) continue;
+                const int k = t / m, i = t % m;
+                const int a = (k == 0) ? 0 : 1 + (k - 1 + r) % (npl - 1);
+                const int b = 1 + (npl - 2 - k + r) % (npl - 1);
+                const int p = a < b ? a : b, q = a < b ? b : a;
+                if (q >= m) continue;
+                const double apq = A[p * m + q];
+                double c = 1.0, sn = 0.0;
+                if (apq != 0.0) {
+                    const double theta = (A[q * m + q] - A[p * m + p]) / (2.0 * apq);
+                    const double tt = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+                    c = 1.0 / sqrt(tt * tt + 1.0);
+                    sn = tt * c;
                 }
-                __syncthreads();
-                if (lane < m) {
-                    int k = lane;
-                    double apk = A[p * m + k], aqk = A[q * m + k];
-                    A[p * m + k] = c * apk - s * aqk;
-                    A[q * m + k] = s * apk + c * aqk;
-                }
-                __syncthreads();
+                tp[u] = p; tq[u] = q; ti[u] = i; tc[u] = c; ts[u] = sn;
             }
+            __syncthreads();
+            // columns: A <- A J, V <- V J
+            for (int u = 0; u < 2; ++u) {
+                if (tp[u] < 0) continue;
+                const int p = tp[u], q = tq[u], i = ti[u];
+                const double c = tc[u], sn = ts[u];
+                const double aip = A[i * m + p], aiq = A[i * m + q];
+                A[i * m + p] = c * aip - sn * aiq;
+                A[i * m + q] = sn * aip + c * aiq;
+                const double vip = V[i * m + p], viq = V[i * m + q];
+                V[i * m + p] = c * vip - sn * viq;
+                V[i * m + q] = sn * vip + c * viq;
+            }
+            __syncthreads();
+            // rows: A <- J^T A
+            for (int u = 0; u < 2; ++u) {
+                if (tp[u] < 0) continue;
+                const int p = tp[u], q = tq[u], j = ti[u];
+                const double c = tc[u], sn = ts[u];
+                const double apj = A[p * m + j], aqj = A[q * m + j];
+                A[p * m + j] = c * apj - sn * aqj;
+                A[q * m + j] = sn * apj + c * aqj;
+            }
+            __syncthreads();
+        }
     }
     // reconstruct with clamped eigenvalues (diagonal of A)
     double out[3];

@@ -23,27 +23,32 @@ __host__ __device__ inline int key_kind(uint64_t k) { return (int)((k >> 46) & 3
 __host__ __device__ inline int key_il(uint64_t k) { return (int)((k >> 23) & KEY_IMASK); }
 __host__ __device__ inline int key_jl(uint64_t k) { return (int)(k & KEY_IMASK); }
 
-// Rest-frame BVH node of one mesh (built once on the host): box in the mesh's rest frame,
-// rounded outward to float; left >= 0 and right are children, left < 0 marks a leaf holding the
-// mesh-local primitive ~left.
-struct RNode {
+// BVH node of one mesh, in preorder: an internal node's first child is the next node and `skip`
+// is the mesh-local index one past its subtree, so a traversal needs no stack (continue at i + 1
+// into a node, at skip past it).  prim >= 0 marks a leaf holding that mesh-local primitive.
+// Rest-frame nodes are built once on the host; world-space copies of every body's nodes (the box
+// of the rest box mapped through the body's affine state) are rebuilt by each detection.
+struct BNode {
     float lo[3], hi[3];
-    int left, right;
+    int prim, skip;
 };
-static_assert(sizeof(RNode) == 32, "RNode is 32 bytes");
+static_assert(sizeof(BNode) == 32, "BNode is 32 bytes");
 
-// Per-body primitive ranges and the rest-frame BVHs of the distinct meshes.  Vertices, edges and
-// triangles are grouped by body; bodies with identical local meshes share one mesh entry.
+// Per-body primitive ranges and BVHs.  Vertices, edges and triangles are grouped by body; bodies
+// with identical local meshes share one mesh entry (one set of rest-frame nodes).
 struct BodyMesh {
-    const int* body_v0;  // [nb+1]
-    const int* body_e0;  // [nb+1]
-    const int* body_t0;  // [nb+1]
+    const int* body_v0;   // [nb+1]
+    const int* body_e0;   // [nb+1]
+    const int* body_t0;   // [nb+1]
     const int* body_mesh;
-    const int* mesh_tri_root;   // INT_MIN: no triangles
-    const int* mesh_edge_root;  // INT_MIN: no edges
-    const RNode* tri_nodes;
-    const RNode* edge_nodes;
-    int n_tri_nodes, n_edge_nodes;
+    const int* mesh_tn0;  // [n_meshes+1] rest triangle-BVH node ranges
+    const int* mesh_en0;  // [n_meshes+1] rest edge-BVH node ranges
+    const int* body_tw0;  // [nb+1] world triangle-BVH node ranges
+    const int* body_ew0;  // [nb+1] world edge-BVH node ranges
+    const BNode* tri_nodes;
+    const BNode* edge_nodes;
+    BNode* tri_world;
+    BNode* edge_world;
 };
 
 // Static scene tables on the device (read-only during a step).
@@ -98,11 +103,12 @@ void launch_xform(hipStream_t s, const Geo& G, const double* q, const double* du
 // abd_broad.hip
 void launch_body_boxes(hipStream_t s, const Geo& G, const BodyMesh& M, const double4* X0, const double4* X1,
                        double half_infl, float* box);
+void launch_world_nodes(hipStream_t s, const Geo& G, const BodyMesh& M, const double* q, const double* du);
 void launch_body_pairs(hipStream_t s, const Geo& G, const long long* pair_off, long long total, const float* box,
                        int2* out, int cap, int* count, int* overflow);
 void launch_pair_queries(hipStream_t s, const Geo& G, const BodyMesh& M, int npairs, const int2* pairs,
-                         const double4* X0, const double4* X1, const double* q, const double* du, const float* bbox,
-                         double infl, uint64_t* cand, int cap, int* count, int* overflow);
+                         const double4* X0, const double4* X1, const float* bbox, double infl, uint64_t* cand,
+                         int cap, int* count, int* overflow);
 void launch_ground_queries(hipStream_t s, const Geo& G, const double4* X0, const double4* X1, const AbdParams& P,
                            double infl, uint64_t* cand, int cap, int* count, int* overflow);
 // abd_kernels.hip

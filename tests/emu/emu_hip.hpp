@@ -190,7 +190,7 @@ void emu_launch(const char* name, K kernel, dim3 grid, dim3 block, Args... args)
         S.fib[t].stack.swap(pool[t]);
     }
     for (unsigned b = 0; b < nblk; ++b) {
-        blockIdx = dim3(b % grid.x, 1, 1);
+        blockIdx = dim3(b % grid.x, (b / grid.x) % grid.y, b / (grid.x * grid.y));
         for (unsigned t = 0; t < nt; ++t) {
             EmuFiber& f = S.fib[t];
             f.done = false;
@@ -218,7 +218,7 @@ void emu_launch(const char* name, K kernel, dim3 grid, dim3 block, Args... args)
                 EmuFiber& f = S.fib[t];
                 if (f.done) continue;
                 S.cur = t;
-                threadIdx = dim3(t % block.x, 1, 1);
+                threadIdx = dim3(t % block.x, (t / block.x) % block.y, t / (block.x * block.y));
                 void* fake = nullptr;
                 EMU_START_SWITCH(&fake, f.stack.data(), f.stack.size());
                 swapcontext(&S.main, &f.ctx);
