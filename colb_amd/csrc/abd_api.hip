@@ -84,11 +84,12 @@ struct AbdCtx {
     int *mesh_tn0 = nullptr, *mesh_en0 = nullptr, *body_tw0 = nullptr, *body_ew0 = nullptr;
     BNode *tri_nodes = nullptr, *edge_nodes = nullptr, *tri_world = nullptr, *edge_world = nullptr;
     int n_meshes = 0;
-    long long *pair_off = nullptr;
-    long long total_body_pairs = 0;
+    int *bp_cnt = nullptr, *bp_off = nullptr;  // [nb+1] per-body pair counts and offsets
     float* body_box = nullptr;
     int2* bpairs = nullptr;
     int bpair_cap = 0, n_bpairs = 0;
+    int max_queries = 0;     // largest per-body-pair query count (2 max verts + max edges)
+
     // candidates
     int cand_cap = 0, n_cand = 0;
     uint64_t *cand = nullptr, *cand_s = nullptr, *act = nullptr;
@@ -385,24 +386,20 @@ int broad_phase(AbdCtx* ctx, const double4* X0, const double4* X1, const double*
     tic(ctx, "broad_phase_pairs");
     launch_body_boxes(ctx->st, ctx->G, ctx->M, X0, X1, 0.5 * infl, ctx->body_box);
     CHK(dbg(ctx, "launch_body_boxes"));
-    for (int attempt = 0; attempt < 8; ++attempt) {
-        HIPCHK(hipMemsetAsync(ctx->cnt + C_COUNT, 0, 2 * sizeof(int), ctx->st));
-        launch_body_pairs(ctx->st, ctx->G, ctx->pair_off, ctx->total_body_pairs, ctx->body_box, ctx->bpairs,
-                          ctx->bpair_cap, ctx->cnt + C_COUNT, ctx->cnt + C_OVERFLOW);
-        CHK(dbg(ctx, "launch_body_pairs"));
-        HIPCHK(hipGetLastError());
-        CHK(sync_counters(ctx));
-        int n = ctx->h_cnt[C_COUNT];
-        if (n <= ctx->bpair_cap) {
-            ctx->n_bpairs = n;
-            break;
-        }
-        if (attempt == 7) {
-            ctx->err = "body-pair list did not fit";
-            return ABD_ERR_CAPACITY;
-        }
-        CHK(grow_body_pairs(ctx, n + n / 2));
+    launch_body_pair_count(ctx->st, ctx->G, ctx->body_box, ctx->bp_cnt);
+    CHK(dbg(ctx, "launch_body_pair_count"));
+    CHK(excl_scan(ctx, ctx->bp_cnt, ctx->bp_off, ctx->nb + 1));
+    HIPCHK(hipMemcpyAsync(ctx->cnt + C_COUNT, ctx->bp_off + ctx->nb, sizeof(int), hipMemcpyDeviceToDevice, ctx->st));
+    CHK(sync_counters(ctx));
+    ctx->n_bpairs = ctx->h_cnt[C_COUNT];
+    if (ctx->n_bpairs < 0) {
+        ctx->err = "body-pair count overflowed";
+        return ABD_ERR_CAPACITY;
     }
+    CHK(grow_body_pairs(ctx, ctx->n_bpairs));
+    launch_body_pair_write(ctx->st, ctx->G, ctx->body_box, ctx->bp_off, ctx->bpairs, ctx->bpair_cap);
+    CHK(dbg(ctx, "launch_body_pair_write"));
+    HIPCHK(hipGetLastError());
     toc(ctx);
     tic(ctx, "broad_phase_nodes");
     launch_world_nodes(ctx->st, ctx->G, ctx->M, ctx->q, du);
@@ -411,8 +408,8 @@ int broad_phase(AbdCtx* ctx, const double4* X0, const double4* X1, const double*
     for (int attempt = 0; attempt < 8; ++attempt) {
         tic(ctx, "broad_phase_query");
         HIPCHK(hipMemsetAsync(ctx->cnt + C_COUNT, 0, 2 * sizeof(int), ctx->st));
-        launch_pair_queries(ctx->st, ctx->G, ctx->M, ctx->n_bpairs, ctx->bpairs, X0, X1, ctx->body_box, infl,
-                            ctx->cand, ctx->cand_cap, ctx->cnt + C_COUNT, ctx->cnt + C_OVERFLOW);
+        launch_pair_queries(ctx->st, ctx->G, ctx->M, ctx->n_bpairs, ctx->max_queries, ctx->bpairs, X0, X1,
+                            ctx->body_box, infl, ctx->cand, ctx->cand_cap, ctx->cnt + C_COUNT, ctx->cnt + C_OVERFLOW);
         CHK(dbg(ctx, "launch_pair_queries"));
         launch_ground_queries(ctx->st, ctx->G, X0, X1, ctx->P, infl, ctx->cand, ctx->cand_cap, ctx->cnt + C_COUNT,
                               ctx->cnt + C_OVERFLOW);
@@ -421,10 +418,13 @@ int broad_phase(AbdCtx* ctx, const double4* X0, const double4* X1, const double*
         toc(ctx);
         CHK(sync_counters(ctx));
         if (ctx->h_cnt[C_OVERFLOW] == 2) {
-            ctx->err = "rest-frame BVH traversal failed (malformed tree or stack overflow)";
+            ctx->err = "BVH traversal failed (malformed tree)";
             return ABD_ERR_CAPACITY;
         }
-        int n = ctx->h_cnt[C_COUNT];
+        const int n = ctx->h_cnt[C_COUNT];
+        if (ctx->P.reserved0 & 4)
+            std::fprintf(stderr, "[abd] broad phase (%s): %d body pairs, %d candidates\n", du ? "swept" : "static",
+                         ctx->n_bpairs, n);
         if (n <= ctx->cand_cap) {
             ctx->n_cand = n;
             break;
@@ -887,6 +887,19 @@ int abd_create(const AbdSceneDesc* d, const AbdParams* p, AbdCtx** out) {
         bt0[b + 1]++;
     }
     for (int b = 0; b < nb; ++b) { bv0[b + 1] += bv0[b]; be0[b + 1] += be0[b]; bt0[b + 1] += bt0[b]; }
+    {
+        long long mv = 0, me = 0;
+        for (int b = 0; b < nb; ++b) {
+            mv = std::max<long long>(mv, bv0[b + 1] - bv0[b]);
+            me = std::max<long long>(me, be0[b + 1] - be0[b]);
+        }
+        const long long mq = 2 * mv + me;
+        if (mq > 64LL * 65535) {
+            ctx->err = "a body has too many vertices and edges for the broad-phase launch shape";
+            return fail(ABD_ERR_ARG);
+        }
+        ctx->max_queries = (int)mq;
+    }
     // distinct local meshes (same rest coordinates and local connectivity) share one BVH pair
     std::vector<int> bmesh(nb), mesh_rep, mtn0(1, 0), men0(1, 0);
     std::vector<BNode> tnodes, enodes;
@@ -931,12 +944,6 @@ int abd_create(const AbdSceneDesc* d, const AbdParams* p, AbdCtx** out) {
         }
     }
     ctx->n_meshes = (int)mesh_rep.size();
-    std::vector<long long> poff(ns + 1, 0);
-    for (int s = 0; s < ns; ++s) {
-        long long n = sb[s + 1] - sb[s];
-        poff[s + 1] = poff[s] + n * (n - 1) / 2;
-    }
-    ctx->total_body_pairs = poff[ns];
     // tiles of at most 5 bodies inside one scene
     std::vector<int> tb0, tb1, tsc, st0(ns + 1, 0);
     for (int s = 0; s < ns; ++s) {
@@ -1006,7 +1013,7 @@ int abd_create(const AbdSceneDesc* d, const AbdParams* p, AbdCtx** out) {
         A(tri_world, std::max<long long>(tw[nb], 1));
         A(edge_world, std::max<long long>(ew[nb], 1));
     }
-    A(pair_off, ns + 1); U(pair_off, poff.data(), ns + 1);
+    A(bp_cnt, nb + 1); A(bp_off, nb + 1);
     A(body_box, 6 * (size_t)nb);
     ctx->M = BodyMesh{ctx->body_v0, ctx->body_e0, ctx->body_t0, ctx->body_mesh, ctx->mesh_tn0, ctx->mesh_en0,
                       ctx->body_tw0, ctx->body_ew0, ctx->tri_nodes, ctx->edge_nodes, ctx->tri_world, ctx->edge_world};
@@ -1026,8 +1033,11 @@ int abd_create(const AbdSceneDesc* d, const AbdParams* p, AbdCtx** out) {
     if ((rc = grow_candidates(ctx, std::max(1024, std::min(p->max_candidates > 0 ? p->max_candidates : (1 << 20), 1 << 26)))) != ABD_OK) return fail(rc);
     if ((rc = grow_active(ctx, 1024)) != ABD_OK) return fail(rc);
     if ((rc = grow_pairs(ctx, 1024)) != ABD_OK) return fail(rc);
-    if ((rc = grow_body_pairs(ctx, (int)std::max<long long>(1024, std::min<long long>(ctx->total_body_pairs, 16LL * nb)))) != ABD_OK)
-        return fail(rc);
+    if ((rc = grow_body_pairs(ctx, std::max(1024, std::min(16 * nb, 1 << 24)))) != ABD_OK) return fail(rc);
+    if (hipMemset(ctx->bp_cnt, 0, (nb + 1) * sizeof(int)) != hipSuccess) {
+        ctx->err = "hipMemset failed";
+        return fail(ABD_ERR_HIP);
+    }
     ctx->G = Geo{nb, nv, ne, nt, ns, ctx->rest, ctx->edges, ctx->tris, ctx->mass4, ctx->pinned, ctx->bscene,
                  ctx->scene_b0, ctx->scene_v0, ctx->scene_e0, ctx->scene_t0, ctx->cnt + C_ERR};
     if (hipMemset(ctx->cnt, 0, C_N * sizeof(int)) != hipSuccess) {

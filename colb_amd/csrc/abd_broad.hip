@@ -5,16 +5,16 @@
 // their rest-frame meshes, so here every distinct mesh carries a rest-frame BVH built once on the
 // host (abd_api.hip), and a detection over the linear path X0 -> X1 is
 //   1. world boxes of the bodies over X0 and X1, inflated by dhat/2 (k_body_boxes);
-//   2. the body pairs of each scene whose boxes overlap and which are not both pinned
-//      (k_body_pairs);
+//   2. the body pairs of each scene whose boxes overlap and which are not both pinned, in (A, B)
+//      order (k_body_pair_count, k_body_pair_write);
 //   3. world-space boxes of every body's BVH nodes: the rest-frame node box mapped through the
 //      affine state x = c + A xr at both ends of the path (positions are linear in the path
 //      parameter, so the swept image of the box lies in the union of its two world AABBs)
 //      (k_world_nodes);
-//   4. one workgroup per body pair: every vertex of either body against the other body's
-//      triangle BVH, every edge of the lower body against the upper body's edge BVH, by a
-//      stackless preorder traversal (k_pair_queries).
-// Leaves apply the exact candidate rule in FP64, the same as the oracle's broad phase:
+//   4. per body pair, every vertex of either body against the other body's triangle BVH, every
+//      edge of the lower body against the upper body's edge BVH, by a stackless preorder
+//      traversal; leaf pairs are queued in LDS and checked 64 at a time (k_pair_queries).
+// The leaf check applies the exact candidate rule in FP64, the same as the oracle's broad phase:
 //   * the swept primitive boxes overlap (point box inflated by dhat, edge boxes by dhat/2 each,
 //     triangle boxes not inflated), and
 //   * the distance at X0 minus the additive-CCD motion bound is below dhat (may_reach).
@@ -80,50 +80,54 @@ void launch_body_boxes(hipStream_t s, const Geo& G, const BodyMesh& M, const dou
 }
 
 // ---------------------------------------------------------------------------------------------
-// 2. body pairs: thread t (grid-stride) takes the t-th pair of the scene-major numbering of the
-//    strict upper triangles; pair_off[s] = pairs of the scenes before s.
+// 2. body pairs (A, B > A) of a scene whose inflated boxes overlap and which are not both pinned:
+//    one wave per body A, lanes over the later bodies of its scene.  The count pass sizes each
+//    body's run, an exclusive scan places the runs, the write pass fills them in order, so the
+//    list is sorted by (A, B) whatever the scheduling.
 // ---------------------------------------------------------------------------------------------
-__global__ void k_body_pairs(Geo G, const long long* __restrict__ pair_off, long long total,
-                             const float* __restrict__ box, int2* __restrict__ out, int cap, int* count,
-                             int* overflow) {
-    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
-         t += (long long)gridDim.x * blockDim.x) {
-        int lo = 0, hi = G.ns;
-        while (hi - lo > 1) {
-            int mid = (lo + hi) >> 1;
-            if (pair_off[mid] <= t) lo = mid; else hi = mid;
-        }
-        const int s = lo;
-        const long long r = t - pair_off[s];
-        const long long n = G.scene_b0[s + 1] - G.scene_b0[s];
-        // row i holds the pairs (i, j > i); it starts at i (2n - i - 1) / 2
-        const double nn = (double)n - 0.5;
-        long long i = (long long)floor(nn - sqrt(fmax(nn * nn - 2.0 * (double)r, 0.0)));
-        if (i < 0) i = 0;
-        if (i > n - 2) i = n - 2;
-        while (i > 0 && i * (2 * n - i - 1) / 2 > r) --i;
-        while (i + 1 <= n - 2 && (i + 1) * (2 * n - i - 2) / 2 <= r) ++i;
-        const long long j = i + 1 + (r - i * (2 * n - i - 1) / 2);
-        if (i < 0 || j <= i || j >= n) {
-            atomicOr(G.err, ERR_PAIR);
-            continue;
-        }
-        const int A = G.scene_b0[s] + (int)i, B = G.scene_b0[s] + (int)j;
-        if (G.pinned[A] && G.pinned[B]) continue;
-        if (!fbox_ov(box + 6 * A, box + 6 * B)) continue;
-        const int slot = atomicAdd(count, 1);
-        if (slot < cap) out[slot] = make_int2(A, B);
-        else atomicMax(overflow, 1);
+__device__ __forceinline__ bool body_pair_ok(const Geo& G, const float* box, int A, int B) {
+    return !(G.pinned[A] && G.pinned[B]) && fbox_ov(box + 6 * A, box + 6 * B);
+}
+
+__global__ __launch_bounds__(64) void k_body_pair_count(Geo G, const float* __restrict__ box, int* __restrict__ cnt) {
+    const int A = blockIdx.x, lane = threadIdx.x;
+    const int end = G.scene_b0[G.bscene[A] + 1];
+    int c = 0;
+    for (int B = A + 1 + lane; B < end; B += 64) c += body_pair_ok(G, box, A, B) ? 1 : 0;
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+    if (lane == 0) cnt[A] = c;
+}
+
+__global__ __launch_bounds__(64) void k_body_pair_write(Geo G, const float* __restrict__ box, const int* __restrict__ off,
+                                                         int2* __restrict__ out, int cap) {
+    const int A = blockIdx.x, lane = threadIdx.x;
+    const int end = G.scene_b0[G.bscene[A] + 1];
+    int pos = off[A];
+    for (int B0 = A + 1; B0 < end; B0 += 64) {
+        const int B = B0 + lane;
+        const bool ok = B < end && body_pair_ok(G, box, A, B);
+#if defined(ABD_EMULATION)
+        // fibre emulation: lanes do not run in lockstep, so rank by a serial pass over the chunk
+        int rank = 0;
+        for (int k = B0; k < B && k < end; ++k) rank += body_pair_ok(G, box, A, k) ? 1 : 0;
+        int tot = 0;
+        for (int k = B0; k < end && k < B0 + 64; ++k) tot += body_pair_ok(G, box, A, k) ? 1 : 0;
+#else
+        const unsigned long long m = __ballot(ok);
+        const int rank = __popcll(m & ((1ull << lane) - 1ull));
+        const int tot = __popcll(m);
+#endif
+        if (ok && pos + rank < cap) out[pos + rank] = make_int2(A, B);
+        pos += tot;
     }
 }
 
-void launch_body_pairs(hipStream_t s, const Geo& G, const long long* pair_off, long long total, const float* box,
-                       int2* out, int cap, int* count, int* overflow) {
-    if (total <= 0) return;
-    long long blocks = (total + 255) / 256;
-    if (blocks > 8192) blocks = 8192;
-    hipLaunchKernelGGL(k_body_pairs, dim3((unsigned)blocks), dim3(256), 0, s, G, pair_off, total, box, out, cap,
-                       count, overflow);
+void launch_body_pair_count(hipStream_t s, const Geo& G, const float* box, int* cnt) {
+    hipLaunchKernelGGL(k_body_pair_count, dim3(G.nb), dim3(64), 0, s, G, box, cnt);
+}
+
+void launch_body_pair_write(hipStream_t s, const Geo& G, const float* box, const int* off, int2* out, int cap) {
+    hipLaunchKernelGGL(k_body_pair_write, dim3(G.nb), dim3(64), 0, s, G, box, off, out, cap);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -288,16 +292,85 @@ struct QueryCtx {
     int* overflow;
 };
 
-__device__ __forceinline__ void emit_key(const QueryCtx& Q, uint64_t key) {
+
+// Exact candidate rule on one leaf pair (global ids; PT: vertex i, triangle j; EE: edges i < j):
+// the FP64 swept boxes overlap (point box inflated by infl, edge boxes by infl/2 each, triangle
+// boxes not inflated; the oracle's broad_phase builds the same boxes with the same operations) and
+// may_reach holds.  Keys of pairs that pass are appended to the candidate list.
+__device__ __forceinline__ void leaf_rule(const QueryCtx& Q, int s, int kind, int i, int j) {
+    const Geo& G = Q.G;
+    int vid[4];
+    if (kind == K_PT) {
+        const int4 tr = G.tris[j];
+        vid[0] = i; vid[1] = tr.x; vid[2] = tr.y; vid[3] = tr.z;
+    } else {
+        const int2 e = G.edges[i], f = G.edges[j];
+        vid[0] = e.x; vid[1] = e.y; vid[2] = f.x; vid[3] = f.y;
+    }
+    D3 a[4], b[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) { a[k] = d3of(Q.X0[vid[k]]); b[k] = d3of(Q.X1[vid[k]]); }
+    const int split = (kind == K_PT) ? 1 : 2;
+    const double r0 = (kind == K_PT) ? Q.infl : 0.5 * Q.infl, r1 = (kind == K_PT) ? 0.0 : 0.5 * Q.infl;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        double lo0 = DBL_MAX, hi0 = -DBL_MAX, lo1 = DBL_MAX, hi1 = -DBL_MAX;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const double u = cmp(a[k], c), w = cmp(b[k], c);
+            if (k < split) {
+                lo0 = fmin(lo0, fmin(u, w)); hi0 = fmax(hi0, fmax(u, w));
+            } else {
+                lo1 = fmin(lo1, fmin(u, w)); hi1 = fmax(hi1, fmax(u, w));
+            }
+        }
+        lo0 -= r0; hi0 += r0; lo1 -= r1; hi1 += r1;
+        if (lo0 > hi1 || lo1 > hi0) return;
+    }
+    if (!may_reach(kind, a, b, Q.infl)) return;
+    const uint64_t key = (kind == K_PT) ? make_key(s, K_PT, i - G.scene_v0[s], j - G.scene_t0[s])
+                                        : make_key(s, K_EE, i - G.scene_e0[s], j - G.scene_e0[s]);
     const int slot = atomicAdd(Q.count, 1);
     if (slot < Q.cap) Q.cand[slot] = key;
     else atomicMax(Q.overflow, 1);
 }
 
-// One 256-thread workgroup per body pair (A < B, same scene).
-__global__ __launch_bounds__(256) void k_pair_queries(QueryCtx Q, const int2* __restrict__ pairs,
-                                                       const float* __restrict__ bbox) {
-    const int2 pr = pairs[blockIdx.x];
+// leaf-pair queue entry: kind (bit 63) | i (bits 32-62) | j (bits 0-31)
+__device__ __forceinline__ uint64_t leaf_pack(int kind, int i, int j) {
+    return ((uint64_t)kind << 63) | ((uint64_t)(unsigned)i << 32) | (uint64_t)(unsigned)j;
+}
+__device__ __forceinline__ void leaf_run(const QueryCtx& Q, int s, uint64_t ent) {
+    const int kind = opaque((int)(ent >> 63));
+    const int i = opaque((int)((ent >> 32) & 0x7fffffffu));
+    const int j = opaque((int)(ent & 0xffffffffu));
+    leaf_rule(Q, s, kind, i, j);
+}
+
+#if defined(ABD_EMULATION)
+#define WAVE_ANY(p) emu_any(p)
+#else
+#define WAVE_ANY(p) __any(p)
+#endif
+
+// Broad-phase queries: one wave per (body pair, chunk of 64 queries).  Query qi of the pair
+// (A < B, same scene) is a vertex of A (qi < nvA), a vertex of B, or an edge of A; it walks the
+// other body's world-space BVH (stackless, one node per lane per round) and pushes the leaves whose
+// node box overlaps its query box into an LDS queue.  Whenever 64 leaf pairs are queued the wave
+// evaluates them together, one per lane (leaf_rule), so the FP64 leaf work runs at full width and
+// only surviving candidates touch the global counter.
+__global__ __launch_bounds__(64) void k_pair_queries(QueryCtx Q, const int2* __restrict__ pairs, int npairs,
+                                                      int chunks, const float* __restrict__ bbox) {
+    __shared__ uint64_t queue[128];
+    __shared__ int qn;
+    // Workgroups are dealt to the 8 XCDs round-robin by linear index; XCD x takes the x-th
+    // contiguous range of (pair, chunk) work items, so consecutive items on one XCD share bodies
+    // (pairs are ordered by their lower body) and their BVH nodes stay in that XCD's L2.
+    const long long nitems = (long long)npairs * chunks;
+    const long long per = (nitems + 7) / 8;
+    const long long item = (long long)(blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    if (item >= nitems) return;
+    const int2 pr = pairs[item / chunks];
+    const int chunk = (int)(item % chunks);
     const Geo& G = Q.G;
     const BodyMesh& M = Q.M;
     const int A = pr.x, B = pr.y;
@@ -305,85 +378,101 @@ __global__ __launch_bounds__(256) void k_pair_queries(QueryCtx Q, const int2* __
         if (threadIdx.x == 0) atomicOr(G.err, ERR_PAIR);
         return;
     }
+    const int lane = threadIdx.x;
     const int s = G.bscene[A];
     const int vA = M.body_v0[A], vB = M.body_v0[B];
     const int nvA = M.body_v0[A + 1] - vA, nvB = M.body_v0[B + 1] - vB;
     const int neA = M.body_e0[A + 1] - M.body_e0[A];
     const int total = nvA + nvB + neA;
+    const int qi = chunk * 64 + lane;
     const double infl = Q.infl, h = 0.5 * Q.infl;
-    const int sv0 = G.scene_v0[s], se0 = G.scene_e0[s], st0 = G.scene_t0[s];
-    for (int qi = threadIdx.x; qi < total; qi += blockDim.x) {
-        if (qi < nvA + nvB) {
-            // vertex v of body P against the triangles of body O
-            const bool fromA = qi < nvA;
-            const int O = fromA ? B : A;
-            const int v = fromA ? vA + qi : vB + (qi - nvA);
-            const double4 x0 = Q.X0[v], x1 = Q.X1[v];
-            DBox vb;
-            dbox_init(vb);
-            dbox_add(vb, x0);
-            dbox_add(vb, x1);
-            dbox_inflate(vb, infl);
-            float qb[6];
-            dbox_to_f(vb, 0.0, qb);
-            if (!fbox_ov(qb, bbox + 6 * O)) continue;
-            const int t0 = M.body_t0[O], t1 = M.body_t0[O + 1];
-            const int w0 = M.body_tw0[O];
-            bool ok = world_query(M.tri_world + w0, M.body_tw0[O + 1] - w0, t1 - t0, qb, [&](int lt) {
-                const int t = t0 + lt;
-                const int4 tr = G.tris[t];
-                const double4 p0 = Q.X0[tr.x], p1 = Q.X0[tr.y], p2 = Q.X0[tr.z];
-                const double4 r0 = Q.X1[tr.x], r1 = Q.X1[tr.y], r2 = Q.X1[tr.z];
-                DBox tb;
-                dbox_init(tb);
-                dbox_add(tb, p0); dbox_add(tb, p1); dbox_add(tb, p2);
-                dbox_add(tb, r0); dbox_add(tb, r1); dbox_add(tb, r2);
-                if (!dbox_ov(vb, tb)) return;
-                D3 y0[4] = {d3of(x0), d3of(p0), d3of(p1), d3of(p2)};
-                D3 y1[4] = {d3of(x1), d3of(r0), d3of(r1), d3of(r2)};
-                if (may_reach(K_PT, y0, y1, infl)) emit_key(Q, make_key(s, K_PT, v - sv0, t - st0));
-            });
-            if (!ok) { atomicMax(Q.overflow, 2); atomicOr(G.err, ERR_BVH); }
-        } else {
-            // edge e of body A against the edges of body B
-            const int e = M.body_e0[A] + (qi - nvA - nvB);
-            const int2 ev = G.edges[e];
-            const double4 p0 = Q.X0[ev.x], p1 = Q.X0[ev.y], r0 = Q.X1[ev.x], r1 = Q.X1[ev.y];
-            DBox eb;
-            dbox_init(eb);
-            dbox_add(eb, p0); dbox_add(eb, p1); dbox_add(eb, r0); dbox_add(eb, r1);
-            dbox_inflate(eb, h);
-            float qb[6];
-            dbox_to_f(eb, 0.0, qb);
-            if (!fbox_ov(qb, bbox + 6 * B)) continue;
-            // the other edge's box is inflated by h as well; node boxes are not
-            dbox_to_f(eb, h, qb);
-            const int f0 = M.body_e0[B], f1 = M.body_e0[B + 1];
-            const int w0 = M.body_ew0[B];
-            bool ok = world_query(M.edge_world + w0, M.body_ew0[B + 1] - w0, f1 - f0, qb, [&](int lf) {
-                const int f = f0 + lf;
-                const int2 fv = G.edges[f];
-                const double4 g0 = Q.X0[fv.x], g1 = Q.X0[fv.y], k0 = Q.X1[fv.x], k1 = Q.X1[fv.y];
-                DBox fb;
-                dbox_init(fb);
-                dbox_add(fb, g0); dbox_add(fb, g1); dbox_add(fb, k0); dbox_add(fb, k1);
-                dbox_inflate(fb, h);
-                if (!dbox_ov(eb, fb)) return;
-                D3 y0[4] = {d3of(p0), d3of(p1), d3of(g0), d3of(g1)};
-                D3 y1[4] = {d3of(r0), d3of(r1), d3of(k0), d3of(k1)};
-                if (may_reach(K_EE, y0, y1, infl)) emit_key(Q, make_key(s, K_EE, e - se0, f - se0));
-            });
-            if (!ok) { atomicMax(Q.overflow, 2); atomicOr(G.err, ERR_BVH); }
+    // per-lane query: kind, primitive id, query box, the BVH to walk and its primitive base
+    int kind = -1, pid = 0, prim0 = 0, nprim = 0, n = 0;
+    const BNode* W = nullptr;
+    float qb[6];
+    if (qi < nvA + nvB) {
+        const bool fromA = qi < nvA;
+        const int O = fromA ? B : A;
+        const int v = fromA ? vA + qi : vB + (qi - nvA);
+        DBox vb;
+        dbox_init(vb);
+        dbox_add(vb, Q.X0[v]);
+        dbox_add(vb, Q.X1[v]);
+        dbox_to_f(vb, infl, qb);
+        if (fbox_ov(qb, bbox + 6 * O)) {
+            kind = K_PT; pid = v;
+            prim0 = M.body_t0[O]; nprim = M.body_t0[O + 1] - prim0;
+            W = M.tri_world + M.body_tw0[O]; n = M.body_tw0[O + 1] - M.body_tw0[O];
         }
+    } else if (qi < total) {
+        const int e = M.body_e0[A] + (qi - nvA - nvB);
+        const int2 ev = G.edges[e];
+        DBox eb;
+        dbox_init(eb);
+        dbox_add(eb, Q.X0[ev.x]); dbox_add(eb, Q.X0[ev.y]); dbox_add(eb, Q.X1[ev.x]); dbox_add(eb, Q.X1[ev.y]);
+        dbox_to_f(eb, h, qb);
+        if (fbox_ov(qb, bbox + 6 * B)) {
+            // the other edge's box is inflated by h as well; node boxes are not
+            dbox_to_f(eb, infl, qb);
+            kind = K_EE; pid = e;
+            prim0 = M.body_e0[B]; nprim = M.body_e0[B + 1] - prim0;
+            W = M.edge_world + M.body_ew0[B]; n = M.body_ew0[B + 1] - M.body_ew0[B];
+        }
+    }
+    if (lane == 0) qn = 0;
+    __syncthreads();
+    int i = 0;
+    bool bad = false;
+    while (WAVE_ANY(i < n)) {
+        if (i < n) {
+            const BNode nd = W[i];
+            int next = nd.skip;
+            if (node_ov(qb, nd)) {
+                if (nd.prim >= 0) {
+                    if (nd.prim < nprim) {
+                        const int slot = atomicAdd(&qn, 1);
+                        queue[slot] = leaf_pack(kind, pid, prim0 + nd.prim);
+                    } else {
+                        bad = true;
+                    }
+                } else {
+                    next = i + 1;
+                }
+            }
+            if (next <= i || next > n) {
+                bad = true;
+                next = n;
+            }
+            i = bad ? n : next;
+        }
+        __syncthreads();
+        const int cnt = qn;
+        if (cnt >= 64) {
+            const uint64_t ent = queue[lane];
+            __syncthreads();
+            if (lane < cnt - 64) queue[lane] = queue[64 + lane];
+            if (lane == 0) qn = cnt - 64;
+            __syncthreads();
+            leaf_run(Q, s, ent);
+        }
+    }
+    const int cnt = qn;
+    if (lane < cnt) leaf_run(Q, s, queue[lane]);
+    if (bad) {
+        atomicMax(Q.overflow, 2);
+        atomicOr(G.err, ERR_BVH);
     }
 }
 
-void launch_pair_queries(hipStream_t s, const Geo& G, const BodyMesh& M, int npairs, const int2* pairs,
-                         const double4* X0, const double4* X1, const float* bbox, double infl, uint64_t* cand,
-                         int cap, int* count, int* overflow) {
-    if (npairs <= 0) return;
+void launch_pair_queries(hipStream_t s, const Geo& G, const BodyMesh& M, int npairs, int max_queries,
+                         const int2* pairs, const double4* X0, const double4* X1, const float* bbox, double infl,
+                         uint64_t* cand, int cap, int* count, int* overflow) {
+    if (npairs <= 0 || max_queries <= 0) return;
     QueryCtx Q{G, M, X0, X1, infl, cand, cap, count, overflow};
-    hipLaunchKernelGGL(k_pair_queries, dim3(npairs), dim3(256), 0, s, Q, pairs, bbox);
+    const int chunks = (max_queries + 63) / 64;
+    const long long nitems = (long long)npairs * chunks;
+    const long long blocks = 8 * ((nitems + 7) / 8);
+    hipLaunchKernelGGL(k_pair_queries, dim3((unsigned)blocks), dim3(64), 0, s, Q, pairs, npairs, chunks, bbox);
 }
 
 // point-ground candidates (one thread per vertex): min(d(X0), d(X1)) < dhat

@@ -155,10 +155,23 @@ __device__ __forceinline__ void set_coeffs(DistEval& E) {
     }
 }
 
+// Runtime-indexed reads of small register arrays as exact 0/1 blends (finite operands): a
+// dynamically indexed private array, or a select chain the compiler folds back into one, lives
+// in scratch memory.
+__device__ __forceinline__ double sel4(const double* a, int i) {
+    return (double)(i == 0) * a[0] + (double)(i == 1) * a[1] + (double)(i == 2) * a[2] + (double)(i == 3) * a[3];
+}
+__device__ __forceinline__ D3 pick4(const D3* x, int i) {
+    const double w0 = (double)(i == 0), w1 = (double)(i == 1), w2 = (double)(i == 2), w3 = (double)(i == 3);
+    return mk(w0 * x[0].x + w1 * x[1].x + w2 * x[2].x + w3 * x[3].x, w0 * x[0].y + w1 * x[1].y + w2 * x[2].y + w3 * x[3].y,
+              w0 * x[0].z + w1 * x[1].z + w2 * x[2].z + w3 * x[3].z);
+}
+
 // x: the pair's 4 slot positions (PT: p,t0,t1,t2; EE: a0,a1,b0,b1)
 __device__ __forceinline__ void dist_eval(int kind, const D3* x, DistEval& E) {
     E.S = (kind == K_PT) ? classify_pt(x) : classify_ee(x);
-    for (int k = 0; k < E.S.ns; ++k) E.y[k] = x[E.S.st[k]];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) E.y[k] = pick4(x, k < E.S.ns ? E.S.st[k] : 0);
     set_coeffs(E);
     const int ns = E.S.ns;
     if (E.S.type == T_PP) {
@@ -168,7 +181,9 @@ __device__ __forceinline__ void dist_eval(int kind, const D3* x, DistEval& E) {
         return;
     }
     E.a = mk(0, 0, 0); E.u = mk(0, 0, 0); E.v = mk(0, 0, 0); E.w = mk(0, 0, 0);
-    for (int k = 0; k < ns; ++k) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (k >= ns) break;
         E.a = E.a + E.al[k] * E.y[k];
         E.u = E.u + E.mu[k] * E.y[k];
         E.v = E.v + E.nu[k] * E.y[k];
@@ -220,19 +235,19 @@ __device__ __forceinline__ bool may_reach(int kind, const D3* x0, const D3* x1, 
 
 // (Dn_i)_{kr} = -mu_i [v]x_{kr} + nu_i [u]x_{kr}
 __device__ __forceinline__ double Dn(const DistEval& E, int i, int k, int r) {
-    return -E.mu[i] * skew_rc(E.v, k, r) + E.nu[i] * skew_rc(E.u, k, r);
+    return -sel4(E.mu, i) * skew_rc(E.v, k, r) + sel4(E.nu, i) * skew_rc(E.u, k, r);
 }
 // d s / d y_{i,r}
 __device__ __forceinline__ double gs_at(const DistEval& E, int R) {
     int i = R / 3, r = R % 3;
-    return E.al[i] * cmp(E.n, r) + E.mu[i] * cmp(E.vxa, r) + E.nu[i] * cmp(E.axu, r);
+    return sel4(E.al, i) * cmp(E.n, r) + sel4(E.mu, i) * cmp(E.vxa, r) + sel4(E.nu, i) * cmp(E.axu, r);
 }
 // d N / d y_{i,r} = 2 (Dn_i^T n)_r
 __device__ __forceinline__ double gN_at(const DistEval& E, int R) {
     int i = R / 3, r = R % 3;
     return 2.0 * (Dn(E, i, 0, r) * E.n.x + Dn(E, i, 1, r) * E.n.y + Dn(E, i, 2, r) * E.n.z);
 }
-__device__ __forceinline__ double gL_at(const DistEval& E, int R) { return 2.0 * E.om[R / 3] * cmp(E.w, R % 3); }
+__device__ __forceinline__ double gL_at(const DistEval& E, int R) { return 2.0 * sel4(E.om, R / 3) * cmp(E.w, R % 3); }
 
 // gradient of d^2 w.r.t. stencil coordinate R (0 <= R < 3 ns)
 __device__ __forceinline__ double d2_grad(const DistEval& E, int R) {
@@ -249,7 +264,7 @@ __device__ __forceinline__ double hN_at(const DistEval& E, int R, int C) {
     double t = 0;
 #pragma unroll
     for (int k = 0; k < 3; ++k) t += Dn(E, i, k, r) * Dn(E, j, k, c);
-    return 2.0 * t + 2.0 * (-E.mu[i] * E.nu[j] + E.nu[i] * E.mu[j]) * skew_rc(E.n, r, c);
+    return 2.0 * t + 2.0 * (-sel4(E.mu, i) * sel4(E.nu, j) + sel4(E.nu, i) * sel4(E.mu, j)) * skew_rc(E.n, r, c);
 }
 
 // Hessian of d^2 w.r.t. stencil coordinates (R, C)
@@ -260,15 +275,16 @@ __device__ __forceinline__ double d2_hess(const DistEval& E, int R, int C) {
         return (i == j) ? 2.0 : -2.0;
     }
     if (E.S.type == T_PE) {
-        double hL = (r == c) ? 2.0 * E.om[i] * E.om[j] : 0.0;
+        double hL = (r == c) ? 2.0 * sel4(E.om, i) * sel4(E.om, j) : 0.0;
         double gNR = gN_at(E, R), gNC = gN_at(E, C), gLR = gL_at(E, R), gLC = gL_at(E, C);
         double L = E.L, N = E.N;
         return hN_at(E, R, C) / L - (gNR * gLC + gLR * gNC) / (L * L) + 2.0 * N / (L * L * L) * gLR * gLC -
                N / (L * L) * hL;
     }
-    double hs = E.al[i] * E.mu[j] * (-skew_rc(E.v, r, c)) + E.al[i] * E.nu[j] * skew_rc(E.u, r, c) +
-                E.mu[i] * E.al[j] * skew_rc(E.v, r, c) + E.mu[i] * E.nu[j] * (-skew_rc(E.a, r, c)) +
-                E.nu[i] * E.al[j] * (-skew_rc(E.u, r, c)) + E.nu[i] * E.mu[j] * skew_rc(E.a, r, c);
+    const double ali = sel4(E.al, i), mui = sel4(E.mu, i), nui = sel4(E.nu, i);
+    const double alj = sel4(E.al, j), muj = sel4(E.mu, j), nuj = sel4(E.nu, j);
+    double hs = ali * muj * (-skew_rc(E.v, r, c)) + ali * nuj * skew_rc(E.u, r, c) + mui * alj * skew_rc(E.v, r, c) +
+                mui * nuj * (-skew_rc(E.a, r, c)) + nui * alj * (-skew_rc(E.u, r, c)) + nui * muj * skew_rc(E.a, r, c);
     double gsR = gs_at(E, R), gsC = gs_at(E, C), gNR = gN_at(E, R), gNC = gN_at(E, C);
     double s = E.s, N = E.N;
     return 2.0 / N * gsR * gsC + 2.0 * s / N * hs - 2.0 * s / (N * N) * (gsR * gNC + gNR * gsC) +

@@ -104,11 +104,11 @@ void launch_xform(hipStream_t s, const Geo& G, const double* q, const double* du
 void launch_body_boxes(hipStream_t s, const Geo& G, const BodyMesh& M, const double4* X0, const double4* X1,
                        double half_infl, float* box);
 void launch_world_nodes(hipStream_t s, const Geo& G, const BodyMesh& M, const double* q, const double* du);
-void launch_body_pairs(hipStream_t s, const Geo& G, const long long* pair_off, long long total, const float* box,
-                       int2* out, int cap, int* count, int* overflow);
-void launch_pair_queries(hipStream_t s, const Geo& G, const BodyMesh& M, int npairs, const int2* pairs,
-                         const double4* X0, const double4* X1, const float* bbox, double infl, uint64_t* cand,
-                         int cap, int* count, int* overflow);
+void launch_body_pair_count(hipStream_t s, const Geo& G, const float* box, int* cnt);
+void launch_body_pair_write(hipStream_t s, const Geo& G, const float* box, const int* off, int2* out, int cap);
+void launch_pair_queries(hipStream_t s, const Geo& G, const BodyMesh& M, int npairs, int max_queries,
+                         const int2* pairs, const double4* X0, const double4* X1, const float* bbox, double infl,
+                         uint64_t* cand, int cap, int* count, int* overflow);
 void launch_ground_queries(hipStream_t s, const Geo& G, const double4* X0, const double4* X1, const AbdParams& P,
                            double infl, uint64_t* cand, int cap, int* count, int* overflow);
 // abd_kernels.hip

@@ -9,6 +9,7 @@
 // Every reduction is per scene, over fixed ranges in a fixed order, so results do not depend
 // on scheduling (the reference's wp.atomic_add sums, abd.py:64, 75, 301-303, do).
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <climits>
 #include <float.h>
 #include <stdint.h>

@@ -35,7 +35,8 @@ class Params:
     ls_floor: float = 1e-2
     pinned_diag: float = 100.0
     max_candidates: int = 1 << 20
-    debug: int = 0  # bit 0: synchronise and check after every launch; bit 1: also log them
+    debug: int = 0  # bit 0: synchronise and check after every launch; bit 1: also log them;
+    #                 bit 2: print broad-phase list sizes
 
     def to_c(self) -> AbdParams:
         p = AbdParams()

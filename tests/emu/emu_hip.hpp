@@ -104,6 +104,19 @@ inline T __shfl_xor(T v, int mask, int width = 64) {
     return r;
 }
 
+// wave vote over the lanes of the caller's 64-lane wave (every lane of the wave must call it)
+inline int emu_any(int pred) {
+    unsigned lane = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
+    emu_sched->xbuf[lane] = pred ? 1.0 : 0.0;
+    emu_yield();
+    unsigned w0 = lane & ~63u, nt = blockDim.x * blockDim.y * blockDim.z;
+    int r = 0;
+    for (unsigned k = w0; k < w0 + 64 && k < nt; ++k)
+        if (emu_sched->xbuf[k] != 0.0) r = 1;
+    emu_yield();
+    return r;
+}
+
 inline int atomicAdd(int* p, int v) { return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST); }
 inline int atomicOr(int* p, int v) { return __atomic_fetch_or(p, v, __ATOMIC_SEQ_CST); }
 inline int atomicMax(int* p, int v) {
