@@ -1,7 +1,7 @@
 // abd_sort.hip — stable LSD radix sort of 64-bit keys (optionally carrying int values) and an
 // exclusive prefix sum of ints, for the candidate, incidence and body-pair lists.
 //
-// Radix sort: 4-bit digits; up to SORT_TILE keys all passes run in one workgroup (k_sort_small).  Per pass, each 256-thread block owns a tile of 2048 consecutive
+// Radix sort: 4-bit digits; up to SORT_TILE keys are ranked by one workgroup (k_sort_small).  Per pass, each 256-thread block owns a tile of 2048 consecutive
 // elements (8 per thread, in order); it counts digits (k_radix_hist), the digit-major histogram
 // [digit][block] is prefix-summed, and the block re-reads its tile, ranks every element among
 // the equal digits of earlier threads and earlier items (a block-wide scan over a [digit][thread]
@@ -165,75 +165,26 @@ __global__ __launch_bounds__(SORT_T) void k_radix_scatter(int n, const uint64_t*
     }
 }
 
-// Whole sort of n <= SORT_TILE keys by one workgroup: every pass ranks the tile as
-// k_radix_scatter does (the digit-major prefix over [digit][thread] counts is then the final
-// position) and scatters through LDS.
+// Whole sort of n <= SORT_TILE keys by one workgroup, by counting ranks: the keys sit in LDS and
+// the final position of key i is #{j : k_j < k_i} + #{j < i : k_j == k_i} (stable; n^2 / 256
+// comparisons per thread, cheaper than 16 radix passes of barriers at the sizes it serves).
 __global__ __launch_bounds__(SORT_T) void k_sort_small(int n, const uint64_t* __restrict__ kin,
                                                         const int* __restrict__ vin, uint64_t* __restrict__ kout,
-                                                        int* __restrict__ vout, int passes) {
+                                                        int* __restrict__ vout) {
     __shared__ uint64_t kb[SORT_TILE];
-    __shared__ int vb[SORT_TILE];
-    __shared__ int m[RADIX * SORT_T];
-    __shared__ int sh[SORT_T];
     const int t = threadIdx.x;
-    const int base = t * SORT_I;
-    uint64_t k[SORT_I];
-    int v[SORT_I];
-    for (int i = 0; i < SORT_I; ++i) {
-        k[i] = (base + i < n) ? kin[base + i] : 0;
-        v[i] = (vin && base + i < n) ? vin[base + i] : 0;
-    }
-    for (int pass = 0; pass < passes; ++pass) {
-        const int shift = 4 * pass;
-        int dg[SORT_I];
-        int c[RADIX];
-        for (int d = 0; d < RADIX; ++d) c[d] = 0;
-        for (int i = 0; i < SORT_I; ++i) {
-            if (base + i < n) {
-                dg[i] = opaque((int)((k[i] >> shift) & (RADIX - 1)));
-                c[dg[i]]++;
-            } else {
-                dg[i] = -1;
-            }
+    for (int i = t; i < n; i += SORT_T) kb[i] = kin[i];
+    __syncthreads();
+    for (int i = t; i < n; i += SORT_T) {
+        const uint64_t k = kb[i];
+        int rank = 0;
+        for (int j = 0; j < n; ++j) {
+            const uint64_t o = kb[j];
+            rank += (o < k || (o == k && j < i)) ? 1 : 0;
         }
-        for (int d = 0; d < RADIX; ++d) m[d * SORT_T + t] = c[d];
-        __syncthreads();
-        int run[RADIX];
-        int s = 0;
-        for (int j = 0; j < RADIX; ++j) {
-            run[j] = m[t * RADIX + j];
-            s += run[j];
-        }
-        int ex = block_excl_scan256(s, sh);
-        for (int j = 0; j < RADIX; ++j) {
-            m[t * RADIX + j] = ex;
-            ex += run[j];
-        }
-        __syncthreads();
-        int seen[RADIX];
-        for (int d = 0; d < RADIX; ++d) seen[d] = 0;
-        for (int i = 0; i < SORT_I; ++i) {
-            const int d = dg[i];
-            if (d < 0) continue;
-            const int pos = m[d * SORT_T + t] + seen[d];
-            seen[d]++;
-            kb[pos] = k[i];
-            vb[pos] = v[i];
-        }
-        __syncthreads();
-        for (int i = 0; i < SORT_I; ++i) {
-            if (base + i < n) {
-                k[i] = kb[base + i];
-                v[i] = vb[base + i];
-            }
-        }
-        __syncthreads();
-    }
-    for (int i = 0; i < SORT_I; ++i) {
-        if (base + i < n) {
-            kout[base + i] = k[i];
-            if (vout) vout[base + i] = v[i];
-        }
+        rank = opaque(rank);
+        kout[rank] = k;
+        if (vout) vout[rank] = vin[i];
     }
 }
 
@@ -258,7 +209,7 @@ hipError_t radix_sort64(hipStream_t s, const uint64_t* kin, uint64_t* kout, cons
     int* bsum = hoff + RADIX * nblk;
     const int passes = (end_bit + 3) / 4;
     if (n <= SORT_TILE) {
-        hipLaunchKernelGGL(k_sort_small, dim3(1), dim3(SORT_T), 0, s, n, kin, vin, kout, vin ? vout : nullptr, passes);
+        hipLaunchKernelGGL(k_sort_small, dim3(1), dim3(SORT_T), 0, s, n, kin, vin, kout, vin ? vout : nullptr);
         return hipGetLastError();
     }
     if (passes == 0) {
