@@ -98,7 +98,7 @@ def main():
     solver.step(args.profile_steps)
     kt = solver.kernel_times()
     solver.enable_timing(False)
-    single = ["pcg_spmv", "pcg_update", "narrow_phase", "contact_hess", "offblock_sum", "body_assemble",
+    single = ["pcg_iter", "broad_phase_query", "narrow_phase", "contact_hess", "offblock_sum", "body_assemble",
               "ccd", "energy"]
     dom = max((k for k in kt if k in single), key=lambda k: kt[k]["ms"], default=None)
     roof = None

@@ -110,6 +110,10 @@ struct AbdCtx {
     int *ent_v = nullptr, *ent_vs = nullptr, *row_ptr = nullptr;
     // PCG
     double *r = nullptr, *z = nullptr, *p0 = nullptr, *p1 = nullptr, *Ap = nullptr, *part = nullptr, *bnorm = nullptr;
+    double *r1 = nullptr, *w1 = nullptr, *s0 = nullptr, *pcg_sc = nullptr;
+    unsigned* pcg_ctl = nullptr;  // persistent solve: [0] barrier count, [1..3] done counters, [4] failure flag
+    unsigned* h_pcg_ctl = nullptr;
+    bool coop_ok = true;          // cleared when a cooperative launch is refused
     int* cg_iters = nullptr;
     int* cg_done = nullptr;
     // per-scene control
@@ -530,35 +534,71 @@ int assemble(AbdCtx* ctx) {
 // --------------------------------------------------------------------------------------------
 // block-Jacobi PCG (compute_du, abd.py:631-658); du <- solution, residual <- ||grad||
 // --------------------------------------------------------------------------------------------
+// one cooperative launch for the whole solve (k_pcg_persist); false when the launch was refused or
+// a grid barrier timed out, in which case the caller solves again with one launch per iteration
+int solve_persistent(AbdCtx* ctx, const PcgBuf& B, bool* ok) {
+    *ok = false;
+    if (!ctx->coop_ok || (ctx->P.reserved0 & 8)) return ABD_OK;
+    HIPCHK(hipMemsetAsync(ctx->cg_iters, 0, sizeof(int) * ctx->ns, ctx->st));
+    HIPCHK(hipMemsetAsync(ctx->pcg_ctl, 0, 8 * sizeof(unsigned), ctx->st));
+    tic(ctx, "pcg_solve");
+    hipError_t e = launch_pcg_persist(ctx->st, ctx->ntiles, ctx->ns, ctx->tile_b0, ctx->tile_b1, ctx->tile_scene,
+                                      ctx->scene_tile0, B, ctx->P.cg_tol, ctx->P.cg_max_iter, ctx->nb, ctx->n_pairs);
+    toc(ctx);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        ctx->coop_ok = false;
+        return ABD_OK;
+    }
+    CHK(dbg(ctx, "launch_pcg_persist"));
+    HIPCHK(hipMemcpyAsync(ctx->h_pcg_ctl, ctx->pcg_ctl, 8 * sizeof(unsigned), hipMemcpyDeviceToHost, ctx->st));
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    if (ctx->h_pcg_ctl[4]) {
+        ctx->coop_ok = false;  // a grid barrier did not fill: do not try again on this context
+        return ABD_OK;
+    }
+    *ok = true;
+    return ABD_OK;
+}
+
 int solve(AbdCtx* ctx) {
     const int nt = ctx->ntiles;
-    launch_pcg_init(ctx->st, nt, ctx->tile_b0, ctx->tile_b1, ctx->grad, ctx->pinv, ctx->du, ctx->r, ctx->z, ctx->p1,
-                    ctx->part);
-    CHK(dbg(ctx, "launch_pcg_init"));
+    PcgBuf B;
+    B.b = ctx->grad; B.diag = ctx->diag; B.pinv = ctx->pinv; B.row_ptr = ctx->row_ptr; B.ent_key = ctx->ent_ks;
+    B.ent_val = ctx->ent_vs; B.off = ctx->off; B.offT = ctx->offT; B.x = ctx->du; B.p = ctx->p0; B.u = ctx->z;
+    B.r[0] = ctx->r; B.r[1] = ctx->r1; B.w[0] = ctx->Ap; B.w[1] = ctx->w1; B.sv[0] = ctx->s0; B.sv[1] = ctx->p1;
+    B.part = ctx->part; B.sc = ctx->pcg_sc; B.bnorm = ctx->bnorm; B.cg_iters = ctx->cg_iters; B.cg_done = ctx->cg_done;
+    B.err = ctx->cnt + C_ERR;
+    B.bar = ctx->pcg_ctl; B.done_cnt = (int*)(ctx->pcg_ctl + 1); B.fail = (int*)(ctx->pcg_ctl + 4);
+    B.bnorm_out = ctx->bnorm; B.residual_out = ctx->C.residual;
+    {
+        bool ok = false;
+        CHK(solve_persistent(ctx, B, &ok));
+        if (ok) {
+            HIPCHK(hipGetLastError());
+            ctx->solved = true;
+            return ABD_OK;
+        }
+    }
+    const double tol = ctx->P.cg_tol;
+    HIPCHK(hipMemsetAsync(ctx->cg_iters, 0, sizeof(int) * ctx->ns, ctx->st));
+    HIPCHK(hipMemsetAsync(ctx->cg_done, 0, sizeof(int) * ctx->ns, ctx->st));
+    launch_pcg_cg(ctx->st, 0, nt, ctx->ns, ctx->tile_b0, ctx->tile_b1, ctx->tile_scene, ctx->scene_tile0, B, tol,
+                  ctx->nb, ctx->n_pairs);
+    CHK(dbg(ctx, "launch_pcg_cg(0)"));
     launch_pcg_tile_sum(ctx->st, ctx->ns, ctx->scene_tile0, ctx->part, nt, 2 * 2 + 0, ctx->bnorm, 1);
     CHK(dbg(ctx, "launch_pcg_tile_sum"));
     hipLaunchKernelGGL(k_copy, dim3(cdiv(ctx->ns, 64)), dim3(64), 0, ctx->st, ctx->ns, ctx->bnorm, ctx->C.residual);
     CHK(dbg(ctx, "k_copy"));
-    HIPCHK(hipMemsetAsync(ctx->cg_iters, 0, sizeof(int) * ctx->ns, ctx->st));
-    HIPCHK(hipMemsetAsync(ctx->cg_done, 0, sizeof(int) * ctx->ns, ctx->st));
-    const double tol = ctx->P.cg_tol;
     const int nblk = ctx->nb + 2 * ctx->n_pairs;
-    const double bytesA = (double)nblk * (1152 + 192) + (double)ctx->nb * 192;
-    const double bytesB = (double)ctx->nb * (1152 + 96 * 7);
+    const double bytes = (double)nblk * 1152 + (double)ctx->n_pairs * 2 * (1152 + 3 * 96) + (double)ctx->nb * (1152 + 14 * 96);
     int poll = 4;
     for (int it = 0; it < ctx->P.cg_max_iter; ++it) {
-        double* pold = (it & 1) ? ctx->p0 : ctx->p1;
-        double* pnew = (it & 1) ? ctx->p1 : ctx->p0;
-        tic(ctx, "pcg_spmv", bytesA);
-        launch_pcg_A(ctx->st, it, nt, ctx->tile_b0, ctx->tile_b1, ctx->tile_scene, ctx->scene_tile0, ctx->diag,
-                     ctx->row_ptr, ctx->ent_ks, ctx->ent_vs, ctx->off, ctx->offT, ctx->z, pold, pnew, ctx->Ap,
-                     ctx->part, ctx->bnorm, tol, ctx->cg_iters, ctx->cg_done, ctx->nb, ctx->n_pairs,
-                     ctx->cnt + C_ERR);
+        tic(ctx, "pcg_iter", bytes);
+        launch_pcg_cg(ctx->st, it + 1, nt, ctx->ns, ctx->tile_b0, ctx->tile_b1, ctx->tile_scene, ctx->scene_tile0, B,
+                      tol, ctx->nb, ctx->n_pairs);
         toc(ctx);
-        tic(ctx, "pcg_update", bytesB);
-        launch_pcg_B(ctx->st, it, nt, ctx->tile_b0, ctx->tile_b1, ctx->tile_scene, ctx->scene_tile0, ctx->pinv, pnew,
-                     ctx->Ap, ctx->du, ctx->r, ctx->z, ctx->part, ctx->bnorm, tol, ctx->cg_done);
-        toc(ctx);
+        CHK(dbg(ctx, "launch_pcg_cg"));
         if (it + 1 == poll || it + 1 == ctx->P.cg_max_iter) {
             HIPCHK(hipMemsetAsync(ctx->cnt + C_ANY, 0, sizeof(int), ctx->st));
             launch_pcg_any(ctx->st, ctx->ns, nt, ctx->scene_tile0, ctx->part, it + 1, ctx->bnorm, tol, ctx->cg_done,
@@ -1019,13 +1059,15 @@ int abd_create(const AbdSceneDesc* d, const AbdParams* p, AbdCtx** out) {
                       ctx->body_tw0, ctx->body_ew0, ctx->tri_nodes, ctx->edge_nodes, ctx->tri_world, ctx->edge_world};
     A(cand_lo, ns); A(cand_hi, ns);
     A(r, 12 * nb); A(z, 12 * nb); A(p0, 12 * nb); A(p1, 12 * nb); A(Ap, 12 * nb);
+    A(r1, 12 * nb); A(w1, 12 * nb); A(s0, 12 * nb); A(pcg_sc, 4 * ns); A(pcg_ctl, 8);
     A(part, 6 * ctx->ntiles); A(bnorm, ns); A(cg_iters, ns); A(cg_done, ns);
     A(C.newton_active, ns); A(C.residual, ns); A(C.alpha, ns); A(C.alpha0, ns); A(C.amax_bits, ns);
     A(C.ls_done, ns); A(C.ls_floor, ns); A(C.E0, ns); A(C.E1, ns);
     A(nact_s, ns); A(act_before, ns); A(report, R_N * ns); A(cnt, C_N); A(row_ptr, nb + 1);
 #undef A
 #undef U
-    if (hipHostMalloc((void**)&ctx->h_cnt, C_N * sizeof(int), hipHostMallocDefault) != hipSuccess ||
+    if (hipHostMalloc((void**)&ctx->h_pcg_ctl, 8 * sizeof(unsigned), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&ctx->h_cnt, C_N * sizeof(int), hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc((void**)&ctx->h_report, R_N * ns * sizeof(double), hipHostMallocDefault) != hipSuccess) {
         ctx->err = "hipHostMalloc failed";
         return fail(ABD_ERR_HIP);
@@ -1060,6 +1102,7 @@ void abd_destroy(AbdCtx* ctx) {
     for (auto e : ctx->T.pool) (void)hipEventDestroy(e);
     for (auto& r : ctx->T.open) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
     if (ctx->h_cnt) (void)hipHostFree(ctx->h_cnt);
+    if (ctx->h_pcg_ctl) (void)hipHostFree(ctx->h_pcg_ctl);
     if (ctx->h_report) (void)hipHostFree(ctx->h_report);
     if (ctx->st) (void)hipStreamDestroy(ctx->st);
     delete ctx;

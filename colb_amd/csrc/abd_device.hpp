@@ -314,8 +314,10 @@ __device__ inline void wave_psd_project(double* A, double* V, int m, int lane) {
     if (fro == 0.0) return;
     // Parallel (round-robin) Jacobi: a sweep is npl - 1 rounds of npl / 2 disjoint rotations
     // (circle method over npl = m rounded up to even players; a pair with the extra player is
-    // skipped).  Lane task t = (pair k, index i): the k-th pair's column rotation of row i and row
-    // rotation of column i.
+    // skipped).  Lane k < npl/2 computes the k-th rotation of the round once (LDS), then lane task
+    // t = (pair k, index i) applies its column rotation of row i and its row rotation of column i.
+    __shared__ double jr_c[8], jr_s[8];
+    __shared__ int jr_p[8], jr_q[8];
     const int npl = m + (m & 1), half = npl / 2, ntask = half * m;
     for (int sweep = 0; sweep < 32; ++sweep) {
         double o = 0;
@@ -326,33 +328,34 @@ __device__ inline void wave_psd_project(double* A, double* V, int m, int lane) {
         double off = wave_sum(o);
         if (off <= 1e-32 * fro) break;
         for (int r = 0; r < npl - 1; ++r) {
-            int tp[2], tq[2], ti[2];
-            double tc[2], ts[2];
-            for (int u = 0; u < 2; ++u) {
-                const int t = lane + 64 * u;
-                tp[u] = -1;
-                if (t >= ntask) continue;
-                const int k = t / m, i = t % m;
+            if (lane < half) {
+                const int k = lane;
                 const int a = (k == 0) ? 0 : 1 + (k - 1 + r) % (npl - 1);
                 const int b = 1 + (npl - 2 - k + r) % (npl - 1);
                 const int p = a < b ? a : b, q = a < b ? b : a;
-                if (q >= m) continue;
-                const double apq = A[p * m + q];
                 double c = 1.0, sn = 0.0;
-                if (apq != 0.0) {
-                    const double theta = (A[q * m + q] - A[p * m + p]) / (2.0 * apq);
-                    const double tt = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
-                    c = 1.0 / sqrt(tt * tt + 1.0);
-                    sn = tt * c;
+                if (q < m) {
+                    const double apq = A[p * m + q];
+                    if (apq != 0.0) {
+                        const double theta = (A[q * m + q] - A[p * m + p]) / (2.0 * apq);
+                        const double tt = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+                        c = 1.0 / sqrt(tt * tt + 1.0);
+                        sn = tt * c;
+                    }
                 }
-                tp[u] = p; tq[u] = q; ti[u] = i; tc[u] = c; ts[u] = sn;
+                jr_c[k] = c;
+                jr_s[k] = sn;
+                jr_p[k] = (q < m && sn != 0.0) ? p : -1;
+                jr_q[k] = q;
             }
             __syncthreads();
             // columns: A <- A J, V <- V J
-            for (int u = 0; u < 2; ++u) {
-                if (tp[u] < 0) continue;
-                const int p = tp[u], q = tq[u], i = ti[u];
-                const double c = tc[u], sn = ts[u];
+            for (int t = lane; t < ntask; t += 64) {
+                const int k = t / m, i = t % m;
+                const int p = jr_p[k];
+                if (p < 0) continue;
+                const int q = jr_q[k];
+                const double c = jr_c[k], sn = jr_s[k];
                 const double aip = A[i * m + p], aiq = A[i * m + q];
                 A[i * m + p] = c * aip - sn * aiq;
                 A[i * m + q] = sn * aip + c * aiq;
@@ -362,10 +365,12 @@ __device__ inline void wave_psd_project(double* A, double* V, int m, int lane) {
             }
             __syncthreads();
             // rows: A <- J^T A
-            for (int u = 0; u < 2; ++u) {
-                if (tp[u] < 0) continue;
-                const int p = tp[u], q = tq[u], j = ti[u];
-                const double c = tc[u], sn = ts[u];
+            for (int t = lane; t < ntask; t += 64) {
+                const int k = t / m, j = t % m;
+                const int p = jr_p[k];
+                if (p < 0) continue;
+                const int q = jr_q[k];
+                const double c = jr_c[k], sn = jr_s[k];
                 const double apj = A[p * m + j], aqj = A[q * m + j];
                 A[p * m + j] = c * apj - sn * aqj;
                 A[q * m + j] = sn * apj + c * aqj;

@@ -83,6 +83,36 @@ struct SceneCtl {
     double* E1;
 };
 
+// Buffers of the block-Jacobi PCG (abd_kernels.hip, k_pcg_cg); r, w, sv are ping-pong pairs.
+struct PcgBuf {
+    const double* b;           // right-hand side (gradient)
+    const double* diag;
+    const double* pinv;
+    const int* row_ptr;
+    const uint64_t* ent_key;
+    const int* ent_val;
+    const double* off;
+    const double* offT;
+    double* x;
+    double* p;
+    double* u;
+    double* r[2];
+    double* w[2];
+    double* sv[2];
+    double* part;
+    double* sc;                // [4 * ns]: g by parity, then alpha by parity
+    const double* bnorm;
+    int* cg_iters;
+    int* cg_done;
+    int* err;
+    // persistent solve: grid barrier counter, 3-slot done counters, failure flag, outputs
+    unsigned* bar;
+    int* done_cnt;
+    int* fail;
+    double* bnorm_out;
+    double* residual_out;
+};
+
 struct GrowBuf {
     void* p = nullptr;
     size_t bytes = 0;
@@ -136,18 +166,13 @@ void launch_body_assemble(hipStream_t s, const Geo& G, const AbdParams& P, const
                           double* e_body, int n_act);
 void launch_scene_sum(hipStream_t s, const Geo& G, const double* e_body, const double* e_cand, const int* cand_lo,
                       const int* cand_hi, double* out);
-void launch_pcg_init(hipStream_t s, int ntiles, const int* tile_b0, const int* tile_b1, const double* g,
-                     const double* pinv, double* x, double* r, double* z, double* p, double* part);
 void launch_pcg_tile_sum(hipStream_t s, int ns, const int* scene_tile0, const double* part, int stride, int which,
                          double* out, int do_sqrt);
-void launch_pcg_A(hipStream_t s, int it, int ntiles, const int* tile_b0, const int* tile_b1, const int* tile_scene,
-                  const int* scene_tile0, const double* diag, const int* row_ptr, const uint64_t* ent_key,
-                  const int* ent_val, const double* off, const double* offT, const double* z, const double* p_old,
-                  double* p_new, double* Ap, double* part, const double* bnorm, double tol, int* cg_iters,
-                  int* cg_done, int nb, int n_pairs, int* err);
-void launch_pcg_B(hipStream_t s, int it, int ntiles, const int* tile_b0, const int* tile_b1, const int* tile_scene,
-                  const int* scene_tile0, const double* pinv, const double* p, const double* Ap, double* x,
-                  double* r, double* z, double* part, const double* bnorm, double tol, const int* cg_done);
+void launch_pcg_cg(hipStream_t s, int k, int ntiles, int ns, const int* tile_b0, const int* tile_b1,
+                   const int* tile_scene, const int* scene_tile0, const PcgBuf& B, double tol, int nb, int n_pairs);
+hipError_t launch_pcg_persist(hipStream_t s, int ntiles, int ns, const int* tile_b0, const int* tile_b1,
+                              const int* tile_scene, const int* scene_tile0, const PcgBuf& B, double tol, int max_iter,
+                              int nb, int n_pairs);
 void launch_pcg_any(hipStream_t s, int ns, int ntiles, const int* scene_tile0, const double* part, int it,
                     const double* bnorm, double tol, const int* cg_done, int* any);
 void launch_ccd(hipStream_t s, const Geo& G, int n, const uint64_t* keys, const double4* X0, const double4* X1,

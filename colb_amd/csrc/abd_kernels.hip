@@ -254,7 +254,8 @@ __global__ __launch_bounds__(64) void k_contact_hess(Geo G, int n, const uint64_
     const int bA = (int)G.rest[p.v[0]].w;
     const int bB = (p.kind == K_PG) ? -1 : (int)G.rest[p.v[3]].w;
     if (lane < ns) {
-        int vid = p.v[st[lane]];
+        const int sl = (lane == 0) ? st[0] : (lane == 1) ? st[1] : (lane == 2) ? st[2] : st[3];
+        const int vid = (sl == 0) * p.v[0] + (sl == 1) * p.v[1] + (sl == 2) * p.v[2] + (sl == 3) * p.v[3];
         double4 r = G.rest[vid];
         side[lane] = ((int)r.w == bA) ? 0 : 1;
         jv[lane][0] = 1.0; jv[lane][1] = r.x; jv[lane][2] = r.y; jv[lane][3] = r.z;
@@ -587,39 +588,6 @@ __device__ __forceinline__ double scene_partial(const double* part, int ntiles, 
     return wave_sum(acc);
 }
 
-__global__ __launch_bounds__(64) void k_pcg_init(int ntiles, const int* __restrict__ tb0, const int* __restrict__ tb1,
-                                                  const double* __restrict__ g, const double* __restrict__ pinv,
-                                                  double* x, double* r, double* z, double* p, double* part) {
-    __shared__ double rs[60];
-    int t = blockIdx.x, lane = threadIdx.x;
-    int b0 = tb0[t], nbt = tb1[t] - b0;
-    int lb = lane / 12, row = lane % 12;
-    bool on = lb < nbt;
-    int idx = 12 * (b0 + lb) + row;
-    double rv = on ? g[idx] : 0.0;
-    if (lane < 60) rs[lane] = rv;
-    __syncthreads();
-    double zv = 0.0;
-    if (on) {
-        const double* Pb = pinv + (size_t)(b0 + lb) * 144 + row * 12;
-        for (int c = 0; c < 12; ++c) zv += Pb[c] * rs[lb * 12 + c];
-        x[idx] = 0.0;
-        r[idx] = rv;
-        z[idx] = zv;
-        p[idx] = 0.0;
-    }
-    double rz = wave_sum(on ? rv * zv : 0.0), rr = wave_sum(on ? rv * rv : 0.0);
-    if (lane == 0) {
-        part[(size_t)(1 * 2 + 0) * ntiles + t] = rz;
-        part[(size_t)(2 * 2 + 0) * ntiles + t] = rr;
-    }
-}
-
-void launch_pcg_init(hipStream_t s, int ntiles, const int* tile_b0, const int* tile_b1, const double* g,
-                     const double* pinv, double* x, double* r, double* z, double* p, double* part) {
-    hipLaunchKernelGGL(k_pcg_init, dim3(ntiles), dim3(64), 0, s, ntiles, tile_b0, tile_b1, g, pinv, x, r, z, p, part);
-}
-
 // per-scene sum of one partial (which = qty*2+par); optional sqrt (residual norms)
 __global__ __launch_bounds__(64) void k_pcg_tile_sum(int ns, const int* __restrict__ st0, const double* __restrict__ part,
                                                       int stride, int which, double* out, int do_sqrt) {
@@ -644,114 +612,366 @@ __device__ __forceinline__ bool pcg_done(const double* part, int ntiles, int par
     return sqrt(rr) <= tol * bn;
 }
 
-// p = z + beta p_old (own rows stored; neighbours recomputed on the fly), Ap = H p, partial p.Ap
-__global__ __launch_bounds__(64) void k_pcg_A(int it, int ntiles, const int* __restrict__ tb0, const int* __restrict__ tb1,
-                                               const int* __restrict__ tscene, const int* __restrict__ st0,
-                                               const double* __restrict__ diag, const int* __restrict__ row_ptr,
-                                               const uint64_t* __restrict__ ent_key, const int* __restrict__ ent_val,
-                                               const double* __restrict__ off, const double* __restrict__ offT,
-                                               const double* __restrict__ z, const double* __restrict__ p_old,
-                                               double* __restrict__ p_new, double* __restrict__ Ap, double* part,
-                                               const double* __restrict__ bnorm, double tol, int* cg_iters,
-                                               int* cg_done, int nb, int n_pairs, int* err) {
-    int t = blockIdx.x, lane = threadIdx.x;
-    int s = tscene[t];
-    int t0 = st0[s], t1 = st0[s + 1];
-    const int par = it & 1;
-    if (pcg_done(part, ntiles, par, t0, t1, lane, bnorm[s], tol, cg_done + s)) {
-        if (t == t0 && lane == 0) cg_done[s] = 1;
-        return;
+// Block-Jacobi PCG in the Chronopoulos-Gear arrangement (one global reduction point, so one
+// launch, per iteration; algebraically the standard PCG of the oracle):
+//   stage 0:   r0 = b, u0 = P r0, w0 = H u0;                     partials g0=(r,u), d0=(w,u), rho0=(r,r)
+//   stage i+1: beta = g_i/g_{i-1}, alpha = g_i/(d_i - beta g_i/alpha_{i-1})  (i = 0: beta 0, alpha g/d)
+//              p = u + beta p, s = w + beta s, x += alpha p, r -= alpha s,
+//              u = P r, w = H u;                                  partials g, d, rho of the new r, u, w
+// H u needs u of neighbouring bodies: each recomputes it from the neighbour's previous r, w, s
+// (ping-pong buffers, slot k&1 written at stage k, slot (k-1)&1 read), 12 lanes of the body
+// computing one row each and sharing it through LDS.  Partials: qty 0 = d, 1 = g, 2 = rho, at
+// parity k&1.  Per-scene scalars sc[s][par] = g_i, sc[ns + ...] = alpha_i, written by the scene's
+// first tile.  A scene stops (sticky flag) once sqrt(rho_i) <= tol * ||b||.
+
+// row of the neighbour's new preconditioned residual: (P_j (r_j - alpha (w_j + beta s_j)))_row
+__device__ __forceinline__ double pcg_r_next(double r, double w, double s, double alpha, double beta) {
+    return r - alpha * (w + beta * s);
+}
+
+__global__ __launch_bounds__(64) void k_pcg_cg(int k, int ntiles, int ns, const int* __restrict__ tb0,
+                                                const int* __restrict__ tb1, const int* __restrict__ tscene,
+                                                const int* __restrict__ st0, PcgBuf B, double tol, int nb,
+                                                int n_pairs) {
+    __shared__ double vec[TILE_BODIES + 1][12];  // lanes 60-63 (lb = 5) only touch the last row
+    const int t = blockIdx.x, lane = threadIdx.x;
+    const int s = tscene[t];
+    const int t0 = st0[s], t1 = st0[s + 1];
+    const bool init = (k == 0);
+    const int rd = (k + 1) & 1, wr = k & 1;
+    double alpha = 0.0, beta = 0.0;
+    if (!init) {
+        const int par = (k - 1) & 1;
+        if (pcg_done(B.part, ntiles, par, t0, t1, lane, B.bnorm[s], tol, B.cg_done + s)) {
+            if (t == t0 && lane == 0) B.cg_done[s] = 1;
+            return;
+        }
+        const int i = k - 1;
+        const double g = scene_partial(B.part, ntiles, 1, par, t0, t1, lane);
+        const double d = scene_partial(B.part, ntiles, 0, par, t0, t1, lane);
+        if (i == 0) {
+            alpha = g / d;
+        } else {
+            const double gp = B.sc[2 * s + ((i - 1) & 1)], ap = B.sc[2 * ns + 2 * s + ((i - 1) & 1)];
+            beta = g / gp;
+            alpha = g / (d - beta * g / ap);
+        }
+        if (t == t0 && lane == 0) {
+            B.sc[2 * s + (i & 1)] = g;
+            B.sc[2 * ns + 2 * s + (i & 1)] = alpha;
+            B.cg_iters[s] = i + 1;
+        }
     }
-    double beta = 0.0;
-    if (it > 0) {
-        double rz_new = scene_partial(part, ntiles, 1, par, t0, t1, lane);
-        double rz_old = scene_partial(part, ntiles, 1, par ^ 1, t0, t1, lane);
-        beta = rz_new / rz_old;
-    }
-    if (t == t0 && lane == 0) cg_iters[s] = it + 1;
-    int b0 = tb0[t], nbt = tb1[t] - b0;
-    int lb = lane / 12, row = lane % 12;
-    bool on = lb < nbt;
-    double apv = 0.0, pv = 0.0;
+    const int b0 = tb0[t], nbt = tb1[t] - b0;
+    const int lb = lane / 12, row = lane % 12;
+    const bool on = lb < nbt;
+    const int bb = b0 + (on ? lb : 0);
+    const int idx = 12 * bb + row;
+    double rn = 0.0;
     if (on) {
-        int b = b0 + lb;
-        int idx = 12 * b + row;
-        pv = z[idx] + beta * p_old[idx];
-        p_new[idx] = pv;
-        const double* D = diag + (size_t)b * 144 + row * 12;
-        for (int c = 0; c < 12; ++c) apv += D[c] * (z[12 * b + c] + beta * p_old[12 * b + c]);
-        for (int e = row_ptr[b]; e < row_ptr[b + 1]; ++e) {
-            int col = opaque((int)(ent_key[e] & 0xFFFFFFFFu));
-            int v = ent_val[e];
-            if (col < 0 || col >= nb || (v >> 1) >= n_pairs || v < 0) {
-                atomicOr(err, ERR_CSR);
+        if (init) {
+            rn = B.b[idx];
+            B.x[idx] = 0.0;
+            B.p[idx] = 0.0;
+            B.sv[wr][idx] = 0.0;
+        } else {
+            const double pv = B.u[idx] + beta * B.p[idx];
+            B.p[idx] = pv;
+            const double sval = B.w[rd][idx] + beta * B.sv[rd][idx];
+            B.sv[wr][idx] = sval;
+            B.x[idx] += alpha * pv;
+            rn = pcg_r_next(B.r[rd][idx], B.w[rd][idx], B.sv[rd][idx], alpha, beta);
+        }
+        B.r[wr][idx] = rn;
+        vec[lb][row] = rn;
+    }
+    __syncthreads();
+    double un = 0.0;
+    if (on) {
+        const double* Pb = B.pinv + (size_t)bb * 144 + row * 12;
+#pragma unroll
+        for (int c = 0; c < 12; ++c) un += Pb[c] * vec[lb][c];
+        B.u[idx] = un;
+    }
+    __syncthreads();
+    if (on) vec[lb][row] = un;
+    __syncthreads();
+    double wv = 0.0;
+    if (on) {
+        const double* D = B.diag + (size_t)bb * 144 + row * 12;
+#pragma unroll
+        for (int c = 0; c < 12; ++c) wv += D[c] * vec[lb][c];
+    }
+    const int e0 = on ? B.row_ptr[bb] : 0, e1 = on ? B.row_ptr[bb + 1] : 0;
+    int mx = e1 - e0;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) mx = max(mx, __shfl_xor(mx, o, 64));
+    __syncthreads();
+    for (int q = 0; q < mx; ++q) {
+        const bool has = on && q < e1 - e0;
+        const double* Bk = nullptr;
+        double uc = 0.0;
+        if (has) {
+            const int e = e0 + q;
+            const int col = opaque((int)(B.ent_key[e] & 0xFFFFFFFFu));
+            const int v = B.ent_val[e];
+            if (col < 0 || col >= nb || v < 0 || (v >> 1) >= n_pairs) {
+                atomicOr(B.err, ERR_CSR);
+            } else {
+                Bk = (opaque(v & 1) ? B.offT : B.off) + (size_t)opaque(v >> 1) * 144 + row * 12;
+                const double* Pc = B.pinv + (size_t)col * 144 + row * 12;
+#pragma unroll
+                for (int c = 0; c < 12; ++c) {
+                    const int j = 12 * col + c;
+                    const double rj = init ? B.b[j] : pcg_r_next(B.r[rd][j], B.w[rd][j], B.sv[rd][j], alpha, beta);
+                    uc += Pc[c] * rj;
+                }
+            }
+        }
+        vec[lb][row] = uc;
+        __syncthreads();
+        if (Bk) {
+#pragma unroll
+            for (int c = 0; c < 12; ++c) wv += Bk[c] * vec[lb][c];
+        }
+        __syncthreads();
+    }
+    if (on) B.w[wr][idx] = wv;
+    const double g = wave_sum(on ? rn * un : 0.0), d = wave_sum(on ? wv * un : 0.0), rr = wave_sum(on ? rn * rn : 0.0);
+    if (lane == 0) {
+        B.part[(size_t)(0 * 2 + wr) * ntiles + t] = d;
+        B.part[(size_t)(1 * 2 + wr) * ntiles + t] = g;
+        B.part[(size_t)(2 * 2 + wr) * ntiles + t] = rr;
+    }
+}
+
+void launch_pcg_cg(hipStream_t s, int k, int ntiles, int ns, const int* tile_b0, const int* tile_b1,
+                   const int* tile_scene, const int* scene_tile0, const PcgBuf& B, double tol, int nb, int n_pairs) {
+    hipLaunchKernelGGL(k_pcg_cg, dim3(ntiles), dim3(64), 0, s, k, ntiles, ns, tile_b0, tile_b1, tile_scene, scene_tile0,
+                       B, tol, nb, n_pairs);
+}
+
+// ---------------------------------------------------------------------------------------------
+// The same PCG as one cooperative launch per solve.  Each workgroup keeps its tile's rows (x, p,
+// u, r, w, s, its rows of P and of the diagonal blocks) in registers across iterations and
+// publishes r, w, s and its partials with agent-scope atomic stores; neighbours and partial sums
+// are read with agent-scope atomic loads, so the exchange is coherent across XCDs.  Iterations are
+// separated by a grid barrier on a monotonic counter (the k-th barrier waits for k * grid
+// arrivals).  Every wait is bounded: a barrier that does not fill within the spin limit sets the
+// failure flag and every workgroup leaves, the host then reruns the solve with one launch per
+// iteration.  A scene is done (sticky, decided identically by all its tiles) once sqrt(rho) <=
+// tol * ||b||; the first tile of every done scene adds one to done_cnt[i % 3] at iteration i, and
+// the loop ends when that count reaches ns.  Slot (i + 1) % 3 is cleared by workgroup 0 during
+// iteration i, between its last read (before the previous barrier) and its next increment.
+// ---------------------------------------------------------------------------------------------
+#define PCG_SPIN_LIMIT (1 << 22)
+
+__device__ __forceinline__ double aload(const double* p) {
+    return __hip_atomic_load(const_cast<double*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void astore(double* p, double v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// grid barrier #k (k = 1, 2, ...); false if it timed out or another workgroup failed
+__device__ bool grid_barrier(unsigned* bar, int* fail, unsigned k, unsigned nblocks) {
+    __shared__ int ok_sh;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int ok = 1;
+        // every value another workgroup reads after this barrier went out through agent-scope
+        // atomic stores; wait for them, then arrive.  Readers use agent-scope atomic loads, so no
+        // cache maintenance is needed on either side.
+        __builtin_amdgcn_s_waitcnt(0);
+        __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned target = k * nblocks;
+        int spins = 0;
+        while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            if (__hip_atomic_load(fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) { ok = 0; break; }
+            if (++spins > PCG_SPIN_LIMIT) {
+                __hip_atomic_store(fail, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                ok = 0;
                 break;
             }
-            const double* Bk = (opaque(v & 1) ? offT : off) + (size_t)opaque(v >> 1) * 144 + row * 12;
-            for (int c = 0; c < 12; ++c) apv += Bk[c] * (z[12 * col + c] + beta * p_old[12 * col + c]);
+            __builtin_amdgcn_s_sleep(1);
         }
-        Ap[idx] = apv;
+        if (ok && __hip_atomic_load(fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) ok = 0;
+        ok_sh = ok;
     }
-    double pAp = wave_sum(on ? pv * apv : 0.0);
-    if (lane == 0) part[(size_t)(0 * 2 + par) * ntiles + t] = pAp;
-}
-
-void launch_pcg_A(hipStream_t s, int it, int ntiles, const int* tile_b0, const int* tile_b1, const int* tile_scene,
-                  const int* scene_tile0, const double* diag, const int* row_ptr, const uint64_t* ent_key,
-                  const int* ent_val, const double* off, const double* offT, const double* z, const double* p_old,
-                  double* p_new, double* Ap, double* part, const double* bnorm, double tol, int* cg_iters,
-                  int* cg_done, int nb, int n_pairs, int* err) {
-    hipLaunchKernelGGL(k_pcg_A, dim3(ntiles), dim3(64), 0, s, it, ntiles, tile_b0, tile_b1, tile_scene, scene_tile0,
-                       diag, row_ptr, ent_key, ent_val, off, offT, z, p_old, p_new, Ap, part, bnorm,
-                       tol, cg_iters, cg_done, nb, n_pairs, err);
-}
-
-// alpha = rz/pAp; x += alpha p; r -= alpha Ap; z = P r; partial r.z and r.r
-__global__ __launch_bounds__(64) void k_pcg_B(int it, int ntiles, const int* __restrict__ tb0, const int* __restrict__ tb1,
-                                               const int* __restrict__ tscene, const int* __restrict__ st0,
-                                               const double* __restrict__ pinv, const double* __restrict__ p,
-                                               const double* __restrict__ Ap, double* x, double* r, double* z,
-                                               double* part, const double* __restrict__ bnorm, double tol,
-                                               const int* __restrict__ cg_done) {
-    __shared__ double rs[60];
-    int t = blockIdx.x, lane = threadIdx.x;
-    int s = tscene[t];
-    int t0 = st0[s], t1 = st0[s + 1];
-    const int par = it & 1;
-    if (pcg_done(part, ntiles, par, t0, t1, lane, bnorm[s], tol, cg_done + s)) return;
-    double rz = scene_partial(part, ntiles, 1, par, t0, t1, lane);
-    double pAp = scene_partial(part, ntiles, 0, par, t0, t1, lane);
-    double alpha = rz / pAp;
-    int b0 = tb0[t], nbt = tb1[t] - b0;
-    int lb = lane / 12, row = lane % 12;
-    bool on = lb < nbt;
-    int idx = 12 * (b0 + lb) + row;
-    double rv = 0.0;
-    if (on) {
-        x[idx] += alpha * p[idx];
-        rv = r[idx] - alpha * Ap[idx];
-        r[idx] = rv;
-    }
-    if (lane < 60) rs[lane] = rv;
     __syncthreads();
-    double zv = 0.0;
-    if (on) {
-        const double* Pb = pinv + (size_t)(b0 + lb) * 144 + row * 12;
-        for (int c = 0; c < 12; ++c) zv += Pb[c] * rs[lb * 12 + c];
-        z[idx] = zv;
-    }
-    double prz = wave_sum(on ? rv * zv : 0.0), prr = wave_sum(on ? rv * rv : 0.0);
-    if (lane == 0) {
-        part[(size_t)(1 * 2 + (par ^ 1)) * ntiles + t] = prz;
-        part[(size_t)(2 * 2 + (par ^ 1)) * ntiles + t] = prr;
-    }
+    return ok_sh != 0;
 }
 
-void launch_pcg_B(hipStream_t s, int it, int ntiles, const int* tile_b0, const int* tile_b1, const int* tile_scene,
-                  const int* scene_tile0, const double* pinv, const double* p, const double* Ap, double* x, double* r,
-                  double* z, double* part, const double* bnorm, double tol, const int* cg_done) {
-    hipLaunchKernelGGL(k_pcg_B, dim3(ntiles), dim3(64), 0, s, it, ntiles, tile_b0, tile_b1, tile_scene, scene_tile0,
-                       pinv, p, Ap, x, r, z, part, bnorm, tol, cg_done);
+#define PCG_WAVES 4
+
+// LDS exchange among the lanes of one wave: order the wave's LDS accesses (no block barrier, the
+// waves of a workgroup run different tiles)
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__global__ __launch_bounds__(64 * PCG_WAVES) void k_pcg_persist(int ntiles, int ns, const int* __restrict__ tb0,
+                                                                 const int* __restrict__ tb1,
+                                                                 const int* __restrict__ tscene,
+                                                                 const int* __restrict__ st0, PcgBuf B, double tol,
+                                                                 int max_iter, int nb, int n_pairs) {
+    __shared__ double vec_all[PCG_WAVES][TILE_BODIES + 1][12];
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    const int t = blockIdx.x * PCG_WAVES + wave;
+    const bool tile = t < ntiles;  // waves past the last tile only take part in the barriers
+    double(*vec)[12] = vec_all[wave];
+    const unsigned nblocks = gridDim.x;
+    const int s = tile ? tscene[t] : 0;
+    const int t0 = tile ? st0[s] : 0, t1 = tile ? st0[s + 1] : 0;
+    const int b0 = tile ? tb0[t] : 0, nbt = tile ? tb1[t] - b0 : 0;
+    const int lb = lane / 12, row = lane % 12;
+    const bool on = lb < nbt;
+    const int bb = b0 + (on ? lb : 0);
+    const int idx = 12 * bb + row;
+    const bool lead = tile && (t == t0 && lane == 0);
+    // static rows of this lane: P and the diagonal block
+    double Prow[12], Drow[12];
+#pragma unroll
+    for (int c = 0; c < 12; ++c) {
+        Prow[c] = on ? B.pinv[(size_t)bb * 144 + row * 12 + c] : 0.0;
+        Drow[c] = on ? B.diag[(size_t)bb * 144 + row * 12 + c] : 0.0;
+    }
+    const int e0 = on ? B.row_ptr[bb] : 0, e1 = on ? B.row_ptr[bb + 1] : 0;
+    int mx = e1 - e0;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) mx = max(mx, __shfl_xor(mx, o, 64));
+
+    double x = 0.0, p = 0.0, u = 0.0, r = 0.0, w = 0.0, sv = 0.0;
+    double alpha = 0.0, beta = 0.0, g_prev = 1.0, a_prev = 1.0, bnorm = 0.0;
+    bool done = !tile;
+    unsigned nbar = 0;
+    for (int k = 0; k <= max_iter; ++k) {
+        const int wr = k & 1, rd = (k + 1) & 1;
+        const int i = k - 1;  // iteration index of this stage (-1: initial stage)
+        if (k > 0) {
+            // slot (i + 1) % 3 was last read before the previous barrier and is next counted at i + 1
+            if (blockIdx.x == 0 && threadIdx.x == 0)
+                __hip_atomic_store(B.done_cnt + ((i + 1) % 3), 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        if (k > 0 && tile) {
+            const int par = (k - 1) & 1;
+            // the three partial sums of the scene in one pass over its tiles
+            double a_rho = 0.0, a_g = 0.0, a_d = 0.0;
+            for (int tt = t0 + lane; tt < t1; tt += 64) {
+                a_d += aload(B.part + (size_t)(0 * 2 + par) * ntiles + tt);
+                a_g += aload(B.part + (size_t)(1 * 2 + par) * ntiles + tt);
+                a_rho += aload(B.part + (size_t)(2 * 2 + par) * ntiles + tt);
+            }
+            const double rho = wave_sum(a_rho), g = wave_sum(a_g), d = wave_sum(a_d);
+            if (k == 1) {
+                bnorm = sqrt(rho);
+                if (lead) { B.bnorm_out[s] = bnorm; B.residual_out[s] = bnorm; }
+            }
+            if (!done && (bnorm == 0.0 || sqrt(rho) <= tol * bnorm)) done = true;
+            if (done && lead) atomicAdd(B.done_cnt + (i % 3), 1);
+            if (!done) {
+                if (i == 0) {
+                    beta = 0.0;
+                    alpha = g / d;
+                } else {
+                    beta = g / g_prev;
+                    alpha = g / (d - beta * g / a_prev);
+                }
+                g_prev = g;
+                a_prev = alpha;
+                if (lead) B.cg_iters[s] = i + 1;
+            }
+        }
+        if (!done) {
+            double rn = 0.0, un = 0.0, wv = 0.0;
+            if (k == 0) {
+                rn = on ? B.b[idx] : 0.0;
+            } else {
+                p = u + beta * p;
+                rn = pcg_r_next(r, w, sv, alpha, beta);  // r_i - alpha (w_i + beta s_{i-1})
+                sv = w + beta * sv;                       // s_i
+                x += alpha * p;
+            }
+            if (on) vec[lb][row] = rn;
+            wave_lds_sync();
+            if (on) {
+#pragma unroll
+                for (int c = 0; c < 12; ++c) un += Prow[c] * vec[lb][c];
+            }
+            wave_lds_sync();
+            if (on) vec[lb][row] = un;
+            wave_lds_sync();
+            if (on) {
+#pragma unroll
+                for (int c = 0; c < 12; ++c) wv += Drow[c] * vec[lb][c];
+            }
+            wave_lds_sync();
+            for (int q = 0; q < mx; ++q) {
+                const bool has = on && q < e1 - e0;
+                const double* Bk = nullptr;
+                double uc = 0.0;
+                if (has) {
+                    const int e = e0 + q;
+                    const int col = opaque((int)(B.ent_key[e] & 0xFFFFFFFFu));
+                    const int v = B.ent_val[e];
+                    if (col < 0 || col >= nb || v < 0 || (v >> 1) >= n_pairs) {
+                        atomicOr(B.err, ERR_CSR);
+                    } else {
+                        Bk = (opaque(v & 1) ? B.offT : B.off) + (size_t)opaque(v >> 1) * 144 + row * 12;
+                        const double* Pc = B.pinv + (size_t)col * 144 + row * 12;
+#pragma unroll
+                        for (int c = 0; c < 12; ++c) {
+                            const int j = 12 * col + c;
+                            const double rj = (k == 0) ? B.b[j]
+                                                       : pcg_r_next(aload(B.r[rd] + j), aload(B.w[rd] + j),
+                                                                    aload(B.sv[rd] + j), alpha, beta);
+                            uc += Pc[c] * rj;
+                        }
+                    }
+                }
+                vec[lb][row] = uc;
+                wave_lds_sync();
+                if (Bk) {
+#pragma unroll
+                    for (int c = 0; c < 12; ++c) wv += Bk[c] * vec[lb][c];
+                }
+                wave_lds_sync();
+            }
+            // publish this stage's r, w and s for the neighbours of the next stage
+            if (on) {
+                astore(B.r[wr] + idx, rn);
+                astore(B.w[wr] + idx, wv);
+                astore(B.sv[wr] + idx, sv);
+            }
+            r = rn;
+            u = un;
+            w = wv;
+            const double gp = wave_sum(on ? rn * un : 0.0), dp = wave_sum(on ? wv * un : 0.0),
+                         rr = wave_sum(on ? rn * rn : 0.0);
+            if (lane == 0) {
+                astore(B.part + (size_t)(0 * 2 + wr) * ntiles + t, dp);
+                astore(B.part + (size_t)(1 * 2 + wr) * ntiles + t, gp);
+                astore(B.part + (size_t)(2 * 2 + wr) * ntiles + t, rr);
+            }
+        }
+        // each wave's stores are complete before the workgroup arrives
+        __builtin_amdgcn_s_waitcnt(0);
+        if (!grid_barrier(B.bar, B.fail, ++nbar, nblocks)) break;
+        if (k > 0 && __hip_atomic_load(B.done_cnt + ((k - 1) % 3), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= ns)
+            break;
+    }
+    if (on) B.x[idx] = x;
+}
+
+hipError_t launch_pcg_persist(hipStream_t s, int ntiles, int ns, const int* tile_b0, const int* tile_b1,
+                              const int* tile_scene, const int* scene_tile0, const PcgBuf& B, double tol, int max_iter,
+                              int nb, int n_pairs) {
+    void* args[] = {(void*)&ntiles, (void*)&ns, (void*)&tile_b0, (void*)&tile_b1, (void*)&tile_scene,
+                    (void*)&scene_tile0, (void*)&B, (void*)&tol, (void*)&max_iter, (void*)&nb, (void*)&n_pairs};
+    const int blocks = (ntiles + PCG_WAVES - 1) / PCG_WAVES;
+    return hipLaunchCooperativeKernel((const void*)k_pcg_persist, dim3(blocks), dim3(64 * PCG_WAVES), args, 0, s);
 }
 
 // any scene still iterating after `it` completed iterations (residual of parity it&1)

@@ -154,7 +154,7 @@ inline unsigned long long max(unsigned long long a, unsigned long long b) { retu
 
 // ---- runtime API ----------------------------------------------------------------------------
 typedef int hipError_t;
-enum { hipSuccess = 0, hipErrorInvalidValue = 1, hipErrorInvalidConfiguration = 9 };
+enum { hipSuccess = 0, hipErrorInvalidValue = 1, hipErrorInvalidConfiguration = 9, hipErrorNotSupported = 801 };
 typedef void* hipStream_t;
 typedef void* hipEvent_t;
 enum hipMemcpyKind { hipMemcpyHostToHost, hipMemcpyHostToDevice, hipMemcpyDeviceToHost, hipMemcpyDeviceToDevice, hipMemcpyDefault };
@@ -164,6 +164,17 @@ inline thread_local hipError_t emu_last_error = hipSuccess;
 inline const char* hipGetErrorString(hipError_t e) { return e ? "emulated error" : "no error"; }
 inline const char* hipGetErrorName(hipError_t e) { return e ? "hipErrorEmulated" : "hipSuccess"; }
 inline hipError_t hipGetLastError() { hipError_t e = emu_last_error; emu_last_error = hipSuccess; return e; }
+// scoped atomics and cooperative launch: the emulation runs blocks one after another, so a
+// cooperative (grid-synchronising) launch is reported as unsupported and callers fall back
+#define __HIP_MEMORY_SCOPE_AGENT 4
+template <class T> inline T __hip_atomic_load(T* p, int, int) { T r; __atomic_load(p, &r, __ATOMIC_SEQ_CST); return r; }
+template <class T> inline void __hip_atomic_store(T* p, T v, int, int) { __atomic_store(p, &v, __ATOMIC_SEQ_CST); }
+template <class T, class U> inline T __hip_atomic_fetch_add(T* p, U v, int, int) { return __atomic_fetch_add(p, (T)v, __ATOMIC_SEQ_CST); }
+inline void __builtin_amdgcn_s_sleep(int) {}
+inline void __builtin_amdgcn_s_waitcnt(int) {}
+inline void __builtin_amdgcn_wave_barrier() {}
+inline void __builtin_amdgcn_fence(int, const char*) {}
+inline hipError_t hipLaunchCooperativeKernel(const void*, dim3, dim3, void**, unsigned, hipStream_t) { return hipErrorNotSupported; }
 inline hipError_t hipMalloc(void** p, size_t n) { *p = std::malloc(n); if (*p) std::memset(*p, 0xA5, n); return *p ? hipSuccess : 2; }
 template <class T> inline hipError_t hipMalloc(T** p, size_t n) { return hipMalloc((void**)p, n); }
 inline hipError_t hipFree(void* p) { std::free(p); return hipSuccess; }
