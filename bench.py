@@ -66,6 +66,7 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=2)
+    ap.add_argument("--no-autotune", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build targets one GPU; --gpus must be 1 (multi-GPU sharding is design-only)"}))
@@ -79,6 +80,15 @@ def main():
     for k in range(args.warmup):
         st = solver.step()[0]
         print(f"# warmup {k}: {st}", file=sys.stderr, flush=True)
+    tune = None
+    if not args.no_autotune:
+        # choose PCG tile size and launch form at the warmed-up state (outside the timed region)
+        st_keep = solver.get_state()
+        bpt, multi, ms = solver.autotune(repeats=3)
+        solver.set_state(*st_keep)
+        tune = {"bodies_per_tile": bpt, "per_iteration_launch": bool(multi),
+                "ms_per_solve": [[round(float(v), 4) for v in row] for row in ms]}
+        print(f"# autotune: {tune}", file=sys.stderr, flush=True)
     state0 = solver.get_state()
     iters, cg = [], []
     t0 = time.perf_counter()
@@ -126,7 +136,7 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic (procedural scene)", "config": cfg,
         "newton_iters_per_step": round(float(np.mean(iters)), 3), "cg_iters_per_step": round(float(np.mean(cg)), 1),
-        "method": "plain",
+        "method": "plain", "autotune": tune,
         "roofline": roof, "cpu_baseline": cpu,
     }
     print(json.dumps(result), flush=True)

@@ -34,6 +34,7 @@ SIGNATURES = {
     "abd_get_contact_points": (C.c_int, [_V, _D, _D, C.c_int, _I]),
     "abd_get_kernel_times": (C.c_int, [_V, C.c_char_p, C.c_int, _D, _I, _D, C.c_int, _I]),
     "abd_enable_timing": (C.c_int, [_V, C.c_int]),
+    "abd_autotune": (C.c_int, [_V, C.c_int, _I, _I, _D]),
 }
 
 _lib = None

@@ -232,6 +232,15 @@ class NewtonAbd:
         return mid[:3 * k].reshape(-1, 3), np.abs(d - self.params.dhat)
 
     # -- per-kernel timing (HIP events) -------------------------------------------------------------
+    def autotune(self, repeats=3):
+        """Pick the PCG tile size and launch form by timing solves at the current state (no
+        counterpart in the reference).  Returns (bodies_per_tile, per_iteration_launch, ms[5, 2])."""
+        bpt, multi = C.c_int(0), C.c_int(0)
+        ms = np.zeros(10)
+        _lib.check(_lib.load().abd_autotune(self._h, int(repeats), C.byref(bpt), C.byref(multi), dptr(ms)), self._h,
+                   "abd_autotune")
+        return bpt.value, multi.value, ms.reshape(5, 2)
+
     def enable_timing(self, on=True):
         _lib.check(_lib.load().abd_enable_timing(self._h, int(on)), self._h, "abd_enable_timing")
 

@@ -114,6 +114,8 @@ struct AbdCtx {
     unsigned* pcg_ctl = nullptr;  // persistent solve: [0] barrier count, [1..3] done counters, [4] failure flag
     unsigned* h_pcg_ctl = nullptr;
     bool coop_ok = true;          // cleared when a cooperative launch is refused
+    bool pcg_multi = false;       // solve with one launch per iteration (autotuner choice)
+    int bodies_per_tile = 5;
     int* cg_iters = nullptr;
     int* cg_done = nullptr;
     // per-scene control
@@ -538,7 +540,7 @@ int assemble(AbdCtx* ctx) {
 // a grid barrier timed out, in which case the caller solves again with one launch per iteration
 int solve_persistent(AbdCtx* ctx, const PcgBuf& B, bool* ok) {
     *ok = false;
-    if (!ctx->coop_ok || (ctx->P.reserved0 & 8)) return ABD_OK;
+    if (!ctx->coop_ok || ctx->pcg_multi || (ctx->P.reserved0 & 8)) return ABD_OK;
     HIPCHK(hipMemsetAsync(ctx->cg_iters, 0, sizeof(int) * ctx->ns, ctx->st));
     HIPCHK(hipMemsetAsync(ctx->pcg_ctl, 0, 8 * sizeof(unsigned), ctx->st));
     tic(ctx, "pcg_solve");
@@ -734,6 +736,44 @@ int step_once(AbdCtx* ctx) {
 }
 
 int check_ctx(AbdCtx* ctx) { return ctx ? ABD_OK : ABD_ERR_ARG; }
+
+// --------------------------------------------------------------------------------------------
+// PCG tiles: runs of at most `bpt` consecutive bodies of one scene (one wave each, 12 lanes per
+// body); the per-tile partial sums are sized with them
+// --------------------------------------------------------------------------------------------
+int set_tiles(AbdCtx* ctx, int bpt) {
+    if (bpt < 1 || bpt > 5) {
+        ctx->err = "bodies per PCG tile must be in [1, 5]";
+        return ABD_ERR_ARG;
+    }
+    const int ns = ctx->ns;
+    const std::vector<int>& sb = ctx->h_scene_b0;
+    std::vector<int> tb0, tb1, tsc, st0(ns + 1, 0);
+    for (int s = 0; s < ns; ++s) {
+        st0[s] = (int)tb0.size();
+        for (int b = sb[s]; b < sb[s + 1]; b += bpt) {
+            tb0.push_back(b);
+            tb1.push_back(std::min(b + bpt, sb[s + 1]));
+            tsc.push_back(s);
+        }
+    }
+    st0[ns] = (int)tb0.size();
+    if (ctx->st) HIPCHK(hipStreamSynchronize(ctx->st));
+    CHK(dfree(ctx, ctx->tile_b0)); CHK(dfree(ctx, ctx->tile_b1)); CHK(dfree(ctx, ctx->tile_scene));
+    CHK(dfree(ctx, ctx->scene_tile0)); CHK(dfree(ctx, ctx->part));
+    const int nt = (int)tb0.size();
+    CHK(dalloc(ctx, &ctx->tile_b0, nt)); CHK(dalloc(ctx, &ctx->tile_b1, nt)); CHK(dalloc(ctx, &ctx->tile_scene, nt));
+    CHK(dalloc(ctx, &ctx->scene_tile0, ns + 1)); CHK(dalloc(ctx, &ctx->part, 6 * (size_t)nt));
+    HIPCHK(hipMemcpy(ctx->tile_b0, tb0.data(), sizeof(int) * nt, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->tile_b1, tb1.data(), sizeof(int) * nt, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->tile_scene, tsc.data(), sizeof(int) * nt, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->scene_tile0, st0.data(), sizeof(int) * (ns + 1), hipMemcpyHostToDevice));
+    ctx->ntiles = nt;
+    ctx->h_tile_b0 = tb0;
+    ctx->h_scene_tile0 = st0;
+    ctx->bodies_per_tile = bpt;
+    return ABD_OK;
+}
 
 // --------------------------------------------------------------------------------------------
 // rest-frame BVHs (host, once per distinct mesh)
@@ -984,21 +1024,6 @@ int abd_create(const AbdSceneDesc* d, const AbdParams* p, AbdCtx** out) {
         }
     }
     ctx->n_meshes = (int)mesh_rep.size();
-    // tiles of at most 5 bodies inside one scene
-    std::vector<int> tb0, tb1, tsc, st0(ns + 1, 0);
-    for (int s = 0; s < ns; ++s) {
-        st0[s] = (int)tb0.size();
-        for (int b = sb[s]; b < sb[s + 1]; b += 5) {
-            tb0.push_back(b);
-            tb1.push_back(std::min(b + 5, sb[s + 1]));
-            tsc.push_back(s);
-        }
-    }
-    st0[ns] = (int)tb0.size();
-    ctx->ntiles = (int)tb0.size();
-    ctx->h_tile_b0 = tb0;
-    ctx->h_scene_tile0 = st0;
-
     {
         hipError_t e = hipStreamCreateWithFlags(&ctx->st, hipStreamNonBlocking);
         if (e != hipSuccess) {
@@ -1020,10 +1045,6 @@ int abd_create(const AbdSceneDesc* d, const AbdParams* p, AbdCtx** out) {
     A(scene_v0, ns + 1); U(scene_v0, sv.data(), ns + 1);
     A(scene_e0, ns + 1); U(scene_e0, se.data(), ns + 1);
     A(scene_t0, ns + 1); U(scene_t0, stt.data(), ns + 1);
-    A(tile_b0, ctx->ntiles); U(tile_b0, tb0.data(), ctx->ntiles);
-    A(tile_b1, ctx->ntiles); U(tile_b1, tb1.data(), ctx->ntiles);
-    A(tile_scene, ctx->ntiles); U(tile_scene, tsc.data(), ctx->ntiles);
-    A(scene_tile0, ns + 1); U(scene_tile0, st0.data(), ns + 1);
     A(q, 12 * nb); A(qnow, 12 * nb); A(qdot, 12 * nb); A(qtilde, 12 * nb); A(qtmp, 12 * nb);
     A(du, 12 * nb); A(grad, 12 * nb); A(diag, 144 * nb); A(pinv, 144 * nb); A(e_body, nb);
     A(X0, nv); A(X1, nv); A(Xt, nv);
@@ -1060,7 +1081,8 @@ int abd_create(const AbdSceneDesc* d, const AbdParams* p, AbdCtx** out) {
     A(cand_lo, ns); A(cand_hi, ns);
     A(r, 12 * nb); A(z, 12 * nb); A(p0, 12 * nb); A(p1, 12 * nb); A(Ap, 12 * nb);
     A(r1, 12 * nb); A(w1, 12 * nb); A(s0, 12 * nb); A(pcg_sc, 4 * ns); A(pcg_ctl, 8);
-    A(part, 6 * ctx->ntiles); A(bnorm, ns); A(cg_iters, ns); A(cg_done, ns);
+    if ((rc = set_tiles(ctx, 5)) != ABD_OK) return fail(rc);
+    A(bnorm, ns); A(cg_iters, ns); A(cg_done, ns);
     A(C.newton_active, ns); A(C.residual, ns); A(C.alpha, ns); A(C.alpha0, ns); A(C.amax_bits, ns);
     A(C.ls_done, ns); A(C.ls_floor, ns); A(C.E0, ns); A(C.E1, ns);
     A(nact_s, ns); A(act_before, ns); A(report, R_N * ns); A(cnt, C_N); A(row_ptr, nb + 1);
@@ -1375,6 +1397,44 @@ int abd_get_contact_points(AbdCtx* ctx, double* midpoints, double* dist, int cap
     HIPCHK(hipStreamSynchronize(ctx->st));
     CHK(dfree(ctx, dm));
     CHK(dfree(ctx, dd));
+    return ABD_OK;
+}
+
+int abd_autotune(AbdCtx* ctx, int repeats, int* bodies_per_tile, int* per_iteration_launch, double* ms) {
+    CHK(check_ctx(ctx));
+    if (repeats < 1) repeats = 1;
+    CHK(fill_i32(ctx, ctx->C.newton_active, ctx->ns, 1));
+    CHK(static_broad_phase(ctx));
+    CHK(assemble(ctx));
+    double best = -1.0;
+    int best_bpt = ctx->bodies_per_tile, best_multi = ctx->pcg_multi ? 1 : 0;
+    for (int bpt = 1; bpt <= 5; ++bpt) {
+        CHK(set_tiles(ctx, bpt));
+        for (int multi = 0; multi < 2; ++multi) {
+            const int slot = 2 * (bpt - 1) + multi;
+            if (ms) ms[slot] = -1.0;
+            if (!multi && !ctx->coop_ok) continue;
+            ctx->pcg_multi = multi != 0;
+            CHK(solve(ctx));  // warm-up (and the cooperative-launch probe)
+            HIPCHK(hipStreamSynchronize(ctx->st));
+            if (!multi && !ctx->coop_ok) continue;
+            auto t0 = std::chrono::steady_clock::now();
+            for (int r = 0; r < repeats; ++r) CHK(solve(ctx));
+            HIPCHK(hipStreamSynchronize(ctx->st));
+            const double t = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / repeats;
+            if (ms) ms[slot] = t;
+            if (best < 0.0 || t < best) {
+                best = t;
+                best_bpt = bpt;
+                best_multi = multi;
+            }
+        }
+    }
+    CHK(set_tiles(ctx, best_bpt));
+    ctx->pcg_multi = best_multi != 0;
+    if (bodies_per_tile) *bodies_per_tile = best_bpt;
+    if (per_iteration_launch) *per_iteration_launch = best_multi;
+    ctx->assembled = ctx->solved = false;
     return ABD_OK;
 }
 

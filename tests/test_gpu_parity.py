@@ -184,3 +184,54 @@ def test_deterministic_repeat():
     for k in range(10):
         a.step(), b.step()
     np.testing.assert_array_equal(a.get_state()[0], b.get_state()[0])
+
+
+def lowered(sc, gap=5e-4):
+    """Shift every body down so the lowest vertex sits `gap` above the ground plane y = 0."""
+    q = sc.q0.copy()
+    qb = q[sc.vert_body]
+    xr = sc.rest_x
+    y = qb[:, 1] + xr[:, 0] * qb[:, 4] + xr[:, 1] * qb[:, 7] + xr[:, 2] * qb[:, 10]
+    sc.q0[:, 1] -= y.min() - gap
+    return sc
+
+
+@pytest.mark.parametrize("case", ["hull_batch", "box_pile"])
+def test_batched_and_box_scenes_match_oracle(O, case):
+    """Batched hull piles of uneven body counts (one context, several scenes) and a box pile, on the
+    ground: trajectories after 6 steps."""
+    if case == "hull_batch":
+        sc = lowered(S.batched_hull_piles(n_scenes=4, lo=4, hi=12))
+        p = north_star(dt=4e-3, ground_enabled=True, k_contact=1e14, newton_tol=1e-9, cg_tol=1e-12, cg_max_iter=2000)
+    else:
+        sc = lowered(S.box_pile(n=27))
+        p = north_star(dt=8e-3, ground_enabled=True, k_contact=1e14, newton_tol=1e-9, cg_tol=1e-12, cg_max_iter=2000)
+    say(f"{case}: {sc.n_bodies} bodies in {sc.n_scenes} scenes")
+    g, o = solver(sc, p), O.Oracle(sc, p)
+    for k in range(6):
+        sg, so = g.step(), o.step()
+        err = rel(g.get_state()[0], o.get_state()[0])
+        say(f"  step {k}: rel {err:.2e} newton {[s['newton_iters'] for s in sg]} active {[s['n_active'] for s in sg]}")
+        assert err < 1e-6
+        assert [s["n_active"] for s in sg] == [s["n_active"] for s in so]
+
+
+def test_pcg_launch_forms_and_autotune_agree():
+    """One cooperative launch per solve, one launch per iteration, and every autotuned tile size
+    give the same solution (to the PCG tolerance)."""
+    sc, p = CASES["cubes_ground"]()
+    q, qd = perturbed(sc, 5, amp=1e-4)
+    g = solver(sc, p)
+    g.set_state(q, sc.q0, qd)
+    g.compute_du()
+    du_a = g._du.copy()
+    g2 = solver(sc, p.replace(debug=8))
+    g2.set_state(q, sc.q0, qd)
+    g2.compute_du()
+    assert rel(g2._du, du_a) < 1e-8
+    bpt, multi, ms = g.autotune(repeats=2)
+    say(f"autotune: {bpt} bodies per tile, per-iteration launch {multi}, ms {ms.tolist()}")
+    assert 1 <= bpt <= 5 and multi in (0, 1) and ms[bpt - 1, multi] > 0
+    g.set_state(q, sc.q0, qd)
+    g.compute_du()
+    assert rel(g._du, du_a) < 1e-8
