@@ -412,7 +412,8 @@ int broad_phase(AbdCtx* ctx, const double4* X0, const double4* X1, const double*
     CHK(dbg(ctx, "launch_world_nodes"));
     toc(ctx);
     for (int attempt = 0; attempt < 8; ++attempt) {
-        tic(ctx, "broad_phase_query");
+        // algorithmic bytes: positions once, primitive indices once, candidate keys out
+        tic(ctx, "broad_phase_query", (double)ctx->nv * 24 + (2.0 * ctx->ne + 3.0 * ctx->nt) * 4);
         HIPCHK(hipMemsetAsync(ctx->cnt + C_COUNT, 0, 2 * sizeof(int), ctx->st));
         launch_pair_queries(ctx->st, ctx->G, ctx->M, ctx->n_bpairs, ctx->max_queries, ctx->bpairs, X0, X1,
                             ctx->body_box, infl, ctx->cand, ctx->cand_cap, ctx->cnt + C_COUNT, ctx->cnt + C_OVERFLOW);
@@ -554,6 +555,15 @@ int solve_persistent(AbdCtx* ctx, const PcgBuf& B, bool* ok) {
     }
     CHK(dbg(ctx, "launch_pcg_persist"));
     HIPCHK(hipMemcpyAsync(ctx->h_pcg_ctl, ctx->pcg_ctl, 8 * sizeof(unsigned), hipMemcpyDeviceToHost, ctx->st));
+    if (ctx->T.on) {
+        // algorithmic bytes of the launch: K iterations x [nnzb * 1156 + nb * (8 * 96 + 1152)]
+        std::vector<int> it(ctx->ns);
+        HIPCHK(hipMemcpyAsync(it.data(), ctx->cg_iters, sizeof(int) * ctx->ns, hipMemcpyDeviceToHost, ctx->st));
+        HIPCHK(hipStreamSynchronize(ctx->st));
+        const int K = it.empty() ? 0 : *std::max_element(it.begin(), it.end());
+        const double nnzb = (double)ctx->nb + 2.0 * ctx->n_pairs;
+        if (!ctx->T.open.empty()) ctx->T.open.back().bytes = (double)K * (nnzb * 1156.0 + (double)ctx->nb * (8 * 96 + 1152));
+    }
     HIPCHK(hipStreamSynchronize(ctx->st));
     if (ctx->h_pcg_ctl[4]) {
         ctx->coop_ok = false;  // a grid barrier did not fill: do not try again on this context

@@ -66,8 +66,8 @@ __global__ __launch_bounds__(SORT_T) void k_scan_block_sums(int nb, int* bsum) {
 }
 
 // bsum == nullptr: a single tile (n <= SORT_TILE), offset 0
-__global__ __launch_bounds__(SORT_T) void k_scan_apply(int n, const int* __restrict__ in, const int* __restrict__ bsum,
-                                                        int* __restrict__ out) {
+// `in` and `out` may be the same array (each element is read before the thread writes it)
+__global__ __launch_bounds__(SORT_T) void k_scan_apply(int n, const int* in, const int* __restrict__ bsum, int* out) {
     __shared__ int sh[SORT_T];
     const int base = blockIdx.x * SORT_TILE + threadIdx.x * SORT_I;
     int v[SORT_I];
@@ -83,19 +83,27 @@ __global__ __launch_bounds__(SORT_T) void k_scan_apply(int n, const int* __restr
     }
 }
 
-// scratch: bsum needs cdiv(n, SORT_TILE) ints
-int scan_scratch_ints(long n) { return cdivs(n, SORT_TILE) + 1; }
+// scratch: the tile sums of every level (cdiv(n, SORT_TILE), then recursively of those)
+int scan_scratch_ints(long n) {
+    const long nb = cdivs(n, SORT_TILE);
+    return (int)(nb + 1 + (nb > SORT_TILE ? scan_scratch_ints(nb) : 0));
+}
 
+// exclusive scan of n ints; levels above the first are scanned recursively (in place)
 hipError_t excl_scan_int(hipStream_t s, const int* in, int* out, int n, int* bsum) {
     if (n <= 0) return hipSuccess;
-    int nb = cdivs(n, SORT_TILE);
-    if (nb > SORT_TILE) return hipErrorInvalidValue;
+    const int nb = cdivs(n, SORT_TILE);
     if (nb == 1) {
         hipLaunchKernelGGL(k_scan_apply, dim3(1), dim3(SORT_T), 0, s, n, in, (const int*)nullptr, out);
         return hipGetLastError();
     }
     hipLaunchKernelGGL(k_scan_tile_sums, dim3(nb), dim3(SORT_T), 0, s, n, in, bsum);
-    hipLaunchKernelGGL(k_scan_block_sums, dim3(1), dim3(SORT_T), 0, s, nb, bsum);
+    if (nb <= SORT_TILE) {
+        hipLaunchKernelGGL(k_scan_block_sums, dim3(1), dim3(SORT_T), 0, s, nb, bsum);
+    } else {
+        hipError_t e = excl_scan_int(s, bsum, bsum, nb, bsum + nb + 1);
+        if (e != hipSuccess) return e;
+    }
     hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(SORT_T), 0, s, n, in, bsum, out);
     return hipGetLastError();
 }
@@ -198,7 +206,6 @@ hipError_t radix_sort64(hipStream_t s, const uint64_t* kin, uint64_t* kout, cons
                         int end_bit, void* scratch) {
     if (n <= 0) return hipSuccess;
     const int nblk = cdivs(n, SORT_TILE);
-    if ((long)RADIX * nblk > (long)SORT_TILE * SORT_TILE) return hipErrorInvalidValue;
     char* p = (char*)scratch;
     uint64_t* ktmp = (uint64_t*)p;
     p += (size_t)n * 8;

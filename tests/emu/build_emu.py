@@ -30,6 +30,18 @@ def build(asan=True):
     return LIB
 
 
+def build_sort_driver():
+    """Unit test of the radix sort and the multi-level scan (tests/emu/sort_driver.cpp)."""
+    os.makedirs(OUT, exist_ok=True)
+    exe = os.path.join(OUT, "sort_driver")
+    flags = [f for f in FLAGS if f != "-fsanitize=address"] + ["-I", HERE]
+    r = subprocess.run(["g++", *flags, "-x", "c++", os.path.join(CSRC, "abd_sort.hip"), "-x", "c++",
+                        os.path.join(HERE, "sort_driver.cpp"), "-o", exe], capture_output=True, text=True)
+    if r.returncode:
+        raise RuntimeError(r.stderr[-3000:])
+    return exe
+
+
 def asan_runtime():
     return subprocess.run(["g++", "-print-file-name=libasan.so"], capture_output=True, text=True).stdout.strip()
 

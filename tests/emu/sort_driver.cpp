@@ -58,5 +58,22 @@ int main() {
             hipFree(dk); hipFree(dko); hipFree(dv); hipFree(dvo); hipFree(tmp);
         }
     }
+    // multi-level scan: more than SORT_TILE^2 elements
+    {
+        const int n = 2048 * 2048 + 4099;
+        std::vector<int> a(n), out(n);
+        for (int i = 0; i < n; ++i) a[i] = (int)(rng() % 3);
+        int *din, *dout, *tmp;
+        hipMalloc(&din, (size_t)n * 4); hipMalloc(&dout, (size_t)n * 4);
+        hipMalloc(&tmp, sizeof(int) * (size_t)abd::scan_scratch_ints(n));
+        hipMemcpy(din, a.data(), (size_t)n * 4, hipMemcpyHostToDevice);
+        bool ok = abd::excl_scan_int(nullptr, din, dout, n, tmp) == hipSuccess;
+        hipMemcpy(out.data(), dout, (size_t)n * 4, hipMemcpyDeviceToHost);
+        long acc = 0;
+        for (int i = 0; i < n && ok; ++i) { ok &= out[i] == acc; acc += a[i]; }
+        std::printf("scan n=%d (three levels) %s\n", n, ok ? "ok" : "FAIL");
+        fails += !ok;
+        hipFree(din); hipFree(dout); hipFree(tmp);
+    }
     return fails;
 }
