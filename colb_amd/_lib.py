@@ -1,4 +1,5 @@
-"""ctypes binding of colb_amd/lib/libabd_hip.so (the C ABI of include/abd_hip.h).
+"""ctypes binding of colb_amd/lib/libabd_hip.so (the C ABI of include/abd_hip.h and
+include/abd_mlp.h).
 
 There is no fallback: if the library is missing the import of the solver fails loudly with the
 build command to run.
@@ -11,8 +12,8 @@ from ._abi import AbdParams, AbdSceneDesc, AbdStepStats
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "libabd_hip.so")
 
-# exported symbol -> (restype, argtypes); must match include/abd_hip.h
-_D, _I, _V = C.POINTER(C.c_double), C.POINTER(C.c_int), C.c_void_p
+# exported symbol -> (restype, argtypes); must match include/abd_hip.h and include/abd_mlp.h
+_D, _I, _V, _F = C.POINTER(C.c_double), C.POINTER(C.c_int), C.c_void_p, C.POINTER(C.c_float)
 SIGNATURES = {
     "abd_create": (C.c_int, [C.POINTER(AbdSceneDesc), C.POINTER(AbdParams), C.POINTER(_V)]),
     "abd_destroy": (None, [_V]),
@@ -35,6 +36,19 @@ SIGNATURES = {
     "abd_get_kernel_times": (C.c_int, [_V, C.c_char_p, C.c_int, _D, _I, _D, C.c_int, _I]),
     "abd_enable_timing": (C.c_int, [_V, C.c_int]),
     "abd_autotune": (C.c_int, [_V, C.c_int, _I, _I, _D]),
+    # include/abd_mlp.h
+    "abd_mlp_create": (C.c_int, [_I, C.c_int, C.POINTER(_V)]),
+    "abd_mlp_destroy": (None, [_V]),
+    "abd_mlp_last_error": (C.c_char_p, [_V]),
+    "abd_mlp_num_params": (C.c_int, [_V]),
+    "abd_mlp_set_params": (C.c_int, [_V, _F]),
+    "abd_mlp_get_params": (C.c_int, [_V, _F]),
+    "abd_mlp_set_loss_weights": (C.c_int, [_V, _F]),
+    "abd_mlp_forward": (C.c_int, [_V, _F, C.c_int, _F]),
+    "abd_mlp_gradient": (C.c_int, [_V, _F, _F, C.c_int, _F, _F]),
+    "abd_mlp_train": (C.c_int, [_V, _F, _F, C.c_int, _I, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float,
+                                C.c_float, _F]),
+    "abd_mlp_gemm_test": (C.c_int, [_V, C.c_int, C.c_int, C.c_int, _F, _F, _F]),
 }
 
 _lib = None

@@ -1,0 +1,129 @@
+"""Learned warm start of the ABD Newton step (no counterpart in the reference).
+
+A per-body MLP maps body features to the body's Newton step du (the 12 DOFs of the PCG
+solution), which then seeds the PCG as its initial guess.  The model runs on the GPU through
+the C ABI of include/abd_mlp.h: BF16 weights read by the matrix cores with FP32 accumulation,
+FP32 master weights trained by Adam, forward and backward written by hand (abd_mlp.hip).
+
+Features per body (37, standardised with the training set's mean and standard deviation):
+    q - q~ (12)          the DOFs relative to the inertial target q~ = q_now + dt qdot + dt^2 g
+    qdot (12)            the DOF velocities
+    n_contacts (1)       active contacts incident to the body
+    g_contact (12)       the summed barrier gradient on the body
+Target per body (12): du, standardised the same way.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._abi import dptr
+
+N_FEATURES = 37
+N_OUT = 12
+DEFAULT_DIMS = (N_FEATURES, 128, 128, 128, N_OUT)
+
+
+def _fptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _iptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def init_params(dims, seed=0):
+    """He-normal weights, zero biases (float32, flat layout of include/abd_mlp.h)."""
+    rng = np.random.default_rng(seed)
+    parts = []
+    for l in range(len(dims) - 1):
+        parts.append((rng.normal(size=(dims[l], dims[l + 1])) * np.sqrt(2.0 / dims[l])).astype(np.float32).reshape(-1))
+        parts.append(np.zeros(dims[l + 1], dtype=np.float32))
+    return np.concatenate(parts)
+
+
+class Mlp:
+    """The GPU MLP (abd_mlp.hip) behind include/abd_mlp.h."""
+
+    def __init__(self, dims=DEFAULT_DIMS, seed=0, params=None):
+        self.dims = tuple(int(d) for d in dims)
+        self._h = C.c_void_p()
+        d = np.asarray(self.dims, dtype=np.int32)
+        L = _lib.load()
+        rc = L.abd_mlp_create(_iptr(d), len(self.dims) - 1, C.byref(self._h))
+        self._check(rc, "abd_mlp_create")
+        self.n_params = L.abd_mlp_num_params(self._h)
+        self.set_params(init_params(self.dims, seed) if params is None else params)
+        self.set_loss_weights(np.ones(self.dims[-1], dtype=np.float32))
+
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = _lib.load().abd_mlp_last_error(self._h).decode() if self._h else ""
+            raise _lib.AbdError(f"{what} failed with status {rc}: {msg}")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.load().abd_mlp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, flat):
+        flat = np.ascontiguousarray(flat, dtype=np.float32)
+        assert flat.size == self.n_params
+        self._check(_lib.load().abd_mlp_set_params(self._h, _fptr(flat)), "abd_mlp_set_params")
+
+    def get_params(self):
+        out = np.zeros(self.n_params, dtype=np.float32)
+        self._check(_lib.load().abd_mlp_get_params(self._h, _fptr(out)), "abd_mlp_get_params")
+        return out
+
+    def set_loss_weights(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        self._check(_lib.load().abd_mlp_set_loss_weights(self._h, _fptr(w)), "abd_mlp_set_loss_weights")
+
+    def forward(self, X):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        Y = np.zeros((X.shape[0], self.dims[-1]), dtype=np.float32)
+        self._check(_lib.load().abd_mlp_forward(self._h, _fptr(X), X.shape[0], _fptr(Y)), "abd_mlp_forward")
+        return Y
+
+    def gradient(self, X, T):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        T = np.ascontiguousarray(T, dtype=np.float32)
+        loss = C.c_float(0.0)
+        g = np.zeros(self.n_params, dtype=np.float32)
+        self._check(_lib.load().abd_mlp_gradient(self._h, _fptr(X), _fptr(T), X.shape[0], C.byref(loss), _fptr(g)),
+                    "abd_mlp_gradient")
+        return float(loss.value), g
+
+    def train(self, X, T, steps, batch=256, lr=1e-3, seed=0, beta1=0.9, beta2=0.999, eps=1e-8):
+        """Adam over `steps` minibatches drawn (seeded) from the rows of X, T; returns the batch losses."""
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        T = np.ascontiguousarray(T, dtype=np.float32)
+        n = X.shape[0]
+        rng = np.random.default_rng(seed)
+        losses = []
+        done = 0
+        while done < steps:
+            k = min(4096, steps - done)
+            perm = np.concatenate([rng.permutation(n) for _ in range((k * batch) // n + 1)])[:k * batch]
+            perm = np.ascontiguousarray(perm, dtype=np.int32)
+            out = np.zeros(k, dtype=np.float32)
+            self._check(_lib.load().abd_mlp_train(self._h, _fptr(X), _fptr(T), n, _iptr(perm), k, batch, lr, beta1,
+                                                  beta2, eps, _fptr(out)), "abd_mlp_train")
+            losses.append(out)
+            done += k
+        return np.concatenate(losses)
+
+    def gemm_test(self, A, B):
+        A = np.ascontiguousarray(A, dtype=np.float32)
+        B = np.ascontiguousarray(B, dtype=np.float32)
+        Cm = np.zeros((A.shape[0], B.shape[1]), dtype=np.float32)
+        self._check(_lib.load().abd_mlp_gemm_test(self._h, A.shape[0], B.shape[1], A.shape[1], _fptr(A), _fptr(B),
+                                                  _fptr(Cm)), "abd_mlp_gemm_test")
+        return Cm
