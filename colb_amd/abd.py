@@ -241,6 +241,34 @@ class NewtonAbd:
                    "abd_autotune")
         return bpt.value, multi.value, ms.reshape(5, 2)
 
+    def log_samples(self, on=True):
+        """Start (and clear) or stop logging (features, du) training samples of every solve."""
+        _lib.check(_lib.load().abd_log_samples(self._h, int(on)), self._h, "abd_log_samples")
+
+    def samples(self):
+        """The logged (features[n, 37], du[n, 12]) samples, float32."""
+        n = C.c_int(0)
+        _lib.check(_lib.load().abd_get_samples(self._h, None, None, 0, C.byref(n)), self._h, "abd_get_samples")
+        f = np.zeros((n.value, 37), dtype=np.float32)
+        d = np.zeros((n.value, 12), dtype=np.float32)
+        fp = f.ctypes.data_as(C.POINTER(C.c_float))
+        dp = d.ctypes.data_as(C.POINTER(C.c_float))
+        _lib.check(_lib.load().abd_get_samples(self._h, fp, dp, n.value, C.byref(n)), self._h, "abd_get_samples")
+        return f, d
+
+    def set_warm_start(self, ws):
+        """Use a trained colb_amd.learned.WarmStart to seed every PCG solve (None: plain PCG)."""
+        L = _lib.load()
+        if ws is None:
+            _lib.check(L.abd_set_warm_start(self._h, None, None, None, None, None), self._h, "abd_set_warm_start")
+            self._warm = None
+            return
+        self._warm_arrays = [np.ascontiguousarray(a, dtype=np.float32) for a in
+                             (ws.feat_mean, ws.feat_std, ws.out_mean, ws.out_std)]
+        a = [x.ctypes.data_as(C.POINTER(C.c_float)) for x in self._warm_arrays]
+        _lib.check(L.abd_set_warm_start(self._h, ws.mlp._h, *a), self._h, "abd_set_warm_start")
+        self._warm = ws  # keeps the model alive while the solver uses it
+
     def enable_timing(self, on=True):
         _lib.check(_lib.load().abd_enable_timing(self._h, int(on)), self._h, "abd_enable_timing")
 

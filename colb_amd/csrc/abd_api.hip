@@ -25,6 +25,7 @@
 #include <vector>
 
 #include "abd_internal.hpp"
+#include "abd_mlp.hpp"
 
 using namespace abd;
 
@@ -115,6 +116,13 @@ struct AbdCtx {
     unsigned* h_pcg_ctl = nullptr;
     bool coop_ok = true;          // cleared when a cooperative launch is refused
     bool pcg_multi = false;       // solve with one launch per iteration (autotuner choice)
+    // learned warm start (abd_set_warm_start) and training-sample log (abd_log_samples)
+    AbdMlp* mlp = nullptr;
+    float *feat_raw = nullptr, *fmean = nullptr, *fistd = nullptr, *omean = nullptr, *ostd = nullptr;
+    double *x0 = nullptr, *rhs = nullptr, *nrm_b = nullptr, *nrm_r = nullptr;
+    int* warm_used = nullptr;
+    bool log_on = false;
+    std::vector<float> log_feat, log_du;
     int bodies_per_tile = 5;
     int* cg_iters = nullptr;
     int* cg_done = nullptr;
@@ -537,6 +545,68 @@ int assemble(AbdCtx* ctx) {
 // --------------------------------------------------------------------------------------------
 // block-Jacobi PCG (compute_du, abd.py:631-658); du <- solution, residual <- ||grad||
 // --------------------------------------------------------------------------------------------
+// Learned warm start: body features of the current assembly -> MLP -> x0; r0 = b - H x0 per scene,
+// kept only where ||r0|| < ||b|| (plain PCG from zero otherwise)
+int warm_start(AbdCtx* ctx) {
+    AbdMlp* m = ctx->mlp;
+    launch_body_features(ctx->st, ctx->G, ctx->q, ctx->qtilde, ctx->qdot, ctx->n_inc, ctx->inc_s, ctx->act_g,
+                         ctx->n_act, ctx->feat_raw);
+    CHK(dbg(ctx, "launch_body_features"));
+    int rc = mlp_reserve(m, ctx->nb);
+    if (rc != ABD_OK) {
+        ctx->err = "warm start: " + m->err;
+        return rc;
+    }
+    const int rows = (ctx->nb + 15) / 16 * 16;
+    launch_features_pack(ctx->st, ctx->nb, rows, m->dp[0], ctx->feat_raw, ctx->fmean, ctx->fistd, m->A[0]);
+    mlp_forward_dev(m, ctx->nb, ctx->st);
+    launch_warm_x0(ctx->st, ctx->G, m->dp[m->L], m->Z[m->L - 1], ctx->omean, ctx->ostd, ctx->x0);
+    CHK(dbg(ctx, "warm start forward"));
+    launch_residual(ctx->st, ctx->nb, ctx->grad, ctx->diag, ctx->row_ptr, ctx->ent_ks, ctx->ent_vs, ctx->off,
+                    ctx->offT, ctx->x0, ctx->rhs, ctx->n_pairs, ctx->cnt + C_ERR);
+    launch_scene_norm(ctx->st, ctx->G, ctx->grad, ctx->nrm_b);
+    launch_scene_norm(ctx->st, ctx->G, ctx->rhs, ctx->nrm_r);
+    launch_warm_select(ctx->st, ctx->G, ctx->nrm_b, ctx->nrm_r, ctx->grad, ctx->rhs, ctx->x0, ctx->warm_used);
+    CHK(dbg(ctx, "warm start residual"));
+    HIPCHK(hipGetLastError());
+    return ABD_OK;
+}
+
+// after the PCG: add the warm start, restore ||b|| as the residual, log training samples
+int solve_finish(AbdCtx* ctx, bool warm) {
+    if (warm) {
+        launch_add_inplace(ctx->st, 12 * ctx->nb, ctx->du, ctx->x0);
+        hipLaunchKernelGGL(k_copy, dim3(cdiv(ctx->ns, 64)), dim3(64), 0, ctx->st, ctx->ns, ctx->nrm_b, ctx->bnorm);
+        hipLaunchKernelGGL(k_copy, dim3(cdiv(ctx->ns, 64)), dim3(64), 0, ctx->st, ctx->ns, ctx->nrm_b, ctx->C.residual);
+        CHK(dbg(ctx, "warm start finish"));
+    }
+    if (ctx->log_on) {
+        launch_body_features(ctx->st, ctx->G, ctx->q, ctx->qtilde, ctx->qdot, ctx->n_inc, ctx->inc_s, ctx->act_g,
+                             ctx->n_act, ctx->feat_raw);
+        const int nb = ctx->nb;
+        std::vector<float> f((size_t)ABD_N_FEATURES * nb);
+        std::vector<double> du(12 * (size_t)nb);
+        std::vector<int> act(ctx->ns);
+        HIPCHK(hipMemcpyAsync(f.data(), ctx->feat_raw, f.size() * sizeof(float), hipMemcpyDeviceToHost, ctx->st));
+        HIPCHK(hipMemcpyAsync(du.data(), ctx->du, du.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->st));
+        HIPCHK(hipMemcpyAsync(act.data(), ctx->C.newton_active, ctx->ns * sizeof(int), hipMemcpyDeviceToHost, ctx->st));
+        HIPCHK(hipStreamSynchronize(ctx->st));
+        std::vector<int> pinned(nb), bscene(nb);
+        for (int s = 0; s < ctx->ns; ++s)
+            for (int b = ctx->h_scene_b0[s]; b < ctx->h_scene_b0[s + 1]; ++b) bscene[b] = s;
+        HIPCHK(hipMemcpy(pinned.data(), ctx->pinned, nb * sizeof(int), hipMemcpyDeviceToHost));
+        for (int b = 0; b < nb; ++b) {
+            if (pinned[b] || !act[bscene[b]]) continue;
+            ctx->log_feat.insert(ctx->log_feat.end(), f.begin() + (size_t)ABD_N_FEATURES * b,
+                                 f.begin() + (size_t)ABD_N_FEATURES * (b + 1));
+            for (int j = 0; j < 12; ++j) ctx->log_du.push_back((float)du[12 * (size_t)b + j]);
+        }
+    }
+    HIPCHK(hipGetLastError());
+    ctx->solved = true;
+    return ABD_OK;
+}
+
 // one cooperative launch for the whole solve (k_pcg_persist); false when the launch was refused or
 // a grid barrier timed out, in which case the caller solves again with one launch per iteration
 int solve_persistent(AbdCtx* ctx, const PcgBuf& B, bool* ok) {
@@ -583,12 +653,19 @@ int solve(AbdCtx* ctx) {
     B.err = ctx->cnt + C_ERR;
     B.bar = ctx->pcg_ctl; B.done_cnt = (int*)(ctx->pcg_ctl + 1); B.fail = (int*)(ctx->pcg_ctl + 4);
     B.bnorm_out = ctx->bnorm; B.residual_out = ctx->C.residual;
+    B.bref = nullptr;
+    const bool warm = ctx->mlp != nullptr;
+    if (warm) {
+        // x0 from the model; PCG then solves H e = b - H x0 (stopping relative to ||b||), du = x0 + e
+        CHK(warm_start(ctx));
+        B.b = ctx->rhs;
+        B.bref = ctx->nrm_b;
+    }
     {
         bool ok = false;
         CHK(solve_persistent(ctx, B, &ok));
         if (ok) {
-            HIPCHK(hipGetLastError());
-            ctx->solved = true;
+            CHK(solve_finish(ctx, warm));
             return ABD_OK;
         }
     }
@@ -600,6 +677,7 @@ int solve(AbdCtx* ctx) {
     CHK(dbg(ctx, "launch_pcg_cg(0)"));
     launch_pcg_tile_sum(ctx->st, ctx->ns, ctx->scene_tile0, ctx->part, nt, 2 * 2 + 0, ctx->bnorm, 1);
     CHK(dbg(ctx, "launch_pcg_tile_sum"));
+    if (warm) hipLaunchKernelGGL(k_copy, dim3(cdiv(ctx->ns, 64)), dim3(64), 0, ctx->st, ctx->ns, ctx->nrm_b, ctx->bnorm);
     hipLaunchKernelGGL(k_copy, dim3(cdiv(ctx->ns, 64)), dim3(64), 0, ctx->st, ctx->ns, ctx->bnorm, ctx->C.residual);
     CHK(dbg(ctx, "k_copy"));
     const int nblk = ctx->nb + 2 * ctx->n_pairs;
@@ -621,8 +699,7 @@ int solve(AbdCtx* ctx) {
             poll += std::min(32, std::max(4, poll / 2));
         }
     }
-    HIPCHK(hipGetLastError());
-    ctx->solved = true;
+    CHK(solve_finish(ctx, warm));
     return ABD_OK;
 }
 
@@ -1091,6 +1168,8 @@ int abd_create(const AbdSceneDesc* d, const AbdParams* p, AbdCtx** out) {
     A(cand_lo, ns); A(cand_hi, ns);
     A(r, 12 * nb); A(z, 12 * nb); A(p0, 12 * nb); A(p1, 12 * nb); A(Ap, 12 * nb);
     A(r1, 12 * nb); A(w1, 12 * nb); A(s0, 12 * nb); A(pcg_sc, 4 * ns); A(pcg_ctl, 8);
+    A(feat_raw, (size_t)ABD_N_FEATURES * nb); A(fmean, ABD_N_FEATURES); A(fistd, ABD_N_FEATURES); A(omean, 12);
+    A(ostd, 12); A(x0, 12 * nb); A(rhs, 12 * nb); A(nrm_b, ns); A(nrm_r, ns); A(warm_used, ns);
     if ((rc = set_tiles(ctx, 5)) != ABD_OK) return fail(rc);
     A(bnorm, ns); A(cg_iters, ns); A(cg_done, ns);
     A(C.newton_active, ns); A(C.residual, ns); A(C.alpha, ns); A(C.alpha0, ns); A(C.amax_bits, ns);
@@ -1445,6 +1524,47 @@ int abd_autotune(AbdCtx* ctx, int repeats, int* bodies_per_tile, int* per_iterat
     if (bodies_per_tile) *bodies_per_tile = best_bpt;
     if (per_iteration_launch) *per_iteration_launch = best_multi;
     ctx->assembled = ctx->solved = false;
+    return ABD_OK;
+}
+
+int abd_set_warm_start(AbdCtx* ctx, AbdMlp* model, const float* feat_mean, const float* feat_std,
+                       const float* out_mean, const float* out_std) {
+    CHK(check_ctx(ctx));
+    if (!model) {
+        ctx->mlp = nullptr;
+        return ABD_OK;
+    }
+    if (!feat_mean || !feat_std || !out_mean || !out_std || model->L < 1 || model->d[0] != ABD_N_FEATURES ||
+        model->d[model->L] != 12) {
+        ctx->err = "warm-start model must map 37 body features to 12 DOFs, with normalisation arrays";
+        return ABD_ERR_ARG;
+    }
+    std::vector<float> istd(ABD_N_FEATURES);
+    for (int j = 0; j < ABD_N_FEATURES; ++j) istd[j] = feat_std[j] > 0.f ? 1.f / feat_std[j] : 0.f;
+    HIPCHK(hipMemcpy(ctx->fmean, feat_mean, ABD_N_FEATURES * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->fistd, istd.data(), ABD_N_FEATURES * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->omean, out_mean, 12 * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->ostd, out_std, 12 * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(hipStreamSynchronize(model->st));
+    ctx->mlp = model;
+    return ABD_OK;
+}
+
+int abd_log_samples(AbdCtx* ctx, int on) {
+    CHK(check_ctx(ctx));
+    ctx->log_on = on != 0;
+    ctx->log_feat.clear();
+    ctx->log_du.clear();
+    return ABD_OK;
+}
+
+int abd_get_samples(const AbdCtx* ctx, float* feat, float* du, int cap, int* n) {
+    if (!ctx || !n) return ABD_ERR_ARG;
+    const int total = (int)(ctx->log_du.size() / 12);
+    *n = total;
+    const int k = std::min(total, cap);
+    if (feat && k > 0) std::memcpy(feat, ctx->log_feat.data(), sizeof(float) * ABD_N_FEATURES * (size_t)k);
+    if (du && k > 0) std::memcpy(du, ctx->log_du.data(), sizeof(float) * 12 * (size_t)k);
     return ABD_OK;
 }
 

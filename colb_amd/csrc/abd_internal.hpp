@@ -111,6 +111,7 @@ struct PcgBuf {
     int* fail;
     double* bnorm_out;
     double* residual_out;
+    const double* bref;        // optional per-scene reference norm for the stopping test (warm start)
 };
 
 struct GrowBuf {
@@ -142,6 +143,20 @@ void launch_pair_queries(hipStream_t s, const Geo& G, const BodyMesh& M, int npa
 void launch_ground_queries(hipStream_t s, const Geo& G, const double4* X0, const double4* X1, const AbdParams& P,
                            double infl, uint64_t* cand, int cap, int* count, int* overflow);
 // abd_kernels.hip
+constexpr int ABD_N_FEATURES = 37;
+void launch_body_features(hipStream_t s, const Geo& G, const double* q, const double* qtilde, const double* qdot,
+                          int n_inc, const uint64_t* inc, const double* g24, int n_act, float* feat);
+void launch_features_pack(hipStream_t s, int nb, int rows, int dp, const float* feat, const float* mean,
+                          const float* istd, uint16_t* out);
+void launch_warm_x0(hipStream_t s, const Geo& G, int dpo, const float* Y, const float* omean, const float* ostd,
+                    double* x0);
+void launch_residual(hipStream_t s, int nb, const double* b, const double* diag, const int* row_ptr,
+                     const uint64_t* ent_key, const int* ent_val, const double* off, const double* offT,
+                     const double* x, double* r, int n_pairs, int* err);
+void launch_scene_norm(hipStream_t s, const Geo& G, const double* v, double* out);
+void launch_warm_select(hipStream_t s, const Geo& G, const double* nb_b, const double* nb_r, const double* b,
+                        double* rhs, double* x0, int* used);
+void launch_add_inplace(hipStream_t s, int n, double* y, const double* x);
 void launch_cand_ranges(hipStream_t s, int n, const uint64_t* keys, int* lo, int* hi);
 void launch_cand_eval(hipStream_t s, const Geo& G, int n, const uint64_t* keys, const double4* X, const AbdParams& P,
                       int* flag, double* energy);

@@ -16,6 +16,7 @@
 
 #include "abd_device.hpp"
 #include "abd_internal.hpp"
+#include "abd_mlp.hpp"
 
 namespace abd {
 
@@ -866,7 +867,7 @@ __global__ __launch_bounds__(64 * PCG_WAVES) void k_pcg_persist(int ntiles, int 
             }
             const double rho = wave_sum(a_rho), g = wave_sum(a_g), d = wave_sum(a_d);
             if (k == 1) {
-                bnorm = sqrt(rho);
+                bnorm = B.bref ? B.bref[s] : sqrt(rho);
                 if (lead) { B.bnorm_out[s] = bnorm; B.residual_out[s] = bnorm; }
             }
             if (!done && (bnorm == 0.0 || sqrt(rho) <= tol * bnorm)) done = true;
@@ -988,6 +989,155 @@ void launch_pcg_any(hipStream_t s, int ns, int ntiles, const int* scene_tile0, c
                     const double* bnorm, double tol, const int* cg_done, int* any) {
     hipLaunchKernelGGL(k_pcg_any, dim3(ns), dim3(64), 0, s, ns, ntiles, scene_tile0, part, it, bnorm, tol, cg_done,
                        any);
+}
+
+// ---------------------------------------------------------------------------------------------
+// learned warm start: body features, the step the MLP predicts, and the residual it leaves
+// ---------------------------------------------------------------------------------------------
+// raw features per body: q - q~ (12), qdot (12), incident active contacts (1), summed contact
+// gradient (12), from the incidences of the last assembly
+__global__ void k_body_features(Geo G, const double* __restrict__ q, const double* __restrict__ qtilde,
+                                const double* __restrict__ qdot, int n_inc, const uint64_t* __restrict__ inc,
+                                const double* __restrict__ g24, int n_act, float* __restrict__ feat) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= G.nb) return;
+    float* f = feat + (size_t)ABD_N_FEATURES * b;
+    for (int j = 0; j < 12; ++j) {
+        f[j] = (float)(q[12 * b + j] - qtilde[12 * b + j]);
+        f[12 + j] = (float)qdot[12 * b + j];
+    }
+    const int lo = lower_bound64(inc, n_inc, (uint64_t)b << 32), hi = lower_bound64(inc, n_inc, (uint64_t)(b + 1) << 32);
+    double gs[12];
+    for (int j = 0; j < 12; ++j) gs[j] = 0.0;
+    for (int m = lo; m < hi; ++m) {
+        const uint32_t w = (uint32_t)inc[m];
+        const int c = opaque((int)(w >> 1)), role = opaque((int)(w & 1));
+        if (c >= n_act) {
+            atomicOr(G.err, ERR_INCIDENCE);
+            break;
+        }
+        for (int j = 0; j < 12; ++j) gs[j] += g24[(size_t)c * 24 + role * 12 + j];
+    }
+    f[24] = (float)(hi - lo);
+    for (int j = 0; j < 12; ++j) f[25 + j] = (float)gs[j];
+}
+
+void launch_body_features(hipStream_t s, const Geo& G, const double* q, const double* qtilde, const double* qdot,
+                          int n_inc, const uint64_t* inc, const double* g24, int n_act, float* feat) {
+    hipLaunchKernelGGL(k_body_features, dim3(cdiv(G.nb, 128)), dim3(128), 0, s, G, q, qtilde, qdot, n_inc, inc, g24,
+                       n_act, feat);
+}
+
+// standardised features -> BF16 model input (rows x dp, zero padding)
+__global__ void k_features_pack(int nb, int rows, int dp, const float* __restrict__ feat, const float* __restrict__ mean,
+                                const float* __restrict__ istd, uint16_t* __restrict__ out) {
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (long)rows * dp) return;
+    const int i = (int)(e / dp), j = (int)(e % dp);
+    float v = 0.f;
+    if (i < nb && j < ABD_N_FEATURES) v = (feat[(size_t)ABD_N_FEATURES * i + j] - mean[j]) * istd[j];
+    out[e] = f2bf(v);
+}
+
+void launch_features_pack(hipStream_t s, int nb, int rows, int dp, const float* feat, const float* mean,
+                          const float* istd, uint16_t* out) {
+    const long n = (long)rows * dp;
+    hipLaunchKernelGGL(k_features_pack, dim3(cdiv(n, 256)), dim3(256), 0, s, nb, rows, dp, feat, mean, istd, out);
+}
+
+// x0 = de-standardised model output (pinned bodies: 0)
+__global__ void k_warm_x0(Geo G, int dpo, const float* __restrict__ Y, const float* __restrict__ omean,
+                          const float* __restrict__ ostd, double* __restrict__ x0) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= 12 * G.nb) return;
+    const int b = e / 12, j = e % 12;
+    x0[e] = G.pinned[b] ? 0.0 : (double)(Y[(size_t)b * dpo + j] * ostd[j] + omean[j]);
+}
+
+void launch_warm_x0(hipStream_t s, const Geo& G, int dpo, const float* Y, const float* omean, const float* ostd,
+                    double* x0) {
+    hipLaunchKernelGGL(k_warm_x0, dim3(cdiv(12 * G.nb, 256)), dim3(256), 0, s, G, dpo, Y, omean, ostd, x0);
+}
+
+// r = b - H x (H: diagonal blocks + CSR off-diagonal blocks), one thread per row
+__global__ void k_residual(int nb, const double* __restrict__ bvec, const double* __restrict__ diag,
+                           const int* __restrict__ row_ptr, const uint64_t* __restrict__ ent_key,
+                           const int* __restrict__ ent_val, const double* __restrict__ off,
+                           const double* __restrict__ offT, const double* __restrict__ x, double* __restrict__ r,
+                           int n_pairs, int* err) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= 12 * nb) return;
+    const int b = e / 12, row = e % 12;
+    double acc = bvec[e];
+    const double* D = diag + (size_t)b * 144 + row * 12;
+    for (int c = 0; c < 12; ++c) acc -= D[c] * x[12 * b + c];
+    for (int k = row_ptr[b]; k < row_ptr[b + 1]; ++k) {
+        const int col = opaque((int)(ent_key[k] & 0xFFFFFFFFu));
+        const int v = ent_val[k];
+        if (col < 0 || col >= nb || v < 0 || (v >> 1) >= n_pairs) {
+            atomicOr(err, ERR_CSR);
+            break;
+        }
+        const double* Bk = (opaque(v & 1) ? offT : off) + (size_t)opaque(v >> 1) * 144 + row * 12;
+        for (int c = 0; c < 12; ++c) acc -= Bk[c] * x[12 * col + c];
+    }
+    r[e] = acc;
+}
+
+void launch_residual(hipStream_t s, int nb, const double* b, const double* diag, const int* row_ptr,
+                     const uint64_t* ent_key, const int* ent_val, const double* off, const double* offT,
+                     const double* x, double* r, int n_pairs, int* err) {
+    hipLaunchKernelGGL(k_residual, dim3(cdiv(12 * nb, 256)), dim3(256), 0, s, nb, b, diag, row_ptr, ent_key, ent_val,
+                       off, offT, x, r, n_pairs, err);
+}
+
+// ||v|| per scene over its bodies' rows (one workgroup per scene, fixed order)
+__global__ __launch_bounds__(256) void k_scene_norm(Geo G, const double* __restrict__ v, double* __restrict__ out) {
+    __shared__ double sh[256];
+    const int s = blockIdx.x;
+    double acc = 0.0;
+    for (int e = 12 * G.scene_b0[s] + threadIdx.x; e < 12 * G.scene_b0[s + 1]; e += 256) acc += v[e] * v[e];
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[s] = sqrt(sh[0]);
+}
+
+void launch_scene_norm(hipStream_t s, const Geo& G, const double* v, double* out) {
+    hipLaunchKernelGGL(k_scene_norm, dim3(G.ns), dim3(256), 0, s, G, v, out);
+}
+
+// Keep the warm start of a scene only if its residual is below ||b|| (else start from zero:
+// rhs = b, x0 = 0); used[s] records the choice
+__global__ void k_warm_select(Geo G, const double* __restrict__ nb_b, const double* __restrict__ nb_r,
+                              const double* __restrict__ bvec, double* __restrict__ rhs, double* __restrict__ x0,
+                              int* __restrict__ used) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= 12 * G.nb) return;
+    const int b = e / 12, s = G.bscene[b];
+    const bool keep = nb_r[s] < nb_b[s];
+    if (!keep) {
+        rhs[e] = bvec[e];
+        x0[e] = 0.0;
+    }
+    if (e == 12 * G.scene_b0[s]) used[s] = keep ? 1 : 0;
+}
+
+void launch_warm_select(hipStream_t s, const Geo& G, const double* nb_b, const double* nb_r, const double* b,
+                        double* rhs, double* x0, int* used) {
+    hipLaunchKernelGGL(k_warm_select, dim3(cdiv(12 * G.nb, 256)), dim3(256), 0, s, G, nb_b, nb_r, b, rhs, x0, used);
+}
+
+__global__ void k_add_inplace(int n, double* __restrict__ y, const double* __restrict__ x) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] += x[i];
+}
+
+void launch_add_inplace(hipStream_t s, int n, double* y, const double* x) {
+    hipLaunchKernelGGL(k_add_inplace, dim3(cdiv(n, 256)), dim3(256), 0, s, n, y, x);
 }
 
 // ---------------------------------------------------------------------------------------------

@@ -23,18 +23,12 @@
 
 namespace abd {
 
-typedef short short4v __attribute__((ext_vector_type(4)));
-typedef float float4v __attribute__((ext_vector_type(4)));
+typedef short short4v __attribute__((vector_size(4 * sizeof(short))));
+typedef float float4v __attribute__((vector_size(4 * sizeof(float))));
 
 static inline int cdivm(long a, int b) { return (int)((a + b - 1) / b); }
 static inline int pad16(int n) { return (n + 15) / 16 * 16; }
 
-// float -> bf16, round to nearest even (finite inputs)
-__device__ __forceinline__ uint16_t f2bf(float f) {
-    uint32_t u = __float_as_uint(f);
-    u += 0x7FFFu + ((u >> 16) & 1u);
-    return (uint16_t)(u >> 16);
-}
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
 // C(i, j) = sum_k A(i, k) B(k, j) [+ bias(j)], A(i, k) = A[i*sa0 + k*sa1], B(k, j) = B[k*sb0 + j*sb1],

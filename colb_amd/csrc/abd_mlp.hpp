@@ -33,6 +33,14 @@ struct AbdMlp {
 
 namespace abd {
 using Mlp = ::AbdMlp;
+
+// float -> bf16, round to nearest even (finite inputs)
+__device__ __forceinline__ uint16_t f2bf(float f) {
+    uint32_t u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
 int mlp_reserve(Mlp* m, int rows);
 void mlp_forward_dev(Mlp* m, int n, hipStream_t s);
 void mlp_backward_dev(Mlp* m, int n, float* loss_dev, hipStream_t s);

@@ -17,7 +17,6 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._abi import dptr
 
 N_FEATURES = 37
 N_OUT = 12
@@ -127,3 +126,44 @@ class Mlp:
         self._check(_lib.load().abd_mlp_gemm_test(self._h, A.shape[0], B.shape[1], A.shape[1], _fptr(A), _fptr(B),
                                                   _fptr(Cm)), "abd_mlp_gemm_test")
         return Cm
+
+
+class WarmStart:
+    """The trained warm-start model with its feature/target standardisation."""
+
+    def __init__(self, dims=DEFAULT_DIMS, seed=0):
+        self.dims = tuple(dims)
+        self.mlp = Mlp(self.dims, seed=seed)
+        self.feat_mean = np.zeros(N_FEATURES, dtype=np.float32)
+        self.feat_std = np.ones(N_FEATURES, dtype=np.float32)
+        self.out_mean = np.zeros(N_OUT, dtype=np.float32)
+        self.out_std = np.ones(N_OUT, dtype=np.float32)
+
+    def fit(self, feat, du, steps=3000, batch=512, lr=1e-3, seed=0):
+        """Standardise, then train with Adam on the GPU; returns the batch losses."""
+        feat = np.asarray(feat, dtype=np.float64)
+        du = np.asarray(du, dtype=np.float64)
+        self.feat_mean = feat.mean(0).astype(np.float32)
+        self.feat_std = np.maximum(feat.std(0), 1e-12).astype(np.float32)
+        self.out_mean = du.mean(0).astype(np.float32)
+        self.out_std = np.maximum(du.std(0), 1e-12).astype(np.float32)
+        X = ((feat - self.feat_mean) / self.feat_std).astype(np.float32)
+        T = ((du - self.out_mean) / self.out_std).astype(np.float32)
+        return self.mlp.train(X, T, steps=steps, batch=min(batch, X.shape[0]), lr=lr, seed=seed)
+
+    def predict(self, feat):
+        X = ((np.asarray(feat, dtype=np.float64) - self.feat_mean) / self.feat_std).astype(np.float32)
+        return self.mlp.forward(X) * self.out_std + self.out_mean
+
+    def save(self, path):
+        np.savez(path, dims=np.asarray(self.dims), params=self.mlp.get_params(), feat_mean=self.feat_mean,
+                 feat_std=self.feat_std, out_mean=self.out_mean, out_std=self.out_std)
+
+    @classmethod
+    def load(cls, path):
+        z = np.load(path)
+        ws = cls(tuple(int(d) for d in z["dims"]))
+        ws.mlp.set_params(z["params"])
+        for k in ("feat_mean", "feat_std", "out_mean", "out_std"):
+            setattr(ws, k, z[k].astype(np.float32))
+        return ws

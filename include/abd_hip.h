@@ -168,6 +168,24 @@ int abd_enable_timing(AbdCtx* ctx, int on);
  * form is unavailable.  The state q is unchanged. */
 int abd_autotune(AbdCtx* ctx, int repeats, int* bodies_per_tile, int* per_iteration_launch, double* ms);
 
+/* No counterpart in the reference (the learned warm start of the north-star configuration): every
+ * later solve computes 37 features per body (q - q~, qdot, incident contact count, summed contact
+ * gradient), standardises them with feat_mean/feat_std, runs the model (include/abd_mlp.h, 37 ->
+ * ... -> 12) on the context's stream and starts PCG from x0 = out * out_std + out_mean where that
+ * leaves a smaller residual than zero (per scene; plain PCG otherwise).  The stopping test stays
+ * relative to ||grad||.  model == NULL switches it off.  The model must outlive its use. */
+#ifndef ABD_MLP_TYPEDEF
+#define ABD_MLP_TYPEDEF
+typedef struct AbdMlp AbdMlp;
+#endif
+int abd_set_warm_start(AbdCtx* ctx, AbdMlp* model, const float* feat_mean, const float* feat_std,
+                       const float* out_mean, const float* out_std);
+/* No counterpart in the reference (training data of the warm start): while on, every solve appends
+ * (raw features[37], du[12]) of each unpinned body of each still-iterating scene; on/off clears. */
+int abd_log_samples(AbdCtx* ctx, int on);
+/* Copies up to cap logged samples (feat: cap x 37, du: cap x 12); *n = number logged. */
+int abd_get_samples(const AbdCtx* ctx, float* feat, float* du, int cap, int* n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -13,7 +13,10 @@
 extern "C" {
 #endif
 
+#ifndef ABD_MLP_TYPEDEF
+#define ABD_MLP_TYPEDEF
 typedef struct AbdMlp AbdMlp;
+#endif
 
 /* dims[0..n_layers]: input width, hidden widths, output width (each <= 4096). */
 int abd_mlp_create(const int* dims, int n_layers, AbdMlp** out);

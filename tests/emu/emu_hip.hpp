@@ -54,6 +54,7 @@ struct EmuSched {
     unsigned cur = 0;
     std::function<void()> body;
     std::vector<double> xbuf;
+    std::vector<float> mbuf;  // matrix-instruction operand exchange, 8 values per lane
     const void* main_bottom = nullptr;
     size_t main_size = 0;
 };
@@ -115,6 +116,34 @@ inline int emu_any(int pred) {
         if (emu_sched->xbuf[k] != 0.0) r = 1;
     emu_yield();
     return r;
+}
+
+inline unsigned __float_as_uint(float f) { unsigned u; std::memcpy(&u, &f, 4); return u; }
+inline float __uint_as_float(unsigned u) { float f; std::memcpy(&f, &u, 4); return f; }
+
+// v_mfma_f32_16x16x16_bf16 over the 64 lanes of the caller's wave: lane l supplies A(l%16, 4(l/16)+u)
+// and B(4(l/16)+u, l%16), u = 0..3, and receives D(4(l/16)+u, l%16) = C + sum_k A(., k) B(k, .)
+template <class S4, class F4>
+inline F4 __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(S4 a, S4 b, F4 c, int, int, int) {
+    const unsigned lane = threadIdx.x % 64, w0 = threadIdx.x - lane;
+    auto bf = [](short h) { return __uint_as_float((unsigned)(unsigned short)h << 16); };
+    float* mb = emu_sched->mbuf.data();
+    for (int u = 0; u < 4; ++u) { mb[8 * threadIdx.x + u] = bf(a[u]); mb[8 * threadIdx.x + 4 + u] = bf(b[u]); }
+    emu_yield();
+    F4 d = c;
+    const int n = lane % 16;
+    for (int u = 0; u < 4; ++u) {
+        const int m = 4 * (lane / 16) + u;
+        float acc = 0.f;
+        for (int k = 0; k < 16; ++k) {
+            const float av = mb[8 * (w0 + m + 16 * (k / 4)) + (k % 4)];
+            const float bv = mb[8 * (w0 + n + 16 * (k / 4)) + 4 + (k % 4)];
+            acc += av * bv;
+        }
+        d[u] = c[u] + acc;
+    }
+    emu_yield();
+    return d;
 }
 
 inline int atomicAdd(int* p, int v) { return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST); }
@@ -201,6 +230,7 @@ void emu_launch(const char* name, K kernel, dim3 grid, dim3 block, Args... args)
     if (trace) std::fprintf(stderr, "launch %s grid %u block %u\n", name, nblk, nt);
     EmuSched S;
     S.xbuf.assign(nt, 0.0);
+    S.mbuf.assign((size_t)nt * 8, 0.f);
     S.body = [&]() { kernel(args...); };
     EmuSched* prev = emu_sched;
     emu_sched = &S;

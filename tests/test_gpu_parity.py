@@ -235,3 +235,44 @@ def test_pcg_launch_forms_and_autotune_agree():
     g.set_state(q, sc.q0, qd)
     g.compute_du()
     assert rel(g._du, du_a) < 1e-8
+
+
+def test_learned_warm_start_trains_and_keeps_the_solution():
+    """The warm start is trained on the solver's own logged Newton iterates (cubes on the ground,
+    two seeds), predicts held-out steps better than a zero guess, and leaves the solution and the
+    trajectory unchanged (PCG stops relative to ||grad|| either way); prints the A/B counts."""
+    from colb_amd.learned import WarmStart
+    p = north_star(dt=8e-3, ground_enabled=True, dhat=1e-3, kappa=1e7, k_contact=1e20, cg_tol=1e-10)
+    F, D = [], []
+    for seed in (11, 12):
+        g = solver(S.cubes_on_ground(seed=seed), p)
+        g.log_samples(True)
+        g.step(60)
+        f, d = g.samples()
+        F.append(f)
+        D.append(d)
+    F, D = np.concatenate(F), np.concatenate(D)
+    ws = WarmStart(seed=0)
+    losses = ws.fit(F, D, steps=1500, batch=512, lr=1e-3)
+    assert losses[-100:].mean() < 0.5 * losses[:20].mean()
+    g = solver(S.cubes_on_ground(seed=13), p)
+    g.log_samples(True)
+    g.step(40)
+    f, d = g.samples()
+    err = np.linalg.norm(ws.predict(f) - d) / np.linalg.norm(d)
+    say(f"held-out relative error of the predicted step: {err:.3f} (zero guess: 1.0)")
+    assert err < 0.5
+    runs = {}
+    for name, w in (("plain", None), ("warm", ws)):
+        g = solver(S.cubes_on_ground(seed=1), p)
+        g.set_warm_start(w)
+        cg = nw = 0
+        for k in range(40):
+            st = g.step()[0]
+            cg += st["cg_iters"]
+            nw += st["newton_iters"]
+        runs[name] = (cg, nw, g.get_state()[0].copy())
+        g.set_warm_start(None)
+    say(f"A/B over 40 steps: plain {runs['plain'][:2]} (cg, newton), warm start {runs['warm'][:2]}")
+    assert rel(runs["warm"][2], runs["plain"][2]) < 1e-6
+    assert abs(runs["warm"][1] - runs["plain"][1]) <= 0.05 * runs["plain"][1] + 1
