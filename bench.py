@@ -67,12 +67,23 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=2)
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--pcg-layout", default=None, metavar="BPT[,multi]",
+                    help="fix the PCG layout (bodies per tile, 1 = one launch per iteration) instead of autotuning")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="also run one 512 MiB device-to-device copy (reference kernel for --pmc byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build targets one GPU; --gpus must be 1 (multi-GPU sharding is design-only)"}))
         sys.exit(2)
 
     from colb_amd.abd import NewtonAbd
+    if args.calib_copy:
+        import torch
+        a = torch.empty(512 << 20, dtype=torch.uint8, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
     sc, p, cfg = workload(args.workload)
     print(f"# workload {cfg['workload']}: {sc.n_bodies} bodies, {sc.n_verts} verts, {sc.n_tris} tris, "
           f"{sc.n_edges} edges", file=sys.stderr, flush=True)
@@ -81,7 +92,12 @@ def main():
         st = solver.step()[0]
         print(f"# warmup {k}: {st}", file=sys.stderr, flush=True)
     tune = None
-    if not args.no_autotune:
+    if args.pcg_layout:
+        parts = [int(x) for x in args.pcg_layout.split(",")]
+        solver.set_pcg_layout(parts[0], bool(parts[1]) if len(parts) > 1 else False)
+        tune = {"bodies_per_tile": parts[0], "per_iteration_launch": bool(parts[1]) if len(parts) > 1 else False,
+                "fixed": True}
+    elif not args.no_autotune:
         # choose PCG tile size and launch form at the warmed-up state (outside the timed region)
         st_keep = solver.get_state()
         bpt, multi, ms = solver.autotune(repeats=3)
@@ -108,7 +124,7 @@ def main():
     solver.step(args.profile_steps)
     kt = solver.kernel_times()
     solver.enable_timing(False)
-    single = ["pcg_iter", "broad_phase_query", "narrow_phase", "contact_hess", "offblock_sum", "body_assemble",
+    single = ["pcg_solve", "pcg_iter", "broad_phase_query", "narrow_phase", "contact_hess", "offblock_sum", "body_assemble",
               "ccd", "energy"]
     dom = max((k for k in kt if k in single), key=lambda k: kt[k]["ms"], default=None)
     roof = None

@@ -36,6 +36,7 @@ SIGNATURES = {
     "abd_get_kernel_times": (C.c_int, [_V, C.c_char_p, C.c_int, _D, _I, _D, C.c_int, _I]),
     "abd_enable_timing": (C.c_int, [_V, C.c_int]),
     "abd_autotune": (C.c_int, [_V, C.c_int, _I, _I, _D]),
+    "abd_set_pcg_layout": (C.c_int, [_V, C.c_int, C.c_int]),
     "abd_set_warm_start": (C.c_int, [_V, _V, _F, _F, _F, _F]),
     "abd_log_samples": (C.c_int, [_V, C.c_int]),
     "abd_get_samples": (C.c_int, [_V, _F, _F, C.c_int, _I]),

@@ -241,6 +241,11 @@ class NewtonAbd:
                    "abd_autotune")
         return bpt.value, multi.value, ms.reshape(5, 2)
 
+    def set_pcg_layout(self, bodies_per_tile, per_iteration_launch=False):
+        """Fix the PCG tile size and launch form (e.g. a result of autotune())."""
+        _lib.check(_lib.load().abd_set_pcg_layout(self._h, int(bodies_per_tile), int(bool(per_iteration_launch))),
+                   self._h, "abd_set_pcg_layout")
+
     def log_samples(self, on=True):
         """Start (and clear) or stop logging (features, du) training samples of every solve."""
         _lib.check(_lib.load().abd_log_samples(self._h, int(on)), self._h, "abd_log_samples")

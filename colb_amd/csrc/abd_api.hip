@@ -1527,6 +1527,14 @@ int abd_autotune(AbdCtx* ctx, int repeats, int* bodies_per_tile, int* per_iterat
     return ABD_OK;
 }
 
+int abd_set_pcg_layout(AbdCtx* ctx, int bodies_per_tile, int per_iteration_launch) {
+    CHK(check_ctx(ctx));
+    CHK(set_tiles(ctx, bodies_per_tile));
+    ctx->pcg_multi = per_iteration_launch != 0;
+    ctx->solved = false;
+    return ABD_OK;
+}
+
 int abd_set_warm_start(AbdCtx* ctx, AbdMlp* model, const float* feat_mean, const float* feat_std,
                        const float* out_mean, const float* out_std) {
     CHK(check_ctx(ctx));

@@ -168,6 +168,11 @@ int abd_enable_timing(AbdCtx* ctx, int on);
  * form is unavailable.  The state q is unchanged. */
 int abd_autotune(AbdCtx* ctx, int repeats, int* bodies_per_tile, int* per_iteration_launch, double* ms);
 
+/* No counterpart in the reference: sets the PCG layout that abd_autotune would choose
+ * (bodies_per_tile in [1, 5]; per_iteration_launch 0 = one cooperative launch per solve where the
+ * device supports it, 1 = one launch per iteration), e.g. to reuse a tuned result. */
+int abd_set_pcg_layout(AbdCtx* ctx, int bodies_per_tile, int per_iteration_launch);
+
 /* No counterpart in the reference (the learned warm start of the north-star configuration): every
  * later solve computes 37 features per body (q - q~, qdot, incident contact count, summed contact
  * gradient), standardises them with feat_mean/feat_std, runs the model (include/abd_mlp.h, 37 ->
