@@ -251,10 +251,11 @@ class NewtonAbd:
         _lib.check(_lib.load().abd_log_samples(self._h, int(on)), self._h, "abd_log_samples")
 
     def samples(self):
-        """The logged (features[n, 37], du[n, 12]) samples, float32."""
+        """The logged (features[n, N_FEATURES], du[n, 12]) samples, float32 (colb_amd.learned.N_FEATURES)."""
+        from .learned import N_FEATURES
         n = C.c_int(0)
         _lib.check(_lib.load().abd_get_samples(self._h, None, None, 0, C.byref(n)), self._h, "abd_get_samples")
-        f = np.zeros((n.value, 37), dtype=np.float32)
+        f = np.zeros((n.value, N_FEATURES), dtype=np.float32)
         d = np.zeros((n.value, 12), dtype=np.float32)
         fp = f.ctypes.data_as(C.POINTER(C.c_float))
         dp = d.ctypes.data_as(C.POINTER(C.c_float))

@@ -116,6 +116,9 @@ struct AbdCtx {
     unsigned* h_pcg_ctl = nullptr;
     bool coop_ok = true;          // cleared when a cooperative launch is refused
     bool pcg_multi = false;       // solve with one launch per iteration (autotuner choice)
+    // mixed precision (AbdParams.precision == 1): FP32 copies of the PCG operands and vectors
+    float *f_vec = nullptr, *f_diag = nullptr, *f_pinv = nullptr, *f_off = nullptr, *f_offT = nullptr;
+    int f_pair_cap = 0;
     // learned warm start (abd_set_warm_start) and training-sample log (abd_log_samples)
     AbdMlp* mlp = nullptr;
     float *feat_raw = nullptr, *fmean = nullptr, *fistd = nullptr, *omean = nullptr, *ostd = nullptr;
@@ -550,7 +553,7 @@ int assemble(AbdCtx* ctx) {
 int warm_start(AbdCtx* ctx) {
     AbdMlp* m = ctx->mlp;
     launch_body_features(ctx->st, ctx->G, ctx->q, ctx->qtilde, ctx->qdot, ctx->n_inc, ctx->inc_s, ctx->act_g,
-                         ctx->n_act, ctx->feat_raw);
+                         ctx->n_act, ctx->pinv, ctx->grad, ctx->feat_raw);
     CHK(dbg(ctx, "launch_body_features"));
     int rc = mlp_reserve(m, ctx->nb);
     if (rc != ABD_OK) {
@@ -560,7 +563,7 @@ int warm_start(AbdCtx* ctx) {
     const int rows = (ctx->nb + 15) / 16 * 16;
     launch_features_pack(ctx->st, ctx->nb, rows, m->dp[0], ctx->feat_raw, ctx->fmean, ctx->fistd, m->A[0]);
     mlp_forward_dev(m, ctx->nb, ctx->st);
-    launch_warm_x0(ctx->st, ctx->G, m->dp[m->L], m->Z[m->L - 1], ctx->omean, ctx->ostd, ctx->x0);
+    launch_warm_x0(ctx->st, ctx->G, m->dp[m->L], m->Z[m->L - 1], ctx->omean, ctx->ostd, ctx->feat_raw, ctx->x0);
     CHK(dbg(ctx, "warm start forward"));
     launch_residual(ctx->st, ctx->nb, ctx->grad, ctx->diag, ctx->row_ptr, ctx->ent_ks, ctx->ent_vs, ctx->off,
                     ctx->offT, ctx->x0, ctx->rhs, ctx->n_pairs, ctx->cnt + C_ERR);
@@ -582,7 +585,7 @@ int solve_finish(AbdCtx* ctx, bool warm) {
     }
     if (ctx->log_on) {
         launch_body_features(ctx->st, ctx->G, ctx->q, ctx->qtilde, ctx->qdot, ctx->n_inc, ctx->inc_s, ctx->act_g,
-                             ctx->n_act, ctx->feat_raw);
+                             ctx->n_act, ctx->pinv, ctx->grad, ctx->feat_raw);
         const int nb = ctx->nb;
         std::vector<float> f((size_t)ABD_N_FEATURES * nb);
         std::vector<double> du(12 * (size_t)nb);
@@ -609,7 +612,8 @@ int solve_finish(AbdCtx* ctx, bool warm) {
 
 // one cooperative launch for the whole solve (k_pcg_persist); false when the launch was refused or
 // a grid barrier timed out, in which case the caller solves again with one launch per iteration
-int solve_persistent(AbdCtx* ctx, const PcgBuf& B, bool* ok) {
+template <typename T>
+int solve_persistent(AbdCtx* ctx, const PcgBufT<T>& B, bool* ok) {
     *ok = false;
     if (!ctx->coop_ok || ctx->pcg_multi || (ctx->P.reserved0 & 8)) return ABD_OK;
     HIPCHK(hipMemsetAsync(ctx->cg_iters, 0, sizeof(int) * ctx->ns, ctx->st));
@@ -626,13 +630,15 @@ int solve_persistent(AbdCtx* ctx, const PcgBuf& B, bool* ok) {
     CHK(dbg(ctx, "launch_pcg_persist"));
     HIPCHK(hipMemcpyAsync(ctx->h_pcg_ctl, ctx->pcg_ctl, 8 * sizeof(unsigned), hipMemcpyDeviceToHost, ctx->st));
     if (ctx->T.on) {
-        // algorithmic bytes of the launch: K iterations x [nnzb * 1156 + nb * (8 * 96 + 1152)]
+        // algorithmic bytes of the launch: K iterations x [nnzb * (144 T + 4) + nb * (8 * 12 T + 144 T)]
         std::vector<int> it(ctx->ns);
         HIPCHK(hipMemcpyAsync(it.data(), ctx->cg_iters, sizeof(int) * ctx->ns, hipMemcpyDeviceToHost, ctx->st));
         HIPCHK(hipStreamSynchronize(ctx->st));
         const int K = it.empty() ? 0 : *std::max_element(it.begin(), it.end());
         const double nnzb = (double)ctx->nb + 2.0 * ctx->n_pairs;
-        if (!ctx->T.open.empty()) ctx->T.open.back().bytes = (double)K * (nnzb * 1156.0 + (double)ctx->nb * (8 * 96 + 1152));
+        const double t = sizeof(T);
+        if (!ctx->T.open.empty())
+            ctx->T.open.back().bytes = (double)K * (nnzb * (144 * t + 4) + (double)ctx->nb * (8 * 12 * t + 144 * t));
     }
     HIPCHK(hipStreamSynchronize(ctx->st));
     if (ctx->h_pcg_ctl[4]) {
@@ -643,32 +649,41 @@ int solve_persistent(AbdCtx* ctx, const PcgBuf& B, bool* ok) {
     return ABD_OK;
 }
 
-int solve(AbdCtx* ctx) {
-    const int nt = ctx->ntiles;
-    PcgBuf B;
-    B.b = ctx->grad; B.diag = ctx->diag; B.pinv = ctx->pinv; B.row_ptr = ctx->row_ptr; B.ent_key = ctx->ent_ks;
-    B.ent_val = ctx->ent_vs; B.off = ctx->off; B.offT = ctx->offT; B.x = ctx->du; B.p = ctx->p0; B.u = ctx->z;
-    B.r[0] = ctx->r; B.r[1] = ctx->r1; B.w[0] = ctx->Ap; B.w[1] = ctx->w1; B.sv[0] = ctx->s0; B.sv[1] = ctx->p1;
+// FP32 buffers of the mixed-precision PCG, sized for the current pair count
+int ensure_f32(AbdCtx* ctx) {
+    const int nb = ctx->nb;
+    if (!ctx->f_vec) {
+        CHK(dalloc(ctx, &ctx->f_vec, 11 * 12 * (size_t)nb));
+        CHK(dalloc(ctx, &ctx->f_diag, 144 * (size_t)nb));
+        CHK(dalloc(ctx, &ctx->f_pinv, 144 * (size_t)nb));
+    }
+    if (ctx->f_pair_cap < ctx->pair_cap || !ctx->f_off) {
+        CHK(dfree(ctx, ctx->f_off));
+        CHK(dfree(ctx, ctx->f_offT));
+        CHK(dalloc(ctx, &ctx->f_off, 144 * (size_t)std::max(ctx->pair_cap, 1)));
+        CHK(dalloc(ctx, &ctx->f_offT, 144 * (size_t)std::max(ctx->pair_cap, 1)));
+        ctx->f_pair_cap = ctx->pair_cap;
+    }
+    return ABD_OK;
+}
+
+template <typename T>
+void pcg_ctl(AbdCtx* ctx, PcgBufT<T>& B) {
+    B.row_ptr = ctx->row_ptr; B.ent_key = ctx->ent_ks; B.ent_val = ctx->ent_vs;
     B.part = ctx->part; B.sc = ctx->pcg_sc; B.bnorm = ctx->bnorm; B.cg_iters = ctx->cg_iters; B.cg_done = ctx->cg_done;
     B.err = ctx->cnt + C_ERR;
     B.bar = ctx->pcg_ctl; B.done_cnt = (int*)(ctx->pcg_ctl + 1); B.fail = (int*)(ctx->pcg_ctl + 4);
     B.bnorm_out = ctx->bnorm; B.residual_out = ctx->C.residual;
     B.bref = nullptr;
-    const bool warm = ctx->mlp != nullptr;
-    if (warm) {
-        // x0 from the model; PCG then solves H e = b - H x0 (stopping relative to ||b||), du = x0 + e
-        CHK(warm_start(ctx));
-        B.b = ctx->rhs;
-        B.bref = ctx->nrm_b;
-    }
-    {
-        bool ok = false;
-        CHK(solve_persistent(ctx, B, &ok));
-        if (ok) {
-            CHK(solve_finish(ctx, warm));
-            return ABD_OK;
-        }
-    }
+}
+
+// the PCG proper: one cooperative launch where possible, else one launch per iteration
+template <typename T>
+int run_pcg(AbdCtx* ctx, const PcgBufT<T>& B, bool warm) {
+    bool ok = false;
+    CHK(solve_persistent(ctx, B, &ok));
+    if (ok) return ABD_OK;
+    const int nt = ctx->ntiles;
     const double tol = ctx->P.cg_tol;
     HIPCHK(hipMemsetAsync(ctx->cg_iters, 0, sizeof(int) * ctx->ns, ctx->st));
     HIPCHK(hipMemsetAsync(ctx->cg_done, 0, sizeof(int) * ctx->ns, ctx->st));
@@ -681,7 +696,9 @@ int solve(AbdCtx* ctx) {
     hipLaunchKernelGGL(k_copy, dim3(cdiv(ctx->ns, 64)), dim3(64), 0, ctx->st, ctx->ns, ctx->bnorm, ctx->C.residual);
     CHK(dbg(ctx, "k_copy"));
     const int nblk = ctx->nb + 2 * ctx->n_pairs;
-    const double bytes = (double)nblk * 1152 + (double)ctx->n_pairs * 2 * (1152 + 3 * 96) + (double)ctx->nb * (1152 + 14 * 96);
+    const double t = sizeof(T);
+    const double bytes = (double)nblk * 144 * t + (double)ctx->n_pairs * 2 * (144 * t + 3 * 12 * t) +
+                         (double)ctx->nb * (144 * t + 14 * 12 * t);
     int poll = 4;
     for (int it = 0; it < ctx->P.cg_max_iter; ++it) {
         tic(ctx, "pcg_iter", bytes);
@@ -698,6 +715,52 @@ int solve(AbdCtx* ctx) {
             if (!ctx->h_cnt[C_ANY]) break;
             poll += std::min(32, std::max(4, poll / 2));
         }
+    }
+    return ABD_OK;
+}
+
+int solve(AbdCtx* ctx) {
+    const bool warm = ctx->mlp != nullptr;
+    const double* b = ctx->grad;
+    const double* bref = nullptr;
+    if (warm) {
+        // x0 from the model; PCG then solves H e = b - H x0 (stopping relative to ||b||), du = x0 + e
+        CHK(warm_start(ctx));
+        b = ctx->rhs;
+        bref = ctx->nrm_b;
+    }
+    if (ctx->P.precision == 1) {
+        // mixed precision: the assembled FP64 system is rounded once to FP32 and solved in FP32
+        // (partial sums and scalars stay FP64); the FP32 solution is widened into du
+        CHK(ensure_f32(ctx));
+        const size_t n12 = 12 * (size_t)ctx->nb;
+        float* v = ctx->f_vec;
+        PcgBuf32 B;
+        pcg_ctl(ctx, B);
+        B.bref = bref;
+        B.b = v; B.x = v + n12; B.p = v + 2 * n12; B.u = v + 3 * n12;
+        B.r[0] = v + 4 * n12; B.r[1] = v + 5 * n12; B.w[0] = v + 6 * n12; B.w[1] = v + 7 * n12;
+        B.sv[0] = v + 8 * n12; B.sv[1] = v + 9 * n12;
+        B.diag = ctx->f_diag; B.pinv = ctx->f_pinv; B.off = ctx->f_off; B.offT = ctx->f_offT;
+        tic(ctx, "pcg_downcast", (double)(n12 + 288 * (size_t)ctx->nb + 288 * (size_t)ctx->n_pairs) * 12);
+        launch_to_f32(ctx->st, (long)n12, b, v);
+        launch_to_f32(ctx->st, 144 * (long)ctx->nb, ctx->diag, ctx->f_diag);
+        launch_to_f32(ctx->st, 144 * (long)ctx->nb, ctx->pinv, ctx->f_pinv);
+        launch_to_f32(ctx->st, 144 * (long)ctx->n_pairs, ctx->off, ctx->f_off);
+        launch_to_f32(ctx->st, 144 * (long)ctx->n_pairs, ctx->offT, ctx->f_offT);
+        toc(ctx);
+        CHK(dbg(ctx, "launch_to_f32"));
+        CHK(run_pcg(ctx, B, warm));
+        launch_to_f64(ctx->st, (long)n12, B.x, ctx->du);
+        CHK(dbg(ctx, "launch_to_f64"));
+    } else {
+        PcgBuf B;
+        pcg_ctl(ctx, B);
+        B.bref = bref;
+        B.b = b; B.diag = ctx->diag; B.pinv = ctx->pinv; B.off = ctx->off; B.offT = ctx->offT;
+        B.x = ctx->du; B.p = ctx->p0; B.u = ctx->z;
+        B.r[0] = ctx->r; B.r[1] = ctx->r1; B.w[0] = ctx->Ap; B.w[1] = ctx->w1; B.sv[0] = ctx->s0; B.sv[1] = ctx->p1;
+        CHK(run_pcg(ctx, B, warm));
     }
     CHK(solve_finish(ctx, warm));
     return ABD_OK;
@@ -969,6 +1032,10 @@ int abd_create(const AbdSceneDesc* d, const AbdParams* p, AbdCtx** out) {
         double l = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
         if (l > 0) { n[0] /= l; n[1] /= l; n[2] /= l; }
     }
+    if (p->precision != 0 && p->precision != 1) {
+        ctx->err = "precision must be 0 (FP64) or 1 (mixed)";
+        return fail(ABD_ERR_ARG);
+    }
     ctx->nb = d->n_bodies; ctx->nv = d->n_verts; ctx->ne = d->n_edges; ctx->nt = d->n_tris; ctx->ns = d->n_scenes;
     const int nb = ctx->nb, nv = ctx->nv, ne = ctx->ne, nt = ctx->nt, ns = ctx->ns;
     if (nb <= 0 || nv <= 0 || ns <= 0 || ns > 65535 || nb >= (1 << 20)) {
@@ -1229,6 +1296,10 @@ int abd_get_params(const AbdCtx* ctx, AbdParams* out) {
 
 int abd_set_params(AbdCtx* ctx, const AbdParams* p) {
     if (!ctx || !p) return ABD_ERR_ARG;
+    if (p->precision != 0 && p->precision != 1) {
+        ctx->err = "precision must be 0 (FP64) or 1 (mixed)";
+        return ABD_ERR_ARG;
+    }
     ctx->P = *p;
     double* n = ctx->P.ground_normal;
     double l = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
@@ -1544,7 +1615,8 @@ int abd_set_warm_start(AbdCtx* ctx, AbdMlp* model, const float* feat_mean, const
     }
     if (!feat_mean || !feat_std || !out_mean || !out_std || model->L < 1 || model->d[0] != ABD_N_FEATURES ||
         model->d[model->L] != 12) {
-        ctx->err = "warm-start model must map 37 body features to 12 DOFs, with normalisation arrays";
+        ctx->err = "warm-start model must map " + std::to_string(ABD_N_FEATURES) +
+                   " body features to 12 DOFs, with normalisation arrays";
         return ABD_ERR_ARG;
     }
     std::vector<float> istd(ABD_N_FEATURES);

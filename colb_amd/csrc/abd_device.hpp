@@ -144,15 +144,16 @@ struct DistEval {
     D3 vxa, axu;
 };
 
+// Stencil coefficients per type (PT: a = y0 - y1, u = y2 - y1, v = y3 - y1; EE: a = y0 - y2,
+// u = y1 - y0, v = y3 - y2; PE: u = y1 - y0, v = y2 - y0, w = y2 - y1; PP: a = y0 - y1), written
+// as blends of the type flags: branches that store into the arrays leave them in scratch memory.
 __device__ __forceinline__ void set_coeffs(DistEval& E) {
-    for (int k = 0; k < 4; ++k) E.al[k] = E.mu[k] = E.nu[k] = E.om[k] = 0.0;
-    if (E.S.type == T_PT) {
-        E.al[0] = 1; E.al[1] = -1; E.mu[1] = -1; E.mu[2] = 1; E.nu[1] = -1; E.nu[3] = 1;
-    } else if (E.S.type == T_EE) {
-        E.al[0] = 1; E.al[2] = -1; E.mu[0] = -1; E.mu[1] = 1; E.nu[2] = -1; E.nu[3] = 1;
-    } else if (E.S.type == T_PE) {
-        E.mu[0] = -1; E.mu[1] = 1; E.nu[0] = -1; E.nu[2] = 1; E.om[1] = -1; E.om[2] = 1;
-    }
+    const double pt = (double)(E.S.type == T_PT), ee = (double)(E.S.type == T_EE), pe = (double)(E.S.type == T_PE),
+                 pp = (double)(E.S.type == T_PP);
+    E.al[0] = pt + ee + pp; E.al[1] = -pt - pp; E.al[2] = -ee; E.al[3] = 0.0;
+    E.mu[0] = -ee - pe; E.mu[1] = -pt + ee + pe; E.mu[2] = pt; E.mu[3] = 0.0;
+    E.nu[0] = -pe; E.nu[1] = -pt; E.nu[2] = -ee + pe; E.nu[3] = pt + ee;
+    E.om[0] = 0.0; E.om[1] = -pe; E.om[2] = pe; E.om[3] = 0.0;
 }
 
 // Runtime-indexed reads of small register arrays as exact 0/1 blends (finite operands): a
@@ -173,17 +174,12 @@ __device__ __forceinline__ void dist_eval(int kind, const D3* x, DistEval& E) {
 #pragma unroll
     for (int k = 0; k < 4; ++k) E.y[k] = pick4(x, k < E.S.ns ? E.S.st[k] : 0);
     set_coeffs(E);
-    const int ns = E.S.ns;
-    if (E.S.type == T_PP) {
-        D3 d = E.y[0] - E.y[1];
-        E.d2 = dot(d, d);
-        E.a = d;
-        return;
-    }
+    // Every field is written on every path, unused ones from zero coefficients (slots k >= ns have
+    // zero coefficients and finite y): stores under type branches are merged by the compiler into
+    // stores through a selected address, which leaves the struct in scratch memory.
     E.a = mk(0, 0, 0); E.u = mk(0, 0, 0); E.v = mk(0, 0, 0); E.w = mk(0, 0, 0);
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        if (k >= ns) break;
         E.a = E.a + E.al[k] * E.y[k];
         E.u = E.u + E.mu[k] * E.y[k];
         E.v = E.v + E.nu[k] * E.y[k];
@@ -191,15 +187,12 @@ __device__ __forceinline__ void dist_eval(int kind, const D3* x, DistEval& E) {
     }
     E.n = cross(E.u, E.v);
     E.N = dot(E.n, E.n);
-    if (E.S.type == T_PE) {
-        E.L = dot(E.w, E.w);
-        E.d2 = E.N / E.L;
-    } else {
-        E.s = dot(E.a, E.n);
-        E.d2 = E.s * E.s / E.N;
-        E.vxa = cross(E.v, E.a);
-        E.axu = cross(E.a, E.u);
-    }
+    E.L = dot(E.w, E.w);
+    E.s = dot(E.a, E.n);
+    E.vxa = cross(E.v, E.a);
+    E.axu = cross(E.a, E.u);
+    const int ty = E.S.type;
+    E.d2 = (ty == T_PP) ? dot(E.a, E.a) : (ty == T_PE) ? E.N / E.L : E.s * E.s / E.N;
 }
 
 // distance only (for CCD and energies)

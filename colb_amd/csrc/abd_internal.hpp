@@ -83,22 +83,25 @@ struct SceneCtl {
     double* E1;
 };
 
-// Buffers of the block-Jacobi PCG (abd_kernels.hip, k_pcg_cg); r, w, sv are ping-pong pairs.
-struct PcgBuf {
-    const double* b;           // right-hand side (gradient)
-    const double* diag;
-    const double* pinv;
+// Buffers of the block-Jacobi PCG (abd_kernels.hip, k_pcg_cg); r, w, sv are ping-pong pairs.  T is
+// the working precision of the matrix, the preconditioner and the vectors (double, or float in the
+// mixed-precision mode); partial sums and per-scene scalars are double in both.
+template <typename T>
+struct PcgBufT {
+    const T* b;                // right-hand side (gradient)
+    const T* diag;
+    const T* pinv;
     const int* row_ptr;
     const uint64_t* ent_key;
     const int* ent_val;
-    const double* off;
-    const double* offT;
-    double* x;
-    double* p;
-    double* u;
-    double* r[2];
-    double* w[2];
-    double* sv[2];
+    const T* off;
+    const T* offT;
+    T* x;
+    T* p;
+    T* u;
+    T* r[2];
+    T* w[2];
+    T* sv[2];
     double* part;
     double* sc;                // [4 * ns]: g by parity, then alpha by parity
     const double* bnorm;
@@ -113,6 +116,8 @@ struct PcgBuf {
     double* residual_out;
     const double* bref;        // optional per-scene reference norm for the stopping test (warm start)
 };
+using PcgBuf = PcgBufT<double>;
+using PcgBuf32 = PcgBufT<float>;
 
 struct GrowBuf {
     void* p = nullptr;
@@ -143,13 +148,16 @@ void launch_pair_queries(hipStream_t s, const Geo& G, const BodyMesh& M, int npa
 void launch_ground_queries(hipStream_t s, const Geo& G, const double4* X0, const double4* X1, const AbdParams& P,
                            double infl, uint64_t* cand, int cap, int* count, int* overflow);
 // abd_kernels.hip
-constexpr int ABD_N_FEATURES = 37;
+constexpr int ABD_N_FEATURES = 49;
+constexpr int ABD_FEAT_Z = 37;          // features 37..48: the block-Jacobi step D_b^-1 grad_b
+constexpr float ABD_FEAT_CLIP = 8.0f;   // standardised features are clipped to this magnitude
 void launch_body_features(hipStream_t s, const Geo& G, const double* q, const double* qtilde, const double* qdot,
-                          int n_inc, const uint64_t* inc, const double* g24, int n_act, float* feat);
+                          int n_inc, const uint64_t* inc, const double* g24, int n_act, const double* pinv,
+                          const double* grad, float* feat);
 void launch_features_pack(hipStream_t s, int nb, int rows, int dp, const float* feat, const float* mean,
                           const float* istd, uint16_t* out);
 void launch_warm_x0(hipStream_t s, const Geo& G, int dpo, const float* Y, const float* omean, const float* ostd,
-                    double* x0);
+                    const float* feat, double* x0);
 void launch_residual(hipStream_t s, int nb, const double* b, const double* diag, const int* row_ptr,
                      const uint64_t* ent_key, const int* ent_val, const double* off, const double* offT,
                      const double* x, double* r, int n_pairs, int* err);
@@ -183,11 +191,16 @@ void launch_scene_sum(hipStream_t s, const Geo& G, const double* e_body, const d
                       const int* cand_hi, double* out);
 void launch_pcg_tile_sum(hipStream_t s, int ns, const int* scene_tile0, const double* part, int stride, int which,
                          double* out, int do_sqrt);
+template <typename T>
 void launch_pcg_cg(hipStream_t s, int k, int ntiles, int ns, const int* tile_b0, const int* tile_b1,
-                   const int* tile_scene, const int* scene_tile0, const PcgBuf& B, double tol, int nb, int n_pairs);
+                   const int* tile_scene, const int* scene_tile0, const PcgBufT<T>& B, double tol, int nb, int n_pairs);
+template <typename T>
 hipError_t launch_pcg_persist(hipStream_t s, int ntiles, int ns, const int* tile_b0, const int* tile_b1,
-                              const int* tile_scene, const int* scene_tile0, const PcgBuf& B, double tol, int max_iter,
-                              int nb, int n_pairs);
+                              const int* tile_scene, const int* scene_tile0, const PcgBufT<T>& B, double tol,
+                              int max_iter, int nb, int n_pairs);
+// mixed precision: FP64 -> FP32 copies of the PCG operands, FP32 -> FP64 of the solution
+void launch_to_f32(hipStream_t s, long n, const double* a, float* b);
+void launch_to_f64(hipStream_t s, long n, const float* a, double* b);
 void launch_pcg_any(hipStream_t s, int ns, int ntiles, const int* scene_tile0, const double* part, int it,
                     const double* bnorm, double tol, const int* cg_done, int* any);
 void launch_ccd(hipStream_t s, const Geo& G, int n, const uint64_t* keys, const double4* X0, const double4* X1,

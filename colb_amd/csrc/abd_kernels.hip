@@ -626,15 +626,17 @@ __device__ __forceinline__ bool pcg_done(const double* part, int ntiles, int par
 // first tile.  A scene stops (sticky flag) once sqrt(rho_i) <= tol * ||b||.
 
 // row of the neighbour's new preconditioned residual: (P_j (r_j - alpha (w_j + beta s_j)))_row
-__device__ __forceinline__ double pcg_r_next(double r, double w, double s, double alpha, double beta) {
+template <typename T>
+__device__ __forceinline__ T pcg_r_next(T r, T w, T s, T alpha, T beta) {
     return r - alpha * (w + beta * s);
 }
 
+template <typename T>
 __global__ __launch_bounds__(64) void k_pcg_cg(int k, int ntiles, int ns, const int* __restrict__ tb0,
                                                 const int* __restrict__ tb1, const int* __restrict__ tscene,
-                                                const int* __restrict__ st0, PcgBuf B, double tol, int nb,
+                                                const int* __restrict__ st0, PcgBufT<T> B, double tol, int nb,
                                                 int n_pairs) {
-    __shared__ double vec[TILE_BODIES + 1][12];  // lanes 60-63 (lb = 5) only touch the last row
+    __shared__ T vec[TILE_BODIES + 1][12];  // lanes 60-63 (lb = 5) only touch the last row
     const int t = blockIdx.x, lane = threadIdx.x;
     const int s = tscene[t];
     const int t0 = st0[s], t1 = st0[s + 1];
@@ -668,28 +670,29 @@ __global__ __launch_bounds__(64) void k_pcg_cg(int k, int ntiles, int ns, const 
     const bool on = lb < nbt;
     const int bb = b0 + (on ? lb : 0);
     const int idx = 12 * bb + row;
-    double rn = 0.0;
+    const T al = (T)alpha, be = (T)beta;
+    T rn = 0;
     if (on) {
         if (init) {
             rn = B.b[idx];
-            B.x[idx] = 0.0;
-            B.p[idx] = 0.0;
-            B.sv[wr][idx] = 0.0;
+            B.x[idx] = 0;
+            B.p[idx] = 0;
+            B.sv[wr][idx] = 0;
         } else {
-            const double pv = B.u[idx] + beta * B.p[idx];
+            const T pv = B.u[idx] + be * B.p[idx];
             B.p[idx] = pv;
-            const double sval = B.w[rd][idx] + beta * B.sv[rd][idx];
+            const T sval = B.w[rd][idx] + be * B.sv[rd][idx];
             B.sv[wr][idx] = sval;
-            B.x[idx] += alpha * pv;
-            rn = pcg_r_next(B.r[rd][idx], B.w[rd][idx], B.sv[rd][idx], alpha, beta);
+            B.x[idx] += al * pv;
+            rn = pcg_r_next(B.r[rd][idx], B.w[rd][idx], B.sv[rd][idx], al, be);
         }
         B.r[wr][idx] = rn;
         vec[lb][row] = rn;
     }
     __syncthreads();
-    double un = 0.0;
+    T un = 0;
     if (on) {
-        const double* Pb = B.pinv + (size_t)bb * 144 + row * 12;
+        const T* Pb = B.pinv + (size_t)bb * 144 + row * 12;
 #pragma unroll
         for (int c = 0; c < 12; ++c) un += Pb[c] * vec[lb][c];
         B.u[idx] = un;
@@ -697,9 +700,9 @@ __global__ __launch_bounds__(64) void k_pcg_cg(int k, int ntiles, int ns, const 
     __syncthreads();
     if (on) vec[lb][row] = un;
     __syncthreads();
-    double wv = 0.0;
+    T wv = 0;
     if (on) {
-        const double* D = B.diag + (size_t)bb * 144 + row * 12;
+        const T* D = B.diag + (size_t)bb * 144 + row * 12;
 #pragma unroll
         for (int c = 0; c < 12; ++c) wv += D[c] * vec[lb][c];
     }
@@ -710,8 +713,8 @@ __global__ __launch_bounds__(64) void k_pcg_cg(int k, int ntiles, int ns, const 
     __syncthreads();
     for (int q = 0; q < mx; ++q) {
         const bool has = on && q < e1 - e0;
-        const double* Bk = nullptr;
-        double uc = 0.0;
+        const T* Bk = nullptr;
+        T uc = 0;
         if (has) {
             const int e = e0 + q;
             const int col = opaque((int)(B.ent_key[e] & 0xFFFFFFFFu));
@@ -720,11 +723,11 @@ __global__ __launch_bounds__(64) void k_pcg_cg(int k, int ntiles, int ns, const 
                 atomicOr(B.err, ERR_CSR);
             } else {
                 Bk = (opaque(v & 1) ? B.offT : B.off) + (size_t)opaque(v >> 1) * 144 + row * 12;
-                const double* Pc = B.pinv + (size_t)col * 144 + row * 12;
+                const T* Pc = B.pinv + (size_t)col * 144 + row * 12;
 #pragma unroll
                 for (int c = 0; c < 12; ++c) {
                     const int j = 12 * col + c;
-                    const double rj = init ? B.b[j] : pcg_r_next(B.r[rd][j], B.w[rd][j], B.sv[rd][j], alpha, beta);
+                    const T rj = init ? B.b[j] : pcg_r_next(B.r[rd][j], B.w[rd][j], B.sv[rd][j], al, be);
                     uc += Pc[c] * rj;
                 }
             }
@@ -738,7 +741,8 @@ __global__ __launch_bounds__(64) void k_pcg_cg(int k, int ntiles, int ns, const 
         __syncthreads();
     }
     if (on) B.w[wr][idx] = wv;
-    const double g = wave_sum(on ? rn * un : 0.0), d = wave_sum(on ? wv * un : 0.0), rr = wave_sum(on ? rn * rn : 0.0);
+    const double g = wave_sum(on ? (double)rn * un : 0.0), d = wave_sum(on ? (double)wv * un : 0.0),
+                 rr = wave_sum(on ? (double)rn * rn : 0.0);
     if (lane == 0) {
         B.part[(size_t)(0 * 2 + wr) * ntiles + t] = d;
         B.part[(size_t)(1 * 2 + wr) * ntiles + t] = g;
@@ -746,11 +750,16 @@ __global__ __launch_bounds__(64) void k_pcg_cg(int k, int ntiles, int ns, const 
     }
 }
 
+template <typename T>
 void launch_pcg_cg(hipStream_t s, int k, int ntiles, int ns, const int* tile_b0, const int* tile_b1,
-                   const int* tile_scene, const int* scene_tile0, const PcgBuf& B, double tol, int nb, int n_pairs) {
-    hipLaunchKernelGGL(k_pcg_cg, dim3(ntiles), dim3(64), 0, s, k, ntiles, ns, tile_b0, tile_b1, tile_scene, scene_tile0,
-                       B, tol, nb, n_pairs);
+                   const int* tile_scene, const int* scene_tile0, const PcgBufT<T>& B, double tol, int nb, int n_pairs) {
+    hipLaunchKernelGGL(k_pcg_cg<T>, dim3(ntiles), dim3(64), 0, s, k, ntiles, ns, tile_b0, tile_b1, tile_scene,
+                       scene_tile0, B, tol, nb, n_pairs);
 }
+template void launch_pcg_cg<double>(hipStream_t, int, int, int, const int*, const int*, const int*, const int*,
+                                    const PcgBufT<double>&, double, int, int);
+template void launch_pcg_cg<float>(hipStream_t, int, int, int, const int*, const int*, const int*, const int*,
+                                   const PcgBufT<float>&, double, int, int);
 
 // ---------------------------------------------------------------------------------------------
 // The same PCG as one cooperative launch per solve.  Each workgroup keeps its tile's rows (x, p,
@@ -767,10 +776,12 @@ void launch_pcg_cg(hipStream_t s, int k, int ntiles, int ns, const int* tile_b0,
 // ---------------------------------------------------------------------------------------------
 #define PCG_SPIN_LIMIT (1 << 22)
 
-__device__ __forceinline__ double aload(const double* p) {
-    return __hip_atomic_load(const_cast<double*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+template <typename T>
+__device__ __forceinline__ T aload(const T* p) {
+    return __hip_atomic_load(const_cast<T*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ __forceinline__ void astore(double* p, double v) {
+template <typename T>
+__device__ __forceinline__ void astore(T* p, T v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
@@ -813,16 +824,17 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+template <typename T>
 __global__ __launch_bounds__(64 * PCG_WAVES) void k_pcg_persist(int ntiles, int ns, const int* __restrict__ tb0,
                                                                  const int* __restrict__ tb1,
                                                                  const int* __restrict__ tscene,
-                                                                 const int* __restrict__ st0, PcgBuf B, double tol,
-                                                                 int max_iter, int nb, int n_pairs) {
-    __shared__ double vec_all[PCG_WAVES][TILE_BODIES + 1][12];
+                                                                 const int* __restrict__ st0, PcgBufT<T> B,
+                                                                 double tol, int max_iter, int nb, int n_pairs) {
+    __shared__ T vec_all[PCG_WAVES][TILE_BODIES + 1][12];
     const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
     const int t = blockIdx.x * PCG_WAVES + wave;
     const bool tile = t < ntiles;  // waves past the last tile only take part in the barriers
-    double(*vec)[12] = vec_all[wave];
+    T(*vec)[12] = vec_all[wave];
     const unsigned nblocks = gridDim.x;
     const int s = tile ? tscene[t] : 0;
     const int t0 = tile ? st0[s] : 0, t1 = tile ? st0[s + 1] : 0;
@@ -833,18 +845,18 @@ __global__ __launch_bounds__(64 * PCG_WAVES) void k_pcg_persist(int ntiles, int 
     const int idx = 12 * bb + row;
     const bool lead = tile && (t == t0 && lane == 0);
     // static rows of this lane: P and the diagonal block
-    double Prow[12], Drow[12];
+    T Prow[12], Drow[12];
 #pragma unroll
     for (int c = 0; c < 12; ++c) {
-        Prow[c] = on ? B.pinv[(size_t)bb * 144 + row * 12 + c] : 0.0;
-        Drow[c] = on ? B.diag[(size_t)bb * 144 + row * 12 + c] : 0.0;
+        Prow[c] = on ? B.pinv[(size_t)bb * 144 + row * 12 + c] : T(0);
+        Drow[c] = on ? B.diag[(size_t)bb * 144 + row * 12 + c] : T(0);
     }
     const int e0 = on ? B.row_ptr[bb] : 0, e1 = on ? B.row_ptr[bb + 1] : 0;
     int mx = e1 - e0;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) mx = max(mx, __shfl_xor(mx, o, 64));
 
-    double x = 0.0, p = 0.0, u = 0.0, r = 0.0, w = 0.0, sv = 0.0;
+    T x = 0, p = 0, u = 0, r = 0, w = 0, sv = 0;
     double alpha = 0.0, beta = 0.0, g_prev = 1.0, a_prev = 1.0, bnorm = 0.0;
     bool done = !tile;
     unsigned nbar = 0;
@@ -886,14 +898,15 @@ __global__ __launch_bounds__(64 * PCG_WAVES) void k_pcg_persist(int ntiles, int 
             }
         }
         if (!done) {
-            double rn = 0.0, un = 0.0, wv = 0.0;
+            const T al = (T)alpha, be = (T)beta;
+            T rn = 0, un = 0, wv = 0;
             if (k == 0) {
-                rn = on ? B.b[idx] : 0.0;
+                rn = on ? B.b[idx] : T(0);
             } else {
-                p = u + beta * p;
-                rn = pcg_r_next(r, w, sv, alpha, beta);  // r_i - alpha (w_i + beta s_{i-1})
-                sv = w + beta * sv;                       // s_i
-                x += alpha * p;
+                p = u + be * p;
+                rn = pcg_r_next(r, w, sv, al, be);  // r_i - alpha (w_i + beta s_{i-1})
+                sv = w + be * sv;                   // s_i
+                x += al * p;
             }
             if (on) vec[lb][row] = rn;
             wave_lds_sync();
@@ -911,8 +924,8 @@ __global__ __launch_bounds__(64 * PCG_WAVES) void k_pcg_persist(int ntiles, int 
             wave_lds_sync();
             for (int q = 0; q < mx; ++q) {
                 const bool has = on && q < e1 - e0;
-                const double* Bk = nullptr;
-                double uc = 0.0;
+                const T* Bk = nullptr;
+                T uc = 0;
                 if (has) {
                     const int e = e0 + q;
                     const int col = opaque((int)(B.ent_key[e] & 0xFFFFFFFFu));
@@ -921,13 +934,13 @@ __global__ __launch_bounds__(64 * PCG_WAVES) void k_pcg_persist(int ntiles, int 
                         atomicOr(B.err, ERR_CSR);
                     } else {
                         Bk = (opaque(v & 1) ? B.offT : B.off) + (size_t)opaque(v >> 1) * 144 + row * 12;
-                        const double* Pc = B.pinv + (size_t)col * 144 + row * 12;
+                        const T* Pc = B.pinv + (size_t)col * 144 + row * 12;
 #pragma unroll
                         for (int c = 0; c < 12; ++c) {
                             const int j = 12 * col + c;
-                            const double rj = (k == 0) ? B.b[j]
-                                                       : pcg_r_next(aload(B.r[rd] + j), aload(B.w[rd] + j),
-                                                                    aload(B.sv[rd] + j), alpha, beta);
+                            const T rj = (k == 0) ? B.b[j]
+                                                  : pcg_r_next(aload(B.r[rd] + j), aload(B.w[rd] + j),
+                                                               aload(B.sv[rd] + j), al, be);
                             uc += Pc[c] * rj;
                         }
                     }
@@ -949,8 +962,8 @@ __global__ __launch_bounds__(64 * PCG_WAVES) void k_pcg_persist(int ntiles, int 
             r = rn;
             u = un;
             w = wv;
-            const double gp = wave_sum(on ? rn * un : 0.0), dp = wave_sum(on ? wv * un : 0.0),
-                         rr = wave_sum(on ? rn * rn : 0.0);
+            const double gp = wave_sum(on ? (double)rn * un : 0.0), dp = wave_sum(on ? (double)wv * un : 0.0),
+                         rr = wave_sum(on ? (double)rn * rn : 0.0);
             if (lane == 0) {
                 astore(B.part + (size_t)(0 * 2 + wr) * ntiles + t, dp);
                 astore(B.part + (size_t)(1 * 2 + wr) * ntiles + t, gp);
@@ -966,13 +979,33 @@ __global__ __launch_bounds__(64 * PCG_WAVES) void k_pcg_persist(int ntiles, int 
     if (on) B.x[idx] = x;
 }
 
+template <typename T>
 hipError_t launch_pcg_persist(hipStream_t s, int ntiles, int ns, const int* tile_b0, const int* tile_b1,
-                              const int* tile_scene, const int* scene_tile0, const PcgBuf& B, double tol, int max_iter,
-                              int nb, int n_pairs) {
+                              const int* tile_scene, const int* scene_tile0, const PcgBufT<T>& B, double tol,
+                              int max_iter, int nb, int n_pairs) {
     void* args[] = {(void*)&ntiles, (void*)&ns, (void*)&tile_b0, (void*)&tile_b1, (void*)&tile_scene,
                     (void*)&scene_tile0, (void*)&B, (void*)&tol, (void*)&max_iter, (void*)&nb, (void*)&n_pairs};
     const int blocks = (ntiles + PCG_WAVES - 1) / PCG_WAVES;
-    return hipLaunchCooperativeKernel((const void*)k_pcg_persist, dim3(blocks), dim3(64 * PCG_WAVES), args, 0, s);
+    return hipLaunchCooperativeKernel((const void*)k_pcg_persist<T>, dim3(blocks), dim3(64 * PCG_WAVES), args, 0, s);
+}
+template hipError_t launch_pcg_persist<double>(hipStream_t, int, int, const int*, const int*, const int*, const int*,
+                                               const PcgBufT<double>&, double, int, int, int);
+template hipError_t launch_pcg_persist<float>(hipStream_t, int, int, const int*, const int*, const int*, const int*,
+                                              const PcgBufT<float>&, double, int, int, int);
+
+__global__ void k_to_f32(long n, const double* __restrict__ a, float* __restrict__ b) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (float)a[i];
+}
+__global__ void k_to_f64(long n, const float* __restrict__ a, double* __restrict__ b) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (double)a[i];
+}
+void launch_to_f32(hipStream_t s, long n, const double* a, float* b) {
+    if (n > 0) hipLaunchKernelGGL(k_to_f32, dim3(cdiv(n, 256)), dim3(256), 0, s, n, a, b);
+}
+void launch_to_f64(hipStream_t s, long n, const float* a, double* b) {
+    if (n > 0) hipLaunchKernelGGL(k_to_f64, dim3(cdiv(n, 256)), dim3(256), 0, s, n, a, b);
 }
 
 // any scene still iterating after `it` completed iterations (residual of parity it&1)
@@ -998,7 +1031,8 @@ void launch_pcg_any(hipStream_t s, int ns, int ntiles, const int* scene_tile0, c
 // gradient (12), from the incidences of the last assembly
 __global__ void k_body_features(Geo G, const double* __restrict__ q, const double* __restrict__ qtilde,
                                 const double* __restrict__ qdot, int n_inc, const uint64_t* __restrict__ inc,
-                                const double* __restrict__ g24, int n_act, float* __restrict__ feat) {
+                                const double* __restrict__ g24, int n_act, const double* __restrict__ pinv,
+                                const double* __restrict__ grad, float* __restrict__ feat) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= G.nb) return;
     float* f = feat + (size_t)ABD_N_FEATURES * b;
@@ -1020,15 +1054,24 @@ __global__ void k_body_features(Geo G, const double* __restrict__ q, const doubl
     }
     f[24] = (float)(hi - lo);
     for (int j = 0; j < 12; ++j) f[25 + j] = (float)gs[j];
+    // block-Jacobi step z = D_b^-1 grad_b (the preconditioner's own guess)
+    const double* P = pinv + 144 * (size_t)b;
+    const double* g = grad + 12 * (size_t)b;
+    for (int i = 0; i < 12; ++i) {
+        double z = 0.0;
+        for (int j = 0; j < 12; ++j) z += P[12 * i + j] * g[j];
+        f[ABD_FEAT_Z + i] = (float)z;
+    }
 }
 
 void launch_body_features(hipStream_t s, const Geo& G, const double* q, const double* qtilde, const double* qdot,
-                          int n_inc, const uint64_t* inc, const double* g24, int n_act, float* feat) {
+                          int n_inc, const uint64_t* inc, const double* g24, int n_act, const double* pinv,
+                          const double* grad, float* feat) {
     hipLaunchKernelGGL(k_body_features, dim3(cdiv(G.nb, 128)), dim3(128), 0, s, G, q, qtilde, qdot, n_inc, inc, g24,
-                       n_act, feat);
+                       n_act, pinv, grad, feat);
 }
 
-// standardised features -> BF16 model input (rows x dp, zero padding)
+// standardised features, clipped to +-ABD_FEAT_CLIP -> BF16 model input (rows x dp, zero padding)
 __global__ void k_features_pack(int nb, int rows, int dp, const float* __restrict__ feat, const float* __restrict__ mean,
                                 const float* __restrict__ istd, uint16_t* __restrict__ out) {
     const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1036,7 +1079,7 @@ __global__ void k_features_pack(int nb, int rows, int dp, const float* __restric
     const int i = (int)(e / dp), j = (int)(e % dp);
     float v = 0.f;
     if (i < nb && j < ABD_N_FEATURES) v = (feat[(size_t)ABD_N_FEATURES * i + j] - mean[j]) * istd[j];
-    out[e] = f2bf(v);
+    out[e] = f2bf(fminf(fmaxf(v, -ABD_FEAT_CLIP), ABD_FEAT_CLIP));
 }
 
 void launch_features_pack(hipStream_t s, int nb, int rows, int dp, const float* feat, const float* mean,
@@ -1045,18 +1088,19 @@ void launch_features_pack(hipStream_t s, int nb, int rows, int dp, const float* 
     hipLaunchKernelGGL(k_features_pack, dim3(cdiv(n, 256)), dim3(256), 0, s, nb, rows, dp, feat, mean, istd, out);
 }
 
-// x0 = de-standardised model output (pinned bodies: 0)
+// x0 = z + de-standardised model output (the model predicts du - z; pinned bodies: 0)
 __global__ void k_warm_x0(Geo G, int dpo, const float* __restrict__ Y, const float* __restrict__ omean,
-                          const float* __restrict__ ostd, double* __restrict__ x0) {
+                          const float* __restrict__ ostd, const float* __restrict__ feat, double* __restrict__ x0) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= 12 * G.nb) return;
     const int b = e / 12, j = e % 12;
-    x0[e] = G.pinned[b] ? 0.0 : (double)(Y[(size_t)b * dpo + j] * ostd[j] + omean[j]);
+    const float z = feat[(size_t)ABD_N_FEATURES * b + ABD_FEAT_Z + j];
+    x0[e] = G.pinned[b] ? 0.0 : (double)z + (double)(Y[(size_t)b * dpo + j] * ostd[j] + omean[j]);
 }
 
 void launch_warm_x0(hipStream_t s, const Geo& G, int dpo, const float* Y, const float* omean, const float* ostd,
-                    double* x0) {
-    hipLaunchKernelGGL(k_warm_x0, dim3(cdiv(12 * G.nb, 256)), dim3(256), 0, s, G, dpo, Y, omean, ostd, x0);
+                    const float* feat, double* x0) {
+    hipLaunchKernelGGL(k_warm_x0, dim3(cdiv(12 * G.nb, 256)), dim3(256), 0, s, G, dpo, Y, omean, ostd, feat, x0);
 }
 
 // r = b - H x (H: diagonal blocks + CSR off-diagonal blocks), one thread per row

@@ -5,12 +5,14 @@ solution), which then seeds the PCG as its initial guess.  The model runs on the
 the C ABI of include/abd_mlp.h: BF16 weights read by the matrix cores with FP32 accumulation,
 FP32 master weights trained by Adam, forward and backward written by hand (abd_mlp.hip).
 
-Features per body (37, standardised with the training set's mean and standard deviation):
+Features per body (49, standardised with the training set's mean and standard deviation, then
+clipped to +-8 so that a state far outside the training set cannot blow the output up):
     q - q~ (12)          the DOFs relative to the inertial target q~ = q_now + dt qdot + dt^2 g
     qdot (12)            the DOF velocities
     n_contacts (1)       active contacts incident to the body
     g_contact (12)       the summed barrier gradient on the body
-Target per body (12): du, standardised the same way.
+    z (12)               the block-Jacobi step D_b^-1 grad_b (D_b the body's 12x12 diagonal block)
+Target per body (12): du - z, standardised the same way; the warm start is x0 = z + prediction.
 """
 import ctypes as C
 
@@ -18,7 +20,9 @@ import numpy as np
 
 from . import _lib
 
-N_FEATURES = 37
+N_FEATURES = 49
+FEAT_Z = 37
+FEAT_CLIP = 8.0
 N_OUT = 12
 DEFAULT_DIMS = (N_FEATURES, 128, 128, 128, N_OUT)
 
@@ -139,21 +143,26 @@ class WarmStart:
         self.out_mean = np.zeros(N_OUT, dtype=np.float32)
         self.out_std = np.ones(N_OUT, dtype=np.float32)
 
+    def _inputs(self, feat):
+        X = (np.asarray(feat, dtype=np.float64) - self.feat_mean) / self.feat_std
+        return np.clip(X, -FEAT_CLIP, FEAT_CLIP).astype(np.float32)
+
     def fit(self, feat, du, steps=3000, batch=512, lr=1e-3, seed=0):
         """Standardise, then train with Adam on the GPU; returns the batch losses."""
         feat = np.asarray(feat, dtype=np.float64)
-        du = np.asarray(du, dtype=np.float64)
+        t = np.asarray(du, dtype=np.float64) - feat[:, FEAT_Z:FEAT_Z + N_OUT]
         self.feat_mean = feat.mean(0).astype(np.float32)
         self.feat_std = np.maximum(feat.std(0), 1e-12).astype(np.float32)
-        self.out_mean = du.mean(0).astype(np.float32)
-        self.out_std = np.maximum(du.std(0), 1e-12).astype(np.float32)
-        X = ((feat - self.feat_mean) / self.feat_std).astype(np.float32)
-        T = ((du - self.out_mean) / self.out_std).astype(np.float32)
+        self.out_mean = t.mean(0).astype(np.float32)
+        self.out_std = np.maximum(t.std(0), 1e-12).astype(np.float32)
+        X = self._inputs(feat)
+        T = ((t - self.out_mean) / self.out_std).astype(np.float32)
         return self.mlp.train(X, T, steps=steps, batch=min(batch, X.shape[0]), lr=lr, seed=seed)
 
     def predict(self, feat):
-        X = ((np.asarray(feat, dtype=np.float64) - self.feat_mean) / self.feat_std).astype(np.float32)
-        return self.mlp.forward(X) * self.out_std + self.out_mean
+        """The warm start x0 = z + model output for each row of raw features."""
+        feat = np.asarray(feat, dtype=np.float64)
+        return feat[:, FEAT_Z:FEAT_Z + N_OUT] + self.mlp.forward(self._inputs(feat)) * self.out_std + self.out_mean
 
     def save(self, path):
         np.savez(path, dims=np.asarray(self.dims), params=self.mlp.get_params(), feat_mean=self.feat_mean,

@@ -174,11 +174,12 @@ int abd_autotune(AbdCtx* ctx, int repeats, int* bodies_per_tile, int* per_iterat
 int abd_set_pcg_layout(AbdCtx* ctx, int bodies_per_tile, int per_iteration_launch);
 
 /* No counterpart in the reference (the learned warm start of the north-star configuration): every
- * later solve computes 37 features per body (q - q~, qdot, incident contact count, summed contact
- * gradient), standardises them with feat_mean/feat_std, runs the model (include/abd_mlp.h, 37 ->
- * ... -> 12) on the context's stream and starts PCG from x0 = out * out_std + out_mean where that
- * leaves a smaller residual than zero (per scene; plain PCG otherwise).  The stopping test stays
- * relative to ||grad||.  model == NULL switches it off.  The model must outlive its use. */
+ * later solve computes 49 features per body (q - q~, qdot, incident contact count, summed contact
+ * gradient, block-Jacobi step z = D_b^-1 grad_b), standardises them with feat_mean/feat_std and
+ * clips them to +-8, runs the model (include/abd_mlp.h, 49 -> ... -> 12) on the context's stream and
+ * starts PCG from x0 = z + out * out_std + out_mean (the model predicts du - z) where that leaves a
+ * smaller residual than zero (per scene; plain PCG otherwise).  The stopping test stays relative to
+ * ||grad||.  model == NULL switches it off.  The model must outlive its use. */
 #ifndef ABD_MLP_TYPEDEF
 #define ABD_MLP_TYPEDEF
 typedef struct AbdMlp AbdMlp;
@@ -186,9 +187,9 @@ typedef struct AbdMlp AbdMlp;
 int abd_set_warm_start(AbdCtx* ctx, AbdMlp* model, const float* feat_mean, const float* feat_std,
                        const float* out_mean, const float* out_std);
 /* No counterpart in the reference (training data of the warm start): while on, every solve appends
- * (raw features[37], du[12]) of each unpinned body of each still-iterating scene; on/off clears. */
+ * (raw features[49], du[12]) of each unpinned body of each still-iterating scene; on/off clears. */
 int abd_log_samples(AbdCtx* ctx, int on);
-/* Copies up to cap logged samples (feat: cap x 37, du: cap x 12); *n = number logged. */
+/* Copies up to cap logged samples (feat: cap x 49, du: cap x 12); *n = number logged. */
 int abd_get_samples(const AbdCtx* ctx, float* feat, float* du, int cap, int* n);
 
 #ifdef __cplusplus
