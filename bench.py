@@ -18,22 +18,54 @@ sys.path.insert(0, ROOT)
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (each calibrated by a
+# 512 MiB device copy in the same run), per workload and precision
+TRAFFIC_FILES = {("chain_net_32x32", "fp64"): "profiles/chain_net_32x32_plain_fp64_hbm_bytes.json"}
+TIMER_KERNEL = {"pcg_solve": "abd::k_pcg_persist<{t}>", "pcg_iter": "abd::k_pcg_cg<{t}>",
+                "broad_phase_query": "abd::k_pair_queries", "contact_hess": "abd::k_contact_hess",
+                "body_assemble": "abd::k_body_assemble", "offblock_sum": "abd::k_offblock_sum",
+                "narrow_phase": "abd::k_cand_eval", "ccd": "abd::k_ccd", "energy": "abd::k_body_energy"}
+
+
+def measured_traffic(workload, precision, timer):
+    """HBM bytes per launch of the kernel behind `timer` from the committed counter summary."""
+    path = TRAFFIC_FILES.get((workload, precision))
+    if not path or timer not in TIMER_KERNEL or not os.path.exists(os.path.join(ROOT, path)):
+        return None, None
+    with open(os.path.join(ROOT, path)) as f:
+        summary = json.load(f)
+    name = TIMER_KERNEL[timer].format(t="double" if precision == "fp64" else "float")
+    k = summary["kernels"].get(name)
+    if not k:
+        return None, None
+    fb = k.get("fetch_bytes_profiled", k["fetch_bytes_mean"])
+    wb = k.get("write_bytes_profiled", k["write_bytes_mean"])
+    return float(fb + wb), path
 METRIC = "sim-ms per wall-ms + Newton iters to 1e-5 residual, learned vs plain ABD"
 
 
-def workload(name):
+def workload(name, precision="fp64"):
     from colb_amd import scene as S
     from colb_amd.params import north_star
+    prec = {"fp64": 0, "mixed": 1}[precision]
     if name == "chain_net_32x32":
         sc = S.chain_net(n=32)
-        p = north_star(dt=8e-3, dhat=1e-3, kappa=1e7, k_contact=1e20, max_newton=16, newton_tol=1e-5)
+        p = north_star(dt=8e-3, dhat=1e-3, kappa=1e7, k_contact=1e20, max_newton=16, newton_tol=1e-5, precision=prec)
         return sc, p, {"workload": "chain_net_32x32", "links": 1024, "verts_per_link": 288,
                        "tris_per_link": 576, "pinned": 4, "dt_ms": 8.0, "dhat": 1e-3, "kappa": 1e7,
-                       "k_contact": 1e20, "max_newton": 16, "newton_tol": 1e-5, "precision": "fp64"}
+                       "k_contact": 1e20, "max_newton": 16, "newton_tol": 1e-5, "precision": precision}
+    if name == "box_pile_4096":
+        sc = S.box_pile(n=4096)
+        q = sc.q0  # lowest vertex 5e-4 above the ground plane y = 0
+        y = q[sc.vert_body, 1] + (sc.rest_x * q[sc.vert_body][:, [4, 7, 10]]).sum(1)
+        sc.q0[:, 1] -= y.min() - 5e-4
+        p = north_star(dt=8e-3, ground_enabled=True, dhat=1e-3, kappa=1e7, k_contact=1e20, precision=prec)
+        return sc, p, {"workload": "box_pile_4096", "bodies": 4096, "edge": [0.2, 1.0], "seed": 4, "dt_ms": 8.0,
+                       "dhat": 1e-3, "kappa": 1e7, "k_contact": 1e20, "ground": True, "precision": precision}
     if name == "cubes_8":
         sc = S.cubes_on_ground()
-        p = north_star(dt=8e-3, ground_enabled=True, dhat=1e-3, kappa=1e7, k_contact=1e20)
-        return sc, p, {"workload": "cubes_8_on_ground", "bodies": 8, "dt_ms": 8.0, "dhat": 1e-3, "precision": "fp64"}
+        p = north_star(dt=8e-3, ground_enabled=True, dhat=1e-3, kappa=1e7, k_contact=1e20, precision=prec)
+        return sc, p, {"workload": "cubes_8_on_ground", "bodies": 8, "dt_ms": 8.0, "dhat": 1e-3, "precision": precision}
     raise SystemExit(f"unknown workload {name}")
 
 
@@ -62,7 +94,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="chain_net_32x32")
+    ap.add_argument("--workload", default="chain_net_32x32", choices=["chain_net_32x32", "box_pile_4096", "cubes_8"])
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"],
+                    help="mixed: FP32 PCG, FP64 near-contact distances, CCD and energies")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=2)
@@ -84,7 +118,7 @@ def main():
         b.copy_(a)
         torch.cuda.synchronize()
         del a, b
-    sc, p, cfg = workload(args.workload)
+    sc, p, cfg = workload(args.workload, args.precision)
     print(f"# workload {cfg['workload']}: {sc.n_bodies} bodies, {sc.n_verts} verts, {sc.n_tris} tris, "
           f"{sc.n_edges} edges", file=sys.stderr, flush=True)
     solver = NewtonAbd(p.dt, sc, params=p)
@@ -133,9 +167,10 @@ def main():
         avg_ms = d["ms"] / d["calls"]
         bytes_per = d["bytes"] / d["calls"]
         ach = bytes_per / (avg_ms * 1e-3) / 1e9
+        traffic, src = measured_traffic(cfg["workload"], args.precision, dom)
         roof = {"bound": "hbm", "kernel": dom, "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 5), "traffic": None, "avg_us": round(avg_ms * 1e3, 3),
-                "bytes_per_launch": bytes_per}
+                "frac": round(ach / HBM_PEAK_GBS, 5), "traffic": traffic, "avg_us": round(avg_ms * 1e3, 3),
+                "bytes_per_launch": bytes_per, "traffic_source": src}
     for k, v in kt.items():
         print(f"# kernel {k:20s} {v['ms']:9.3f} ms {v['calls']:6d} calls {v['bytes'] / max(v['ms'], 1e-9) / 1e6:8.1f} GB/s",
               file=sys.stderr, flush=True)
@@ -149,7 +184,8 @@ def main():
     result = {
         "metric": METRIC, "value": round(sim_per_wall, 6), "unit": "sim-ms/wall-ms", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32+f64",
         "data": "synthetic (procedural scene)", "config": cfg,
         "newton_iters_per_step": round(float(np.mean(iters)), 3), "cg_iters_per_step": round(float(np.mean(cg)), 1),
         "method": "plain", "autotune": tune,

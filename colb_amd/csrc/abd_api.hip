@@ -114,7 +114,8 @@ struct AbdCtx {
     double *r1 = nullptr, *w1 = nullptr, *s0 = nullptr, *pcg_sc = nullptr;
     unsigned* pcg_ctl = nullptr;  // persistent solve: [0] barrier count, [1..3] done counters, [4] failure flag
     unsigned* h_pcg_ctl = nullptr;
-    bool coop_ok = true;          // cleared when a cooperative launch is refused
+    bool coop_ok = true;          // cleared when a cooperative launch fails for a reason other than its size
+    int coop_blocks[2] = {-1, -1};  // co-resident k_pcg_persist workgroups (FP64, FP32); -1 = not queried
     bool pcg_multi = false;       // solve with one launch per iteration (autotuner choice)
     // mixed precision (AbdParams.precision == 1): FP32 copies of the PCG operands and vectors
     float *f_vec = nullptr, *f_diag = nullptr, *f_pinv = nullptr, *f_off = nullptr, *f_offT = nullptr;
@@ -612,10 +613,25 @@ int solve_finish(AbdCtx* ctx, bool warm) {
 
 // one cooperative launch for the whole solve (k_pcg_persist); false when the launch was refused or
 // a grid barrier timed out, in which case the caller solves again with one launch per iteration
+// does one cooperative launch of k_pcg_persist<T> over the current tiles fit on the device?
+template <typename T>
+bool persist_fits(AbdCtx* ctx) {
+    int& cap = ctx->coop_blocks[sizeof(T) == 4 ? 1 : 0];
+    if (cap < 0 && pcg_persist_max_blocks<T>(&cap) != hipSuccess) {
+        (void)hipGetLastError();
+        cap = 0;
+    }
+    return (ctx->ntiles + PCG_TILES_PER_BLOCK - 1) / PCG_TILES_PER_BLOCK <= cap;
+}
+
+bool persist_ok(AbdCtx* ctx) {
+    return ctx->coop_ok && (ctx->P.precision == 1 ? persist_fits<float>(ctx) : persist_fits<double>(ctx));
+}
+
 template <typename T>
 int solve_persistent(AbdCtx* ctx, const PcgBufT<T>& B, bool* ok) {
     *ok = false;
-    if (!ctx->coop_ok || ctx->pcg_multi || (ctx->P.reserved0 & 8)) return ABD_OK;
+    if (!ctx->coop_ok || ctx->pcg_multi || (ctx->P.reserved0 & 8) || !persist_fits<T>(ctx)) return ABD_OK;
     HIPCHK(hipMemsetAsync(ctx->cg_iters, 0, sizeof(int) * ctx->ns, ctx->st));
     HIPCHK(hipMemsetAsync(ctx->pcg_ctl, 0, 8 * sizeof(unsigned), ctx->st));
     tic(ctx, "pcg_solve");
@@ -1573,11 +1589,11 @@ int abd_autotune(AbdCtx* ctx, int repeats, int* bodies_per_tile, int* per_iterat
         for (int multi = 0; multi < 2; ++multi) {
             const int slot = 2 * (bpt - 1) + multi;
             if (ms) ms[slot] = -1.0;
-            if (!multi && !ctx->coop_ok) continue;
+            if (!multi && !persist_ok(ctx)) continue;
             ctx->pcg_multi = multi != 0;
             CHK(solve(ctx));  // warm-up (and the cooperative-launch probe)
             HIPCHK(hipStreamSynchronize(ctx->st));
-            if (!multi && !ctx->coop_ok) continue;
+            if (!multi && !persist_ok(ctx)) continue;
             auto t0 = std::chrono::steady_clock::now();
             for (int r = 0; r < repeats; ++r) CHK(solve(ctx));
             HIPCHK(hipStreamSynchronize(ctx->st));

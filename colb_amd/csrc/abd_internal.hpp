@@ -198,6 +198,9 @@ template <typename T>
 hipError_t launch_pcg_persist(hipStream_t s, int ntiles, int ns, const int* tile_b0, const int* tile_b1,
                               const int* tile_scene, const int* scene_tile0, const PcgBufT<T>& B, double tol,
                               int max_iter, int nb, int n_pairs);
+template <typename T>
+hipError_t pcg_persist_max_blocks(int* out);
+constexpr int PCG_TILES_PER_BLOCK = 4;  // waves (tiles) per workgroup of k_pcg_persist
 // mixed precision: FP64 -> FP32 copies of the PCG operands, FP32 -> FP64 of the solution
 void launch_to_f32(hipStream_t s, long n, const double* a, float* b);
 void launch_to_f64(hipStream_t s, long n, const float* a, double* b);

@@ -814,7 +814,7 @@ __device__ bool grid_barrier(unsigned* bar, int* fail, unsigned k, unsigned nblo
     return ok_sh != 0;
 }
 
-#define PCG_WAVES 4
+#define PCG_WAVES PCG_TILES_PER_BLOCK
 
 // LDS exchange among the lanes of one wave: order the wave's LDS accesses (no block barrier, the
 // waves of a workgroup run different tiles)
@@ -988,6 +988,20 @@ hipError_t launch_pcg_persist(hipStream_t s, int ntiles, int ns, const int* tile
     const int blocks = (ntiles + PCG_WAVES - 1) / PCG_WAVES;
     return hipLaunchCooperativeKernel((const void*)k_pcg_persist<T>, dim3(blocks), dim3(64 * PCG_WAVES), args, 0, s);
 }
+// how many workgroups of k_pcg_persist<T> the device holds at once (a cooperative launch must fit)
+template <typename T>
+hipError_t pcg_persist_max_blocks(int* out) {
+    int dev = 0, cus = 0, per_cu = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e == hipSuccess)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_pcg_persist<T>, 64 * PCG_WAVES, 0);
+    *out = (e == hipSuccess) ? per_cu * cus : 0;
+    return e;
+}
+template hipError_t pcg_persist_max_blocks<double>(int*);
+template hipError_t pcg_persist_max_blocks<float>(int*);
+
 template hipError_t launch_pcg_persist<double>(hipStream_t, int, int, const int*, const int*, const int*, const int*,
                                                const PcgBufT<double>&, double, int, int, int);
 template hipError_t launch_pcg_persist<float>(hipStream_t, int, int, const int*, const int*, const int*, const int*,

@@ -204,6 +204,10 @@ inline void __builtin_amdgcn_s_waitcnt(int) {}
 inline void __builtin_amdgcn_wave_barrier() {}
 inline void __builtin_amdgcn_fence(int, const char*) {}
 inline hipError_t hipLaunchCooperativeKernel(const void*, dim3, dim3, void**, unsigned, hipStream_t) { return hipErrorNotSupported; }
+#define hipDeviceAttributeMultiprocessorCount 0
+inline hipError_t hipGetDevice(int* d) { *d = 0; return hipSuccess; }
+inline hipError_t hipDeviceGetAttribute(int* v, int, int) { *v = 1; return hipSuccess; }
+inline hipError_t hipOccupancyMaxActiveBlocksPerMultiprocessor(int* n, const void*, int, size_t) { *n = 0; return hipErrorNotSupported; }
 inline hipError_t hipMalloc(void** p, size_t n) { *p = std::malloc(n); if (*p) std::memset(*p, 0xA5, n); return *p ? hipSuccess : 2; }
 template <class T> inline hipError_t hipMalloc(T** p, size_t n) { return hipMalloc((void**)p, n); }
 inline hipError_t hipFree(void* p) { std::free(p); return hipSuccess; }

@@ -280,3 +280,34 @@ def test_learned_warm_start_trains_and_keeps_the_solution():
     say(f"A/B over 40 steps: plain {runs['plain'][:2]} (cg, newton), warm start {runs['warm'][:2]}")
     assert rel(runs["warm"][2], runs["plain"][2]) < 1e-6
     assert abs(runs["warm"][1] - runs["plain"][1]) <= 0.05 * runs["plain"][1] + 1
+
+
+def test_mixed_precision_tracks_fp64():
+    """FP32 PCG (AbdParams.precision = 1) against the all-FP64 path on a box pile dropped on the
+    ground.  One FP32 solve is inexact (the contact blocks have condition numbers far beyond FP32's
+    reach: error up to ~1e-4 of the step, against ~1e-13 for FP64 here), but Newton's FP64 gradient
+    corrects it: the positions, the energy and the gradient (at the start of step 21) after 20
+    steps, every Newton solve converged, agree within 1e-6."""
+    sc = lowered(S.box_pile(n=64, density=1e1))
+    p = north_star(dt=8e-3, ground_enabled=True, k_contact=1e20)
+    gx, g64, g32 = solver(sc, p.replace(cg_tol=1e-13, cg_max_iter=5000)), solver(sc, p), solver(sc, p.replace(precision=1))
+    q, qd = perturbed(sc, 3, amp=1e-4)
+    for g in (gx, g64, g32):
+        g.set_state(q, sc.q0, qd)
+        g.compute_du()
+    e64, e32 = rel(g64._du, gx._du), rel(g32._du, gx._du)
+    say(f"one solve at the default PCG tolerance: du error FP64 {e64:.2e}, mixed {e32:.2e} (cg {g64.last_cg_iters}, "
+        f"{g32.last_cg_iters})")
+    assert e32 < 1e-3
+    g64, g32 = solver(sc, p), solver(sc, p.replace(precision=1))
+    for k in range(20):
+        s64, s32 = g64.step()[0], g32.step()[0]
+        assert s64["converged"] and s32["converged"]
+    ex = rel(g32.get_state()[0], g64.get_state()[0])
+    for g in (g64, g32):
+        g.set_state(*g.get_state())  # energy and gradient at the start of the next step (q~ of step 21)
+    E64, E32 = g64.compute_g(), g32.compute_g()
+    gr64, gr32 = g64.assemble()[0], g32.assemble()[0]
+    eE, eg = abs(E32 - E64) / abs(E64), rel(gr32, gr64)
+    say(f"after 20 steps: x rel {ex:.2e}, E rel {eE:.2e}, grad rel {eg:.2e}")
+    assert ex < 1e-6 and eE < 1e-6 and eg < 1e-6
