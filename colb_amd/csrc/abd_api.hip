@@ -106,7 +106,7 @@ struct AbdCtx {
     int n_inc = 0, n_pk = 0, n_pairs = 0;
     // BSR
     int pair_cap = 0;
-    double *off = nullptr, *offT = nullptr;
+    double *off = nullptr, *offT = nullptr, *mq = nullptr;  // mq: 144 per CSR entry (persistent PCG)
     uint64_t *ent_k = nullptr, *ent_ks = nullptr;
     int *ent_v = nullptr, *ent_vs = nullptr, *row_ptr = nullptr;
     // PCG
@@ -118,7 +118,7 @@ struct AbdCtx {
     int coop_blocks[2] = {-1, -1};  // co-resident k_pcg_persist workgroups (FP64, FP32); -1 = not queried
     bool pcg_multi = false;       // solve with one launch per iteration (autotuner choice)
     // mixed precision (AbdParams.precision == 1): FP32 copies of the PCG operands and vectors
-    float *f_vec = nullptr, *f_diag = nullptr, *f_pinv = nullptr, *f_off = nullptr, *f_offT = nullptr;
+    float *f_vec = nullptr, *f_diag = nullptr, *f_pinv = nullptr, *f_off = nullptr, *f_offT = nullptr, *f_mq = nullptr;
     int f_pair_cap = 0;
     // learned warm start (abd_set_warm_start) and training-sample log (abd_log_samples)
     AbdMlp* mlp = nullptr;
@@ -377,8 +377,9 @@ int grow_pairs(AbdCtx* ctx, int cap) {
     if (cap <= ctx->pair_cap) return ABD_OK;
     cap = std::max(cap, ctx->pair_cap * 2);
     CHK(dfree(ctx, ctx->off)); CHK(dfree(ctx, ctx->offT)); CHK(dfree(ctx, ctx->ent_k)); CHK(dfree(ctx, ctx->ent_ks));
-    CHK(dfree(ctx, ctx->ent_v)); CHK(dfree(ctx, ctx->ent_vs));
+    CHK(dfree(ctx, ctx->ent_v)); CHK(dfree(ctx, ctx->ent_vs)); CHK(dfree(ctx, ctx->mq));
     CHK(dalloc(ctx, &ctx->off, (size_t)cap * 144)); CHK(dalloc(ctx, &ctx->offT, (size_t)cap * 144));
+    CHK(dalloc(ctx, &ctx->mq, 2 * (size_t)cap * 144));
     CHK(dalloc(ctx, &ctx->ent_k, 2 * (size_t)cap)); CHK(dalloc(ctx, &ctx->ent_ks, 2 * (size_t)cap));
     CHK(dalloc(ctx, &ctx->ent_v, 2 * (size_t)cap)); CHK(dalloc(ctx, &ctx->ent_vs, 2 * (size_t)cap));
     ctx->pair_cap = cap;
@@ -634,6 +635,11 @@ int solve_persistent(AbdCtx* ctx, const PcgBufT<T>& B, bool* ok) {
     if (!ctx->coop_ok || ctx->pcg_multi || (ctx->P.reserved0 & 8) || !persist_fits<T>(ctx)) return ABD_OK;
     HIPCHK(hipMemsetAsync(ctx->cg_iters, 0, sizeof(int) * ctx->ns, ctx->st));
     HIPCHK(hipMemsetAsync(ctx->pcg_ctl, 0, 8 * sizeof(unsigned), ctx->st));
+    if (sizeof(T) == 8) {  // the FP64 solve multiplies by M_e = B_e P_col (the FP32 one by P, then B)
+        launch_pcg_mblocks(ctx->st, 2 * ctx->n_pairs, ctx->ent_ks, ctx->ent_vs, ctx->off, ctx->offT, ctx->pinv,
+                           ctx->nb, ctx->n_pairs, B.mq, ctx->cnt + C_ERR);
+        CHK(dbg(ctx, "launch_pcg_mblocks"));
+    }
     tic(ctx, "pcg_solve");
     hipError_t e = launch_pcg_persist(ctx->st, ctx->ntiles, ctx->ns, ctx->tile_b0, ctx->tile_b1, ctx->tile_scene,
                                       ctx->scene_tile0, B, ctx->P.cg_tol, ctx->P.cg_max_iter, ctx->nb, ctx->n_pairs);
@@ -676,8 +682,10 @@ int ensure_f32(AbdCtx* ctx) {
     if (ctx->f_pair_cap < ctx->pair_cap || !ctx->f_off) {
         CHK(dfree(ctx, ctx->f_off));
         CHK(dfree(ctx, ctx->f_offT));
+        CHK(dfree(ctx, ctx->f_mq));
         CHK(dalloc(ctx, &ctx->f_off, 144 * (size_t)std::max(ctx->pair_cap, 1)));
         CHK(dalloc(ctx, &ctx->f_offT, 144 * (size_t)std::max(ctx->pair_cap, 1)));
+        CHK(dalloc(ctx, &ctx->f_mq, 2 * 144 * (size_t)std::max(ctx->pair_cap, 1)));
         ctx->f_pair_cap = ctx->pair_cap;
     }
     return ABD_OK;
@@ -757,7 +765,7 @@ int solve(AbdCtx* ctx) {
         B.b = v; B.x = v + n12; B.p = v + 2 * n12; B.u = v + 3 * n12;
         B.r[0] = v + 4 * n12; B.r[1] = v + 5 * n12; B.w[0] = v + 6 * n12; B.w[1] = v + 7 * n12;
         B.sv[0] = v + 8 * n12; B.sv[1] = v + 9 * n12;
-        B.diag = ctx->f_diag; B.pinv = ctx->f_pinv; B.off = ctx->f_off; B.offT = ctx->f_offT;
+        B.diag = ctx->f_diag; B.pinv = ctx->f_pinv; B.off = ctx->f_off; B.offT = ctx->f_offT; B.mq = ctx->f_mq;
         tic(ctx, "pcg_downcast", (double)(n12 + 288 * (size_t)ctx->nb + 288 * (size_t)ctx->n_pairs) * 12);
         launch_to_f32(ctx->st, (long)n12, b, v);
         launch_to_f32(ctx->st, 144 * (long)ctx->nb, ctx->diag, ctx->f_diag);
@@ -773,7 +781,7 @@ int solve(AbdCtx* ctx) {
         PcgBuf B;
         pcg_ctl(ctx, B);
         B.bref = bref;
-        B.b = b; B.diag = ctx->diag; B.pinv = ctx->pinv; B.off = ctx->off; B.offT = ctx->offT;
+        B.b = b; B.diag = ctx->diag; B.pinv = ctx->pinv; B.off = ctx->off; B.offT = ctx->offT; B.mq = ctx->mq;
         B.x = ctx->du; B.p = ctx->p0; B.u = ctx->z;
         B.r[0] = ctx->r; B.r[1] = ctx->r1; B.w[0] = ctx->Ap; B.w[1] = ctx->w1; B.sv[0] = ctx->s0; B.sv[1] = ctx->p1;
         CHK(run_pcg(ctx, B, warm));

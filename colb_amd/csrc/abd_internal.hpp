@@ -102,6 +102,7 @@ struct PcgBufT {
     T* r[2];
     T* w[2];
     T* sv[2];
+    T* mq;                     // persistent solve: M_e = B_e P_col(e) per CSR entry e (144 T each)
     double* part;
     double* sc;                // [4 * ns]: g by parity, then alpha by parity
     const double* bnorm;
@@ -200,6 +201,9 @@ hipError_t launch_pcg_persist(hipStream_t s, int ntiles, int ns, const int* tile
                               int max_iter, int nb, int n_pairs);
 template <typename T>
 hipError_t pcg_persist_max_blocks(int* out);
+template <typename T>
+void launch_pcg_mblocks(hipStream_t s, int n_ent, const uint64_t* ent_key, const int* ent_val, const double* off,
+                        const double* offT, const double* pinv, int nb, int n_pairs, T* mq, int* err);
 constexpr int PCG_TILES_PER_BLOCK = 4;  // waves (tiles) per workgroup of k_pcg_persist
 // mixed precision: FP64 -> FP32 copies of the PCG operands, FP32 -> FP64 of the solution
 void launch_to_f32(hipStream_t s, long n, const double* a, float* b);

@@ -815,6 +815,56 @@ __device__ bool grid_barrier(unsigned* bar, int* fail, unsigned k, unsigned nblo
 }
 
 #define PCG_WAVES PCG_TILES_PER_BLOCK
+#ifndef PCG_NBR_CACHE
+#define PCG_NBR_CACHE 16  // neighbours per chunk of the persistent PCG
+#endif
+
+// column of CSR entry e, or -1 (and the error bit) if the entry is out of range
+template <typename T>
+__device__ __forceinline__ int pcg_entry(const PcgBufT<T>& B, int e, int nb, int n_pairs) {
+    const int col = opaque((int)(B.ent_key[e] & 0xFFFFFFFFu));
+    const int v = B.ent_val[e];
+    if (col < 0 || col >= nb || v < 0 || (v >> 1) >= n_pairs) {
+        atomicOr(B.err, ERR_CSR);
+        return -1;
+    }
+    return col;
+}
+
+// M_e = B_e P_col(e) for every CSR entry e (B_e = off or off^T of the pair), FP64 products rounded
+// to T; one thread per element.  Invalid entries get M = 0 and the error bit.
+template <typename T>
+__global__ void k_pcg_mblocks(int n_ent, const uint64_t* __restrict__ ent_key, const int* __restrict__ ent_val,
+                              const double* __restrict__ off, const double* __restrict__ offT,
+                              const double* __restrict__ pinv, int nb, int n_pairs, T* __restrict__ mq, int* err) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long)n_ent * 144) return;
+    const int e = (int)(i / 144), rc = (int)(i % 144), r = rc / 12, c = rc % 12;
+    const int col = opaque((int)(ent_key[e] & 0xFFFFFFFFu));
+    const int v = ent_val[e];
+    double m = 0.0;
+    if (col < 0 || col >= nb || v < 0 || (v >> 1) >= n_pairs) {
+        if (rc == 0) atomicOr(err, ERR_CSR);
+    } else {
+        const double* Bk = (opaque(v & 1) ? offT : off) + (size_t)opaque(v >> 1) * 144 + r * 12;
+        const double* Pc = pinv + (size_t)col * 144 + c;
+#pragma unroll
+        for (int k = 0; k < 12; ++k) m += Bk[k] * Pc[12 * k];
+    }
+    mq[i] = (T)m;
+}
+
+template <typename T>
+void launch_pcg_mblocks(hipStream_t s, int n_ent, const uint64_t* ent_key, const int* ent_val, const double* off,
+                        const double* offT, const double* pinv, int nb, int n_pairs, T* mq, int* err) {
+    if (n_ent <= 0) return;
+    hipLaunchKernelGGL(k_pcg_mblocks<T>, dim3(cdiv((long)n_ent * 144, 256)), dim3(256), 0, s, n_ent, ent_key, ent_val,
+                       off, offT, pinv, nb, n_pairs, mq, err);
+}
+template void launch_pcg_mblocks<double>(hipStream_t, int, const uint64_t*, const int*, const double*, const double*,
+                                         const double*, int, int, double*, int*);
+template void launch_pcg_mblocks<float>(hipStream_t, int, const uint64_t*, const int*, const double*, const double*,
+                                        const double*, int, int, float*, int*);
 
 // LDS exchange among the lanes of one wave: order the wave's LDS accesses (no block barrier, the
 // waves of a workgroup run different tiles)
@@ -831,10 +881,16 @@ __global__ __launch_bounds__(64 * PCG_WAVES) void k_pcg_persist(int ntiles, int 
                                                                  const int* __restrict__ st0, PcgBufT<T> B,
                                                                  double tol, int max_iter, int nb, int n_pairs) {
     __shared__ T vec_all[PCG_WAVES][TILE_BODIES + 1][12];
+    // CSR columns of each body of the tile in chunks of PCG_NBR_CACHE (the first chunk stays), and
+    // the neighbours' new residual rows of the current chunk
+    __shared__ int nbr_all[PCG_WAVES][TILE_BODIES + 1][PCG_NBR_CACHE];
+    __shared__ T rb_all[PCG_WAVES][TILE_BODIES + 1][PCG_NBR_CACHE][12];
     const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
     const int t = blockIdx.x * PCG_WAVES + wave;
     const bool tile = t < ntiles;  // waves past the last tile only take part in the barriers
     T(*vec)[12] = vec_all[wave];
+    int(*nbr)[PCG_NBR_CACHE] = nbr_all[wave];
+    T(*rb)[PCG_NBR_CACHE][12] = rb_all[wave];
     const unsigned nblocks = gridDim.x;
     const int s = tile ? tscene[t] : 0;
     const int t0 = tile ? st0[s] : 0, t1 = tile ? st0[s + 1] : 0;
@@ -855,6 +911,15 @@ __global__ __launch_bounds__(64 * PCG_WAVES) void k_pcg_persist(int ntiles, int 
     int mx = e1 - e0;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) mx = max(mx, __shfl_xor(mx, o, 64));
+    // Off-diagonal part of w = H u = H P r: sum over entries e of M_e r_col, with M_e = B_e P_col
+    // (k_pcg_mblocks, FP64 products stored in T) fixed during the solve.
+    if (on && row == 0) {
+        for (int q = 0; q < min(e1 - e0, PCG_NBR_CACHE); ++q) {
+            const int col = opaque((int)(B.ent_key[e0 + q] & 0xFFFFFFFFu));
+            nbr[lb][q] = (col >= 0 && col < nb) ? col : -1;
+        }
+    }
+    wave_lds_sync();
 
     T x = 0, p = 0, u = 0, r = 0, w = 0, sv = 0;
     double alpha = 0.0, beta = 0.0, g_prev = 1.0, a_prev = 1.0, bnorm = 0.0;
@@ -922,34 +987,71 @@ __global__ __launch_bounds__(64 * PCG_WAVES) void k_pcg_persist(int ntiles, int 
                 for (int c = 0; c < 12; ++c) wv += Drow[c] * vec[lb][c];
             }
             wave_lds_sync();
-            for (int q = 0; q < mx; ++q) {
-                const bool has = on && q < e1 - e0;
-                const T* Bk = nullptr;
-                T uc = 0;
-                if (has) {
-                    const int e = e0 + q;
-                    const int col = opaque((int)(B.ent_key[e] & 0xFFFFFFFFu));
-                    const int v = B.ent_val[e];
-                    if (col < 0 || col >= nb || v < 0 || (v >> 1) >= n_pairs) {
-                        atomicOr(B.err, ERR_CSR);
-                    } else {
-                        Bk = (opaque(v & 1) ? B.offT : B.off) + (size_t)opaque(v >> 1) * 144 + row * 12;
-                        const T* Pc = B.pinv + (size_t)col * 144 + row * 12;
-#pragma unroll
-                        for (int c = 0; c < 12; ++c) {
-                            const int j = 12 * col + c;
-                            const T rj = (k == 0) ? B.b[j]
-                                                  : pcg_r_next(aload(B.r[rd] + j), aload(B.w[rd] + j),
-                                                               aload(B.sv[rd] + j), al, be);
-                            uc += Pc[c] * rj;
+            // neighbours in chunks of PCG_NBR_CACHE: all rows of a chunk are fetched in one round of
+            // loads (lane `row` fetches row `row` of each neighbour's new residual), then every lane
+            // multiplies its rows of M with them from LDS
+            for (int q0 = 0; q0 < mx; q0 += PCG_NBR_CACHE) {
+                if (mx > PCG_NBR_CACHE) {  // more than one chunk: (re)load this chunk's columns
+                    wave_lds_sync();
+                    if (on && row == 0)
+                        for (int q = 0; q < PCG_NBR_CACHE && q0 + q < e1 - e0; ++q) {
+                            const int col = opaque((int)(B.ent_key[e0 + q0 + q] & 0xFFFFFFFFu));
+                            nbr[lb][q] = (col >= 0 && col < nb) ? col : -1;
                         }
+                    wave_lds_sync();
+                }
+                const int nq = on ? min(e1 - e0 - q0, PCG_NBR_CACHE) : 0;
+                T vr[PCG_NBR_CACHE], vw[PCG_NBR_CACHE], vs[PCG_NBR_CACHE];
+#pragma unroll
+                for (int q = 0; q < PCG_NBR_CACHE; ++q) {
+                    int col = (q < nq) ? nbr[lb][q] : -1;
+                    col = col < 0 ? bb : col;  // in-range address for unused slots
+                    const int j = 12 * col + row;
+                    if (k == 0) {
+                        vr[q] = B.b[j];
+                    } else {
+                        vr[q] = aload(B.r[rd] + j);
+                        vw[q] = aload(B.w[rd] + j);
+                        vs[q] = aload(B.sv[rd] + j);
                     }
                 }
-                vec[lb][row] = uc;
-                wave_lds_sync();
-                if (Bk) {
 #pragma unroll
-                    for (int c = 0; c < 12; ++c) wv += Bk[c] * vec[lb][c];
+                for (int q = 0; q < PCG_NBR_CACHE; ++q) {
+                    const bool use = q < nq && nbr[lb][q] >= 0;
+                    const T rj = (k == 0) ? vr[q] : pcg_r_next(vr[q], vw[q], vs[q], al, be);
+                    if (on) rb[lb][q][row] = use ? rj : T(0);
+                }
+                wave_lds_sync();
+                if constexpr (sizeof(T) == 8) {
+                    for (int q = 0; q < nq; ++q) {
+                        const T* M = B.mq + (size_t)(e0 + q0 + q) * 144 + row * 12;
+#pragma unroll
+                        for (int c = 0; c < 12; ++c) wv += M[c] * rb[lb][q][c];
+                    }
+                } else {
+                    // FP32: u_col = P_col r_col first, then B_e u_col (rounding M = B_e P_col to FP32
+                    // loses the stiff contact directions)
+                    const int nqw = min(mx - q0, PCG_NBR_CACHE);
+                    for (int q = 0; q < nqw; ++q) {
+                        const bool use = q < nq && nbr[lb][q] >= 0;
+                        T uc = 0, bk[12];
+                        if (use) {
+                            const int col = nbr[lb][q], v = B.ent_val[e0 + q0 + q];
+                            const T* Bk = (opaque(v & 1) ? B.offT : B.off) + (size_t)opaque(v >> 1) * 144 + row * 12;
+                            const T* Pc = B.pinv + (size_t)col * 144 + row * 12;
+#pragma unroll
+                            for (int c = 0; c < 12; ++c) bk[c] = Bk[c];
+#pragma unroll
+                            for (int c = 0; c < 12; ++c) uc += Pc[c] * rb[lb][q][c];
+                        }
+                        vec[lb][row] = uc;
+                        wave_lds_sync();
+                        if (use) {
+#pragma unroll
+                            for (int c = 0; c < 12; ++c) wv += bk[c] * vec[lb][c];
+                        }
+                        wave_lds_sync();
+                    }
                 }
                 wave_lds_sync();
             }
@@ -986,7 +1088,16 @@ hipError_t launch_pcg_persist(hipStream_t s, int ntiles, int ns, const int* tile
     void* args[] = {(void*)&ntiles, (void*)&ns, (void*)&tile_b0, (void*)&tile_b1, (void*)&tile_scene,
                     (void*)&scene_tile0, (void*)&B, (void*)&tol, (void*)&max_iter, (void*)&nb, (void*)&n_pairs};
     const int blocks = (ntiles + PCG_WAVES - 1) / PCG_WAVES;
+#if defined(ABD_EMULATION)
+    // the CPU emulation runs one workgroup at a time: a single-workgroup grid is co-resident
+    (void)args;
+    if (blocks != 1) return hipErrorNotSupported;
+    hipLaunchKernelGGL(k_pcg_persist<T>, dim3(1), dim3(64 * PCG_WAVES), 0, s, ntiles, ns, tile_b0, tile_b1, tile_scene,
+                       scene_tile0, B, tol, max_iter, nb, n_pairs);
+    return hipSuccess;
+#else
     return hipLaunchCooperativeKernel((const void*)k_pcg_persist<T>, dim3(blocks), dim3(64 * PCG_WAVES), args, 0, s);
+#endif
 }
 // how many workgroups of k_pcg_persist<T> the device holds at once (a cooperative launch must fit)
 template <typename T>

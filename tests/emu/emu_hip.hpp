@@ -1,10 +1,11 @@
 // emu_hip.hpp — CPU emulation of the HIP runtime subset used by colb_amd/csrc, for bounds-checked
 // (AddressSanitizer) host builds of the kernels.  TEST INFRASTRUCTURE ONLY.
 //
-// Kernels run block by block; each GPU thread of a block is an OS thread.  __shared__ variables
-// are function statics (blocks run one at a time), __syncthreads is a std::barrier over the
-// block, __shfl_xor goes through a block-wide exchange buffer, and threads that return early
-// drop out of the barrier (as exited waves do on the GPU).
+// Kernels run block by block; each GPU thread of a block is a fibre (ucontext).  __shared__
+// variables are function statics (blocks run one at a time).  __syncthreads waits for every live
+// fibre of the block, wave operations (__shfl_xor, votes, the matrix instruction, wave barriers)
+// for every live fibre of the caller's 64-lane wave; fibres that return early drop out of both
+// (as exited waves do on the GPU).  A barrier that can never complete aborts with a message.
 #pragma once
 #include <atomic>
 #include <ucontext.h>
@@ -47,6 +48,7 @@ struct EmuFiber {
     ucontext_t ctx;
     std::vector<char> stack;
     bool done = false;
+    int wait = 0;  // 0 runnable, 1 at a block barrier, 2 at a wave barrier
 };
 struct EmuSched {
     ucontext_t main;
@@ -86,7 +88,9 @@ inline void emu_fiber_entry() {
     EMU_START_SWITCH(nullptr, S->main_bottom, S->main_size);
 }
 
-inline void __syncthreads() { emu_yield(); }
+inline void emu_block_sync() { emu_sched->fib[emu_sched->cur].wait = 1; emu_yield(); }
+inline void emu_wave_sync() { emu_sched->fib[emu_sched->cur].wait = 2; emu_yield(); }
+inline void __syncthreads() { emu_block_sync(); }
 inline void __threadfence() {}
 
 template <class T>
@@ -97,9 +101,9 @@ inline T __shfl_xor(T v, int mask, int width = 64) {
     std::memcpy(&w, &v, sizeof(T));
     unsigned lane = threadIdx.x;
     emu_sched->xbuf[lane] = w;
-    emu_yield();
+    emu_wave_sync();
     double o = emu_sched->xbuf[lane ^ (unsigned)mask];
-    emu_yield();
+    emu_wave_sync();
     T r;
     std::memcpy(&r, &o, sizeof(T));
     return r;
@@ -109,12 +113,12 @@ inline T __shfl_xor(T v, int mask, int width = 64) {
 inline int emu_any(int pred) {
     unsigned lane = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
     emu_sched->xbuf[lane] = pred ? 1.0 : 0.0;
-    emu_yield();
+    emu_wave_sync();
     unsigned w0 = lane & ~63u, nt = blockDim.x * blockDim.y * blockDim.z;
     int r = 0;
     for (unsigned k = w0; k < w0 + 64 && k < nt; ++k)
         if (emu_sched->xbuf[k] != 0.0) r = 1;
-    emu_yield();
+    emu_wave_sync();
     return r;
 }
 
@@ -129,7 +133,7 @@ inline F4 __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(S4 a, S4 b, F4 c, int, int, 
     auto bf = [](short h) { return __uint_as_float((unsigned)(unsigned short)h << 16); };
     float* mb = emu_sched->mbuf.data();
     for (int u = 0; u < 4; ++u) { mb[8 * threadIdx.x + u] = bf(a[u]); mb[8 * threadIdx.x + 4 + u] = bf(b[u]); }
-    emu_yield();
+    emu_wave_sync();
     F4 d = c;
     const int n = lane % 16;
     for (int u = 0; u < 4; ++u) {
@@ -142,7 +146,7 @@ inline F4 __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(S4 a, S4 b, F4 c, int, int, 
         }
         d[u] = c[u] + acc;
     }
-    emu_yield();
+    emu_wave_sync();
     return d;
 }
 
@@ -199,15 +203,16 @@ inline hipError_t hipGetLastError() { hipError_t e = emu_last_error; emu_last_er
 template <class T> inline T __hip_atomic_load(T* p, int, int) { T r; __atomic_load(p, &r, __ATOMIC_SEQ_CST); return r; }
 template <class T> inline void __hip_atomic_store(T* p, T v, int, int) { __atomic_store(p, &v, __ATOMIC_SEQ_CST); }
 template <class T, class U> inline T __hip_atomic_fetch_add(T* p, U v, int, int) { return __atomic_fetch_add(p, (T)v, __ATOMIC_SEQ_CST); }
-inline void __builtin_amdgcn_s_sleep(int) {}
+inline void __builtin_amdgcn_s_sleep(int) { emu_yield(); }
 inline void __builtin_amdgcn_s_waitcnt(int) {}
-inline void __builtin_amdgcn_wave_barrier() {}
+inline void __builtin_amdgcn_wave_barrier() { emu_wave_sync(); }
 inline void __builtin_amdgcn_fence(int, const char*) {}
 inline hipError_t hipLaunchCooperativeKernel(const void*, dim3, dim3, void**, unsigned, hipStream_t) { return hipErrorNotSupported; }
 #define hipDeviceAttributeMultiprocessorCount 0
 inline hipError_t hipGetDevice(int* d) { *d = 0; return hipSuccess; }
 inline hipError_t hipDeviceGetAttribute(int* v, int, int) { *v = 1; return hipSuccess; }
-inline hipError_t hipOccupancyMaxActiveBlocksPerMultiprocessor(int* n, const void*, int, size_t) { *n = 0; return hipErrorNotSupported; }
+// one co-resident workgroup: cooperative launches of a single block run (emu_launch_cooperative)
+inline hipError_t hipOccupancyMaxActiveBlocksPerMultiprocessor(int* n, const void*, int, size_t) { *n = 1; return hipSuccess; }
 inline hipError_t hipMalloc(void** p, size_t n) { *p = std::malloc(n); if (*p) std::memset(*p, 0xA5, n); return *p ? hipSuccess : 2; }
 template <class T> inline hipError_t hipMalloc(T** p, size_t n) { return hipMalloc((void**)p, n); }
 inline hipError_t hipFree(void* p) { std::free(p); return hipSuccess; }
@@ -258,30 +263,53 @@ void emu_launch(const char* name, K kernel, dim3 grid, dim3 block, Args... args)
             f.ctx.uc_link = &S.main;
             makecontext(&f.ctx, emu_fiber_entry, 0);
         }
-        unsigned live = nt;
+        for (unsigned t = 0; t < nt; ++t) S.fib[t].wait = 0;
         static const bool shuffle = std::getenv("EMU_SHUFFLE") != nullptr;
         std::vector<unsigned> order(nt);
         for (unsigned t = 0; t < nt; ++t) order[t] = t;
         static unsigned long long rng = 88172645463325252ull;
-        while (live > 0) {
-            live = 0;
+        while (true) {
             if (shuffle)
                 for (unsigned t = nt - 1; t > 0; --t) {
                     rng ^= rng << 13; rng ^= rng >> 7; rng ^= rng << 17;
                     unsigned j = (unsigned)(rng % (t + 1));
                     unsigned tmp = order[t]; order[t] = order[j]; order[j] = tmp;
                 }
+            bool ran = false;
             for (unsigned oi = 0; oi < nt; ++oi) {
                 unsigned t = order[oi];
                 EmuFiber& f = S.fib[t];
-                if (f.done) continue;
+                if (f.done || f.wait) continue;
+                ran = true;
                 S.cur = t;
                 threadIdx = dim3(t % block.x, (t / block.x) % block.y, t / (block.x * block.y));
                 void* fake = nullptr;
                 EMU_START_SWITCH(&fake, f.stack.data(), f.stack.size());
                 swapcontext(&S.main, &f.ctx);
                 EMU_FINISH_SWITCH(fake, nullptr, nullptr);
-                if (!f.done) ++live;
+            }
+            // release the barriers every live fibre of their scope has reached
+            unsigned live = 0, at_block = 0;
+            for (unsigned t = 0; t < nt; ++t)
+                if (!S.fib[t].done) { ++live; at_block += S.fib[t].wait == 1; }
+            if (live == 0) break;
+            bool released = false;
+            if (at_block == live) {
+                for (unsigned t = 0; t < nt; ++t) S.fib[t].wait = 0;
+                released = true;
+            }
+            for (unsigned w0 = 0; w0 < nt && !released; w0 += 64) {
+                bool all = true, any = false;
+                for (unsigned t = w0; t < w0 + 64 && t < nt; ++t)
+                    if (!S.fib[t].done) { any = true; all = all && S.fib[t].wait == 2; }
+                if (any && all) {
+                    for (unsigned t = w0; t < w0 + 64 && t < nt; ++t) S.fib[t].wait = 0;
+                    released = true;
+                }
+            }
+            if (!ran && !released) {
+                std::fprintf(stderr, "emu: %s block %u: barrier can never complete\n", name, b);
+                std::abort();
             }
         }
     }
