@@ -155,7 +155,9 @@ def main():
 
     # per-kernel HIP-event timing over a few extra steps (kept out of the timed region)
     solver.enable_timing(True)
-    solver.step(args.profile_steps)
+    prof_cg = 0
+    for k in range(args.profile_steps):
+        prof_cg += sum(st_["cg_iters"] for st_ in solver.step())
     kt = solver.kernel_times()
     solver.enable_timing(False)
     single = ["pcg_solve", "pcg_iter", "broad_phase_query", "narrow_phase", "contact_hess", "offblock_sum", "body_assemble",
@@ -171,6 +173,8 @@ def main():
         roof = {"bound": "hbm", "kernel": dom, "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(ach / HBM_PEAK_GBS, 5), "traffic": traffic, "avg_us": round(avg_ms * 1e3, 3),
                 "bytes_per_launch": bytes_per, "traffic_source": src}
+    pcg_ms = sum(kt[k]["ms"] for k in ("pcg_solve", "pcg_iter") if k in kt)
+    pcg_us_per_iter = round(pcg_ms * 1e3 / prof_cg, 3) if prof_cg else None
     for k, v in kt.items():
         print(f"# kernel {k:20s} {v['ms']:9.3f} ms {v['calls']:6d} calls {v['bytes'] / max(v['ms'], 1e-9) / 1e6:8.1f} GB/s",
               file=sys.stderr, flush=True)
@@ -188,7 +192,7 @@ def main():
         "dtype": "f64" if args.precision == "fp64" else "f32+f64",
         "data": "synthetic (procedural scene)", "config": cfg,
         "newton_iters_per_step": round(float(np.mean(iters)), 3), "cg_iters_per_step": round(float(np.mean(cg)), 1),
-        "method": "plain", "autotune": tune,
+        "method": "plain", "autotune": tune, "pcg_us_per_iteration": pcg_us_per_iter,
         "roofline": roof, "cpu_baseline": cpu,
     }
     print(json.dumps(result), flush=True)

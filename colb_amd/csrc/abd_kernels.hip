@@ -816,7 +816,7 @@ __device__ bool grid_barrier(unsigned* bar, int* fail, unsigned k, unsigned nblo
 
 #define PCG_WAVES PCG_TILES_PER_BLOCK
 #ifndef PCG_NBR_CACHE
-#define PCG_NBR_CACHE 16  // neighbours per chunk of the persistent PCG
+#define PCG_NBR_CACHE 8  // neighbours per chunk of the persistent PCG
 #endif
 
 // column of CSR entry e, or -1 (and the error bit) if the entry is out of range
@@ -925,6 +925,36 @@ __global__ __launch_bounds__(64 * PCG_WAVES) void k_pcg_persist(int ntiles, int 
     double alpha = 0.0, beta = 0.0, g_prev = 1.0, a_prev = 1.0, bnorm = 0.0;
     bool done = !tile;
     unsigned nbar = 0;
+    // columns of the chunk starting at q0 into nbr (row-0 lanes; -1 for an out-of-range column)
+    auto load_cols = [&](int q0) {
+        if (on && row == 0)
+            for (int q = 0; q < PCG_NBR_CACHE && q0 + q < e1 - e0; ++q) {
+                const int col = opaque((int)(B.ent_key[e0 + q0 + q] & 0xFFFFFFFFu));
+                nbr[lb][q] = (col >= 0 && col < nb) ? col : -1;
+            }
+    };
+    // neighbour rows of the chunk starting at q0: lane `row` loads row `row` of b (stage 0) or of
+    // r, w, s of the previous stage, for the slots any body of the wave uses (wave-uniform count),
+    // all loads in flight together
+    T vr[PCG_NBR_CACHE], vw[PCG_NBR_CACHE], vs[PCG_NBR_CACHE];
+    auto fetch = [&](int q0, int k, int rd) {
+        const int nq = on ? min(e1 - e0 - q0, PCG_NBR_CACHE) : 0;
+        const int nqw = min(mx - q0, PCG_NBR_CACHE);
+#pragma unroll
+        for (int q = 0; q < PCG_NBR_CACHE; ++q) {
+            if (q >= nqw) break;
+            int col = (q < nq) ? nbr[lb][q] : -1;
+            col = col < 0 ? bb : col;  // in-range address for unused slots
+            const int j = 12 * col + row;
+            if (k == 0) {
+                vr[q] = B.b[j];
+            } else {
+                vr[q] = aload(B.r[rd] + j);
+                vw[q] = aload(B.w[rd] + j);
+                vs[q] = aload(B.sv[rd] + j);
+            }
+        }
+    };
     for (int k = 0; k <= max_iter; ++k) {
         const int wr = k & 1, rd = (k + 1) & 1;
         const int i = k - 1;  // iteration index of this stage (-1: initial stage)
@@ -933,6 +963,9 @@ __global__ __launch_bounds__(64 * PCG_WAVES) void k_pcg_persist(int ntiles, int 
             if (blockIdx.x == 0 && threadIdx.x == 0)
                 __hip_atomic_store(B.done_cnt + ((i + 1) % 3), 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
+        // the first chunk's neighbour rows do not depend on this stage's scalars: their loads go out
+        // together with the partial-sum loads below (one round trip for both)
+        if (!done && mx > 0) fetch(0, k, rd);
         if (k > 0 && tile) {
             const int par = (k - 1) & 1;
             // the three partial sums of the scene in one pass over its tiles
@@ -991,32 +1024,17 @@ __global__ __launch_bounds__(64 * PCG_WAVES) void k_pcg_persist(int ntiles, int 
             // loads (lane `row` fetches row `row` of each neighbour's new residual), then every lane
             // multiplies its rows of M with them from LDS
             for (int q0 = 0; q0 < mx; q0 += PCG_NBR_CACHE) {
-                if (mx > PCG_NBR_CACHE) {  // more than one chunk: (re)load this chunk's columns
+                if (q0 > 0) {  // later chunks: their columns, then their rows
                     wave_lds_sync();
-                    if (on && row == 0)
-                        for (int q = 0; q < PCG_NBR_CACHE && q0 + q < e1 - e0; ++q) {
-                            const int col = opaque((int)(B.ent_key[e0 + q0 + q] & 0xFFFFFFFFu));
-                            nbr[lb][q] = (col >= 0 && col < nb) ? col : -1;
-                        }
+                    load_cols(q0);
                     wave_lds_sync();
+                    fetch(q0, k, rd);
                 }
                 const int nq = on ? min(e1 - e0 - q0, PCG_NBR_CACHE) : 0;
-                T vr[PCG_NBR_CACHE], vw[PCG_NBR_CACHE], vs[PCG_NBR_CACHE];
+                const int nqw = min(mx - q0, PCG_NBR_CACHE);  // slots used by some body of the wave
 #pragma unroll
                 for (int q = 0; q < PCG_NBR_CACHE; ++q) {
-                    int col = (q < nq) ? nbr[lb][q] : -1;
-                    col = col < 0 ? bb : col;  // in-range address for unused slots
-                    const int j = 12 * col + row;
-                    if (k == 0) {
-                        vr[q] = B.b[j];
-                    } else {
-                        vr[q] = aload(B.r[rd] + j);
-                        vw[q] = aload(B.w[rd] + j);
-                        vs[q] = aload(B.sv[rd] + j);
-                    }
-                }
-#pragma unroll
-                for (int q = 0; q < PCG_NBR_CACHE; ++q) {
+                    if (q >= nqw) break;
                     const bool use = q < nq && nbr[lb][q] >= 0;
                     const T rj = (k == 0) ? vr[q] : pcg_r_next(vr[q], vw[q], vs[q], al, be);
                     if (on) rb[lb][q][row] = use ? rj : T(0);
@@ -1031,7 +1049,6 @@ __global__ __launch_bounds__(64 * PCG_WAVES) void k_pcg_persist(int ntiles, int 
                 } else {
                     // FP32: u_col = P_col r_col first, then B_e u_col (rounding M = B_e P_col to FP32
                     // loses the stiff contact directions)
-                    const int nqw = min(mx - q0, PCG_NBR_CACHE);
                     for (int q = 0; q < nqw; ++q) {
                         const bool use = q < nq && nbr[lb][q] >= 0;
                         T uc = 0, bk[12];
@@ -1053,6 +1070,10 @@ __global__ __launch_bounds__(64 * PCG_WAVES) void k_pcg_persist(int ntiles, int 
                         wave_lds_sync();
                     }
                 }
+                wave_lds_sync();
+            }
+            if (mx > PCG_NBR_CACHE) {  // the first chunk's columns again, for the next prefetch
+                load_cols(0);
                 wave_lds_sync();
             }
             // publish this stage's r, w and s for the neighbours of the next stage
