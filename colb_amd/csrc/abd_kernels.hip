@@ -1041,8 +1041,11 @@ __global__ __launch_bounds__(64 * PCG_WAVES) void k_pcg_persist(int ntiles, int 
                 }
                 wave_lds_sync();
                 if constexpr (sizeof(T) == 8) {
-                    for (int q = 0; q < nq; ++q) {
-                        const T* M = B.mq + (size_t)(e0 + q0 + q) * 144 + row * 12;
+                    // unused slots hold rb = 0 and read M of entry 0 (present whenever mx > 0):
+                    // no branch, so the unrolled M-row loads go out together
+#pragma unroll 4
+                    for (int q = 0; q < nqw; ++q) {
+                        const T* M = B.mq + (size_t)(q < nq ? e0 + q0 + q : 0) * 144 + row * 12;
 #pragma unroll
                         for (int c = 0; c < 12; ++c) wv += M[c] * rb[lb][q][c];
                     }
