@@ -1590,8 +1590,9 @@ int abd_autotune(AbdCtx* ctx, int repeats, int* bodies_per_tile, int* per_iterat
     CHK(fill_i32(ctx, ctx->C.newton_active, ctx->ns, 1));
     CHK(static_broad_phase(ctx));
     CHK(assemble(ctx));
-    double best = -1.0;
-    int best_bpt = ctx->bodies_per_tile, best_multi = ctx->pcg_multi ? 1 : 0;
+    double best = -1.0, t_cur = -1.0;
+    const int cur_bpt = ctx->bodies_per_tile, cur_multi = ctx->pcg_multi ? 1 : 0;
+    int best_bpt = cur_bpt, best_multi = cur_multi;
     for (int bpt = 1; bpt <= 5; ++bpt) {
         CHK(set_tiles(ctx, bpt));
         for (int multi = 0; multi < 2; ++multi) {
@@ -1607,12 +1608,19 @@ int abd_autotune(AbdCtx* ctx, int repeats, int* bodies_per_tile, int* per_iterat
             HIPCHK(hipStreamSynchronize(ctx->st));
             const double t = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / repeats;
             if (ms) ms[slot] = t;
+            if (bpt == cur_bpt && multi == cur_multi) t_cur = t;
             if (best < 0.0 || t < best) {
                 best = t;
                 best_bpt = bpt;
                 best_multi = multi;
             }
         }
+    }
+    // keep the layout in effect unless another is clearly faster (timing noise must not flip it:
+    // a different layout sums the PCG partials in a different order)
+    if (t_cur > 0.0 && t_cur <= 1.03 * best) {
+        best_bpt = cur_bpt;
+        best_multi = cur_multi;
     }
     CHK(set_tiles(ctx, best_bpt));
     ctx->pcg_multi = best_multi != 0;

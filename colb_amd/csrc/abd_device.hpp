@@ -307,19 +307,39 @@ __device__ inline void wave_psd_project(double* A, double* V, int m, int lane) {
     if (fro == 0.0) return;
     // Parallel (round-robin) Jacobi: a sweep is npl - 1 rounds of npl / 2 disjoint rotations
     // (circle method over npl = m rounded up to even players; a pair with the extra player is
-    // skipped).  Lane k < npl/2 computes the k-th rotation of the round once (LDS), then lane task
-    // t = (pair k, index i) applies its column rotation of row i and its row rotation of column i.
-    __shared__ double jr_c[8], jr_s[8];
-    __shared__ int jr_p[8], jr_q[8];
-    const int npl = m + (m & 1), half = npl / 2, ntask = half * m;
+    // skipped).  Lane k < npl/2 computes the k-th rotation of the round once and records, for each
+    // index, its own weight and its partner's (J restricted to the pair).  Then every element of
+    // A' = J^T A J (upper triangle, mirrored) and V' = V J is formed from the old matrices in one
+    // pass into the other buffer: two barriers per round.
+    __shared__ double A2[144], V2[144];
+    __shared__ double jw_self[12], jw_part[12];
+    __shared__ int j_part[12];
+    double* Ac = A;
+    double* Vc = V;
+    double* An = A2;
+    double* Vn = V2;
+    const int npl = m + (m & 1), half = npl / 2;
+    const int nup = m * (m + 1) / 2;  // <= 78: each lane owns upper-triangle elements lane, lane + 64
+    int ui[2] = {-1, -1}, uj[2] = {0, 0};
+    for (int h = 0; h < 2; ++h) {
+        const int t = lane + 64 * h;
+        if (t < nup) {
+            int i = 0, rem = t;
+            while (rem >= m - i) { rem -= m - i; ++i; }
+            ui[h] = i;
+            uj[h] = i + rem;
+        }
+    }
     for (int sweep = 0; sweep < 32; ++sweep) {
         double o = 0;
         for (int e = lane; e < mm; e += 64) {
             int p = e / m, q = e % m;
-            if (q > p) o += A[e] * A[e];
+            if (q > p) o += Ac[e] * Ac[e];
         }
         double off = wave_sum(o);
-        if (off <= 1e-32 * fro) break;
+        // converged once the off-diagonal part is below 1e-14 of the Frobenius norm (rounding keeps
+        // it from going much lower: a tighter test only adds sweeps)
+        if (off <= 1e-28 * fro) break;
         for (int r = 0; r < npl - 1; ++r) {
             if (lane < half) {
                 const int k = lane;
@@ -328,56 +348,50 @@ __device__ inline void wave_psd_project(double* A, double* V, int m, int lane) {
                 const int p = a < b ? a : b, q = a < b ? b : a;
                 double c = 1.0, sn = 0.0;
                 if (q < m) {
-                    const double apq = A[p * m + q];
+                    const double apq = Ac[p * m + q];
                     if (apq != 0.0) {
-                        const double theta = (A[q * m + q] - A[p * m + p]) / (2.0 * apq);
+                        const double theta = (Ac[q * m + q] - Ac[p * m + p]) / (2.0 * apq);
                         const double tt = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
                         c = 1.0 / sqrt(tt * tt + 1.0);
                         sn = tt * c;
                     }
+                    // column p of J is (c at p, -s at q), column q is (s at p, c at q)
+                    jw_self[p] = c; jw_part[p] = -sn; j_part[p] = q;
+                    jw_self[q] = c; jw_part[q] = sn; j_part[q] = p;
+                } else if (p < m) {
+                    jw_self[p] = 1.0; jw_part[p] = 0.0; j_part[p] = p;  // paired with the extra player
                 }
-                jr_c[k] = c;
-                jr_s[k] = sn;
-                jr_p[k] = (q < m && sn != 0.0) ? p : -1;
-                jr_q[k] = q;
             }
             __syncthreads();
-            // columns: A <- A J, V <- V J
-            for (int t = lane; t < ntask; t += 64) {
-                const int k = t / m, i = t % m;
-                const int p = jr_p[k];
-                if (p < 0) continue;
-                const int q = jr_q[k];
-                const double c = jr_c[k], sn = jr_s[k];
-                const double aip = A[i * m + p], aiq = A[i * m + q];
-                A[i * m + p] = c * aip - sn * aiq;
-                A[i * m + q] = sn * aip + c * aiq;
-                const double vip = V[i * m + p], viq = V[i * m + q];
-                V[i * m + p] = c * vip - sn * viq;
-                V[i * m + q] = sn * vip + c * viq;
+            // A'[i][j] = sum over a in {i, i'}, b in {j, j'} of J[a][i] A[a][b] J[b][j]
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int i = ui[h], j = uj[h];
+                if (i < 0) continue;
+                const int ip = j_part[i], jp = j_part[j];
+                const double wi = jw_self[i], wip = jw_part[i], wj = jw_self[j], wjp = jw_part[j];
+                const double v = wi * (wj * Ac[i * m + j] + wjp * Ac[i * m + jp]) +
+                                 wip * (wj * Ac[ip * m + j] + wjp * Ac[ip * m + jp]);
+                An[i * m + j] = v;
+                An[j * m + i] = v;
+            }
+            // V'[i][j] = V[i][j] J[j][j] + V[i][j'] J[j'][j]
+            for (int e = lane; e < mm; e += 64) {
+                const int i = e / m, j = e % m;
+                Vn[e] = jw_self[j] * Vc[i * m + j] + jw_part[j] * Vc[i * m + j_part[j]];
             }
             __syncthreads();
-            // rows: A <- J^T A
-            for (int t = lane; t < ntask; t += 64) {
-                const int k = t / m, j = t % m;
-                const int p = jr_p[k];
-                if (p < 0) continue;
-                const int q = jr_q[k];
-                const double c = jr_c[k], sn = jr_s[k];
-                const double apj = A[p * m + j], aqj = A[q * m + j];
-                A[p * m + j] = c * apj - sn * aqj;
-                A[q * m + j] = sn * apj + c * aqj;
-            }
-            __syncthreads();
+            double* tA = Ac; Ac = An; An = tA;
+            double* tV = Vc; Vc = Vn; Vn = tV;
         }
     }
-    // reconstruct with clamped eigenvalues (diagonal of A)
+    // reconstruct with clamped eigenvalues (diagonal of the final A) into the caller's A
     double out[3];
     int cnt = 0;
     for (int e = lane; e < mm; e += 64) {
         int r = e / m, c = e % m;
         double h = 0;
-        for (int k = 0; k < m; ++k) h += V[r * m + k] * fmax(A[k * m + k], 0.0) * V[c * m + k];
+        for (int k = 0; k < m; ++k) h += Vc[r * m + k] * fmax(Ac[k * m + k], 0.0) * Vc[c * m + k];
         out[cnt++] = h;
     }
     __syncthreads();

@@ -163,7 +163,8 @@ int abd_enable_timing(AbdCtx* ctx, int on);
 /* No counterpart in the reference (the tile autotuner of the batched configuration): assembles
  * the system at the current state, then times `repeats` PCG solves for every PCG tile size
  * (1..5 bodies per 64-lane tile) with the whole solve in one cooperative launch and with one
- * launch per iteration, keeps the fastest and reports it.  ms[10] (optional) receives the mean
+ * launch per iteration, keeps the fastest (the layout in effect stays unless another is more than
+ * 3 % faster) and reports it.  ms[10] (optional) receives the mean
  * milliseconds per solve, index 2*(bodies_per_tile-1) + per_iteration_launch, -1 when a launch
  * form is unavailable.  The state q is unchanged. */
 int abd_autotune(AbdCtx* ctx, int repeats, int* bodies_per_tile, int* per_iteration_launch, double* ms);
