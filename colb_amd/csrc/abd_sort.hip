@@ -1,7 +1,7 @@
 // abd_sort.hip — stable LSD radix sort of 64-bit keys (optionally carrying int values) and an
 // exclusive prefix sum of ints, for the candidate, incidence and body-pair lists.
 //
-// Radix sort: 4-bit digits; up to SORT_TILE keys are ranked by one workgroup (k_sort_small).  Per pass, each 256-thread block owns a tile of 2048 consecutive
+// Radix sort: 4-bit digits; up to SORT_TILE keys are ranked directly (k_sort_small).  Per pass, each 256-thread block owns a tile of 2048 consecutive
 // elements (8 per thread, in order); it counts digits (k_radix_hist), the digit-major histogram
 // [digit][block] is prefix-summed, and the block re-reads its tile, ranks every element among
 // the equal digits of earlier threads and earlier items (a block-wide scan over a [digit][thread]
@@ -173,9 +173,11 @@ __global__ __launch_bounds__(SORT_T) void k_radix_scatter(int n, const uint64_t*
     }
 }
 
-// Whole sort of n <= SORT_TILE keys by one workgroup, by counting ranks: the keys sit in LDS and
-// the final position of key i is #{j : k_j < k_i} + #{j < i : k_j == k_i} (stable; n^2 / 256
-// comparisons per thread, cheaper than 16 radix passes of barriers at the sizes it serves).
+// Whole sort of n <= SORT_TILE keys by counting ranks: the final position of key i is
+// #{j : k_j < k_i} + #{j < i : k_j == k_i} (stable).  Every workgroup holds all n keys in LDS and
+// ranks 64 of them, four lanes per key each counting over a quarter of the keys (n / 4
+// comparisons per lane; the keys a wave reads at one step are a few LDS addresses, broadcast).
+#define SORT_SMALL_KEYS 64
 __global__ __launch_bounds__(SORT_T) void k_sort_small(int n, const uint64_t* __restrict__ kin,
                                                         const int* __restrict__ vin, uint64_t* __restrict__ kout,
                                                         int* __restrict__ vout) {
@@ -183,13 +185,17 @@ __global__ __launch_bounds__(SORT_T) void k_sort_small(int n, const uint64_t* __
     const int t = threadIdx.x;
     for (int i = t; i < n; i += SORT_T) kb[i] = kin[i];
     __syncthreads();
-    for (int i = t; i < n; i += SORT_T) {
-        const uint64_t k = kb[i];
-        int rank = 0;
-        for (int j = 0; j < n; ++j) {
-            const uint64_t o = kb[j];
-            rank += (o < k || (o == k && j < i)) ? 1 : 0;
-        }
+    const int i = blockIdx.x * SORT_SMALL_KEYS + t / 4, part = t % 4;
+    const int chunk = (n + 3) / 4, j0 = part * chunk, j1 = min(n, j0 + chunk);
+    const uint64_t k = i < n ? kb[i] : 0;
+    int rank = 0;
+    for (int j = j0; j < j1; ++j) {
+        const uint64_t o = kb[j];
+        rank += (o < k || (o == k && j < i)) ? 1 : 0;
+    }
+    rank += __shfl_xor(rank, 1, 64);
+    rank += __shfl_xor(rank, 2, 64);
+    if (i < n && part == 0) {
         rank = opaque(rank);
         kout[rank] = k;
         if (vout) vout[rank] = vin[i];
@@ -216,7 +222,8 @@ hipError_t radix_sort64(hipStream_t s, const uint64_t* kin, uint64_t* kout, cons
     int* bsum = hoff + RADIX * nblk;
     const int passes = (end_bit + 3) / 4;
     if (n <= SORT_TILE) {
-        hipLaunchKernelGGL(k_sort_small, dim3(1), dim3(SORT_T), 0, s, n, kin, vin, kout, vin ? vout : nullptr);
+        hipLaunchKernelGGL(k_sort_small, dim3(cdivs(n, SORT_SMALL_KEYS)), dim3(SORT_T), 0, s, n, kin, vin, kout,
+                           vin ? vout : nullptr);
         return hipGetLastError();
     }
     if (passes == 0) {
