@@ -293,6 +293,33 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
+// Cholesky test of A + delta I, delta = 1e-10 trace(A), for the symmetric m x m matrix A (LDS,
+// m <= 12) by one wave, in the scratch L: true when it succeeds, i.e. every eigenvalue of A
+// exceeds -delta, so that A differs from its PSD projection by at most delta (positive
+// semidefinite matrices with a null space, such as the orthogonality Hessian at a rotation, pass).
+__device__ inline bool wave_is_psd(const double* A, double* L, int m, int lane) {
+    const int mm = m * m;
+    const double tr = wave_sum(lane < m ? A[lane * m + lane] : 0.0);
+    const double delta = 1e-10 * fabs(tr);
+    for (int e = lane; e < mm; e += 64) L[e] = A[e] + ((e / m == e % m) ? delta : 0.0);
+    __syncthreads();
+    if (!(tr > 0.0)) return false;
+    for (int k = 0; k < m; ++k) {
+        const double d = L[k * m + k];
+        __syncthreads();  // every lane has read d before any lane writes L again (here or in the caller)
+        if (!(d > 0.0)) return false;  // same value in every lane: uniform exit
+        const double sd = sqrt(d);
+        if (lane > k && lane < m) L[lane * m + k] /= sd;
+        __syncthreads();
+        for (int e = lane; e < mm; e += 64) {
+            const int i = e / m, jj = e % m;
+            if (jj > k && i >= jj) L[e] -= L[i * m + k] * L[jj * m + k];
+        }
+        __syncthreads();
+    }
+    return true;
+}
+
 // Jacobi PSD projection of the symmetric m x m matrix A (LDS, row-major, m <= 12), by one wave:
 // A <- V max(Lambda, 0) V^T (Golub & Van Loan 8.5, parallel ordering 8.5.3).  The projection is a
 // unique matrix function, so it agrees with the oracle's cyclic-order psd_project to rounding;

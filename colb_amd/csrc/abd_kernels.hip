@@ -485,7 +485,9 @@ __global__ __launch_bounds__(64) void k_body_assemble(Geo G, AbdParams P, const 
         A[e] = 4.0 * kap * h;
     }
     __syncthreads();
-    wave_psd_project(A, V, 9, lane);
+    // near a rotation the ortho Hessian is already positive semidefinite: project only otherwise
+    if (!wave_is_psd(A, V, 9, lane)) wave_psd_project(A, V, 9, lane);
+    __syncthreads();
     for (int e = lane; e < 144; e += 64) {
         int R = e / 12, C = e % 12;
         int i = R / 3, r = R % 3, j = C / 3, c = C % 3;

@@ -49,6 +49,7 @@ struct EmuFiber {
     std::vector<char> stack;
     bool done = false;
     int wait = 0;  // 0 runnable, 1 at a block barrier, 2 at a wave barrier
+    int line = 0;  // source line of the block barrier it waits at
 };
 struct EmuSched {
     ucontext_t main;
@@ -88,9 +89,9 @@ inline void emu_fiber_entry() {
     EMU_START_SWITCH(nullptr, S->main_bottom, S->main_size);
 }
 
-inline void emu_block_sync() { emu_sched->fib[emu_sched->cur].wait = 1; emu_yield(); }
+inline void emu_block_sync(int line = 0) { emu_sched->fib[emu_sched->cur].wait = 1; emu_sched->fib[emu_sched->cur].line = line; emu_yield(); }
 inline void emu_wave_sync() { emu_sched->fib[emu_sched->cur].wait = 2; emu_yield(); }
-inline void __syncthreads() { emu_block_sync(); }
+#define __syncthreads() emu_block_sync(__LINE__)
 inline void __threadfence() {}
 
 template <class T>
@@ -308,7 +309,10 @@ void emu_launch(const char* name, K kernel, dim3 grid, dim3 block, Args... args)
                 }
             }
             if (!ran && !released) {
-                std::fprintf(stderr, "emu: %s block %u: barrier can never complete\n", name, b);
+                std::fprintf(stderr, "emu: %s block %u: barrier can never complete; fibres (wait, done):", name, b);
+                for (unsigned t = 0; t < nt; ++t)
+                    std::fprintf(stderr, " %u:%d%s@%d", t, S.fib[t].wait, S.fib[t].done ? "d" : "", S.fib[t].line);
+                std::fprintf(stderr, "\n");
                 std::abort();
             }
         }
