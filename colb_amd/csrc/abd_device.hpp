@@ -293,6 +293,21 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
+// Reciprocal and reciprocal square root: the hardware approximations refined by two Newton steps
+// (each squares the relative error), full double precision for the Jacobi rotations.
+__device__ __forceinline__ double rcp_nr(double a) {
+    double x = __builtin_amdgcn_rcp(a);
+    x = fma(x, fma(-a, x, 1.0), x);
+    x = fma(x, fma(-a, x, 1.0), x);
+    return x;
+}
+__device__ __forceinline__ double rsq_nr(double a) {
+    double y = __builtin_amdgcn_rsq(a);
+    y = fma(0.5 * y, fma(-a * y, y, 1.0), y);
+    y = fma(0.5 * y, fma(-a * y, y, 1.0), y);
+    return y;
+}
+
 // Cholesky test of A + delta I, delta = 1e-10 trace(A), for the symmetric m x m matrix A (LDS,
 // m <= 12) by one wave, in the scratch L: true when it succeeds, i.e. every eigenvalue of A
 // exceeds -delta, so that A differs from its PSD projection by at most delta (positive
@@ -369,17 +384,25 @@ __device__ inline void wave_psd_project(double* A, double* V, int m, int lane) {
         if (off <= 1e-28 * fro) break;
         for (int r = 0; r < npl - 1; ++r) {
             if (lane < half) {
-                const int k = lane;
-                const int a = (k == 0) ? 0 : 1 + (k - 1 + r) % (npl - 1);
-                const int b = 1 + (npl - 2 - k + r) % (npl - 1);
+                const int k = lane, n1 = npl - 1;
+                int a = k - 1 + r, b = npl - 2 - k + r;  // both < 2 (npl - 1): one conditional wrap
+                a = (k == 0) ? 0 : 1 + (a >= n1 ? a - n1 : a);
+                b = 1 + (b >= n1 ? b - n1 : b);
                 const int p = a < b ? a : b, q = a < b ? b : a;
                 double c = 1.0, sn = 0.0;
                 if (q < m) {
                     const double apq = Ac[p * m + q];
                     if (apq != 0.0) {
-                        const double theta = (Ac[q * m + q] - Ac[p * m + p]) / (2.0 * apq);
-                        const double tt = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
-                        c = 1.0 / sqrt(tt * tt + 1.0);
+                        // tan of the rotation angle, t = sign(theta) / (|theta| + sqrt(theta^2 + 1))
+                        const double theta = (Ac[q * m + q] - Ac[p * m + p]) * 0.5 * rcp_nr(apq);
+                        double tt;
+                        if (fabs(theta) > 1e150) {
+                            tt = 0.5 * rcp_nr(theta);
+                        } else {
+                            const double h = fma(theta, theta, 1.0);
+                            tt = (theta >= 0 ? 1.0 : -1.0) * rcp_nr(fabs(theta) + h * rsq_nr(h));
+                        }
+                        c = rsq_nr(fma(tt, tt, 1.0));
                         sn = tt * c;
                     }
                     // column p of J is (c at p, -s at q), column q is (s at p, c at q)

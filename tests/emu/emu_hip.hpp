@@ -204,6 +204,8 @@ inline hipError_t hipGetLastError() { hipError_t e = emu_last_error; emu_last_er
 template <class T> inline T __hip_atomic_load(T* p, int, int) { T r; __atomic_load(p, &r, __ATOMIC_SEQ_CST); return r; }
 template <class T> inline void __hip_atomic_store(T* p, T v, int, int) { __atomic_store(p, &v, __ATOMIC_SEQ_CST); }
 template <class T, class U> inline T __hip_atomic_fetch_add(T* p, U v, int, int) { return __atomic_fetch_add(p, (T)v, __ATOMIC_SEQ_CST); }
+inline double __builtin_amdgcn_rcp(double a) { return 1.0 / a; }
+inline double __builtin_amdgcn_rsq(double a) { return 1.0 / std::sqrt(a); }
 inline void __builtin_amdgcn_s_sleep(int) { emu_yield(); }
 inline void __builtin_amdgcn_s_waitcnt(int) {}
 inline void __builtin_amdgcn_wave_barrier() { emu_wave_sync(); }
