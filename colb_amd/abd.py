@@ -275,6 +275,21 @@ class NewtonAbd:
         _lib.check(L.abd_set_warm_start(self._h, ws.mlp._h, *a), self._h, "abd_set_warm_start")
         self._warm = ws  # keeps the model alive while the solver uses it
 
+    def set_preconditioner(self, lp):
+        """Use a trained colb_amd.learned.LearnedPreconditioner in every PCG solve in place of the
+        block-Jacobi inverses (None: block Jacobi)."""
+        L = _lib.load()
+        if lp is None:
+            _lib.check(L.abd_set_preconditioner(self._h, None, None, None, None, None), self._h,
+                       "abd_set_preconditioner")
+            self._precond = None
+            return
+        self._precond_arrays = [np.ascontiguousarray(a, dtype=np.float32) for a in
+                                (lp.feat_mean, lp.feat_std, lp.out_mean, lp.out_std)]
+        a = [x.ctypes.data_as(C.POINTER(C.c_float)) for x in self._precond_arrays]
+        _lib.check(L.abd_set_preconditioner(self._h, lp.mlp._h, *a), self._h, "abd_set_preconditioner")
+        self._precond = lp  # keeps the model alive while the solver uses it
+
     def enable_timing(self, on=True):
         _lib.check(_lib.load().abd_enable_timing(self._h, int(on)), self._h, "abd_enable_timing")
 

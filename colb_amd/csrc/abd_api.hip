@@ -120,6 +120,9 @@ struct AbdCtx {
     // mixed precision (AbdParams.precision == 1): FP32 copies of the PCG operands and vectors
     float *f_vec = nullptr, *f_diag = nullptr, *f_pinv = nullptr, *f_off = nullptr, *f_offT = nullptr, *f_mq = nullptr;
     int f_pair_cap = 0;
+    // learned block preconditioner (abd_set_preconditioner)
+    AbdMlp* pre_mlp = nullptr;
+    float *pfmean = nullptr, *pfistd = nullptr, *pomean = nullptr, *postd = nullptr;
     // learned warm start (abd_set_warm_start) and training-sample log (abd_log_samples)
     AbdMlp* mlp = nullptr;
     float *feat_raw = nullptr, *fmean = nullptr, *fistd = nullptr, *omean = nullptr, *ostd = nullptr;
@@ -577,6 +580,27 @@ int warm_start(AbdCtx* ctx) {
     return ABD_OK;
 }
 
+// Learned block preconditioner: body features of the current assembly (with the block-Jacobi
+// inverse still in pinv) -> MLP -> pinv <- L_D^-T T T^T L_D^-1 per unpinned body
+int learned_precond(AbdCtx* ctx) {
+    AbdMlp* m = ctx->pre_mlp;
+    launch_body_features(ctx->st, ctx->G, ctx->q, ctx->qtilde, ctx->qdot, ctx->n_inc, ctx->inc_s, ctx->act_g,
+                         ctx->n_act, ctx->pinv, ctx->grad, ctx->feat_raw);
+    CHK(dbg(ctx, "launch_body_features"));
+    int rc = mlp_reserve(m, ctx->nb);
+    if (rc != ABD_OK) {
+        ctx->err = "learned preconditioner: " + m->err;
+        return rc;
+    }
+    const int rows = (ctx->nb + 15) / 16 * 16;
+    launch_features_pack(ctx->st, ctx->nb, rows, m->dp[0], ctx->feat_raw, ctx->pfmean, ctx->pfistd, m->A[0]);
+    mlp_forward_dev(m, ctx->nb, ctx->st);
+    launch_learned_pinv(ctx->st, ctx->G, m->dp[m->L], m->Z[m->L - 1], ctx->pomean, ctx->postd, ctx->diag, ctx->pinv);
+    CHK(dbg(ctx, "learned preconditioner"));
+    HIPCHK(hipGetLastError());
+    return ABD_OK;
+}
+
 // after the PCG: add the warm start, restore ||b|| as the residual, log training samples
 int solve_finish(AbdCtx* ctx, bool warm) {
     if (warm) {
@@ -753,6 +777,7 @@ int solve(AbdCtx* ctx) {
         b = ctx->rhs;
         bref = ctx->nrm_b;
     }
+    if (ctx->pre_mlp) CHK(learned_precond(ctx));  // after the warm start's features, which use D^-1
     if (ctx->P.precision == 1) {
         // mixed precision: the assembled FP64 system is rounded once to FP32 and solved in FP32
         // (partial sums and scalars stay FP64); the FP32 solution is widened into du
@@ -1261,6 +1286,7 @@ int abd_create(const AbdSceneDesc* d, const AbdParams* p, AbdCtx** out) {
     A(r1, 12 * nb); A(w1, 12 * nb); A(s0, 12 * nb); A(pcg_sc, 4 * ns); A(pcg_ctl, 8);
     A(feat_raw, (size_t)ABD_N_FEATURES * nb); A(fmean, ABD_N_FEATURES); A(fistd, ABD_N_FEATURES); A(omean, 12);
     A(ostd, 12); A(x0, 12 * nb); A(rhs, 12 * nb); A(nrm_b, ns); A(nrm_r, ns); A(warm_used, ns);
+    A(pfmean, ABD_N_FEATURES); A(pfistd, ABD_N_FEATURES); A(pomean, ABD_N_PRECOND); A(postd, ABD_N_PRECOND);
     if ((rc = set_tiles(ctx, 5)) != ABD_OK) return fail(rc);
     A(bnorm, ns); A(cg_iters, ns); A(cg_done, ns);
     A(C.newton_active, ns); A(C.residual, ns); A(C.alpha, ns); A(C.alpha0, ns); A(C.amax_bits, ns);
@@ -1659,6 +1685,31 @@ int abd_set_warm_start(AbdCtx* ctx, AbdMlp* model, const float* feat_mean, const
     HIPCHK(hipMemcpy(ctx->ostd, out_std, 12 * sizeof(float), hipMemcpyHostToDevice));
     HIPCHK(hipStreamSynchronize(model->st));
     ctx->mlp = model;
+    return ABD_OK;
+}
+
+int abd_set_preconditioner(AbdCtx* ctx, AbdMlp* model, const float* feat_mean, const float* feat_std,
+                           const float* out_mean, const float* out_std) {
+    CHK(check_ctx(ctx));
+    if (!model) {
+        ctx->pre_mlp = nullptr;
+        return ABD_OK;
+    }
+    if (!feat_mean || !feat_std || !out_mean || !out_std || model->L < 1 || model->d[0] != ABD_N_FEATURES ||
+        model->d[model->L] != ABD_N_PRECOND) {
+        ctx->err = "preconditioner model must map " + std::to_string(ABD_N_FEATURES) + " body features to " +
+                   std::to_string(ABD_N_PRECOND) + " factor entries, with normalisation arrays";
+        return ABD_ERR_ARG;
+    }
+    std::vector<float> istd(ABD_N_FEATURES);
+    for (int j = 0; j < ABD_N_FEATURES; ++j) istd[j] = feat_std[j] > 0.f ? 1.f / feat_std[j] : 0.f;
+    HIPCHK(hipMemcpy(ctx->pfmean, feat_mean, ABD_N_FEATURES * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->pfistd, istd.data(), ABD_N_FEATURES * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->pomean, out_mean, ABD_N_PRECOND * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->postd, out_std, ABD_N_PRECOND * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(hipStreamSynchronize(model->st));
+    ctx->pre_mlp = model;
+    ctx->solved = false;
     return ABD_OK;
 }
 

@@ -159,6 +159,9 @@ void launch_features_pack(hipStream_t s, int nb, int rows, int dp, const float* 
                           const float* istd, uint16_t* out);
 void launch_warm_x0(hipStream_t s, const Geo& G, int dpo, const float* Y, const float* omean, const float* ostd,
                     const float* feat, double* x0);
+constexpr int ABD_N_PRECOND = 78;  // learned preconditioner outputs: 12 log-diagonal + 66 lower entries
+void launch_learned_pinv(hipStream_t s, const Geo& G, int dpo, const float* Y, const float* omean, const float* ostd,
+                         const double* diag, double* pinv);
 void launch_residual(hipStream_t s, int nb, const double* b, const double* diag, const int* row_ptr,
                      const uint64_t* ent_key, const int* ent_val, const double* off, const double* offT,
                      const double* x, double* r, int n_pairs, int* err);

@@ -1254,6 +1254,76 @@ void launch_warm_x0(hipStream_t s, const Geo& G, int dpo, const float* Y, const 
     hipLaunchKernelGGL(k_warm_x0, dim3(cdiv(12 * G.nb, 256)), dim3(256), 0, s, G, dpo, Y, omean, ostd, feat, x0);
 }
 
+// Learned block preconditioner: P_b = L_D^-T (T T^T) L_D^-1, where D_b = L_D L_D^T is the body's
+// diagonal block and T the lower-triangular factor the model predicts (diagonal exp of its first 12
+// outputs, the 66 entries below the diagonal row by row from the rest), both de-standardised.
+// T = I gives the block-Jacobi D_b^-1; any T with a positive diagonal gives an SPD P_b.  One wave
+// per body; pinned bodies and blocks whose Cholesky fails keep the block-Jacobi inverse.
+__global__ __launch_bounds__(64) void k_learned_pinv(Geo G, int dpo, const float* __restrict__ Y,
+                                                      const float* __restrict__ omean, const float* __restrict__ ostd,
+                                                      const double* __restrict__ diag, double* __restrict__ pinv) {
+    __shared__ double Lb[144], Li[144], Tm[144], X[144];
+    const int b = blockIdx.x, lane = threadIdx.x;
+    if (G.pinned[b]) return;
+    for (int e = lane; e < 144; e += 64) {
+        Lb[e] = diag[(size_t)b * 144 + e];
+        Tm[e] = 0.0;
+    }
+    __syncthreads();
+    // Cholesky D = L L^T in place (lower triangle)
+    for (int k = 0; k < 12; ++k) {
+        const double d = Lb[k * 12 + k];
+        __syncthreads();  // every lane has read the pivot before it is overwritten
+        if (!(d > 0.0)) return;  // same value in every lane: uniform exit, block-Jacobi kept
+        const double sd = sqrt(d);
+        if (lane == 0) Lb[k * 12 + k] = sd;
+        if (lane > k && lane < 12) Lb[lane * 12 + k] /= sd;
+        __syncthreads();
+        for (int e = lane; e < 144; e += 64) {
+            const int i = e / 12, j = e % 12;
+            if (j > k && i >= j) Lb[e] -= Lb[i * 12 + k] * Lb[j * 12 + k];
+        }
+        __syncthreads();
+    }
+    // Li = L^-1 (lower): lane j solves L x = e_j by forward substitution
+    if (lane < 12) {
+        const int j = lane;
+        for (int i = 0; i < 12; ++i) {
+            double v = (i == j) ? 1.0 : 0.0;
+            for (int k = j; k < i; ++k) v -= Lb[i * 12 + k] * Li[k * 12 + j];
+            Li[i * 12 + j] = (i < j) ? 0.0 : v / Lb[i * 12 + i];
+        }
+    }
+    // T from the model output row of this body
+    const float* y = Y + (size_t)b * dpo;
+    if (lane < 12) Tm[lane * 12 + lane] = exp((double)(y[lane] * ostd[lane] + omean[lane]));
+    for (int m = lane; m < 66; m += 64) {
+        int i = 1, rem = m;
+        while (rem >= i) { rem -= i; ++i; }  // m-th entry below the diagonal, row by row
+        Tm[i * 12 + rem] = (double)(y[12 + m] * ostd[12 + m] + omean[12 + m]);
+    }
+    __syncthreads();
+    // X = Li^T T, then P = X X^T
+    for (int e = lane; e < 144; e += 64) {
+        const int i = e / 12, j = e % 12;
+        double v = 0.0;
+        for (int k = 0; k < 12; ++k) v += Li[k * 12 + i] * Tm[k * 12 + j];
+        X[e] = v;
+    }
+    __syncthreads();
+    for (int e = lane; e < 144; e += 64) {
+        const int i = e / 12, j = e % 12;
+        double v = 0.0;
+        for (int k = 0; k < 12; ++k) v += X[i * 12 + k] * X[j * 12 + k];
+        pinv[(size_t)b * 144 + e] = v;
+    }
+}
+
+void launch_learned_pinv(hipStream_t s, const Geo& G, int dpo, const float* Y, const float* omean, const float* ostd,
+                         const double* diag, double* pinv) {
+    hipLaunchKernelGGL(k_learned_pinv, dim3(G.nb), dim3(64), 0, s, G, dpo, Y, omean, ostd, diag, pinv);
+}
+
 // r = b - H x (H: diagonal blocks + CSR off-diagonal blocks), one thread per row
 __global__ void k_residual(int nb, const double* __restrict__ bvec, const double* __restrict__ diag,
                            const int* __restrict__ row_ptr, const uint64_t* __restrict__ ent_key,

@@ -1,4 +1,5 @@
-"""Learned warm start of the ABD Newton step (no counterpart in the reference).
+"""Learned warm start and learned block preconditioner of the ABD Newton step (no counterpart in
+the reference).
 
 A per-body MLP maps body features to the body's Newton step du (the 12 DOFs of the PCG
 solution), which then seeds the PCG as its initial guess.  The model runs on the GPU through
@@ -176,3 +177,89 @@ class WarmStart:
         for k in ("feat_mean", "feat_std", "out_mean", "out_std"):
             setattr(ws, k, z[k].astype(np.float32))
         return ws
+
+
+N_PRECOND = 78
+PRECOND_DIMS = (N_FEATURES, 128, 128, 128, N_PRECOND)
+_TRIL = np.tril_indices(12, -1)
+
+
+def precond_target(D, Z):
+    """Target outputs of the learned preconditioner for one body: with D = L L^T its diagonal block
+    and Z the body's diagonal block of H^-1, T = chol(L^T Z L) (T = I when the body is uncoupled);
+    returns [log diag T (12), T below the diagonal row by row (66)]."""
+    L = np.linalg.cholesky(D)
+    T = np.linalg.cholesky(L.T @ Z @ L)
+    return np.concatenate([np.log(np.diag(T)), T[_TRIL]])
+
+
+def precond_matrix(D, out):
+    """P = L^-T T T^T L^-1 from the 78 (de-standardised) outputs, as the device builds it."""
+    L = np.linalg.cholesky(D)
+    T = np.diag(np.exp(out[:12]))
+    T[_TRIL] = out[12:]
+    X = np.linalg.solve(L.T, T)
+    return X @ X.T
+
+
+def precond_samples(solver, steps):
+    """(features, targets) for the learned preconditioner from `steps` steps of a small scene: at
+    the start of every step the assembled Hessian is inverted densely (12 nb x 12 nb) and each
+    unpinned body's diagonal block of H^-1 gives its target.  Advances the solver by `steps`."""
+    F, T = [], []
+    pinned = np.asarray(solver.scene.body_pinned, dtype=bool)
+    for _ in range(steps):
+        g, D, pij, off, _e = solver.assemble()
+        nb = D.shape[0]
+        H = np.zeros((12 * nb, 12 * nb))
+        for b in range(nb):
+            H[12 * b:12 * b + 12, 12 * b:12 * b + 12] = D[b]
+        for (i, j), B in zip(pij, off):
+            H[12 * i:12 * i + 12, 12 * j:12 * j + 12] = B
+            H[12 * j:12 * j + 12, 12 * i:12 * i + 12] = B.T
+        Hi = np.linalg.inv(H)
+        solver.log_samples(True)
+        solver.compute_du()
+        f, _ = solver.samples()
+        solver.log_samples(False)
+        rows = [b for b in range(nb) if not pinned[b]]
+        if len(rows) == len(f):
+            for k, b in enumerate(rows):
+                Z = Hi[12 * b:12 * b + 12, 12 * b:12 * b + 12]
+                Zs = 0.5 * (Z + Z.T)
+                F.append(f[k])
+                T.append(precond_target(D[b], Zs))
+        solver.step()
+    return np.asarray(F, dtype=np.float32), np.asarray(T, dtype=np.float32)
+
+
+class LearnedPreconditioner:
+    """The trained block-preconditioner model with its feature/target standardisation."""
+
+    def __init__(self, dims=PRECOND_DIMS, seed=0):
+        self.dims = tuple(dims)
+        self.mlp = Mlp(self.dims, seed=seed)
+        self.feat_mean = np.zeros(N_FEATURES, dtype=np.float32)
+        self.feat_std = np.ones(N_FEATURES, dtype=np.float32)
+        self.out_mean = np.zeros(N_PRECOND, dtype=np.float32)
+        self.out_std = np.ones(N_PRECOND, dtype=np.float32)
+
+    def _inputs(self, feat):
+        X = (np.asarray(feat, dtype=np.float64) - self.feat_mean) / self.feat_std
+        return np.clip(X, -FEAT_CLIP, FEAT_CLIP).astype(np.float32)
+
+    def fit(self, feat, target, steps=3000, batch=512, lr=1e-3, seed=0):
+        """Standardise, then train with Adam on the GPU; returns the batch losses."""
+        feat = np.asarray(feat, dtype=np.float64)
+        t = np.asarray(target, dtype=np.float64)
+        self.feat_mean = feat.mean(0).astype(np.float32)
+        self.feat_std = np.maximum(feat.std(0), 1e-12).astype(np.float32)
+        self.out_mean = t.mean(0).astype(np.float32)
+        self.out_std = np.maximum(t.std(0), 1e-12).astype(np.float32)
+        X = self._inputs(feat)
+        T = ((t - self.out_mean) / self.out_std).astype(np.float32)
+        return self.mlp.train(X, T, steps=steps, batch=min(batch, X.shape[0]), lr=lr, seed=seed)
+
+    def predict(self, feat):
+        """De-standardised outputs (78 per row)."""
+        return self.mlp.forward(self._inputs(feat)) * self.out_std + self.out_mean

@@ -187,6 +187,15 @@ typedef struct AbdMlp AbdMlp;
 #endif
 int abd_set_warm_start(AbdCtx* ctx, AbdMlp* model, const float* feat_mean, const float* feat_std,
                        const float* out_mean, const float* out_std);
+/* No counterpart in the reference (the learned block preconditioner of the north-star
+ * configuration): every later solve computes the 49 body features (as for the warm start, with the
+ * block-Jacobi step), standardises and clips them, runs the model (49 -> ... -> 78) and replaces the
+ * block-Jacobi inverse D_b^-1 of each unpinned body by P_b = L^-T T T^T L^-1, D_b = L L^T, where T is
+ * lower triangular with diagonal exp(out[0..11]) and the entries below it, row by row, out[12..77]
+ * (out de-standardised by out_mean/out_std).  T = I is block Jacobi; P_b is SPD for any output, so
+ * the PCG and its stopping test are unchanged.  model == NULL switches it off. */
+int abd_set_preconditioner(AbdCtx* ctx, AbdMlp* model, const float* feat_mean, const float* feat_std,
+                           const float* out_mean, const float* out_std);
 /* No counterpart in the reference (training data of the warm start): while on, every solve appends
  * (raw features[49], du[12]) of each unpinned body of each still-iterating scene; on/off clears. */
 int abd_log_samples(AbdCtx* ctx, int on);

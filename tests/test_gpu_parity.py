@@ -312,3 +312,97 @@ def test_mixed_precision_tracks_fp64():
     say(f"after 20 steps: x rel {ex:.2e}, E rel {eE:.2e}, grad rel {eg:.2e}")
     # the gradient magnifies a position difference by the barrier stiffness of the contacts
     assert ex < 1e-6 and eE < 1e-6 and eg < 1e-3
+
+
+def _dense_H(D, pij, off):
+    nb = D.shape[0]
+    H = np.zeros((12 * nb, 12 * nb))
+    for b in range(nb):
+        H[12 * b:12 * b + 12, 12 * b:12 * b + 12] = D[b]
+    for (i, j), B in zip(pij, off):
+        H[12 * i:12 * i + 12, 12 * j:12 * j + 12] = B
+        H[12 * j:12 * j + 12, 12 * i:12 * i + 12] = B.T
+    return H
+
+
+def _block_pcg(H, b, blocks, tol):
+    """Textbook PCG with a block-diagonal preconditioner (reference for the device solve)."""
+    def prec(v):
+        return np.concatenate([P @ v[12 * k:12 * k + 12] for k, P in enumerate(blocks)])
+    x, r = np.zeros(len(b)), b.copy()
+    z = prec(r)
+    pv, rz, bn, it = z.copy(), r @ z, np.linalg.norm(b), 0
+    while np.linalg.norm(r) > tol * bn and it < 5000:
+        Ap = H @ pv
+        a = rz / (pv @ Ap)
+        x += a * pv
+        r -= a * Ap
+        it += 1
+        z = prec(r)
+        rz2 = r @ z
+        pv = z + (rz2 / rz) * pv
+        rz = rz2
+    return x, it
+
+
+@pytest.mark.parametrize("c_scale", [0.0, 0.3])
+def test_learned_preconditioner_matches_block_pcg(c_scale):
+    """A preconditioner model with constant output c gives P_b = L^-T T T^T L^-1 (D_b = L L^T) on every
+    unpinned body: the device PCG agrees with a textbook block PCG using precond_matrix(D_b, c) in
+    solution (1e-8) and iteration count (+-2); c = 0 is block Jacobi (same count as plain PCG)."""
+    from colb_amd.learned import LearnedPreconditioner, precond_matrix, N_PRECOND
+    sc, p = CASES["chain3"]()
+    p = p.replace(cg_tol=1e-12, cg_max_iter=2000)
+    q, qd = perturbed(sc, 1, amp=1e-4)
+    g = solver(sc, p)
+    g.set_state(q, sc.q0, qd)
+    gg, D, pij, off, _ = g.assemble()
+    g.compute_du()
+    plain_it = int(g.last_cg_iters[0])
+    c = c_scale * np.random.default_rng(5).normal(size=N_PRECOND)
+    lp = LearnedPreconditioner()
+    lp.mlp.set_params(np.zeros(lp.mlp.n_params, dtype=np.float32))  # output 0, de-standardised to c
+    lp.out_mean = c.astype(np.float32)
+    lp.out_std = np.ones(N_PRECOND, dtype=np.float32)
+    g.set_preconditioner(lp)
+    g.compute_du()
+    pinned = sc.body_pinned.astype(bool)
+    blocks = [np.eye(12) / p.pinned_diag if pinned[b] else precond_matrix(D[b], c) for b in range(D.shape[0])]
+    x, it = _block_pcg(_dense_H(D, pij, off), gg.reshape(-1), blocks, 1e-12)
+    say(f"c_scale {c_scale}: device {int(g.last_cg_iters[0])} iterations, textbook {it}, plain {plain_it}")
+    assert rel(g._du.reshape(-1), x) < 1e-8
+    assert abs(int(g.last_cg_iters[0]) - it) <= 2
+    if c_scale == 0.0:
+        assert int(g.last_cg_iters[0]) == plain_it
+
+
+def test_learned_preconditioner_trains_and_keeps_the_solution():
+    """The preconditioner is trained on (features, chol(L^T Z L)) pairs, Z the body's diagonal block of
+    the exact H^-1 of the solver's own assembled systems (cubes on the ground, eight seeds); on a
+    held-out seed it leaves the trajectory and the Newton iterations unchanged (1e-6) and prints the
+    PCG iteration counts of both preconditioners."""
+    from colb_amd.learned import LearnedPreconditioner, precond_samples
+    p = north_star(dt=8e-3, ground_enabled=True, dhat=1e-3, kappa=1e7, k_contact=1e20, cg_tol=1e-10)
+    F, Tt = [], []
+    for seed in range(11, 19):
+        f, t = precond_samples(solver(S.cubes_on_ground(seed=seed), p), 40)
+        F.append(f)
+        Tt.append(t)
+    F, Tt = np.concatenate(F), np.concatenate(Tt)
+    lp = LearnedPreconditioner(seed=0)
+    losses = lp.fit(F, Tt, steps=2000, batch=512, lr=1e-3)
+    assert losses[-100:].mean() < 0.5 * losses[:20].mean()
+    runs = {}
+    for name, pre in (("block Jacobi", None), ("learned", lp)):
+        g = solver(S.cubes_on_ground(seed=19), p)
+        g.set_preconditioner(pre)
+        cg = nw = 0
+        for k in range(40):
+            st = g.step()[0]
+            cg += st["cg_iters"]
+            nw += st["newton_iters"]
+        runs[name] = (cg, nw, g.get_state()[0].copy())
+    say(f"A/B over 40 steps (held-out seed): block Jacobi {runs['block Jacobi'][:2]} (cg, newton), "
+        f"learned {runs['learned'][:2]}; training rows {len(F)}")
+    assert rel(runs["learned"][2], runs["block Jacobi"][2]) < 1e-6
+    assert runs["learned"][1] == runs["block Jacobi"][1]
