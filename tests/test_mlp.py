@@ -90,3 +90,20 @@ def test_training_lowers_the_loss():
     l1, _ = m.gradient(X, T)
     assert np.all(np.isfinite(losses))
     assert l1 < 0.2 * l0
+
+
+def test_preconditioner_factor_round_trip():
+    """The learned preconditioner's parametrisation: precond_matrix(D, precond_target(D, Z)) == Z for
+    an SPD block D and an SPD target Z (the exact block of H^-1 of a coupled system), and all-zero
+    outputs give D^-1 (block Jacobi)."""
+    from colb_amd.learned import precond_matrix, precond_target
+    rng = np.random.default_rng(3)
+    A = rng.normal(size=(24, 24))
+    H = A @ A.T + 24 * np.eye(24)
+    D = H[:12, :12]
+    Z = np.linalg.inv(H)[:12, :12]
+    Z = 0.5 * (Z + Z.T)
+    out = precond_target(D, Z)
+    assert out.shape == (78,)
+    assert np.allclose(precond_matrix(D, out), Z, rtol=1e-10, atol=1e-12 * np.abs(Z).max())
+    assert np.allclose(precond_matrix(D, np.zeros(78)), np.linalg.inv(D), rtol=1e-10, atol=1e-14)
