@@ -110,6 +110,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=2)
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--method", default="plain", choices=["plain", "learned"],
+                    help="learned: train the warm start on the iterates logged during warmup (on the GPU), "
+                         "then time the steps with it")
     ap.add_argument("--pcg-layout", default=None, metavar="BPT[,multi]",
                     help="fix the PCG layout (bodies per tile, 1 = one launch per iteration) instead of autotuning")
     ap.add_argument("--calib-copy", action="store_true",
@@ -131,9 +134,25 @@ def main():
     print(f"# workload {cfg['workload']}: {sc.n_bodies} bodies, {sc.n_verts} verts, {sc.n_tris} tris, "
           f"{sc.n_edges} edges", file=sys.stderr, flush=True)
     solver = NewtonAbd(p.dt, sc, params=p)
+    learned = None
+    if args.method == "learned":
+        solver.log_samples(True)
     for k in range(args.warmup):
         st = solver.step()[0]
         print(f"# warmup {k}: {st}", file=sys.stderr, flush=True)
+    if args.method == "learned":
+        from colb_amd.learned import WarmStart
+        F, D = solver.samples()
+        solver.log_samples(False)
+        t_fit = time.perf_counter()
+        ws = WarmStart(seed=0)
+        losses = ws.fit(F, D, steps=2000, batch=512, lr=1e-3)
+        learned = {"model": "warm start 49-128-128-128-12, BF16 weights", "training_rows": int(len(F)),
+                   "adam_steps": 2000, "loss_first": round(float(losses[:20].mean()), 4),
+                   "loss_last": round(float(losses[-100:].mean()), 4),
+                   "train_s": round(time.perf_counter() - t_fit, 2)}
+        solver.set_warm_start(ws)
+        print(f"# learned: {learned}", file=sys.stderr, flush=True)
     tune = None
     if args.pcg_layout:
         parts = [int(x) for x in args.pcg_layout.split(",")]
@@ -203,7 +222,7 @@ def main():
         "dtype": "f64" if args.precision == "fp64" else "f32+f64",
         "data": "synthetic (procedural scene)", "config": cfg,
         "newton_iters_per_step": round(float(np.mean(iters)), 3), "cg_iters_per_step": round(float(np.mean(cg)), 1),
-        "method": "plain", "autotune": tune, "pcg_us_per_iteration": pcg_us_per_iter,
+        "method": args.method, "learned": learned, "autotune": tune, "pcg_us_per_iteration": pcg_us_per_iter,
         "roofline": roof, "cpu_baseline": cpu,
     }
     print(json.dumps(result), flush=True)
