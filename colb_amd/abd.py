@@ -69,6 +69,7 @@ class NewtonAbd:
         self.profiler = Profiller(max_iters=self.params.max_newton, frames=max_frames)
         self.last_stats = None
         self._du = None
+        self.pcg_layout = (5, False)  # PCG bodies per tile, one launch per iteration
         L = _lib.load()
         self._desc = make_scene_desc(scene)
         self._cp = self.params.to_c()
@@ -93,6 +94,7 @@ class NewtonAbd:
 
     def set_params(self, params: Params):
         self.params = params
+        self.dt = float(params.dt)
         self._cp = params.to_c()
         _lib.check(_lib.load().abd_set_params(self._h, C.byref(self._cp)), self._h, "abd_set_params")
 
@@ -239,12 +241,25 @@ class NewtonAbd:
         ms = np.zeros(10)
         _lib.check(_lib.load().abd_autotune(self._h, int(repeats), C.byref(bpt), C.byref(multi), dptr(ms)), self._h,
                    "abd_autotune")
+        self.pcg_layout = (bpt.value, bool(multi.value))
         return bpt.value, multi.value, ms.reshape(5, 2)
 
     def set_pcg_layout(self, bodies_per_tile, per_iteration_launch=False):
         """Fix the PCG tile size and launch form (e.g. a result of autotune())."""
         _lib.check(_lib.load().abd_set_pcg_layout(self._h, int(bodies_per_tile), int(bool(per_iteration_launch))),
                    self._h, "abd_set_pcg_layout")
+        self.pcg_layout = (int(bodies_per_tile), bool(per_iteration_launch))
+
+    def save_checkpoint(self, path):
+        """Write state, frame, parameters and PCG layout to an .npz (colb_amd.checkpoint); a
+        solver on the same scene that loads it continues the run bit for bit."""
+        from . import checkpoint
+        checkpoint.save(self, path)
+
+    def load_checkpoint(self, path):
+        """Restore what save_checkpoint wrote (the scene must be the same)."""
+        from . import checkpoint
+        return checkpoint.load(self, path)
 
     def log_samples(self, on=True):
         """Start (and clear) or stop logging (features, du) training samples of every solve."""
