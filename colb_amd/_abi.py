@@ -38,6 +38,8 @@ class AbdParams(C.Structure):
         ("pinned_diag", C.c_double),
         ("max_candidates", C.c_int),
         ("reserved0", C.c_int),
+        ("solver", C.c_int),
+        ("direct_max_dofs", C.c_int),
     ]
 
 

@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "lib")
 LIB = os.path.join(OUT_DIR, "libabd_hip.so")
-SOURCES = ["abd_kernels.hip", "abd_broad.hip", "abd_sort.hip", "abd_mlp.hip", "abd_api.hip"]
+SOURCES = ["abd_kernels.hip", "abd_broad.hip", "abd_sort.hip", "abd_mlp.hip", "abd_direct.hip", "abd_api.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=fast",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -130,6 +130,14 @@ struct AbdCtx {
     int* warm_used = nullptr;
     bool log_on = false;
     std::vector<float> log_feat, log_du;
+    // direct solver (AbdParams.solver 1 or 2): dense per-scene factors, sized on first use
+    DirectBuf dn;
+    long long* dn_off = nullptr;
+    int *dn_N = nullptr, *dn_mask = nullptr, *dn_fail = nullptr;
+    double *dn_A = nullptr, *dn_L = nullptr;
+    size_t dn_cap = 0;  // doubles allocated at dn_A
+    int dn_maxd = -1;   // direct_max_dofs the tables were built for
+    std::vector<int> h_dn_N;
     int bodies_per_tile = 5;
     int* cg_iters = nullptr;
     int* cg_done = nullptr;
@@ -602,6 +610,112 @@ int learned_precond(AbdCtx* ctx) {
 }
 
 // after the PCG: add the warm start, restore ||b|| as the residual, log training samples
+int check_params(AbdCtx* ctx, const AbdParams* p) {
+    if (p->precision != 0 && p->precision != 1) {
+        ctx->err = "precision must be 0 (FP64) or 1 (mixed)";
+        return ABD_ERR_ARG;
+    }
+    if (p->solver < 0 || p->solver > 2) {
+        ctx->err = "solver must be 0 (PCG), 1 (direct) or 2 (PCG with direct fallback)";
+        return ABD_ERR_ARG;
+    }
+    if (p->direct_max_dofs > DIRECT_MAX_DOFS) {
+        ctx->err = "direct_max_dofs above " + std::to_string(DIRECT_MAX_DOFS);
+        return ABD_ERR_ARG;
+    }
+    return ABD_OK;
+}
+
+// per-scene dense matrix tables of the direct solver: padded order N_s (0 for a scene above
+// direct_max_dofs) and offsets; the matrices are allocated once for every scene that fits
+int ensure_dense(AbdCtx* ctx) {
+    const int maxd = ctx->P.direct_max_dofs <= 0 ? DIRECT_MAX_DOFS : ctx->P.direct_max_dofs;
+    if (ctx->dn_maxd == maxd) return ABD_OK;
+    const int ns = ctx->ns;
+    std::vector<long long> off(ns, 0);
+    std::vector<int> N(ns, 0);
+    long long total = 0;
+    int nmax = 0;
+    for (int s = 0; s < ns; ++s) {
+        const int nr = 12 * (ctx->h_scene_b0[s + 1] - ctx->h_scene_b0[s]);
+        if (nr > maxd) continue;
+        const int n = (nr + DIRECT_TILE - 1) / DIRECT_TILE * DIRECT_TILE;
+        off[s] = total;
+        N[s] = n;
+        total += (long long)n * n;
+        nmax = std::max(nmax, n);
+    }
+    if (total * 8 > (64LL << 30)) {
+        ctx->err = "direct solver: the dense matrices would take " + std::to_string(total * 8 >> 20) +
+                   " MiB (limit 64 GiB); lower direct_max_dofs";
+        return ABD_ERR_ARG;
+    }
+    if (!ctx->dn_N) {
+        CHK(dalloc(ctx, &ctx->dn_off, ns));
+        CHK(dalloc(ctx, &ctx->dn_N, ns));
+        CHK(dalloc(ctx, &ctx->dn_mask, ns));
+        CHK(dalloc(ctx, &ctx->dn_fail, ns));
+        CHK(dalloc(ctx, &ctx->dn_L, (size_t)ns * DIRECT_TILE * DIRECT_TILE));
+    }
+    if ((size_t)total > ctx->dn_cap) {
+        CHK(dfree(ctx, ctx->dn_A));
+        CHK(dalloc(ctx, &ctx->dn_A, (size_t)total));
+        ctx->dn_cap = (size_t)total;
+    }
+    HIPCHK(hipMemcpyAsync(ctx->dn_off, off.data(), ns * sizeof(long long), hipMemcpyHostToDevice, ctx->st));
+    HIPCHK(hipMemcpyAsync(ctx->dn_N, N.data(), ns * sizeof(int), hipMemcpyHostToDevice, ctx->st));
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    DirectBuf& D = ctx->dn;
+    D.A = ctx->dn_A; D.Lkk = ctx->dn_L; D.off = ctx->dn_off; D.N = ctx->dn_N; D.mask = ctx->dn_mask;
+    D.fail = ctx->dn_fail; D.n_max = nmax;
+    ctx->h_dn_N = N;
+    ctx->dn_maxd = maxd;
+    return ABD_OK;
+}
+
+// direct solve of H du = grad (compute_du's direct branch, abd.py:637-654): every active scene
+// (fallback false), or the active scenes whose PCG ended at cg_max_iter (fallback true)
+int solve_direct(AbdCtx* ctx, bool fallback) {
+    CHK(ensure_dense(ctx));
+    const int ns = ctx->ns;
+    if (!fallback)
+        for (int s = 0; s < ns; ++s)
+            if (ctx->h_dn_N[s] == 0) {
+                ctx->err = "direct solver: scene " + std::to_string(s) + " has " +
+                           std::to_string(12 * (ctx->h_scene_b0[s + 1] - ctx->h_scene_b0[s])) +
+                           " unknowns, above direct_max_dofs " + std::to_string(ctx->dn_maxd);
+                return ABD_ERR_ARG;
+            }
+    HIPCHK(hipMemsetAsync(ctx->cnt + C_ANY, 0, sizeof(int), ctx->st));
+    launch_direct_mask(ctx->st, ns, fallback ? 1 : 0, ctx->P.cg_max_iter, ctx->dn_N, ctx->C.newton_active,
+                       ctx->cg_iters, ctx->dn_mask, ctx->cnt + C_ANY);
+    CHK(dbg(ctx, "launch_direct_mask"));
+    if (fallback) {
+        CHK(sync_counters(ctx));
+        if (!ctx->h_cnt[C_ANY]) return ABD_OK;
+    }
+    HIPCHK(hipMemsetAsync(ctx->dn_fail, 0, ns * sizeof(int), ctx->st));
+    tic(ctx, "direct_factor");
+    launch_direct_factor(ctx->st, ctx->dn, ns, ctx->nb, ctx->n_pairs, ctx->bscene, ctx->scene_b0, ctx->diag,
+                         ctx->pair_b, ctx->offT);
+    toc(ctx);
+    CHK(dbg(ctx, "launch_direct_factor"));
+    tic(ctx, "direct_solve");
+    launch_direct_solve(ctx->st, ctx->dn, ns, ctx->scene_b0, ctx->grad, ctx->du, ctx->bnorm, ctx->C.residual,
+                        ctx->cg_iters, fallback ? 0 : 1);
+    toc(ctx);
+    CHK(dbg(ctx, "launch_direct_solve"));
+    std::vector<int> bad(ns);
+    HIPCHK(hipMemcpyAsync(bad.data(), ctx->dn_fail, ns * sizeof(int), hipMemcpyDeviceToHost, ctx->st));
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    for (int s = 0; s < ns; ++s)
+        if (bad[s]) {
+            ctx->err = "direct solver: the system of scene " + std::to_string(s) + " is not positive definite";
+            return ABD_ERR_STATE;
+        }
+    return ABD_OK;
+}
+
 int solve_finish(AbdCtx* ctx, bool warm) {
     if (warm) {
         launch_add_inplace(ctx->st, 12 * ctx->nb, ctx->du, ctx->x0);
@@ -609,6 +723,7 @@ int solve_finish(AbdCtx* ctx, bool warm) {
         hipLaunchKernelGGL(k_copy, dim3(cdiv(ctx->ns, 64)), dim3(64), 0, ctx->st, ctx->ns, ctx->nrm_b, ctx->C.residual);
         CHK(dbg(ctx, "warm start finish"));
     }
+    if (ctx->P.solver == 2) CHK(solve_direct(ctx, true));
     if (ctx->log_on) {
         launch_body_features(ctx->st, ctx->G, ctx->q, ctx->qtilde, ctx->qdot, ctx->n_inc, ctx->inc_s, ctx->act_g,
                              ctx->n_act, ctx->pinv, ctx->grad, ctx->feat_raw);
@@ -768,6 +883,10 @@ int run_pcg(AbdCtx* ctx, const PcgBufT<T>& B, bool warm) {
 }
 
 int solve(AbdCtx* ctx) {
+    if (ctx->P.solver == 1) {
+        CHK(solve_direct(ctx, false));
+        return solve_finish(ctx, false);
+    }
     const bool warm = ctx->mlp != nullptr;
     const double* b = ctx->grad;
     const double* bref = nullptr;
@@ -1081,10 +1200,7 @@ int abd_create(const AbdSceneDesc* d, const AbdParams* p, AbdCtx** out) {
         double l = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
         if (l > 0) { n[0] /= l; n[1] /= l; n[2] /= l; }
     }
-    if (p->precision != 0 && p->precision != 1) {
-        ctx->err = "precision must be 0 (FP64) or 1 (mixed)";
-        return fail(ABD_ERR_ARG);
-    }
+    if (check_params(ctx, p) != ABD_OK) return fail(ABD_ERR_ARG);
     ctx->nb = d->n_bodies; ctx->nv = d->n_verts; ctx->ne = d->n_edges; ctx->nt = d->n_tris; ctx->ns = d->n_scenes;
     const int nb = ctx->nb, nv = ctx->nv, ne = ctx->ne, nt = ctx->nt, ns = ctx->ns;
     if (nb <= 0 || nv <= 0 || ns <= 0 || ns > 65535 || nb >= (1 << 20)) {
@@ -1346,10 +1462,7 @@ int abd_get_params(const AbdCtx* ctx, AbdParams* out) {
 
 int abd_set_params(AbdCtx* ctx, const AbdParams* p) {
     if (!ctx || !p) return ABD_ERR_ARG;
-    if (p->precision != 0 && p->precision != 1) {
-        ctx->err = "precision must be 0 (FP64) or 1 (mixed)";
-        return ABD_ERR_ARG;
-    }
+    CHK(check_params(ctx, p));
     ctx->P = *p;
     double* n = ctx->P.ground_normal;
     double l = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);

@@ -222,6 +222,24 @@ void launch_ls_decide(hipStream_t s, int ns, const AbdParams& P, SceneCtl c, int
 void launch_apply(hipStream_t s, const Geo& G, double* q, const double* du, const double* alpha, const int* active);
 void launch_forward(hipStream_t s, int nb, double dt, const double* q, double* qnow, double* qdot);
 void launch_newton_check(hipStream_t s, int ns, const AbdParams& P, SceneCtl c);
+// direct solver (abd_direct.hip): per-scene dense Cholesky of the assembled system
+constexpr int DIRECT_TILE = 32;        // tile edge; a scene's matrix order is padded to a multiple
+constexpr int DIRECT_MAX_DOFS = 6144;  // largest padded order (the solve keeps the vector in LDS)
+struct DirectBuf {
+    double* A = nullptr;            // dense lower-triangular factors of every directly solved scene
+    double* Lkk = nullptr;          // [ns][DIRECT_TILE^2] factored diagonal tile of the current column
+    const long long* off = nullptr; // [ns] start of scene s in A
+    const int* N = nullptr;         // [ns] padded order, 0 = the scene is not solved directly
+    int* mask = nullptr;            // [ns] scenes of the current solve
+    int* fail = nullptr;            // [ns] a pivot was not positive
+    int n_max = 0;                  // largest N
+};
+void launch_direct_mask(hipStream_t st, int ns, int fallback, int cg_max_iter, const int* N, const int* active,
+                        const int* cg_iters, int* mask, int* any);
+void launch_direct_factor(hipStream_t st, const DirectBuf& D, int ns, int nb, int n_pairs, const int* bscene,
+                          const int* scene_b0, const double* diag, const int* pair_b, const double* offT);
+void launch_direct_solve(hipStream_t st, const DirectBuf& D, int ns, const int* scene_b0, const double* b,
+                         double* du, double* bnorm, double* residual, int* cg_iters, int zero_iters);
 void launch_contact_points(hipStream_t s, const Geo& G, int n, const uint64_t* keys, const double4* X,
                            const AbdParams& P, double* mid, double* dist);
 

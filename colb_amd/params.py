@@ -35,6 +35,8 @@ class Params:
     ls_floor: float = 1e-2
     pinned_diag: float = 100.0
     max_candidates: int = 1 << 20
+    solver: int = 0  # 0 = PCG, 1 = direct (dense Cholesky per scene), 2 = PCG + direct fallback
+    direct_max_dofs: int = 6144  # largest scene (12 x bodies) the direct solve takes
     debug: int = 0  # bit 0: synchronise and check after every launch; bit 1: also log them;
     #                 bit 2: print broad-phase list sizes
 
@@ -61,6 +63,8 @@ class Params:
         p.pinned_diag = self.pinned_diag
         p.max_candidates = self.max_candidates
         p.reserved0 = self.debug
+        p.solver = self.solver
+        p.direct_max_dofs = self.direct_max_dofs
         return p
 
     def replace(self, **kw):

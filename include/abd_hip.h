@@ -62,6 +62,12 @@ typedef struct AbdParams {
     double pinned_diag;        /* Hessian diagonal of pinned bodies (abd.py:146, 100) */
     int    max_candidates;     /* capacity of the candidate-pair list (grown on demand) */
     int    reserved0;
+    int    solver;             /* 0 = block-Jacobi PCG (abd.py:633-636, "cg"); 1 = direct: dense
+                                  Cholesky per scene (the sparse direct branch, abd.py:637-654);
+                                  2 = PCG, then the direct solve for every scene whose PCG ended
+                                  at cg_max_iter */
+    int    direct_max_dofs;    /* largest scene (12 x bodies) the direct solve takes, <= 6144;
+                                  <= 0: 6144.  solver = 1 rejects a larger scene */
 } AbdParams;
 
 typedef struct AbdSceneDesc {

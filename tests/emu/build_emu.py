@@ -17,7 +17,7 @@ def build(asan=True):
     os.makedirs(OUT, exist_ok=True)
     flags = FLAGS if asan else [f for f in FLAGS if f != "-fsanitize=address"]
     objs = []
-    for src in ("abd_kernels.hip", "abd_broad.hip", "abd_sort.hip", "abd_mlp.hip", "abd_api.hip"):
+    for src in ("abd_kernels.hip", "abd_broad.hip", "abd_sort.hip", "abd_mlp.hip", "abd_direct.hip", "abd_api.hip"):
         obj = os.path.join(OUT, src.replace(".hip", ".o"))
         r = subprocess.run(["g++", *flags, "-x", "c++", "-c", os.path.join(CSRC, src), "-o", obj],
                            capture_output=True, text=True)

@@ -110,6 +110,23 @@ inline T __shfl_xor(T v, int mask, int width = 64) {
     return r;
 }
 
+// read lane `src` (0..63) of the caller's wave
+template <class T>
+inline T __shfl(T v, int src, int width = 64) {
+    (void)width;
+    static_assert(sizeof(T) <= sizeof(double), "shfl of wide type");
+    double w = 0;
+    std::memcpy(&w, &v, sizeof(T));
+    unsigned lane = threadIdx.x;
+    emu_sched->xbuf[lane] = w;
+    emu_wave_sync();
+    double o = emu_sched->xbuf[(lane & ~63u) + ((unsigned)src & 63u)];
+    emu_wave_sync();
+    T r;
+    std::memcpy(&r, &o, sizeof(T));
+    return r;
+}
+
 // wave vote over the lanes of the caller's 64-lane wave (every lane of the wave must call it)
 inline int emu_any(int pred) {
     unsigned lane = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);

@@ -431,3 +431,90 @@ def test_learned_preconditioner_trains_and_keeps_the_solution():
         f"learned {runs['learned'][:2]}; training rows {len(F)}")
     assert rel(runs["learned"][2], runs["block Jacobi"][2]) < 1e-6
     assert runs["learned"][1] == runs["block Jacobi"][1]
+
+
+# --- direct solver (AbdParams.solver 1 and 2) -------------------------------------------------------
+def _batch_of_sizes():
+    """Three scenes of 8, 9 and 1 bodies: padded orders 96, 128 and 32 in one set of launches."""
+    return S.concat_scenes([S.cubes_on_ground(n=2, lift=0.02), S.chain_net(n=3), S.cubes_on_ground(n=1, lift=0.02)])
+
+
+DIRECT_CASES = dict(CASES, batch=lambda: (_batch_of_sizes(), north_star(ground_enabled=True, dhat=0.05,
+                                                                        k_contact=1e14, cg_tol=1e-12)))
+
+
+@pytest.mark.parametrize("case", ["two_cubes", "cubes_ground", "chain3", "ref_chain", "batch"])
+def test_direct_solver_matches_dense_solve(O, case):
+    """solver = 1: the device's dense Cholesky per scene against numpy's LU solve of the same
+    assembled system (both FP64: 1e-9 relative, residual 1e-12) and, for one scene, against the
+    oracle's PCG solved to 1e-12 (1e-7, the PCG tolerance the other solve tests use)."""
+    say("case", case)
+    sc, p = DIRECT_CASES[case]()
+    q, qd = perturbed(sc, 2, amp=1e-4)
+    g = solver(sc, p.replace(solver=1))
+    g.set_state(q, sc.q0, qd)
+    grad, D, pij, off, E = g.assemble()
+    H = _dense_H(D, pij, off)
+    b = grad.reshape(-1)
+    res = np.atleast_1d(g.compute_du())
+    du = g._du.reshape(-1)
+    x = np.linalg.solve(H, b)
+    resid = np.linalg.norm(H @ du - b) / np.linalg.norm(b)
+    say("  n", len(b), "vs numpy", rel(du, x), "residual", resid, "cg", g.last_cg_iters)
+    assert rel(du, x) < 1e-9
+    assert resid < 1e-12
+    assert np.all(g.last_cg_iters == 0)
+    b0 = np.searchsorted(sc.body_scene, np.arange(sc.n_scenes + 1))
+    for s in range(sc.n_scenes):
+        assert res[s] == pytest.approx(np.linalg.norm(b[12 * b0[s]:12 * b0[s + 1]]), rel=1e-12)
+    if sc.n_scenes == 1:
+        orc = O.Oracle(sc, p)
+        orc.set_state(q, sc.q0, qd)
+        duo, ito, reso = orc.solve()
+        assert rel(du, duo) < 1e-7
+
+
+@pytest.mark.parametrize("case,nsteps", [("cubes_ground", 25), ("chain3", 5)])
+def test_direct_trajectory_matches_oracle(O, case, nsteps):
+    say("case", case)
+    sc, p = CASES[case]()
+    p = p.replace(newton_tol=1e-9, cg_tol=1e-12, cg_max_iter=2000)
+    orc = O.Oracle(sc, p)
+    gpu = solver(sc, p.replace(solver=1))
+    for k in range(nsteps):
+        so = orc.step()[0]
+        sg = gpu.step()[0]
+        err = rel(gpu.get_state()[0], orc.get_state()[0])
+        say(f"  step {k} newton {so['newton_iters']}/{sg['newton_iters']} err {err:.2e}")
+        assert err < 1e-6
+        assert sg["cg_iters"] == 0
+
+
+def test_direct_fallback_replaces_capped_pcg():
+    """solver = 2 with a PCG cap of 3: every solve ends at the cap, so every du comes from the direct
+    solve and the run equals solver = 1 bit for bit; without the cap the PCG run stays within 1e-6."""
+    sc, p = CASES["cubes_ground"]()
+    p = p.replace(newton_tol=1e-9)
+    a = solver(sc, p.replace(solver=1))
+    b = solver(sc, p.replace(solver=2, cg_max_iter=3))
+    c = solver(sc, p.replace(solver=2))
+    for k in range(5):
+        sa, sb, s_c = a.step()[0], b.step()[0], c.step()[0]
+        say(f"  step {k} newton {sa['newton_iters']}/{sb['newton_iters']}/{s_c['newton_iters']} "
+            f"cg {sa['cg_iters']}/{sb['cg_iters']}/{s_c['cg_iters']}")
+        assert sb["cg_iters"] == 3 * sb["newton_iters"]
+        assert sa["cg_iters"] == 0 and s_c["cg_iters"] > 3 * s_c["newton_iters"]
+    np.testing.assert_array_equal(a.get_state()[0], b.get_state()[0])
+    assert rel(c.get_state()[0], a.get_state()[0]) < 1e-6
+
+
+def test_direct_solver_rejects_what_it_cannot_hold():
+    from colb_amd._lib import AbdError
+    sc, p = CASES["two_cubes"]()
+    g = solver(sc, p.replace(solver=1, direct_max_dofs=12))
+    with pytest.raises(AbdError, match="direct_max_dofs"):
+        g.step()
+    with pytest.raises(AbdError, match="solver must be"):
+        solver(sc, p.replace(solver=3))
+    with pytest.raises(AbdError, match="direct_max_dofs above"):
+        solver(sc, p.replace(solver=1, direct_max_dofs=1 << 20))
