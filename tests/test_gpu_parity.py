@@ -471,7 +471,7 @@ def test_direct_solver_matches_dense_solve(O, case):
         orc = O.Oracle(sc, p)
         orc.set_state(q, sc.q0, qd)
         duo, ito, reso = orc.solve()
-        assert rel(du, duo) < 1e-7
+        assert rel(du, np.reshape(duo, -1)) < 1e-7
 
 
 @pytest.mark.parametrize("case,nsteps", [("cubes_ground", 25), ("chain3", 5)])
@@ -491,21 +491,24 @@ def test_direct_trajectory_matches_oracle(O, case, nsteps):
 
 
 def test_direct_fallback_replaces_capped_pcg():
-    """solver = 2 with a PCG cap of 3: every solve ends at the cap, so every du comes from the direct
-    solve and the run equals solver = 1 bit for bit; without the cap the PCG run stays within 1e-6."""
+    """solver = 2 with a PCG cap of 3: each solve that ends at the cap is redone by the direct solve,
+    so the run follows solver = 1 (1e-6, the trajectory tolerance: the solves the PCG finishes within
+    3 iterations are converged to 1e-12), while the same cap without the fallback leaves inexact
+    Newton steps that drift at least 100 times further."""
     sc, p = CASES["cubes_ground"]()
     p = p.replace(newton_tol=1e-9)
     a = solver(sc, p.replace(solver=1))
     b = solver(sc, p.replace(solver=2, cg_max_iter=3))
-    c = solver(sc, p.replace(solver=2))
+    d = solver(sc, p.replace(solver=0, cg_max_iter=3))
     for k in range(5):
-        sa, sb, s_c = a.step()[0], b.step()[0], c.step()[0]
-        say(f"  step {k} newton {sa['newton_iters']}/{sb['newton_iters']}/{s_c['newton_iters']} "
-            f"cg {sa['cg_iters']}/{sb['cg_iters']}/{s_c['cg_iters']}")
-        assert sb["cg_iters"] == 3 * sb["newton_iters"]
-        assert sa["cg_iters"] == 0 and s_c["cg_iters"] > 3 * s_c["newton_iters"]
-    np.testing.assert_array_equal(a.get_state()[0], b.get_state()[0])
-    assert rel(c.get_state()[0], a.get_state()[0]) < 1e-6
+        sa, sb, sd = a.step()[0], b.step()[0], d.step()[0]
+        eb = rel(b.get_state()[0], a.get_state()[0])
+        ed = rel(d.get_state()[0], a.get_state()[0])
+        say(f"  step {k} newton {sa['newton_iters']}/{sb['newton_iters']}/{sd['newton_iters']} "
+            f"cg {sa['cg_iters']}/{sb['cg_iters']}/{sd['cg_iters']} err fallback {eb:.2e} capped {ed:.2e}")
+        assert sa["cg_iters"] == 0 and 0 < sb["cg_iters"] <= 3 * sb["newton_iters"]
+        assert eb < 1e-6
+    assert ed > 100 * eb
 
 
 def test_direct_solver_rejects_what_it_cannot_hold():
