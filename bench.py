@@ -18,6 +18,7 @@ sys.path.insert(0, ROOT)
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_FLOPS = 78.6e12  # MI355X dense FP64
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (each calibrated by a
 # 512 MiB device copy in the same run), per workload and precision
 TRAFFIC_FILES = {("chain_net_32x32", "fp64"): "profiles/chain_net_32x32_plain_fp64_hbm_bytes.json"}
@@ -71,6 +72,14 @@ def workload(name, precision="fp64"):
         return sc, p, {"workload": "hull_piles_256", "scenes": 256, "bodies": int(sc.n_bodies), "seed": 3,
                        "bodies_per_scene": "log-uniform in [4, 512]", "dt_ms": 4.0, "dhat": 1e-3, "kappa": 1e7,
                        "k_contact": 1e20, "ground": True, "precision": precision}
+    if name == "ref_chain_144":
+        from colb_amd.params import reference_compat
+        here = os.path.dirname(os.path.abspath(__file__))
+        sc = S.load_json_scene(os.path.join(here, "tests", "golden", "ref_assets", "chains.json"))
+        p = reference_compat(precision=prec)
+        return sc, p, {"workload": "ref_chain_144", "links": int(sc.n_bodies), "scene": "the reference's chains.json",
+                       "dt_ms": 8.0, "dhat": p.dhat, "kappa": p.kappa, "k_contact": p.k_contact,
+                       "params": "reference_compat", "precision": precision}
     if name == "cubes_8":
         sc = S.cubes_on_ground()
         p = north_star(dt=8e-3, ground_enabled=True, dhat=1e-3, kappa=1e7, k_contact=1e20, precision=prec)
@@ -103,7 +112,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="chain_net_32x32", choices=["chain_net_32x32", "box_pile_4096", "hull_piles_256", "cubes_8"])
+    ap.add_argument("--workload", default="chain_net_32x32", choices=["chain_net_32x32", "box_pile_4096", "hull_piles_256", "ref_chain_144",
+                                                                   "cubes_8"])
+    ap.add_argument("--solver", default="pcg", choices=["pcg", "direct", "fallback"],
+                    help="linear solve: block-Jacobi PCG, dense Cholesky per scene, or PCG with the direct "
+                         "solve for scenes whose PCG ends at its iteration cap")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"],
                     help="mixed: FP32 PCG, FP64 near-contact distances, CCD and energies")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
@@ -131,6 +144,8 @@ def main():
         torch.cuda.synchronize()
         del a, b
     sc, p, cfg = workload(args.workload, args.precision)
+    p = p.replace(solver={"pcg": 0, "direct": 1, "fallback": 2}[args.solver])
+    cfg["solver"] = args.solver
     print(f"# workload {cfg['workload']}: {sc.n_bodies} bodies, {sc.n_verts} verts, {sc.n_tris} tris, "
           f"{sc.n_edges} edges", file=sys.stderr, flush=True)
     solver = NewtonAbd(p.dt, sc, params=p)
@@ -159,7 +174,7 @@ def main():
         solver.set_pcg_layout(parts[0], bool(parts[1]) if len(parts) > 1 else False)
         tune = {"bodies_per_tile": parts[0], "per_iteration_launch": bool(parts[1]) if len(parts) > 1 else False,
                 "fixed": True}
-    elif not args.no_autotune:
+    elif not args.no_autotune and args.solver != "direct":
         # choose PCG tile size and launch form at the warmed-up state (outside the timed region)
         st_keep = solver.get_state()
         bpt, multi, ms = solver.autotune(repeats=3)
@@ -203,6 +218,18 @@ def main():
         roof = {"bound": "hbm", "kernel": dom, "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(ach / HBM_PEAK_GBS, 5), "traffic": traffic, "avg_us": round(avg_ms * 1e3, 3),
                 "bytes_per_launch": bytes_per, "traffic_source": src}
+    direct = None
+    if "direct_factor" in kt:
+        # dense Cholesky flops per factorization: sum over scenes of N^3 / 3 (N = 12 x bodies padded to 32)
+        nbs = np.bincount(sc.body_scene, minlength=sc.n_scenes)
+        N = (12 * nbs + 31) // 32 * 32
+        flops = float(np.sum(N.astype(np.float64) ** 3) / 3.0)
+        f, s_ = kt["direct_factor"], kt.get("direct_solve", {"ms": 0.0, "calls": 1})
+        fus = f["ms"] * 1e3 / f["calls"]
+        direct = {"factor_us": round(fus, 2), "solve_us": round(s_["ms"] * 1e3 / s_["calls"], 2),
+                  "factorizations": f["calls"], "order": [int(N.min()), int(N.max())],
+                  "gflops": round(flops / (fus * 1e-6) / 1e9, 2),
+                  "frac_fp64_peak": round(flops / (fus * 1e-6) / FP64_PEAK_FLOPS, 5)}
     pcg_ms = sum(kt[k]["ms"] for k in ("pcg_solve", "pcg_iter") if k in kt)
     pcg_us_per_iter = round(pcg_ms * 1e3 / prof_cg, 3) if prof_cg else None
     for k, v in kt.items():
@@ -223,7 +250,7 @@ def main():
         "data": "synthetic (procedural scene)", "config": cfg,
         "newton_iters_per_step": round(float(np.mean(iters)), 3), "cg_iters_per_step": round(float(np.mean(cg)), 1),
         "method": args.method, "learned": learned, "autotune": tune, "pcg_us_per_iteration": pcg_us_per_iter,
-        "roofline": roof, "cpu_baseline": cpu,
+        "roofline": roof, "direct": direct, "cpu_baseline": cpu,
     }
     print(json.dumps(result), flush=True)
 

@@ -66,32 +66,11 @@ __global__ void k_dense_scatter(int nb, int n_pairs, const int* __restrict__ bsc
     A[off[s] + i * n + j] = v;
 }
 
-// unblocked Cholesky of the lower triangle of a DN_T x DN_T tile held in LDS, by one wave;
-// a pivot that is not positive sets *fail and is replaced by 1 (the result is then discarded)
-__device__ void tile_cholesky(double (*L)[DN_T + 1], int* fail) {
-    const int lane = threadIdx.x;
-    for (int j = 0; j < DN_T; ++j) {
-        __syncthreads();
-        double d = L[j][j];
-        if (!(d > 0.0)) {
-            if (lane == 0) atomicOr(fail, 1);
-            d = 1.0;
-        }
-        const double sq = sqrt(d), inv = 1.0 / sq;
-        if (lane > j && lane < DN_T) L[lane][j] *= inv;
-        __syncthreads();
-        if (lane == 0) L[j][j] = sq;
-        for (int idx = lane; idx < DN_T * DN_T; idx += 64) {
-            const int i = idx / DN_T, m = idx % DN_T;
-            if (m > j && m <= i) L[i][m] -= L[i][j] * L[m][j];
-        }
-    }
-    __syncthreads();
-}
-
-// column tile k: every workgroup factors the diagonal tile A_kk (bit-identical copies), workgroup 0
-// keeps L_kk in Lkk[s] (written into A by the following update launch: A_kk is still being read
-// here), and each thread solves one row i of the panel below: L_ik = A_ik L_kk^-T
+// column tile k: every workgroup (one wave) factors the diagonal tile A_kk (bit-identical copies):
+// the tile sits in LDS, lane i updates row i after reading column j into registers.  A pivot that is
+// not positive sets fail[s] and is replaced by 1 (the result is then discarded).  Workgroup 0 keeps
+// L_kk in Lkk[s] (the following update launch writes it into A: A_kk is still being read here), and
+// each thread solves one row i of the panel below: L_ik = A_ik L_kk^-T
 __global__ __launch_bounds__(64) void k_chol_panel(int k, const long long* __restrict__ off,
                                                    const int* __restrict__ N, const int* __restrict__ mask,
                                                    double* A, double* Lkk, int* fail) {
@@ -101,20 +80,46 @@ __global__ __launch_bounds__(64) void k_chol_panel(int k, const long long* __res
     __shared__ double L[DN_T][DN_T + 1];
     double* As = A + off[s];
     const int c0 = k * DN_T;
-    for (int idx = threadIdx.x; idx < DN_T * DN_T; idx += 64) {
-        const int i = idx / DN_T, m = idx % DN_T;
-        L[i][m] = (m <= i) ? As[(long)(c0 + i) * n + c0 + m] : 0.0;
+    const int lane = threadIdx.x, li = lane & (DN_T - 1);  // lanes 32..63 mirror lanes 0..31
+    for (int idx = lane; idx < DN_T * DN_T; idx += 64) {
+        const int r = idx / DN_T, m = idx % DN_T;
+        L[r][m] = (m <= r) ? As[(long)(c0 + r) * n + c0 + m] : 0.0;
     }
-    tile_cholesky(L, fail + s);
+    bool bad = false;
+    const bool own = lane < DN_T;
+#pragma unroll
+    for (int j = 0; j < DN_T; ++j) {
+        __syncthreads();
+        double d = L[j][j];
+        if (!(d > 0.0)) {
+            bad = true;
+            d = 1.0;
+        }
+        const double sq = sqrt(d);
+        if (own && li > j) L[li][j] /= sq;
+        __syncthreads();
+        if (own && li == j) L[j][j] = sq;
+        // row li -= L[li][j] * column j: the column first (other lanes' rows), then the own row
+        double cj[DN_T];
+#pragma unroll
+        for (int m = j + 1; m < DN_T; ++m) cj[m] = L[m][j];
+        const double lij = L[li][j];
+#pragma unroll
+        for (int m = j + 1; m < DN_T; ++m)
+            if (own && m <= li) L[li][m] -= lij * cj[m];
+    }
+    __syncthreads();
+    if (bad && lane == 0) atomicOr(fail + s, 1);
     if (blockIdx.x == 0)
-        for (int idx = threadIdx.x; idx < DN_T * DN_T; idx += 64)
+        for (int idx = lane; idx < DN_T * DN_T; idx += 64)
             Lkk[(size_t)s * DN_T * DN_T + idx] = L[idx / DN_T][idx % DN_T];
-    const long i = (long)c0 + DN_T + (long)blockIdx.x * 64 + threadIdx.x;
+    const long i = (long)c0 + DN_T + (long)blockIdx.x * 64 + lane;
     if (i >= n) return;
     double* row = As + i * n + c0;
     double x[DN_T];
 #pragma unroll
     for (int c = 0; c < DN_T; ++c) x[c] = row[c];
+    // x L_kk^T = a by forward substitution along the row
 #pragma unroll
     for (int c = 0; c < DN_T; ++c) {
         double t = x[c];
@@ -208,6 +213,7 @@ __global__ __launch_bounds__(256) void k_chol_solve(const long long* __restrict_
         const int r = tid / 8, sl = tid % 8;
         const double* row = As + ((long)kt * DN_T + r) * n;
         double acc = 0.0;
+#pragma unroll 8
         for (int c = sl; c < kt * DN_T; c += 8) acc += row[c] * y[c];
         acc += __shfl_xor(acc, 1, 64);
         acc += __shfl_xor(acc, 2, 64);
@@ -217,10 +223,15 @@ __global__ __launch_bounds__(256) void k_chol_solve(const long long* __restrict_
         if (wave == 0) {
             const int rr = lane & (DN_T - 1);
             const double* Lr = As + ((long)kt * DN_T + rr) * n + (long)kt * DN_T;
+            double lrow[DN_T];
+#pragma unroll
+            for (int m = 0; m < DN_T; ++m) lrow[m] = Lr[m];  // row rr of the diagonal tile
+            const double dg = Lr[rr];
             double t = y[kt * DN_T + rr] - red[0][rr];
+#pragma unroll
             for (int m = 0; m < DN_T; ++m) {
-                const double ym = shfl_d(t / Lr[rr], m);  // lane m holds row m: its quotient is y_m
-                if (rr > m) t -= Lr[m] * ym;
+                const double ym = shfl_d(t / dg, m);  // lane m holds row m: its quotient is y_m
+                if (rr > m) t -= lrow[m] * ym;
                 if (rr == m) t = ym;
             }
             if (lane < DN_T) y[kt * DN_T + rr] = t;
@@ -231,6 +242,7 @@ __global__ __launch_bounds__(256) void k_chol_solve(const long long* __restrict_
     for (int kt = T - 1; kt >= 0; --kt) {
         const int r = tid % DN_T, sl = tid / DN_T;
         double acc = 0.0;
+#pragma unroll 8
         for (int c = (kt + 1) * DN_T + sl; c < n; c += 8) acc += As[(long)c * n + (long)kt * DN_T + r] * y[c];
         red[sl][r] = acc;
         __syncthreads();
@@ -238,11 +250,16 @@ __global__ __launch_bounds__(256) void k_chol_solve(const long long* __restrict_
             const int rr = lane & (DN_T - 1);
             double t = y[kt * DN_T + rr];
             for (int q = 0; q < 8; ++q) t -= red[q][rr];
-            // lane rr keeps column rr of the diagonal tile: Lc[m] = L[m][rr]
+            // lane rr keeps column rr of the diagonal tile: lcol[m] = L[m][rr]
             const double* Lc = As + (long)kt * DN_T * n + (long)kt * DN_T + rr;
+            double lcol[DN_T];
+#pragma unroll
+            for (int m = 0; m < DN_T; ++m) lcol[m] = Lc[(long)m * n];
+            const double dg = Lc[(long)rr * n];
+#pragma unroll
             for (int m = DN_T - 1; m >= 0; --m) {
-                const double xm = shfl_d(t / Lc[(long)m * n], m);  // valid on lane m: L[m][m]
-                if (rr < m) t -= Lc[(long)m * n] * xm;
+                const double xm = shfl_d(t / dg, m);  // valid on lane m
+                if (rr < m) t -= lcol[m] * xm;
                 if (rr == m) t = xm;
             }
             if (lane < DN_T) y[kt * DN_T + rr] = t;
