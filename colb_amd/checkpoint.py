@@ -5,7 +5,8 @@ triple that ``forward_states`` (abd.py:574-581) rolls forward: ``q`` (the Newton
 BDFAffine.nxt), ``q_now`` (the last accepted state) and ``qdot``.  A step of this solver depends
 on nothing else that outlives a step except the parameters and the PCG layout (the tile size sets
 the order of the PCG's partial sums), so a file holding those, the frame number and a fingerprint
-of the scene resumes a run bit for bit.
+of the scene resumes a run bit for bit.  A learned warm start or preconditioner is not part of the
+file: a run that uses one resumes bit for bit once the same model is set again.
 
 Format: one ``.npz`` (numpy, no pickled objects), keys
 
