@@ -95,18 +95,22 @@ __global__ __launch_bounds__(64) void k_chol_panel(int k, const long long* __res
             bad = true;
             d = 1.0;
         }
-        const double sq = sqrt(d);
-        if (own && li > j) L[li][j] /= sq;
+        const double sq = sqrt(d), inv = 1.0 / sq;
+        if (own && li > j) L[li][j] *= inv;
         __syncthreads();
-        if (own && li == j) L[j][j] = sq;
-        // row li -= L[li][j] * column j: the column first (other lanes' rows), then the own row
-        double cj[DN_T];
+        if (own) {
+            if (li == j) L[j][j] = sq;
+            // row li -= L[li][j] * column j: the column first (other lanes' rows), then the own row.
+            // The update runs over the whole row without a per-entry test: entries right of the
+            // diagonal (and rows li <= j, whose L[li][j] is not a factor entry) take values that
+            // are never read, since every read below is of the lower triangle.
+            double cj[DN_T];
 #pragma unroll
-        for (int m = j + 1; m < DN_T; ++m) cj[m] = L[m][j];
-        const double lij = L[li][j];
+            for (int m = j + 1; m < DN_T; ++m) cj[m] = L[m][j];
+            const double lij = L[li][j];
 #pragma unroll
-        for (int m = j + 1; m < DN_T; ++m)
-            if (own && m <= li) L[li][m] -= lij * cj[m];
+            for (int m = j + 1; m < DN_T; ++m) L[li][m] -= lij * cj[m];
+        }
     }
     __syncthreads();
     if (bad && lane == 0) atomicOr(fail + s, 1);
@@ -212,9 +216,12 @@ __global__ __launch_bounds__(256) void k_chol_solve(const long long* __restrict_
     for (int kt = 0; kt < T; ++kt) {
         const int r = tid / 8, sl = tid % 8;
         const double* row = As + ((long)kt * DN_T + r) * n;
-        double acc = 0.0;
-#pragma unroll 8
-        for (int c = sl; c < kt * DN_T; c += 8) acc += row[c] * y[c];
+        double a4[4] = {0.0, 0.0, 0.0, 0.0};  // four chains: kt * DN_T is a multiple of 32
+        for (int c = sl; c < kt * DN_T; c += 32) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) a4[q] += row[c + 8 * q] * y[c + 8 * q];
+        }
+        double acc = (a4[0] + a4[1]) + (a4[2] + a4[3]);
         acc += __shfl_xor(acc, 1, 64);
         acc += __shfl_xor(acc, 2, 64);
         acc += __shfl_xor(acc, 4, 64);
@@ -226,11 +233,11 @@ __global__ __launch_bounds__(256) void k_chol_solve(const long long* __restrict_
             double lrow[DN_T];
 #pragma unroll
             for (int m = 0; m < DN_T; ++m) lrow[m] = Lr[m];  // row rr of the diagonal tile
-            const double dg = Lr[rr];
+            const double rdg = 1.0 / Lr[rr];
             double t = y[kt * DN_T + rr] - red[0][rr];
 #pragma unroll
             for (int m = 0; m < DN_T; ++m) {
-                const double ym = shfl_d(t / dg, m);  // lane m holds row m: its quotient is y_m
+                const double ym = shfl_d(t * rdg, m);  // lane m holds row m: its quotient is y_m
                 if (rr > m) t -= lrow[m] * ym;
                 if (rr == m) t = ym;
             }
@@ -241,10 +248,12 @@ __global__ __launch_bounds__(256) void k_chol_solve(const long long* __restrict_
     // back substitution with L^T, last tile first
     for (int kt = T - 1; kt >= 0; --kt) {
         const int r = tid % DN_T, sl = tid / DN_T;
-        double acc = 0.0;
-#pragma unroll 8
-        for (int c = (kt + 1) * DN_T + sl; c < n; c += 8) acc += As[(long)c * n + (long)kt * DN_T + r] * y[c];
-        red[sl][r] = acc;
+        double a4[4] = {0.0, 0.0, 0.0, 0.0};  // four chains: n - (kt + 1) * DN_T is a multiple of 32
+        for (int c = (kt + 1) * DN_T + sl; c < n; c += 32) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) a4[q] += As[(long)(c + 8 * q) * n + (long)kt * DN_T + r] * y[c + 8 * q];
+        }
+        red[sl][r] = (a4[0] + a4[1]) + (a4[2] + a4[3]);
         __syncthreads();
         if (wave == 0) {
             const int rr = lane & (DN_T - 1);
@@ -255,10 +264,10 @@ __global__ __launch_bounds__(256) void k_chol_solve(const long long* __restrict_
             double lcol[DN_T];
 #pragma unroll
             for (int m = 0; m < DN_T; ++m) lcol[m] = Lc[(long)m * n];
-            const double dg = Lc[(long)rr * n];
+            const double rdg = 1.0 / Lc[(long)rr * n];
 #pragma unroll
             for (int m = DN_T - 1; m >= 0; --m) {
-                const double xm = shfl_d(t / dg, m);  // valid on lane m
+                const double xm = shfl_d(t * rdg, m);  // valid on lane m
                 if (rr < m) t -= lcol[m] * xm;
                 if (rr == m) t = xm;
             }
