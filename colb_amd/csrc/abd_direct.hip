@@ -98,18 +98,25 @@ __global__ __launch_bounds__(64) void k_chol_panel(int k, const long long* __res
         const double sq = sqrt(d), inv = 1.0 / sq;
         if (own && li > j) L[li][j] *= inv;
         __syncthreads();
-        if (own) {
-            if (li == j) L[j][j] = sq;
-            // row li -= L[li][j] * column j: the column first (other lanes' rows), then the own row.
-            // The update runs over the whole row without a per-entry test: entries right of the
-            // diagonal (and rows li <= j, whose L[li][j] is not a factor entry) take values that
-            // are never read, since every read below is of the lower triangle.
-            double cj[DN_T];
+        if (own && li == j) L[j][j] = sq;
+        // row li -= L[li][j] * column j, entries j+1+h, j+3+h, ... by lane half h (lanes li and
+        // li + 32 share row li): the column first (other lanes' rows), then the own row.  The
+        // update runs over the whole row without a per-entry test: entries right of the diagonal
+        // (and rows li <= j, whose L[li][j] is not a factor entry) take values that are never
+        // read, since every read below is of the lower triangle.
+        constexpr int half = 32;
+        const int h = lane / half;
+        const double lij = L[li][j];
+        double cj[(DN_T + 1) / 2];
 #pragma unroll
-            for (int m = j + 1; m < DN_T; ++m) cj[m] = L[m][j];
-            const double lij = L[li][j];
+        for (int p = 0; p < (DN_T - j) / 2; ++p) {
+            const int m = j + 1 + 2 * p + h;
+            cj[p] = m < DN_T ? L[m][j] : 0.0;
+        }
 #pragma unroll
-            for (int m = j + 1; m < DN_T; ++m) L[li][m] -= lij * cj[m];
+        for (int p = 0; p < (DN_T - j) / 2; ++p) {
+            const int m = j + 1 + 2 * p + h;
+            if (m < DN_T) L[li][m] -= lij * cj[p];
         }
     }
     __syncthreads();
@@ -217,6 +224,7 @@ __global__ __launch_bounds__(256) void k_chol_solve(const long long* __restrict_
         const int r = tid / 8, sl = tid % 8;
         const double* row = As + ((long)kt * DN_T + r) * n;
         double a4[4] = {0.0, 0.0, 0.0, 0.0};  // four chains: kt * DN_T is a multiple of 32
+#pragma unroll 4
         for (int c = sl; c < kt * DN_T; c += 32) {
 #pragma unroll
             for (int q = 0; q < 4; ++q) a4[q] += row[c + 8 * q] * y[c + 8 * q];
@@ -249,6 +257,7 @@ __global__ __launch_bounds__(256) void k_chol_solve(const long long* __restrict_
     for (int kt = T - 1; kt >= 0; --kt) {
         const int r = tid % DN_T, sl = tid / DN_T;
         double a4[4] = {0.0, 0.0, 0.0, 0.0};  // four chains: n - (kt + 1) * DN_T is a multiple of 32
+#pragma unroll 4
         for (int c = (kt + 1) * DN_T + sl; c < n; c += 32) {
 #pragma unroll
             for (int q = 0; q < 4; ++q) a4[q] += As[(long)(c + 8 * q) * n + (long)kt * DN_T + r] * y[c + 8 * q];
