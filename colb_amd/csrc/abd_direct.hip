@@ -98,25 +98,18 @@ __global__ __launch_bounds__(64) void k_chol_panel(int k, const long long* __res
         const double sq = sqrt(d), inv = 1.0 / sq;
         if (own && li > j) L[li][j] *= inv;
         __syncthreads();
-        if (own && li == j) L[j][j] = sq;
-        // row li -= L[li][j] * column j, entries j+1+h, j+3+h, ... by lane half h (lanes li and
-        // li + 32 share row li): the column first (other lanes' rows), then the own row.  The
-        // update runs over the whole row without a per-entry test: entries right of the diagonal
-        // (and rows li <= j, whose L[li][j] is not a factor entry) take values that are never
-        // read, since every read below is of the lower triangle.
-        constexpr int half = 32;
-        const int h = lane / half;
-        const double lij = L[li][j];
-        double cj[(DN_T + 1) / 2];
+        if (own) {
+            if (li == j) L[j][j] = sq;
+            // row li -= L[li][j] * column j: the column first (other lanes' rows), then the own row.
+            // The update runs over the whole row without a per-entry test: entries right of the
+            // diagonal (and rows li <= j, whose L[li][j] is not a factor entry) take values that
+            // are never read, since every read below is of the lower triangle.
+            double cj[DN_T];
 #pragma unroll
-        for (int p = 0; p < (DN_T - j) / 2; ++p) {
-            const int m = j + 1 + 2 * p + h;
-            cj[p] = m < DN_T ? L[m][j] : 0.0;
-        }
+            for (int m = j + 1; m < DN_T; ++m) cj[m] = L[m][j];
+            const double lij = L[li][j];
 #pragma unroll
-        for (int p = 0; p < (DN_T - j) / 2; ++p) {
-            const int m = j + 1 + 2 * p + h;
-            if (m < DN_T) L[li][m] -= lij * cj[p];
+            for (int m = j + 1; m < DN_T; ++m) L[li][m] -= lij * cj[m];
         }
     }
     __syncthreads();
