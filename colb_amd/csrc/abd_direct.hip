@@ -87,29 +87,56 @@ __global__ __launch_bounds__(64) void k_chol_panel(int k, const long long* __res
     }
     bool bad = false;
     const bool own = lane < DN_T;
+    // two pivots per round: columns j and j + 1 are factored together (every lane computes the
+    // 2 x 2 leading block from three LDS reads), then each row takes a rank-2 update
 #pragma unroll
-    for (int j = 0; j < DN_T; ++j) {
+    for (int j = 0; j < DN_T; j += 2) {
         __syncthreads();
-        double d = L[j][j];
-        if (!(d > 0.0)) {
+        double d0 = L[j][j];
+        if (!(d0 > 0.0)) {
             bad = true;
-            d = 1.0;
+            d0 = 1.0;
         }
-        const double sq = sqrt(d), inv = 1.0 / sq;
-        if (own && li > j) L[li][j] *= inv;
+        const double s0 = sqrt(d0), i0 = 1.0 / s0;
+        const double l10 = L[j + 1][j] * i0;
+        double d1 = L[j + 1][j + 1] - l10 * l10;
+        if (!(d1 > 0.0)) {
+            bad = true;
+            d1 = 1.0;
+        }
+        const double s1 = sqrt(d1), i1 = 1.0 / s1;
+        // rows below the block: their entries in columns j, j + 1; x0 = x1 = 0 on the other rows
+        double x0 = 0.0, x1 = 0.0;
+        if (li > j + 1) {
+            x0 = L[li][j] * i0;
+            x1 = (L[li][j + 1] - x0 * l10) * i1;
+        }
         __syncthreads();
         if (own) {
-            if (li == j) L[j][j] = sq;
-            // row li -= L[li][j] * column j: the column first (other lanes' rows), then the own row.
-            // The update runs over the whole row without a per-entry test: entries right of the
-            // diagonal (and rows li <= j, whose L[li][j] is not a factor entry) take values that
-            // are never read, since every read below is of the lower triangle.
-            double cj[DN_T];
+            if (li > j + 1) {
+                L[li][j] = x0;
+                L[li][j + 1] = x1;
+            } else if (li == j + 1) {
+                L[j + 1][j] = l10;
+                L[j + 1][j + 1] = s1;
+            } else if (li == j) {
+                L[j][j] = s0;
+            }
+        }
+        __syncthreads();
+        if (own && j + 2 < DN_T) {
+            // row li -= x0 * column j + x1 * column j + 1: the columns first (other lanes' rows),
+            // then the own row.  The update runs over the whole row without a per-entry test:
+            // entries right of the diagonal take values that are never read (every read below is
+            // of the lower triangle), and rows li <= j + 1 have x0 = x1 = 0.
+            double c0v[DN_T], c1v[DN_T];
 #pragma unroll
-            for (int m = j + 1; m < DN_T; ++m) cj[m] = L[m][j];
-            const double lij = L[li][j];
+            for (int m = j + 2; m < DN_T; ++m) {
+                c0v[m] = L[m][j];
+                c1v[m] = L[m][j + 1];
+            }
 #pragma unroll
-            for (int m = j + 1; m < DN_T; ++m) L[li][m] -= lij * cj[m];
+            for (int m = j + 2; m < DN_T; ++m) L[li][m] -= x0 * c0v[m] + x1 * c1v[m];
         }
     }
     __syncthreads();
