@@ -252,7 +252,8 @@ def main():
         "vs_baseline": (round(sim_per_wall / REF_PUBLISHED[args.workload], 3) if args.workload in REF_PUBLISHED
                         else None),
         "dtype": "f64" if args.precision == "fp64" else "f32+f64",
-        "data": "synthetic (procedural scene)", "config": cfg,
+        "data": ("the reference's own scene (chains.json and its meshes)" if args.workload == "ref_chain_144"
+                 else "synthetic (procedural scene)"), "config": cfg,
         "newton_iters_per_step": round(float(np.mean(iters)), 3), "cg_iters_per_step": round(float(np.mean(cg)), 1),
         "method": args.method, "learned": learned, "autotune": tune, "pcg_us_per_iteration": pcg_us_per_iter,
         "roofline": roof, "direct": direct, "cpu_baseline": cpu,
