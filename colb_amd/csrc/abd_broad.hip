@@ -153,28 +153,44 @@ __device__ __forceinline__ Aff load_aff(const double* q, const double* du, int b
     return f;
 }
 
-// World AABB of a rest-frame box under an affine state, widened by a relative 1e-12 of the
-// magnitudes involved (far above the rounding of x = c + A xr in FP64) and rounded outward.
-__device__ __forceinline__ void map_box(const Aff& f, const float* rlo, const float* rhi, float* out) {
-    double ctr[3], half[3], mag[3];
+// World AABB of a rest-frame box under an affine state x = c + A xr, in FP32, widened and rounded
+// outward so that it contains the exact image.  Its rounding error is below
+// 11 u * sc (u = 2^-24, sc = |c| + sum_k |A_ik| max(|lo_k|, |hi_k|)): the FP32 copies of c and A
+// (u each), the box centre and half-width (u), and the two 3-term FMA chains (4u and 3u); the
+// slack 4e-6 * sc is six times that, and the final subtraction is covered by the outward ulp step.
+struct Aff32 {
+    float c[3], A[3][3];
+};
+__device__ __forceinline__ Aff32 aff32(const Aff& f) {
+    Aff32 g;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        g.c[i] = (float)f.c[i];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) g.A[i][k] = (float)f.A[i][k];
+    }
+    return g;
+}
+__device__ __forceinline__ void map_box32(const Aff32& f, const float* rlo, const float* rhi, float* out) {
+    float ctr[3], half[3], mag[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        ctr[k] = 0.5 * ((double)rlo[k] + (double)rhi[k]);
-        half[k] = 0.5 * ((double)rhi[k] - (double)rlo[k]);
-        mag[k] = fmax(fabs((double)rlo[k]), fabs((double)rhi[k]));
+        ctr[k] = 0.5f * (rlo[k] + rhi[k]);
+        half[k] = 0.5f * (rhi[k] - rlo[k]);
+        mag[k] = fmaxf(fabsf(rlo[k]), fabsf(rhi[k]));
     }
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
-        double cw = f.c[i], hw = 0.0, sc = fabs(f.c[i]);
+        float cw = f.c[i], hw = 0.0f, sc = fabsf(f.c[i]);
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            cw += f.A[i][k] * ctr[k];
-            hw += fabs(f.A[i][k]) * half[k];
-            sc += fabs(f.A[i][k]) * mag[k];
+            cw = fmaf(f.A[i][k], ctr[k], cw);
+            hw = fmaf(fabsf(f.A[i][k]), half[k], hw);
+            sc = fmaf(fabsf(f.A[i][k]), mag[k], sc);
         }
-        const double slack = 1e-12 * sc + 1e-280;
-        out[i] = flo(cw - hw - slack);
-        out[3 + i] = fhi(cw + hw + slack);
+        const float slack = 4e-6f * sc + 1e-30f;
+        out[i] = nextafterf(cw - hw - slack, -FLT_MAX);
+        out[3 + i] = nextafterf(cw + hw + slack, FLT_MAX);
     }
 }
 
@@ -199,15 +215,15 @@ __global__ __launch_bounds__(256) void k_world_nodes(Geo G, BodyMesh M, const do
     }
     const BNode* __restrict__ R = (tri ? M.tri_nodes : M.edge_nodes) + r0;
     BNode* __restrict__ W = (tri ? M.tri_world : M.edge_world) + w0;
-    const Aff s0 = load_aff(q, nullptr, b);
-    const Aff s1 = du ? load_aff(q, du, b) : s0;
+    const Aff32 s0 = aff32(load_aff(q, nullptr, b));
+    const Aff32 s1 = du ? aff32(load_aff(q, du, b)) : s0;
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
         const BNode r = R[i];
         float w[6];
-        map_box(s0, r.lo, r.hi, w);
+        map_box32(s0, r.lo, r.hi, w);
         if (du) {
             float w1[6];
-            map_box(s1, r.lo, r.hi, w1);
+            map_box32(s1, r.lo, r.hi, w1);
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
                 w[k] = fminf(w[k], w1[k]);
