@@ -521,3 +521,48 @@ def test_direct_solver_rejects_what_it_cannot_hold():
         solver(sc, p.replace(solver=3))
     with pytest.raises(AbdError, match="direct_max_dofs above"):
         solver(sc, p.replace(solver=1, direct_max_dofs=1 << 20))
+
+
+# --- line search: the accepted step against a brute-force search on the oracle's energy -----------
+@pytest.mark.parametrize("case,amp", [("two_cubes", 3e-3), ("cubes_ground", 3e-3), ("chain3", 1e-3),
+                                      ("ref_chain", 3e-2)])
+def test_line_search_alpha_matches_brute_force(O, case, amp):
+    """line_search_iterative (gauss_newton.py:98-115) with the CCD bound as the first trial: from
+    the device's own du and CCD bound alpha0, halve alpha until the oracle's energy (pinned by
+    finite differences and closed forms) falls below E(q), or until 0.5 alpha < ls_floor alpha0.
+    The device must accept exactly that alpha (both are alpha0 / 2^k) and apply q - alpha du.
+    Decisions closer than 1e-9 relative between E(q - alpha du) and E(q) are ties the two
+    summation orders may split: comparison stops at the first such iteration (a tie at the first
+    skips the case)."""
+    say("case", case)
+    sc, p = CASES[case]()
+    q0, qd = perturbed(sc, 3, amp=amp)
+    orc = O.Oracle(sc, p)
+    orc.set_state(q0, sc.q0, qd)
+    g = solver(sc, p)
+    g.set_state(q0, sc.q0, qd)
+    halvings = 0
+    for it in range(4):
+        q = g.get_state()[0].copy()
+        g.compute_du()
+        du = g._du.copy()
+        a0 = float(g.ccd(du)[0]) if p.ccd_enabled else 1.0
+        E0 = orc.energy(q)[0]
+        a, margins = a0, []
+        while True:
+            E1 = orc.energy(q - a * du)[0]
+            margins.append(abs(E1 - E0) / max(abs(E0), 1e-300))
+            if E1 < E0 or 0.5 * a < p.ls_floor * a0:
+                break
+            a *= 0.5
+        if min(margins) < 1e-9:
+            if it == 0:
+                pytest.skip(f"energy tie at the first iteration (margin {min(margins):.1e})")
+            say(f"  iteration {it}: energy tie (margin {min(margins):.1e}), stop comparing")
+            break
+        alpha = g.line_search()
+        say(f"  iteration {it}: alpha0 {a0:.6g} device {alpha:.6g} brute force {a:.6g} trials {len(margins)}")
+        assert alpha == a
+        halvings += len(margins) - 1
+        assert rel(g.get_state()[0], q - alpha * du) < 1e-14
+    say("  halvings", halvings)
