@@ -800,16 +800,21 @@ __device__ bool grid_barrier(unsigned* bar, int* fail, unsigned k, unsigned nblo
         __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         const unsigned target = k * nblocks;
         int spins = 0;
+        // one agent-scope load per spin; the failure flag (set by a workgroup whose wait ran out)
+        // is read every 32 spins, so a failed solve still ends every workgroup within bounded
+        // time, while a barrier that fills costs no extra round trip.  A workgroup that passes a
+        // barrier another one gave up on stops at the next barrier, which cannot fill.
         while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-            if (__hip_atomic_load(fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) { ok = 0; break; }
-            if (++spins > PCG_SPIN_LIMIT) {
-                __hip_atomic_store(fail, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                ok = 0;
-                break;
+            if ((++spins & 31) == 0) {
+                if (__hip_atomic_load(fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) { ok = 0; break; }
+                if (spins > PCG_SPIN_LIMIT) {
+                    __hip_atomic_store(fail, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    ok = 0;
+                    break;
+                }
             }
             __builtin_amdgcn_s_sleep(1);
         }
-        if (ok && __hip_atomic_load(fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) ok = 0;
         ok_sh = ok;
     }
     __syncthreads();
