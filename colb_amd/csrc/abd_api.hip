@@ -239,8 +239,15 @@ __global__ void k_fill_u64(int n, unsigned long long* p, unsigned long long v) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
 }
-__global__ void k_nact_scene(int ns, const int* lo, const int* hi, const int* flag, const int* pos, int* out) {
+// active contacts per scene from the flags and their exclusive scan; thread 0 also stores the
+// scan's last entry and last flag (their sum is the active count) for the host's counter read
+__global__ void k_nact_scene(int ns, const int* lo, const int* hi, const int* flag, const int* pos, int* out, int nc,
+                             int* cnt_pos, int* cnt_flag) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s == 0 && nc > 0) {
+        *cnt_pos = pos[nc - 1];
+        *cnt_flag = flag[nc - 1];
+    }
     if (s >= ns) return;
     int a = lo[s], b = hi[s];
     out[s] = (b > a) ? (pos[b - 1] + flag[b - 1] - pos[a]) : 0;
@@ -497,12 +504,8 @@ int assemble(AbdCtx* ctx) {
     CHK(dbg(ctx, "launch_cand_eval"));
     toc(ctx);
     CHK(excl_scan(ctx, ctx->flag, ctx->pos, nc));
-    if (nc > 0) {
-        HIPCHK(hipMemcpyAsync(ctx->cnt + C_NACT, ctx->pos + nc - 1, sizeof(int), hipMemcpyDeviceToDevice, ctx->st));
-        HIPCHK(hipMemcpyAsync(ctx->cnt + C_ANY, ctx->flag + nc - 1, sizeof(int), hipMemcpyDeviceToDevice, ctx->st));
-    }
     hipLaunchKernelGGL(k_nact_scene, dim3(cdiv(ctx->ns, 128)), dim3(128), 0, ctx->st, ctx->ns, ctx->cand_lo,
-                       ctx->cand_hi, ctx->flag, ctx->pos, ctx->nact_s);
+                       ctx->cand_hi, ctx->flag, ctx->pos, ctx->nact_s, nc, ctx->cnt + C_NACT, ctx->cnt + C_ANY);
     CHK(dbg(ctx, "k_nact_scene"));
     CHK(sync_counters(ctx));
     int na = nc > 0 ? ctx->h_cnt[C_NACT] + ctx->h_cnt[C_ANY] : 0;
@@ -1012,8 +1015,7 @@ int step_once(AbdCtx* ctx) {
         CHK(ccd_bound(ctx));
         CHK(line_search(ctx));
         // keep the pre-check flags for the report, then stop converged scenes (abd.py:699)
-        HIPCHK(hipMemcpyAsync(ctx->act_before, ctx->C.newton_active, sizeof(int) * ns, hipMemcpyDeviceToDevice, ctx->st));
-        launch_newton_check(ctx->st, ns, ctx->P, ctx->C);
+        launch_newton_check(ctx->st, ns, ctx->P, ctx->C, ctx->act_before);
         CHK(dbg(ctx, "launch_newton_check"));
         hipLaunchKernelGGL(k_report, dim3(cdiv(ns, 128)), dim3(128), 0, ctx->st, ns, ctx->C, ctx->cg_iters,
                            ctx->nact_s, ctx->cand_lo, ctx->cand_hi, ctx->act_before, ctx->report);

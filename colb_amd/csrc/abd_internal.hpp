@@ -221,7 +221,7 @@ void launch_ls_init(hipStream_t s, int ns, const AbdParams& P, const int* newton
 void launch_ls_decide(hipStream_t s, int ns, const AbdParams& P, SceneCtl c, int* pending);
 void launch_apply(hipStream_t s, const Geo& G, double* q, const double* du, const double* alpha, const int* active);
 void launch_forward(hipStream_t s, int nb, double dt, const double* q, double* qnow, double* qdot);
-void launch_newton_check(hipStream_t s, int ns, const AbdParams& P, SceneCtl c);
+void launch_newton_check(hipStream_t s, int ns, const AbdParams& P, SceneCtl c, int* active_before);
 // direct solver (abd_direct.hip): per-scene dense Cholesky of the assembled system
 constexpr int DIRECT_TILE = 32;        // tile edge; a scene's matrix order is padded to a multiple
 constexpr int DIRECT_MAX_DOFS = 6144;  // largest padded order (the solve keeps the vector in LDS)

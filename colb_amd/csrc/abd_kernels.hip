@@ -1545,14 +1545,17 @@ void launch_forward(hipStream_t s, int nb, double dt, const double* q, double* q
 }
 
 // abd.py:699: stop once the residual of the pre-update iterate is below tol
-__global__ void k_newton_check(int ns, AbdParams P, SceneCtl c) {
+// stop converged scenes (abd.py:699); the flags before the check go to active_before
+__global__ void k_newton_check(int ns, AbdParams P, SceneCtl c, int* active_before) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= ns) return;
-    if (c.newton_active[s] && c.residual[s] < P.newton_tol) c.newton_active[s] = 0;
+    const int a = c.newton_active[s];
+    active_before[s] = a;
+    if (a && c.residual[s] < P.newton_tol) c.newton_active[s] = 0;
 }
 
-void launch_newton_check(hipStream_t s, int ns, const AbdParams& P, SceneCtl c) {
-    hipLaunchKernelGGL(k_newton_check, dim3(cdiv(ns, 128)), dim3(128), 0, s, ns, P, c);
+void launch_newton_check(hipStream_t s, int ns, const AbdParams& P, SceneCtl c, int* active_before) {
+    hipLaunchKernelGGL(k_newton_check, dim3(cdiv(ns, 128)), dim3(128), 0, s, ns, P, c, active_before);
 }
 
 // get_contact_points (abd.py:759-766): midpoint of the closest points and the distance
