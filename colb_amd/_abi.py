@@ -40,6 +40,10 @@ class AbdParams(C.Structure):
         ("reserved0", C.c_int),
         ("solver", C.c_int),
         ("direct_max_dofs", C.c_int),
+        ("predictor", C.c_int),
+        ("hess_proj", C.c_int),
+        ("k_max", C.c_double),
+        ("k_raise_gap", C.c_double),
     ]
 
 
@@ -73,11 +77,14 @@ class AbdStepStats(C.Structure):
         ("alpha_min", C.c_double),
         ("energy", C.c_double),
         ("ccd_alpha_min", C.c_double),
-        ("reserved", C.c_double),
+        ("k_contact", C.c_double),
+        ("k_raises", C.c_int),
+        ("pcg_fallbacks", C.c_int),
+        ("predictor_alpha", C.c_double),
     ]
 
     def as_dict(self):
-        return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
+        return {name: getattr(self, name) for name, _ in self._fields_}
 
 
 def dptr(a):

@@ -24,12 +24,15 @@ SIGNATURES = {
     "abd_get_state": (C.c_int, [_V, _D, _D, _D]),
     "abd_step": (C.c_int, [_V, C.c_int, C.POINTER(AbdStepStats)]),
     "abd_get_iter_log": (C.c_int, [_V, _D, _D, _I]),
+    "abd_get_stiffness": (C.c_int, [_V, _D]),
+    "abd_set_stiffness": (C.c_int, [_V, _D]),
     "abd_detect": (C.c_int, [_V, _I, _D, C.c_int, _I]),
     "abd_assemble": (C.c_int, [_V, _D, _D, _I, _D, C.c_int, _I, _D]),
     "abd_solve": (C.c_int, [_V, _D, _D, _I]),
     "abd_energy": (C.c_int, [_V, _D, _D]),
     "abd_ccd": (C.c_int, [_V, _D, _D]),
     "abd_line_search": (C.c_int, [_V, _D]),
+    "abd_line_search_along": (C.c_int, [_V, _D, _D]),
     "abd_forward": (C.c_int, [_V]),
     "abd_get_vertices": (C.c_int, [_V, _D]),
     "abd_get_contact_points": (C.c_int, [_V, _D, _D, C.c_int, _I]),

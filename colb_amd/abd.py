@@ -18,6 +18,14 @@ from .profiler import Profiller
 from .scene import Scene, load_json_scene
 
 
+class _Positions(np.ndarray):
+    """World vertex positions (n_nodes, 3) float64.  `.numpy()` returns the plain array, as the Warp
+    array the reference hands out does (viewer.py:6 reads ``soup.x_transformed.numpy()``)."""
+
+    def numpy(self):
+        return np.asarray(self)
+
+
 class _Soup:
     """Read-only view mirroring geometry.Soup (geometry.py:22-28)."""
 
@@ -31,7 +39,7 @@ class _Soup:
 
     @property
     def x_transformed(self):
-        return self._s.compute_V(ret=True)
+        return self._s.compute_V(ret=True).copy().view(_Positions)
 
 
 class NewtonAbd:
@@ -195,6 +203,25 @@ class NewtonAbd:
         qq = None if q is None else np.ascontiguousarray(q, dtype=np.float64).reshape(-1)
         _lib.check(_lib.load().abd_energy(self._h, dptr(qq), dptr(E)), self._h, "abd_energy")
         return float(E[0]) if self.n_scenes == 1 else E
+
+    def line_search_along(self, du):
+        """The CCD-filtered line search along a given direction du (applies q -= alpha du)."""
+        a = np.zeros(self.n_scenes)
+        _lib.check(_lib.load().abd_line_search_along(self._h, dptr(np.ascontiguousarray(du, dtype=np.float64).reshape(-1)),
+                                                     dptr(a)), self._h, "abd_line_search_along")
+        return a
+
+    def stiffness(self):
+        """Current contact stiffness of every body (adaptive when params.k_max > k_contact)."""
+        k = np.zeros(self.n_bodies)
+        _lib.check(_lib.load().abd_get_stiffness(self._h, dptr(k)), self._h, "abd_get_stiffness")
+        return k
+
+    def set_stiffness(self, k):
+        k = np.ascontiguousarray(k, dtype=np.float64).reshape(-1)
+        if k.size != self.n_bodies:
+            raise ValueError("one stiffness per body")
+        _lib.check(_lib.load().abd_set_stiffness(self._h, dptr(k)), self._h, "abd_set_stiffness")
 
     def ccd(self, du):
         a = np.zeros(self.n_scenes)

@@ -4,8 +4,9 @@ The reference keeps its state in Warp arrays and has no file format for it; its 
 triple that ``forward_states`` (abd.py:574-581) rolls forward: ``q`` (the Newton iterate,
 BDFAffine.nxt), ``q_now`` (the last accepted state) and ``qdot``.  A step of this solver depends
 on nothing else that outlives a step except the parameters and the PCG layout (the tile size sets
-the order of the PCG's partial sums), so a file holding those, the frame number and a fingerprint
-of the scene resumes a run bit for bit.  A learned warm start or preconditioner is not part of the
+the order of the PCG's partial sums) and the bodies' contact stiffness (raised by the adaptive rule
+of ``Params.k_max``), so a file holding those, the frame number and a fingerprint of the scene
+resumes a run bit for bit.  A learned warm start or preconditioner is not part of the
 file: a run that uses one resumes bit for bit once the same model is set again.
 
 Format: one ``.npz`` (numpy, no pickled objects), keys
@@ -15,6 +16,8 @@ Format: one ``.npz`` (numpy, no pickled objects), keys
 ``frame``         ``int64`` frames stepped so far
 ``params``        JSON of :class:`colb_amd.params.Params` (tuples as lists)
 ``pcg_layout``    ``int32[2]``: bodies per tile, one launch per iteration (0/1)
+``stiffness``     ``float64[n_bodies]`` contact stiffness of each body (absent in older files:
+                  every body at ``params.k_contact``)
 ``scene_sha256``  SHA-256 over the scene's geometry, topology, masses, pins and scene ids
 """
 import hashlib
@@ -47,6 +50,7 @@ def save(solver, path):
         np.savez(f, format=np.array(FORMAT), q=q, q_now=qn, qdot=qd, frame=np.int64(solver.frame),
                  params=np.array(json.dumps(solver.params.as_dict())),
                  pcg_layout=np.array([bpt, int(bool(multi))], dtype=np.int32),
+                 stiffness=np.asarray(solver.stiffness(), dtype=np.float64),
                  scene_sha256=np.array(scene_fingerprint(solver.scene)))
 
 
@@ -62,7 +66,7 @@ def read(path):
             raw[k] = tuple(v)
     return {"q": d["q"], "q_now": d["q_now"], "qdot": d["qdot"], "frame": int(d["frame"]),
             "params": Params(**raw), "pcg_layout": (int(d["pcg_layout"][0]), bool(d["pcg_layout"][1])),
-            "scene_sha256": str(d["scene_sha256"])}
+            "stiffness": d.get("stiffness"), "scene_sha256": str(d["scene_sha256"])}
 
 
 def load(solver, path):
@@ -78,6 +82,10 @@ def load(solver, path):
     solver.set_params(d["params"])
     solver.set_pcg_layout(*d["pcg_layout"])
     solver.set_state(d["q"], d["q_now"], d["qdot"])
+    if d["stiffness"] is not None:
+        if d["stiffness"].shape != (nb,):
+            raise ValueError(f"{path}: stiffness has shape {d['stiffness'].shape}, expected {(nb,)}")
+        solver.set_stiffness(d["stiffness"])
     solver.frame = d["frame"]
     solver.compute_V()
     return d

@@ -115,6 +115,7 @@ struct AbdCtx {
     unsigned* pcg_ctl = nullptr;  // persistent solve: [0] barrier count, [1..3] done counters, [4] failure flag
     unsigned* h_pcg_ctl = nullptr;
     bool coop_ok = true;          // cleared when a cooperative launch fails for a reason other than its size
+    int pcg_fallbacks = 0;        // solves of the current step rerun with one launch per iteration
     int coop_blocks[2] = {-1, -1};  // co-resident k_pcg_persist workgroups (FP64, FP32); -1 = not queried
     bool pcg_multi = false;       // solve with one launch per iteration (autotuner choice)
     // mixed precision (AbdParams.precision == 1): FP32 copies of the PCG operands and vectors
@@ -164,7 +165,7 @@ namespace {
 
 enum { C_COUNT = 0, C_OVERFLOW = 1, C_NINC = 2, C_NPK = 3, C_NUNIQ = 4, C_ANY = 5, C_PENDING = 6, C_NACT = 7, C_ERR = 8,
        C_N = 9 };
-enum { R_RES = 0, R_ALPHA, R_ALPHA0, R_FLOOR, R_ACTIVE, R_CG, R_E1, R_NACT, R_NCAND, R_N };
+enum { R_RES = 0, R_ALPHA, R_ALPHA0, R_FLOOR, R_ACTIVE, R_CG, R_E1, R_NACT, R_NCAND, R_KC, R_RAISES, R_PALPHA, R_N };
 
 #define HIPCHK(expr)                                                                               \
     do {                                                                                           \
@@ -266,6 +267,9 @@ __global__ void k_report(int ns, SceneCtl c, const int* cg, const int* nact, con
     R[R_E1] = c.E1[s];
     R[R_NACT] = nact[s];
     R[R_NCAND] = hi[s] - lo[s];
+    R[R_KC] = c.kc[s];
+    R[R_RAISES] = c.k_raises[s];
+    R[R_PALPHA] = c.pred_alpha[s];
 }
 __global__ void k_copy(int n, const double* a, double* b) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -343,8 +347,11 @@ int sync_counters(AbdCtx* ctx) {
     HIPCHK(hipMemcpyAsync(ctx->h_cnt, ctx->cnt, C_N * sizeof(int), hipMemcpyDeviceToHost, ctx->st));
     HIPCHK(hipStreamSynchronize(ctx->st));
     if (ctx->h_cnt[C_ERR]) {
-        char buf[128];
-        std::snprintf(buf, sizeof buf, "device index check failed (error bits 0x%x)", ctx->h_cnt[C_ERR]);
+        char buf[160];
+        if (ctx->h_cnt[C_ERR] == ERR_PENETRATION)
+            std::snprintf(buf, sizeof buf, "a vertex is on or below the ground plane (infeasible state)");
+        else
+            std::snprintf(buf, sizeof buf, "device index check failed (error bits 0x%x)", ctx->h_cnt[C_ERR]);
         ctx->err = buf;
         return ABD_ERR_STATE;
     }
@@ -614,6 +621,35 @@ int learned_precond(AbdCtx* ctx) {
 
 // after the PCG: add the warm start, restore ||b|| as the residual, log training samples
 int check_params(AbdCtx* ctx, const AbdParams* p) {
+    auto bad = [&](const char* what) {
+        ctx->err = std::string("invalid parameter: ") + what;
+        return ABD_ERR_ARG;
+    };
+    auto finite = [](double v) { return std::isfinite(v); };
+    if (!(p->dt > 0.0) || !finite(p->dt)) return bad("dt must be positive");
+    if (!(p->dhat > 0.0) || !finite(p->dhat)) return bad("dhat must be positive");
+    if (!(p->kappa >= 0.0) || !finite(p->kappa)) return bad("kappa must be >= 0");
+    if (!(p->k_contact >= 0.0) || !finite(p->k_contact)) return bad("k_contact must be >= 0");
+    if (!(p->newton_tol >= 0.0) || !finite(p->newton_tol)) return bad("newton_tol must be >= 0");
+    if (p->max_newton < 1) return bad("max_newton must be >= 1");
+    if (p->cg_max_iter < 1) return bad("cg_max_iter must be >= 1");
+    if (!(p->cg_tol > 0.0) || !(p->cg_tol < 1.0)) return bad("cg_tol must be in (0, 1)");
+    if (!(p->ccd_gap_ratio > 0.0) || !(p->ccd_gap_ratio < 1.0)) return bad("ccd_gap_ratio must be in (0, 1)");
+    if (p->ccd_max_iter < 1) return bad("ccd_max_iter must be >= 1");
+    if (!(p->ls_floor > 0.0) || !(p->ls_floor < 1.0)) return bad("ls_floor must be in (0, 1)");
+    if (!(p->pinned_diag > 0.0) || !finite(p->pinned_diag)) return bad("pinned_diag must be positive");
+    for (int k = 0; k < 3; ++k)
+        if (!finite(p->gravity[k]) || !finite(p->ground_normal[k])) return bad("gravity and ground_normal must be finite");
+    if (!finite(p->ground_offset)) return bad("ground_offset must be finite");
+    if (p->ground_enabled &&
+        !(p->ground_normal[0] * p->ground_normal[0] + p->ground_normal[1] * p->ground_normal[1] +
+              p->ground_normal[2] * p->ground_normal[2] > 0.0))
+        return bad("ground_normal must be non-zero");
+    if (p->predictor < 0 || p->predictor > 3) return bad("predictor must be 0, 1, 2 or 3");
+    if (p->hess_proj != 0 && p->hess_proj != 1) return bad("hess_proj must be 0 or 1");
+    if (!finite(p->k_max)) return bad("k_max must be finite");
+    if (p->k_max > p->k_contact && (!(p->k_raise_gap > 0.0) || !(p->k_raise_gap < 1.0)))
+        return bad("k_raise_gap must be in (0, 1)");
     if (p->precision != 0 && p->precision != 1) {
         ctx->err = "precision must be 0 (FP64) or 1 (mixed)";
         return ABD_ERR_ARG;
@@ -728,8 +764,7 @@ int solve_finish(AbdCtx* ctx, bool warm) {
     }
     if (ctx->P.solver == 2) CHK(solve_direct(ctx, true));
     if (ctx->log_on) {
-        launch_body_features(ctx->st, ctx->G, ctx->q, ctx->qtilde, ctx->qdot, ctx->n_inc, ctx->inc_s, ctx->act_g,
-                             ctx->n_act, ctx->pinv, ctx->grad, ctx->feat_raw);
+        // features were computed from the block-Jacobi inverse before the solve (log_features)
         const int nb = ctx->nb;
         std::vector<float> f((size_t)ABD_N_FEATURES * nb);
         std::vector<double> du(12 * (size_t)nb);
@@ -777,6 +812,10 @@ int solve_persistent(AbdCtx* ctx, const PcgBufT<T>& B, bool* ok) {
     if (!ctx->coop_ok || ctx->pcg_multi || (ctx->P.reserved0 & 8) || !persist_fits<T>(ctx)) return ABD_OK;
     HIPCHK(hipMemsetAsync(ctx->cg_iters, 0, sizeof(int) * ctx->ns, ctx->st));
     HIPCHK(hipMemsetAsync(ctx->pcg_ctl, 0, 8 * sizeof(unsigned), ctx->st));
+    // debug bit 4: raise the failure flag before the launch, so every workgroup gives up at its first
+    // barrier that does not fill at once and the solve is redone with one launch per iteration
+    const bool forced = (ctx->P.reserved0 & 16) != 0;
+    if (forced) hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, ctx->st, 1, (int*)(ctx->pcg_ctl + 4), 1);
     if (sizeof(T) == 8) {  // the FP64 solve multiplies by M_e = B_e P_col (the FP32 one by P, then B)
         launch_pcg_mblocks(ctx->st, 2 * ctx->n_pairs, ctx->ent_ks, ctx->ent_vs, ctx->off, ctx->offT, ctx->pinv,
                            ctx->nb, ctx->n_pairs, B.mq, ctx->cnt + C_ERR);
@@ -789,6 +828,7 @@ int solve_persistent(AbdCtx* ctx, const PcgBufT<T>& B, bool* ok) {
     if (e != hipSuccess) {
         (void)hipGetLastError();
         ctx->coop_ok = false;
+        ctx->pcg_fallbacks += 1;
         return ABD_OK;
     }
     CHK(dbg(ctx, "launch_pcg_persist"));
@@ -806,7 +846,9 @@ int solve_persistent(AbdCtx* ctx, const PcgBufT<T>& B, bool* ok) {
     }
     HIPCHK(hipStreamSynchronize(ctx->st));
     if (ctx->h_pcg_ctl[4]) {
-        ctx->coop_ok = false;  // a grid barrier did not fill: do not try again on this context
+        // a grid barrier did not fill: do not try again on this context (unless forced for a test)
+        if (!forced) ctx->coop_ok = false;
+        ctx->pcg_fallbacks += 1;
         return ABD_OK;
     }
     *ok = true;
@@ -899,7 +941,13 @@ int solve(AbdCtx* ctx) {
         b = ctx->rhs;
         bref = ctx->nrm_b;
     }
-    if (ctx->pre_mlp) CHK(learned_precond(ctx));  // after the warm start's features, which use D^-1
+    if (ctx->log_on && !warm) {
+        // logged features use the block-Jacobi inverse, as at inference (the warm start computed them)
+        launch_body_features(ctx->st, ctx->G, ctx->q, ctx->qtilde, ctx->qdot, ctx->n_inc, ctx->inc_s, ctx->act_g,
+                             ctx->n_act, ctx->pinv, ctx->grad, ctx->feat_raw);
+        CHK(dbg(ctx, "launch_body_features"));
+    }
+    if (ctx->pre_mlp) CHK(learned_precond(ctx));  // after the features, which use D^-1
     if (ctx->P.precision == 1) {
         // mixed precision: the assembled FP64 system is rounded once to FP32 and solved in FP32
         // (partial sums and scalars stay FP64); the FP32 solution is widened into du
@@ -959,7 +1007,9 @@ int ccd_bound(AbdCtx* ctx) {
     return ABD_OK;
 }
 
-int line_search(AbdCtx* ctx) {
+// floor_accept: the Newton line search keeps the floor step without a decrease (the reference); the
+// predictor's search takes no step then
+int line_search(AbdCtx* ctx, int floor_accept = 1) {
     // E0 at q over the swept candidate list
     launch_body_energy(ctx->st, ctx->G, ctx->P, ctx->q, nullptr, nullptr, ctx->qtilde, ctx->e_body);
     CHK(dbg(ctx, "launch_body_energy"));
@@ -983,7 +1033,7 @@ int line_search(AbdCtx* ctx) {
         launch_scene_sum(ctx->st, ctx->G, ctx->qtmp, ctx->e_cand, ctx->cand_lo, ctx->cand_hi, ctx->C.E1);
         CHK(dbg(ctx, "launch_scene_sum"));
         HIPCHK(hipMemsetAsync(ctx->cnt + C_PENDING, 0, sizeof(int), ctx->st));
-        launch_ls_decide(ctx->st, ctx->ns, ctx->P, ctx->C, ctx->cnt + C_PENDING);
+        launch_ls_decide(ctx->st, ctx->ns, ctx->P, ctx->C, floor_accept, ctx->cnt + C_PENDING);
         CHK(dbg(ctx, "launch_ls_decide"));
         HIPCHK(hipGetLastError());
         CHK(sync_counters(ctx));
@@ -997,6 +1047,35 @@ int line_search(AbdCtx* ctx) {
     return ABD_OK;
 }
 
+// adaptive stiffness after a Newton iteration: closest contact of each scene at the accepted iterate
+// (Xt holds it: the last trial of the line search used every scene's accepted step)
+int adapt_stiffness(AbdCtx* ctx) {
+    if (!(ctx->P.k_max > ctx->P.k_contact)) return ABD_OK;
+    hipLaunchKernelGGL(k_fill_u64, dim3(cdiv(ctx->nb, 128)), dim3(128), 0, ctx->st, ctx->nb, ctx->C.dmin_bits,
+                       (unsigned long long)0x7FF0000000000000ull);  // +inf
+    launch_dmin(ctx->st, ctx->G, ctx->n_cand, ctx->cand_s, ctx->Xt, ctx->P, ctx->C.dmin_bits);
+    launch_stiffness_update(ctx->st, ctx->G, ctx->P, ctx->C.newton_active, ctx->C);
+    CHK(dbg(ctx, "adapt_stiffness"));
+    HIPCHK(hipGetLastError());
+    return ABD_OK;
+}
+
+// first iterate of the step from the predictor (AbdParams.predictor 1, 2): q moves from q_now toward
+// the prediction by the largest CCD-safe halving step that lowers the energy
+int predict(AbdCtx* ctx) {
+    if (ctx->P.predictor == 3) {
+        ctx->err = "predictor 3 (learned) is not available";
+        return ABD_ERR_ARG;
+    }
+    launch_predict_du(ctx->st, ctx->G, ctx->P.predictor, ctx->P.dt, ctx->q, ctx->qnow, ctx->qdot, ctx->qtilde, ctx->du);
+    CHK(dbg(ctx, "launch_predict_du"));
+    CHK(ccd_bound(ctx));
+    CHK(line_search(ctx, 0));
+    hipLaunchKernelGGL(k_copy, dim3(cdiv(ctx->ns, 64)), dim3(64), 0, ctx->st, ctx->ns, ctx->C.alpha, ctx->C.pred_alpha);
+    CHK(dbg(ctx, "predictor"));
+    return ABD_OK;
+}
+
 int step_once(AbdCtx* ctx) {
     const int ns = ctx->ns, mi = ctx->P.max_newton;
     launch_qtilde(ctx->st, ctx->nb, ctx->qnow, ctx->qdot, ctx->P.dt, ctx->P.gravity, ctx->qtilde);
@@ -1007,6 +1086,13 @@ int step_once(AbdCtx* ctx) {
     ctx->log_cg.assign((size_t)ns * mi, 0);
     ctx->stats.assign(ns, AbdStepStats{});
     for (auto& s : ctx->stats) { s.alpha_min = 1.0; s.ccd_alpha_min = 1.0; }
+    ctx->pcg_fallbacks = 0;
+    hipLaunchKernelGGL(k_fill_u64, dim3(cdiv(ctx->nb, 128)), dim3(128), 0, ctx->st, ctx->nb,
+                       (unsigned long long*)ctx->C.dmin_prev, (unsigned long long)0x7FF0000000000000ull);  // +inf
+    CHK(fill_i32(ctx, ctx->C.k_raises, ns, 0));
+    hipLaunchKernelGGL(k_fill_u64, dim3(cdiv(ns, 128)), dim3(128), 0, ctx->st, ns,
+                       (unsigned long long*)ctx->C.pred_alpha, 0ull);
+    if (ctx->P.predictor > 0) CHK(predict(ctx));
     std::vector<int> active(ns, 1);
     for (int it = 0; it < mi; ++it) {
         if (!ctx->cand_valid) CHK(static_broad_phase(ctx));
@@ -1014,9 +1100,11 @@ int step_once(AbdCtx* ctx) {
         CHK(solve(ctx));
         CHK(ccd_bound(ctx));
         CHK(line_search(ctx));
+        CHK(adapt_stiffness(ctx));
         // keep the pre-check flags for the report, then stop converged scenes (abd.py:699)
         launch_newton_check(ctx->st, ns, ctx->P, ctx->C, ctx->act_before);
         CHK(dbg(ctx, "launch_newton_check"));
+        launch_scene_kmax(ctx->st, ctx->G, ctx->C);
         hipLaunchKernelGGL(k_report, dim3(cdiv(ns, 128)), dim3(128), 0, ctx->st, ns, ctx->C, ctx->cg_iters,
                            ctx->nact_s, ctx->cand_lo, ctx->cand_hi, ctx->act_before, ctx->report);
         CHK(dbg(ctx, "k_report"));
@@ -1044,6 +1132,10 @@ int step_once(AbdCtx* ctx) {
                 S.energy = R[R_E1];
                 if (now[s] == 0) S.converged = 1;
             }
+            ctx->stats[s].k_contact = R[R_KC];
+            ctx->stats[s].k_raises = (int)R[R_RAISES];
+            ctx->stats[s].predictor_alpha = R[R_PALPHA];
+            ctx->stats[s].pcg_fallbacks = ctx->pcg_fallbacks;
             active[s] = now[s];
             any |= now[s] != 0;
         }
@@ -1056,6 +1148,15 @@ int step_once(AbdCtx* ctx) {
 }
 
 int check_ctx(AbdCtx* ctx) { return ctx ? ABD_OK : ABD_ERR_ARG; }
+
+// every body's contact stiffness back to k_contact
+int reset_stiffness(AbdCtx* ctx) {
+    std::vector<double> k(ctx->nb, ctx->P.k_contact);
+    HIPCHK(hipMemcpyAsync(ctx->C.kb, k.data(), ctx->nb * sizeof(double), hipMemcpyHostToDevice, ctx->st));
+    launch_scene_kmax(ctx->st, ctx->G, ctx->C);
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    return ABD_OK;
+}
 
 // --------------------------------------------------------------------------------------------
 // PCG tiles: runs of at most `bpt` consecutive bodies of one scene (one wave each, 12 lanes per
@@ -1409,6 +1510,7 @@ int abd_create(const AbdSceneDesc* d, const AbdParams* p, AbdCtx** out) {
     A(bnorm, ns); A(cg_iters, ns); A(cg_done, ns);
     A(C.newton_active, ns); A(C.residual, ns); A(C.alpha, ns); A(C.alpha0, ns); A(C.amax_bits, ns);
     A(C.ls_done, ns); A(C.ls_floor, ns); A(C.E0, ns); A(C.E1, ns);
+    A(C.kb, nb); A(C.dmin_prev, nb); A(C.dmin_bits, nb); A(C.k_raises, ns); A(C.kc, ns); A(C.pred_alpha, ns);
     A(nact_s, ns); A(act_before, ns); A(report, R_N * ns); A(cnt, C_N); A(row_ptr, nb + 1);
 #undef A
 #undef U
@@ -1427,7 +1529,8 @@ int abd_create(const AbdSceneDesc* d, const AbdParams* p, AbdCtx** out) {
         return fail(ABD_ERR_HIP);
     }
     ctx->G = Geo{nb, nv, ne, nt, ns, ctx->rest, ctx->edges, ctx->tris, ctx->mass4, ctx->pinned, ctx->bscene,
-                 ctx->scene_b0, ctx->scene_v0, ctx->scene_e0, ctx->scene_t0, ctx->cnt + C_ERR};
+                 ctx->scene_b0, ctx->scene_v0, ctx->scene_e0, ctx->scene_t0, ctx->cnt + C_ERR, ctx->C.kb};
+    if ((rc = reset_stiffness(ctx)) != ABD_OK) return fail(rc);
     if (hipMemset(ctx->cnt, 0, C_N * sizeof(int)) != hipSuccess) {
         ctx->err = "hipMemset failed";
         return fail(ABD_ERR_HIP);
@@ -1471,7 +1574,7 @@ int abd_set_params(AbdCtx* ctx, const AbdParams* p) {
     if (l > 0) { n[0] /= l; n[1] /= l; n[2] /= l; }
     ctx->cand_valid = false;
     ctx->assembled = ctx->solved = false;
-    return ABD_OK;
+    return reset_stiffness(ctx);
 }
 
 int abd_set_state(AbdCtx* ctx, const double* q, const double* q_now, const double* qdot) {
@@ -1515,6 +1618,30 @@ int abd_step(AbdCtx* ctx, int nsteps, AbdStepStats* stats) {
             ctx->err = "non-finite residual or energy";
             return ABD_ERR_NAN;
         }
+    return ABD_OK;
+}
+
+int abd_get_stiffness(const AbdCtx* cctx, double* k) {
+    AbdCtx* ctx = const_cast<AbdCtx*>(cctx);
+    CHK(check_ctx(ctx));
+    if (!k) return ABD_ERR_ARG;
+    HIPCHK(hipMemcpyAsync(k, ctx->C.kb, ctx->nb * sizeof(double), hipMemcpyDeviceToHost, ctx->st));
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    return ABD_OK;
+}
+
+int abd_set_stiffness(AbdCtx* ctx, const double* k) {
+    CHK(check_ctx(ctx));
+    if (!k) return ABD_ERR_ARG;
+    for (int b = 0; b < ctx->nb; ++b)
+        if (!(k[b] >= 0.0) || !std::isfinite(k[b])) {
+            ctx->err = "contact stiffness must be finite and >= 0";
+            return ABD_ERR_ARG;
+        }
+    HIPCHK(hipMemcpyAsync(ctx->C.kb, k, ctx->nb * sizeof(double), hipMemcpyHostToDevice, ctx->st));
+    launch_scene_kmax(ctx->st, ctx->G, ctx->C);
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    ctx->assembled = ctx->solved = false;
     return ABD_OK;
 }
 
@@ -1618,14 +1745,19 @@ int abd_energy(AbdCtx* ctx, const double* q, double* energy) {
     }
     int rc = static_broad_phase(ctx);
     if (rc == ABD_OK) {
+        // a failure below is recorded, and q is still restored before returning
         launch_cand_eval(ctx->st, ctx->G, ctx->n_cand, ctx->cand_s, ctx->X0, ctx->P, nullptr, ctx->e_cand);
-        CHK(dbg(ctx, "launch_cand_eval"));
-        launch_body_energy(ctx->st, ctx->G, ctx->P, ctx->q, nullptr, nullptr, ctx->qtilde, ctx->e_body);
-        CHK(dbg(ctx, "launch_body_energy"));
-        launch_scene_sum(ctx->st, ctx->G, ctx->e_body, ctx->e_cand, ctx->cand_lo, ctx->cand_hi, ctx->C.E0);
-        CHK(dbg(ctx, "launch_scene_sum"));
-        HIPCHK(hipMemcpyAsync(energy, ctx->C.E0, ctx->ns * sizeof(double), hipMemcpyDeviceToHost, ctx->st));
+        rc = dbg(ctx, "launch_cand_eval");
     }
+    if (rc == ABD_OK) {
+        launch_body_energy(ctx->st, ctx->G, ctx->P, ctx->q, nullptr, nullptr, ctx->qtilde, ctx->e_body);
+        rc = dbg(ctx, "launch_body_energy");
+    }
+    if (rc == ABD_OK) {
+        launch_scene_sum(ctx->st, ctx->G, ctx->e_body, ctx->e_cand, ctx->cand_lo, ctx->cand_hi, ctx->C.E0);
+        rc = dbg(ctx, "launch_scene_sum");
+    }
+    if (rc == ABD_OK) HIPCHK(hipMemcpyAsync(energy, ctx->C.E0, ctx->ns * sizeof(double), hipMemcpyDeviceToHost, ctx->st));
     if (q) {
         HIPCHK(hipMemcpyAsync(ctx->q, ctx->qtmp, 12 * ctx->nb * sizeof(double), hipMemcpyDeviceToDevice, ctx->st));
         ctx->cand_valid = false;
@@ -1657,6 +1789,19 @@ int abd_line_search(AbdCtx* ctx, double* alpha) {
         ctx->err = "abd_line_search before abd_solve";
         return ABD_ERR_STATE;
     }
+    CHK(ccd_bound(ctx));
+    CHK(line_search(ctx));
+    if (alpha) HIPCHK(hipMemcpyAsync(alpha, ctx->C.alpha, ctx->ns * sizeof(double), hipMemcpyDeviceToHost, ctx->st));
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    ctx->assembled = ctx->solved = false;
+    return ABD_OK;
+}
+
+int abd_line_search_along(AbdCtx* ctx, const double* du, double* alpha) {
+    CHK(check_ctx(ctx));
+    if (!du) return ABD_ERR_ARG;
+    HIPCHK(hipMemcpyAsync(ctx->du, du, 12 * ctx->nb * sizeof(double), hipMemcpyHostToDevice, ctx->st));
+    CHK(fill_i32(ctx, ctx->C.newton_active, ctx->ns, 1));
     CHK(ccd_bound(ctx));
     CHK(line_search(ctx));
     if (alpha) HIPCHK(hipMemcpyAsync(alpha, ctx->C.alpha, ctx->ns * sizeof(double), hipMemcpyDeviceToHost, ctx->st));

@@ -338,8 +338,9 @@ __device__ inline bool wave_is_psd(const double* A, double* L, int m, int lane) 
 // Jacobi PSD projection of the symmetric m x m matrix A (LDS, row-major, m <= 12), by one wave:
 // A <- V max(Lambda, 0) V^T (Golub & Van Loan 8.5, parallel ordering 8.5.3).  The projection is a
 // unique matrix function, so it agrees with the oracle's cyclic-order psd_project to rounding;
-// same stopping rule.  V is LDS scratch of m*m doubles.
-__device__ inline void wave_psd_project(double* A, double* V, int m, int lane) {
+// same stopping rule.  V is LDS scratch of m*m doubles.  absolute: A <- V |Lambda| V^T instead
+// (negative eigenvalues mirrored rather than clamped).
+__device__ inline void wave_psd_project(double* A, double* V, int m, int lane, bool absolute = false) {
     const int mm = m * m;
     for (int e = lane; e < mm; e += 64) V[e] = (e / m == e % m) ? 1.0 : 0.0;
     double f = 0;
@@ -441,7 +442,10 @@ __device__ inline void wave_psd_project(double* A, double* V, int m, int lane) {
     for (int e = lane; e < mm; e += 64) {
         int r = e / m, c = e % m;
         double h = 0;
-        for (int k = 0; k < m; ++k) h += Vc[r * m + k] * fmax(Ac[k * m + k], 0.0) * Vc[c * m + k];
+        for (int k = 0; k < m; ++k) {
+            const double lam = Ac[k * m + k];
+            h += Vc[r * m + k] * (absolute ? fabs(lam) : fmax(lam, 0.0)) * Vc[c * m + k];
+        }
         out[cnt++] = h;
     }
     __syncthreads();

@@ -65,10 +65,13 @@ struct Geo {
     const int* scene_e0;
     const int* scene_t0;
     int* err;             // device error word: bits set by failed index checks
+    const double* kb;     // [nb] contact stiffness of each body; a contact uses the larger of its two
+                          // bodies' (adaptive; k_contact everywhere when fixed)
 };
 
 // error bits of Geo::err
-enum : int { ERR_DECODE = 1, ERR_BVH = 2, ERR_INCIDENCE = 4, ERR_PAIR = 8, ERR_CSR = 16, ERR_BODY = 32 };
+enum : int { ERR_DECODE = 1, ERR_BVH = 2, ERR_INCIDENCE = 4, ERR_PAIR = 8, ERR_CSR = 16, ERR_BODY = 32,
+             ERR_PENETRATION = 64 };
 
 // Device-resident per-scene solver controls
 struct SceneCtl {
@@ -81,6 +84,12 @@ struct SceneCtl {
     int* ls_floor;       // 1 if the last line search ended at the floor
     double* E0;
     double* E1;
+    double* kb;          // [nb] body contact stiffness (Geo::kb points here)
+    double* dmin_prev;   // [nb] smallest contact distance of the body after the previous iteration
+    unsigned long long* dmin_bits;  // [nb] smallest d^2 at the accepted iterate (bit pattern, atomicMin)
+    int* k_raises;       // [ns] body stiffness doublings of the scene in the current step
+    double* kc;          // [ns] largest body stiffness of the scene (reported)
+    double* pred_alpha;  // [ns] step taken toward the predicted first iterate
 };
 
 // Buffers of the block-Jacobi PCG (abd_kernels.hip, k_pcg_cg); r, w, sv are ping-pong pairs.  T is
@@ -218,7 +227,13 @@ void launch_ccd(hipStream_t s, const Geo& G, int n, const uint64_t* keys, const 
 void launch_body_energy(hipStream_t s, const Geo& G, const AbdParams& P, const double* q, const double* du,
                         const double* alpha_scene, const double* qtilde, double* e_body);
 void launch_ls_init(hipStream_t s, int ns, const AbdParams& P, const int* newton_active, SceneCtl c);
-void launch_ls_decide(hipStream_t s, int ns, const AbdParams& P, SceneCtl c, int* pending);
+void launch_ls_decide(hipStream_t s, int ns, const AbdParams& P, SceneCtl c, int floor_accept, int* pending);
+void launch_predict_du(hipStream_t s, const Geo& G, int mode, double dt, const double* q, const double* qnow,
+                       const double* qdot, const double* qtilde, double* du);
+void launch_dmin(hipStream_t s, const Geo& G, int n, const uint64_t* keys, const double4* X, const AbdParams& P,
+                 unsigned long long* dmin_bits);
+void launch_stiffness_update(hipStream_t s, const Geo& G, const AbdParams& P, const int* active, SceneCtl c);
+void launch_scene_kmax(hipStream_t s, const Geo& G, SceneCtl c);
 void launch_apply(hipStream_t s, const Geo& G, double* q, const double* du, const double* alpha, const int* active);
 void launch_forward(hipStream_t s, int nb, double dt, const double* q, double* qnow, double* qdot);
 void launch_newton_check(hipStream_t s, int ns, const AbdParams& P, SceneCtl c, int* active_before);

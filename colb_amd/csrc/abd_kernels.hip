@@ -134,10 +134,25 @@ __device__ __forceinline__ D3 ld3(const double4* X, int v) {
     return mk(a.x, a.y, a.z);
 }
 
+// contact stiffness of a pair: the larger of its bodies' (the first and last stencil slot belong to
+// the two bodies; ground pairs have one)
+__device__ __forceinline__ int pair_body(const Geo& G, const PairV& p, int slot) {
+    return (int)G.rest[p.v[slot]].w;
+}
+__device__ __forceinline__ double pair_stiffness(const Geo& G, const PairV& p) {
+    const double ka = G.kb[pair_body(G, p, 0)];
+    return p.kind == K_PG ? ka : fmax(ka, G.kb[pair_body(G, p, 3)]);
+}
+
+// signed distance of vertex v above the ground plane n.x = h
+__device__ __forceinline__ double ground_gap(const double4* X, int v, const AbdParams& P) {
+    D3 x = ld3(X, v);
+    return P.ground_normal[0] * x.x + P.ground_normal[1] * x.y + P.ground_normal[2] * x.z - P.ground_offset;
+}
+
 __device__ __forceinline__ double cand_d2(const Geo& G, const PairV& p, const double4* X, const AbdParams& P) {
     if (p.kind == K_PG) {
-        D3 x = ld3(X, p.v[0]);
-        double d = P.ground_normal[0] * x.x + P.ground_normal[1] * x.y + P.ground_normal[2] * x.z - P.ground_offset;
+        const double d = ground_gap(X, p.v[0], P);
         return d * d;
     }
     D3 x[4];
@@ -162,7 +177,10 @@ __global__ void k_cand_eval(Geo G, int n, const uint64_t* __restrict__ keys, con
     double B, B1, B2;
     barrier3(d2, shat, B, B1, B2);
     if (flag) flag[i] = d2 < shat ? 1 : 0;
-    if (energy) energy[i] = P.dt * P.dt * P.k_contact * B;
+    // a vertex on or below the ground plane is infeasible: infinite energy, so no line-search
+    // trial that reaches it is accepted
+    const bool through = p.kind == K_PG && ground_gap(X, p.v[0], P) <= 0.0;
+    if (energy) energy[i] = through ? (double)INFINITY : P.dt * P.dt * pair_stiffness(G, p) * B;
 }
 
 void launch_cand_eval(hipStream_t s, const Geo& G, int n, const uint64_t* keys, const double4* X, const AbdParams& P,
@@ -217,7 +235,7 @@ __global__ __launch_bounds__(64) void k_contact_hess(Geo G, int n, const uint64_
         if (lane == 0) body[c] = make_int2(-1, -1);
         return;
     }
-    const double kd = P.k_contact * P.dt * P.dt;
+    const double kd = pair_stiffness(G, p) * P.dt * P.dt;
     const double shat = P.dhat * P.dhat;
     int m, ns;
     int st[4] = {0, 1, 2, 3};
@@ -225,9 +243,15 @@ __global__ __launch_bounds__(64) void k_contact_hess(Geo G, int n, const uint64_
         D3 x = ld3(X, p.v[0]);
         D3 nrm = mk(P.ground_normal[0], P.ground_normal[1], P.ground_normal[2]);
         double d = dot(nrm, x) - P.ground_offset;
+        if (!(d > 0.0)) {
+            // on or below the plane: an infeasible state (the host reports it), no contribution
+            if (lane == 0) atomicOr(G.err, ERR_PENETRATION);
+            d = 0.0;
+        }
         double B, B1, B2;
         barrier3(d * d, shat, B, B1, B2);
-        double lam = fmax(0.0, kd * (2.0 * B1 + 4.0 * B2 * d * d));
+        const double lam0 = kd * (2.0 * B1 + 4.0 * B2 * d * d);
+        const double lam = P.hess_proj == 1 ? fabs(lam0) : fmax(0.0, lam0);
         m = 3; ns = 1;
         if (lane < 9) A[lane] = lam * cmp(nrm, lane / 3) * cmp(nrm, lane % 3);
         if (lane < 3) gx[lane] = kd * B1 * 2.0 * d * cmp(nrm, lane);
@@ -249,7 +273,7 @@ __global__ __launch_bounds__(64) void k_contact_hess(Geo G, int n, const uint64_
         }
         if (lane < m) gx[lane] = kd * B1 * d2_grad(E, lane);
         __syncthreads();
-        wave_psd_project(A, V, m, lane);
+        wave_psd_project(A, V, m, lane, P.hess_proj == 1);
     }
     // stencil point k -> vertex -> body side and Jacobian row [1, xr]
     const int bA = (int)G.rest[p.v[0]].w;
@@ -1445,7 +1469,8 @@ __global__ void k_ccd(Geo G, int n, const uint64_t* __restrict__ keys, const dou
         D3 nrm = mk(P.ground_normal[0], P.ground_normal[1], P.ground_normal[2]);
         double d0 = dot(nrm, a) - P.ground_offset;
         double v = -(dot(nrm, b) - dot(nrm, a));
-        toi = (v <= 0.0 || d0 <= 0.0) ? 1.0 : fmin(1.0, (1.0 - P.ccd_gap_ratio) * d0 / v);
+        // moving away: no bound; moving down from on or below the plane: no step at all
+        toi = (v <= 0.0) ? 1.0 : (d0 <= 0.0 ? 0.0 : fmin(1.0, (1.0 - P.ccd_gap_ratio) * d0 / v));
     } else {
         D3 a[4], b[4];
         for (int k = 0; k < 4; ++k) { a[k] = ld3(X0, p.v[k]); b[k] = ld3(X1, p.v[k]); }
@@ -1501,7 +1526,10 @@ void launch_ls_init(hipStream_t s, int ns, const AbdParams& P, const int* newton
     hipLaunchKernelGGL(k_ls_init, dim3(cdiv(ns, 128)), dim3(128), 0, s, ns, P, newton_active, c);
 }
 
-__global__ void k_ls_decide(int ns, AbdParams P, SceneCtl c, int* pending) {
+// Accept the first halving step that lowers the energy.  At the floor without a decrease the
+// Newton line search keeps the floor step (gauss_newton.py:110-111) unless its energy is not finite
+// (a trial through the ground plane), the predictor's search takes no step (floor_accept 0).
+__global__ void k_ls_decide(int ns, AbdParams P, SceneCtl c, int floor_accept, int* pending) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= ns || c.ls_done[s]) return;
     if (c.E1[s] < c.E0[s]) {
@@ -1509,14 +1537,84 @@ __global__ void k_ls_decide(int ns, AbdParams P, SceneCtl c, int* pending) {
     } else if (c.alpha[s] * 0.5 < P.ls_floor * c.alpha0[s]) {
         c.ls_done[s] = 1;
         c.ls_floor[s] = 1;
+        if (!floor_accept || !isfinite(c.E1[s])) c.alpha[s] = 0.0;
     } else {
         c.alpha[s] *= 0.5;
         atomicOr(pending, 1);
     }
 }
 
-void launch_ls_decide(hipStream_t s, int ns, const AbdParams& P, SceneCtl c, int* pending) {
-    hipLaunchKernelGGL(k_ls_decide, dim3(cdiv(ns, 128)), dim3(128), 0, s, ns, P, c, pending);
+void launch_ls_decide(hipStream_t s, int ns, const AbdParams& P, SceneCtl c, int floor_accept, int* pending) {
+    hipLaunchKernelGGL(k_ls_decide, dim3(cdiv(ns, 128)), dim3(128), 0, s, ns, P, c, floor_accept, pending);
+}
+
+// Direction toward the predicted first iterate, as a Newton step du (q - alpha du moves toward the
+// target): mode 1 = q_now + dt qdot, mode 2 = q~; pinned bodies stay
+__global__ void k_predict_du(Geo G, int mode, double dt, const double* __restrict__ q, const double* __restrict__ qnow,
+                             const double* __restrict__ qdot, const double* __restrict__ qtilde,
+                             double* __restrict__ du) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 12 * G.nb) return;
+    const double target = (mode == 2) ? qtilde[i] : qnow[i] + dt * qdot[i];
+    du[i] = G.pinned[i / 12] ? 0.0 : q[i] - target;
+}
+
+void launch_predict_du(hipStream_t s, const Geo& G, int mode, double dt, const double* q, const double* qnow,
+                       const double* qdot, const double* qtilde, double* du) {
+    hipLaunchKernelGGL(k_predict_du, dim3(cdiv(12 * G.nb, 256)), dim3(256), 0, s, G, mode, dt, q, qnow, qdot, qtilde,
+                       du);
+}
+
+// smallest d^2 per body over the candidate pairs at X (ground pairs: the squared gap)
+__global__ void k_dmin(Geo G, int n, const uint64_t* __restrict__ keys, const double4* __restrict__ X, AbdParams P,
+                       unsigned long long* dmin_bits) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    PairV p = decode(G, keys[i]);
+    if (p.kind < 0) return;
+    const unsigned long long d2 = (unsigned long long)__double_as_longlong(cand_d2(G, p, X, P));
+    atomicMin(&dmin_bits[pair_body(G, p, 0)], d2);
+    if (p.kind != K_PG) atomicMin(&dmin_bits[pair_body(G, p, 3)], d2);
+}
+
+void launch_dmin(hipStream_t s, const Geo& G, int n, const uint64_t* keys, const double4* X, const AbdParams& P,
+                 unsigned long long* dmin_bits) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_dmin, dim3(cdiv(n, 256)), dim3(256), 0, s, G, n, keys, X, P, dmin_bits);
+}
+
+// Adaptive contact stiffness, per body (after Li et al. 2020, Incremental Potential Contact,
+// section 4.3, where one stiffness is raised for the whole scene): a body of an iterating scene
+// doubles its stiffness, up to k_max, when its closest contact is nearer than k_raise_gap * dhat and
+// nearer than after its previous iteration of the step.
+__global__ void k_stiffness_update(Geo G, AbdParams P, const int* __restrict__ active, SceneCtl c) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= G.nb) return;
+    const int s = G.bscene[b];
+    if (!active[s]) return;
+    const double d = sqrt(__longlong_as_double((long long)c.dmin_bits[b]));
+    if (P.k_max > c.kb[b] && d < P.k_raise_gap * P.dhat && d < c.dmin_prev[b]) {
+        c.kb[b] = fmin(2.0 * c.kb[b], P.k_max);
+        atomicAdd(&c.k_raises[s], 1);
+    }
+    c.dmin_prev[b] = d;
+}
+
+void launch_stiffness_update(hipStream_t s, const Geo& G, const AbdParams& P, const int* active, SceneCtl c) {
+    hipLaunchKernelGGL(k_stiffness_update, dim3(cdiv(G.nb, 128)), dim3(128), 0, s, G, P, active, c);
+}
+
+// largest body stiffness of each scene (one thread per scene)
+__global__ void k_scene_kmax(Geo G, SceneCtl c) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= G.ns) return;
+    double k = 0.0;
+    for (int b = G.scene_b0[s]; b < G.scene_b0[s + 1]; ++b) k = fmax(k, c.kb[b]);
+    c.kc[s] = k;
+}
+
+void launch_scene_kmax(hipStream_t s, const Geo& G, SceneCtl c) {
+    hipLaunchKernelGGL(k_scene_kmax, dim3(cdiv(G.ns, 64)), dim3(64), 0, s, G, c);
 }
 
 // add_du_kernel (abd.py:780-785): q -= alpha du

@@ -37,6 +37,11 @@ class Params:
     max_candidates: int = 1 << 20
     solver: int = 0  # 0 = PCG, 1 = direct (dense Cholesky per scene), 2 = PCG + direct fallback
     direct_max_dofs: int = 6144  # largest scene (12 x bodies) the direct solve takes
+    predictor: int = 0  # first Newton iterate: 0 = q_now (the reference), 1 = q_now + dt qdot,
+    #                     2 = the inertial target q~, 3 = learned (NewtonAbd.set_predictor)
+    hess_proj: int = 0  # contact Hessian projection: 0 = clamp eigenvalues at 0, 1 = absolute values
+    k_max: float = 0.0  # adaptive contact stiffness cap (<= k_contact: fixed stiffness)
+    k_raise_gap: float = 0.5  # raise the stiffness when a contact gets closer than this * dhat
     debug: int = 0  # bit 0: synchronise and check after every launch; bit 1: also log them;
     #                 bit 2: print broad-phase list sizes
 
@@ -65,6 +70,10 @@ class Params:
         p.reserved0 = self.debug
         p.solver = self.solver
         p.direct_max_dofs = self.direct_max_dofs
+        p.predictor = self.predictor
+        p.hess_proj = self.hess_proj
+        p.k_max = self.k_max
+        p.k_raise_gap = self.k_raise_gap
         return p
 
     def replace(self, **kw):

@@ -68,6 +68,21 @@ typedef struct AbdParams {
                                   at cg_max_iter */
     int    direct_max_dofs;    /* largest scene (12 x bodies) the direct solve takes, <= 6144;
                                   <= 0: 6144.  solver = 1 rejects a larger scene */
+    int    predictor;          /* first Newton iterate of a step: 0 = the state at the start of the
+                                  step (abd.py:674-699); 1 = extrapolated by dt * qdot; 2 = the
+                                  inertial target q~; 3 = the learned predictor (abd_set_predictor).
+                                  1-3 move from q_now toward the prediction by the largest CCD-safe
+                                  halving step that lowers the energy (none: stay at q_now) */
+    int    hess_proj;          /* projection of each contact's barrier Hessian onto the PSD cone:
+                                  0 = negative eigenvalues clamped to zero (IPC); 1 = their
+                                  absolute values (eigenvalue mirroring) */
+    double k_max;              /* adaptive contact stiffness: each body's stiffness starts at
+                                  k_contact and doubles (up to k_max) after a Newton iteration whose
+                                  accepted iterate puts one of its contacts closer than
+                                  k_raise_gap * dhat and closer than after the previous iteration of
+                                  the step; a contact uses the larger stiffness of its two bodies.
+                                  k_max <= k_contact: fixed stiffness (the reference, abd.py:30) */
+    double k_raise_gap;        /* fraction of dhat in (0, 1) below which the stiffness is raised */
 } AbdParams;
 
 typedef struct AbdSceneDesc {
@@ -95,7 +110,11 @@ typedef struct AbdStepStats {
     double alpha_min;          /* smallest accepted line-search step */
     double energy;             /* incremental potential at the end of the step */
     double ccd_alpha_min;      /* smallest CCD bound seen */
-    double reserved;
+    double k_contact;          /* largest body contact stiffness of the scene at the end of the step */
+    int    k_raises;           /* adaptive-stiffness doublings of the scene's bodies during the step */
+    int    pcg_fallbacks;      /* solves rerun with one launch per PCG iteration because their
+                                  single cooperative launch was refused or a grid barrier timed out */
+    double predictor_alpha;    /* step taken toward the predicted first iterate (0: none) */
 } AbdStepStats;
 
 typedef struct AbdCtx AbdCtx;
@@ -116,6 +135,12 @@ int abd_get_state(const AbdCtx* ctx, double* q, double* q_now, double* qdot);
 /* NewtonAbd.step (abd.py:674-702): nsteps full time steps, device resident.  stats (may be
  * NULL) receives n_scenes entries for the last step. */
 int abd_step(AbdCtx* ctx, int nsteps, AbdStepStats* stats);
+
+/* No counterpart in the reference (its contact stiffness is the constant contact_stiffness,
+ * abd.py:30): each body's current contact stiffness k[n_bodies], raised by the adaptive rule of
+ * AbdParams.k_max and reset to k_contact by abd_create and abd_set_params (e.g. for checkpoints). */
+int abd_get_stiffness(const AbdCtx* ctx, double* k);
+int abd_set_stiffness(AbdCtx* ctx, const double* k);
 
 /* Profiller arrays for the last step (abd.py:695-696, utils/data_collector.py:8-9):
  * residual[s*max_newton + it], alpha[...], cg_iters[...]; unused entries are 0. */
@@ -148,6 +173,11 @@ int abd_ccd(AbdCtx* ctx, const double* du, double* alpha);
 /* line_search_iterative (gauss_newton.py:98-115) with the CCD filter: uses the last solve's du,
  * applies q -= alpha*du per scene, returns alpha[n_scenes]. */
 int abd_line_search(AbdCtx* ctx, double* alpha);
+
+/* line_search_iterative (gauss_newton.py:98-115) with the CCD filter along a caller-given direction
+ * du[n_bodies*12] (host) instead of the last solve's: applies q -= alpha*du per scene, returns
+ * alpha[n_scenes] (e.g. a non-descent direction ends at the floor step). */
+int abd_line_search_along(AbdCtx* ctx, const double* du, double* alpha);
 
 /* forward_states (abd.py:574-581, 670-672): qdot = (q - q_now)/dt, q_now = q. */
 int abd_forward(AbdCtx* ctx);

@@ -11,6 +11,7 @@
 // an independent check (closed form, finite differences, brute force, numpy/scipy) are listed
 // in tests/test_oracle_*.py; see the header comment of each function for what it follows.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -314,10 +315,11 @@ double pair_dist2(int kind, const V3* x, double* g12, double* H144, int* dtype) 
 
 // ---------------------------------------------------------------------------------------------
 // PSD projection by the cyclic Jacobi eigenvalue method (Golub & Van Loan, section 8.5):
-// H <- V max(Lambda, 0) V^T.  Replaces ipctk_wp's x_to_grad_psd_hess_ee projection
-// (abd.py:247), applied to the whole barrier Hessian so that PCG sees an SPD system.
+// H <- V max(Lambda, 0) V^T (absolute: V |Lambda| V^T).  Replaces ipctk_wp's
+// x_to_grad_psd_hess_ee projection (abd.py:247), applied to the whole barrier Hessian so that PCG
+// sees an SPD system.
 // ---------------------------------------------------------------------------------------------
-void psd_project(int n, double* H) {
+void psd_project(int n, double* H, bool absolute = false) {
     double A[144], V[144];
     std::memcpy(A, H, sizeof(double) * n * n);
     for (int i = 0; i < n * n; ++i) V[i] = 0;
@@ -356,7 +358,7 @@ void psd_project(int n, double* H) {
             }
     }
     double lam[12];
-    for (int i = 0; i < n; ++i) lam[i] = std::max(0.0, A[i * n + i]);
+    for (int i = 0; i < n; ++i) lam[i] = absolute ? std::fabs(A[i * n + i]) : std::max(0.0, A[i * n + i]);
     for (int r = 0; r < n; ++r)
         for (int c = 0; c < n; ++c) {
             double h = 0;
@@ -412,9 +414,11 @@ double accd(int kind, const V3* x0, const V3* x1, double s, double tmax, int max
     return t;  // iteration cap: the t reached so far is still conservative
 }
 
-// point vs. plane n.x = h, linear trajectory: gap(t) = d0 - t*v; keep >= s*d0
+// point vs. plane n.x = h, linear trajectory: gap(t) = d0 - t*v; keep >= s*d0.  Moving away: no
+// bound; moving down from on or below the plane: no step at all.
 double ccd_ground(double d0, double v, double s, double tmax) {
-    if (v <= 0.0 || d0 <= 0.0) return tmax;
+    if (v <= 0.0) return tmax;
+    if (d0 <= 0.0) return 0.0;
     return std::min(tmax, (1.0 - s) * d0 / v);
 }
 
@@ -447,6 +451,8 @@ struct Ctx {
     std::vector<double> log_res, log_alpha;
     std::vector<int> log_cg;
     std::vector<AbdStepStats> stats;
+    std::vector<double> kb;  // contact stiffness per body (adaptive rule of AbdParams.k_max)
+    mutable std::atomic<int> err{0};  // 64: a vertex on or below the ground plane was assembled
     int threads = 1;
 };
 
@@ -639,6 +645,20 @@ double cand_dist2(const Ctx& C, const Pair& pr, const std::vector<V3>& X, int* d
     return pair_dist2(pr.kind, x, nullptr, nullptr, dtype);
 }
 
+// bodies of a pair (b1 = -1 for ground)
+void pair_bodies(const Ctx& C, const Pair& pr, int* b0, int* b1) {
+    if (pr.kind == ABD_PT) { *b0 = C.vbody[pr.i]; *b1 = C.vbody[C.tris[3 * pr.j]]; }
+    else if (pr.kind == ABD_EE) { *b0 = C.vbody[C.edges[2 * pr.i]]; *b1 = C.vbody[C.edges[2 * pr.j]]; }
+    else { *b0 = C.vbody[pr.i]; *b1 = -1; }
+}
+
+// contact stiffness of a pair: the larger of its bodies' stiffness (abd.py:30 when fixed)
+double pair_stiffness(const Ctx& C, const Pair& pr) {
+    int b0, b1;
+    pair_bodies(C, pr, &b0, &b1);
+    return b1 < 0 ? C.kb[b0] : std::max(C.kb[b0], C.kb[b1]);
+}
+
 // energy_inertia (abd.py:53-64): 1/2 |q - q~|_M^2 + dt^2 Psi(q), pinned bodies excluded.
 double body_energy(const Ctx& C, const double* q, int b) {
     if (C.pinned[b]) return 0.0;
@@ -659,9 +679,11 @@ double body_energy(const Ctx& C, const double* q, int b) {
 // energy_contact (abd.py:66-75): dt^2 k B(d^2) over active pairs.
 double contact_energy(const Ctx& C, const Pair& pr, const std::vector<V3>& X) {
     double d2 = cand_dist2(C, pr, X, nullptr);
+    if (pr.kind == ABD_PG && !(dot(v3(C.P.ground_normal), X[pr.i]) - C.P.ground_offset > 0.0))
+        return HUGE_VAL;  // on or below the plane: infeasible
     double B, B1, B2;
     barrier(d2, C.P.dhat * C.P.dhat, B, B1, B2);
-    return C.P.dt * C.P.dt * C.P.k_contact * B;
+    return C.P.dt * C.P.dt * pair_stiffness(C, pr) * B;
 }
 
 // Incremental potential per scene at q, over the active pairs at q (compute_g, abd.py:725-735:
@@ -745,6 +767,11 @@ void contact_term(const Ctx& C, const Pair& pr, const std::vector<V3>& X, Contac
     if (pr.kind == ABD_PG) {
         V3 n = v3(C.P.ground_normal);
         double d = dot(n, x[0]) - C.P.ground_offset;
+        if (!(d > 0.0)) {
+            // on or below the plane: an infeasible state (reported), no contribution
+            C.err |= 64;
+            d = 0.0;
+        }
         d2 = d * d;
         for (int k = 0; k < 12; ++k) gx[k] = 0;
         for (int k = 0; k < 144; ++k) Hx[k] = 0;
@@ -758,18 +785,20 @@ void contact_term(const Ctx& C, const Pair& pr, const std::vector<V3>& X, Contac
     }
     double B, B1, B2;
     barrier(d2, C.P.dhat * C.P.dhat, B, B1, B2);
-    double kd = C.P.k_contact * C.P.dt * C.P.dt;
+    double kd = pair_stiffness(C, pr) * C.P.dt * C.P.dt;
+    const bool absolute = C.P.hess_proj == 1;
     if (nvp == 1) {
         // point-plane: H = kd (B1 * 2 n n^T + B2 (2 d n)(2 d n)^T) = kd (2 B1 + 4 B2 d^2) n n^T
-        // (unit n); its PSD projection clamps the single eigenvalue at zero.
+        // (unit n); its PSD projection clamps the single eigenvalue at zero (or mirrors it).
         V3 n = v3(C.P.ground_normal);
-        double lam = std::max(0.0, kd * (2.0 * B1 + 4.0 * B2 * d2));
+        const double lam0 = kd * (2.0 * B1 + 4.0 * B2 * d2);
+        double lam = absolute ? std::fabs(lam0) : std::max(0.0, lam0);
         for (int r = 0; r < 3; ++r)
             for (int c = 0; c < 3; ++c) Hx[r * 12 + c] = lam * comp(n, r) * comp(n, c);
     } else {
         for (int R = 0; R < 12; ++R)
             for (int Cc = 0; Cc < 12; ++Cc) Hx[R * 12 + Cc] = kd * (B1 * Hx[R * 12 + Cc] + B2 * gx[R] * gx[Cc]);
-        psd_project(12, Hx);
+        psd_project(12, Hx, absolute);
     }
     for (int k = 0; k < 12; ++k) gx[k] *= kd * B1;
     int bod[4];
@@ -1004,8 +1033,84 @@ void compute_qtilde(Ctx& C) {
         }
 }
 
+// line_search_iterative (gauss_newton.py:98-115) per scene along q -> q - alpha du from the CCD
+// bound: halve until the energy decreases.  At the floor without a decrease the Newton search keeps
+// the floor step (gauss_newton.py:110-111) unless its energy is not finite; the predictor's search
+// (floor_accept false) takes no step.  Applies the step to the iterating scenes.
+void line_search(Ctx& C, const std::vector<double>& du, const std::vector<char>& active, bool floor_accept,
+                 std::vector<double>& alpha, std::vector<double>& amax, std::vector<int>& floor_hit) {
+    amax.assign(C.ns, 1.0);
+    if (C.P.ccd_enabled) ccd_bound(C, C.q.data(), du, amax);
+    std::vector<double> E0(C.ns), E1(C.ns);
+    scene_energy(C, C.q.data(), E0.data());
+    std::vector<double> a0(C.ns);
+    alpha.assign(C.ns, 0.0);
+    floor_hit.assign(C.ns, 0);
+    std::vector<char> ls_done(C.ns, 0);
+    for (int s = 0; s < C.ns; ++s) {
+        a0[s] = alpha[s] = amax[s];
+        if (!active[s]) { ls_done[s] = 1; alpha[s] = 0.0; }
+    }
+    std::vector<double> qt(C.q.size());
+    while (true) {
+        bool pending = false;
+        for (int s = 0; s < C.ns; ++s) pending |= !ls_done[s];
+        if (!pending) break;
+        for (int b = 0; b < C.nb; ++b) {
+            int s = C.bscene[b];
+            double al = ls_done[s] ? 0.0 : alpha[s];
+            for (int k = 0; k < 12; ++k) qt[12 * b + k] = C.q[12 * b + k] - al * du[12 * b + k];
+        }
+        scene_energy(C, qt.data(), E1.data());
+        for (int s = 0; s < C.ns; ++s) {
+            if (ls_done[s]) continue;
+            C.stats[s].energy = E1[s];
+            if (E1[s] < E0[s]) ls_done[s] = 1;
+            else if (alpha[s] * 0.5 < C.P.ls_floor * a0[s]) {
+                ls_done[s] = 1;
+                floor_hit[s] = 1;
+                if (!floor_accept || !std::isfinite(E1[s])) alpha[s] = 0.0;
+            } else alpha[s] *= 0.5;
+        }
+    }
+    for (int b = 0; b < C.nb; ++b) {
+        int s = C.bscene[b];
+        if (!active[s]) continue;
+        for (int k = 0; k < 12; ++k) C.q[12 * b + k] -= alpha[s] * du[12 * b + k];
+    }
+}
+
+// Adaptive contact stiffness per body (the rule of AbdParams.k_max): a body of an iterating scene
+// doubles its stiffness, up to k_max, when its closest contact at the accepted iterate is nearer
+// than k_raise_gap * dhat and nearer than after its previous iteration of the step.  Contacts at or
+// beyond dhat never trigger it, so the distances over the active set decide as over any superset.
+void adapt_stiffness(Ctx& C, const std::vector<char>& active, std::vector<double>& dmin_prev) {
+    if (!(C.P.k_max > C.P.k_contact)) return;
+    std::vector<Pair> act;
+    std::vector<double> d2;
+    active_pairs(C, C.q.data(), act, &d2);
+    std::vector<double> dmin(C.nb, HUGE_VAL);
+    for (size_t c = 0; c < act.size(); ++c) {
+        int b0, b1;
+        pair_bodies(C, act[c], &b0, &b1);
+        const double d = std::sqrt(d2[c]);
+        dmin[b0] = std::min(dmin[b0], d);
+        if (b1 >= 0) dmin[b1] = std::min(dmin[b1], d);
+    }
+    for (int b = 0; b < C.nb; ++b) {
+        const int s = C.bscene[b];
+        if (!active[s]) continue;
+        if (C.P.k_max > C.kb[b] && dmin[b] < C.P.k_raise_gap * C.P.dhat && dmin[b] < dmin_prev[b]) {
+            C.kb[b] = std::min(2.0 * C.kb[b], C.P.k_max);
+            C.stats[s].k_raises++;
+        }
+        dmin_prev[b] = dmin[b];
+    }
+}
+
 // NewtonAbd.step (abd.py:674-702): detect, assemble, solve, line search, until the residual of
-// the pre-update iterate is below tol or max_newton solves; then forward_states.
+// the pre-update iterate is below tol or max_newton solves; then forward_states.  With a predictor
+// (AbdParams.predictor 1, 2) the first iterate moves from q_now toward q_now + dt qdot or q~ first.
 void step_once(Ctx& C) {
     compute_qtilde(C);
     const int mi = C.P.max_newton;
@@ -1015,6 +1120,21 @@ void step_once(Ctx& C) {
     C.stats.assign(C.ns, AbdStepStats{});
     for (auto& st : C.stats) { st.alpha_min = 1.0; st.ccd_alpha_min = 1.0; }
     std::vector<char> active(C.ns, 1);
+    std::vector<double> dmin_prev(C.nb, HUGE_VAL);
+    std::vector<double> alpha, amax;
+    std::vector<int> floor_hit;
+    if (C.P.predictor == 1 || C.P.predictor == 2) {
+        std::vector<double> dup(12 * C.nb, 0.0);
+        for (int b = 0; b < C.nb; ++b)
+            if (!C.pinned[b])
+                for (int k = 0; k < 12; ++k) {
+                    const int i = 12 * b + k;
+                    const double target = C.P.predictor == 2 ? C.qtilde[i] : C.qnow[i] + C.P.dt * C.qdot[i];
+                    dup[i] = C.q[i] - target;
+                }
+        line_search(C, dup, active, false, alpha, amax, floor_hit);
+        for (int s = 0; s < C.ns; ++s) C.stats[s].predictor_alpha = alpha[s];
+    }
     for (int it = 0; it < mi; ++it) {
         bool any = false;
         for (int s = 0; s < C.ns; ++s) any |= active[s] != 0;
@@ -1038,44 +1158,12 @@ void step_once(Ctx& C) {
         std::vector<double> du;
         std::vector<int> cgit;
         pcg(C, S, du, cgit);
-        std::vector<double> amax(C.ns, 1.0);
-        if (C.P.ccd_enabled) ccd_bound(C, C.q.data(), du, amax);
-        std::vector<double> E0(C.ns), E1(C.ns);
-        scene_energy(C, C.q.data(), E0.data());
-        std::vector<double> alpha(C.ns), a0(C.ns);
-        std::vector<char> ls_done(C.ns, 0);
-        for (int s = 0; s < C.ns; ++s) {
-            a0[s] = alpha[s] = amax[s];
-            if (!active[s]) ls_done[s] = 1;
-        }
-        std::vector<double> qt(C.q.size());
-        while (true) {
-            bool pending = false;
-            for (int s = 0; s < C.ns; ++s) pending |= !ls_done[s];
-            if (!pending) break;
-            for (int b = 0; b < C.nb; ++b) {
-                int s = C.bscene[b];
-                double al = ls_done[s] ? 0.0 : alpha[s];
-                if (!active[s]) al = 0.0;
-                for (int k = 0; k < 12; ++k) qt[12 * b + k] = C.q[12 * b + k] - al * du[12 * b + k];
-            }
-            scene_energy(C, qt.data(), E1.data());
-            for (int s = 0; s < C.ns; ++s) {
-                if (ls_done[s]) continue;
-                C.stats[s].energy = E1[s];
-                if (E1[s] < E0[s]) ls_done[s] = 1;
-                else if (alpha[s] * 0.5 < C.P.ls_floor * a0[s]) { ls_done[s] = 1; C.stats[s].ls_floor_hits++; }
-                else alpha[s] *= 0.5;
-            }
-        }
-        for (int b = 0; b < C.nb; ++b) {
-            int s = C.bscene[b];
-            if (!active[s]) continue;
-            for (int k = 0; k < 12; ++k) C.q[12 * b + k] -= alpha[s] * du[12 * b + k];
-        }
+        line_search(C, du, active, true, alpha, amax, floor_hit);
+        adapt_stiffness(C, active, dmin_prev);
         for (int s = 0; s < C.ns; ++s) {
             if (!active[s]) continue;
             AbdStepStats& st = C.stats[s];
+            st.ls_floor_hits += floor_hit[s];
             C.log_res[s * mi + it] = res[s];
             C.log_alpha[s * mi + it] = alpha[s];
             C.log_cg[s * mi + it] = cgit[s];
@@ -1087,6 +1175,11 @@ void step_once(Ctx& C) {
             st.ccd_alpha_min = std::min(st.ccd_alpha_min, amax[s]);
             if (res[s] < C.P.newton_tol) { st.converged = 1; active[s] = 0; }
         }
+    }
+    for (int s = 0; s < C.ns; ++s) {
+        double k = 0.0;
+        for (int b = C.scene_b0[s]; b < C.scene_b1[s]; ++b) k = std::max(k, C.kb[b]);
+        C.stats[s].k_contact = k;
     }
     // forward_states (abd.py:574-581)
     for (int i = 0; i < 12 * C.nb; ++i) {
@@ -1162,6 +1255,7 @@ void* orc_create(const AbdSceneDesc* d, const AbdParams* p) {
     for (int b = 0; b < C->nb; ++b) { C->q[12 * b + 3] = C->q[12 * b + 7] = C->q[12 * b + 11] = 1.0; }
     C->qnow = C->q;
     C->qdot.assign(12 * C->nb, 0.0);
+    C->kb.assign(C->nb, C->P.k_contact);
     compute_qtilde(*C);
     return C;
 }
@@ -1170,7 +1264,28 @@ void orc_destroy(void* c) { delete (Ctx*)c; }
 
 void orc_set_threads(void* c, int n) { ((Ctx*)c)->threads = std::max(1, n); }
 
-void orc_set_params(void* c, const AbdParams* p) { ((Ctx*)c)->P = *p; compute_qtilde(*(Ctx*)c); }
+void orc_set_params(void* c, const AbdParams* p) {
+    Ctx* C = (Ctx*)c;
+    C->P = *p;
+    double* n = C->P.ground_normal;
+    double l = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    if (l > 0) { n[0] /= l; n[1] /= l; n[2] /= l; }
+    C->kb.assign(C->nb, C->P.k_contact);
+    compute_qtilde(*C);
+}
+
+void orc_get_stiffness(void* c, double* k) {
+    Ctx* C = (Ctx*)c;
+    std::memcpy(k, C->kb.data(), C->nb * sizeof(double));
+}
+
+void orc_set_stiffness(void* c, const double* k) {
+    Ctx* C = (Ctx*)c;
+    std::memcpy(C->kb.data(), k, C->nb * sizeof(double));
+}
+
+// error bits raised since the last call (64: a vertex on or below the ground plane was assembled)
+int orc_take_error(void* c) { return ((Ctx*)c)->err.exchange(0); }
 
 void orc_set_state(void* c, const double* q, const double* qnow, const double* qdot) {
     Ctx* C = (Ctx*)c;
@@ -1269,6 +1384,17 @@ void orc_ccd(void* c, const double* du, double* alpha) {
     std::vector<double> d(du, du + 12 * C->nb), a;
     ccd_bound(*C, C->q.data(), d, a);
     for (int s = 0; s < C->ns; ++s) alpha[s] = a[s];
+}
+
+// the line search along a given direction (the Newton search's rules); alpha[ns]
+void orc_line_search(void* c, const double* du, double* alpha_out) {
+    Ctx* C = (Ctx*)c;
+    std::vector<double> d(du, du + 12 * C->nb), alpha, amax;
+    std::vector<int> fh;
+    std::vector<char> active(C->ns, 1);
+    C->stats.assign(C->ns, AbdStepStats{});
+    line_search(*C, d, active, true, alpha, amax, fh);
+    for (int s = 0; s < C->ns; ++s) alpha_out[s] = alpha[s];
 }
 
 void orc_step(void* c, int nsteps, AbdStepStats* stats) {

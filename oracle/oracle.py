@@ -56,6 +56,10 @@ def lib():
             "orc_step": (None, [V, C.c_int, C.POINTER(AbdStepStats)]),
             "orc_get_iter_log": (None, [V, D, D, I]),
             "orc_get_vertices": (None, [V, D]),
+            "orc_get_stiffness": (None, [V, D]),
+            "orc_set_stiffness": (None, [V, D]),
+            "orc_take_error": (C.c_int, [V]),
+            "orc_line_search": (None, [V, D, D]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -151,6 +155,18 @@ class Oracle:
         a = [None if x is None else f64(x).reshape(-1) for x in (q, q_now, qdot)]
         lib().orc_set_state(self.h, *[dptr(x) for x in a])
 
+    def stiffness(self):
+        k = np.zeros(self.nb)
+        lib().orc_get_stiffness(self.h, dptr(k))
+        return k
+
+    def set_stiffness(self, k):
+        lib().orc_set_stiffness(self.h, dptr(f64(k).reshape(-1)))
+
+    def take_error(self):
+        """Error bits raised since the last call (64: a vertex on or below the ground plane)."""
+        return int(lib().orc_take_error(self.h))
+
     def get_state(self):
         q, qn, qd = (np.zeros(12 * self.nb) for _ in range(3))
         lib().orc_get_state(self.h, dptr(q), dptr(qn), dptr(qd))
@@ -208,6 +224,12 @@ class Oracle:
     def ccd(self, du):
         a = np.zeros(self.ns)
         lib().orc_ccd(self.h, dptr(f64(du).reshape(-1)), dptr(a))
+        return a
+
+    def line_search(self, du):
+        """Line search along du (q -= alpha du applied); alpha per scene."""
+        a = np.zeros(self.ns)
+        lib().orc_line_search(self.h, dptr(f64(du).reshape(-1)), dptr(a))
         return a
 
     def step(self, nsteps=1):

@@ -2,6 +2,7 @@
 TEST INFRASTRUCTURE ONLY: the result is loaded by tests that check kernel indexing on the CPU."""
 import os
 import subprocess
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
@@ -13,17 +14,23 @@ FLAGS = ["-std=c++20", "-DABD_EMULATION", "-O1", "-g", "-fPIC", "-fno-omit-frame
          "-Wno-unused-result", "-Wno-attributes"]
 
 
-def build(asan=True):
+SOURCES = ("abd_kernels.hip", "abd_broad.hip", "abd_sort.hip", "abd_mlp.hip", "abd_direct.hip", "abd_api.hip")
+
+
+def build(asan=True, jobs=4):
     os.makedirs(OUT, exist_ok=True)
     flags = FLAGS if asan else [f for f in FLAGS if f != "-fsanitize=address"]
-    objs = []
-    for src in ("abd_kernels.hip", "abd_broad.hip", "abd_sort.hip", "abd_mlp.hip", "abd_direct.hip", "abd_api.hip"):
+
+    def compile_one(src):
         obj = os.path.join(OUT, src.replace(".hip", ".o"))
         r = subprocess.run(["g++", *flags, "-x", "c++", "-c", os.path.join(CSRC, src), "-o", obj],
                            capture_output=True, text=True)
         if r.returncode:
             raise RuntimeError(r.stderr[-5000:])
-        objs.append(obj)
+        return obj
+
+    with ThreadPoolExecutor(jobs) as ex:
+        objs = list(ex.map(compile_one, SOURCES))
     r = subprocess.run(["g++", *flags, "-shared", "-o", LIB, *objs], capture_output=True, text=True)
     if r.returncode:
         raise RuntimeError(r.stderr[-3000:])

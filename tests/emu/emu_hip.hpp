@@ -197,7 +197,7 @@ inline float __double2float_ru(double x) {
 }
 inline long long __double_as_longlong(double d) { long long r; std::memcpy(&r, &d, 8); return r; }
 inline double __longlong_as_double(long long v) { double r; std::memcpy(&r, &v, 8); return r; }
-using std::fmin; using std::fmax; using std::sqrt; using std::fabs;
+using std::fmin; using std::fmax; using std::sqrt; using std::fabs; using std::isfinite;
 inline int min(int a, int b) { return a < b ? a : b; }
 inline int max(int a, int b) { return a > b ? a : b; }
 inline unsigned long long min(unsigned long long a, unsigned long long b) { return a < b ? a : b; }

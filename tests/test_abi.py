@@ -42,14 +42,17 @@ def test_struct_layout_matches_header(tmp_path):
         "int main(void){printf(\"%zu %zu %zu %zu %zu %zu %zu %zu\\n\", sizeof(AbdParams), sizeof(AbdSceneDesc),"
         " sizeof(AbdStepStats), offsetof(AbdParams, ground_normal), offsetof(AbdParams, max_candidates),"
         " offsetof(AbdStepStats, residual_first), offsetof(AbdParams, solver),"
-        " offsetof(AbdParams, direct_max_dofs)); return 0;}\n")
+        " offsetof(AbdParams, direct_max_dofs)); printf(\"%zu %zu %zu %zu\\n\", offsetof(AbdParams, k_max),"
+        " offsetof(AbdParams, k_raise_gap), offsetof(AbdStepStats, k_raises), offsetof(AbdStepStats, predictor_alpha));"
+        " return 0;}\n")
     exe = tmp_path / "layout"
     subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
     got = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
     want = [C.sizeof(_abi.AbdParams), C.sizeof(_abi.AbdSceneDesc), C.sizeof(_abi.AbdStepStats),
             _abi.AbdParams.ground_normal.offset, _abi.AbdParams.max_candidates.offset,
             _abi.AbdStepStats.residual_first.offset, _abi.AbdParams.solver.offset,
-            _abi.AbdParams.direct_max_dofs.offset]
+            _abi.AbdParams.direct_max_dofs.offset, _abi.AbdParams.k_max.offset, _abi.AbdParams.k_raise_gap.offset,
+            _abi.AbdStepStats.k_raises.offset, _abi.AbdStepStats.predictor_alpha.offset]
     assert got == want
 
 
@@ -59,3 +62,58 @@ def test_params_reach_the_c_struct():
     assert (c.solver, c.direct_max_dofs, c.precision, c.reserved0) == (2, 1536, 1, 4)
     d = north_star().to_c()
     assert (d.solver, d.direct_max_dofs) == (0, 6144)
+
+
+INVALID = [("dt", 0.0), ("dt", -1e-3), ("dhat", 0.0), ("kappa", -1.0), ("k_contact", -1.0), ("newton_tol", -1e-5),
+           ("max_newton", 0), ("cg_max_iter", 0), ("cg_tol", 0.0), ("cg_tol", 1.5), ("ccd_gap_ratio", 0.0),
+           ("ccd_gap_ratio", 1.0), ("ccd_max_iter", 0), ("ls_floor", 0.0), ("ls_floor", 1.0), ("pinned_diag", 0.0),
+           ("dt", float("nan")), ("gravity", (0.0, float("inf"), 0.0)), ("predictor", 4), ("hess_proj", 2),
+           ("solver", 3), ("precision", 2), ("direct_max_dofs", 1 << 20),
+           ("k_raise_gap", 1.0)]
+
+
+@pytest.mark.parametrize("field,value", INVALID)
+def test_create_rejects_invalid_parameters(field, value):
+    """abd_create validates the parameters before it touches the device (no GPU call is made)."""
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("HIP library not built")
+    from colb_amd import scene as S
+    from colb_amd.params import north_star
+    sc = S.cubes_on_ground(n=1)
+    kw = {field: value}
+    if field == "k_raise_gap":
+        kw["k_max"] = 1e30  # the gap is checked when the stiffness adapts
+    p = north_star(**kw).to_c()
+    desc = _abi.make_scene_desc(sc)
+    L = _lib.load()
+    h = C.c_void_p()
+    rc = L.abd_create(C.byref(desc), C.byref(p), C.byref(h))
+    msg = L.abd_last_error(h).decode()
+    L.abd_destroy(h)
+    assert rc == _abi.ABD_ERR_ARG
+    assert field.split("_")[0] in msg or field in msg, msg
+
+
+def test_profiler_npz_round_trip(tmp_path):
+    """Profiller (utils/data_collector.py:4-55): the npz keeps `convergence` and `alphas` per frame."""
+    import numpy as np
+    from colb_amd.profiler import Profiller
+    p = Profiller(max_iters=16, frames=5)
+    rng = np.random.default_rng(0)
+    rows = [rng.uniform(size=n) for n in (3, 16, 7)]
+    for f, r in enumerate(rows):
+        p.record(f, r, 0.5 * r)
+    p.record(9, np.ones(4), np.ones(4))  # beyond the frames: ignored
+    path = tmp_path / "profiler.npz"
+    p.save(str(path))
+    with np.load(path, allow_pickle=False) as z:
+        assert sorted(z.files) == ["alphas", "convergence"]
+        assert z["convergence"].shape == (5, 16)
+        for f, r in enumerate(rows):
+            np.testing.assert_array_equal(z["convergence"][f, :len(r)], r)
+            np.testing.assert_array_equal(z["alphas"][f, :len(r)], 0.5 * r)
+            assert np.all(z["convergence"][f, len(r):] == 0)
+    q = Profiller(max_iters=16, frames=5)
+    q.load(str(path))
+    np.testing.assert_array_equal(q.convergence, p.convergence)
+    np.testing.assert_array_equal(q.alphas, p.alphas)

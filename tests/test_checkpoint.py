@@ -16,6 +16,7 @@ class _Stub:
         self.scene, self.params, self.n_bodies = sc, params, sc.n_bodies
         self.frame, self.pcg_layout = 0, (5, False)
         self.state = (sc.q0.copy(), sc.q0.copy(), sc.qdot0.copy())
+        self.k = np.full(sc.n_bodies, params.k_contact)
         self.calls = []
 
     def get_state(self):
@@ -33,6 +34,13 @@ class _Stub:
         self.pcg_layout = (bpt, bool(multi))
         self.calls.append("set_pcg_layout")
 
+    def stiffness(self):
+        return self.k
+
+    def set_stiffness(self, k):
+        self.k = np.array(k)
+        self.calls.append("set_stiffness")
+
     def compute_V(self):
         self.calls.append("compute_V")
 
@@ -44,6 +52,7 @@ def test_round_trip(tmp_path):
     rng = np.random.default_rng(0)
     a.state = tuple(rng.normal(size=(sc.n_bodies, 12)) for _ in range(3))
     a.frame, a.pcg_layout = 17, (3, True)
+    a.k = rng.uniform(1e14, 1e16, size=sc.n_bodies)
     f = tmp_path / "ck.npz"
     checkpoint.save(a, f)
     b = _Stub(sc, north_star())
@@ -52,7 +61,8 @@ def test_round_trip(tmp_path):
         np.testing.assert_array_equal(x, y)
     assert b.frame == 17 and b.pcg_layout == (3, True)
     assert b.params == p and isinstance(b.params.gravity, tuple)
-    assert b.calls == ["set_params", "set_pcg_layout", "set_state", "compute_V"]
+    np.testing.assert_array_equal(a.k, b.k)
+    assert b.calls == ["set_params", "set_pcg_layout", "set_state", "set_stiffness", "compute_V"]
     assert d["scene_sha256"] == checkpoint.scene_fingerprint(sc)
 
 
