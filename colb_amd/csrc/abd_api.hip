@@ -1580,12 +1580,20 @@ int abd_set_params(AbdCtx* ctx, const AbdParams* p) {
 int abd_set_state(AbdCtx* ctx, const double* q, const double* q_now, const double* qdot) {
     CHK(check_ctx(ctx));
     size_t n = 12 * (size_t)ctx->nb * sizeof(double);
+    for (const double* a : {q, q_now, qdot})
+        if (a)
+            for (size_t i = 0; i < 12 * (size_t)ctx->nb; ++i)
+                if (!std::isfinite(a[i])) {
+                    ctx->err = "abd_set_state: non-finite entry in the state";
+                    return ABD_ERR_NAN;
+                }
     if (q) HIPCHK(hipMemcpyAsync(ctx->q, q, n, hipMemcpyHostToDevice, ctx->st));
     if (q_now) HIPCHK(hipMemcpyAsync(ctx->qnow, q_now, n, hipMemcpyHostToDevice, ctx->st));
     if (qdot) HIPCHK(hipMemcpyAsync(ctx->qdot, qdot, n, hipMemcpyHostToDevice, ctx->st));
     launch_qtilde(ctx->st, ctx->nb, ctx->qnow, ctx->qdot, ctx->P.dt, ctx->P.gravity, ctx->qtilde);
     CHK(dbg(ctx, "launch_qtilde"));
     CHK(fill_i32(ctx, ctx->C.newton_active, ctx->ns, 1));
+    CHK(fill_i32(ctx, ctx->cnt + C_ERR, 1, 0));  // a new state: forget an error reported for the old one
     HIPCHK(hipStreamSynchronize(ctx->st));
     ctx->cand_valid = false;
     ctx->assembled = ctx->solved = false;

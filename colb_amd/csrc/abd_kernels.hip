@@ -1556,7 +1556,8 @@ __global__ void k_predict_du(Geo G, int mode, double dt, const double* __restric
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= 12 * G.nb) return;
     const double target = (mode == 2) ? qtilde[i] : qnow[i] + dt * qdot[i];
-    du[i] = G.pinned[i / 12] ? 0.0 : q[i] - target;
+    const double d = q[i] - target;
+    du[i] = (G.pinned[i / 12] || !isfinite(d)) ? 0.0 : d;
 }
 
 void launch_predict_du(hipStream_t s, const Geo& G, int mode, double dt, const double* q, const double* qnow,

@@ -1130,7 +1130,8 @@ void step_once(Ctx& C) {
                 for (int k = 0; k < 12; ++k) {
                     const int i = 12 * b + k;
                     const double target = C.P.predictor == 2 ? C.qtilde[i] : C.qnow[i] + C.P.dt * C.qdot[i];
-                    dup[i] = C.q[i] - target;
+                    const double d = C.q[i] - target;
+                    dup[i] = std::isfinite(d) ? d : 0.0;
                 }
         line_search(C, dup, active, false, alpha, amax, floor_hit);
         for (int s = 0; s < C.ns; ++s) C.stats[s].predictor_alpha = alpha[s];

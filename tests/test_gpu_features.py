@@ -1,0 +1,326 @@
+"""GPU parity of the solver options and host-forced failure paths, through the C ABI, against the
+CPU FP64 oracle on the same inputs.
+
+Tolerances as in test_gpu_parity: d^2, energies, gradients and Hessian blocks within 1e-9 relative,
+line-search steps equal (halvings of a CCD bound that agrees to 1e-9), trajectories within 1e-6
+relative while both sides converge Newton.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from colb_amd import scene as S
+from colb_amd.params import north_star
+
+sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+from test_oracle_ground import DHAT, cube_on_ground, ground_params  # noqa: E402
+from test_oracle_scene import fd_params, two_cube_scene  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def O():
+    from oracle import oracle as O
+    O.build(force=False)
+    return O
+
+
+def solver(sc, p):
+    from colb_amd.abd import NewtonAbd
+    return NewtonAbd(p.dt, sc, params=p)
+
+
+def rel(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+# ------------------------------------------------------------------------------------------------
+# ground contact, both sides of the projection clamp
+# ------------------------------------------------------------------------------------------------
+@pytest.mark.parametrize("gap,hess_proj", [(0.2 * DHAT, 0), (0.7 * DHAT, 0), (0.2 * DHAT, 1)])
+def test_ground_assembly_matches_oracle(O, gap, hess_proj):
+    say("gap", gap, "hess_proj", hess_proj)
+    sc = cube_on_ground(gap, tilt_deg=(1.2, 0.0, 0.8), n=2)
+    p = ground_params(hess_proj=hess_proj)
+    rng = np.random.default_rng(7)
+    q = sc.q0 + 1e-4 * rng.normal(size=sc.q0.shape)
+    qd = 0.1 * rng.normal(size=sc.q0.shape)
+    orc, gpu = O.Oracle(sc, p), solver(sc, p)
+    orc.set_state(q, sc.q0, qd)
+    gpu.set_state(q, sc.q0, qd)
+    po, d2o = orc.detect()
+    pg, d2g = gpu.detect_collision()
+    assert sorted(map(tuple, po.tolist())) == sorted(map(tuple, pg.tolist()))
+    assert np.sum(po[:, 0] == 2) > 0
+    go, Do, pijo, offo = orc.assemble()
+    gg, Dg, pijg, offg, Eg = gpu.assemble()
+    assert Eg[0] == pytest.approx(orc.energy()[0], rel=1e-11)
+    assert rel(gg, go) < 1e-9
+    assert rel(Dg, Do) < 1e-9
+
+
+def test_penetrating_state_is_reported():
+    from colb_amd._lib import AbdError
+    sc = cube_on_ground(0.5 * DHAT)
+    p = ground_params()
+    gpu = solver(sc, p)
+    q = sc.q0.copy()
+    q[:, 1] -= 0.5 * DHAT + 1e-3
+    gpu.set_state(q, q, np.zeros_like(q))
+    assert gpu.compute_g() == np.inf
+    with pytest.raises(AbdError, match="ground plane"):
+        gpu.assemble()
+    gpu.set_state(sc.q0, sc.q0, np.zeros_like(q))
+    gpu.assemble()  # the error word was read and cleared
+
+
+@pytest.mark.parametrize("case,d0", [("gap", 0.5 * DHAT), ("touch", 0.0), ("below", -1e-16)])
+def test_ground_ccd_matches_oracle(O, case, d0):
+    sc = cube_on_ground(0.5 * DHAT, tilt_deg=(0.0, 0.0, 0.0))
+    y = sc.world_vertices()[:, 1]
+    p = ground_params(ground_offset=float(y.min() - d0))
+    du = np.zeros_like(sc.q0)
+    du[:, 1] = 1.0
+    a_o = O.Oracle(sc, p).ccd(du)[0]
+    a_g = solver(sc, p).ccd(du)[0]
+    say(case, a_o, a_g)
+    assert a_g == pytest.approx(a_o, rel=1e-12, abs=0.0)
+    if d0 <= 0:
+        assert a_g == 0.0
+
+
+# ------------------------------------------------------------------------------------------------
+# line search along a given direction, including the floor branch
+# ------------------------------------------------------------------------------------------------
+@pytest.mark.parametrize("sign", [1.0, -1.0])
+def test_line_search_along_matches_oracle(O, sign):
+    """sign -1: an ascent direction, which the reference's search ends at the floor alpha0/64."""
+    sc = two_cube_scene()
+    p = fd_params()
+    rng = np.random.default_rng(6)
+    q = sc.q0 + 1e-3 * rng.normal(size=sc.q0.shape)
+    qd = 0.1 * rng.normal(size=sc.q0.shape)
+    orc, gpu = O.Oracle(sc, p), solver(sc, p)
+    orc.set_state(q, sc.q0, qd)
+    gpu.set_state(q, sc.q0, qd)
+    du = sign * orc.solve()[0]
+    a0 = orc.ccd(du)[0]
+    ao = orc.line_search(du)[0]
+    ag = gpu.line_search_along(du)[0]
+    say("alpha0", a0, "oracle", ao, "gpu", ag)
+    if sign < 0:
+        assert ao == a0 / 64
+    assert ag == pytest.approx(ao, rel=1e-9)
+    assert rel(gpu.get_state()[0], orc.get_state()[0]) < 1e-12
+
+
+def test_line_search_through_the_plane_takes_no_step(O):
+    sc = cube_on_ground(0.5 * DHAT)
+    p = ground_params(ccd_enabled=False)
+    gpu = solver(sc, p)
+    du = np.zeros_like(sc.q0)
+    du[:, 1] = 10.0
+    assert gpu.line_search_along(du)[0] == 0.0
+    np.testing.assert_array_equal(gpu.get_state()[0], sc.q0)
+    assert O.Oracle(sc, p).line_search(du)[0] == 0.0
+
+
+# ------------------------------------------------------------------------------------------------
+# predictor, adaptive stiffness, eigenvalue mirroring: trajectories against the oracle
+# ------------------------------------------------------------------------------------------------
+TRAJ = {
+    "predictor1": lambda: (S.cubes_on_ground(lift=0.02),
+                           north_star(ground_enabled=True, dhat=0.05, k_contact=1e14, predictor=1), 20),
+    "predictor2": lambda: (S.cubes_on_ground(lift=0.02),
+                           north_star(ground_enabled=True, dhat=0.05, k_contact=1e14, predictor=2), 20),
+    "adaptive": lambda: (cube_on_ground(0.8 * DHAT, tilt_deg=(0.0, 0.0, 0.0), density=1e4),
+                         ground_params(k_contact=1e8, k_max=1e14, k_raise_gap=0.45), 25),
+    "mirror": lambda: (S.chain_net(n=3), north_star(dhat=2e-3, k_contact=1e19, hess_proj=1), 5),
+}
+
+
+@pytest.mark.parametrize("case", list(TRAJ))
+def test_option_trajectories_match_oracle(O, case):
+    sc, p, nsteps = TRAJ[case]()
+    p = p.replace(newton_tol=1e-9, cg_tol=1e-12, cg_max_iter=2000)
+    orc, gpu = O.Oracle(sc, p), solver(sc, p)
+    raises = 0
+    for k in range(nsteps):
+        so, sg = orc.step()[0], gpu.step()[0]
+        err = rel(gpu.get_state()[0], orc.get_state()[0])
+        raises += sg["k_raises"]
+        say(f"  {case} step {k} newton {so['newton_iters']}/{sg['newton_iters']} active {so['n_active']}/"
+            f"{sg['n_active']} pred {so['predictor_alpha']:.3g}/{sg['predictor_alpha']:.3g} "
+            f"k {so['k_contact']:.3g}/{sg['k_contact']:.3g} err {err:.2e}")
+        assert err < 1e-6
+        assert sg["n_active"] == so["n_active"]
+        assert sg["k_raises"] == so["k_raises"]
+        assert sg["predictor_alpha"] == pytest.approx(so["predictor_alpha"], rel=1e-9, abs=0.0)
+    np.testing.assert_array_equal(gpu.stiffness(), orc.stiffness())
+    if case == "adaptive":
+        assert raises > 0
+
+
+def test_stiffness_round_trip_and_reset():
+    sc = cube_on_ground(0.5 * DHAT, n=2)
+    p = ground_params()
+    gpu = solver(sc, p)
+    np.testing.assert_array_equal(gpu.stiffness(), p.k_contact)
+    k = np.array([2e12, 5e12])
+    gpu.set_stiffness(k)
+    np.testing.assert_array_equal(gpu.stiffness(), k)
+    gpu.set_params(p)
+    np.testing.assert_array_equal(gpu.stiffness(), p.k_contact)
+
+
+# ------------------------------------------------------------------------------------------------
+# host-forced failure paths
+# ------------------------------------------------------------------------------------------------
+def test_forced_pcg_fallback_gives_the_same_step():
+    """Debug bit 4 raises the persistent PCG's failure flag before its launch: every workgroup leaves
+    at its first barrier that does not fill at once, the host counts the fallback and reruns the
+    solve with one launch per iteration.  The result equals a run that uses that form directly."""
+    sc = S.chain_net(n=3)
+    p = north_star(dhat=2e-3, k_contact=1e19, cg_tol=1e-12, newton_tol=1e-9)
+    ref = solver(sc, p)
+    ref.set_pcg_layout(5, True)
+    forced = solver(sc, p.replace(debug=16))
+    normal = solver(sc, p)
+    for k in range(4):
+        sr, sf, sn = ref.step()[0], forced.step()[0], normal.step()[0]
+        say(k, sr["newton_iters"], sf["newton_iters"], sf["pcg_fallbacks"], sn["pcg_fallbacks"])
+        assert sf["pcg_fallbacks"] == sf["newton_iters"]
+        assert sn["pcg_fallbacks"] == 0
+    np.testing.assert_array_equal(forced.get_state()[0], ref.get_state()[0])
+    assert rel(normal.get_state()[0], ref.get_state()[0]) < 1e-6
+
+
+def test_non_finite_state_is_rejected():
+    from colb_amd._lib import AbdError
+    sc = two_cube_scene()
+    gpu = solver(sc, fd_params())
+    q = sc.q0.copy()
+    q[1, 4] = np.nan
+    with pytest.raises(AbdError, match="non-finite"):
+        gpu.set_state(q)
+    qd = np.zeros_like(q)
+    qd[0, 0] = np.inf
+    with pytest.raises(AbdError, match="non-finite"):
+        gpu.set_state(None, None, qd)
+    gpu.step()  # the context is still usable
+
+
+def test_newton_cap_is_reported():
+    sc = two_cube_scene(gap=0.01)
+    p = fd_params().replace(max_newton=1, newton_tol=1e-12)
+    st = solver(sc, p).step()[0]
+    assert st["newton_iters"] == 1 and st["converged"] == 0 and st["residual_last"] > 1e-12
+
+
+def test_capacity_growth_gives_the_same_result():
+    """The smallest initial capacities force every list to grow during the first detections."""
+    sc = S.chain_net(n=3)
+    p = north_star(dhat=0.2, k_contact=1e12, cg_tol=1e-12, newton_tol=1e-9)
+    big, small = solver(sc, p), solver(sc, p.replace(max_candidates=1))
+    for k in range(3):
+        sb, ss = big.step()[0], small.step()[0]
+        say(k, sb["n_candidates"], sb["n_active"])
+    assert sb["n_candidates"] > 1024  # beyond the smallest candidate capacity
+    np.testing.assert_array_equal(small.get_state()[0], big.get_state()[0])
+
+
+def test_corrupted_learned_output_falls_back_to_plain_pcg():
+    """A warm-start model whose weights are NaN predicts NaN: the residual guard rejects it and the
+    solve equals plain PCG."""
+    from colb_amd.learned import WarmStart
+    sc = S.cubes_on_ground(lift=0.02)
+    p = north_star(ground_enabled=True, dhat=0.05, k_contact=1e14, cg_tol=1e-12, newton_tol=1e-9)
+    plain, warm = solver(sc, p), solver(sc, p)
+    ws = WarmStart(seed=0)
+    flat = np.full(ws.mlp.n_params, np.nan, dtype=np.float32)
+    ws.mlp.set_params(flat)
+    ws.feat_mean = np.zeros(49, np.float32)
+    ws.feat_std = np.ones(49, np.float32)
+    ws.out_mean = np.zeros(12, np.float32)
+    ws.out_std = np.ones(12, np.float32)
+    warm.set_warm_start(ws)
+    for k in range(5):
+        sp, sw = plain.step()[0], warm.step()[0]
+        assert sw["newton_iters"] == sp["newton_iters"]
+    # same solve; only the norm of the right-hand side is summed in another order
+    assert rel(warm.get_state()[0], plain.get_state()[0]) < 1e-10
+
+
+# ------------------------------------------------------------------------------------------------
+# readback for viewers and the Profiller
+# ------------------------------------------------------------------------------------------------
+def test_readback_matches_oracle(O, tmp_path):
+    """compute_V, soup.x_transformed.numpy() (viewer.py:6, 46) and get_contact_points (abd.py:748-757)
+    against the oracle's vertices and a numpy restatement of the closest points."""
+    sc = two_cube_scene(gap=0.01)
+    p = fd_params()
+    orc, gpu = O.Oracle(sc, p), solver(sc, p)
+    orc.step()
+    gpu.step()
+    Vg = gpu.compute_V()
+    assert rel(Vg, orc.vertices()) < 1e-9
+    xt = gpu.soup.x_transformed.numpy()
+    assert isinstance(xt, np.ndarray) and type(xt) is np.ndarray
+    np.testing.assert_array_equal(xt, Vg)
+    pts, mag = gpu.get_contact_points()
+    pairs, d2 = gpu.detect_collision()
+    assert len(pts) == len(pairs) > 0
+    np.testing.assert_allclose(mag, np.abs(np.sqrt(d2) - p.dhat), rtol=1e-9)
+    X = Vg
+    from scipy.optimize import minimize
+    for (kind, i, j), m, dd in zip(pairs, pts, d2):
+        # closest points by a bounded minimisation over the primitives' parameters, from a grid start
+        if kind == 0:
+            P, tri = X[i], X[sc.tris[j]]
+            f = lambda w: np.sum((tri[0] + w[0] * (tri[1] - tri[0]) + w[1] * (tri[2] - tri[0]) - P) ** 2)
+            cons = [{"type": "ineq", "fun": lambda w: 1.0 - w[0] - w[1]}]
+            grid = [(u, v) for u in np.linspace(0, 1, 21) for v in np.linspace(0, 1, 21) if u + v <= 1]
+            w0 = min(grid, key=f)
+            w = minimize(f, w0, method="SLSQP", bounds=[(0, 1), (0, 1)], constraints=cons, options={"ftol": 1e-16}).x
+            c0, c1 = P, tri[0] + w[0] * (tri[1] - tri[0]) + w[1] * (tri[2] - tri[0])
+        else:
+            a, b = X[sc.edges[i]], X[sc.edges[j]]
+            f = lambda st: np.sum((a[0] + st[0] * (a[1] - a[0]) - b[0] - st[1] * (b[1] - b[0])) ** 2)
+            grid = [(u, v) for u in np.linspace(0, 1, 21) for v in np.linspace(0, 1, 21)]
+            st = minimize(f, min(grid, key=f), method="SLSQP", bounds=[(0, 1), (0, 1)], options={"ftol": 1e-16}).x
+            c0, c1 = a[0] + st[0] * (a[1] - a[0]), b[0] + st[1] * (b[1] - b[0])
+        assert np.linalg.norm(c0 - c1) == pytest.approx(np.sqrt(dd), rel=1e-6)
+        assert np.linalg.norm(m - 0.5 * (c0 + c1)) < 1e-6 * max(1.0, np.abs(X).max())
+
+
+def test_iter_log_and_profiler_match_oracle(O, tmp_path):
+    sc = two_cube_scene(gap=0.01)
+    p = fd_params().replace(newton_tol=1e-8)
+    orc, gpu = O.Oracle(sc, p), solver(sc, p)
+    for f in range(6):
+        so, sg = orc.step()[0], gpu.step()[0]
+        ro, ao, co = orc.iter_log()
+        rg, ag, cg = gpu.iter_log()
+        n = sg["newton_iters"]
+        assert n == so["newton_iters"]
+        # iterates that differ by rounding: residuals agree to 1e-6 of the step's largest
+        np.testing.assert_allclose(rg[0, :n], ro[0, :n], rtol=0, atol=1e-6 * ro[0, :n].max())
+        # steps from iterates within 1e3 tol of convergence change the energy by less than the 1e-11
+        # to which the two energies agree: accepting or reaching the floor there is a rounding tie
+        big = ro[0, :n] > 1e3 * p.newton_tol
+        np.testing.assert_allclose(ag[0, :n][big], ao[0, :n][big], rtol=1e-6)
+        assert np.all(rg[0, n:] == 0) and np.all(ag[0, n:] == 0)
+        np.testing.assert_array_equal(gpu.profiler.convergence[f, :n], rg[0, :n])
+    path = tmp_path / "profiler.npz"
+    gpu.profiler.save(str(path))
+    with np.load(path, allow_pickle=False) as z:
+        np.testing.assert_array_equal(z["convergence"], gpu.profiler.convergence)
+        np.testing.assert_array_equal(z["alphas"], gpu.profiler.alphas)

@@ -435,8 +435,11 @@ def test_learned_preconditioner_trains_and_keeps_the_solution():
 
 # --- direct solver (AbdParams.solver 1 and 2) -------------------------------------------------------
 def _batch_of_sizes():
-    """Three scenes of 8, 9 and 1 bodies: padded orders 96, 128 and 32 in one set of launches."""
-    return S.concat_scenes([S.cubes_on_ground(n=2, lift=0.02), S.chain_net(n=3), S.cubes_on_ground(n=1, lift=0.02)])
+    """Three scenes of 8, 9 and 1 bodies: padded orders 96, 128 and 32 in one set of launches (the
+    chain net lifted clear of the ground plane)."""
+    net = S.chain_net(n=3)
+    net.q0[:, 1] += 0.02 - net.world_vertices()[:, 1].min()
+    return S.concat_scenes([S.cubes_on_ground(n=2, lift=0.02), net, S.cubes_on_ground(n=1, lift=0.02)])
 
 
 DIRECT_CASES = dict(CASES, batch=lambda: (_batch_of_sizes(), north_star(ground_enabled=True, dhat=0.05,
