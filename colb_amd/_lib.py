@@ -53,6 +53,7 @@ SIGNATURES = {
     "abd_mlp_get_params": (C.c_int, [_V, _F]),
     "abd_mlp_set_loss_weights": (C.c_int, [_V, _F]),
     "abd_mlp_forward": (C.c_int, [_V, _F, C.c_int, _F]),
+    "abd_mlp_forward_static": (C.c_int, [_V, _F, C.c_int, _F]),
     "abd_mlp_gradient": (C.c_int, [_V, _F, _F, C.c_int, _F, _F]),
     "abd_mlp_train": (C.c_int, [_V, _F, _F, C.c_int, _I, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float,
                                 C.c_float, _F]),

@@ -15,6 +15,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -230,17 +231,65 @@ int mlp_reserve(Mlp* m, int rows) {
 }
 
 // forward over the packed rows already in A[0] (rows padded to 16): Z[L-1] is the output
-void mlp_forward_dev(Mlp* m, int n, hipStream_t st) {
+void mlp_forward_bufs(Mlp* m, int n, hipStream_t st, uint16_t* const* A, float* const* Z) {
     const int rows = pad16(std::max(n, 1));
     for (int l = 0; l < m->L; ++l) {
-        gemm(st, rows, m->dp[l + 1], m->dp[l], m->A[l], m->dp[l], 1, m->Wb[l], m->dp[l + 1], 1, m->Z[l], m->dp[l + 1],
+        gemm(st, rows, m->dp[l + 1], m->dp[l], A[l], m->dp[l], 1, m->Wb[l], m->dp[l + 1], 1, Z[l], m->dp[l + 1],
              m->b[l]);
         if (l + 1 < m->L) {
             const long tot = (long)rows * m->dp[l + 1];
-            hipLaunchKernelGGL(k_relu_bf16, dim3(cdivm(tot, 256)), dim3(256), 0, st, tot, n, m->dp[l + 1], m->Z[l],
-                               m->A[l + 1]);
+            hipLaunchKernelGGL(k_relu_bf16, dim3(cdivm(tot, 256)), dim3(256), 0, st, tot, n, m->dp[l + 1], Z[l],
+                               A[l + 1]);
         }
     }
+}
+
+void mlp_forward_dev(Mlp* m, int n, hipStream_t st) { mlp_forward_bufs(m, n, st, m->A.data(), m->Z.data()); }
+
+// ---------------------------------------------------------------------------------------------
+// Scratch of abd_mlp_forward_static: the staged input rows and every layer's activations, in one
+// static device array that each GPU holds (at its own address) until the process ends.  Host code
+// reaches it only through the address hipGetSymbolAddress gives on that device.
+// ---------------------------------------------------------------------------------------------
+constexpr size_t MLP_STATIC_BYTES = 64ull << 20;
+static __device__ __align__(256) unsigned char scratch[MLP_STATIC_BYTES];
+
+constexpr int MLP_MAX_DEVICES = 64;
+static void* g_scratch_ptr[MLP_MAX_DEVICES];
+static std::mutex g_scratch_mu;
+
+// the current device's address of `scratch`, looked up once per device
+static int scratch_address(Mlp* m, void** out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess || dev < 0 || dev >= MLP_MAX_DEVICES) {
+        m->err = "static scratch: no usable current device";
+        return ABD_ERR_HIP;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        e = hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) {
+            m->err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+            return ABD_ERR_HIP;
+        }
+        g_scratch_ptr[dev] = ptr;
+    }
+    m->dev = dev;
+    *out = g_scratch_ptr[dev];
+    return ABD_OK;
+}
+
+static size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+
+// bytes of the static scratch a forward of n rows uses: staged input, then A[0..L], then Z[0..L-1]
+static size_t static_bytes(const Mlp* m, int n) {
+    const size_t rows = (size_t)pad16(std::max(n, 1));
+    size_t b = align256((size_t)n * m->d[0] * sizeof(float));
+    for (int l = 0; l <= m->L; ++l) b += align256(rows * m->dp[l] * sizeof(uint16_t));
+    for (int l = 0; l < m->L; ++l) b += align256(rows * m->dp[l + 1] * sizeof(float));
+    return b;
 }
 
 // gradients of the batch loss into gW/gb (targets already packed in T); loss into *loss_dev
@@ -317,6 +366,7 @@ int abd_mlp_create(const int* dims, int n_layers, AbdMlp** out) {
     }
     if ((rc = malloc_dev(m, &m->colw, (size_t)m->dp[L])) != ABD_OK) return rc;
     if ((rc = malloc_dev(m, &m->loss_dev, 4096)) != ABD_OK) return rc;
+    if ((rc = scratch_address(m, &m->scratch)) != ABD_OK) return rc;
     return mlp_reserve(m, 1024);
 }
 
@@ -405,6 +455,57 @@ int abd_mlp_forward(AbdMlp* m, const float* X, int n, float* Y) {
     (void)hipFree(dX);
     for (int i = 0; i < n; ++i)
         for (int j = 0; j < m->d[m->L]; ++j) Y[(size_t)i * m->d[m->L] + j] = z[(size_t)i * m->dp[m->L] + j];
+    return ABD_OK;
+}
+
+// abd_mlp_forward without any allocation: the staged input and the activations live in the
+// device's static scratch, the output rows go from the last pre-activation straight to Y
+int abd_mlp_forward_static(AbdMlp* m, const float* X, int n, float* Y) {
+    if (!m || !X || !Y || n < 1) return ABD_ERR_ARG;
+    if (!m->scratch) {
+        m->err = "static scratch address unknown (model not created)";
+        return ABD_ERR_STATE;
+    }
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev != m->dev) {
+        m->err = "abd_mlp_forward_static: the current device is not the model's";
+        return ABD_ERR_STATE;
+    }
+    const size_t need = static_bytes(m, n);
+    if (need > MLP_STATIC_BYTES) {
+        int hi = n;
+        while (hi > 1 && static_bytes(m, hi) > MLP_STATIC_BYTES) hi /= 2;
+        int lo = hi;
+        while (static_bytes(m, lo + 1) <= MLP_STATIC_BYTES) ++lo;
+        m->err = "abd_mlp_forward_static: " + std::to_string(n) + " rows need " + std::to_string(need >> 10) +
+                 " KiB of static scratch (" + std::to_string(MLP_STATIC_BYTES >> 20) + " MiB): at most " +
+                 std::to_string(lo) + " rows for this model";
+        return ABD_ERR_CAPACITY;
+    }
+    const int L = m->L, rows = pad16(n);
+    unsigned char* base = (unsigned char*)m->scratch;
+    size_t off = 0;
+    float* dX = (float*)(base + off);
+    off += align256((size_t)n * m->d[0] * sizeof(float));
+    uint16_t* A[9];
+    float* Z[8];
+    for (int l = 0; l <= L; ++l) {
+        A[l] = (uint16_t*)(base + off);
+        off += align256((size_t)rows * m->dp[l] * sizeof(uint16_t));
+    }
+    for (int l = 0; l < L; ++l) {
+        Z[l] = (float*)(base + off);
+        off += align256((size_t)rows * m->dp[l + 1] * sizeof(float));
+    }
+    MCHK(hipMemcpyAsync(dX, X, (size_t)n * m->d[0] * sizeof(float), hipMemcpyHostToDevice, m->st));
+    const long tot = (long)rows * m->dp[0];
+    hipLaunchKernelGGL(k_pack_rows, dim3(cdivm(tot, 256)), dim3(256), 0, m->st, n, rows, m->d[0], m->dp[0], dX,
+                       (const int*)nullptr, A[0]);
+    mlp_forward_bufs(m, n, m->st, A, Z);
+    MCHK(hipGetLastError());
+    MCHK(hipMemcpy2DAsync(Y, (size_t)m->d[L] * sizeof(float), Z[L - 1], (size_t)m->dp[L] * sizeof(float),
+                          (size_t)m->d[L] * sizeof(float), (size_t)n, hipMemcpyDeviceToHost, m->st));
+    MCHK(hipStreamSynchronize(m->st));
     return ABD_OK;
 }
 

@@ -29,6 +29,8 @@ struct AbdMlp {
     float* loss_dev = nullptr;
     int rows_cap = 0;
     long adam_t = 0;
+    int dev = -1;                   // device the model was created on
+    void* scratch = nullptr;        // that device's address of the static scratch (abd_mlp_forward_static)
 };
 
 namespace abd {
@@ -43,5 +45,7 @@ __device__ __forceinline__ uint16_t f2bf(float f) {
 
 int mlp_reserve(Mlp* m, int rows);
 void mlp_forward_dev(Mlp* m, int n, hipStream_t s);
+// forward over packed rows in A[0] with caller-given activation buffers A[0..L], Z[0..L-1]
+void mlp_forward_bufs(Mlp* m, int n, hipStream_t s, uint16_t* const* A, float* const* Z);
 void mlp_backward_dev(Mlp* m, int n, float* loss_dev, hipStream_t s);
 }  // namespace abd

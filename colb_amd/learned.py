@@ -96,6 +96,14 @@ class Mlp:
         self._check(_lib.load().abd_mlp_forward(self._h, _fptr(X), X.shape[0], _fptr(Y)), "abd_mlp_forward")
         return Y
 
+    def forward_static(self, X):
+        """forward() through abd_mlp_forward_static: no device allocation (static scratch per GPU)."""
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        Y = np.zeros((X.shape[0], self.dims[-1]), dtype=np.float32)
+        self._check(_lib.load().abd_mlp_forward_static(self._h, _fptr(X), X.shape[0], _fptr(Y)),
+                    "abd_mlp_forward_static")
+        return Y
+
     def gradient(self, X, T):
         X = np.ascontiguousarray(X, dtype=np.float32)
         T = np.ascontiguousarray(T, dtype=np.float32)

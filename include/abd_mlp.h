@@ -30,6 +30,16 @@ int abd_mlp_get_params(const AbdMlp* m, float* flat);
 int abd_mlp_set_loss_weights(AbdMlp* m, const float* w);
 /* Y (n x d_L) = model(X (n x d_0)), host arrays. */
 int abd_mlp_forward(AbdMlp* m, const float* X, int n, float* Y);
+/* No counterpart in the reference: abd_mlp_forward without any allocation, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * it).  Same arguments and result as abd_mlp_forward.  The staged input rows and every layer's
+ * activations live in one static device array of 64 MiB per GPU (held until the process ends), so n
+ * is limited by the model's widths: per row 4 d_0 bytes of input plus 2 d_l (l = 0..L) and 4 d_l
+ * (l = 1..L) bytes of activations, widths padded to 16 and each buffer to 256 bytes (about 24 000 rows
+ * for 49-128-128-128-12); larger n returns ABD_ERR_CAPACITY with the limit in the message.  Every call
+ * on one GPU shares that array: calls that overlap on one GPU race.  The current device must be the
+ * one the model was created on. */
+int abd_mlp_forward_static(AbdMlp* m, const float* X, int n, float* Y);
 /* Loss of one batch and its gradient in the flat layout (no update). */
 int abd_mlp_gradient(AbdMlp* m, const float* X, const float* T, int n, float* loss, float* grad_flat);
 /* Adam over `steps` minibatches of `batch` rows given by perm[steps*batch]; losses[steps] gets
