@@ -1143,6 +1143,10 @@ int step_once(AbdCtx* ctx) {
     }
     launch_forward(ctx->st, ctx->nb, ctx->P.dt, ctx->q, ctx->qnow, ctx->qdot);
     CHK(dbg(ctx, "launch_forward"));
+    // the inertial target of the next step, so that the hooks called between steps (compute_g,
+    // assemble) see the new state's potential, as the reference's compute_g does
+    launch_qtilde(ctx->st, ctx->nb, ctx->qnow, ctx->qdot, ctx->P.dt, ctx->P.gravity, ctx->qtilde);
+    CHK(dbg(ctx, "launch_qtilde"));
     HIPCHK(hipGetLastError());
     return ABD_OK;
 }

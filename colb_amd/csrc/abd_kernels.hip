@@ -1537,7 +1537,10 @@ __global__ void k_ls_decide(int ns, AbdParams P, SceneCtl c, int floor_accept, i
     } else if (c.alpha[s] * 0.5 < P.ls_floor * c.alpha0[s]) {
         c.ls_done[s] = 1;
         c.ls_floor[s] = 1;
-        if (!floor_accept || !isfinite(c.E1[s])) c.alpha[s] = 0.0;
+        if (!floor_accept || !isfinite(c.E1[s])) {
+            c.alpha[s] = 0.0;  // no step: the accepted iterate is q itself
+            c.E1[s] = c.E0[s];
+        }
     } else {
         c.alpha[s] *= 0.5;
         atomicOr(pending, 1);

@@ -1069,7 +1069,10 @@ void line_search(Ctx& C, const std::vector<double>& du, const std::vector<char>&
             else if (alpha[s] * 0.5 < C.P.ls_floor * a0[s]) {
                 ls_done[s] = 1;
                 floor_hit[s] = 1;
-                if (!floor_accept || !std::isfinite(E1[s])) alpha[s] = 0.0;
+                if (!floor_accept || !std::isfinite(E1[s])) {
+                    alpha[s] = 0.0;  // no step: the accepted iterate is q itself
+                    C.stats[s].energy = E0[s];
+                }
             } else alpha[s] *= 0.5;
         }
     }
