@@ -44,6 +44,7 @@ class AbdParams(C.Structure):
         ("hess_proj", C.c_int),
         ("k_max", C.c_double),
         ("k_raise_gap", C.c_double),
+        ("cand_margin", C.c_double),
     ]
 
 
@@ -81,6 +82,8 @@ class AbdStepStats(C.Structure):
         ("k_raises", C.c_int),
         ("pcg_fallbacks", C.c_int),
         ("predictor_alpha", C.c_double),
+        ("traversals", C.c_int),
+        ("detections", C.c_int),
     ]
 
     def as_dict(self):

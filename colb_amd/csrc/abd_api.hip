@@ -91,6 +91,14 @@ struct AbdCtx {
     int bpair_cap = 0, n_bpairs = 0;
     int max_queries = 0;     // largest per-body-pair query count (2 max verts + max edges)
 
+    // candidate cache (AbdParams.cand_margin > 0): the box-level pairs of the last traversal and the
+    // path it was taken over
+    int cache_cap = 0, n_cache = 0;
+    uint64_t *cache = nullptr, *cache_s = nullptr;
+    int *cache_flag = nullptr, *cache_pos = nullptr;
+    double4 *Xc0 = nullptr, *Xc1 = nullptr;
+    bool cache_valid = false;
+    int step_traversals = 0, step_detections = 0;
     // candidates
     int cand_cap = 0, n_cand = 0;
     uint64_t *cand = nullptr, *cand_s = nullptr, *act = nullptr;
@@ -164,7 +172,7 @@ struct AbdCtx {
 namespace {
 
 enum { C_COUNT = 0, C_OVERFLOW = 1, C_NINC = 2, C_NPK = 3, C_NUNIQ = 4, C_ANY = 5, C_PENDING = 6, C_NACT = 7, C_ERR = 8,
-       C_N = 9 };
+       C_OUTSIDE = 9, C_N = 10 };
 enum { R_RES = 0, R_ALPHA, R_ALPHA0, R_FLOOR, R_ACTIVE, R_CG, R_E1, R_NACT, R_NCAND, R_KC, R_RAISES, R_PALPHA, R_N };
 
 #define HIPCHK(expr)                                                                               \
@@ -425,8 +433,23 @@ int grow_body_pairs(AbdCtx* ctx, int cap) {
     return ABD_OK;
 }
 
-int broad_phase(AbdCtx* ctx, const double4* X0, const double4* X1, const double* du) {
-    const double infl = ctx->P.dhat;
+int grow_cache(AbdCtx* ctx, int cap) {
+    if (cap <= ctx->cache_cap) return ABD_OK;
+    cap = std::max(cap, ctx->cache_cap * 2);
+    CHK(dfree(ctx, ctx->cache)); CHK(dfree(ctx, ctx->cache_s)); CHK(dfree(ctx, ctx->cache_flag));
+    CHK(dfree(ctx, ctx->cache_pos));
+    CHK(dalloc(ctx, &ctx->cache, cap)); CHK(dalloc(ctx, &ctx->cache_s, cap)); CHK(dalloc(ctx, &ctx->cache_flag, cap));
+    CHK(dalloc(ctx, &ctx->cache_pos, cap));
+    ctx->cache_cap = cap;
+    return ABD_OK;
+}
+
+// BVH traversal over the path X0 -> X1 (q -> q - du): the pairs meeting the candidate rule with
+// inflation infl (box_only: those whose boxes overlap) into the unsorted list of the target
+// (the candidate buffers, or the cache's), growing it as needed; *n receives the count
+int traverse(AbdCtx* ctx, const double4* X0, const double4* X1, const double* du, double infl, int box_only,
+             bool to_cache, int* n_out) {
+    ctx->step_traversals += 1;
     tic(ctx, "broad_phase_pairs");
     launch_body_boxes(ctx->st, ctx->G, ctx->M, X0, X1, 0.5 * infl, ctx->body_box);
     CHK(dbg(ctx, "launch_body_boxes"));
@@ -450,14 +473,16 @@ int broad_phase(AbdCtx* ctx, const double4* X0, const double4* X1, const double*
     CHK(dbg(ctx, "launch_world_nodes"));
     toc(ctx);
     for (int attempt = 0; attempt < 8; ++attempt) {
+        uint64_t* out = to_cache ? ctx->cache : ctx->cand;
+        const int cap = to_cache ? ctx->cache_cap : ctx->cand_cap;
         // algorithmic bytes: positions once, primitive indices once, candidate keys out
         tic(ctx, "broad_phase_query", (double)ctx->nv * 24 + (2.0 * ctx->ne + 3.0 * ctx->nt) * 4);
         HIPCHK(hipMemsetAsync(ctx->cnt + C_COUNT, 0, 2 * sizeof(int), ctx->st));
         launch_pair_queries(ctx->st, ctx->G, ctx->M, ctx->n_bpairs, ctx->max_queries, ctx->bpairs, X0, X1,
-                            ctx->body_box, infl, ctx->cand, ctx->cand_cap, ctx->cnt + C_COUNT, ctx->cnt + C_OVERFLOW);
+                            ctx->body_box, infl, out, cap, ctx->cnt + C_COUNT, ctx->cnt + C_OVERFLOW, box_only);
         CHK(dbg(ctx, "launch_pair_queries"));
-        launch_ground_queries(ctx->st, ctx->G, X0, X1, ctx->P, infl, ctx->cand, ctx->cand_cap, ctx->cnt + C_COUNT,
-                              ctx->cnt + C_OVERFLOW);
+        launch_ground_queries(ctx->st, ctx->G, X0, X1, ctx->P, infl, out, cap, ctx->cnt + C_COUNT,
+                              ctx->cnt + C_OVERFLOW, box_only);
         CHK(dbg(ctx, "launch_ground_queries"));
         HIPCHK(hipGetLastError());
         toc(ctx);
@@ -468,27 +493,89 @@ int broad_phase(AbdCtx* ctx, const double4* X0, const double4* X1, const double*
         }
         const int n = ctx->h_cnt[C_COUNT];
         if (ctx->P.reserved0 & 4)
-            std::fprintf(stderr, "[abd] broad phase (%s): %d body pairs, %d candidates\n", du ? "swept" : "static",
-                         ctx->n_bpairs, n);
-        if (n <= ctx->cand_cap) {
-            ctx->n_cand = n;
-            break;
+            std::fprintf(stderr, "[abd] broad phase (%s%s): %d body pairs, %d pairs\n", du ? "swept" : "static",
+                         to_cache ? ", cache" : "", ctx->n_bpairs, n);
+        if (n <= cap) {
+            *n_out = n;
+            return ABD_OK;
         }
         if (attempt == 7) {
             ctx->err = "candidate list did not fit";
             return ABD_ERR_CAPACITY;
         }
-        CHK(grow_candidates(ctx, n + n / 2));
+        CHK(to_cache ? grow_cache(ctx, n + n / 2) : grow_candidates(ctx, n + n / 2));
     }
-    tic(ctx, "candidate_sort");
-    CHK(sort_keys(ctx, ctx->cand, ctx->cand_s, ctx->n_cand, 48 + bits_for(ctx->ns + 1)));
+    return ABD_OK;
+}
+
+// per-scene ranges of the sorted candidates
+int cand_ranges(AbdCtx* ctx) {
     HIPCHK(hipMemsetAsync(ctx->cand_lo, 0, sizeof(int) * ctx->ns, ctx->st));
     HIPCHK(hipMemsetAsync(ctx->cand_hi, 0, sizeof(int) * ctx->ns, ctx->st));
     launch_cand_ranges(ctx->st, ctx->n_cand, ctx->cand_s, ctx->cand_lo, ctx->cand_hi);
     CHK(dbg(ctx, "launch_cand_ranges"));
     HIPCHK(hipGetLastError());
-    toc(ctx);
     return ABD_OK;
+}
+
+// the candidates over X0 -> X1 from the cache: rebuilt by a widened traversal when a vertex of the
+// path leaves the cached region, then filtered by the exact rule (sorted order kept)
+int cached_broad_phase(AbdCtx* ctx, const double4* X0, const double4* X1, const double* du) {
+    const double m = ctx->P.cand_margin;
+    if (ctx->cache_valid) {
+        tic(ctx, "cache_check", (double)ctx->nv * 128);
+        CHK(fill_i32(ctx, ctx->cnt + C_OUTSIDE, 1, 0));
+        launch_cache_contains(ctx->st, ctx->nv, ctx->Xc0, ctx->Xc1, X0, X1, 0.5 * m, ctx->cnt + C_OUTSIDE);
+        CHK(dbg(ctx, "launch_cache_contains"));
+        toc(ctx);
+        CHK(sync_counters(ctx));
+        if (ctx->h_cnt[C_OUTSIDE]) ctx->cache_valid = false;
+    }
+    if (!ctx->cache_valid) {
+        int n = 0;
+        CHK(traverse(ctx, X0, X1, du, ctx->P.dhat + 2.0 * m, 1, true, &n));
+        ctx->n_cache = n;
+        tic(ctx, "candidate_sort");
+        CHK(sort_keys(ctx, ctx->cache, ctx->cache_s, n, 48 + bits_for(ctx->ns + 1)));
+        toc(ctx);
+        HIPCHK(hipMemcpyAsync(ctx->Xc0, X0, ctx->nv * sizeof(double4), hipMemcpyDeviceToDevice, ctx->st));
+        HIPCHK(hipMemcpyAsync(ctx->Xc1, X1, ctx->nv * sizeof(double4), hipMemcpyDeviceToDevice, ctx->st));
+        ctx->cache_valid = true;
+    }
+    const int nc = ctx->n_cache;
+    CHK(grow_candidates(ctx, nc));
+    ctx->n_cand = 0;
+    if (nc > 0) {
+        // algorithmic bytes: the cached key, four vertices at both ends, the flag
+        tic(ctx, "broad_phase_filter", (double)nc * (8 + 8 * 32 + 4));
+        launch_cache_filter(ctx->st, ctx->G, nc, ctx->cache_s, X0, X1, ctx->P, ctx->P.dhat, ctx->cache_flag);
+        CHK(dbg(ctx, "launch_cache_filter"));
+        CHK(excl_scan(ctx, ctx->cache_flag, ctx->cache_pos, nc));
+        launch_compact(ctx->st, nc, ctx->cache_flag, ctx->cache_pos, ctx->cache_s, ctx->cand_s);
+        CHK(dbg(ctx, "launch_compact"));
+        HIPCHK(hipMemcpyAsync(ctx->cnt + C_COUNT, ctx->cache_pos + nc - 1, sizeof(int), hipMemcpyDeviceToDevice, ctx->st));
+        HIPCHK(hipMemcpyAsync(ctx->cnt + C_OVERFLOW, ctx->cache_flag + nc - 1, sizeof(int), hipMemcpyDeviceToDevice,
+                              ctx->st));
+        toc(ctx);
+        CHK(sync_counters(ctx));
+        ctx->n_cand = ctx->h_cnt[C_COUNT] + ctx->h_cnt[C_OVERFLOW];
+    }
+    if (ctx->P.reserved0 & 4)
+        std::fprintf(stderr, "[abd] cached broad phase (%s): %d cached, %d candidates\n", du ? "swept" : "static",
+                     nc, ctx->n_cand);
+    return cand_ranges(ctx);
+}
+
+int broad_phase(AbdCtx* ctx, const double4* X0, const double4* X1, const double* du) {
+    ctx->step_detections += 1;
+    if (ctx->P.cand_margin > 0.0) return cached_broad_phase(ctx, X0, X1, du);
+    int n = 0;
+    CHK(traverse(ctx, X0, X1, du, ctx->P.dhat, 0, false, &n));
+    ctx->n_cand = n;
+    tic(ctx, "candidate_sort");
+    CHK(sort_keys(ctx, ctx->cand, ctx->cand_s, ctx->n_cand, 48 + bits_for(ctx->ns + 1)));
+    toc(ctx);
+    return cand_ranges(ctx);
 }
 
 int static_broad_phase(AbdCtx* ctx) {
@@ -647,6 +734,7 @@ int check_params(AbdCtx* ctx, const AbdParams* p) {
         return bad("ground_normal must be non-zero");
     if (p->predictor < 0 || p->predictor > 3) return bad("predictor must be 0, 1, 2 or 3");
     if (p->hess_proj != 0 && p->hess_proj != 1) return bad("hess_proj must be 0 or 1");
+    if (!finite(p->cand_margin)) return bad("cand_margin must be finite");
     if (!finite(p->k_max)) return bad("k_max must be finite");
     if (p->k_max > p->k_contact && (!(p->k_raise_gap > 0.0) || !(p->k_raise_gap < 1.0)))
         return bad("k_raise_gap must be in (0, 1)");
@@ -1087,6 +1175,7 @@ int step_once(AbdCtx* ctx) {
     ctx->stats.assign(ns, AbdStepStats{});
     for (auto& s : ctx->stats) { s.alpha_min = 1.0; s.ccd_alpha_min = 1.0; }
     ctx->pcg_fallbacks = 0;
+    ctx->step_traversals = ctx->step_detections = 0;
     hipLaunchKernelGGL(k_fill_u64, dim3(cdiv(ctx->nb, 128)), dim3(128), 0, ctx->st, ctx->nb,
                        (unsigned long long*)ctx->C.dmin_prev, (unsigned long long)0x7FF0000000000000ull);  // +inf
     CHK(fill_i32(ctx, ctx->C.k_raises, ns, 0));
@@ -1136,6 +1225,8 @@ int step_once(AbdCtx* ctx) {
             ctx->stats[s].k_raises = (int)R[R_RAISES];
             ctx->stats[s].predictor_alpha = R[R_PALPHA];
             ctx->stats[s].pcg_fallbacks = ctx->pcg_fallbacks;
+            ctx->stats[s].traversals = ctx->step_traversals;
+            ctx->stats[s].detections = ctx->step_detections;
             active[s] = now[s];
             any |= now[s] != 0;
         }
@@ -1473,7 +1564,7 @@ int abd_create(const AbdSceneDesc* d, const AbdParams* p, AbdCtx** out) {
     A(scene_t0, ns + 1); U(scene_t0, stt.data(), ns + 1);
     A(q, 12 * nb); A(qnow, 12 * nb); A(qdot, 12 * nb); A(qtilde, 12 * nb); A(qtmp, 12 * nb);
     A(du, 12 * nb); A(grad, 12 * nb); A(diag, 144 * nb); A(pinv, 144 * nb); A(e_body, nb);
-    A(X0, nv); A(X1, nv); A(Xt, nv);
+    A(X0, nv); A(X1, nv); A(Xt, nv); A(Xc0, nv); A(Xc1, nv);
     A(body_v0, nb + 1); U(body_v0, bv0.data(), nb + 1);
     A(body_e0, nb + 1); U(body_e0, be0.data(), nb + 1);
     A(body_t0, nb + 1); U(body_t0, bt0.data(), nb + 1);
@@ -1526,6 +1617,7 @@ int abd_create(const AbdSceneDesc* d, const AbdParams* p, AbdCtx** out) {
     }
     if ((rc = grow_candidates(ctx, std::max(1024, std::min(p->max_candidates > 0 ? p->max_candidates : (1 << 20), 1 << 26)))) != ABD_OK) return fail(rc);
     if ((rc = grow_active(ctx, 1024)) != ABD_OK) return fail(rc);
+    if ((rc = grow_cache(ctx, 1024)) != ABD_OK) return fail(rc);
     if ((rc = grow_pairs(ctx, 1024)) != ABD_OK) return fail(rc);
     if ((rc = grow_body_pairs(ctx, std::max(1024, std::min(16 * nb, 1 << 24)))) != ABD_OK) return fail(rc);
     if (hipMemset(ctx->bp_cnt, 0, (nb + 1) * sizeof(int)) != hipSuccess) {
@@ -1577,6 +1669,7 @@ int abd_set_params(AbdCtx* ctx, const AbdParams* p) {
     double l = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
     if (l > 0) { n[0] /= l; n[1] /= l; n[2] /= l; }
     ctx->cand_valid = false;
+    ctx->cache_valid = false;  // margin, dhat or the ground plane may have changed
     ctx->assembled = ctx->solved = false;
     return reset_stiffness(ctx);
 }

@@ -306,15 +306,16 @@ struct QueryCtx {
     int cap;
     int* count;
     int* overflow;
+    int box_only;  // keep every leaf pair whose boxes overlap (the candidate cache), not the full rule
 };
 
 
 // Exact candidate rule on one leaf pair (global ids; PT: vertex i, triangle j; EE: edges i < j):
 // the FP64 swept boxes overlap (point box inflated by infl, edge boxes by infl/2 each, triangle
 // boxes not inflated; the oracle's broad_phase builds the same boxes with the same operations) and
-// may_reach holds.  Keys of pairs that pass are appended to the candidate list.
-__device__ __forceinline__ void leaf_rule(const QueryCtx& Q, int s, int kind, int i, int j) {
-    const Geo& G = Q.G;
+// may_reach holds.  box_only: the box test alone.
+__device__ __forceinline__ bool leaf_pass(const Geo& G, const double4* __restrict__ X0, const double4* __restrict__ X1,
+                                          double infl, int box_only, int kind, int i, int j) {
     int vid[4];
     if (kind == K_PT) {
         const int4 tr = G.tris[j];
@@ -325,9 +326,9 @@ __device__ __forceinline__ void leaf_rule(const QueryCtx& Q, int s, int kind, in
     }
     D3 a[4], b[4];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) { a[k] = d3of(Q.X0[vid[k]]); b[k] = d3of(Q.X1[vid[k]]); }
+    for (int k = 0; k < 4; ++k) { a[k] = d3of(X0[vid[k]]); b[k] = d3of(X1[vid[k]]); }
     const int split = (kind == K_PT) ? 1 : 2;
-    const double r0 = (kind == K_PT) ? Q.infl : 0.5 * Q.infl, r1 = (kind == K_PT) ? 0.0 : 0.5 * Q.infl;
+    const double r0 = (kind == K_PT) ? infl : 0.5 * infl, r1 = (kind == K_PT) ? 0.0 : 0.5 * infl;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         double lo0 = DBL_MAX, hi0 = -DBL_MAX, lo1 = DBL_MAX, hi1 = -DBL_MAX;
@@ -341,9 +342,15 @@ __device__ __forceinline__ void leaf_rule(const QueryCtx& Q, int s, int kind, in
             }
         }
         lo0 -= r0; hi0 += r0; lo1 -= r1; hi1 += r1;
-        if (lo0 > hi1 || lo1 > hi0) return;
+        if (lo0 > hi1 || lo1 > hi0) return false;
     }
-    if (!may_reach(kind, a, b, Q.infl)) return;
+    return box_only || may_reach(kind, a, b, infl);
+}
+
+// appends the key of a leaf pair that passes the rule to the candidate list
+__device__ __forceinline__ void leaf_rule(const QueryCtx& Q, int s, int kind, int i, int j) {
+    const Geo& G = Q.G;
+    if (!leaf_pass(G, Q.X0, Q.X1, Q.infl, Q.box_only, kind, i, j)) return;
     const uint64_t key = (kind == K_PT) ? make_key(s, K_PT, i - G.scene_v0[s], j - G.scene_t0[s])
                                         : make_key(s, K_EE, i - G.scene_e0[s], j - G.scene_e0[s]);
     const int slot = atomicAdd(Q.count, 1);
@@ -482,27 +489,37 @@ __global__ __launch_bounds__(64) void k_pair_queries(QueryCtx Q, const int2* __r
 
 void launch_pair_queries(hipStream_t s, const Geo& G, const BodyMesh& M, int npairs, int max_queries,
                          const int2* pairs, const double4* X0, const double4* X1, const float* bbox, double infl,
-                         uint64_t* cand, int cap, int* count, int* overflow) {
+                         uint64_t* cand, int cap, int* count, int* overflow, int box_only) {
     if (npairs <= 0 || max_queries <= 0) return;
-    QueryCtx Q{G, M, X0, X1, infl, cand, cap, count, overflow};
+    QueryCtx Q{G, M, X0, X1, infl, cand, cap, count, overflow, box_only};
     const int chunks = (max_queries + 63) / 64;
     const long long nitems = (long long)npairs * chunks;
     const long long blocks = 8 * ((nitems + 7) / 8);
     hipLaunchKernelGGL(k_pair_queries, dim3((unsigned)blocks), dim3(64), 0, s, Q, pairs, npairs, chunks, bbox);
 }
 
-// point-ground candidates (one thread per vertex): min(d(X0), d(X1)) < dhat
+// point-ground candidates (one thread per vertex): min(d(X0), d(X1)) < dhat; box_only (the
+// candidate cache): the lowest point of the vertex's box over the path comes within infl
 __global__ void k_ground_queries(Geo G, const double4* __restrict__ X0, const double4* __restrict__ X1, AbdParams P,
-                                 double infl, uint64_t* cand, int cap, int* count, int* overflow) {
+                                 double infl, uint64_t* cand, int cap, int* count, int* overflow, int box_only) {
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= G.nv) return;
     const int b = (int)G.rest[v].w;
     if (G.pinned[b]) return;
     const double4 a = X0[v], c = X1[v];
     const double* n = P.ground_normal;
-    const double d0 = n[0] * a.x + n[1] * a.y + n[2] * a.z - P.ground_offset;
-    const double d1 = n[0] * c.x + n[1] * c.y + n[2] * c.z - P.ground_offset;
-    if (!(fmin(d0, d1) < infl)) return;
+    double dmin;
+    if (box_only) {
+        const double pa[3] = {a.x, a.y, a.z}, pc[3] = {c.x, c.y, c.z};
+        dmin = -P.ground_offset;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) dmin += fmin(n[k] * fmin(pa[k], pc[k]), n[k] * fmax(pa[k], pc[k]));
+    } else {
+        const double d0 = n[0] * a.x + n[1] * a.y + n[2] * a.z - P.ground_offset;
+        const double d1 = n[0] * c.x + n[1] * c.y + n[2] * c.z - P.ground_offset;
+        dmin = fmin(d0, d1);
+    }
+    if (!(dmin < infl)) return;
     const int s = G.bscene[b];
     const int slot = atomicAdd(count, 1);
     if (slot < cap) cand[slot] = make_key(s, K_PG, v - G.scene_v0[s], 0);
@@ -510,10 +527,76 @@ __global__ void k_ground_queries(Geo G, const double4* __restrict__ X0, const do
 }
 
 void launch_ground_queries(hipStream_t s, const Geo& G, const double4* X0, const double4* X1, const AbdParams& P,
-                           double infl, uint64_t* cand, int cap, int* count, int* overflow) {
+                           double infl, uint64_t* cand, int cap, int* count, int* overflow, int box_only) {
     if (!P.ground_enabled || G.nv <= 0) return;
     hipLaunchKernelGGL(k_ground_queries, dim3(cdivb(G.nv, 256)), dim3(256), 0, s, G, X0, X1, P, infl, cand, cap,
-                       count, overflow);
+                       count, overflow, box_only);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Candidate cache.  A traversal over a path Xc0 -> Xc1 with every box widened by the margin m
+// (query inflation dhat + 2m, box-only leaves) lists every pair whose boxes can overlap on any path
+// each of whose vertex positions stays within the margin of that vertex's box over Xc0 -> Xc1.
+// A later path X0 -> X1 inside that region (every vertex's end positions within m/2 of its cached
+// box, the other half covering rounding) needs no traversal: the cached list, filtered by the exact
+// rule above, is its candidate list, in the same sorted order.
+// ---------------------------------------------------------------------------------------------
+__global__ void k_cache_contains(int nv, const double4* __restrict__ Xc0, const double4* __restrict__ Xc1,
+                                 const double4* __restrict__ X0, const double4* __restrict__ X1, double m,
+                                 int* __restrict__ outside) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nv) return;
+    const double4 c0 = Xc0[v], c1 = Xc1[v], a = X0[v], b = X1[v];
+    const double cl[3] = {fmin(c0.x, c1.x) - m, fmin(c0.y, c1.y) - m, fmin(c0.z, c1.z) - m};
+    const double ch[3] = {fmax(c0.x, c1.x) + m, fmax(c0.y, c1.y) + m, fmax(c0.z, c1.z) + m};
+    const double pa[3] = {a.x, a.y, a.z}, pb[3] = {b.x, b.y, b.z};
+    bool in = true;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) in = in && pa[k] >= cl[k] && pa[k] <= ch[k] && pb[k] >= cl[k] && pb[k] <= ch[k];
+    if (!in) atomicOr(outside, 1);
+}
+
+void launch_cache_contains(hipStream_t s, int nv, const double4* Xc0, const double4* Xc1, const double4* X0,
+                           const double4* X1, double m, int* outside) {
+    if (nv <= 0) return;
+    hipLaunchKernelGGL(k_cache_contains, dim3(cdivb(nv, 256)), dim3(256), 0, s, nv, Xc0, Xc1, X0, X1, m, outside);
+}
+
+// flag[c] = 1 when cached pair c meets the exact candidate rule on X0 -> X1 (ground pairs: the
+// vertex comes within infl of the plane)
+__global__ void k_cache_filter(Geo G, int n, const uint64_t* __restrict__ keys, const double4* __restrict__ X0,
+                               const double4* __restrict__ X1, AbdParams P, double infl, int* __restrict__ flag) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    const uint64_t key = keys[c];
+    const int s = key_scene(key), kind = key_kind(key);
+    const int il = opaque(key_il(key)), jl = opaque(key_jl(key));
+    bool ok = false;
+    if (s < G.ns) {
+        if (kind == K_PT) {
+            const int i = G.scene_v0[s] + il, j = G.scene_t0[s] + jl;
+            ok = i < G.scene_v0[s + 1] && j < G.scene_t0[s + 1] && leaf_pass(G, X0, X1, infl, 0, K_PT, i, j);
+        } else if (kind == K_EE) {
+            const int i = G.scene_e0[s] + il, j = G.scene_e0[s] + jl;
+            ok = i < G.scene_e0[s + 1] && j < G.scene_e0[s + 1] && leaf_pass(G, X0, X1, infl, 0, K_EE, i, j);
+        } else if (kind == K_PG) {
+            const int v = G.scene_v0[s] + il;
+            if (v < G.scene_v0[s + 1]) {
+                const double4 a = X0[v], b = X1[v];
+                const double* nr = P.ground_normal;
+                const double d0 = nr[0] * a.x + nr[1] * a.y + nr[2] * a.z - P.ground_offset;
+                const double d1 = nr[0] * b.x + nr[1] * b.y + nr[2] * b.z - P.ground_offset;
+                ok = fmin(d0, d1) < infl;
+            }
+        }
+    }
+    flag[c] = ok ? 1 : 0;
+}
+
+void launch_cache_filter(hipStream_t s, const Geo& G, int n, const uint64_t* keys, const double4* X0,
+                         const double4* X1, const AbdParams& P, double infl, int* flag) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_cache_filter, dim3(cdivb(n, 256)), dim3(256), 0, s, G, n, keys, X0, X1, P, infl, flag);
 }
 
 }  // namespace abd

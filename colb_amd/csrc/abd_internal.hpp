@@ -154,9 +154,13 @@ void launch_body_pair_count(hipStream_t s, const Geo& G, const float* box, int* 
 void launch_body_pair_write(hipStream_t s, const Geo& G, const float* box, const int* off, int2* out, int cap);
 void launch_pair_queries(hipStream_t s, const Geo& G, const BodyMesh& M, int npairs, int max_queries,
                          const int2* pairs, const double4* X0, const double4* X1, const float* bbox, double infl,
-                         uint64_t* cand, int cap, int* count, int* overflow);
+                         uint64_t* cand, int cap, int* count, int* overflow, int box_only);
+void launch_cache_contains(hipStream_t s, int nv, const double4* Xc0, const double4* Xc1, const double4* X0,
+                           const double4* X1, double m, int* outside);
+void launch_cache_filter(hipStream_t s, const Geo& G, int n, const uint64_t* keys, const double4* X0,
+                         const double4* X1, const AbdParams& P, double infl, int* flag);
 void launch_ground_queries(hipStream_t s, const Geo& G, const double4* X0, const double4* X1, const AbdParams& P,
-                           double infl, uint64_t* cand, int cap, int* count, int* overflow);
+                           double infl, uint64_t* cand, int cap, int* count, int* overflow, int box_only);
 // abd_kernels.hip
 constexpr int ABD_N_FEATURES = 49;
 constexpr int ABD_FEAT_Z = 37;          // features 37..48: the block-Jacobi step D_b^-1 grad_b

@@ -42,6 +42,7 @@ class Params:
     hess_proj: int = 0  # contact Hessian projection: 0 = clamp eigenvalues at 0, 1 = absolute values
     k_max: float = 0.0  # adaptive contact stiffness cap (<= k_contact: fixed stiffness)
     k_raise_gap: float = 0.5  # raise the stiffness when a contact gets closer than this * dhat
+    cand_margin: float = None  # candidate-cache margin (length); None: 4 dhat; <= 0: no cache
     debug: int = 0  # bit 0: synchronise and check after every launch; bit 1: also log them;
     #                 bit 2: print broad-phase list sizes
 
@@ -74,6 +75,7 @@ class Params:
         p.hess_proj = self.hess_proj
         p.k_max = self.k_max
         p.k_raise_gap = self.k_raise_gap
+        p.cand_margin = 4.0 * self.dhat if self.cand_margin is None else self.cand_margin
         return p
 
     def replace(self, **kw):

@@ -83,6 +83,11 @@ typedef struct AbdParams {
                                   the step; a contact uses the larger stiffness of its two bodies.
                                   k_max <= k_contact: fixed stiffness (the reference, abd.py:30) */
     double k_raise_gap;        /* fraction of dhat in (0, 1) below which the stiffness is raised */
+    double cand_margin;        /* candidate cache: > 0 keeps the pairs of a BVH traversal whose boxes,
+                                  widened by this length, overlap, and answers every later detection
+                                  whose vertices stay within half of it of the cached boxes by the
+                                  exact candidate rule over that list (the same candidates, without a
+                                  traversal); <= 0: a traversal at every detection (contact.py:244-279) */
 } AbdParams;
 
 typedef struct AbdSceneDesc {
@@ -115,6 +120,9 @@ typedef struct AbdStepStats {
     int    pcg_fallbacks;      /* solves rerun with one launch per PCG iteration because their
                                   single cooperative launch was refused or a grid barrier timed out */
     double predictor_alpha;    /* step taken toward the predicted first iterate (0: none) */
+    int    traversals;         /* broad-phase BVH traversals during the step (the other detections of
+                                  the step were answered by the candidate cache) */
+    int    detections;         /* broad-phase detections during the step */
 } AbdStepStats;
 
 typedef struct AbdCtx AbdCtx;

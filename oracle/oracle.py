@@ -20,9 +20,8 @@ _lib = None
 def build(force=True):
     """Compile the oracle (g++, OpenMP) into oracle/_build/."""
     os.makedirs(os.path.join(HERE, "_build"), exist_ok=True)
-    if force or not os.path.exists(LIB_PATH):
-        subprocess.run(["make", "-B" if force else "-s", "-C", HERE], check=True,
-                       stdout=subprocess.DEVNULL)
+    # make rebuilds when the source or the ABI header is newer than the library
+    subprocess.run(["make", "-s"] + (["-B"] if force else []) + ["-C", HERE], check=True, stdout=subprocess.DEVNULL)
     return LIB_PATH
 
 

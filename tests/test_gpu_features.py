@@ -324,3 +324,33 @@ def test_iter_log_and_profiler_match_oracle(O, tmp_path):
     with np.load(path, allow_pickle=False) as z:
         np.testing.assert_array_equal(z["convergence"], gpu.profiler.convergence)
         np.testing.assert_array_equal(z["alphas"], gpu.profiler.alphas)
+
+
+# ------------------------------------------------------------------------------------------------
+# candidate cache: the same candidates as a traversal at every detection, so the same bits
+# ------------------------------------------------------------------------------------------------
+CACHE_CASES = {
+    "cubes_ground": lambda: (S.cubes_on_ground(lift=0.02), north_star(ground_enabled=True, dhat=0.05, k_contact=1e14),
+                             15),
+    "chain3": lambda: (S.chain_net(n=3), north_star(dhat=2e-3, k_contact=1e19), 8),
+    "box27": lambda: (cube_on_ground(0.5 * DHAT, n=3), ground_params(), 15),
+}
+
+
+@pytest.mark.parametrize("case", list(CACHE_CASES))
+@pytest.mark.parametrize("margin", [None, 1e-4])
+def test_candidate_cache_gives_the_same_bits(case, margin):
+    """margin None: 4 dhat (few traversals); 1e-4: small enough that iterates leave the cached region
+    and the cache is rebuilt within steps."""
+    sc, p, nsteps = CACHE_CASES[case]()
+    full, cached = solver(sc, p.replace(cand_margin=0.0)), solver(sc, p.replace(cand_margin=margin))
+    trav = det = 0
+    for k in range(nsteps):
+        sf, sc_ = full.step()[0], cached.step()[0]
+        assert sf["traversals"] == sf["detections"]
+        trav += sc_["traversals"]
+        det += sc_["detections"]
+        assert sc_["n_candidates"] == sf["n_candidates"] and sc_["n_active"] == sf["n_active"]
+        np.testing.assert_array_equal(cached.get_state()[0], full.get_state()[0])
+    say(f"  {case} margin {margin}: {trav} traversals for {det} detections")
+    assert 0 < trav < det
