@@ -42,6 +42,9 @@ SIGNATURES = {
     "abd_set_pcg_layout": (C.c_int, [_V, C.c_int, C.c_int]),
     "abd_set_warm_start": (C.c_int, [_V, _V, _F, _F, _F, _F]),
     "abd_set_preconditioner": (C.c_int, [_V, _V, _F, _F, _F, _F]),
+    "abd_set_predictor": (C.c_int, [_V, _V, _F, _F, _F, _F]),
+    "abd_log_steps": (C.c_int, [_V, C.c_int]),
+    "abd_get_step_samples": (C.c_int, [_V, _F, _F, C.c_int, _I]),
     "abd_log_samples": (C.c_int, [_V, C.c_int]),
     "abd_get_samples": (C.c_int, [_V, _F, _F, C.c_int, _I]),
     # include/abd_mlp.h

@@ -304,6 +304,38 @@ class NewtonAbd:
         _lib.check(_lib.load().abd_get_samples(self._h, fp, dp, n.value, C.byref(n)), self._h, "abd_get_samples")
         return f, d
 
+    def log_steps(self, on=True):
+        """Start (and clear) or stop logging step-predictor rows (features at the step's start, the
+        step q - q_now) of every unpinned body at every step."""
+        _lib.check(_lib.load().abd_log_steps(self._h, int(on)), self._h, "abd_log_steps")
+
+    def step_samples(self):
+        """The logged (features[n, N_FEATURES], dq[n, 12]) step rows, float32."""
+        from .learned import N_FEATURES
+        n = C.c_int(0)
+        L = _lib.load()
+        _lib.check(L.abd_get_step_samples(self._h, None, None, 0, C.byref(n)), self._h, "abd_get_step_samples")
+        f = np.zeros((n.value, N_FEATURES), dtype=np.float32)
+        d = np.zeros((n.value, 12), dtype=np.float32)
+        fp = f.ctypes.data_as(C.POINTER(C.c_float))
+        dp = d.ctypes.data_as(C.POINTER(C.c_float))
+        _lib.check(L.abd_get_step_samples(self._h, fp, dp, n.value, C.byref(n)), self._h, "abd_get_step_samples")
+        return f, d
+
+    def set_predictor(self, sp):
+        """Use a trained colb_amd.learned.StepPredictor for the first iterate of every step when
+        params.predictor is 3 (None: no model)."""
+        L = _lib.load()
+        if sp is None:
+            _lib.check(L.abd_set_predictor(self._h, None, None, None, None, None), self._h, "abd_set_predictor")
+            self._predictor = None
+            return
+        self._pred_arrays = [np.ascontiguousarray(a, dtype=np.float32) for a in
+                             (sp.feat_mean, sp.feat_std, sp.out_mean, sp.out_std)]
+        a = [x.ctypes.data_as(C.POINTER(C.c_float)) for x in self._pred_arrays]
+        _lib.check(L.abd_set_predictor(self._h, sp.mlp._h, *a), self._h, "abd_set_predictor")
+        self._predictor = sp  # keeps the model alive while the solver uses it
+
     def set_warm_start(self, ws):
         """Use a trained colb_amd.learned.WarmStart to seed every PCG solve (None: plain PCG)."""
         L = _lib.load()

@@ -132,6 +132,12 @@ struct AbdCtx {
     // learned block preconditioner (abd_set_preconditioner)
     AbdMlp* pre_mlp = nullptr;
     float *pfmean = nullptr, *pfistd = nullptr, *pomean = nullptr, *postd = nullptr;
+    // learned step predictor (abd_set_predictor, AbdParams.predictor 3) and its training log
+    AbdMlp* pred_mlp = nullptr;
+    float *prfmean = nullptr, *prfistd = nullptr, *promean = nullptr, *prostd = nullptr;
+    float* feat_step = nullptr;   // features of every body at the start of the step
+    bool log_steps = false;
+    std::vector<float> step_feat, step_dq;
     // learned warm start (abd_set_warm_start) and training-sample log (abd_log_samples)
     AbdMlp* mlp = nullptr;
     float *feat_raw = nullptr, *fmean = nullptr, *fistd = nullptr, *omean = nullptr, *ostd = nullptr;
@@ -1148,19 +1154,67 @@ int adapt_stiffness(AbdCtx* ctx) {
     return ABD_OK;
 }
 
-// first iterate of the step from the predictor (AbdParams.predictor 1, 2): q moves from q_now toward
+// features of every body at the current state (the step's start): an assembly, then the 49 body
+// features of the warm start into feat_step
+int step_features(AbdCtx* ctx) {
+    if (!ctx->cand_valid) CHK(static_broad_phase(ctx));
+    CHK(assemble(ctx));
+    launch_body_features(ctx->st, ctx->G, ctx->q, ctx->qtilde, ctx->qdot, ctx->n_inc, ctx->inc_s, ctx->act_g,
+                         ctx->n_act, ctx->pinv, ctx->grad, ctx->feat_step);
+    CHK(dbg(ctx, "launch_body_features"));
+    return ABD_OK;
+}
+
+// first iterate of the step from the predictor (AbdParams.predictor 1-3): q moves from q_now toward
 // the prediction by the largest CCD-safe halving step that lowers the energy
 int predict(AbdCtx* ctx) {
     if (ctx->P.predictor == 3) {
-        ctx->err = "predictor 3 (learned) is not available";
-        return ABD_ERR_ARG;
+        AbdMlp* m = ctx->pred_mlp;
+        if (!m) {
+            ctx->err = "predictor 3 needs a model (abd_set_predictor)";
+            return ABD_ERR_STATE;
+        }
+        int rc = mlp_reserve(m, ctx->nb);
+        if (rc != ABD_OK) {
+            ctx->err = "learned predictor: " + m->err;
+            return rc;
+        }
+        const int rows = (ctx->nb + 15) / 16 * 16;
+        launch_features_pack(ctx->st, ctx->nb, rows, m->dp[0], ctx->feat_step, ctx->prfmean, ctx->prfistd, m->A[0]);
+        mlp_forward_dev(m, ctx->nb, ctx->st);
+        launch_step_pred(ctx->st, ctx->G, m->dp[m->L], m->Z[m->L - 1], ctx->promean, ctx->prostd, ctx->P.dt, ctx->q,
+                         ctx->qnow, ctx->qdot, ctx->du);
+        CHK(dbg(ctx, "learned predictor"));
+    } else {
+        launch_predict_du(ctx->st, ctx->G, ctx->P.predictor, ctx->P.dt, ctx->q, ctx->qnow, ctx->qdot, ctx->qtilde,
+                          ctx->du);
+        CHK(dbg(ctx, "launch_predict_du"));
     }
-    launch_predict_du(ctx->st, ctx->G, ctx->P.predictor, ctx->P.dt, ctx->q, ctx->qnow, ctx->qdot, ctx->qtilde, ctx->du);
-    CHK(dbg(ctx, "launch_predict_du"));
     CHK(ccd_bound(ctx));
     CHK(line_search(ctx, 0));
     hipLaunchKernelGGL(k_copy, dim3(cdiv(ctx->ns, 64)), dim3(64), 0, ctx->st, ctx->ns, ctx->C.alpha, ctx->C.pred_alpha);
     CHK(dbg(ctx, "predictor"));
+    return ABD_OK;
+}
+
+// training rows of the step predictor: every unpinned body's features at the step's start and its
+// step q - q_now
+int log_step(AbdCtx* ctx) {
+    const int nb = ctx->nb;
+    std::vector<float> f((size_t)ABD_N_FEATURES * nb);
+    std::vector<double> q(12 * (size_t)nb), qn(12 * (size_t)nb);
+    std::vector<int> pinned(nb);
+    HIPCHK(hipMemcpyAsync(f.data(), ctx->feat_step, f.size() * sizeof(float), hipMemcpyDeviceToHost, ctx->st));
+    HIPCHK(hipMemcpyAsync(q.data(), ctx->q, q.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->st));
+    HIPCHK(hipMemcpyAsync(qn.data(), ctx->qnow, qn.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->st));
+    HIPCHK(hipMemcpyAsync(pinned.data(), ctx->pinned, nb * sizeof(int), hipMemcpyDeviceToHost, ctx->st));
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    for (int b = 0; b < nb; ++b) {
+        if (pinned[b]) continue;
+        ctx->step_feat.insert(ctx->step_feat.end(), f.begin() + (size_t)ABD_N_FEATURES * b,
+                              f.begin() + (size_t)ABD_N_FEATURES * (b + 1));
+        for (int j = 0; j < 12; ++j) ctx->step_dq.push_back((float)(q[12 * (size_t)b + j] - qn[12 * (size_t)b + j]));
+    }
     return ABD_OK;
 }
 
@@ -1181,6 +1235,7 @@ int step_once(AbdCtx* ctx) {
     CHK(fill_i32(ctx, ctx->C.k_raises, ns, 0));
     hipLaunchKernelGGL(k_fill_u64, dim3(cdiv(ns, 128)), dim3(128), 0, ctx->st, ns,
                        (unsigned long long*)ctx->C.pred_alpha, 0ull);
+    if (ctx->log_steps || ctx->P.predictor == 3) CHK(step_features(ctx));
     if (ctx->P.predictor > 0) CHK(predict(ctx));
     std::vector<int> active(ns, 1);
     for (int it = 0; it < mi; ++it) {
@@ -1232,6 +1287,7 @@ int step_once(AbdCtx* ctx) {
         }
         if (!any) break;
     }
+    if (ctx->log_steps) CHK(log_step(ctx));
     launch_forward(ctx->st, ctx->nb, ctx->P.dt, ctx->q, ctx->qnow, ctx->qdot);
     CHK(dbg(ctx, "launch_forward"));
     // the inertial target of the next step, so that the hooks called between steps (compute_g,
@@ -1601,6 +1657,8 @@ int abd_create(const AbdSceneDesc* d, const AbdParams* p, AbdCtx** out) {
     A(feat_raw, (size_t)ABD_N_FEATURES * nb); A(fmean, ABD_N_FEATURES); A(fistd, ABD_N_FEATURES); A(omean, 12);
     A(ostd, 12); A(x0, 12 * nb); A(rhs, 12 * nb); A(nrm_b, ns); A(nrm_r, ns); A(warm_used, ns);
     A(pfmean, ABD_N_FEATURES); A(pfistd, ABD_N_FEATURES); A(pomean, ABD_N_PRECOND); A(postd, ABD_N_PRECOND);
+    A(prfmean, ABD_N_FEATURES); A(prfistd, ABD_N_FEATURES); A(promean, 12); A(prostd, 12);
+    A(feat_step, (size_t)ABD_N_FEATURES * nb);
     if ((rc = set_tiles(ctx, 5)) != ABD_OK) return fail(rc);
     A(bnorm, ns); A(cg_iters, ns); A(cg_done, ns);
     A(C.newton_active, ns); A(C.residual, ns); A(C.alpha, ns); A(C.alpha0, ns); A(C.amax_bits, ns);
@@ -2075,6 +2133,48 @@ int abd_set_preconditioner(AbdCtx* ctx, AbdMlp* model, const float* feat_mean, c
     HIPCHK(hipStreamSynchronize(model->st));
     ctx->pre_mlp = model;
     ctx->solved = false;
+    return ABD_OK;
+}
+
+int abd_set_predictor(AbdCtx* ctx, AbdMlp* model, const float* feat_mean, const float* feat_std,
+                      const float* out_mean, const float* out_std) {
+    CHK(check_ctx(ctx));
+    if (!model) {
+        ctx->pred_mlp = nullptr;
+        return ABD_OK;
+    }
+    if (!feat_mean || !feat_std || !out_mean || !out_std || model->L < 1 || model->d[0] != ABD_N_FEATURES ||
+        model->d[model->L] != 12) {
+        ctx->err = "step-predictor model must map " + std::to_string(ABD_N_FEATURES) +
+                   " body features to 12 DOFs, with normalisation arrays";
+        return ABD_ERR_ARG;
+    }
+    std::vector<float> istd(ABD_N_FEATURES);
+    for (int j = 0; j < ABD_N_FEATURES; ++j) istd[j] = feat_std[j] > 0.f ? 1.f / feat_std[j] : 0.f;
+    HIPCHK(hipMemcpy(ctx->prfmean, feat_mean, ABD_N_FEATURES * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->prfistd, istd.data(), ABD_N_FEATURES * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->promean, out_mean, 12 * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->prostd, out_std, 12 * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(hipStreamSynchronize(model->st));
+    ctx->pred_mlp = model;
+    return ABD_OK;
+}
+
+int abd_log_steps(AbdCtx* ctx, int on) {
+    CHK(check_ctx(ctx));
+    ctx->log_steps = on != 0;
+    ctx->step_feat.clear();
+    ctx->step_dq.clear();
+    return ABD_OK;
+}
+
+int abd_get_step_samples(const AbdCtx* ctx, float* feat, float* dq, int cap, int* n) {
+    if (!ctx || !n) return ABD_ERR_ARG;
+    const int total = (int)(ctx->step_dq.size() / 12);
+    *n = total;
+    const int k = std::min(total, cap);
+    if (feat && k > 0) std::memcpy(feat, ctx->step_feat.data(), sizeof(float) * ABD_N_FEATURES * (size_t)k);
+    if (dq && k > 0) std::memcpy(dq, ctx->step_dq.data(), sizeof(float) * 12 * (size_t)k);
     return ABD_OK;
 }
 

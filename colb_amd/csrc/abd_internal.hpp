@@ -234,6 +234,8 @@ void launch_ls_init(hipStream_t s, int ns, const AbdParams& P, const int* newton
 void launch_ls_decide(hipStream_t s, int ns, const AbdParams& P, SceneCtl c, int floor_accept, int* pending);
 void launch_predict_du(hipStream_t s, const Geo& G, int mode, double dt, const double* q, const double* qnow,
                        const double* qdot, const double* qtilde, double* du);
+void launch_step_pred(hipStream_t s, const Geo& G, int dpo, const float* Y, const float* omean, const float* ostd,
+                      double dt, const double* q, const double* qnow, const double* qdot, double* du);
 void launch_dmin(hipStream_t s, const Geo& G, int n, const uint64_t* keys, const double4* X, const AbdParams& P,
                  unsigned long long* dmin_bits);
 void launch_stiffness_update(hipStream_t s, const Geo& G, const AbdParams& P, const int* active, SceneCtl c);

@@ -1551,6 +1551,26 @@ void launch_ls_decide(hipStream_t s, int ns, const AbdParams& P, SceneCtl c, int
     hipLaunchKernelGGL(k_ls_decide, dim3(cdiv(ns, 128)), dim3(128), 0, s, ns, P, c, floor_accept, pending);
 }
 
+// Direction toward the learned predicted first iterate q_now + dq, dq = dt qdot + de-standardised
+// model output (the model predicts the step's correction over the extrapolation); pinned bodies and
+// non-finite predictions stay
+__global__ void k_step_pred(Geo G, int dpo, const float* __restrict__ Y, const float* __restrict__ omean,
+                            const float* __restrict__ ostd, double dt, const double* __restrict__ q,
+                            const double* __restrict__ qnow, const double* __restrict__ qdot, double* __restrict__ du) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= 12 * G.nb) return;
+    const int b = e / 12, j = e % 12;
+    const double dq = dt * qdot[e] + (double)(Y[(size_t)b * dpo + j] * ostd[j] + omean[j]);
+    const double d = q[e] - qnow[e] - dq;
+    du[e] = (G.pinned[b] || !isfinite(d)) ? 0.0 : d;
+}
+
+void launch_step_pred(hipStream_t s, const Geo& G, int dpo, const float* Y, const float* omean, const float* ostd,
+                      double dt, const double* q, const double* qnow, const double* qdot, double* du) {
+    hipLaunchKernelGGL(k_step_pred, dim3(cdiv(12 * G.nb, 256)), dim3(256), 0, s, G, dpo, Y, omean, ostd, dt, q, qnow,
+                       qdot, du);
+}
+
 // Direction toward the predicted first iterate, as a Newton step du (q - alpha du moves toward the
 // target): mode 1 = q_now + dt qdot, mode 2 = q~; pinned bodies stay
 __global__ void k_predict_du(Geo G, int mode, double dt, const double* __restrict__ q, const double* __restrict__ qnow,

@@ -1,5 +1,5 @@
-"""Learned warm start and learned block preconditioner of the ABD Newton step (no counterpart in
-the reference).
+"""Learned warm start, learned block preconditioner and learned step predictor of the ABD Newton
+step (no counterpart in the reference).
 
 A per-body MLP maps body features to the body's Newton step du (the 12 DOFs of the PCG
 solution), which then seeds the PCG as its initial guess.  The model runs on the GPU through
@@ -14,6 +14,11 @@ clipped to +-8 so that a state far outside the training set cannot blow the outp
     g_contact (12)       the summed barrier gradient on the body
     z (12)               the block-Jacobi step D_b^-1 grad_b (D_b the body's 12x12 diagonal block)
 Target per body (12): du - z, standardised the same way; the warm start is x0 = z + prediction.
+
+The step predictor (StepPredictor, AbdParams.predictor = 3) reads the same 49 features at the
+step's start q_now and predicts the step's correction over the extrapolation, dq - dt qdot; the
+first Newton iterate moves toward q_now + dt qdot + prediction (a CCD-safe, energy-decreasing
+fraction of the way).
 """
 import ctypes as C
 
@@ -185,6 +190,28 @@ class WarmStart:
         for k in ("feat_mean", "feat_std", "out_mean", "out_std"):
             setattr(ws, k, z[k].astype(np.float32))
         return ws
+
+
+class StepPredictor(WarmStart):
+    """The trained step-predictor model: features at the step's start -> dq - dt qdot."""
+
+    def fit(self, feat, dq, dt, steps=3000, batch=512, lr=1e-3, seed=0):
+        """Standardise, then train with Adam on the GPU; returns the batch losses.  feat, dq: the
+        rows the solver logs with log_steps (features at the step's start, the step q - q_now)."""
+        feat = np.asarray(feat, dtype=np.float64)
+        t = np.asarray(dq, dtype=np.float64) - dt * feat[:, 12:24]
+        self.feat_mean = feat.mean(0).astype(np.float32)
+        self.feat_std = np.maximum(feat.std(0), 1e-12).astype(np.float32)
+        self.out_mean = t.mean(0).astype(np.float32)
+        self.out_std = np.maximum(t.std(0), 1e-12).astype(np.float32)
+        X = self._inputs(feat)
+        T = ((t - self.out_mean) / self.out_std).astype(np.float32)
+        return self.mlp.train(X, T, steps=steps, batch=min(batch, X.shape[0]), lr=lr, seed=seed)
+
+    def predict(self, feat, dt):
+        """The predicted step dq = dt qdot + model output for each row of raw features."""
+        feat = np.asarray(feat, dtype=np.float64)
+        return dt * feat[:, 12:24] + self.mlp.forward(self._inputs(feat)) * self.out_std + self.out_mean
 
 
 N_PRECOND = 78

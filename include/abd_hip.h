@@ -240,6 +240,22 @@ int abd_set_warm_start(AbdCtx* ctx, AbdMlp* model, const float* feat_mean, const
  * the PCG and its stopping test are unchanged.  model == NULL switches it off. */
 int abd_set_preconditioner(AbdCtx* ctx, AbdMlp* model, const float* feat_mean, const float* feat_std,
                            const float* out_mean, const float* out_std);
+/* No counterpart in the reference (the learned Newton step predictor of the north-star
+ * configuration): with AbdParams.predictor = 3 every step first computes the 49 body features at the
+ * step's start q_now (an assembly there; features as for the warm start), standardises and clips
+ * them, runs the model (49 -> ... -> 12) and moves the first Newton iterate from q_now toward
+ * q_now + dq, dq = dt qdot + out * out_std + out_mean (the model predicts the step's correction over
+ * the extrapolation), by the largest CCD-safe halving step that lowers the energy, none if no trial
+ * does.  model == NULL switches the model off (predictor 3 then fails with ABD_ERR_STATE).  The
+ * model must outlive its use. */
+int abd_set_predictor(AbdCtx* ctx, AbdMlp* model, const float* feat_mean, const float* feat_std,
+                      const float* out_mean, const float* out_std);
+/* No counterpart in the reference (training data of the step predictor): while on, every step
+ * appends (features[49] at the step's start, the step q - q_now [12]) of each unpinned body; on/off
+ * clears. */
+int abd_log_steps(AbdCtx* ctx, int on);
+/* Copies up to cap logged step rows (feat: cap x 49, dq: cap x 12); *n = number logged. */
+int abd_get_step_samples(const AbdCtx* ctx, float* feat, float* dq, int cap, int* n);
 /* No counterpart in the reference (training data of the warm start): while on, every solve appends
  * (raw features[49], du[12]) of each unpinned body of each still-iterating scene; on/off clears. */
 int abd_log_samples(AbdCtx* ctx, int on);

@@ -452,6 +452,7 @@ struct Ctx {
     std::vector<int> log_cg;
     std::vector<AbdStepStats> stats;
     std::vector<double> kb;  // contact stiffness per body (adaptive rule of AbdParams.k_max)
+    std::vector<double> pred_dq;  // the step predictor's dq (AbdParams.predictor 3), set per step
     mutable std::atomic<int> err{0};  // 64: a vertex on or below the ground plane was assembled
     int threads = 1;
 };
@@ -1126,13 +1127,17 @@ void step_once(Ctx& C) {
     std::vector<double> dmin_prev(C.nb, HUGE_VAL);
     std::vector<double> alpha, amax;
     std::vector<int> floor_hit;
-    if (C.P.predictor == 1 || C.P.predictor == 2) {
+    if (C.P.predictor >= 1 && C.P.predictor <= 3) {
         std::vector<double> dup(12 * C.nb, 0.0);
+        const bool learned = C.P.predictor == 3 && C.pred_dq.size() == 12 * (size_t)C.nb;
+        if (C.P.predictor == 3 && !learned) C.err |= 128;  // no prediction set
         for (int b = 0; b < C.nb; ++b)
             if (!C.pinned[b])
                 for (int k = 0; k < 12; ++k) {
                     const int i = 12 * b + k;
-                    const double target = C.P.predictor == 2 ? C.qtilde[i] : C.qnow[i] + C.P.dt * C.qdot[i];
+                    const double target = C.P.predictor == 2 ? C.qtilde[i]
+                                          : learned           ? C.qnow[i] + C.pred_dq[i]
+                                                              : C.qnow[i] + C.P.dt * C.qdot[i];
                     const double d = C.q[i] - target;
                     dup[i] = std::isfinite(d) ? d : 0.0;
                 }
@@ -1388,6 +1393,60 @@ void orc_ccd(void* c, const double* du, double* alpha) {
     std::vector<double> d(du, du + 12 * C->nb), a;
     ccd_bound(*C, C->q.data(), d, a);
     for (int s = 0; s < C->ns; ++s) alpha[s] = a[s];
+}
+
+// the 49 body features of the learned models at the current iterate (float feat[49 nb]): q - q~,
+// qdot, the number of active contacts incident to the body and their summed gradient (unpinned
+// bodies only), and the block-Jacobi step z = D_b^-1 grad_b
+void orc_features(void* c, float* feat) {
+    Ctx* C = (Ctx*)c;
+    const double* q = C->q.data();
+    std::vector<Pair> act;
+    active_pairs(*C, q, act, nullptr);
+    System S;
+    assemble(*C, q, act, S);
+    std::vector<V3> X;
+    all_positions(*C, q, X);
+    std::vector<double> gs(12 * (size_t)C->nb, 0.0);
+    std::vector<int> cnt(C->nb, 0);
+    for (auto& pr : act) {
+        ContactTerm T;
+        contact_term(*C, pr, X, T);
+        for (int side = 0; side < 2; ++side) {
+            const int b = T.b[side];
+            if (b < 0 || C->pinned[b]) continue;
+            cnt[b]++;
+            for (int k = 0; k < 12; ++k) gs[12 * b + k] += T.g[side][k];
+        }
+    }
+    for (int b = 0; b < C->nb; ++b) {
+        float* f = feat + 49 * (size_t)b;
+        for (int k = 0; k < 12; ++k) {
+            f[k] = (float)(q[12 * b + k] - C->qtilde[12 * b + k]);
+            f[12 + k] = (float)C->qdot[12 * b + k];
+            f[25 + k] = (float)gs[12 * b + k];
+        }
+        f[24] = (float)cnt[b];
+        double P[144];
+        if (C->pinned[b]) {
+            for (int k = 0; k < 144; ++k) P[k] = (k % 13 == 0) ? 1.0 / C->P.pinned_diag : 0.0;
+        } else if (!chol_inverse12(&S.diag[144 * b], P)) {
+            C->err |= 256;
+            for (int k = 0; k < 144; ++k) P[k] = 0.0;
+        }
+        for (int i = 0; i < 12; ++i) {
+            double z = 0.0;
+            for (int j = 0; j < 12; ++j) z += P[12 * i + j] * S.grad[12 * b + j];
+            f[37 + i] = (float)z;
+        }
+    }
+}
+
+// the step predictor's dq[12 nb] for the following steps with AbdParams.predictor 3 (NULL clears)
+void orc_set_prediction(void* c, const double* dq) {
+    Ctx* C = (Ctx*)c;
+    if (dq) C->pred_dq.assign(dq, dq + 12 * (size_t)C->nb);
+    else C->pred_dq.clear();
 }
 
 // the line search along a given direction (the Newton search's rules); alpha[ns]

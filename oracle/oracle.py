@@ -59,6 +59,8 @@ def lib():
             "orc_set_stiffness": (None, [V, D]),
             "orc_take_error": (C.c_int, [V]),
             "orc_line_search": (None, [V, D, D]),
+            "orc_features": (None, [V, C.POINTER(C.c_float)]),
+            "orc_set_prediction": (None, [V, D]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -162,8 +164,20 @@ class Oracle:
     def set_stiffness(self, k):
         lib().orc_set_stiffness(self.h, dptr(f64(k).reshape(-1)))
 
+    def features(self):
+        """The 49 body features of the learned models at the current iterate, float32 [nb, 49]."""
+        f = np.zeros((self.nb, 49), dtype=np.float32)
+        lib().orc_features(self.h, f.ctypes.data_as(C.POINTER(C.c_float)))
+        return f
+
+    def set_prediction(self, dq):
+        """The step predictor's dq [nb, 12] for the following steps with predictor 3 (None clears)."""
+        lib().orc_set_prediction(self.h, None if dq is None else dptr(f64(dq).reshape(-1)))
+
     def take_error(self):
-        """Error bits raised since the last call (64: a vertex on or below the ground plane)."""
+        """Error bits raised since the last call (64: a vertex on or below the ground plane, 128: a
+        step with predictor 3 and no prediction set, 256: a diagonal block without a Cholesky
+        factor in features())."""
         return int(lib().orc_take_error(self.h))
 
     def get_state(self):

@@ -1,7 +1,9 @@
 """Runs the NewtonAbd step on the CPU emulation of the HIP sources (tests/emu/_build/libabd_emu.so,
 AddressSanitizer on every allocation) and compares it with the oracle.  TEST INFRASTRUCTURE ONLY:
 started by tests/test_emu.py in a child process whose LD_PRELOAD holds the sanitizer runtime.
-Prints one line `result <max relative state error> <active contacts emulated> <active oracle>`."""
+Prints `result <max relative state error> <active contacts emulated> <active oracle>` and, for two
+steps with the learned step predictor (a small untrained model), `learned <feature error> <state
+error> <predictor alpha emulated> <predictor alpha oracle>`."""
 import os
 import sys
 
@@ -32,6 +34,39 @@ def main():
         sg, so = g.step()[0], o.step()[0]
     qg, qo = g.get_state()[0], o.get_state()[0]
     print("result", np.abs(qg - qo).max() / np.abs(qo).max(), sg["n_active"], so["n_active"], flush=True)
+    g.close()
+    learned(sc, p)
+
+
+def learned(sc, p):
+    from colb_amd.learned import StepPredictor
+    from oracle import mlp_oracle
+    sp = StepPredictor(dims=(49, 32, 12), seed=1)
+    rng = np.random.default_rng(2)
+    sp.feat_mean = rng.normal(size=49).astype(np.float32)
+    sp.feat_std = np.exp(rng.normal(size=49)).astype(np.float32)
+    sp.out_std = np.full(12, 1e-4, dtype=np.float32)
+    p3 = p.replace(predictor=3)
+    g = NewtonAbd(p.dt, sc, params=p3)
+    o = O.Oracle(sc, p3)
+    g.set_predictor(sp)
+    g.log_steps(True)
+    ferr = serr = 0.0
+    ncont = 0
+    for _ in range(2):
+        fo = o.features()
+        ncont += int(fo[:, 24].sum())
+        qn, qd = o.get_state()[1:]
+        y = mlp_oracle.forward(sp.mlp.get_params(), sp.dims, sp._inputs(fo))[1][-1]
+        o.set_prediction(p.dt * qd.reshape(-1, 12) + (y * sp.out_std + sp.out_mean).astype(np.float64))
+        sg, so = g.step()[0], o.step()[0]
+        fg = g.step_samples()[0][-fo.shape[0]:]
+        scale = np.maximum(np.abs(fo).max(0), 1e-30)
+        ferr = max(ferr, float((np.abs(fg - fo) / scale).max()))
+        qg, qo = g.get_state()[0], o.get_state()[0]
+        serr = max(serr, float(np.abs(qg - qo).max() / np.abs(qo).max()))
+    assert o.take_error() == 0 and ncont > 0
+    print("learned", ferr, serr, sg["predictor_alpha"], so["predictor_alpha"], flush=True)
     g.close()
 
 

@@ -1,7 +1,8 @@
 """The HIP sources built for the CPU against the fibre-based HIP emulation in tests/emu (g++,
 AddressSanitizer): the radix sort and scan unit driver, and two Newton steps of a two-cube scene
 through the C ABI compared with the oracle (indexing errors surface here as sanitizer reports,
-before any GPU run)."""
+before any GPU run), then two steps with the learned step predictor (body features against the
+oracle's, the step against the oracle given the same prediction)."""
 import os
 import subprocess
 import sys
@@ -37,3 +38,8 @@ def test_emulated_step_matches_oracle(emu_lib):
     _, err, na_emu, na_orc = res[-1].split()
     assert float(err) < 1e-6
     assert int(na_emu) == int(na_orc)
+    lrn = [line for line in out.stdout.splitlines() if line.startswith("learned")]
+    assert lrn, out.stdout[-2000:]
+    _, ferr, serr, pa_emu, pa_orc = lrn[-1].split()
+    assert float(ferr) < 1e-5 and float(serr) < 1e-6
+    assert float(pa_emu) == float(pa_orc)

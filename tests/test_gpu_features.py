@@ -354,3 +354,117 @@ def test_candidate_cache_gives_the_same_bits(case, margin):
         np.testing.assert_array_equal(cached.get_state()[0], full.get_state()[0])
     say(f"  {case} margin {margin}: {trav} traversals for {det} detections")
     assert 0 < trav < det
+
+
+# ------------------------------------------------------------------------------------------------
+# learned step predictor: features and the predicted first iterate against the oracle
+# ------------------------------------------------------------------------------------------------
+def _untrained_predictor(seed=1):
+    from colb_amd.learned import StepPredictor
+    sp = StepPredictor(dims=(49, 64, 64, 12), seed=seed)
+    rng = np.random.default_rng(seed)
+    sp.feat_mean = rng.normal(size=49).astype(np.float32)
+    sp.feat_std = np.exp(rng.normal(size=49)).astype(np.float32)
+    sp.out_std = np.full(12, 1e-4, dtype=np.float32)
+    return sp
+
+
+LEARNED_CASES = {
+    "two_cubes": lambda: (two_cube_scene(gap=0.01), fd_params()),
+    "chain3": lambda: (S.chain_net(n=3), north_star(dhat=2e-3, k_contact=1e19)),
+}
+
+
+@pytest.mark.parametrize("case", list(LEARNED_CASES))
+def test_learned_predictor_matches_oracle(O, case):
+    """Each step: the GPU's logged features at the step's start equal the oracle's (FP32 rounding of
+    FP64 values: 1e-5 of the largest magnitude of the feature's group: q - q~, qdot, contact count,
+    contact gradient, block-Jacobi step; a component that is zero up to rounding differs with the
+    order of the FP64 sums), and the step equals the oracle's given
+    the prediction of the CPU model restatement on the oracle's features (predictor step equal).
+    The scenes are ones whose Newton iterates do not magnify the model's rounding (a BF16 output
+    differs between the two models by its last bits)."""
+    from oracle import mlp_oracle
+    sc, p = LEARNED_CASES[case]()
+    p = p.replace(predictor=3, newton_tol=1e-9, cg_tol=1e-12, cg_max_iter=2000)
+    sp = _untrained_predictor()
+    gpu, orc = solver(sc, p), O.Oracle(sc, p)
+    gpu.set_predictor(sp)
+    gpu.log_steps(True)
+    contacts = 0
+    for k in range(8):
+        fo = orc.features()
+        contacts += int(fo[:, 24].sum())
+        qd = orc.get_state()[2].reshape(-1, 12)
+        y = mlp_oracle.forward(sp.mlp.get_params(), sp.dims, sp._inputs(fo))[1][-1]
+        orc.set_prediction(p.dt * qd + (y * sp.out_std + sp.out_mean).astype(np.float64))
+        so, sg = orc.step()[0], gpu.step()[0]
+        free = sc.body_pinned == 0  # the log holds the unpinned bodies
+        fo = fo[free]
+        fg = gpu.step_samples()[0][-len(fo):]
+        scale = np.zeros(49)
+        for lo, hi in ((0, 12), (12, 24), (24, 25), (25, 37), (37, 49)):
+            scale[lo:hi] = np.abs(fo[:, lo:hi]).max()
+        cerr = (np.abs(fg - fo) / np.maximum(scale, 1e-30)).max(0)
+        ferr = cerr.max()
+        err = rel(gpu.get_state()[0], orc.get_state()[0])
+        say(f"  {case} step {k}: features {ferr:.2e} (column {int(np.argmax(cerr))}) pred {so['predictor_alpha']:.3g}/{sg['predictor_alpha']:.3g} "
+            f"newton {so['newton_iters']}/{sg['newton_iters']} err {err:.2e}")
+        assert ferr < 1e-5
+        assert sg["predictor_alpha"] == so["predictor_alpha"]
+        assert err < 1e-6
+    assert orc.take_error() == 0 and contacts > 0
+
+
+def test_learned_predictor_without_a_model_is_an_error():
+    from colb_amd._lib import AbdError
+    sc, p, _ = CACHE_CASES["cubes_ground"]()
+    gpu = solver(sc, p.replace(predictor=3))
+    with pytest.raises(AbdError, match="abd_set_predictor"):
+        gpu.step()
+
+
+def test_learned_predictor_trains_and_keeps_the_solution():
+    """Trained on the logged steps of runs from other initial velocities, the predictor leaves the
+    trajectory unchanged (1e-6) over steps that all converge (from the state after the landing
+    step, which reaches the Newton cap) and is reported as taken (predictor step > 0)."""
+    from colb_amd.learned import StepPredictor
+    sc = S.cubes_on_ground(lift=0.02)
+    p = north_star(ground_enabled=True, dhat=0.05, k_contact=1e14, newton_tol=1e-6, cg_tol=1e-12, cg_max_iter=2000)
+    rng = np.random.default_rng(4)
+    g = solver(sc, p)
+    g.log_steps(True)
+    for r in range(3):
+        qd = sc.qdot0.copy()
+        qd[:, :3] += rng.normal(scale=0.3, size=qd[:, :3].shape)
+        g.set_state(sc.q0, sc.q0, qd)
+        for _ in range(20):
+            g.step()
+    F, D = g.step_samples()
+    assert len(F) == 3 * 20 * int((sc.body_pinned == 0).sum())
+    sp = StepPredictor(seed=0)
+    losses = sp.fit(F, D, p.dt, steps=1500, batch=256, lr=1e-3)
+    assert losses[-100:].mean() < 0.5 * losses[:20].mean()
+    g.log_steps(False)
+    g.set_state(sc.q0, sc.q0, sc.qdot0)
+    for _ in range(2):
+        g.step()
+    start = g.get_state()
+    runs = {}
+    for name, pred in (("plain", 0), ("learned", 3)):
+        s = solver(sc, p.replace(predictor=pred))
+        s.set_state(*start)
+        if pred == 3:
+            s.set_predictor(sp)
+        it, taken, conv = 0, 0, True
+        for _ in range(20):
+            st = s.step()[0]
+            it += st["newton_iters"]
+            taken += st["predictor_alpha"] > 0
+            conv &= bool(st["converged"])
+        runs[name] = (it, taken, conv, s.get_state()[0])
+    say(f"  newton iterations over 20 steps: plain {runs['plain'][0]}, learned {runs['learned'][0]} "
+        f"(prediction taken in {runs['learned'][1]} steps)")
+    assert runs["plain"][2] and runs["learned"][2]
+    assert rel(runs["learned"][3], runs["plain"][3]) < 1e-6
+    assert runs["learned"][1] > 0
