@@ -45,6 +45,8 @@ class AbdParams(C.Structure):
         ("k_max", C.c_double),
         ("k_raise_gap", C.c_double),
         ("cand_margin", C.c_double),
+        ("pcg_x0", C.c_int),
+        ("reserved1", C.c_int),
     ]
 
 

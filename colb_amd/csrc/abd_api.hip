@@ -172,6 +172,7 @@ struct AbdCtx {
     std::vector<int> log_cg;
     std::vector<AbdStepStats> stats;
     bool assembled = false, solved = false;
+    int newton_it = -1;  // Newton iteration of the running step (-1: outside a step)
     Timer T;
 };
 
@@ -336,17 +337,18 @@ int ensure_tmp(AbdCtx* ctx, size_t bytes) {
     return ABD_OK;
 }
 
-int sort_keys(AbdCtx* ctx, const uint64_t* in, uint64_t* out, int n, int end_bit = 64) {
+// sig: the bits that may be set in any key
+int sort_keys(AbdCtx* ctx, const uint64_t* in, uint64_t* out, int n, uint64_t sig) {
     if (n <= 0) return ABD_OK;
     CHK(ensure_tmp(ctx, radix_scratch_bytes(n)));
-    HIPCHK(radix_sort64(ctx->st, in, out, nullptr, nullptr, n, end_bit, ctx->sort_tmp));
+    HIPCHK(radix_sort64(ctx->st, in, out, nullptr, nullptr, n, sig, ctx->sort_tmp));
     return dbg(ctx, "radix_sort64(keys)");
 }
 
-int sort_pairs(AbdCtx* ctx, const uint64_t* kin, uint64_t* kout, const int* vin, int* vout, int n, int end_bit = 64) {
+int sort_pairs(AbdCtx* ctx, const uint64_t* kin, uint64_t* kout, const int* vin, int* vout, int n, uint64_t sig) {
     if (n <= 0) return ABD_OK;
     CHK(ensure_tmp(ctx, radix_scratch_bytes(n)));
-    HIPCHK(radix_sort64(ctx->st, kin, kout, vin, vout, n, end_bit, ctx->sort_tmp));
+    HIPCHK(radix_sort64(ctx->st, kin, kout, vin, vout, n, sig, ctx->sort_tmp));
     return dbg(ctx, "radix_sort64(pairs)");
 }
 
@@ -377,6 +379,9 @@ int bits_for(int n) {
     while ((1 << b) < n) ++b;
     return b;
 }
+
+// mask of the low `bits` bits
+uint64_t low_bits(int bits) { return bits >= 64 ? ~0ull : ((1ull << bits) - 1); }
 
 // --------------------------------------------------------------------------------------------
 // capacities
@@ -542,7 +547,7 @@ int cached_broad_phase(AbdCtx* ctx, const double4* X0, const double4* X1, const 
         CHK(traverse(ctx, X0, X1, du, ctx->P.dhat + 2.0 * m, 1, true, &n));
         ctx->n_cache = n;
         tic(ctx, "candidate_sort");
-        CHK(sort_keys(ctx, ctx->cache, ctx->cache_s, n, 48 + bits_for(ctx->ns + 1)));
+        CHK(sort_keys(ctx, ctx->cache, ctx->cache_s, n, low_bits(48 + bits_for(ctx->ns + 1))));
         toc(ctx);
         HIPCHK(hipMemcpyAsync(ctx->Xc0, X0, ctx->nv * sizeof(double4), hipMemcpyDeviceToDevice, ctx->st));
         HIPCHK(hipMemcpyAsync(ctx->Xc1, X1, ctx->nv * sizeof(double4), hipMemcpyDeviceToDevice, ctx->st));
@@ -579,7 +584,7 @@ int broad_phase(AbdCtx* ctx, const double4* X0, const double4* X1, const double*
     CHK(traverse(ctx, X0, X1, du, ctx->P.dhat, 0, false, &n));
     ctx->n_cand = n;
     tic(ctx, "candidate_sort");
-    CHK(sort_keys(ctx, ctx->cand, ctx->cand_s, ctx->n_cand, 48 + bits_for(ctx->ns + 1)));
+    CHK(sort_keys(ctx, ctx->cand, ctx->cand_s, ctx->n_cand, low_bits(48 + bits_for(ctx->ns + 1))));
     toc(ctx);
     return cand_ranges(ctx);
 }
@@ -625,8 +630,10 @@ int assemble(AbdCtx* ctx) {
     ctx->n_inc = ctx->h_cnt[C_NINC];
     ctx->n_pk = ctx->h_cnt[C_NPK];
     tic(ctx, "block_sort");
-    CHK(sort_keys(ctx, ctx->inc, ctx->inc_s, ctx->n_inc));
-    CHK(sort_keys(ctx, ctx->pk, ctx->pk_s, ctx->n_pk));
+    // incidence keys body << 32 | contact << 1 | role; body-pair keys lo << 44 | hi << 24 | contact
+    const uint64_t bbits = low_bits(bits_for(ctx->nb)), cbits = low_bits(bits_for(na));
+    CHK(sort_keys(ctx, ctx->inc, ctx->inc_s, ctx->n_inc, (bbits << 32) | (cbits << 1) | 1ull));
+    CHK(sort_keys(ctx, ctx->pk, ctx->pk_s, ctx->n_pk, (bbits << 44) | (bbits << 24) | cbits));
     toc(ctx);
     ctx->n_pairs = 0;
     if (ctx->n_pk > 0) {
@@ -646,7 +653,8 @@ int assemble(AbdCtx* ctx) {
         toc(ctx);
         launch_csr_keys(ctx->st, ctx->n_pairs, ctx->pair_b, ctx->ent_k, ctx->ent_v);
         CHK(dbg(ctx, "launch_csr_keys"));
-        CHK(sort_pairs(ctx, ctx->ent_k, ctx->ent_ks, ctx->ent_v, ctx->ent_vs, 2 * ctx->n_pairs));
+        CHK(sort_pairs(ctx, ctx->ent_k, ctx->ent_ks, ctx->ent_v, ctx->ent_vs, 2 * ctx->n_pairs,
+                       (bbits << 32) | bbits));  // body << 32 | body
     }
     launch_row_ptr(ctx->st, ctx->nb, 2 * ctx->n_pairs, ctx->ent_ks, ctx->row_ptr);
     CHK(dbg(ctx, "launch_row_ptr"));
@@ -668,19 +676,25 @@ int assemble(AbdCtx* ctx) {
 // kept only where ||r0|| < ||b|| (plain PCG from zero otherwise)
 int warm_start(AbdCtx* ctx) {
     AbdMlp* m = ctx->mlp;
-    launch_body_features(ctx->st, ctx->G, ctx->q, ctx->qtilde, ctx->qdot, ctx->n_inc, ctx->inc_s, ctx->act_g,
-                         ctx->n_act, ctx->pinv, ctx->grad, ctx->feat_raw);
-    CHK(dbg(ctx, "launch_body_features"));
-    int rc = mlp_reserve(m, ctx->nb);
-    if (rc != ABD_OK) {
-        ctx->err = "warm start: " + m->err;
-        return rc;
+    if (m) {
+        launch_body_features(ctx->st, ctx->G, ctx->q, ctx->qtilde, ctx->qdot, ctx->n_inc, ctx->inc_s, ctx->act_g,
+                             ctx->n_act, ctx->pinv, ctx->grad, ctx->feat_raw);
+        CHK(dbg(ctx, "launch_body_features"));
+        int rc = mlp_reserve(m, ctx->nb);
+        if (rc != ABD_OK) {
+            ctx->err = "warm start: " + m->err;
+            return rc;
+        }
+        const int rows = (ctx->nb + 15) / 16 * 16;
+        launch_features_pack(ctx->st, ctx->nb, rows, m->dp[0], ctx->feat_raw, ctx->fmean, ctx->fistd, m->A[0]);
+        mlp_forward_dev(m, ctx->nb, ctx->st);
+        launch_warm_x0(ctx->st, ctx->G, m->dp[m->L], m->Z[m->L - 1], ctx->omean, ctx->ostd, ctx->feat_raw, ctx->x0);
+        CHK(dbg(ctx, "warm start forward"));
+    } else {
+        // AbdParams.pcg_x0 = 1: the part of the previous direction its line search did not take
+        launch_prev_x0(ctx->st, ctx->G, ctx->C.alpha, ctx->du, ctx->x0);
+        CHK(dbg(ctx, "launch_prev_x0"));
     }
-    const int rows = (ctx->nb + 15) / 16 * 16;
-    launch_features_pack(ctx->st, ctx->nb, rows, m->dp[0], ctx->feat_raw, ctx->fmean, ctx->fistd, m->A[0]);
-    mlp_forward_dev(m, ctx->nb, ctx->st);
-    launch_warm_x0(ctx->st, ctx->G, m->dp[m->L], m->Z[m->L - 1], ctx->omean, ctx->ostd, ctx->feat_raw, ctx->x0);
-    CHK(dbg(ctx, "warm start forward"));
     launch_residual(ctx->st, ctx->nb, ctx->grad, ctx->diag, ctx->row_ptr, ctx->ent_ks, ctx->ent_vs, ctx->off,
                     ctx->offT, ctx->x0, ctx->rhs, ctx->n_pairs, ctx->cnt + C_ERR);
     launch_scene_norm(ctx->st, ctx->G, ctx->grad, ctx->nrm_b);
@@ -740,6 +754,7 @@ int check_params(AbdCtx* ctx, const AbdParams* p) {
         return bad("ground_normal must be non-zero");
     if (p->predictor < 0 || p->predictor > 3) return bad("predictor must be 0, 1, 2 or 3");
     if (p->hess_proj != 0 && p->hess_proj != 1) return bad("hess_proj must be 0 or 1");
+    if (p->pcg_x0 != 0 && p->pcg_x0 != 1) return bad("pcg_x0 must be 0 or 1");
     if (!finite(p->cand_margin)) return bad("cand_margin must be finite");
     if (!finite(p->k_max)) return bad("k_max must be finite");
     if (p->k_max > p->k_contact && (!(p->k_raise_gap > 0.0) || !(p->k_raise_gap < 1.0)))
@@ -1026,7 +1041,7 @@ int solve(AbdCtx* ctx) {
         CHK(solve_direct(ctx, false));
         return solve_finish(ctx, false);
     }
-    const bool warm = ctx->mlp != nullptr;
+    const bool warm = ctx->mlp != nullptr || (ctx->P.pcg_x0 == 1 && ctx->newton_it > 0);
     const double* b = ctx->grad;
     const double* bref = nullptr;
     if (warm) {
@@ -1035,7 +1050,7 @@ int solve(AbdCtx* ctx) {
         b = ctx->rhs;
         bref = ctx->nrm_b;
     }
-    if (ctx->log_on && !warm) {
+    if (ctx->log_on && !ctx->mlp) {
         // logged features use the block-Jacobi inverse, as at inference (the warm start computed them)
         launch_body_features(ctx->st, ctx->G, ctx->q, ctx->qtilde, ctx->qdot, ctx->n_inc, ctx->inc_s, ctx->act_g,
                              ctx->n_act, ctx->pinv, ctx->grad, ctx->feat_raw);
@@ -1239,6 +1254,7 @@ int step_once(AbdCtx* ctx) {
     if (ctx->P.predictor > 0) CHK(predict(ctx));
     std::vector<int> active(ns, 1);
     for (int it = 0; it < mi; ++it) {
+        ctx->newton_it = it;
         if (!ctx->cand_valid) CHK(static_broad_phase(ctx));
         CHK(assemble(ctx));
         CHK(solve(ctx));
@@ -1287,6 +1303,7 @@ int step_once(AbdCtx* ctx) {
         }
         if (!any) break;
     }
+    ctx->newton_it = -1;
     if (ctx->log_steps) CHK(log_step(ctx));
     launch_forward(ctx->st, ctx->nb, ctx->P.dt, ctx->q, ctx->qnow, ctx->qdot);
     CHK(dbg(ctx, "launch_forward"));
@@ -1882,6 +1899,7 @@ int abd_assemble(AbdCtx* ctx, double* grad, double* diag, int* pair_ij, double* 
         HIPCHK(hipMemcpyAsync(energy, ctx->C.E0, ctx->ns * sizeof(double), hipMemcpyDeviceToHost, ctx->st));
     }
     HIPCHK(hipStreamSynchronize(ctx->st));
+    timer_collect(ctx);
     return ABD_OK;
 }
 

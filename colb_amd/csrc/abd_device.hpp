@@ -340,8 +340,13 @@ __device__ inline bool wave_is_psd(const double* A, double* L, int m, int lane) 
 // unique matrix function, so it agrees with the oracle's cyclic-order psd_project to rounding;
 // same stopping rule.  V is LDS scratch of m*m doubles.  absolute: A <- V |Lambda| V^T instead
 // (negative eigenvalues mirrored rather than clamped).
-__device__ inline void wave_psd_project(double* A, double* V, int m, int lane, bool absolute = false) {
-    const int mm = m * m;
+// The size is a template parameter (every index below is then a compile-time division);
+// wave_psd_project dispatches on the runtime size.
+template <int M>
+__device__ inline void wave_psd_project_m(double* A, double* V, int lane, bool absolute, double* A2, double* V2,
+                                          double* jw_self, double* jw_part, int* j_part) {
+    constexpr int m = M;
+    constexpr int mm = m * m;
     for (int e = lane; e < mm; e += 64) V[e] = (e / m == e % m) ? 1.0 : 0.0;
     double f = 0;
     for (int e = lane; e < mm; e += 64) f += A[e] * A[e];
@@ -354,9 +359,6 @@ __device__ inline void wave_psd_project(double* A, double* V, int m, int lane, b
     // index, its own weight and its partner's (J restricted to the pair).  Then every element of
     // A' = J^T A J (upper triangle, mirrored) and V' = V J is formed from the old matrices in one
     // pass into the other buffer: two barriers per round.
-    __shared__ double A2[144], V2[144];
-    __shared__ double jw_self[12], jw_part[12];
-    __shared__ int j_part[12];
     double* Ac = A;
     double* Vc = V;
     double* An = A2;
@@ -452,6 +454,20 @@ __device__ inline void wave_psd_project(double* A, double* V, int m, int lane, b
     cnt = 0;
     for (int e = lane; e < mm; e += 64) A[e] = out[cnt++];
     __syncthreads();
+}
+
+__device__ inline void wave_psd_project(double* A, double* V, int m, int lane, bool absolute = false) {
+    // the rotated copies of A and V and the rotation weights of a round
+    __shared__ double A2[144], V2[144];
+    __shared__ double jw_self[12], jw_part[12];
+    __shared__ int j_part[12];
+    switch (m) {
+        case 3: wave_psd_project_m<3>(A, V, lane, absolute, A2, V2, jw_self, jw_part, j_part); break;
+        case 6: wave_psd_project_m<6>(A, V, lane, absolute, A2, V2, jw_self, jw_part, j_part); break;
+        case 9: wave_psd_project_m<9>(A, V, lane, absolute, A2, V2, jw_self, jw_part, j_part); break;
+        case 12: wave_psd_project_m<12>(A, V, lane, absolute, A2, V2, jw_self, jw_part, j_part); break;
+        default: break;  // callers pass 3, 6, 9 or 12
+    }
 }
 
 }  // namespace abd

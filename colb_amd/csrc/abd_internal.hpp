@@ -141,8 +141,9 @@ struct GrowBuf {
 hipError_t excl_scan_int(hipStream_t s, const int* in, int* out, int n, int* bsum);
 int scan_scratch_ints(long n);
 size_t radix_scratch_bytes(long n);
+// sig: the bits that may be set in any key (stable sort over those bits)
 hipError_t radix_sort64(hipStream_t s, const uint64_t* kin, uint64_t* kout, const int* vin, int* vout, int n,
-                        int end_bit, void* scratch);
+                        uint64_t sig, void* scratch);
 
 void launch_qtilde(hipStream_t s, int nb, const double* qnow, const double* qdot, double dt, const double* g, double* qtilde);
 void launch_xform(hipStream_t s, const Geo& G, const double* q, const double* du, const double* alpha_scene, double4* X);
@@ -234,6 +235,7 @@ void launch_ls_init(hipStream_t s, int ns, const AbdParams& P, const int* newton
 void launch_ls_decide(hipStream_t s, int ns, const AbdParams& P, SceneCtl c, int floor_accept, int* pending);
 void launch_predict_du(hipStream_t s, const Geo& G, int mode, double dt, const double* q, const double* qnow,
                        const double* qdot, const double* qtilde, double* du);
+void launch_prev_x0(hipStream_t s, const Geo& G, const double* alpha, const double* du, double* x0);
 void launch_step_pred(hipStream_t s, const Geo& G, int dpo, const float* Y, const float* omean, const float* ostd,
                       double dt, const double* q, const double* qnow, const double* qdot, double* du);
 void launch_dmin(hipStream_t s, const Geo& G, int n, const uint64_t* keys, const double4* X, const AbdParams& P,

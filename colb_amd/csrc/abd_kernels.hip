@@ -1283,6 +1283,21 @@ void launch_warm_x0(hipStream_t s, const Geo& G, int dpo, const float* Y, const 
     hipLaunchKernelGGL(k_warm_x0, dim3(cdiv(12 * G.nb, 256)), dim3(256), 0, s, G, dpo, Y, omean, ostd, feat, x0);
 }
 
+// x0 = (1 - alpha_s) du: the part of the previous Newton direction that its line search did not
+// take (pinned bodies and non-finite values: 0)
+__global__ void k_prev_x0(Geo G, const double* __restrict__ alpha, const double* __restrict__ du,
+                          double* __restrict__ x0) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= 12 * G.nb) return;
+    const int b = e / 12;
+    const double v = (1.0 - alpha[G.bscene[b]]) * du[e];
+    x0[e] = (G.pinned[b] || !isfinite(v)) ? 0.0 : v;
+}
+
+void launch_prev_x0(hipStream_t s, const Geo& G, const double* alpha, const double* du, double* x0) {
+    hipLaunchKernelGGL(k_prev_x0, dim3(cdiv(12 * G.nb, 256)), dim3(256), 0, s, G, alpha, du, x0);
+}
+
 // Learned block preconditioner: P_b = L_D^-T (T T^T) L_D^-1, where D_b = L_D L_D^T is the body's
 // diagonal block and T the lower-triangular factor the model predicts (diagonal exp of its first 12
 // outputs, the 66 entries below the diagonal row by row from the rest), both de-standardised.

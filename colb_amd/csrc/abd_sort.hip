@@ -1,7 +1,10 @@
 // abd_sort.hip — stable LSD radix sort of 64-bit keys (optionally carrying int values) and an
 // exclusive prefix sum of ints, for the candidate, incidence and body-pair lists.
 //
-// Radix sort: 4-bit digits; up to SORT_TILE keys are ranked directly (k_sort_small).  Per pass, each 256-thread block owns a tile of 2048 consecutive
+// Radix sort: 4-bit digits over the bits the caller says may be set (a mask: one pass per 4-bit
+// window that holds a significant bit, windows placed greedily from the lowest such bit, so a key
+// made of a few packed fields costs the passes of its fields' widths only); up to SORT_SMALL_MAX
+// keys are ranked directly (k_sort_small).  Per pass, each 256-thread block owns a tile of 2048 consecutive
 // elements (8 per thread, in order); it counts digits (k_radix_hist), the digit-major histogram
 // [digit][block] is prefix-summed, and the block re-reads its tile, ranks every element among
 // the equal digits of earlier threads and earlier items (a block-wide scan over a [digit][thread]
@@ -173,15 +176,16 @@ __global__ __launch_bounds__(SORT_T) void k_radix_scatter(int n, const uint64_t*
     }
 }
 
-// Whole sort of n <= SORT_TILE keys by counting ranks: the final position of key i is
+// Whole sort of n <= SORT_SMALL_MAX keys by counting ranks: the final position of key i is
 // #{j : k_j < k_i} + #{j < i : k_j == k_i} (stable).  Every workgroup holds all n keys in LDS and
 // ranks 64 of them, four lanes per key each counting over a quarter of the keys (n / 4
 // comparisons per lane; the keys a wave reads at one step are a few LDS addresses, broadcast).
 #define SORT_SMALL_KEYS 64
+#define SORT_SMALL_MAX 4096
 __global__ __launch_bounds__(SORT_T) void k_sort_small(int n, const uint64_t* __restrict__ kin,
                                                         const int* __restrict__ vin, uint64_t* __restrict__ kout,
                                                         int* __restrict__ vout) {
-    __shared__ uint64_t kb[SORT_TILE];
+    __shared__ uint64_t kb[SORT_SMALL_MAX];
     const int t = threadIdx.x;
     for (int i = t; i < n; i += SORT_T) kb[i] = kin[i];
     __syncthreads();
@@ -208,8 +212,20 @@ size_t radix_scratch_bytes(long n) {
     return (size_t)n * 8 + (size_t)n * 4 + (size_t)(2 * RADIX * nblk + scan_scratch_ints(RADIX * nblk) + 64) * 4 + 1024;
 }
 
+// the 4-bit windows (their lowest bit) that cover every set bit of `sig`, lowest first
+static int radix_windows(uint64_t sig, int* shift) {
+    int n = 0;
+    for (int b = 0; b < 64;) {
+        if ((sig >> b) == 0) break;
+        if (((sig >> b) & 1ull) == 0) { ++b; continue; }
+        shift[n++] = b;
+        b += 4;
+    }
+    return n;
+}
+
 hipError_t radix_sort64(hipStream_t s, const uint64_t* kin, uint64_t* kout, const int* vin, int* vout, int n,
-                        int end_bit, void* scratch) {
+                        uint64_t sig, void* scratch) {
     if (n <= 0) return hipSuccess;
     const int nblk = cdivs(n, SORT_TILE);
     char* p = (char*)scratch;
@@ -220,8 +236,9 @@ hipError_t radix_sort64(hipStream_t s, const uint64_t* kin, uint64_t* kout, cons
     int* hist = (int*)p;
     int* hoff = hist + RADIX * nblk;
     int* bsum = hoff + RADIX * nblk;
-    const int passes = (end_bit + 3) / 4;
-    if (n <= SORT_TILE) {
+    int shifts[64];
+    const int passes = radix_windows(sig, shifts);
+    if (n <= SORT_SMALL_MAX) {
         hipLaunchKernelGGL(k_sort_small, dim3(cdivs(n, SORT_SMALL_KEYS)), dim3(SORT_T), 0, s, n, kin, vin, kout,
                            vin ? vout : nullptr);
         return hipGetLastError();
@@ -238,7 +255,7 @@ hipError_t radix_sort64(hipStream_t s, const uint64_t* kin, uint64_t* kout, cons
         bool to_out = ((passes - 1 - pass) % 2) == 0;
         uint64_t* kd = to_out ? kout : ktmp;
         int* vd = vin ? (to_out ? vout : vtmp) : nullptr;
-        int shift = 4 * pass;
+        const int shift = shifts[pass];
         hipLaunchKernelGGL(k_radix_hist, dim3(nblk), dim3(SORT_T), 0, s, n, ks, shift, nblk, hist);
         hipError_t e = excl_scan_int(s, hist, hoff, RADIX * nblk, bsum);
         if (e != hipSuccess) return e;

@@ -43,6 +43,7 @@ class Params:
     k_max: float = 0.0  # adaptive contact stiffness cap (<= k_contact: fixed stiffness)
     k_raise_gap: float = 0.5  # raise the stiffness when a contact gets closer than this * dhat
     cand_margin: float = None  # candidate-cache margin (length); None: 4 dhat; <= 0: no cache
+    pcg_x0: int = 0  # PCG initial guess: 0 = zero (the reference), 1 = the untaken part of the previous direction
     debug: int = 0  # bit 0: synchronise and check after every launch; bit 1: also log them;
     #                 bit 2: print broad-phase list sizes
 
@@ -76,6 +77,7 @@ class Params:
         p.k_max = self.k_max
         p.k_raise_gap = self.k_raise_gap
         p.cand_margin = 4.0 * self.dhat if self.cand_margin is None else self.cand_margin
+        p.pcg_x0 = self.pcg_x0
         return p
 
     def replace(self, **kw):

@@ -88,6 +88,12 @@ typedef struct AbdParams {
                                   whose vertices stay within half of it of the cached boxes by the
                                   exact candidate rule over that list (the same candidates, without a
                                   traversal); <= 0: a traversal at every detection (contact.py:244-279) */
+    int    pcg_x0;             /* initial guess of the PCG: 0 = zero (abd.py:632); 1 = from the second
+                                  Newton iteration of a step on, the part (1 - alpha) du of the previous
+                                  direction that its line search did not take, kept per scene where
+                                  its residual is below ||grad|| (the stopping test stays relative
+                                  to ||grad||).  A learned warm start (abd_set_warm_start) replaces it */
+    int    reserved1;
 } AbdParams;
 
 typedef struct AbdSceneDesc {

@@ -931,13 +931,35 @@ void spmv(const Ctx& C, const System& S, const std::vector<double>& x, std::vect
 }
 
 // Block-Jacobi PCG per scene (replaces warp.optim.linear.bicgstab, abd.py:636).  Starts from
-// du = 0 (abd.py:632).  Returns per-scene iteration counts.
-void pcg(const Ctx& C, const System& S, std::vector<double>& du, std::vector<int>& iters) {
+// du = 0 (abd.py:632), or with x0 (AbdParams.pcg_x0) from x0 in every scene where the residual
+// grad - H x0 is smaller than grad (the stopping test stays relative to ||grad||).  Returns
+// per-scene iteration counts.
+void pcg(const Ctx& C, const System& S, std::vector<double>& du, std::vector<int>& iters,
+         const std::vector<double>* x0in = nullptr) {
     int n = 12 * C.nb;
     std::vector<double> Pinv(144 * C.nb);
     for (int b = 0; b < C.nb; ++b) chol_inverse12(&S.diag[144 * b], &Pinv[144 * b]);
     du.assign(n, 0.0);
     std::vector<double> r = S.grad, z(n), p(n), Ap;
+    std::vector<double> x0, bref(C.ns, -1.0);
+    if (x0in) {
+        x0 = *x0in;
+        spmv(C, S, x0, Ap);
+        for (int s = 0; s < C.ns; ++s) {
+            double nb2 = 0, nr2 = 0;
+            for (int i = 12 * C.scene_b0[s]; i < 12 * C.scene_b1[s]; ++i) {
+                nb2 += S.grad[i] * S.grad[i];
+                const double ri = S.grad[i] - Ap[i];
+                nr2 += ri * ri;
+            }
+            bref[s] = std::sqrt(nb2);
+            const bool keep = std::sqrt(nr2) < bref[s];
+            for (int i = 12 * C.scene_b0[s]; i < 12 * C.scene_b1[s]; ++i) {
+                if (keep) r[i] = S.grad[i] - Ap[i];
+                else x0[i] = 0.0;
+            }
+        }
+    }
     auto prec = [&](const std::vector<double>& in, std::vector<double>& out) {
         for (int b = 0; b < C.nb; ++b)
             for (int i = 0; i < 12; ++i) {
@@ -958,7 +980,7 @@ void pcg(const Ctx& C, const System& S, std::vector<double>& du, std::vector<int
     iters.assign(C.ns, 0);
     for (int s = 0; s < C.ns; ++s) {
         rz[s] = sdot(r, z, s);
-        bn[s] = std::sqrt(sdot(r, r, s));
+        bn[s] = bref[s] >= 0.0 ? bref[s] : std::sqrt(sdot(r, r, s));
         if (bn[s] == 0) done[s] = 1;
     }
     for (int it = 0; it < C.P.cg_max_iter; ++it) {
@@ -989,6 +1011,8 @@ void pcg(const Ctx& C, const System& S, std::vector<double>& du, std::vector<int
             iters[s] = it + 1;
         }
     }
+    if (x0in)
+        for (int i = 0; i < n; ++i) du[i] += x0[i];
 }
 
 // Per-scene CCD bound of q -> q - du over swept candidates (inflated by dhat).
@@ -1125,7 +1149,7 @@ void step_once(Ctx& C) {
     for (auto& st : C.stats) { st.alpha_min = 1.0; st.ccd_alpha_min = 1.0; }
     std::vector<char> active(C.ns, 1);
     std::vector<double> dmin_prev(C.nb, HUGE_VAL);
-    std::vector<double> alpha, amax;
+    std::vector<double> alpha, amax, du_prev;
     std::vector<int> floor_hit;
     if (C.P.predictor >= 1 && C.P.predictor <= 3) {
         std::vector<double> dup(12 * C.nb, 0.0);
@@ -1166,8 +1190,23 @@ void step_once(Ctx& C) {
                 for (int i = 12 * C.scene_b0[s]; i < 12 * C.scene_b1[s]; ++i) S.grad[i] = 0;
         std::vector<double> du;
         std::vector<int> cgit;
-        pcg(C, S, du, cgit);
+        if (C.P.pcg_x0 == 1 && it > 0) {
+            // the part of the previous direction its line search did not take
+            std::vector<double> x0(12 * C.nb, 0.0);
+            for (int b = 0; b < C.nb; ++b) {
+                if (C.pinned[b]) continue;
+                const double a = alpha[C.bscene[b]];
+                for (int k = 0; k < 12; ++k) {
+                    const double v = (1.0 - a) * du_prev[12 * b + k];
+                    x0[12 * b + k] = std::isfinite(v) ? v : 0.0;
+                }
+            }
+            pcg(C, S, du, cgit, &x0);
+        } else {
+            pcg(C, S, du, cgit);
+        }
         line_search(C, du, active, true, alpha, amax, floor_hit);
+        du_prev = du;
         adapt_stiffness(C, active, dmin_prev);
         for (int s = 0; s < C.ns; ++s) {
             if (!active[s]) continue;

@@ -10,17 +10,20 @@ hipError_t excl_scan_int(hipStream_t s, const int* in, int* out, int n, int* bsu
 int scan_scratch_ints(long n);
 size_t radix_scratch_bytes(long n);
 hipError_t radix_sort64(hipStream_t s, const uint64_t* kin, uint64_t* kout, const int* vin, int* vout, int n,
-                        int end_bit, void* scratch);
+                        uint64_t sig, void* scratch);
 }  // namespace abd
 int main() {
     std::mt19937_64 rng(1);
     int fails = 0;
-    for (int n : {1, 2, 7, 100, 2047, 2048, 2049, 5000, 40000}) {
-        for (int bits : {13, 33, 64}) {
+    // masks: low bits, and packed fields with gaps (windows that overlap the next field)
+    const uint64_t masks[] = {(1ull << 13) - 1, (1ull << 33) - 1, ~0ull, (0x3FFull << 32) | 0x1FFFull,
+                              (0x3FFull << 44) | (0x3FFull << 24) | 0xFFFull, (0x7ull << 61) | 0x5ull};
+    for (int n : {1, 2, 7, 100, 2047, 2048, 2049, 4096, 4097, 5000, 40000}) {
+        for (uint64_t mask : masks) {
             std::vector<uint64_t> k(n);
             std::vector<int> v(n);
             for (int i = 0; i < n; ++i) {
-                k[i] = bits == 64 ? rng() : (rng() & ((1ull << bits) - 1));
+                k[i] = rng() & mask;
                 if (i % 3 == 0 && i) k[i] = k[i - 1];
                 v[i] = i;
             }
@@ -31,7 +34,7 @@ int main() {
             hipMalloc(&tmp, abd::radix_scratch_bytes(n));
             hipMemcpy(dk, k.data(), n * 8, hipMemcpyHostToDevice);
             hipMemcpy(dv, v.data(), n * 4, hipMemcpyHostToDevice);
-            abd::radix_sort64(nullptr, dk, dko, dv, dvo, n, bits, tmp);
+            abd::radix_sort64(nullptr, dk, dko, dv, dvo, n, mask, tmp);
             std::vector<uint64_t> ko(n);
             std::vector<int> vo(n);
             hipMemcpy(ko.data(), dko, n * 8, hipMemcpyDeviceToHost);
@@ -42,7 +45,7 @@ int main() {
             bool ok = true;
             for (int i = 0; i < n; ++i) ok &= (ko[i] == k[idx[i]] && vo[i] == idx[i]);
             // keys-only
-            abd::radix_sort64(nullptr, dk, dko, nullptr, nullptr, n, bits, tmp);
+            abd::radix_sort64(nullptr, dk, dko, nullptr, nullptr, n, mask, tmp);
             hipMemcpy(ko.data(), dko, n * 8, hipMemcpyDeviceToHost);
             for (int i = 0; i < n; ++i) ok &= (ko[i] == k[idx[i]]);
             // scan of small ints
@@ -53,7 +56,7 @@ int main() {
             hipMemcpy(out.data(), dvo, n * 4, hipMemcpyDeviceToHost);
             int acc = 0;
             for (int i = 0; i < n; ++i) { ok &= out[i] == acc; acc += a[i]; }
-            std::printf("n=%6d bits=%2d %s\n", n, bits, ok ? "ok" : "FAIL");
+            std::printf("n=%6d mask=%016llx %s\n", n, (unsigned long long)mask, ok ? "ok" : "FAIL");
             fails += !ok;
             hipFree(dk); hipFree(dko); hipFree(dv); hipFree(dvo); hipFree(tmp);
         }

@@ -144,6 +144,8 @@ TRAJ = {
     "adaptive": lambda: (cube_on_ground(0.8 * DHAT, tilt_deg=(0.0, 0.0, 0.0), density=1e4),
                          ground_params(k_contact=1e8, k_max=1e14, k_raise_gap=0.45), 25),
     "mirror": lambda: (S.chain_net(n=3), north_star(dhat=2e-3, k_contact=1e19, hess_proj=1), 5),
+    "pcg_x0": lambda: (S.cubes_on_ground(lift=0.02),
+                       north_star(ground_enabled=True, dhat=0.05, k_contact=1e14, pcg_x0=1), 20),
 }
 
 
