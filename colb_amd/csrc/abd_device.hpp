@@ -335,37 +335,45 @@ __device__ inline bool wave_is_psd(const double* A, double* L, int m, int lane) 
     return true;
 }
 
-// Jacobi PSD projection of the symmetric m x m matrix A (LDS, row-major, m <= 12), by one wave:
-// A <- V max(Lambda, 0) V^T (Golub & Van Loan 8.5, parallel ordering 8.5.3).  The projection is a
-// unique matrix function, so it agrees with the oracle's cyclic-order psd_project to rounding;
-// same stopping rule.  V is LDS scratch of m*m doubles.  absolute: A <- V |Lambda| V^T instead
-// (negative eigenvalues mirrored rather than clamped).
-// The size is a template parameter (every index below is then a compile-time division);
-// wave_psd_project dispatches on the runtime size.
+// PSD projection of the symmetric m x m matrix A (LDS, row-major, m <= 12) by one wave, through
+// the matrix sign function: A+ = (A + A S) / 2 with S = sign(A) (A S = |A|), the projection onto
+// the PSD cone that clamping the eigenvalues at zero gives (a unique matrix function, so it agrees
+// with the oracle's Jacobi-based psd_project to rounding).  S comes from Newton-Schulz iterations
+// on X0 = A / ||A||_F (eigenvalues in [-1, 1]; Higham, Functions of Matrices, 5.3): 24 quintic
+// steps X <- X (a + b X^2 + c X^4), (a, b, c) = (3.4445, -4.7750, 2.0315) (the quintic of Jordan et
+// al.'s Muon optimiser), which multiply small eigenvalues by 3.44 per step and keep the others in
+// [0.68, 1.21], then 6 cubic steps X <- X (3 - X^2) / 2, which converge quadratically to +-1.  An
+// eigenvalue below ~1e-14 ||A||_F may end with a partly converged sign; its contribution to A+ is
+// bounded by its own size, so the result is accurate to ~1e-14 of ||A||.  Every product is of two
+// polynomials in A, i.e. symmetric: each lane forms upper-triangle entries (lane, lane + 64) and
+// mirrors them.  V and the two static buffers are the iteration's LDS scratch.  absolute: A <- |A|
+// (eigenvalues mirrored rather than clamped).
 template <int M>
-__device__ inline void wave_psd_project_m(double* A, double* V, int lane, bool absolute, double* A2, double* V2,
-                                          double* jw_self, double* jw_part, int* j_part) {
+__device__ inline void sym_product(const double* X, const double* Y, double* Z, const int* ui, const int* uj) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int i = ui[h], j = uj[h];
+        if (i < 0) continue;
+        double z = 0.0;
+#pragma unroll
+        for (int k = 0; k < M; ++k) z = fma(X[i * M + k], Y[j * M + k], z);  // Y symmetric: Y[k][j] = Y[j][k]
+        Z[i * M + j] = z;
+        Z[j * M + i] = z;
+    }
+    __syncthreads();
+}
+
+template <int M>
+__device__ inline void wave_psd_project_m(double* A, double* V, int lane, bool absolute, double* T1, double* T2) {
     constexpr int m = M;
     constexpr int mm = m * m;
-    for (int e = lane; e < mm; e += 64) V[e] = (e / m == e % m) ? 1.0 : 0.0;
+    constexpr int nup = m * (m + 1) / 2;  // <= 78
     double f = 0;
     for (int e = lane; e < mm; e += 64) f += A[e] * A[e];
-    double fro = wave_sum(f);
-    __syncthreads();
-    if (fro == 0.0) return;
-    // Parallel (round-robin) Jacobi: a sweep is npl - 1 rounds of npl / 2 disjoint rotations
-    // (circle method over npl = m rounded up to even players; a pair with the extra player is
-    // skipped).  Lane k < npl/2 computes the k-th rotation of the round once and records, for each
-    // index, its own weight and its partner's (J restricted to the pair).  Then every element of
-    // A' = J^T A J (upper triangle, mirrored) and V' = V J is formed from the old matrices in one
-    // pass into the other buffer: two barriers per round.
-    double* Ac = A;
-    double* Vc = V;
-    double* An = A2;
-    double* Vn = V2;
-    const int npl = m + (m & 1), half = npl / 2;
-    const int nup = m * (m + 1) / 2;  // <= 78: each lane owns upper-triangle elements lane, lane + 64
+    const double fro = wave_sum(f);
+    if (fro == 0.0 || !(fro < INFINITY)) return;
     int ui[2] = {-1, -1}, uj[2] = {0, 0};
+#pragma unroll
     for (int h = 0; h < 2; ++h) {
         const int t = lane + 64 * h;
         if (t < nup) {
@@ -375,97 +383,36 @@ __device__ inline void wave_psd_project_m(double* A, double* V, int lane, bool a
             uj[h] = i + rem;
         }
     }
-    for (int sweep = 0; sweep < 32; ++sweep) {
-        double o = 0;
-        for (int e = lane; e < mm; e += 64) {
-            int p = e / m, q = e % m;
-            if (q > p) o += Ac[e] * Ac[e];
-        }
-        double off = wave_sum(o);
-        // converged once the off-diagonal part is below 1e-14 of the Frobenius norm (rounding keeps
-        // it from going much lower: a tighter test only adds sweeps)
-        if (off <= 1e-28 * fro) break;
-        for (int r = 0; r < npl - 1; ++r) {
-            if (lane < half) {
-                const int k = lane, n1 = npl - 1;
-                int a = k - 1 + r, b = npl - 2 - k + r;  // both < 2 (npl - 1): one conditional wrap
-                a = (k == 0) ? 0 : 1 + (a >= n1 ? a - n1 : a);
-                b = 1 + (b >= n1 ? b - n1 : b);
-                const int p = a < b ? a : b, q = a < b ? b : a;
-                double c = 1.0, sn = 0.0;
-                if (q < m) {
-                    const double apq = Ac[p * m + q];
-                    if (apq != 0.0) {
-                        // tan of the rotation angle, t = sign(theta) / (|theta| + sqrt(theta^2 + 1))
-                        const double theta = (Ac[q * m + q] - Ac[p * m + p]) * 0.5 * rcp_nr(apq);
-                        double tt;
-                        if (fabs(theta) > 1e150) {
-                            tt = 0.5 * rcp_nr(theta);
-                        } else {
-                            const double h = fma(theta, theta, 1.0);
-                            tt = (theta >= 0 ? 1.0 : -1.0) * rcp_nr(fabs(theta) + h * rsq_nr(h));
-                        }
-                        c = rsq_nr(fma(tt, tt, 1.0));
-                        sn = tt * c;
-                    }
-                    // column p of J is (c at p, -s at q), column q is (s at p, c at q)
-                    jw_self[p] = c; jw_part[p] = -sn; j_part[p] = q;
-                    jw_self[q] = c; jw_part[q] = sn; j_part[q] = p;
-                } else if (p < m) {
-                    jw_self[p] = 1.0; jw_part[p] = 0.0; j_part[p] = p;  // paired with the extra player
-                }
-            }
-            __syncthreads();
-            // A'[i][j] = sum over a in {i, i'}, b in {j, j'} of J[a][i] A[a][b] J[b][j]
-#pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                const int i = ui[h], j = uj[h];
-                if (i < 0) continue;
-                const int ip = j_part[i], jp = j_part[j];
-                const double wi = jw_self[i], wip = jw_part[i], wj = jw_self[j], wjp = jw_part[j];
-                const double v = wi * (wj * Ac[i * m + j] + wjp * Ac[i * m + jp]) +
-                                 wip * (wj * Ac[ip * m + j] + wjp * Ac[ip * m + jp]);
-                An[i * m + j] = v;
-                An[j * m + i] = v;
-            }
-            // V'[i][j] = V[i][j] J[j][j] + V[i][j'] J[j'][j]
-            for (int e = lane; e < mm; e += 64) {
-                const int i = e / m, j = e % m;
-                Vn[e] = jw_self[j] * Vc[i * m + j] + jw_part[j] * Vc[i * m + j_part[j]];
-            }
-            __syncthreads();
-            double* tA = Ac; Ac = An; An = tA;
-            double* tV = Vc; Vc = Vn; Vn = tV;
-        }
-    }
-    // reconstruct with clamped eigenvalues (diagonal of the final A) into the caller's A
-    double out[3];
-    int cnt = 0;
-    for (int e = lane; e < mm; e += 64) {
-        int r = e / m, c = e % m;
-        double h = 0;
-        for (int k = 0; k < m; ++k) {
-            const double lam = Ac[k * m + k];
-            h += Vc[r * m + k] * (absolute ? fabs(lam) : fmax(lam, 0.0)) * Vc[c * m + k];
-        }
-        out[cnt++] = h;
-    }
+    const double inv = rsq_nr(fro);
+    double* X = V;
+    for (int e = lane; e < mm; e += 64) X[e] = A[e] * inv;
     __syncthreads();
-    cnt = 0;
-    for (int e = lane; e < mm; e += 64) A[e] = out[cnt++];
+    for (int it = 0; it < 30; ++it) {
+        const bool quintic = it < 24;
+        sym_product<M>(X, X, T1, ui, uj);  // X^2
+        if (quintic) {
+            sym_product<M>(T1, T1, T2, ui, uj);  // X^4
+            for (int e = lane; e < mm; e += 64)
+                T2[e] = 2.0315 * T2[e] - 4.7750 * T1[e] + ((e / m == e % m) ? 3.4445 : 0.0);
+        } else {
+            for (int e = lane; e < mm; e += 64) T2[e] = ((e / m == e % m) ? 1.5 : 0.0) - 0.5 * T1[e];
+        }
+        __syncthreads();
+        sym_product<M>(X, T2, T1, ui, uj);  // X p(X^2)
+        double* t = X; X = T1; T1 = t;
+    }
+    sym_product<M>(A, X, T1, ui, uj);  // |A| = A S
+    for (int e = lane; e < mm; e += 64) A[e] = absolute ? T1[e] : 0.5 * (A[e] + T1[e]);
     __syncthreads();
 }
 
 __device__ inline void wave_psd_project(double* A, double* V, int m, int lane, bool absolute = false) {
-    // the rotated copies of A and V and the rotation weights of a round
-    __shared__ double A2[144], V2[144];
-    __shared__ double jw_self[12], jw_part[12];
-    __shared__ int j_part[12];
+    __shared__ double T1[144], T2[144];
     switch (m) {
-        case 3: wave_psd_project_m<3>(A, V, lane, absolute, A2, V2, jw_self, jw_part, j_part); break;
-        case 6: wave_psd_project_m<6>(A, V, lane, absolute, A2, V2, jw_self, jw_part, j_part); break;
-        case 9: wave_psd_project_m<9>(A, V, lane, absolute, A2, V2, jw_self, jw_part, j_part); break;
-        case 12: wave_psd_project_m<12>(A, V, lane, absolute, A2, V2, jw_self, jw_part, j_part); break;
+        case 3: wave_psd_project_m<3>(A, V, lane, absolute, T1, T2); break;
+        case 6: wave_psd_project_m<6>(A, V, lane, absolute, T1, T2); break;
+        case 9: wave_psd_project_m<9>(A, V, lane, absolute, T1, T2); break;
+        case 12: wave_psd_project_m<12>(A, V, lane, absolute, T1, T2); break;
         default: break;  // callers pass 3, 6, 9 or 12
     }
 }

@@ -311,14 +311,17 @@ def test_iter_log_and_profiler_match_oracle(O, tmp_path):
         so, sg = orc.step()[0], gpu.step()[0]
         ro, ao, co = orc.iter_log()
         rg, ag, cg = gpu.iter_log()
-        n = sg["newton_iters"]
-        assert n == so["newton_iters"]
-        # iterates that differ by rounding: residuals agree to 1e-6 of the step's largest
-        np.testing.assert_allclose(rg[0, :n], ro[0, :n], rtol=0, atol=1e-6 * ro[0, :n].max())
+        n, no = sg["newton_iters"], so["newton_iters"]
+        assert sg["converged"] and so["converged"]
         # steps from iterates within 1e3 tol of convergence change the energy by less than the 1e-11
-        # to which the two energies agree: accepting or reaching the floor there is a rounding tie
-        big = ro[0, :n] > 1e3 * p.newton_tol
-        np.testing.assert_allclose(ag[0, :n][big], ao[0, :n][big], rtol=1e-6)
+        # to which the two energies agree: accepting or reaching the floor there is a rounding tie,
+        # after which the two iteration paths (and counts) may differ.  Up to that point both logs
+        # hold the same iterations: residuals within 1e-6 of the step's largest, equal steps.
+        far = int(np.argmax(ro[0, :no] <= 1e3 * p.newton_tol))
+        assert far > 0 and int(np.argmax(rg[0, :n] <= 1e3 * p.newton_tol)) == far
+        np.testing.assert_allclose(rg[0, :far + 1], ro[0, :far + 1], rtol=0, atol=1e-6 * ro[0, :no].max())
+        np.testing.assert_allclose(ag[0, :far], ao[0, :far], rtol=1e-6)
+        assert rg[0, n - 1] < p.newton_tol and ro[0, no - 1] < p.newton_tol
         assert np.all(rg[0, n:] == 0) and np.all(ag[0, n:] == 0)
         np.testing.assert_array_equal(gpu.profiler.convergence[f, :n], rg[0, :n])
     path = tmp_path / "profiler.npz"
