@@ -2,10 +2,12 @@
 MI355X, on the 32x32 torus chain net (1024 links of 288 verts / 576 tris, four corners pinned,
 FP64, dt 8 ms, dhat 1e-3, kappa 1e7, 16 Newton iterations, tolerance 1e-5).
 
-Each method (plain: the reference's Newton step; learned: the first iterate of every step moved
-toward a learned prediction) runs the workload's steps from its initial state after a few untimed
-warmup steps; `value` is the first method's rate, `methods` holds every method's rate, Newton
-iterations, converged fraction and worst final residual.  The CPU baseline is the project's FP64
+Each method (plain: the reference's Newton step; warm: every PCG after a step's first Newton
+iteration starts from the part of the previous direction its line search did not take; learned:
+warm plus the first iterate of every step moved toward a learned per-body prediction) runs the
+workload's steps from its initial state after a few untimed warmup steps; `value` is the first
+method's rate, `methods` holds every method's rate, Newton iterations, converged fraction and worst
+final residual.  The CPU baseline is the project's FP64
 oracle (oracle/, "port") timed on this host on a bounded sample of the same workload (whole steps
 from the GPU's state at the middle of the timed run), single-threaded and with OpenMP.
 """
@@ -127,33 +129,46 @@ METHODS = {
     # name -> (parameter changes, description)
     "plain": ({}, "the reference's Newton step: first iterate q_now (abd.py:674-699)"),
     "extrapolate": ({"predictor": 1}, "first iterate moved toward q_now + dt qdot (CCD-safe, energy-decreasing)"),
-    "learned": ({"predictor": 3}, "first iterate moved toward the learned per-body prediction of the step"),
+    "warm": ({"pcg_x0": 1}, "PCG from the untaken part (1 - alpha) du of the previous Newton direction"),
+    "learned": ({"predictor": 3, "pcg_x0": 1},
+                "warm, and the first iterate moved toward the learned per-body prediction of the step"),
 }
 
 
 def train_learned(sc, p, args):
-    """The learned predictor, trained on the GPU on the (features, step) pairs that a plain run of a
-    perturbed copy of the workload logs (other initial velocities: a trajectory the timed run never
-    sees)."""
+    """The learned predictor, trained on the GPU on the (features, step) pairs of the converged steps
+    that plain runs of perturbed copies of the workload log (other initial velocities: trajectories
+    the timed run never sees; a step that ends at the Newton cap is no target)."""
     from colb_amd.abd import NewtonAbd
     from colb_amd.learned import StepPredictor
-    rng = np.random.default_rng(12345)
-    q0, qn, qd = sc.q0.copy(), sc.q0.copy(), sc.qdot0.copy()
-    qd[:, :3] += rng.normal(scale=0.05, size=qd[:, :3].shape) * (1 - sc.body_pinned[:, None])
-    g = NewtonAbd(p.dt, sc, params=p)
-    g.set_state(q0, qn, qd)
-    g.log_steps(True)
+    free = int((sc.body_pinned == 0).sum())  # rows per step (one scene)
+    Fs, Ds, conv_steps = [], [], 0
     t0 = time.perf_counter()
-    for _ in range(args.train_steps):
-        g.step()
-    F, D = g.step_samples()
-    g.close()
+    for run in range(args.train_runs):
+        rng = np.random.default_rng(12345 + run)
+        qd = sc.qdot0.copy()
+        qd[:, :3] += rng.normal(scale=0.05, size=qd[:, :3].shape) * (1 - sc.body_pinned[:, None])
+        g = NewtonAbd(p.dt, sc, params=p)
+        g.set_state(sc.q0, sc.q0, qd)
+        g.log_steps(True)
+        conv = []
+        for _ in range(args.train_steps):
+            sts = g.step()
+            conv.append(all(x["converged"] for x in sts))
+        F, D = g.step_samples()
+        g.close()
+        keep = np.repeat(np.array(conv, dtype=bool), free) if sc.n_scenes == 1 else np.ones(len(F), dtype=bool)
+        Fs.append(F[keep])
+        Ds.append(D[keep])
+        conv_steps += int(sum(conv))
+    F, D = np.concatenate(Fs), np.concatenate(Ds)
     t_log = time.perf_counter() - t0
     sp = StepPredictor(seed=0)
     t1 = time.perf_counter()
     losses = sp.fit(F, D, p.dt, steps=args.train_iters, batch=1024, lr=1e-3)
     info = {"model": "49-128-128-128-12 MLP, BF16 weights, FP32 accumulation", "training_rows": int(len(F)),
-            "training_steps_logged": args.train_steps, "adam_steps": args.train_iters,
+            "training_runs": args.train_runs, "training_steps_logged": args.train_runs * args.train_steps,
+            "training_steps_converged": conv_steps, "adam_steps": args.train_iters,
             "loss_first": round(float(losses[:20].mean()), 4), "loss_last": round(float(losses[-100:].mean()), 4),
             "log_s": round(t_log, 2), "train_s": round(time.perf_counter() - t1, 2)}
     return sp, info
@@ -245,8 +260,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5, help="untimed steps that size the buffers before the run")
     ap.add_argument("--workload", default="chain_net_32x32", choices=["chain_net_32x32", "box_pile_4096",
                                                                    "hull_piles_256", "ref_chain_144", "cubes_8"])
-    ap.add_argument("--methods", default="plain,learned",
-                    help="comma-separated: plain (the reference's step), extrapolate, learned (trained first)")
+    ap.add_argument("--methods", default="plain,warm,learned",
+                    help="comma-separated: plain (the reference's step), extrapolate, warm, learned (trained first)")
     ap.add_argument("--solver", default="pcg", choices=["pcg", "direct", "fallback"],
                     help="linear solve: block-Jacobi PCG, dense Cholesky per scene, or PCG with the direct "
                          "solve for scenes whose PCG ends at its iteration cap")
@@ -256,7 +271,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=2)
     ap.add_argument("--no-autotune", action="store_true")
-    ap.add_argument("--train-steps", type=int, default=60, help="steps of the perturbed run the learned model uses")
+    ap.add_argument("--train-steps", type=int, default=100, help="steps of each perturbed run the learned model uses")
+    ap.add_argument("--train-runs", type=int, default=2, help="perturbed runs logged for the learned model")
     ap.add_argument("--train-iters", type=int, default=3000, help="Adam steps of the learned model")
     ap.add_argument("--pcg-layout", default=None, metavar="BPT[,multi]",
                     help="fix the PCG layout (bodies per tile, 1 = one launch per iteration) instead of autotuning")
@@ -295,6 +311,8 @@ def main():
             roof, kt, pcg_us = kernel_roofline(solver, cfg, args.precision, args.profile_steps)
             out["pcg_us_per_iteration"] = pcg_us
         solver.close()
+        if model is not None:
+            model.mlp.close()  # device memory released before the interpreter's teardown
     head = results[methods[0]]
     cpu = None
     if not args.no_cpu_baseline and mid_state is not None:
@@ -312,6 +330,8 @@ def main():
     ratio = None
     if "learned" in results and "plain" in results:
         ratio = round(results["learned"]["sim_ms_per_wall_ms"] / results["plain"]["sim_ms_per_wall_ms"], 4)
+    vs_plain = {m: round(results[m]["sim_ms_per_wall_ms"] / results["plain"]["sim_ms_per_wall_ms"], 4)
+                for m in results if m != "plain"} if "plain" in results else None
     result = {
         "metric": METRIC, "value": head["sim_ms_per_wall_ms"], "unit": "sim-ms/wall-ms", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": head["ms_per_step"],
@@ -324,6 +344,7 @@ def main():
         "method": methods[0], "newton_iters_per_step": head["newton_iters_per_step"],
         "converged_fraction": head["converged_fraction"], "worst_final_residual": head["worst_final_residual"],
         "cg_iters_per_step": head["cg_iters_per_step"], "methods": results, "learned_vs_plain": ratio,
+        "vs_plain": vs_plain,
         "learned": learned_info, "roofline": roof, "cpu_baseline": cpu,
     }
     print(json.dumps(result), flush=True)

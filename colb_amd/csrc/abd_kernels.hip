@@ -1643,17 +1643,21 @@ void launch_stiffness_update(hipStream_t s, const Geo& G, const AbdParams& P, co
     hipLaunchKernelGGL(k_stiffness_update, dim3(cdiv(G.nb, 128)), dim3(128), 0, s, G, P, active, c);
 }
 
-// largest body stiffness of each scene (one thread per scene)
-__global__ void k_scene_kmax(Geo G, SceneCtl c) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= G.ns) return;
+// largest body stiffness of each scene (one workgroup per scene)
+__global__ __launch_bounds__(256) void k_scene_kmax(Geo G, SceneCtl c) {
+    __shared__ double sh[4];
+    const int s = blockIdx.x;
     double k = 0.0;
-    for (int b = G.scene_b0[s]; b < G.scene_b0[s + 1]; ++b) k = fmax(k, c.kb[b]);
-    c.kc[s] = k;
+    for (int b = G.scene_b0[s] + threadIdx.x; b < G.scene_b0[s + 1]; b += 256) k = fmax(k, c.kb[b]);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) k = fmax(k, __shfl_xor(k, o, 64));
+    if (threadIdx.x % 64 == 0) sh[threadIdx.x / 64] = k;
+    __syncthreads();
+    if (threadIdx.x == 0) c.kc[s] = fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
 }
 
 void launch_scene_kmax(hipStream_t s, const Geo& G, SceneCtl c) {
-    hipLaunchKernelGGL(k_scene_kmax, dim3(cdiv(G.ns, 64)), dim3(64), 0, s, G, c);
+    hipLaunchKernelGGL(k_scene_kmax, dim3(G.ns), dim3(256), 0, s, G, c);
 }
 
 // add_du_kernel (abd.py:780-785): q -= alpha du

@@ -125,12 +125,22 @@ __global__ void k_relu_back(long total, const float* __restrict__ dA, const floa
 }
 
 // column sums of a BF16 matrix (bias gradients), fixed order
-__global__ void k_colsum(int rows, int dp, const uint16_t* __restrict__ G, float* __restrict__ out) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= dp) return;
+// column sums: 16 columns per workgroup, 16 row strides per column summed in a fixed order
+__global__ __launch_bounds__(256) void k_colsum(int rows, int dp, const uint16_t* __restrict__ G,
+                                                float* __restrict__ out) {
+    __shared__ float sh[16][17];
+    const int c = threadIdx.x % 16, r = threadIdx.x / 16;
+    const int j = blockIdx.x * 16 + c;
     float s = 0.f;
-    for (int i = 0; i < rows; ++i) s += bf2f(G[(size_t)i * dp + j]);
-    out[j] = s;
+    if (j < dp)
+        for (int i = r; i < rows; i += 16) s += bf2f(G[(size_t)i * dp + j]);
+    sh[r][c] = s;
+    __syncthreads();
+    if (r == 0 && j < dp) {
+        float t = 0.f;
+        for (int k = 0; k < 16; ++k) t += sh[k][c];
+        out[j] = t;
+    }
 }
 
 // fixed-order sum of n floats by one workgroup
@@ -303,7 +313,7 @@ void mlp_backward_dev(Mlp* m, int n, float* loss_dev, hipStream_t st) {
         // dW_l = A_l^T G_l  (dp[l] x dp[l+1]), K = rows
         gemm(st, m->dp[l], m->dp[l + 1], rows, m->A[l], 1, m->dp[l], m->G[l], m->dp[l + 1], 1, m->gW[l],
              m->dp[l + 1], nullptr);
-        hipLaunchKernelGGL(k_colsum, dim3(cdivm(m->dp[l + 1], 64)), dim3(64), 0, st, rows, m->dp[l + 1], m->G[l],
+        hipLaunchKernelGGL(k_colsum, dim3(cdivm(m->dp[l + 1], 16)), dim3(256), 0, st, rows, m->dp[l + 1], m->G[l],
                            m->gb[l]);
         if (l > 0) {
             // dA_l = G_l W_l^T (rows x dp[l]); G_{l-1} = dA_l * (Z_{l-1} > 0)

@@ -178,9 +178,9 @@ __global__ __launch_bounds__(SORT_T) void k_radix_scatter(int n, const uint64_t*
 
 // Whole sort of n <= SORT_SMALL_MAX keys by counting ranks: the final position of key i is
 // #{j : k_j < k_i} + #{j < i : k_j == k_i} (stable).  Every workgroup holds all n keys in LDS and
-// ranks 64 of them, four lanes per key each counting over a quarter of the keys (n / 4
-// comparisons per lane; the keys a wave reads at one step are a few LDS addresses, broadcast).
-#define SORT_SMALL_KEYS 64
+// ranks 16 of them, sixteen lanes per key, lane l counting over keys l, l + 16, ... (n / 16
+// comparisons per lane; at one step a wave reads 16 consecutive keys, the same for its 4 keys).
+#define SORT_SMALL_KEYS 16
 #define SORT_SMALL_MAX 4096
 __global__ __launch_bounds__(SORT_T) void k_sort_small(int n, const uint64_t* __restrict__ kin,
                                                         const int* __restrict__ vin, uint64_t* __restrict__ kout,
@@ -189,16 +189,15 @@ __global__ __launch_bounds__(SORT_T) void k_sort_small(int n, const uint64_t* __
     const int t = threadIdx.x;
     for (int i = t; i < n; i += SORT_T) kb[i] = kin[i];
     __syncthreads();
-    const int i = blockIdx.x * SORT_SMALL_KEYS + t / 4, part = t % 4;
-    const int chunk = (n + 3) / 4, j0 = part * chunk, j1 = min(n, j0 + chunk);
+    const int i = blockIdx.x * SORT_SMALL_KEYS + t / 16, part = t % 16;
     const uint64_t k = i < n ? kb[i] : 0;
     int rank = 0;
-    for (int j = j0; j < j1; ++j) {
+    for (int j = part; j < n; j += 16) {
         const uint64_t o = kb[j];
         rank += (o < k || (o == k && j < i)) ? 1 : 0;
     }
-    rank += __shfl_xor(rank, 1, 64);
-    rank += __shfl_xor(rank, 2, 64);
+#pragma unroll
+    for (int o = 1; o < 16; o <<= 1) rank += __shfl_xor(rank, o, 64);
     if (i < n && part == 0) {
         rank = opaque(rank);
         kout[rank] = k;
