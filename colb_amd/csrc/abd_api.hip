@@ -125,6 +125,7 @@ struct AbdCtx {
     bool coop_ok = true;          // cleared when a cooperative launch fails for a reason other than its size
     int pcg_fallbacks = 0;        // solves of the current step rerun with one launch per iteration
     int coop_blocks[2] = {-1, -1};  // co-resident k_pcg_persist workgroups (FP64, FP32); -1 = not queried
+    int multi_blocks[2] = {-1, -1}; // co-resident k_pcg_multi workgroups (FP64, FP32); -1 = not queried
     bool pcg_multi = false;       // solve with one launch per iteration (autotuner choice)
     // mixed precision (AbdParams.precision == 1): FP32 copies of the PCG operands and vectors
     float *f_vec = nullptr, *f_diag = nullptr, *f_pinv = nullptr, *f_off = nullptr, *f_offT = nullptr, *f_mq = nullptr;
@@ -911,28 +912,56 @@ bool persist_fits(AbdCtx* ctx) {
     return (ctx->ntiles + PCG_TILES_PER_BLOCK - 1) / PCG_TILES_PER_BLOCK <= cap;
 }
 
+// co-resident workgroups of k_pcg_multi<T> (the persistent form for more tiles than fit at once)
+template <typename T>
+int multi_cap(AbdCtx* ctx) {
+    int& cap = ctx->multi_blocks[sizeof(T) == 4 ? 1 : 0];
+    if (cap < 0 && pcg_multi_max_blocks<T>(&cap) != hipSuccess) {
+        (void)hipGetLastError();
+        cap = 0;
+    }
+    return cap;
+}
+
+// one persistent launch per solve is possible (k_pcg_persist or k_pcg_multi)
 bool persist_ok(AbdCtx* ctx) {
-    return ctx->coop_ok && (ctx->P.precision == 1 ? persist_fits<float>(ctx) : persist_fits<double>(ctx));
+    if (!ctx->coop_ok) return false;
+    return ctx->P.precision == 1 ? (persist_fits<float>(ctx) || multi_cap<float>(ctx) > 0)
+                                 : (persist_fits<double>(ctx) || multi_cap<double>(ctx) > 0);
 }
 
 template <typename T>
 int solve_persistent(AbdCtx* ctx, const PcgBufT<T>& B, bool* ok) {
     *ok = false;
-    if (!ctx->coop_ok || ctx->pcg_multi || (ctx->P.reserved0 & 8) || !persist_fits<T>(ctx)) return ABD_OK;
+    if (!ctx->coop_ok || ctx->pcg_multi || (ctx->P.reserved0 & 8)) return ABD_OK;
+    // one tile per wave when the tiles fit on the device at once (k_pcg_persist), else waves that
+    // take several tiles per stage (k_pcg_multi)
+    // debug bit 32: the multi-tile form even where the tiles fit (a test of that form at small sizes)
+    const bool fits = persist_fits<T>(ctx) && !(ctx->P.reserved0 & 32);
+    const int mcap = fits ? 0 : multi_cap<T>(ctx);
+    if (!fits && mcap <= 0) return ABD_OK;
     HIPCHK(hipMemsetAsync(ctx->cg_iters, 0, sizeof(int) * ctx->ns, ctx->st));
     HIPCHK(hipMemsetAsync(ctx->pcg_ctl, 0, 8 * sizeof(unsigned), ctx->st));
+    if (!fits) HIPCHK(hipMemsetAsync(ctx->cg_done, 0, sizeof(int) * ctx->ns, ctx->st));
     // debug bit 4: raise the failure flag before the launch, so every workgroup gives up at its first
     // barrier that does not fill at once and the solve is redone with one launch per iteration
     const bool forced = (ctx->P.reserved0 & 16) != 0;
     if (forced) hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, ctx->st, 1, (int*)(ctx->pcg_ctl + 4), 1);
-    if (sizeof(T) == 8) {  // the FP64 solve multiplies by M_e = B_e P_col (the FP32 one by P, then B)
+    if (fits && sizeof(T) == 8) {  // the FP64 solve multiplies by M_e = B_e P_col (the FP32 one by P, then B)
         launch_pcg_mblocks(ctx->st, 2 * ctx->n_pairs, ctx->ent_ks, ctx->ent_vs, ctx->off, ctx->offT, ctx->pinv,
                            ctx->nb, ctx->n_pairs, B.mq, ctx->cnt + C_ERR);
         CHK(dbg(ctx, "launch_pcg_mblocks"));
     }
     tic(ctx, "pcg_solve");
-    hipError_t e = launch_pcg_persist(ctx->st, ctx->ntiles, ctx->ns, ctx->tile_b0, ctx->tile_b1, ctx->tile_scene,
-                                      ctx->scene_tile0, B, ctx->P.cg_tol, ctx->P.cg_max_iter, ctx->nb, ctx->n_pairs);
+    hipError_t e;
+    if (fits) {
+        e = launch_pcg_persist(ctx->st, ctx->ntiles, ctx->ns, ctx->tile_b0, ctx->tile_b1, ctx->tile_scene,
+                               ctx->scene_tile0, B, ctx->P.cg_tol, ctx->P.cg_max_iter, ctx->nb, ctx->n_pairs);
+    } else {
+        const int blocks = std::min(mcap, (ctx->ntiles + PCG_TILES_PER_BLOCK - 1) / PCG_TILES_PER_BLOCK);
+        e = launch_pcg_multi(ctx->st, blocks, ctx->ntiles, ctx->ns, ctx->tile_b0, ctx->tile_b1, ctx->tile_scene,
+                             ctx->scene_tile0, B, ctx->P.cg_tol, ctx->P.cg_max_iter, ctx->nb, ctx->n_pairs);
+    }
     toc(ctx);
     if (e != hipSuccess) {
         (void)hipGetLastError();
@@ -1914,6 +1943,7 @@ int abd_solve(AbdCtx* ctx, double* du, double* residual, int* cg) {
     if (residual) HIPCHK(hipMemcpyAsync(residual, ctx->bnorm, ctx->ns * sizeof(double), hipMemcpyDeviceToHost, ctx->st));
     if (cg) HIPCHK(hipMemcpyAsync(cg, ctx->cg_iters, ctx->ns * sizeof(int), hipMemcpyDeviceToHost, ctx->st));
     HIPCHK(hipStreamSynchronize(ctx->st));
+    timer_collect(ctx);
     return ABD_OK;
 }
 

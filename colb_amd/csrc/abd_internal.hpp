@@ -219,9 +219,15 @@ hipError_t launch_pcg_persist(hipStream_t s, int ntiles, int ns, const int* tile
 template <typename T>
 hipError_t pcg_persist_max_blocks(int* out);
 template <typename T>
+hipError_t launch_pcg_multi(hipStream_t s, int blocks, int ntiles, int ns, const int* tile_b0, const int* tile_b1,
+                            const int* tile_scene, const int* scene_tile0, const PcgBufT<T>& B, double tol,
+                            int max_iter, int nb, int n_pairs);
+template <typename T>
+hipError_t pcg_multi_max_blocks(int* out);
+template <typename T>
 void launch_pcg_mblocks(hipStream_t s, int n_ent, const uint64_t* ent_key, const int* ent_val, const double* off,
                         const double* offT, const double* pinv, int nb, int n_pairs, T* mq, int* err);
-constexpr int PCG_TILES_PER_BLOCK = 4;  // waves (tiles) per workgroup of k_pcg_persist
+constexpr int PCG_TILES_PER_BLOCK = 4;  // waves per workgroup of k_pcg_persist (one tile each) and k_pcg_multi
 // mixed precision: FP64 -> FP32 copies of the PCG operands, FP32 -> FP64 of the solution
 void launch_to_f32(hipStream_t s, long n, const double* a, float* b);
 void launch_to_f64(hipStream_t s, long n, const float* a, double* b);

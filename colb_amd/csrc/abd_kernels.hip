@@ -658,12 +658,51 @@ __device__ __forceinline__ T pcg_r_next(T r, T w, T s, T alpha, T beta) {
 }
 
 template <typename T>
-__global__ __launch_bounds__(64) void k_pcg_cg(int k, int ntiles, int ns, const int* __restrict__ tb0,
-                                                const int* __restrict__ tb1, const int* __restrict__ tscene,
-                                                const int* __restrict__ st0, PcgBufT<T> B, double tol, int nb,
-                                                int n_pairs) {
-    __shared__ T vec[TILE_BODIES + 1][12];  // lanes 60-63 (lb = 5) only touch the last row
-    const int t = blockIdx.x, lane = threadIdx.x;
+__device__ __forceinline__ T aload(const T* p) {
+    return __hip_atomic_load(const_cast<T*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ void astore(T* p, T v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// LDS exchange among the lanes of one wave: order the wave's LDS accesses (no block barrier, the
+// waves of a workgroup run different tiles)
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// plain or agent-scope (coherent across XCDs within one launch) access to the PCG's shared state
+template <bool A, typename V>
+__device__ __forceinline__ V ldp(const V* p) {
+    if constexpr (A) return aload(p);
+    else return *p;
+}
+template <bool A, typename V>
+__device__ __forceinline__ void stp(V* p, V v) {
+    if constexpr (A) astore(p, v);
+    else *p = v;
+}
+template <bool A>
+__device__ __forceinline__ double scene_partial_p(const double* part, int ntiles, int qty, int par, int t0, int t1,
+                                                  int lane) {
+    const double* p = part + (size_t)(qty * 2 + par) * ntiles;
+    double acc = 0.0;
+    for (int t = t0 + lane; t < t1; t += 64) acc += ldp<A>(p + t);
+    return wave_sum(acc);
+}
+
+// Stage k of the PCG for tile t by one wave (lane 0-63; vec: the wave's LDS rows).  A = false:
+// one launch per stage (k_pcg_cg; the scene's ||b|| in B.bnorm, computed after stage 0); A = true:
+// the stages of one persistent launch (k_pcg_multi; shared state through agent-scope atomics,
+// ||b|| taken at stage 1 and kept in B.bnorm_out, every scene found done at stage k counted once in
+// done_cnt[(k - 1) % 3]).
+template <typename T, bool A>
+__device__ void pcg_stage(int k, int t, int lane, T (*vec)[12], const PcgBufT<T>& B, int ntiles, int ns,
+                          const int* __restrict__ tb0, const int* __restrict__ tb1, const int* __restrict__ tscene,
+                          const int* __restrict__ st0, double tol, int nb, int n_pairs) {
     const int s = tscene[t];
     const int t0 = st0[s], t1 = st0[s + 1];
     const bool init = (k == 0);
@@ -671,23 +710,43 @@ __global__ __launch_bounds__(64) void k_pcg_cg(int k, int ntiles, int ns, const 
     double alpha = 0.0, beta = 0.0;
     if (!init) {
         const int par = (k - 1) & 1;
-        if (pcg_done(B.part, ntiles, par, t0, t1, lane, B.bnorm[s], tol, B.cg_done + s)) {
-            if (t == t0 && lane == 0) B.cg_done[s] = 1;
+        const int i = k - 1;
+        double bn;
+        if constexpr (A) {
+            if (k == 1) {
+                bn = B.bref ? B.bref[s] : sqrt(scene_partial_p<A>(B.part, ntiles, 2, 0, t0, t1, lane));
+                if (t == t0 && lane == 0) {
+                    astore(B.bnorm_out + s, bn);
+                    astore(B.residual_out + s, bn);
+                }
+            } else {
+                bn = aload(B.bnorm_out + s);
+            }
+        } else {
+            bn = B.bnorm[s];
+        }
+        // sticky: later stages see the other parity's (older) partials of a done scene
+        const bool done = bn == 0.0 || ldp<A>(B.cg_done + s) != 0 ||
+                          sqrt(scene_partial_p<A>(B.part, ntiles, 2, par, t0, t1, lane)) <= tol * bn;
+        if (done) {
+            if (t == t0 && lane == 0) {
+                stp<A>(B.cg_done + s, 1);
+                if constexpr (A) atomicAdd(B.done_cnt + (i % 3), 1);
+            }
             return;
         }
-        const int i = k - 1;
-        const double g = scene_partial(B.part, ntiles, 1, par, t0, t1, lane);
-        const double d = scene_partial(B.part, ntiles, 0, par, t0, t1, lane);
+        const double g = scene_partial_p<A>(B.part, ntiles, 1, par, t0, t1, lane);
+        const double d = scene_partial_p<A>(B.part, ntiles, 0, par, t0, t1, lane);
         if (i == 0) {
             alpha = g / d;
         } else {
-            const double gp = B.sc[2 * s + ((i - 1) & 1)], ap = B.sc[2 * ns + 2 * s + ((i - 1) & 1)];
+            const double gp = ldp<A>(B.sc + 2 * s + ((i - 1) & 1)), ap = ldp<A>(B.sc + 2 * ns + 2 * s + ((i - 1) & 1));
             beta = g / gp;
             alpha = g / (d - beta * g / ap);
         }
         if (t == t0 && lane == 0) {
-            B.sc[2 * s + (i & 1)] = g;
-            B.sc[2 * ns + 2 * s + (i & 1)] = alpha;
+            stp<A>(B.sc + 2 * s + (i & 1), g);
+            stp<A>(B.sc + 2 * ns + 2 * s + (i & 1), alpha);
             B.cg_iters[s] = i + 1;
         }
     }
@@ -699,23 +758,24 @@ __global__ __launch_bounds__(64) void k_pcg_cg(int k, int ntiles, int ns, const 
     const T al = (T)alpha, be = (T)beta;
     T rn = 0;
     if (on) {
+        // x, p, u rows belong to this tile alone: plain accesses
         if (init) {
             rn = B.b[idx];
             B.x[idx] = 0;
             B.p[idx] = 0;
-            B.sv[wr][idx] = 0;
+            stp<A>(B.sv[wr] + idx, T(0));
         } else {
+            const T wo = ldp<A>(B.w[rd] + idx), so = ldp<A>(B.sv[rd] + idx), ro = ldp<A>(B.r[rd] + idx);
             const T pv = B.u[idx] + be * B.p[idx];
             B.p[idx] = pv;
-            const T sval = B.w[rd][idx] + be * B.sv[rd][idx];
-            B.sv[wr][idx] = sval;
+            stp<A>(B.sv[wr] + idx, wo + be * so);
             B.x[idx] += al * pv;
-            rn = pcg_r_next(B.r[rd][idx], B.w[rd][idx], B.sv[rd][idx], al, be);
+            rn = pcg_r_next(ro, wo, so, al, be);
         }
-        B.r[wr][idx] = rn;
+        stp<A>(B.r[wr] + idx, rn);
         vec[lb][row] = rn;
     }
-    __syncthreads();
+    wave_lds_sync();
     T un = 0;
     if (on) {
         const T* Pb = B.pinv + (size_t)bb * 144 + row * 12;
@@ -723,9 +783,9 @@ __global__ __launch_bounds__(64) void k_pcg_cg(int k, int ntiles, int ns, const 
         for (int c = 0; c < 12; ++c) un += Pb[c] * vec[lb][c];
         B.u[idx] = un;
     }
-    __syncthreads();
+    wave_lds_sync();
     if (on) vec[lb][row] = un;
-    __syncthreads();
+    wave_lds_sync();
     T wv = 0;
     if (on) {
         const T* D = B.diag + (size_t)bb * 144 + row * 12;
@@ -736,7 +796,7 @@ __global__ __launch_bounds__(64) void k_pcg_cg(int k, int ntiles, int ns, const 
     int mx = e1 - e0;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) mx = max(mx, __shfl_xor(mx, o, 64));
-    __syncthreads();
+    wave_lds_sync();
     for (int q = 0; q < mx; ++q) {
         const bool has = on && q < e1 - e0;
         const T* Bk = nullptr;
@@ -753,27 +813,37 @@ __global__ __launch_bounds__(64) void k_pcg_cg(int k, int ntiles, int ns, const 
 #pragma unroll
                 for (int c = 0; c < 12; ++c) {
                     const int j = 12 * col + c;
-                    const T rj = init ? B.b[j] : pcg_r_next(B.r[rd][j], B.w[rd][j], B.sv[rd][j], al, be);
+                    const T rj = init ? B.b[j]
+                                      : pcg_r_next(ldp<A>(B.r[rd] + j), ldp<A>(B.w[rd] + j), ldp<A>(B.sv[rd] + j), al, be);
                     uc += Pc[c] * rj;
                 }
             }
         }
         vec[lb][row] = uc;
-        __syncthreads();
+        wave_lds_sync();
         if (Bk) {
 #pragma unroll
             for (int c = 0; c < 12; ++c) wv += Bk[c] * vec[lb][c];
         }
-        __syncthreads();
+        wave_lds_sync();
     }
-    if (on) B.w[wr][idx] = wv;
+    if (on) stp<A>(B.w[wr] + idx, wv);
     const double g = wave_sum(on ? (double)rn * un : 0.0), d = wave_sum(on ? (double)wv * un : 0.0),
                  rr = wave_sum(on ? (double)rn * rn : 0.0);
     if (lane == 0) {
-        B.part[(size_t)(0 * 2 + wr) * ntiles + t] = d;
-        B.part[(size_t)(1 * 2 + wr) * ntiles + t] = g;
-        B.part[(size_t)(2 * 2 + wr) * ntiles + t] = rr;
+        stp<A>(B.part + (size_t)(0 * 2 + wr) * ntiles + t, d);
+        stp<A>(B.part + (size_t)(1 * 2 + wr) * ntiles + t, g);
+        stp<A>(B.part + (size_t)(2 * 2 + wr) * ntiles + t, rr);
     }
+}
+
+template <typename T>
+__global__ __launch_bounds__(64) void k_pcg_cg(int k, int ntiles, int ns, const int* __restrict__ tb0,
+                                                const int* __restrict__ tb1, const int* __restrict__ tscene,
+                                                const int* __restrict__ st0, PcgBufT<T> B, double tol, int nb,
+                                                int n_pairs) {
+    __shared__ T vec[TILE_BODIES + 1][12];  // lanes 60-63 (lb = 5) only touch the last row
+    pcg_stage<T, false>(k, blockIdx.x, threadIdx.x, vec, B, ntiles, ns, tb0, tb1, tscene, st0, tol, nb, n_pairs);
 }
 
 template <typename T>
@@ -802,14 +872,6 @@ template void launch_pcg_cg<float>(hipStream_t, int, int, int, const int*, const
 // ---------------------------------------------------------------------------------------------
 #define PCG_SPIN_LIMIT (1 << 22)
 
-template <typename T>
-__device__ __forceinline__ T aload(const T* p) {
-    return __hip_atomic_load(const_cast<T*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-template <typename T>
-__device__ __forceinline__ void astore(T* p, T v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
 
 // grid barrier #k (k = 1, 2, ...); false if it timed out or another workgroup failed
 __device__ bool grid_barrier(unsigned* bar, int* fail, unsigned k, unsigned nblocks) {
@@ -897,13 +959,6 @@ template void launch_pcg_mblocks<double>(hipStream_t, int, const uint64_t*, cons
 template void launch_pcg_mblocks<float>(hipStream_t, int, const uint64_t*, const int*, const double*, const double*,
                                         const double*, int, int, float*, int*);
 
-// LDS exchange among the lanes of one wave: order the wave's LDS accesses (no block barrier, the
-// waves of a workgroup run different tiles)
-__device__ __forceinline__ void wave_lds_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 template <typename T>
 __global__ __launch_bounds__(64 * PCG_WAVES) void k_pcg_persist(int ntiles, int ns, const int* __restrict__ tb0,
@@ -1170,6 +1225,69 @@ template hipError_t pcg_persist_max_blocks<float>(int*);
 
 template hipError_t launch_pcg_persist<double>(hipStream_t, int, int, const int*, const int*, const int*, const int*,
                                                const PcgBufT<double>&, double, int, int, int);
+
+// ---------------------------------------------------------------------------------------------
+// The PCG as one cooperative launch when the tiles outnumber the co-resident waves (the batched
+// scenes): a grid of co-resident workgroups whose waves take tiles gw, gw + W, ... (W waves in the
+// grid) at every stage, with every vector in memory (pcg_stage<T, true>: the arithmetic of the
+// per-stage launches), the stages separated by the grid barrier and ended by the done counters of
+// k_pcg_persist.
+// ---------------------------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(64 * PCG_WAVES) void k_pcg_multi(int ntiles, int ns, const int* __restrict__ tb0,
+                                                               const int* __restrict__ tb1,
+                                                               const int* __restrict__ tscene,
+                                                               const int* __restrict__ st0, PcgBufT<T> B, double tol,
+                                                               int max_iter, int nb, int n_pairs) {
+    __shared__ T vec_all[PCG_WAVES][TILE_BODIES + 1][12];
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    const int gw = blockIdx.x * PCG_WAVES + wave, nw = gridDim.x * PCG_WAVES;
+    unsigned nbar = 0;
+    for (int k = 0; k <= max_iter; ++k) {
+        // slot k % 3 is next counted at stage k + 1; it was last read before the previous barrier
+        if (k > 0 && blockIdx.x == 0 && threadIdx.x == 0) astore(B.done_cnt + (k % 3), 0);
+        for (int t = gw; t < ntiles; t += nw)
+            pcg_stage<T, true>(k, t, lane, vec_all[wave], B, ntiles, ns, tb0, tb1, tscene, st0, tol, nb, n_pairs);
+        __builtin_amdgcn_s_waitcnt(0);
+        if (!grid_barrier(B.bar, B.fail, ++nbar, gridDim.x)) break;
+        if (k > 0 && aload(B.done_cnt + ((k - 1) % 3)) >= ns) break;
+    }
+}
+
+template <typename T>
+hipError_t launch_pcg_multi(hipStream_t s, int blocks, int ntiles, int ns, const int* tile_b0, const int* tile_b1,
+                            const int* tile_scene, const int* scene_tile0, const PcgBufT<T>& B, double tol,
+                            int max_iter, int nb, int n_pairs) {
+    void* args[] = {(void*)&ntiles, (void*)&ns, (void*)&tile_b0, (void*)&tile_b1, (void*)&tile_scene,
+                    (void*)&scene_tile0, (void*)&B, (void*)&tol, (void*)&max_iter, (void*)&nb, (void*)&n_pairs};
+#if defined(ABD_EMULATION)
+    (void)args;
+    if (blocks != 1) return hipErrorNotSupported;
+    hipLaunchKernelGGL(k_pcg_multi<T>, dim3(1), dim3(64 * PCG_WAVES), 0, s, ntiles, ns, tile_b0, tile_b1, tile_scene,
+                       scene_tile0, B, tol, max_iter, nb, n_pairs);
+    return hipSuccess;
+#else
+    return hipLaunchCooperativeKernel((const void*)k_pcg_multi<T>, dim3(blocks), dim3(64 * PCG_WAVES), args, 0, s);
+#endif
+}
+
+// co-resident workgroups of k_pcg_multi<T>
+template <typename T>
+hipError_t pcg_multi_max_blocks(int* out) {
+    int dev = 0, cus = 0, per_cu = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e == hipSuccess)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_pcg_multi<T>, 64 * PCG_WAVES, 0);
+    *out = (e == hipSuccess) ? per_cu * cus : 0;
+    return e;
+}
+template hipError_t pcg_multi_max_blocks<double>(int*);
+template hipError_t pcg_multi_max_blocks<float>(int*);
+template hipError_t launch_pcg_multi<double>(hipStream_t, int, int, int, const int*, const int*, const int*, const int*,
+                                             const PcgBufT<double>&, double, int, int, int);
+template hipError_t launch_pcg_multi<float>(hipStream_t, int, int, int, const int*, const int*, const int*, const int*,
+                                            const PcgBufT<float>&, double, int, int, int);
 template hipError_t launch_pcg_persist<float>(hipStream_t, int, int, const int*, const int*, const int*, const int*,
                                               const PcgBufT<float>&, double, int, int, int);
 

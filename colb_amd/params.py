@@ -45,7 +45,9 @@ class Params:
     cand_margin: float = None  # candidate-cache margin (length); None: 4 dhat; <= 0: no cache
     pcg_x0: int = 0  # PCG initial guess: 0 = zero (the reference), 1 = the untaken part of the previous direction
     debug: int = 0  # bit 0: synchronise and check after every launch; bit 1: also log them;
-    #                 bit 2: print broad-phase list sizes
+    #                 bit 2: print broad-phase list sizes; bit 3: one PCG launch per iteration;
+    #                 bit 4: raise the persistent PCG's failure flag before its launch (fallback test);
+    #                 bit 5: the multi-tile persistent PCG even where one tile per wave fits
 
     def to_c(self) -> AbdParams:
         p = AbdParams()

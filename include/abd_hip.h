@@ -61,7 +61,10 @@ typedef struct AbdParams {
                                   (gauss_newton.py:110, 1e-2) */
     double pinned_diag;        /* Hessian diagonal of pinned bodies (abd.py:146, 100) */
     int    max_candidates;     /* capacity of the candidate-pair list (grown on demand) */
-    int    reserved0;
+    int    reserved0;          /* debug bits (1: check every launch, 2: log launches, 4: print list
+                                  sizes, 8: one PCG launch per iteration, 16: force the persistent
+                                  PCG's fallback, 32: multi-tile persistent PCG where one tile per
+                                  wave fits) */
     int    solver;             /* 0 = block-Jacobi PCG (abd.py:633-636, "cg"); 1 = direct: dense
                                   Cholesky per scene (the sparse direct branch, abd.py:637-654);
                                   2 = PCG, then the direct solve for every scene whose PCG ended

@@ -35,6 +35,18 @@ def main():
     qg, qo = g.get_state()[0], o.get_state()[0]
     print("result", np.abs(qg - qo).max() / np.abs(qo).max(), sg["n_active"], so["n_active"], flush=True)
     g.close()
+    # the multi-tile persistent PCG (debug bit 32; one body per tile: 8 tiles over the 4 waves of
+    # the emulated grid) against the oracle, on the cubes on the ground
+    from colb_amd import scene as S
+    sg8 = S.cubes_on_ground(lift=0.02)
+    pg = north_star(ground_enabled=True, dhat=0.05, k_contact=1e14, newton_tol=1e-9, cg_tol=1e-10, cg_max_iter=500,
+                    max_newton=3)
+    gm, om = NewtonAbd(pg.dt, sg8, params=pg.replace(debug=32)), O.Oracle(sg8, pg)
+    gm.set_pcg_layout(1, False)
+    gm.step(), om.step()
+    qm, qo = gm.get_state()[0], om.get_state()[0]
+    print("multi", np.abs(qm - qo).max() / np.abs(qo).max(), flush=True)
+    gm.close()
     learned(sc, p)
 
 

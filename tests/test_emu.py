@@ -2,7 +2,8 @@
 AddressSanitizer): the radix sort and scan unit driver, and two Newton steps of a two-cube scene
 through the C ABI compared with the oracle (indexing errors surface here as sanitizer reports,
 before any GPU run), then two steps with the learned step predictor (body features against the
-oracle's, the step against the oracle given the same prediction)."""
+oracle's, the step against the oracle given the same prediction), and two steps with the
+multi-tile persistent PCG."""
 import os
 import subprocess
 import sys
@@ -38,6 +39,8 @@ def test_emulated_step_matches_oracle(emu_lib):
     _, err, na_emu, na_orc = res[-1].split()
     assert float(err) < 1e-6
     assert int(na_emu) == int(na_orc)
+    mul = [line for line in out.stdout.splitlines() if line.startswith("multi")]
+    assert mul and float(mul[-1].split()[1]) < 1e-6, out.stdout[-2000:]
     lrn = [line for line in out.stdout.splitlines() if line.startswith("learned")]
     assert lrn, out.stdout[-2000:]
     _, ferr, serr, pa_emu, pa_orc = lrn[-1].split()

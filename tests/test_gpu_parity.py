@@ -242,8 +242,9 @@ def test_batched_and_box_scenes_match_oracle(O, case):
 
 
 def test_pcg_launch_forms_and_autotune_agree():
-    """One cooperative launch per solve, one launch per iteration, and every autotuned tile size
-    give the same solution (to the PCG tolerance)."""
+    """One cooperative launch per solve, one launch per iteration, the multi-tile cooperative launch
+    (its per-stage arithmetic is the per-iteration launches': the same bits), and every autotuned
+    tile size give the same solution (to the PCG tolerance)."""
     sc, p = CASES["cubes_ground"]()
     q, qd = perturbed(sc, 5, amp=1e-4)
     g = solver(sc, p)
@@ -254,6 +255,11 @@ def test_pcg_launch_forms_and_autotune_agree():
     g2.set_state(q, sc.q0, qd)
     g2.compute_du()
     assert rel(g2._du, du_a) < 1e-8
+    g3 = solver(sc, p.replace(debug=32))
+    g3.set_state(q, sc.q0, qd)
+    g3.compute_du()
+    np.testing.assert_array_equal(g3._du, g2._du)
+    np.testing.assert_array_equal(g3.last_cg_iters, g2.last_cg_iters)
     bpt, multi, ms = g.autotune(repeats=2)
     say(f"autotune: {bpt} bodies per tile, per-iteration launch {multi}, ms {ms.tolist()}")
     assert 1 <= bpt <= 5 and multi in (0, 1) and ms[bpt - 1, multi] > 0
