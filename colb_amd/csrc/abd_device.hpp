@@ -348,13 +348,13 @@ __device__ inline bool wave_is_psd(const double* A, double* L, int m, int lane) 
 // polynomials in A, i.e. symmetric: each lane forms upper-triangle entries (lane, lane + 64) and
 // mirrors them.  V and the two static buffers are the iteration's LDS scratch.  absolute: A <- |A|
 // (eigenvalues mirrored rather than clamped).
-template <int M>
-__device__ inline void sym_product(const double* X, const double* Y, double* Z, const int* ui, const int* uj) {
+template <int M, typename HT>
+__device__ inline void sym_product(const HT* X, const HT* Y, HT* Z, const int* ui, const int* uj) {
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
         const int i = ui[h], j = uj[h];
         if (i < 0) continue;
-        double z = 0.0;
+        HT z = 0;
 #pragma unroll
         for (int k = 0; k < M; ++k) z = fma(X[i * M + k], Y[j * M + k], z);  // Y symmetric: Y[k][j] = Y[j][k]
         Z[i * M + j] = z;
@@ -363,13 +363,17 @@ __device__ inline void sym_product(const double* X, const double* Y, double* Z, 
     __syncthreads();
 }
 
-template <int M>
-__device__ inline void wave_psd_project_m(double* A, double* V, int lane, bool absolute, double* T1, double* T2) {
+// HT = float (the mixed-precision assembly): 11 quintic and 5 cubic steps, since eigenvalues below
+// ~1e-6 ||A|| are at the level of the FP32 rounding (more quintic steps would amplify the rounding
+// noise of the null space past the iteration's basin); the result is accurate to ~1e-6 of ||A||.
+template <int M, typename HT>
+__device__ inline void wave_psd_project_m(HT* A, HT* V, int lane, bool absolute, HT* T1, HT* T2) {
     constexpr int m = M;
     constexpr int mm = m * m;
     constexpr int nup = m * (m + 1) / 2;  // <= 78
+    constexpr int n_quintic = sizeof(HT) == 8 ? 24 : 11, n_cubic = sizeof(HT) == 8 ? 6 : 5;
     double f = 0;
-    for (int e = lane; e < mm; e += 64) f += A[e] * A[e];
+    for (int e = lane; e < mm; e += 64) f += (double)A[e] * (double)A[e];
     const double fro = wave_sum(f);
     if (fro == 0.0 || !(fro < INFINITY)) return;
     int ui[2] = {-1, -1}, uj[2] = {0, 0};
@@ -384,30 +388,31 @@ __device__ inline void wave_psd_project_m(double* A, double* V, int lane, bool a
         }
     }
     const double inv = rsq_nr(fro);
-    double* X = V;
-    for (int e = lane; e < mm; e += 64) X[e] = A[e] * inv;
+    HT* X = V;
+    for (int e = lane; e < mm; e += 64) X[e] = (HT)((double)A[e] * inv);
     __syncthreads();
-    for (int it = 0; it < 30; ++it) {
-        const bool quintic = it < 24;
+    for (int it = 0; it < n_quintic + n_cubic; ++it) {
+        const bool quintic = it < n_quintic;
         sym_product<M>(X, X, T1, ui, uj);  // X^2
         if (quintic) {
             sym_product<M>(T1, T1, T2, ui, uj);  // X^4
             for (int e = lane; e < mm; e += 64)
-                T2[e] = 2.0315 * T2[e] - 4.7750 * T1[e] + ((e / m == e % m) ? 3.4445 : 0.0);
+                T2[e] = HT(2.0315) * T2[e] - HT(4.7750) * T1[e] + ((e / m == e % m) ? HT(3.4445) : HT(0));
         } else {
-            for (int e = lane; e < mm; e += 64) T2[e] = ((e / m == e % m) ? 1.5 : 0.0) - 0.5 * T1[e];
+            for (int e = lane; e < mm; e += 64) T2[e] = ((e / m == e % m) ? HT(1.5) : HT(0)) - HT(0.5) * T1[e];
         }
         __syncthreads();
         sym_product<M>(X, T2, T1, ui, uj);  // X p(X^2)
-        double* t = X; X = T1; T1 = t;
+        HT* t = X; X = T1; T1 = t;
     }
     sym_product<M>(A, X, T1, ui, uj);  // |A| = A S
-    for (int e = lane; e < mm; e += 64) A[e] = absolute ? T1[e] : 0.5 * (A[e] + T1[e]);
+    for (int e = lane; e < mm; e += 64) A[e] = absolute ? T1[e] : HT(0.5) * (A[e] + T1[e]);
     __syncthreads();
 }
 
-__device__ inline void wave_psd_project(double* A, double* V, int m, int lane, bool absolute = false) {
-    __shared__ double T1[144], T2[144];
+template <typename HT>
+__device__ inline void wave_psd_project(HT* A, HT* V, int m, int lane, bool absolute = false) {
+    __shared__ HT T1[144], T2[144];
     switch (m) {
         case 3: wave_psd_project_m<3>(A, V, lane, absolute, T1, T2); break;
         case 6: wave_psd_project_m<6>(A, V, lane, absolute, T1, T2); break;

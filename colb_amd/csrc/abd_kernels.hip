@@ -217,12 +217,16 @@ void launch_compact(hipStream_t s, int n, const int* flag, const int* pos, const
 // contact_hessian_ee (abd.py:198-571), one wave per active contact: d^2 gradient/Hessian,
 // barrier chain rule, PSD projection of the whole barrier Hessian, then the affine Jacobian
 // J = [1, xr^T] (x) I3 (abd.py:256-263) maps it to the three 12x12 body blocks
-// (A,A), (B,B), (A,B) and two 12-vectors.
+// (A,A), (B,B), (A,B) and two 12-vectors.  HT: the type of the barrier Hessian's projection
+// (double; float in the mixed-precision assembly, whose Hessian feeds only the FP32 PCG); the
+// distances, the gradient and the block sums stay FP64.
+template <typename HT>
 __global__ __launch_bounds__(64) void k_contact_hess(Geo G, int n, const uint64_t* __restrict__ act,
                                                       const double4* __restrict__ X, AbdParams P,
                                                       int2* __restrict__ body, double* __restrict__ g24,
                                                       double* __restrict__ H432) {
-    __shared__ double A[144], V[144], gx[12], jv[4][4];
+    __shared__ HT A[144], V[144];
+    __shared__ double gx[12], jv[4][4];
     __shared__ int side[4];
     const int c = blockIdx.x;
     const int lane = threadIdx.x;
@@ -253,7 +257,7 @@ __global__ __launch_bounds__(64) void k_contact_hess(Geo G, int n, const uint64_
         const double lam0 = kd * (2.0 * B1 + 4.0 * B2 * d * d);
         const double lam = P.hess_proj == 1 ? fabs(lam0) : fmax(0.0, lam0);
         m = 3; ns = 1;
-        if (lane < 9) A[lane] = lam * cmp(nrm, lane / 3) * cmp(nrm, lane % 3);
+        if (lane < 9) A[lane] = (HT)(lam * cmp(nrm, lane / 3) * cmp(nrm, lane % 3));
         if (lane < 3) gx[lane] = kd * B1 * 2.0 * d * cmp(nrm, lane);
         __syncthreads();
     } else {
@@ -269,7 +273,7 @@ __global__ __launch_bounds__(64) void k_contact_hess(Geo G, int n, const uint64_
         barrier3(E.d2, shat, B, B1, B2);
         for (int e = lane; e < m * m; e += 64) {
             int R = e / m, C = e % m;
-            A[e] = kd * (B1 * d2_hess(E, R, C) + B2 * d2_grad(E, R) * d2_grad(E, C));
+            A[e] = (HT)(kd * (B1 * d2_hess(E, R, C) + B2 * d2_grad(E, R) * d2_grad(E, C)));
         }
         if (lane < m) gx[lane] = kd * B1 * d2_grad(E, lane);
         __syncthreads();
@@ -297,7 +301,7 @@ __global__ __launch_bounds__(64) void k_contact_hess(Geo G, int n, const uint64_
             if (side[k] != sa) continue;
             for (int l = 0; l < ns; ++l) {
                 if (side[l] != sb) continue;
-                h += jv[k][i] * jv[l][j] * A[(3 * k + r) * m + 3 * l + cc];
+                h += jv[k][i] * jv[l][j] * (double)A[(3 * k + r) * m + 3 * l + cc];
             }
         }
         Hout[e] = h;
@@ -315,7 +319,10 @@ __global__ __launch_bounds__(64) void k_contact_hess(Geo G, int n, const uint64_
 void launch_contact_hess(hipStream_t s, const Geo& G, int n, const uint64_t* act, const double4* X, const AbdParams& P,
                          int2* body, double* g24, double* H432) {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_contact_hess, dim3(n), dim3(64), 0, s, G, n, act, X, P, body, g24, H432);
+    if (P.precision == 1)
+        hipLaunchKernelGGL(k_contact_hess<float>, dim3(n), dim3(64), 0, s, G, n, act, X, P, body, g24, H432);
+    else
+        hipLaunchKernelGGL(k_contact_hess<double>, dim3(n), dim3(64), 0, s, G, n, act, X, P, body, g24, H432);
 }
 
 // incidence keys (body << 32 | contact << 1 | role) and body-pair keys (min(20) | max(20) | contact(24))
