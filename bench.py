@@ -196,6 +196,8 @@ def run_method(name, sc, p, args, learned_model=None, timing=False):
     # the timed run starts from the workload's initial state (buffers already sized by the warmup)
     solver.set_state(sc.q0, sc.q0, sc.qdot0)
     solver.set_stiffness(np.full(sc.n_bodies, pm.k_contact))
+    if timing:
+        solver.enable_timing(2)  # HIP events around the PCG launches of the timed steps only
     recs = []
     mid = None
     paused = 0.0
@@ -204,7 +206,7 @@ def run_method(name, sc, p, args, learned_model=None, timing=False):
         sts = solver.step()
         recs.append((max(x["newton_iters"] for x in sts), all(x["converged"] for x in sts),
                      max(x["residual_last"] for x in sts), max(x["cg_iters"] for x in sts),
-                     sts[0]["traversals"], sts[0]["detections"]))
+                     sts[0]["traversals"], sts[0]["detections"], sum(x["pcg_fallbacks"] for x in sts)))
         if k == args.steps // 2 - 1 and timing:
             tp = time.perf_counter()
             mid = solver.get_state()  # the CPU baseline's start state, copied outside the timed work
@@ -217,25 +219,23 @@ def run_method(name, sc, p, args, learned_model=None, timing=False):
            "ms_per_step": round(wall_ms / args.steps, 3), "newton_iters_per_step": round(float(newton.mean()), 3),
            "converged_fraction": round(float(conv.mean()), 4),
            "worst_final_residual": float(max(r[2] for r in recs)),
-           "cg_iters_per_step": round(float(np.mean([r[3] for r in recs])), 1),
+           "cg_iters_per_step": round(float(np.mean([r[3] for r in recs])), 1), "steps": args.steps,
            "traversals_per_step": round(float(np.mean([r[4] for r in recs])), 3),
-           "detections_per_step": round(float(np.mean([r[5] for r in recs])), 3), "autotune": tune}
+           "detections_per_step": round(float(np.mean([r[5] for r in recs])), 3),
+           "pcg_fallbacks": int(sum(r[6] for r in recs)), "autotune": tune}
+    if timing:
+        out["pcg_times"] = solver.kernel_times()  # HIP events of the timed steps' PCG launches
+        solver.enable_timing(0)
     print(f"# {name}: {out}", file=sys.stderr, flush=True)
     return solver, out, mid
 
 
-def kernel_roofline(solver, cfg, precision, steps):
-    """Per-kernel HIP-event times over `steps` extra steps (outside the timed region); the dominant
-    single kernel's achieved algorithmic GB/s against HBM peak."""
-    solver.enable_timing(True)
-    prof_cg = 0
-    for _ in range(steps):
-        prof_cg += sum(st_["cg_iters"] for st_ in solver.step())
-    kt = solver.kernel_times()
-    solver.enable_timing(False)
-    single = ["pcg_solve", "pcg_iter", "broad_phase_query", "broad_phase_filter", "narrow_phase", "contact_hess",
-              "offblock_sum", "body_assemble", "ccd", "energy"]
-    dom = max((k for k in kt if k in single), key=lambda k: kt[k]["ms"], default=None)
+def kernel_roofline(solver, out, cfg, precision, steps):
+    """Roofline of the dominant kernel, the PCG solve: its HIP-event times over the timed steps
+    (out["pcg_times"]) give the average launch duration and the algorithmic bytes per launch.  Then
+    a per-stage breakdown over `steps` extra steps after the timed region (printed only)."""
+    kt = out.pop("pcg_times", {}) or {}
+    dom = next((k for k in ("pcg_solve", "pcg_iter") if k in kt and kt[k]["calls"] > 0), None)
     roof = None
     if dom:
         d = kt[dom]
@@ -245,12 +245,21 @@ def kernel_roofline(solver, cfg, precision, steps):
         traffic, src = measured_traffic(cfg["workload"], precision, dom)
         roof = {"bound": "hbm", "kernel": dom, "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(ach / HBM_PEAK_GBS, 5), "traffic": traffic, "avg_us": round(avg_ms * 1e3, 3),
-                "bytes_per_launch": bytes_per, "traffic_source": src}
-    for k, v in kt.items():
-        print(f"# kernel {k:20s} {v['ms']:9.3f} ms {v['calls']:6d} calls "
-              f"{v['bytes'] / max(v['ms'], 1e-9) / 1e6:8.1f} GB/s", file=sys.stderr, flush=True)
+                "launches": d["calls"], "bytes_per_launch": bytes_per, "traffic_source": src,
+                "timed_over": "the timed steps (HIP events on the solver's stream)"}
     pcg_ms = sum(kt[k]["ms"] for k in ("pcg_solve", "pcg_iter") if k in kt)
-    return roof, kt, (round(pcg_ms * 1e3 / prof_cg, 3) if prof_cg else None)
+    timed_cg = out["cg_iters_per_step"] * out["steps"]
+    pcg_us = round(pcg_ms * 1e3 / timed_cg, 3) if timed_cg else None
+    if steps > 0:
+        solver.enable_timing(True)
+        for _ in range(steps):
+            solver.step()
+        stages = solver.kernel_times()
+        solver.enable_timing(False)
+        for k, v in stages.items():
+            print(f"# stage {k:20s} {v['ms']:9.3f} ms {v['calls']:6d} calls "
+                  f"{v['bytes'] / max(v['ms'], 1e-9) / 1e6:8.1f} GB/s", file=sys.stderr, flush=True)
+    return roof, pcg_us
 
 
 def main():
@@ -308,7 +317,7 @@ def main():
         results[m] = out
         if m == methods[0]:
             mid_state = mid
-            roof, kt, pcg_us = kernel_roofline(solver, cfg, args.precision, args.profile_steps)
+            roof, pcg_us = kernel_roofline(solver, out, cfg, args.precision, args.profile_steps)
             out["pcg_us_per_iteration"] = pcg_us
         solver.close()
         if model is not None:

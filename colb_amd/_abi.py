@@ -86,6 +86,7 @@ class AbdStepStats(C.Structure):
         ("predictor_alpha", C.c_double),
         ("traversals", C.c_int),
         ("detections", C.c_int),
+        ("slow_iterations", C.c_int),
     ]
 
     def as_dict(self):

@@ -365,6 +365,8 @@ class NewtonAbd:
         self._precond = lp  # keeps the model alive while the solver uses it
 
     def enable_timing(self, on=True):
+        """HIP-event timing of the step's stages: False/0 off, True/1 every stage, 2 the PCG solve
+        launches only (no host synchronisation; light enough for a timed run)."""
         _lib.check(_lib.load().abd_enable_timing(self._h, int(on)), self._h, "abd_enable_timing")
 
     def kernel_times(self, cap=64):

@@ -33,11 +33,21 @@ namespace {
 
 struct Timer {
     bool on = false;
+    bool pcg_only = false;     // time the PCG solve launches only (light enough for a timed run)
     struct Rec {
         std::string name;
         hipEvent_t a, b;
         double bytes;
+        // kslot >= 0: bytes = K x (fixed + per_pair x P), with K (iterations of a persistent solve,
+        // else 1) and P (body pairs) left in device slot kslot by the launch's own stream
+        int kslot = -1;
+        double fixed = 0.0, per_pair = 0.0;
     };
+    int* kslots = nullptr;     // device: [K, P] per timed launch whose bytes depend on device counts
+    int* h_kslots = nullptr;   // pinned host copy
+    int nk = 0;
+    int last_idx = -1;         // record opened by the last tic (-1: that tic was not timed)
+    static constexpr int KSLOTS = 1 << 14;
     std::vector<Rec> open;
     std::vector<hipEvent_t> pool;
     std::map<std::string, double> ms, bytes;
@@ -99,19 +109,22 @@ struct AbdCtx {
     double4 *Xc0 = nullptr, *Xc1 = nullptr;
     bool cache_valid = false;
     int step_traversals = 0, step_detections = 0;
-    // candidates
-    int cand_cap = 0, n_cand = 0;
+    int step_slow = 0;  // Newton iterations of the step that took a second host round trip
+    // candidates: nc_bound bounds the current list's length (its kernels are sized by it), the
+    // length itself is the device counter C_NCAND
+    int cand_cap = 0, nc_bound = 0;
     uint64_t *cand = nullptr, *cand_s = nullptr, *act = nullptr;
     int *flag = nullptr, *pos = nullptr, *cand_lo = nullptr, *cand_hi = nullptr;
     double* e_cand = nullptr;
     bool cand_valid = false;
     // active contacts
-    int act_cap = 0, n_act = 0;
+    int act_cap = 0;
     int2* act_body = nullptr;
     double *act_g = nullptr, *act_H = nullptr;
     uint64_t *inc = nullptr, *inc_s = nullptr, *pk = nullptr, *pk_s = nullptr;
     int *head = nullptr, *hpos = nullptr, *pair_b = nullptr, *run_start = nullptr;
-    int n_inc = 0, n_pk = 0, n_pairs = 0;
+    // exact lengths of the last assembly, read back only for the calls that hand them out (read_counts)
+    int n_cand = 0, n_act = 0, n_inc = 0, n_pk = 0, n_pairs = 0;
     // BSR
     int pair_cap = 0;
     double *off = nullptr, *offT = nullptr, *mq = nullptr;  // mq: 144 per CSR entry (persistent PCG)
@@ -123,6 +136,9 @@ struct AbdCtx {
     unsigned* pcg_ctl = nullptr;  // persistent solve: [0] barrier count, [1..3] done counters, [4] failure flag
     unsigned* h_pcg_ctl = nullptr;
     bool coop_ok = true;          // cleared when a cooperative launch fails for a reason other than its size
+    bool defer_pcg_check = false; // the persistent PCG's failure flag is read with the iteration's report
+    bool pcg_unchecked = false;   // the last solve was persistent and its failure flag is not read yet
+    bool force_iter_pcg = false;  // solve with one launch per iteration (redo after a failed persistent solve)
     int pcg_fallbacks = 0;        // solves of the current step rerun with one launch per iteration
     int coop_blocks[2] = {-1, -1};  // co-resident k_pcg_persist workgroups (FP64, FP32); -1 = not queried
     int multi_blocks[2] = {-1, -1}; // co-resident k_pcg_multi workgroups (FP64, FP32); -1 = not queried
@@ -179,8 +195,11 @@ struct AbdCtx {
 
 namespace {
 
+// device counters: list lengths the kernels of one Newton iteration hand to each other (C_NCAND
+// candidates, C_NACT active contacts, C_NINC incidences, C_NPK contact body pairs, C_NUNIQ
+// distinct body pairs, C_NENT CSR entries), flags and the error word
 enum { C_COUNT = 0, C_OVERFLOW = 1, C_NINC = 2, C_NPK = 3, C_NUNIQ = 4, C_ANY = 5, C_PENDING = 6, C_NACT = 7, C_ERR = 8,
-       C_OUTSIDE = 9, C_N = 10 };
+       C_OUTSIDE = 9, C_NCAND = 10, C_NENT = 11, C_SKIP = 12, C_N = 13 };
 enum { R_RES = 0, R_ALPHA, R_ALPHA0, R_FLOOR, R_ACTIVE, R_CG, R_E1, R_NACT, R_NCAND, R_KC, R_RAISES, R_PALPHA, R_N };
 
 #define HIPCHK(expr)                                                                               \
@@ -217,21 +236,41 @@ int dfree(AbdCtx* ctx, T*& p) {
     return ABD_OK;
 }
 
+bool timed(const AbdCtx* ctx, const char* name) {
+    return ctx->T.on && (!ctx->T.pcg_only || std::strncmp(name, "pcg_", 4) == 0);
+}
+
 void tic(AbdCtx* ctx, const char* name, double bytes = 0.0) {
-    if (!ctx->T.on) return;
+    ctx->T.last_idx = -1;
+    if (!timed(ctx, name)) return;
     Timer::Rec r{name, ctx->T.get(), ctx->T.get(), bytes};
     (void)hipEventRecord(r.a, ctx->st);
     ctx->T.open.push_back(r);
+    ctx->T.last_idx = (int)ctx->T.open.size() - 1;
 }
 
+// tic/toc pairs do not nest: toc closes the record its tic opened, if that tic was timed
 void toc(AbdCtx* ctx) {
-    if (!ctx->T.on || ctx->T.open.empty()) return;
-    (void)hipEventRecord(ctx->T.open.back().b, ctx->st);
+    if (ctx->T.last_idx < 0) return;
+    (void)hipEventRecord(ctx->T.open[ctx->T.last_idx].b, ctx->st);
 }
+
+// the bytes of the record the last tic opened: K x (fixed + per_pair x body pairs), K the most
+// iterations of any scene (kmax, a persistent solve) or 1, both read on the device
+int time_bytes(AbdCtx* ctx, const int* kmax, double fixed, double per_pair);
 
 void timer_collect(AbdCtx* ctx) {
     if (!ctx->T.on) return;
     (void)hipStreamSynchronize(ctx->st);
+    if (ctx->T.nk > 0) {
+        (void)hipMemcpy(ctx->T.h_kslots, ctx->T.kslots, sizeof(int) * 2 * ctx->T.nk, hipMemcpyDeviceToHost);
+        for (auto& r : ctx->T.open)
+            if (r.kslot >= 0)
+                r.bytes = (double)ctx->T.h_kslots[2 * r.kslot] *
+                          (r.fixed + r.per_pair * (double)ctx->T.h_kslots[2 * r.kslot + 1]);
+        ctx->T.nk = 0;
+    }
+    ctx->T.last_idx = -1;
     for (auto& r : ctx->T.open) {
         float ms = 0.f;
         (void)hipEventElapsedTime(&ms, r.a, r.b);
@@ -257,14 +296,12 @@ __global__ void k_fill_u64(int n, unsigned long long* p, unsigned long long v) {
     if (i < n) p[i] = v;
 }
 // active contacts per scene from the flags and their exclusive scan; thread 0 also stores the
-// scan's last entry and last flag (their sum is the active count) for the host's counter read
-__global__ void k_nact_scene(int ns, const int* lo, const int* hi, const int* flag, const int* pos, int* out, int nc,
-                             int* cnt_pos, int* cnt_flag) {
+// total (the scan's last entry plus the last flag)
+__global__ void k_nact_scene(int ns, const int* lo, const int* hi, const int* flag, const int* pos, int* out, Cnt cn,
+                             int* n_act) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s == 0 && nc > 0) {
-        *cnt_pos = pos[nc - 1];
-        *cnt_flag = flag[nc - 1];
-    }
+    const int nc = cnt_of(cn);
+    if (s == 0) *n_act = nc > 0 ? pos[nc - 1] + flag[nc - 1] : 0;
     if (s >= ns) return;
     int a = lo[s], b = hi[s];
     out[s] = (b > a) ? (pos[b - 1] + flag[b - 1] - pos[a]) : 0;
@@ -286,6 +323,31 @@ __global__ void k_report(int ns, SceneCtl c, const int* cg, const int* nact, con
     R[R_KC] = c.kc[s];
     R[R_RAISES] = c.k_raises[s];
     R[R_PALPHA] = c.pred_alpha[s];
+}
+// timing slot: out[0] = max(a[0..n)) (1 without a), out[1] = *np; one wave
+__global__ void k_timing_slot(int n, const int* a, const int* np, int* out) {
+    int m = a ? 0 : 1;
+    if (a)
+        for (int i = threadIdx.x; i < n; i += 64) m = max(m, a[i]);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o, 64));
+    if (threadIdx.x == 0) {
+        out[0] = m;
+        out[1] = *np;
+    }
+}
+// C_SKIP: the speculative part of a Newton iteration is discarded when a scene's line search is still
+// halving (C_PENDING), the swept path left the cached candidate region (C_OUTSIDE), the persistent
+// PCG gave up at a grid barrier (*pcg_fail, when that solve's flag is unread) or the assembly's lists
+// outgrew their buffers
+__global__ void k_skip_flag(int* cnt, const unsigned* pcg_fail, int act_cap, int pair_cap) {
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        cnt[C_SKIP] = (cnt[C_PENDING] != 0 || cnt[C_OUTSIDE] != 0 || (pcg_fail && *pcg_fail != 0u) ||
+                       cnt[C_NACT] > act_cap || cnt[C_NUNIQ] > pair_cap) ? 1 : 0;
+}
+// *n_out = pos[n - 1] + flag[n - 1] (the length of a compacted list), 0 for an empty input
+__global__ void k_scan_total(int n, const int* flag, const int* pos, int* n_out) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *n_out = n > 0 ? pos[n - 1] + flag[n - 1] : 0;
 }
 __global__ void k_copy(int n, const double* a, double* b) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -324,6 +386,22 @@ int fill_i32(AbdCtx* ctx, int* p, int n, int v) {
     return ABD_OK;
 }
 
+int time_bytes(AbdCtx* ctx, const int* kmax, double fixed, double per_pair) {
+    if (ctx->T.last_idx < 0 || ctx->T.nk >= Timer::KSLOTS) return ABD_OK;
+    if (!ctx->T.kslots) {
+        HIPCHK(hipMalloc((void**)&ctx->T.kslots, sizeof(int) * 2 * Timer::KSLOTS));
+        HIPCHK(hipHostMalloc((void**)&ctx->T.h_kslots, sizeof(int) * 2 * Timer::KSLOTS, hipHostMallocDefault));
+    }
+    const int slot = ctx->T.nk++;
+    hipLaunchKernelGGL(k_timing_slot, dim3(1), dim3(64), 0, ctx->st, ctx->ns, kmax, ctx->cnt + C_NUNIQ,
+                       ctx->T.kslots + 2 * slot);
+    Timer::Rec& r = ctx->T.open[ctx->T.last_idx];
+    r.kslot = slot;
+    r.fixed = fixed;
+    r.per_pair = per_pair;
+    return ABD_OK;
+}
+
 // --------------------------------------------------------------------------------------------
 // sort / scan (abd_sort.hip) with scratch grown on demand
 // --------------------------------------------------------------------------------------------
@@ -339,23 +417,23 @@ int ensure_tmp(AbdCtx* ctx, size_t bytes) {
 }
 
 // sig: the bits that may be set in any key
-int sort_keys(AbdCtx* ctx, const uint64_t* in, uint64_t* out, int n, uint64_t sig) {
-    if (n <= 0) return ABD_OK;
-    CHK(ensure_tmp(ctx, radix_scratch_bytes(n)));
+int sort_keys(AbdCtx* ctx, const uint64_t* in, uint64_t* out, Cnt n, uint64_t sig) {
+    if (n.cap <= 0) return ABD_OK;
+    CHK(ensure_tmp(ctx, radix_scratch_bytes(n.cap)));
     HIPCHK(radix_sort64(ctx->st, in, out, nullptr, nullptr, n, sig, ctx->sort_tmp));
     return dbg(ctx, "radix_sort64(keys)");
 }
 
-int sort_pairs(AbdCtx* ctx, const uint64_t* kin, uint64_t* kout, const int* vin, int* vout, int n, uint64_t sig) {
-    if (n <= 0) return ABD_OK;
-    CHK(ensure_tmp(ctx, radix_scratch_bytes(n)));
+int sort_pairs(AbdCtx* ctx, const uint64_t* kin, uint64_t* kout, const int* vin, int* vout, Cnt n, uint64_t sig) {
+    if (n.cap <= 0) return ABD_OK;
+    CHK(ensure_tmp(ctx, radix_scratch_bytes(n.cap)));
     HIPCHK(radix_sort64(ctx->st, kin, kout, vin, vout, n, sig, ctx->sort_tmp));
     return dbg(ctx, "radix_sort64(pairs)");
 }
 
-int excl_scan(AbdCtx* ctx, const int* in, int* out, int n) {
-    if (n <= 0) return ABD_OK;
-    CHK(ensure_tmp(ctx, sizeof(int) * (size_t)scan_scratch_ints(n)));
+int excl_scan(AbdCtx* ctx, const int* in, int* out, Cnt n) {
+    if (n.cap <= 0) return ABD_OK;
+    CHK(ensure_tmp(ctx, sizeof(int) * (size_t)scan_scratch_ints(n.cap)));
     HIPCHK(excl_scan_int(ctx->st, in, out, n, (int*)ctx->sort_tmp));
     return dbg(ctx, "excl_scan_int");
 }
@@ -467,7 +545,7 @@ int traverse(AbdCtx* ctx, const double4* X0, const double4* X1, const double* du
     CHK(dbg(ctx, "launch_body_boxes"));
     launch_body_pair_count(ctx->st, ctx->G, ctx->body_box, ctx->bp_cnt);
     CHK(dbg(ctx, "launch_body_pair_count"));
-    CHK(excl_scan(ctx, ctx->bp_cnt, ctx->bp_off, ctx->nb + 1));
+    CHK(excl_scan(ctx, ctx->bp_cnt, ctx->bp_off, host_cnt(ctx->nb + 1)));
     HIPCHK(hipMemcpyAsync(ctx->cnt + C_COUNT, ctx->bp_off + ctx->nb, sizeof(int), hipMemcpyDeviceToDevice, ctx->st));
     CHK(sync_counters(ctx));
     ctx->n_bpairs = ctx->h_cnt[C_COUNT];
@@ -520,19 +598,26 @@ int traverse(AbdCtx* ctx, const double4* X0, const double4* X1, const double* du
     return ABD_OK;
 }
 
+// the candidate list's length: bounded by nc_bound on the host, exact in C_NCAND on the device
+Cnt cand_cnt(const AbdCtx* ctx) { return Cnt{ctx->nc_bound, ctx->cnt + C_NCAND}; }
+
 // per-scene ranges of the sorted candidates
 int cand_ranges(AbdCtx* ctx) {
-    HIPCHK(hipMemsetAsync(ctx->cand_lo, 0, sizeof(int) * ctx->ns, ctx->st));
-    HIPCHK(hipMemsetAsync(ctx->cand_hi, 0, sizeof(int) * ctx->ns, ctx->st));
-    launch_cand_ranges(ctx->st, ctx->n_cand, ctx->cand_s, ctx->cand_lo, ctx->cand_hi);
+    CHK(fill_i32(ctx, ctx->cand_lo, ctx->ns, 0));
+    CHK(fill_i32(ctx, ctx->cand_hi, ctx->ns, 0));
+    launch_cand_ranges(ctx->st, cand_cnt(ctx), ctx->cand_s, ctx->cand_lo, ctx->cand_hi);
     CHK(dbg(ctx, "launch_cand_ranges"));
     HIPCHK(hipGetLastError());
     return ABD_OK;
 }
 
 // the candidates over X0 -> X1 from the cache: rebuilt by a widened traversal when a vertex of the
-// path leaves the cached region, then filtered by the exact rule (sorted order kept)
-int cached_broad_phase(AbdCtx* ctx, const double4* X0, const double4* X1, const double* du) {
+// path leaves the cached region, then filtered by the exact rule (sorted order kept); the filtered
+// length stays on the device (C_NCAND), bounded by the cache's length
+//
+// speculative: the containment test's answer (C_OUTSIDE) is not waited for; the cached list is used
+// either way and the caller discards the iteration when the flag turns out to be set.
+int cached_broad_phase(AbdCtx* ctx, const double4* X0, const double4* X1, const double* du, bool speculative) {
     const double m = ctx->P.cand_margin;
     if (ctx->cache_valid) {
         tic(ctx, "cache_check", (double)ctx->nv * 128);
@@ -540,15 +625,17 @@ int cached_broad_phase(AbdCtx* ctx, const double4* X0, const double4* X1, const 
         launch_cache_contains(ctx->st, ctx->nv, ctx->Xc0, ctx->Xc1, X0, X1, 0.5 * m, ctx->cnt + C_OUTSIDE);
         CHK(dbg(ctx, "launch_cache_contains"));
         toc(ctx);
-        CHK(sync_counters(ctx));
-        if (ctx->h_cnt[C_OUTSIDE]) ctx->cache_valid = false;
+        if (!speculative) {
+            CHK(sync_counters(ctx));
+            if (ctx->h_cnt[C_OUTSIDE]) ctx->cache_valid = false;
+        }
     }
     if (!ctx->cache_valid) {
         int n = 0;
         CHK(traverse(ctx, X0, X1, du, ctx->P.dhat + 2.0 * m, 1, true, &n));
         ctx->n_cache = n;
         tic(ctx, "candidate_sort");
-        CHK(sort_keys(ctx, ctx->cache, ctx->cache_s, n, low_bits(48 + bits_for(ctx->ns + 1))));
+        CHK(sort_keys(ctx, ctx->cache, ctx->cache_s, host_cnt(n), low_bits(48 + bits_for(ctx->ns + 1))));
         toc(ctx);
         HIPCHK(hipMemcpyAsync(ctx->Xc0, X0, ctx->nv * sizeof(double4), hipMemcpyDeviceToDevice, ctx->st));
         HIPCHK(hipMemcpyAsync(ctx->Xc1, X1, ctx->nv * sizeof(double4), hipMemcpyDeviceToDevice, ctx->st));
@@ -556,36 +643,37 @@ int cached_broad_phase(AbdCtx* ctx, const double4* X0, const double4* X1, const 
     }
     const int nc = ctx->n_cache;
     CHK(grow_candidates(ctx, nc));
-    ctx->n_cand = 0;
+    ctx->nc_bound = nc;
     if (nc > 0) {
         // algorithmic bytes: the cached key, four vertices at both ends, the flag
         tic(ctx, "broad_phase_filter", (double)nc * (8 + 8 * 32 + 4));
         launch_cache_filter(ctx->st, ctx->G, nc, ctx->cache_s, X0, X1, ctx->P, ctx->P.dhat, ctx->cache_flag);
         CHK(dbg(ctx, "launch_cache_filter"));
-        CHK(excl_scan(ctx, ctx->cache_flag, ctx->cache_pos, nc));
-        launch_compact(ctx->st, nc, ctx->cache_flag, ctx->cache_pos, ctx->cache_s, ctx->cand_s);
+        CHK(excl_scan(ctx, ctx->cache_flag, ctx->cache_pos, host_cnt(nc)));
+        launch_compact(ctx->st, host_cnt(nc), ctx->cache_flag, ctx->cache_pos, ctx->cache_s, ctx->cand_s);
         CHK(dbg(ctx, "launch_compact"));
-        HIPCHK(hipMemcpyAsync(ctx->cnt + C_COUNT, ctx->cache_pos + nc - 1, sizeof(int), hipMemcpyDeviceToDevice, ctx->st));
-        HIPCHK(hipMemcpyAsync(ctx->cnt + C_OVERFLOW, ctx->cache_flag + nc - 1, sizeof(int), hipMemcpyDeviceToDevice,
-                              ctx->st));
         toc(ctx);
-        CHK(sync_counters(ctx));
-        ctx->n_cand = ctx->h_cnt[C_COUNT] + ctx->h_cnt[C_OVERFLOW];
     }
-    if (ctx->P.reserved0 & 4)
+    hipLaunchKernelGGL(k_scan_total, dim3(1), dim3(64), 0, ctx->st, nc, ctx->cache_flag, ctx->cache_pos,
+                       ctx->cnt + C_NCAND);
+    CHK(dbg(ctx, "k_scan_total"));
+    if (ctx->P.reserved0 & 4) {
+        CHK(sync_counters(ctx));
         std::fprintf(stderr, "[abd] cached broad phase (%s): %d cached, %d candidates\n", du ? "swept" : "static",
-                     nc, ctx->n_cand);
+                     nc, ctx->h_cnt[C_NCAND]);
+    }
     return cand_ranges(ctx);
 }
 
-int broad_phase(AbdCtx* ctx, const double4* X0, const double4* X1, const double* du) {
+int broad_phase(AbdCtx* ctx, const double4* X0, const double4* X1, const double* du, bool speculative = false) {
     ctx->step_detections += 1;
-    if (ctx->P.cand_margin > 0.0) return cached_broad_phase(ctx, X0, X1, du);
+    if (ctx->P.cand_margin > 0.0) return cached_broad_phase(ctx, X0, X1, du, speculative);
     int n = 0;
     CHK(traverse(ctx, X0, X1, du, ctx->P.dhat, 0, false, &n));
-    ctx->n_cand = n;
+    ctx->nc_bound = n;
+    CHK(fill_i32(ctx, ctx->cnt + C_NCAND, 1, n));
     tic(ctx, "candidate_sort");
-    CHK(sort_keys(ctx, ctx->cand, ctx->cand_s, ctx->n_cand, low_bits(48 + bits_for(ctx->ns + 1))));
+    CHK(sort_keys(ctx, ctx->cand, ctx->cand_s, host_cnt(n), low_bits(48 + bits_for(ctx->ns + 1))));
     toc(ctx);
     return cand_ranges(ctx);
 }
@@ -601,72 +689,111 @@ int static_broad_phase(AbdCtx* ctx) {
 // --------------------------------------------------------------------------------------------
 // assembly at the current iterate over the current candidate list
 // --------------------------------------------------------------------------------------------
+// The assembly's lists live on the device, bounded by the buffers: at most act_cap active contacts
+// (2 act_cap incidences, act_cap contact body pairs) and pair_cap distinct body pairs (2 pair_cap CSR
+// entries).  Kernels read the lengths on the device and clamp them to these capacities, so an
+// assembly whose active set outgrew them stays in bounds and is flagged instead (assembly_overflow);
+// the caller grows the buffers to the counts and assembles again.
+Cnt act_cnt(const AbdCtx* ctx) { return Cnt{ctx->act_cap, ctx->cnt + C_NACT}; }
+Cnt inc_cnt(const AbdCtx* ctx) { return Cnt{2 * ctx->act_cap, ctx->cnt + C_NINC}; }
+Cnt pk_cnt(const AbdCtx* ctx) { return Cnt{ctx->act_cap, ctx->cnt + C_NPK}; }
+Cnt pair_cnt(const AbdCtx* ctx) { return Cnt{ctx->pair_cap, ctx->cnt + C_NUNIQ}; }
+Cnt ent_cnt(const AbdCtx* ctx) { return Cnt{2 * ctx->pair_cap, ctx->cnt + C_NENT}; }
+
+// the counters of the last assembly (h_cnt, after a round trip) exceed the buffers
+bool assembly_overflow(const AbdCtx* ctx) {
+    return ctx->h_cnt[C_NACT] > ctx->act_cap || ctx->h_cnt[C_NUNIQ] > ctx->pair_cap;
+}
+
+// buffers for the counts of the last assembly, with room to grow
+int grow_to_counts(AbdCtx* ctx) {
+    CHK(grow_active(ctx, ctx->h_cnt[C_NACT] + ctx->h_cnt[C_NACT] / 2 + 64));
+    CHK(grow_pairs(ctx, ctx->h_cnt[C_NUNIQ] + ctx->h_cnt[C_NUNIQ] / 2 + 64));
+    return ABD_OK;
+}
+
 int assemble(AbdCtx* ctx) {
-    const int nc = ctx->n_cand;
+    const int nc = ctx->nc_bound;
+    const Cnt cc = cand_cnt(ctx), ca = act_cnt(ctx);
+    if (ctx->act_cap == 0) CHK(grow_active(ctx, 1024));
+    if (ctx->pair_cap == 0) CHK(grow_pairs(ctx, 1024));
     launch_xform(ctx->st, ctx->G, ctx->q, nullptr, nullptr, ctx->X0);
     CHK(dbg(ctx, "launch_xform"));
     tic(ctx, "narrow_phase", (double)nc * (8 + 4 * 32 + 4 + 8));
-    launch_cand_eval(ctx->st, ctx->G, nc, ctx->cand_s, ctx->X0, ctx->P, ctx->flag, ctx->e_cand);
+    launch_cand_eval(ctx->st, ctx->G, cc, ctx->cand_s, ctx->X0, ctx->P, ctx->flag, ctx->e_cand);
     CHK(dbg(ctx, "launch_cand_eval"));
     toc(ctx);
-    CHK(excl_scan(ctx, ctx->flag, ctx->pos, nc));
+    CHK(excl_scan(ctx, ctx->flag, ctx->pos, cc));
     hipLaunchKernelGGL(k_nact_scene, dim3(cdiv(ctx->ns, 128)), dim3(128), 0, ctx->st, ctx->ns, ctx->cand_lo,
-                       ctx->cand_hi, ctx->flag, ctx->pos, ctx->nact_s, nc, ctx->cnt + C_NACT, ctx->cnt + C_ANY);
+                       ctx->cand_hi, ctx->flag, ctx->pos, ctx->nact_s, cc, ctx->cnt + C_NACT);
     CHK(dbg(ctx, "k_nact_scene"));
-    CHK(sync_counters(ctx));
-    int na = nc > 0 ? ctx->h_cnt[C_NACT] + ctx->h_cnt[C_ANY] : 0;
-    ctx->n_act = na;
-    CHK(grow_active(ctx, na));
-    launch_compact(ctx->st, nc, ctx->flag, ctx->pos, ctx->cand_s, ctx->act);
+    launch_compact(ctx->st, cc, ctx->flag, ctx->pos, ctx->cand_s, ctx->act);
     CHK(dbg(ctx, "launch_compact"));
-    tic(ctx, "contact_hess", (double)na * (8 + 4 * 32 + 432 * 8 + 24 * 8 + 8));
-    launch_contact_hess(ctx->st, ctx->G, na, ctx->act, ctx->X0, ctx->P, ctx->act_body, ctx->act_g, ctx->act_H);
+    tic(ctx, "contact_hess", (double)ctx->act_cap * (8 + 4 * 32 + 432 * 8 + 24 * 8 + 8));
+    launch_contact_hess(ctx->st, ctx->G, ca, ctx->act, ctx->X0, ctx->P, ctx->act_body, ctx->act_g, ctx->act_H);
     CHK(dbg(ctx, "launch_contact_hess"));
     toc(ctx);
-    HIPCHK(hipMemsetAsync(ctx->cnt + C_NINC, 0, 2 * sizeof(int), ctx->st));
-    launch_incidence(ctx->st, ctx->G, na, ctx->act_body, ctx->inc, ctx->cnt + C_NINC, ctx->pk, ctx->cnt + C_NPK);
+    CHK(fill_i32(ctx, ctx->cnt + C_NINC, 2, 0));  // C_NINC, C_NPK
+    launch_incidence(ctx->st, ctx->G, ca, ctx->act_body, ctx->inc, ctx->cnt + C_NINC, ctx->pk, ctx->cnt + C_NPK);
     CHK(dbg(ctx, "launch_incidence"));
     HIPCHK(hipGetLastError());
-    CHK(sync_counters(ctx));
-    ctx->n_inc = ctx->h_cnt[C_NINC];
-    ctx->n_pk = ctx->h_cnt[C_NPK];
+    const Cnt cpk = pk_cnt(ctx);
     tic(ctx, "block_sort");
     // incidence keys body << 32 | contact << 1 | role; body-pair keys lo << 44 | hi << 24 | contact
-    const uint64_t bbits = low_bits(bits_for(ctx->nb)), cbits = low_bits(bits_for(na));
-    CHK(sort_keys(ctx, ctx->inc, ctx->inc_s, ctx->n_inc, (bbits << 32) | (cbits << 1) | 1ull));
-    CHK(sort_keys(ctx, ctx->pk, ctx->pk_s, ctx->n_pk, (bbits << 44) | (bbits << 24) | cbits));
+    const uint64_t bbits = low_bits(bits_for(ctx->nb)), cbits = low_bits(bits_for(std::max(ctx->act_cap, 1)));
+    CHK(sort_keys(ctx, ctx->inc, ctx->inc_s, inc_cnt(ctx), (bbits << 32) | (cbits << 1) | 1ull));
+    CHK(sort_keys(ctx, ctx->pk, ctx->pk_s, cpk, (bbits << 44) | (bbits << 24) | cbits));
     toc(ctx);
-    ctx->n_pairs = 0;
-    if (ctx->n_pk > 0) {
-        launch_run_heads(ctx->st, ctx->n_pk, ctx->pk_s, 24, ctx->head);
-        CHK(dbg(ctx, "launch_run_heads"));
-        CHK(excl_scan(ctx, ctx->head, ctx->hpos, ctx->n_pk));
-        launch_pair_unique(ctx->st, ctx->n_pk, ctx->pk_s, ctx->head, ctx->hpos, ctx->pair_b, ctx->run_start,
-                           ctx->cnt + C_NUNIQ);
-        CHK(dbg(ctx, "launch_pair_unique"));
-        CHK(sync_counters(ctx));
-        ctx->n_pairs = ctx->h_cnt[C_NUNIQ];
-        CHK(grow_pairs(ctx, ctx->n_pairs));
-        tic(ctx, "offblock_sum", (double)ctx->n_pk * 144 * 8 + (double)ctx->n_pairs * 2 * 144 * 8);
-        launch_offblock_sum(ctx->st, ctx->n_pairs, ctx->pair_b, ctx->run_start, ctx->n_pk, ctx->pk_s,
-                            ctx->act_body, ctx->act_H, ctx->off, ctx->offT, ctx->n_act, ctx->cnt + C_ERR);
-        CHK(dbg(ctx, "launch_offblock_sum"));
-        toc(ctx);
-        launch_csr_keys(ctx->st, ctx->n_pairs, ctx->pair_b, ctx->ent_k, ctx->ent_v);
-        CHK(dbg(ctx, "launch_csr_keys"));
-        CHK(sort_pairs(ctx, ctx->ent_k, ctx->ent_ks, ctx->ent_v, ctx->ent_vs, 2 * ctx->n_pairs,
-                       (bbits << 32) | bbits));  // body << 32 | body
-    }
-    launch_row_ptr(ctx->st, ctx->nb, 2 * ctx->n_pairs, ctx->ent_ks, ctx->row_ptr);
+    launch_run_heads(ctx->st, cpk, ctx->pk_s, 24, ctx->head);
+    CHK(dbg(ctx, "launch_run_heads"));
+    CHK(excl_scan(ctx, ctx->head, ctx->hpos, cpk));
+    launch_pair_unique(ctx->st, cpk, ctx->pk_s, ctx->head, ctx->hpos, ctx->pair_b, ctx->run_start,
+                       ctx->cnt + C_NUNIQ, ctx->cnt + C_NENT);
+    CHK(dbg(ctx, "launch_pair_unique"));
+    tic(ctx, "offblock_sum", (double)ctx->pair_cap * 144 * 8 * 3);
+    launch_offblock_sum(ctx->st, pair_cnt(ctx), ctx->pair_b, ctx->run_start, ctx->pk_s, ctx->act_body, ctx->act_H,
+                        ctx->off, ctx->offT, ca, ctx->cnt + C_ERR);
+    CHK(dbg(ctx, "launch_offblock_sum"));
+    toc(ctx);
+    launch_csr_keys(ctx->st, pair_cnt(ctx), ctx->pair_b, ctx->ent_k, ctx->ent_v);
+    CHK(dbg(ctx, "launch_csr_keys"));
+    CHK(sort_pairs(ctx, ctx->ent_k, ctx->ent_ks, ctx->ent_v, ctx->ent_vs, ent_cnt(ctx),
+                   (bbits << 32) | bbits));  // body << 32 | body
+    launch_row_ptr(ctx->st, ctx->nb, ent_cnt(ctx), ctx->ent_ks, ctx->row_ptr);
     CHK(dbg(ctx, "launch_row_ptr"));
-    tic(ctx, "body_assemble", (double)ctx->nb * (3 * 96 + 128 + 3 * 1152 + 96 + 8) + (double)ctx->n_inc * (1152 + 96));
-    launch_body_assemble(ctx->st, ctx->G, ctx->P, ctx->q, ctx->qtilde, ctx->n_inc, ctx->inc_s, ctx->act_body,
+    tic(ctx, "body_assemble", (double)ctx->nb * (3 * 96 + 128 + 3 * 1152 + 96 + 8));
+    launch_body_assemble(ctx->st, ctx->G, ctx->P, ctx->q, ctx->qtilde, inc_cnt(ctx), ctx->inc_s, ctx->act_body,
                          ctx->act_g, ctx->act_H, ctx->C.newton_active, ctx->grad, ctx->diag, ctx->pinv, ctx->e_body,
-                         ctx->n_act);
+                         ca);
     CHK(dbg(ctx, "launch_body_assemble"));
     toc(ctx);
     HIPCHK(hipGetLastError());
     ctx->assembled = true;
+    return ABD_OK;
+}
+
+// an assembly whose lists fit its buffers (one host round trip; a second assembly after growing
+// them if they did not)
+int assemble_checked(AbdCtx* ctx) {
+    for (int attempt = 0; attempt < 4; ++attempt) {
+        CHK(assemble(ctx));
+        CHK(sync_counters(ctx));
+        if (!assembly_overflow(ctx)) return ABD_OK;
+        CHK(grow_to_counts(ctx));
+    }
+    ctx->err = "assembly lists did not fit after growing their buffers";
+    return ABD_ERR_CAPACITY;
+}
+
+// exact list lengths of the last broad phase and assembly, read back for the calls that hand them
+// out (one host round trip)
+int read_counts(AbdCtx* ctx) {
+    CHK(sync_counters(ctx));
+    ctx->n_cand = ctx->h_cnt[C_NCAND];
+    ctx->n_act = ctx->h_cnt[C_NACT];
+    ctx->n_inc = ctx->h_cnt[C_NINC];
+    ctx->n_pk = ctx->h_cnt[C_NPK];
+    ctx->n_pairs = ctx->h_cnt[C_NUNIQ];
     return ABD_OK;
 }
 
@@ -678,8 +805,8 @@ int assemble(AbdCtx* ctx) {
 int warm_start(AbdCtx* ctx) {
     AbdMlp* m = ctx->mlp;
     if (m) {
-        launch_body_features(ctx->st, ctx->G, ctx->q, ctx->qtilde, ctx->qdot, ctx->n_inc, ctx->inc_s, ctx->act_g,
-                             ctx->n_act, ctx->pinv, ctx->grad, ctx->feat_raw);
+        launch_body_features(ctx->st, ctx->G, ctx->q, ctx->qtilde, ctx->qdot, inc_cnt(ctx), ctx->inc_s, ctx->act_g,
+                             act_cnt(ctx), ctx->pinv, ctx->grad, ctx->feat_raw);
         CHK(dbg(ctx, "launch_body_features"));
         int rc = mlp_reserve(m, ctx->nb);
         if (rc != ABD_OK) {
@@ -697,7 +824,7 @@ int warm_start(AbdCtx* ctx) {
         CHK(dbg(ctx, "launch_prev_x0"));
     }
     launch_residual(ctx->st, ctx->nb, ctx->grad, ctx->diag, ctx->row_ptr, ctx->ent_ks, ctx->ent_vs, ctx->off,
-                    ctx->offT, ctx->x0, ctx->rhs, ctx->n_pairs, ctx->cnt + C_ERR);
+                    ctx->offT, ctx->x0, ctx->rhs, ctx->pair_cap, ctx->cnt + C_ERR);
     launch_scene_norm(ctx->st, ctx->G, ctx->grad, ctx->nrm_b);
     launch_scene_norm(ctx->st, ctx->G, ctx->rhs, ctx->nrm_r);
     launch_warm_select(ctx->st, ctx->G, ctx->nrm_b, ctx->nrm_r, ctx->grad, ctx->rhs, ctx->x0, ctx->warm_used);
@@ -710,8 +837,8 @@ int warm_start(AbdCtx* ctx) {
 // inverse still in pinv) -> MLP -> pinv <- L_D^-T T T^T L_D^-1 per unpinned body
 int learned_precond(AbdCtx* ctx) {
     AbdMlp* m = ctx->pre_mlp;
-    launch_body_features(ctx->st, ctx->G, ctx->q, ctx->qtilde, ctx->qdot, ctx->n_inc, ctx->inc_s, ctx->act_g,
-                         ctx->n_act, ctx->pinv, ctx->grad, ctx->feat_raw);
+    launch_body_features(ctx->st, ctx->G, ctx->q, ctx->qtilde, ctx->qdot, inc_cnt(ctx), ctx->inc_s, ctx->act_g,
+                         act_cnt(ctx), ctx->pinv, ctx->grad, ctx->feat_raw);
     CHK(dbg(ctx, "launch_body_features"));
     int rc = mlp_reserve(m, ctx->nb);
     if (rc != ABD_OK) {
@@ -844,6 +971,7 @@ int solve_direct(AbdCtx* ctx, bool fallback) {
         if (!ctx->h_cnt[C_ANY]) return ABD_OK;
     }
     HIPCHK(hipMemsetAsync(ctx->dn_fail, 0, ns * sizeof(int), ctx->st));
+    CHK(read_counts(ctx));  // the scatter launch covers the exact body-pair count
     tic(ctx, "direct_factor");
     launch_direct_factor(ctx->st, ctx->dn, ns, ctx->nb, ctx->n_pairs, ctx->bscene, ctx->scene_b0, ctx->diag,
                          ctx->pair_b, ctx->offT);
@@ -933,34 +1061,34 @@ bool persist_ok(AbdCtx* ctx) {
 template <typename T>
 int solve_persistent(AbdCtx* ctx, const PcgBufT<T>& B, bool* ok) {
     *ok = false;
-    if (!ctx->coop_ok || ctx->pcg_multi || (ctx->P.reserved0 & 8)) return ABD_OK;
+    if (!ctx->coop_ok || ctx->pcg_multi || ctx->force_iter_pcg || (ctx->P.reserved0 & 8)) return ABD_OK;
     // one tile per wave when the tiles fit on the device at once (k_pcg_persist), else waves that
     // take several tiles per stage (k_pcg_multi)
     // debug bit 32: the multi-tile form even where the tiles fit (a test of that form at small sizes)
     const bool fits = persist_fits<T>(ctx) && !(ctx->P.reserved0 & 32);
     const int mcap = fits ? 0 : multi_cap<T>(ctx);
     if (!fits && mcap <= 0) return ABD_OK;
-    HIPCHK(hipMemsetAsync(ctx->cg_iters, 0, sizeof(int) * ctx->ns, ctx->st));
-    HIPCHK(hipMemsetAsync(ctx->pcg_ctl, 0, 8 * sizeof(unsigned), ctx->st));
-    if (!fits) HIPCHK(hipMemsetAsync(ctx->cg_done, 0, sizeof(int) * ctx->ns, ctx->st));
+    CHK(fill_i32(ctx, ctx->cg_iters, ctx->ns, 0));
+    CHK(fill_i32(ctx, (int*)ctx->pcg_ctl, 8, 0));
+    if (!fits) CHK(fill_i32(ctx, ctx->cg_done, ctx->ns, 0));
     // debug bit 4: raise the failure flag before the launch, so every workgroup gives up at its first
     // barrier that does not fill at once and the solve is redone with one launch per iteration
     const bool forced = (ctx->P.reserved0 & 16) != 0;
     if (forced) hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, ctx->st, 1, (int*)(ctx->pcg_ctl + 4), 1);
     if (fits && sizeof(T) == 8) {  // the FP64 solve multiplies by M_e = B_e P_col (the FP32 one by P, then B)
-        launch_pcg_mblocks(ctx->st, 2 * ctx->n_pairs, ctx->ent_ks, ctx->ent_vs, ctx->off, ctx->offT, ctx->pinv,
-                           ctx->nb, ctx->n_pairs, B.mq, ctx->cnt + C_ERR);
+        launch_pcg_mblocks(ctx->st, ent_cnt(ctx), ctx->ent_ks, ctx->ent_vs, ctx->off, ctx->offT, ctx->pinv,
+                           ctx->nb, ctx->pair_cap, B.mq, ctx->cnt + C_ERR);
         CHK(dbg(ctx, "launch_pcg_mblocks"));
     }
     tic(ctx, "pcg_solve");
     hipError_t e;
     if (fits) {
         e = launch_pcg_persist(ctx->st, ctx->ntiles, ctx->ns, ctx->tile_b0, ctx->tile_b1, ctx->tile_scene,
-                               ctx->scene_tile0, B, ctx->P.cg_tol, ctx->P.cg_max_iter, ctx->nb, ctx->n_pairs);
+                               ctx->scene_tile0, B, ctx->P.cg_tol, ctx->P.cg_max_iter, ctx->nb, ctx->pair_cap);
     } else {
         const int blocks = std::min(mcap, (ctx->ntiles + PCG_TILES_PER_BLOCK - 1) / PCG_TILES_PER_BLOCK);
         e = launch_pcg_multi(ctx->st, blocks, ctx->ntiles, ctx->ns, ctx->tile_b0, ctx->tile_b1, ctx->tile_scene,
-                             ctx->scene_tile0, B, ctx->P.cg_tol, ctx->P.cg_max_iter, ctx->nb, ctx->n_pairs);
+                             ctx->scene_tile0, B, ctx->P.cg_tol, ctx->P.cg_max_iter, ctx->nb, ctx->pair_cap);
     }
     toc(ctx);
     if (e != hipSuccess) {
@@ -971,16 +1099,18 @@ int solve_persistent(AbdCtx* ctx, const PcgBufT<T>& B, bool* ok) {
     }
     CHK(dbg(ctx, "launch_pcg_persist"));
     HIPCHK(hipMemcpyAsync(ctx->h_pcg_ctl, ctx->pcg_ctl, 8 * sizeof(unsigned), hipMemcpyDeviceToHost, ctx->st));
-    if (ctx->T.on) {
-        // algorithmic bytes of the launch: K iterations x [nnzb * (144 T + 4) + nb * (8 * 12 T + 144 T)]
-        std::vector<int> it(ctx->ns);
-        HIPCHK(hipMemcpyAsync(it.data(), ctx->cg_iters, sizeof(int) * ctx->ns, hipMemcpyDeviceToHost, ctx->st));
-        HIPCHK(hipStreamSynchronize(ctx->st));
-        const int K = it.empty() ? 0 : *std::max_element(it.begin(), it.end());
-        const double nnzb = (double)ctx->nb + 2.0 * ctx->n_pairs;
+    {
+        // algorithmic bytes of the launch: K iterations x [nnzb * (144 T + 4) + nb * (8 * 12 T + 144 T)],
+        // nnzb = nb + 2 pairs
         const double t = sizeof(T);
-        if (!ctx->T.open.empty())
-            ctx->T.open.back().bytes = (double)K * (nnzb * (144 * t + 4) + (double)ctx->nb * (8 * 12 * t + 144 * t));
+        CHK(time_bytes(ctx, ctx->cg_iters, (double)ctx->nb * (144 * t + 4 + 8 * 12 * t + 144 * t), 2.0 * (144 * t + 4)));
+    }
+    if (ctx->defer_pcg_check && !forced) {
+        // the failure flag arrives with the iteration's report; the iteration is discarded and redone
+        // with one launch per PCG iteration if it is set
+        ctx->pcg_unchecked = true;
+        *ok = true;
+        return ABD_OK;
     }
     HIPCHK(hipStreamSynchronize(ctx->st));
     if (ctx->h_pcg_ctl[4]) {
@@ -1034,23 +1164,22 @@ int run_pcg(AbdCtx* ctx, const PcgBufT<T>& B, bool warm) {
     HIPCHK(hipMemsetAsync(ctx->cg_iters, 0, sizeof(int) * ctx->ns, ctx->st));
     HIPCHK(hipMemsetAsync(ctx->cg_done, 0, sizeof(int) * ctx->ns, ctx->st));
     launch_pcg_cg(ctx->st, 0, nt, ctx->ns, ctx->tile_b0, ctx->tile_b1, ctx->tile_scene, ctx->scene_tile0, B, tol,
-                  ctx->nb, ctx->n_pairs);
+                  ctx->nb, ctx->pair_cap);
     CHK(dbg(ctx, "launch_pcg_cg(0)"));
     launch_pcg_tile_sum(ctx->st, ctx->ns, ctx->scene_tile0, ctx->part, nt, 2 * 2 + 0, ctx->bnorm, 1);
     CHK(dbg(ctx, "launch_pcg_tile_sum"));
     if (warm) hipLaunchKernelGGL(k_copy, dim3(cdiv(ctx->ns, 64)), dim3(64), 0, ctx->st, ctx->ns, ctx->nrm_b, ctx->bnorm);
     hipLaunchKernelGGL(k_copy, dim3(cdiv(ctx->ns, 64)), dim3(64), 0, ctx->st, ctx->ns, ctx->bnorm, ctx->C.residual);
     CHK(dbg(ctx, "k_copy"));
-    const int nblk = ctx->nb + 2 * ctx->n_pairs;
+    // algorithmic bytes per launch: (nb + 2 pairs) 144 T + pairs 2 (144 T + 36 T) + nb (144 T + 168 T)
     const double t = sizeof(T);
-    const double bytes = (double)nblk * 144 * t + (double)ctx->n_pairs * 2 * (144 * t + 3 * 12 * t) +
-                         (double)ctx->nb * (144 * t + 14 * 12 * t);
     int poll = 4;
     for (int it = 0; it < ctx->P.cg_max_iter; ++it) {
-        tic(ctx, "pcg_iter", bytes);
+        tic(ctx, "pcg_iter");
         launch_pcg_cg(ctx->st, it + 1, nt, ctx->ns, ctx->tile_b0, ctx->tile_b1, ctx->tile_scene, ctx->scene_tile0, B,
-                      tol, ctx->nb, ctx->n_pairs);
+                      tol, ctx->nb, ctx->pair_cap);
         toc(ctx);
+        CHK(time_bytes(ctx, nullptr, (double)ctx->nb * (144 * t + 144 * t + 168 * t), 2.0 * 144 * t + 2.0 * 180 * t));
         CHK(dbg(ctx, "launch_pcg_cg"));
         if (it + 1 == poll || it + 1 == ctx->P.cg_max_iter) {
             HIPCHK(hipMemsetAsync(ctx->cnt + C_ANY, 0, sizeof(int), ctx->st));
@@ -1081,8 +1210,8 @@ int solve(AbdCtx* ctx) {
     }
     if (ctx->log_on && !ctx->mlp) {
         // logged features use the block-Jacobi inverse, as at inference (the warm start computed them)
-        launch_body_features(ctx->st, ctx->G, ctx->q, ctx->qtilde, ctx->qdot, ctx->n_inc, ctx->inc_s, ctx->act_g,
-                             ctx->n_act, ctx->pinv, ctx->grad, ctx->feat_raw);
+        launch_body_features(ctx->st, ctx->G, ctx->q, ctx->qtilde, ctx->qdot, inc_cnt(ctx), ctx->inc_s, ctx->act_g,
+                             act_cnt(ctx), ctx->pinv, ctx->grad, ctx->feat_raw);
         CHK(dbg(ctx, "launch_body_features"));
     }
     if (ctx->pre_mlp) CHK(learned_precond(ctx));  // after the features, which use D^-1
@@ -1099,12 +1228,14 @@ int solve(AbdCtx* ctx) {
         B.r[0] = v + 4 * n12; B.r[1] = v + 5 * n12; B.w[0] = v + 6 * n12; B.w[1] = v + 7 * n12;
         B.sv[0] = v + 8 * n12; B.sv[1] = v + 9 * n12;
         B.diag = ctx->f_diag; B.pinv = ctx->f_pinv; B.off = ctx->f_off; B.offT = ctx->f_offT; B.mq = ctx->f_mq;
-        tic(ctx, "pcg_downcast", (double)(n12 + 288 * (size_t)ctx->nb + 288 * (size_t)ctx->n_pairs) * 12);
+        // the off-diagonal blocks are converted up to the pair bound (entries past the pair count are
+        // never read)
+        tic(ctx, "pcg_downcast", (double)(n12 + 288 * (size_t)ctx->nb + 288 * (size_t)ctx->pair_cap) * 12);
         launch_to_f32(ctx->st, (long)n12, b, v);
         launch_to_f32(ctx->st, 144 * (long)ctx->nb, ctx->diag, ctx->f_diag);
         launch_to_f32(ctx->st, 144 * (long)ctx->nb, ctx->pinv, ctx->f_pinv);
-        launch_to_f32(ctx->st, 144 * (long)ctx->n_pairs, ctx->off, ctx->f_off);
-        launch_to_f32(ctx->st, 144 * (long)ctx->n_pairs, ctx->offT, ctx->f_offT);
+        launch_to_f32(ctx->st, 144 * (long)ctx->pair_cap, ctx->off, ctx->f_off);
+        launch_to_f32(ctx->st, 144 * (long)ctx->pair_cap, ctx->offT, ctx->f_offT);
         toc(ctx);
         CHK(dbg(ctx, "launch_to_f32"));
         CHK(run_pcg(ctx, B, warm));
@@ -1126,18 +1257,20 @@ int solve(AbdCtx* ctx) {
 // --------------------------------------------------------------------------------------------
 // CCD bound + backtracking line search (line_search_iterative, gauss_newton.py:98-115)
 // --------------------------------------------------------------------------------------------
-int ccd_bound(AbdCtx* ctx) {
+// speculative: see cached_broad_phase
+int ccd_bound(AbdCtx* ctx, bool speculative = false) {
     launch_xform(ctx->st, ctx->G, ctx->q, nullptr, nullptr, ctx->X0);
     CHK(dbg(ctx, "launch_xform"));
     launch_xform(ctx->st, ctx->G, ctx->q, ctx->du, nullptr, ctx->X1);
     CHK(dbg(ctx, "launch_xform"));
-    CHK(broad_phase(ctx, ctx->X0, ctx->X1, ctx->du));
+    CHK(fill_i32(ctx, ctx->cnt + C_OUTSIDE, 1, 0));
+    CHK(broad_phase(ctx, ctx->X0, ctx->X1, ctx->du, speculative));
     hipLaunchKernelGGL(k_fill_u64, dim3(cdiv(ctx->ns, 128)), dim3(128), 0, ctx->st, ctx->ns, ctx->C.amax_bits,
                        (unsigned long long)0x3FF0000000000000ull);
     CHK(dbg(ctx, "k_fill_u64"));
     if (ctx->P.ccd_enabled) {
-        tic(ctx, "ccd", (double)ctx->n_cand * (8 + 8 * 32));
-        launch_ccd(ctx->st, ctx->G, ctx->n_cand, ctx->cand_s, ctx->X0, ctx->X1, ctx->P, ctx->C.amax_bits);
+        tic(ctx, "ccd", (double)ctx->nc_bound * (8 + 8 * 32));
+        launch_ccd(ctx->st, ctx->G, cand_cnt(ctx), ctx->cand_s, ctx->X0, ctx->X1, ctx->P, ctx->C.amax_bits);
         CHK(dbg(ctx, "launch_ccd"));
         toc(ctx);
     }
@@ -1145,39 +1278,57 @@ int ccd_bound(AbdCtx* ctx) {
     return ABD_OK;
 }
 
-// floor_accept: the Newton line search keeps the floor step without a decrease (the reference); the
-// predictor's search takes no step then
-int line_search(AbdCtx* ctx, int floor_accept = 1) {
-    // E0 at q over the swept candidate list
+// the line search's start: E0 at q over the swept candidate list, alpha0 from the CCD bound
+int ls_begin(AbdCtx* ctx) {
     launch_body_energy(ctx->st, ctx->G, ctx->P, ctx->q, nullptr, nullptr, ctx->qtilde, ctx->e_body);
     CHK(dbg(ctx, "launch_body_energy"));
-    tic(ctx, "energy", (double)ctx->n_cand * (8 + 4 * 32 + 8));
-    launch_cand_eval(ctx->st, ctx->G, ctx->n_cand, ctx->cand_s, ctx->X0, ctx->P, nullptr, ctx->e_cand);
+    tic(ctx, "energy", (double)ctx->nc_bound * (8 + 4 * 32 + 8));
+    launch_cand_eval(ctx->st, ctx->G, cand_cnt(ctx), ctx->cand_s, ctx->X0, ctx->P, nullptr, ctx->e_cand);
     CHK(dbg(ctx, "launch_cand_eval"));
     toc(ctx);
     launch_scene_sum(ctx->st, ctx->G, ctx->e_body, ctx->e_cand, ctx->cand_lo, ctx->cand_hi, ctx->C.E0);
     CHK(dbg(ctx, "launch_scene_sum"));
     launch_ls_init(ctx->st, ctx->ns, ctx->P, ctx->C.newton_active, ctx->C);
     CHK(dbg(ctx, "launch_ls_init"));
-    for (int trial = 0; trial < 64; ++trial) {
-        launch_xform(ctx->st, ctx->G, ctx->q, ctx->du, ctx->C.alpha, ctx->Xt);
-        CHK(dbg(ctx, "launch_xform"));
-        tic(ctx, "energy", (double)ctx->n_cand * (8 + 4 * 32 + 8));
-        launch_cand_eval(ctx->st, ctx->G, ctx->n_cand, ctx->cand_s, ctx->Xt, ctx->P, nullptr, ctx->e_cand);
-        CHK(dbg(ctx, "launch_cand_eval"));
-        toc(ctx);
-        launch_body_energy(ctx->st, ctx->G, ctx->P, ctx->q, ctx->du, ctx->C.alpha, ctx->qtilde, ctx->qtmp);
-        CHK(dbg(ctx, "launch_body_energy"));
-        launch_scene_sum(ctx->st, ctx->G, ctx->qtmp, ctx->e_cand, ctx->cand_lo, ctx->cand_hi, ctx->C.E1);
-        CHK(dbg(ctx, "launch_scene_sum"));
-        HIPCHK(hipMemsetAsync(ctx->cnt + C_PENDING, 0, sizeof(int), ctx->st));
-        launch_ls_decide(ctx->st, ctx->ns, ctx->P, ctx->C, floor_accept, ctx->cnt + C_PENDING);
-        CHK(dbg(ctx, "launch_ls_decide"));
-        HIPCHK(hipGetLastError());
+    return ABD_OK;
+}
+
+// one trial of every undecided scene: E1 at q - alpha du, accept / halve / floor (C_PENDING: some
+// scene halved)
+int ls_trial(AbdCtx* ctx, int floor_accept) {
+    launch_xform(ctx->st, ctx->G, ctx->q, ctx->du, ctx->C.alpha, ctx->Xt);
+    CHK(dbg(ctx, "launch_xform"));
+    tic(ctx, "energy", (double)ctx->nc_bound * (8 + 4 * 32 + 8));
+    launch_cand_eval(ctx->st, ctx->G, cand_cnt(ctx), ctx->cand_s, ctx->Xt, ctx->P, nullptr, ctx->e_cand);
+    CHK(dbg(ctx, "launch_cand_eval"));
+    toc(ctx);
+    launch_body_energy(ctx->st, ctx->G, ctx->P, ctx->q, ctx->du, ctx->C.alpha, ctx->qtilde, ctx->qtmp);
+    CHK(dbg(ctx, "launch_body_energy"));
+    launch_scene_sum(ctx->st, ctx->G, ctx->qtmp, ctx->e_cand, ctx->cand_lo, ctx->cand_hi, ctx->C.E1);
+    CHK(dbg(ctx, "launch_scene_sum"));
+    CHK(fill_i32(ctx, ctx->cnt + C_PENDING, 1, 0));
+    launch_ls_decide(ctx->st, ctx->ns, ctx->P, ctx->C, floor_accept, ctx->cnt + C_PENDING);
+    CHK(dbg(ctx, "launch_ls_decide"));
+    HIPCHK(hipGetLastError());
+    return ABD_OK;
+}
+
+// trials until every scene has decided (one host round trip each)
+int ls_until_done(AbdCtx* ctx, int floor_accept, int first_trial) {
+    for (int trial = first_trial; trial < 64; ++trial) {
+        CHK(ls_trial(ctx, floor_accept));
         CHK(sync_counters(ctx));
         if (!ctx->h_cnt[C_PENDING]) break;
     }
-    launch_apply(ctx->st, ctx->G, ctx->q, ctx->du, ctx->C.alpha, ctx->C.newton_active);
+    return ABD_OK;
+}
+
+// floor_accept: the Newton line search keeps the floor step without a decrease (the reference); the
+// predictor's search takes no step then
+int line_search(AbdCtx* ctx, int floor_accept = 1) {
+    CHK(ls_begin(ctx));
+    CHK(ls_until_done(ctx, floor_accept, 0));
+    launch_apply(ctx->st, ctx->G, ctx->q, ctx->du, ctx->C.alpha, ctx->C.newton_active, nullptr);
     CHK(dbg(ctx, "launch_apply"));
     HIPCHK(hipGetLastError());
     // the accepted iterate lies inside the swept boxes: the candidates stay valid
@@ -1186,13 +1337,14 @@ int line_search(AbdCtx* ctx, int floor_accept = 1) {
 }
 
 // adaptive stiffness after a Newton iteration: closest contact of each scene at the accepted iterate
-// (Xt holds it: the last trial of the line search used every scene's accepted step)
-int adapt_stiffness(AbdCtx* ctx) {
+// (Xt holds it: the last trial of the line search used every scene's accepted step); skip: see
+// k_stiffness_update
+int adapt_stiffness(AbdCtx* ctx, const int* skip = nullptr) {
     if (!(ctx->P.k_max > ctx->P.k_contact)) return ABD_OK;
     hipLaunchKernelGGL(k_fill_u64, dim3(cdiv(ctx->nb, 128)), dim3(128), 0, ctx->st, ctx->nb, ctx->C.dmin_bits,
                        (unsigned long long)0x7FF0000000000000ull);  // +inf
-    launch_dmin(ctx->st, ctx->G, ctx->n_cand, ctx->cand_s, ctx->Xt, ctx->P, ctx->C.dmin_bits);
-    launch_stiffness_update(ctx->st, ctx->G, ctx->P, ctx->C.newton_active, ctx->C);
+    launch_dmin(ctx->st, ctx->G, cand_cnt(ctx), ctx->cand_s, ctx->Xt, ctx->P, ctx->C.dmin_bits);
+    launch_stiffness_update(ctx->st, ctx->G, ctx->P, ctx->C.newton_active, ctx->C, skip);
     CHK(dbg(ctx, "adapt_stiffness"));
     HIPCHK(hipGetLastError());
     return ABD_OK;
@@ -1202,9 +1354,9 @@ int adapt_stiffness(AbdCtx* ctx) {
 // features of the warm start into feat_step
 int step_features(AbdCtx* ctx) {
     if (!ctx->cand_valid) CHK(static_broad_phase(ctx));
-    CHK(assemble(ctx));
-    launch_body_features(ctx->st, ctx->G, ctx->q, ctx->qtilde, ctx->qdot, ctx->n_inc, ctx->inc_s, ctx->act_g,
-                         ctx->n_act, ctx->pinv, ctx->grad, ctx->feat_step);
+    CHK(assemble_checked(ctx));
+    launch_body_features(ctx->st, ctx->G, ctx->q, ctx->qtilde, ctx->qdot, inc_cnt(ctx), ctx->inc_s, ctx->act_g,
+                         act_cnt(ctx), ctx->pinv, ctx->grad, ctx->feat_step);
     CHK(dbg(ctx, "launch_body_features"));
     return ABD_OK;
 }
@@ -1262,6 +1414,96 @@ int log_step(AbdCtx* ctx) {
     return ABD_OK;
 }
 
+// The kernels after the line search (q -= alpha du, adaptive stiffness, the convergence test, the
+// report) and the one host round trip of a Newton iteration: report, convergence flags and counters
+// come back together.  skip (optional): a device flag under which nothing is committed.
+int commit_iteration(AbdCtx* ctx, const int* skip, std::vector<int>& now) {
+    const int ns = ctx->ns;
+    launch_apply(ctx->st, ctx->G, ctx->q, ctx->du, ctx->C.alpha, ctx->C.newton_active, skip);
+    CHK(dbg(ctx, "launch_apply"));
+    CHK(adapt_stiffness(ctx, skip));
+    // keep the pre-check flags for the report, then stop converged scenes (abd.py:699)
+    launch_newton_check(ctx->st, ns, ctx->P, ctx->C, ctx->act_before, skip);
+    CHK(dbg(ctx, "launch_newton_check"));
+    launch_scene_kmax(ctx->st, ctx->G, ctx->C);
+    hipLaunchKernelGGL(k_report, dim3(cdiv(ns, 128)), dim3(128), 0, ctx->st, ns, ctx->C, ctx->cg_iters, ctx->nact_s,
+                       ctx->cand_lo, ctx->cand_hi, ctx->act_before, ctx->report);
+    CHK(dbg(ctx, "k_report"));
+    HIPCHK(hipMemcpyAsync(ctx->h_report, ctx->report, sizeof(double) * R_N * ns, hipMemcpyDeviceToHost, ctx->st));
+    HIPCHK(hipMemcpyAsync(now.data(), ctx->C.newton_active, sizeof(int) * ns, hipMemcpyDeviceToHost, ctx->st));
+    if (ctx->pcg_unchecked)
+        HIPCHK(hipMemcpyAsync(ctx->h_pcg_ctl, ctx->pcg_ctl, 8 * sizeof(unsigned), hipMemcpyDeviceToHost, ctx->st));
+    return sync_counters(ctx);
+}
+
+// One Newton iteration (abd.py:679-699).  fast: the persistent PCG's failure flag, the candidate
+// cache's containment test and the first line-search trial are not waited for; the iteration is
+// committed under the device flag C_SKIP and its report comes back in one round trip.  Only when the
+// flag is set does the host redo the part that needs it (the solve with one launch per PCG
+// iteration, a BVH traversal, or the remaining halvings) and commit again.
+int newton_iteration(AbdCtx* ctx, bool fast, std::vector<int>& now) {
+    if (!ctx->cand_valid) CHK(static_broad_phase(ctx));
+    CHK(fast ? assemble(ctx) : assemble_checked(ctx));
+    ctx->pcg_unchecked = false;
+    ctx->defer_pcg_check = fast;
+    const int rc = solve(ctx);
+    ctx->defer_pcg_check = false;
+    CHK(rc);
+    if (!fast) {
+        CHK(ccd_bound(ctx));
+        CHK(ls_begin(ctx));
+        CHK(ls_until_done(ctx, 1, 0));
+        CHK(commit_iteration(ctx, nullptr, now));  // applies the accepted step
+        ctx->cand_valid = true;
+        return ABD_OK;
+    }
+    CHK(ccd_bound(ctx, true));
+    CHK(ls_begin(ctx));
+    CHK(ls_trial(ctx, 1));
+    hipLaunchKernelGGL(k_skip_flag, dim3(1), dim3(64), 0, ctx->st, ctx->cnt,
+                       ctx->pcg_unchecked ? (const unsigned*)(ctx->pcg_ctl + 4) : (const unsigned*)nullptr,
+                       ctx->act_cap, ctx->pair_cap);
+    CHK(dbg(ctx, "k_skip_flag"));
+    CHK(commit_iteration(ctx, ctx->cnt + C_SKIP, now));
+    if (ctx->h_cnt[C_SKIP]) {
+        ctx->step_slow += 1;
+        if (assembly_overflow(ctx)) {
+            // the active set outgrew the buffers: grow them and redo the iteration with round trips
+            // between its stages
+            CHK(grow_to_counts(ctx));
+            ctx->pcg_unchecked = false;
+            return newton_iteration(ctx, false, now);
+        }
+        if (ctx->pcg_unchecked && ctx->h_pcg_ctl[4]) {
+            // a grid barrier did not fill: solve again with one launch per PCG iteration, and do not
+            // try the persistent form again on this context
+            ctx->coop_ok = false;
+            ctx->pcg_fallbacks += 1;
+            ctx->pcg_unchecked = false;
+            ctx->force_iter_pcg = true;
+            const int rc2 = solve(ctx);
+            ctx->force_iter_pcg = false;
+            CHK(rc2);
+            CHK(ccd_bound(ctx));
+            CHK(ls_begin(ctx));
+            CHK(ls_until_done(ctx, 1, 0));
+        } else if (ctx->h_cnt[C_OUTSIDE]) {
+            ctx->cache_valid = false;  // a traversal over the path, then the line search again
+            CHK(ccd_bound(ctx));
+            CHK(ls_begin(ctx));
+            CHK(ls_until_done(ctx, 1, 0));
+        } else {
+            CHK(ls_until_done(ctx, 1, 1));  // the halvings after the first trial
+        }
+        ctx->pcg_unchecked = false;
+        CHK(commit_iteration(ctx, nullptr, now));
+    }
+    ctx->pcg_unchecked = false;
+    // the accepted iterate lies inside the swept boxes: the candidates stay valid
+    ctx->cand_valid = true;
+    return ABD_OK;
+}
+
 int step_once(AbdCtx* ctx) {
     const int ns = ctx->ns, mi = ctx->P.max_newton;
     launch_qtilde(ctx->st, ctx->nb, ctx->qnow, ctx->qdot, ctx->P.dt, ctx->P.gravity, ctx->qtilde);
@@ -1273,7 +1515,7 @@ int step_once(AbdCtx* ctx) {
     ctx->stats.assign(ns, AbdStepStats{});
     for (auto& s : ctx->stats) { s.alpha_min = 1.0; s.ccd_alpha_min = 1.0; }
     ctx->pcg_fallbacks = 0;
-    ctx->step_traversals = ctx->step_detections = 0;
+    ctx->step_traversals = ctx->step_detections = ctx->step_slow = 0;
     hipLaunchKernelGGL(k_fill_u64, dim3(cdiv(ctx->nb, 128)), dim3(128), 0, ctx->st, ctx->nb,
                        (unsigned long long*)ctx->C.dmin_prev, (unsigned long long)0x7FF0000000000000ull);  // +inf
     CHK(fill_i32(ctx, ctx->C.k_raises, ns, 0));
@@ -1281,26 +1523,13 @@ int step_once(AbdCtx* ctx) {
                        (unsigned long long*)ctx->C.pred_alpha, 0ull);
     if (ctx->log_steps || ctx->P.predictor == 3) CHK(step_features(ctx));
     if (ctx->P.predictor > 0) CHK(predict(ctx));
-    std::vector<int> active(ns, 1);
+    // one host round trip per Newton iteration unless a mode needs the state between stages (the
+    // direct solver, sample logging)
+    const bool fast = ctx->P.solver == 0 && !ctx->log_on;
+    std::vector<int> now(ns);
     for (int it = 0; it < mi; ++it) {
         ctx->newton_it = it;
-        if (!ctx->cand_valid) CHK(static_broad_phase(ctx));
-        CHK(assemble(ctx));
-        CHK(solve(ctx));
-        CHK(ccd_bound(ctx));
-        CHK(line_search(ctx));
-        CHK(adapt_stiffness(ctx));
-        // keep the pre-check flags for the report, then stop converged scenes (abd.py:699)
-        launch_newton_check(ctx->st, ns, ctx->P, ctx->C, ctx->act_before);
-        CHK(dbg(ctx, "launch_newton_check"));
-        launch_scene_kmax(ctx->st, ctx->G, ctx->C);
-        hipLaunchKernelGGL(k_report, dim3(cdiv(ns, 128)), dim3(128), 0, ctx->st, ns, ctx->C, ctx->cg_iters,
-                           ctx->nact_s, ctx->cand_lo, ctx->cand_hi, ctx->act_before, ctx->report);
-        CHK(dbg(ctx, "k_report"));
-        HIPCHK(hipMemcpyAsync(ctx->h_report, ctx->report, sizeof(double) * R_N * ns, hipMemcpyDeviceToHost, ctx->st));
-        std::vector<int> now(ns);
-        HIPCHK(hipMemcpyAsync(now.data(), ctx->C.newton_active, sizeof(int) * ns, hipMemcpyDeviceToHost, ctx->st));
-        HIPCHK(hipStreamSynchronize(ctx->st));
+        CHK(newton_iteration(ctx, fast, now));
         bool any = false;
         for (int s = 0; s < ns; ++s) {
             const double* R = ctx->h_report + (size_t)s * R_N;
@@ -1327,7 +1556,7 @@ int step_once(AbdCtx* ctx) {
             ctx->stats[s].pcg_fallbacks = ctx->pcg_fallbacks;
             ctx->stats[s].traversals = ctx->step_traversals;
             ctx->stats[s].detections = ctx->step_detections;
-            active[s] = now[s];
+            ctx->stats[s].slow_iterations = ctx->step_slow;
             any |= now[s] != 0;
         }
         if (!any) break;
@@ -1750,6 +1979,8 @@ void abd_destroy(AbdCtx* ctx) {
     for (void* p : ctx->allocs) (void)hipFree(p);
     for (auto e : ctx->T.pool) (void)hipEventDestroy(e);
     for (auto& r : ctx->T.open) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
+    if (ctx->T.kslots) (void)hipFree(ctx->T.kslots);
+    if (ctx->T.h_kslots) (void)hipHostFree(ctx->T.h_kslots);
     if (ctx->h_cnt) (void)hipHostFree(ctx->h_cnt);
     if (ctx->h_pcg_ctl) (void)hipHostFree(ctx->h_pcg_ctl);
     if (ctx->h_report) (void)hipHostFree(ctx->h_report);
@@ -1865,12 +2096,13 @@ int abd_get_iter_log(const AbdCtx* ctx, double* residual, double* alpha, int* cg
 int abd_detect(AbdCtx* ctx, int* pairs, double* d2, int cap, int* n) {
     CHK(check_ctx(ctx));
     CHK(static_broad_phase(ctx));
+    CHK(read_counts(ctx));
     const int nc = ctx->n_cand;
     std::vector<uint64_t> keys(nc);
     std::vector<int> flag(nc);
     std::vector<double> dd(nc);
     if (nc > 0) {
-        launch_cand_eval(ctx->st, ctx->G, nc, ctx->cand_s, ctx->X0, ctx->P, ctx->flag, nullptr);
+        launch_cand_eval(ctx->st, ctx->G, host_cnt(nc), ctx->cand_s, ctx->X0, ctx->P, ctx->flag, nullptr);
         CHK(dbg(ctx, "launch_cand_eval"));
         launch_pair_d2(ctx->st, ctx->G, nc, ctx->cand_s, ctx->X0, ctx->P, ctx->e_cand);
         CHK(dbg(ctx, "launch_pair_d2"));
@@ -1908,7 +2140,8 @@ int abd_assemble(AbdCtx* ctx, double* grad, double* diag, int* pair_ij, double* 
     CHK(check_ctx(ctx));
     CHK(fill_i32(ctx, ctx->C.newton_active, ctx->ns, 1));
     CHK(static_broad_phase(ctx));
-    CHK(assemble(ctx));
+    CHK(assemble_checked(ctx));
+    CHK(read_counts(ctx));
     const int nb = ctx->nb;
     if (grad) HIPCHK(hipMemcpyAsync(grad, ctx->grad, 12 * nb * sizeof(double), hipMemcpyDeviceToHost, ctx->st));
     if (diag) HIPCHK(hipMemcpyAsync(diag, ctx->diag, 144 * (size_t)nb * sizeof(double), hipMemcpyDeviceToHost, ctx->st));
@@ -1957,7 +2190,7 @@ int abd_energy(AbdCtx* ctx, const double* q, double* energy) {
     int rc = static_broad_phase(ctx);
     if (rc == ABD_OK) {
         // a failure below is recorded, and q is still restored before returning
-        launch_cand_eval(ctx->st, ctx->G, ctx->n_cand, ctx->cand_s, ctx->X0, ctx->P, nullptr, ctx->e_cand);
+        launch_cand_eval(ctx->st, ctx->G, cand_cnt(ctx), ctx->cand_s, ctx->X0, ctx->P, nullptr, ctx->e_cand);
         rc = dbg(ctx, "launch_cand_eval");
     }
     if (rc == ABD_OK) {
@@ -2048,16 +2281,14 @@ int abd_get_vertices(AbdCtx* ctx, double* xyz) {
 int abd_get_contact_points(AbdCtx* ctx, double* midpoints, double* dist, int cap, int* n) {
     CHK(check_ctx(ctx));
     CHK(static_broad_phase(ctx));
+    CHK(read_counts(ctx));
     const int nc = ctx->n_cand;
-    launch_cand_eval(ctx->st, ctx->G, nc, ctx->cand_s, ctx->X0, ctx->P, ctx->flag, nullptr);
+    launch_cand_eval(ctx->st, ctx->G, host_cnt(nc), ctx->cand_s, ctx->X0, ctx->P, ctx->flag, nullptr);
     CHK(dbg(ctx, "launch_cand_eval"));
-    CHK(excl_scan(ctx, ctx->flag, ctx->pos, nc));
-    if (nc > 0) {
-        HIPCHK(hipMemcpyAsync(ctx->cnt + C_NACT, ctx->pos + nc - 1, sizeof(int), hipMemcpyDeviceToDevice, ctx->st));
-        HIPCHK(hipMemcpyAsync(ctx->cnt + C_ANY, ctx->flag + nc - 1, sizeof(int), hipMemcpyDeviceToDevice, ctx->st));
-    }
+    CHK(excl_scan(ctx, ctx->flag, ctx->pos, host_cnt(nc)));
+    hipLaunchKernelGGL(k_scan_total, dim3(1), dim3(64), 0, ctx->st, nc, ctx->flag, ctx->pos, ctx->cnt + C_NACT);
     CHK(sync_counters(ctx));
-    int na = nc > 0 ? ctx->h_cnt[C_NACT] + ctx->h_cnt[C_ANY] : 0;
+    int na = ctx->h_cnt[C_NACT];
     *n = na;
     if (na > cap) {
         ctx->err = "abd_get_contact_points: capacity too small";
@@ -2065,7 +2296,7 @@ int abd_get_contact_points(AbdCtx* ctx, double* midpoints, double* dist, int cap
     }
     if (na == 0) return ABD_OK;
     CHK(grow_active(ctx, na));
-    launch_compact(ctx->st, nc, ctx->flag, ctx->pos, ctx->cand_s, ctx->act);
+    launch_compact(ctx->st, host_cnt(nc), ctx->flag, ctx->pos, ctx->cand_s, ctx->act);
     CHK(dbg(ctx, "launch_compact"));
     double *dm = nullptr, *dd = nullptr;
     CHK(dalloc(ctx, &dm, 3 * na));
@@ -2086,7 +2317,7 @@ int abd_autotune(AbdCtx* ctx, int repeats, int* bodies_per_tile, int* per_iterat
     if (repeats < 1) repeats = 1;
     CHK(fill_i32(ctx, ctx->C.newton_active, ctx->ns, 1));
     CHK(static_broad_phase(ctx));
-    CHK(assemble(ctx));
+    CHK(assemble_checked(ctx));
     double best = -1.0, t_cur = -1.0;
     const int cur_bpt = ctx->bodies_per_tile, cur_multi = ctx->pcg_multi ? 1 : 0;
     int best_bpt = cur_bpt, best_multi = cur_multi;
@@ -2246,7 +2477,14 @@ int abd_get_samples(const AbdCtx* ctx, float* feat, float* du, int cap, int* n) 
 
 int abd_enable_timing(AbdCtx* ctx, int on) {
     CHK(check_ctx(ctx));
+    if (on < 0 || on > 2) {
+        ctx->err = "timing mode must be 0, 1 or 2";
+        return ABD_ERR_ARG;
+    }
+    if (ctx->st) HIPCHK(hipStreamSynchronize(ctx->st));
+    timer_collect(ctx);  // records still open belong to the previous mode
     ctx->T.on = on != 0;
+    ctx->T.pcg_only = on == 2;
     ctx->T.ms.clear(); ctx->T.calls.clear(); ctx->T.bytes.clear(); ctx->T.order.clear();
     return ABD_OK;
 }

@@ -1,0 +1,21 @@
+// abd_cnt.hpp — list lengths that live on the device.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace abd {
+
+// Length of a list that an earlier launch on the same stream computed on the device: kernels are
+// sized by `cap` (a bound the host knows) and read the actual length from `dev` (when non-null)
+// at run time, so no host round trip sits between the producer and the consumer.
+struct Cnt {
+    int cap;
+    const int* dev;
+};
+inline Cnt host_cnt(int n) { return Cnt{n, nullptr}; }
+__device__ __forceinline__ int cnt_of(const Cnt& c) {
+    if (!c.dev) return c.cap;
+    const int n = *c.dev;
+    return n < 0 ? 0 : (n > c.cap ? c.cap : n);
+}
+
+}  // namespace abd

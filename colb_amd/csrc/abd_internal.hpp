@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "../../include/abd_hip.h"
+#include "abd_cnt.hpp"
 
 namespace abd {
 
@@ -138,11 +139,11 @@ struct GrowBuf {
 // kernel launchers (abd_kernels.hip)
 // ---------------------------------------------------------------------------------------------
 // abd_sort.hip
-hipError_t excl_scan_int(hipStream_t s, const int* in, int* out, int n, int* bsum);
+hipError_t excl_scan_int(hipStream_t s, const int* in, int* out, Cnt n, int* bsum);
 int scan_scratch_ints(long n);
 size_t radix_scratch_bytes(long n);
 // sig: the bits that may be set in any key (stable sort over those bits)
-hipError_t radix_sort64(hipStream_t s, const uint64_t* kin, uint64_t* kout, const int* vin, int* vout, int n,
+hipError_t radix_sort64(hipStream_t s, const uint64_t* kin, uint64_t* kout, const int* vin, int* vout, Cnt n,
                         uint64_t sig, void* scratch);
 
 void launch_qtilde(hipStream_t s, int nb, const double* qnow, const double* qdot, double dt, const double* g, double* qtilde);
@@ -167,7 +168,7 @@ constexpr int ABD_N_FEATURES = 49;
 constexpr int ABD_FEAT_Z = 37;          // features 37..48: the block-Jacobi step D_b^-1 grad_b
 constexpr float ABD_FEAT_CLIP = 8.0f;   // standardised features are clipped to this magnitude
 void launch_body_features(hipStream_t s, const Geo& G, const double* q, const double* qtilde, const double* qdot,
-                          int n_inc, const uint64_t* inc, const double* g24, int n_act, const double* pinv,
+                          Cnt n_inc, const uint64_t* inc, const double* g24, Cnt n_act, const double* pinv,
                           const double* grad, float* feat);
 void launch_features_pack(hipStream_t s, int nb, int rows, int dp, const float* feat, const float* mean,
                           const float* istd, uint16_t* out);
@@ -183,28 +184,28 @@ void launch_scene_norm(hipStream_t s, const Geo& G, const double* v, double* out
 void launch_warm_select(hipStream_t s, const Geo& G, const double* nb_b, const double* nb_r, const double* b,
                         double* rhs, double* x0, int* used);
 void launch_add_inplace(hipStream_t s, int n, double* y, const double* x);
-void launch_cand_ranges(hipStream_t s, int n, const uint64_t* keys, int* lo, int* hi);
-void launch_cand_eval(hipStream_t s, const Geo& G, int n, const uint64_t* keys, const double4* X, const AbdParams& P,
+void launch_cand_ranges(hipStream_t s, Cnt n, const uint64_t* keys, int* lo, int* hi);
+void launch_cand_eval(hipStream_t s, const Geo& G, Cnt n, const uint64_t* keys, const double4* X, const AbdParams& P,
                       int* flag, double* energy);
 void launch_pair_d2(hipStream_t s, const Geo& G, int n, const uint64_t* keys, const double4* X, const AbdParams& P,
                     double* d2);
-void launch_compact(hipStream_t s, int n, const int* flag, const int* pos, const uint64_t* keys, uint64_t* out);
-void launch_contact_hess(hipStream_t s, const Geo& G, int n, const uint64_t* act, const double4* X, const AbdParams& P,
+void launch_compact(hipStream_t s, Cnt n, const int* flag, const int* pos, const uint64_t* keys, uint64_t* out);
+void launch_contact_hess(hipStream_t s, const Geo& G, Cnt n, const uint64_t* act, const double4* X, const AbdParams& P,
                          int2* body, double* g24, double* H432);
-void launch_incidence(hipStream_t s, const Geo& G, int n, const int2* body, uint64_t* inc, int* n_inc,
+void launch_incidence(hipStream_t s, const Geo& G, Cnt n, const int2* body, uint64_t* inc, int* n_inc,
                       uint64_t* pairk, int* n_pair);
-void launch_run_heads(hipStream_t s, int n, const uint64_t* sorted, int shift, int* head);
-void launch_pair_unique(hipStream_t s, int n, const uint64_t* sorted, const int* head, const int* pos, int* pair_b,
-                        int* run_start, int* n_unique);
-void launch_offblock_sum(hipStream_t s, int n_pairs, const int* pair_b, const int* run_start, int n_keys,
+void launch_run_heads(hipStream_t s, Cnt n, const uint64_t* sorted, int shift, int* head);
+void launch_pair_unique(hipStream_t s, Cnt n, const uint64_t* sorted, const int* head, const int* pos, int* pair_b,
+                        int* run_start, int* n_unique, int* n_ent);
+void launch_offblock_sum(hipStream_t s, Cnt n_pairs, const int* pair_b, const int* run_start,
                          const uint64_t* sorted, const int2* body, const double* H432, double* off, double* offT,
-                         int n_act, int* err);
-void launch_csr_keys(hipStream_t s, int n_pairs, const int* pair_b, uint64_t* keys, int* vals);
-void launch_row_ptr(hipStream_t s, int nb, int n_ent, const uint64_t* sorted, int* row_ptr);
+                         Cnt n_act, int* err);
+void launch_csr_keys(hipStream_t s, Cnt n_pairs, const int* pair_b, uint64_t* keys, int* vals);
+void launch_row_ptr(hipStream_t s, int nb, Cnt n_ent, const uint64_t* sorted, int* row_ptr);
 void launch_body_assemble(hipStream_t s, const Geo& G, const AbdParams& P, const double* q, const double* qtilde,
-                          int n_inc, const uint64_t* inc_sorted, const int2* body, const double* g24,
+                          Cnt n_inc, const uint64_t* inc_sorted, const int2* body, const double* g24,
                           const double* H432, const int* newton_active, double* grad, double* diag, double* pinv,
-                          double* e_body, int n_act);
+                          double* e_body, Cnt n_act);
 void launch_scene_sum(hipStream_t s, const Geo& G, const double* e_body, const double* e_cand, const int* cand_lo,
                       const int* cand_hi, double* out);
 void launch_pcg_tile_sum(hipStream_t s, int ns, const int* scene_tile0, const double* part, int stride, int which,
@@ -225,7 +226,7 @@ hipError_t launch_pcg_multi(hipStream_t s, int blocks, int ntiles, int ns, const
 template <typename T>
 hipError_t pcg_multi_max_blocks(int* out);
 template <typename T>
-void launch_pcg_mblocks(hipStream_t s, int n_ent, const uint64_t* ent_key, const int* ent_val, const double* off,
+void launch_pcg_mblocks(hipStream_t s, Cnt n_ent, const uint64_t* ent_key, const int* ent_val, const double* off,
                         const double* offT, const double* pinv, int nb, int n_pairs, T* mq, int* err);
 constexpr int PCG_TILES_PER_BLOCK = 4;  // waves per workgroup of k_pcg_persist (one tile each) and k_pcg_multi
 // mixed precision: FP64 -> FP32 copies of the PCG operands, FP32 -> FP64 of the solution
@@ -233,7 +234,7 @@ void launch_to_f32(hipStream_t s, long n, const double* a, float* b);
 void launch_to_f64(hipStream_t s, long n, const float* a, double* b);
 void launch_pcg_any(hipStream_t s, int ns, int ntiles, const int* scene_tile0, const double* part, int it,
                     const double* bnorm, double tol, const int* cg_done, int* any);
-void launch_ccd(hipStream_t s, const Geo& G, int n, const uint64_t* keys, const double4* X0, const double4* X1,
+void launch_ccd(hipStream_t s, const Geo& G, Cnt n, const uint64_t* keys, const double4* X0, const double4* X1,
                 const AbdParams& P, unsigned long long* amax_bits);
 void launch_body_energy(hipStream_t s, const Geo& G, const AbdParams& P, const double* q, const double* du,
                         const double* alpha_scene, const double* qtilde, double* e_body);
@@ -244,13 +245,16 @@ void launch_predict_du(hipStream_t s, const Geo& G, int mode, double dt, const d
 void launch_prev_x0(hipStream_t s, const Geo& G, const double* alpha, const double* du, double* x0);
 void launch_step_pred(hipStream_t s, const Geo& G, int dpo, const float* Y, const float* omean, const float* ostd,
                       double dt, const double* q, const double* qnow, const double* qdot, double* du);
-void launch_dmin(hipStream_t s, const Geo& G, int n, const uint64_t* keys, const double4* X, const AbdParams& P,
+void launch_dmin(hipStream_t s, const Geo& G, Cnt n, const uint64_t* keys, const double4* X, const AbdParams& P,
                  unsigned long long* dmin_bits);
-void launch_stiffness_update(hipStream_t s, const Geo& G, const AbdParams& P, const int* active, SceneCtl c);
+void launch_stiffness_update(hipStream_t s, const Geo& G, const AbdParams& P, const int* active, SceneCtl c,
+                             const int* skip);
 void launch_scene_kmax(hipStream_t s, const Geo& G, SceneCtl c);
-void launch_apply(hipStream_t s, const Geo& G, double* q, const double* du, const double* alpha, const int* active);
+void launch_apply(hipStream_t s, const Geo& G, double* q, const double* du, const double* alpha, const int* active,
+                  const int* skip);
 void launch_forward(hipStream_t s, int nb, double dt, const double* q, double* qnow, double* qdot);
-void launch_newton_check(hipStream_t s, int ns, const AbdParams& P, SceneCtl c, int* active_before);
+void launch_newton_check(hipStream_t s, int ns, const AbdParams& P, SceneCtl c, int* active_before,
+                         const int* skip);
 // direct solver (abd_direct.hip): per-scene dense Cholesky of the assembled system
 constexpr int DIRECT_TILE = 32;        // tile edge; a scene's matrix order is padded to a multiple
 constexpr int DIRECT_MAX_DOFS = 6144;  // largest padded order (the solve keeps the vector in LDS)

@@ -71,7 +71,8 @@ void launch_xform(hipStream_t s, const Geo& G, const double* q, const double* du
 // ---------------------------------------------------------------------------------------------
 // per-scene ranges of the sorted candidate keys
 // ---------------------------------------------------------------------------------------------
-__global__ void k_cand_ranges(int n, const uint64_t* __restrict__ k, int* lo, int* hi) {
+__global__ void k_cand_ranges(Cnt cn, const uint64_t* __restrict__ k, int* lo, int* hi) {
+    const int n = cnt_of(cn);
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     int s = dkey_scene(k[i]);
@@ -79,9 +80,9 @@ __global__ void k_cand_ranges(int n, const uint64_t* __restrict__ k, int* lo, in
     if (i == n - 1 || dkey_scene(k[i + 1]) != s) hi[s] = i + 1;
 }
 
-void launch_cand_ranges(hipStream_t s, int n, const uint64_t* keys, int* lo, int* hi) {
-    if (n <= 0) return;
-    hipLaunchKernelGGL(k_cand_ranges, dim3(cdiv(n, 256)), dim3(256), 0, s, n, keys, lo, hi);
+void launch_cand_ranges(hipStream_t s, Cnt n, const uint64_t* keys, int* lo, int* hi) {
+    if (n.cap <= 0) return;
+    hipLaunchKernelGGL(k_cand_ranges, dim3(cdiv(n.cap, 256)), dim3(256), 0, s, n, keys, lo, hi);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -162,8 +163,9 @@ __device__ __forceinline__ double cand_d2(const Geo& G, const PairV& p, const do
 }
 
 // energy_contact (abd.py:66-75): dt^2 k B(d^2) for active pairs (d < dhat)
-__global__ void k_cand_eval(Geo G, int n, const uint64_t* __restrict__ keys, const double4* __restrict__ X, AbdParams P,
+__global__ void k_cand_eval(Geo G, Cnt cn, const uint64_t* __restrict__ keys, const double4* __restrict__ X, AbdParams P,
                             int* __restrict__ flag, double* __restrict__ energy) {
+    const int n = cnt_of(cn);
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     PairV p = decode(G, keys[i]);
@@ -183,10 +185,10 @@ __global__ void k_cand_eval(Geo G, int n, const uint64_t* __restrict__ keys, con
     if (energy) energy[i] = through ? (double)INFINITY : P.dt * P.dt * pair_stiffness(G, p) * B;
 }
 
-void launch_cand_eval(hipStream_t s, const Geo& G, int n, const uint64_t* keys, const double4* X, const AbdParams& P,
+void launch_cand_eval(hipStream_t s, const Geo& G, Cnt n, const uint64_t* keys, const double4* X, const AbdParams& P,
                       int* flag, double* energy) {
-    if (n <= 0) return;
-    hipLaunchKernelGGL(k_cand_eval, dim3(cdiv(n, 256)), dim3(256), 0, s, G, n, keys, X, P, flag, energy);
+    if (n.cap <= 0) return;
+    hipLaunchKernelGGL(k_cand_eval, dim3(cdiv(n.cap, 256)), dim3(256), 0, s, G, n, keys, X, P, flag, energy);
 }
 
 __global__ void k_pair_d2(Geo G, int n, const uint64_t* __restrict__ keys, const double4* __restrict__ X, AbdParams P,
@@ -203,15 +205,16 @@ void launch_pair_d2(hipStream_t s, const Geo& G, int n, const uint64_t* keys, co
     hipLaunchKernelGGL(k_pair_d2, dim3(cdiv(n, 256)), dim3(256), 0, s, G, n, keys, X, P, d2);
 }
 
-__global__ void k_compact(int n, const int* __restrict__ flag, const int* __restrict__ pos,
+__global__ void k_compact(Cnt cn, const int* __restrict__ flag, const int* __restrict__ pos,
                           const uint64_t* __restrict__ keys, uint64_t* __restrict__ out) {
+    const int n = cnt_of(cn);
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n && flag[i]) out[pos[i]] = keys[i];
 }
 
-void launch_compact(hipStream_t s, int n, const int* flag, const int* pos, const uint64_t* keys, uint64_t* out) {
-    if (n <= 0) return;
-    hipLaunchKernelGGL(k_compact, dim3(cdiv(n, 256)), dim3(256), 0, s, n, flag, pos, keys, out);
+void launch_compact(hipStream_t s, Cnt n, const int* flag, const int* pos, const uint64_t* keys, uint64_t* out) {
+    if (n.cap <= 0) return;
+    hipLaunchKernelGGL(k_compact, dim3(cdiv(n.cap, 256)), dim3(256), 0, s, n, flag, pos, keys, out);
 }
 
 // contact_hessian_ee (abd.py:198-571), one wave per active contact: d^2 gradient/Hessian,
@@ -221,7 +224,7 @@ void launch_compact(hipStream_t s, int n, const int* flag, const int* pos, const
 // (double; float in the mixed-precision assembly, whose Hessian feeds only the FP32 PCG); the
 // distances, the gradient and the block sums stay FP64.
 template <typename HT>
-__global__ __launch_bounds__(64) void k_contact_hess(Geo G, int n, const uint64_t* __restrict__ act,
+__global__ __launch_bounds__(64) void k_contact_hess(Geo G, Cnt cn, const uint64_t* __restrict__ act,
                                                       const double4* __restrict__ X, AbdParams P,
                                                       int2* __restrict__ body, double* __restrict__ g24,
                                                       double* __restrict__ H432) {
@@ -230,7 +233,7 @@ __global__ __launch_bounds__(64) void k_contact_hess(Geo G, int n, const uint64_
     __shared__ int side[4];
     const int c = blockIdx.x;
     const int lane = threadIdx.x;
-    if (c >= n) return;
+    if (c >= cnt_of(cn)) return;
     PairV p = decode(G, act[c]);
     if (p.kind < 0) {
         // invalid key (already flagged): contribute nothing
@@ -316,18 +319,19 @@ __global__ __launch_bounds__(64) void k_contact_hess(Geo G, int n, const uint64_
     if (lane == 0) body[c] = make_int2(bA, bB);
 }
 
-void launch_contact_hess(hipStream_t s, const Geo& G, int n, const uint64_t* act, const double4* X, const AbdParams& P,
+void launch_contact_hess(hipStream_t s, const Geo& G, Cnt n, const uint64_t* act, const double4* X, const AbdParams& P,
                          int2* body, double* g24, double* H432) {
-    if (n <= 0) return;
+    if (n.cap <= 0) return;
     if (P.precision == 1)
-        hipLaunchKernelGGL(k_contact_hess<float>, dim3(n), dim3(64), 0, s, G, n, act, X, P, body, g24, H432);
+        hipLaunchKernelGGL(k_contact_hess<float>, dim3(n.cap), dim3(64), 0, s, G, n, act, X, P, body, g24, H432);
     else
-        hipLaunchKernelGGL(k_contact_hess<double>, dim3(n), dim3(64), 0, s, G, n, act, X, P, body, g24, H432);
+        hipLaunchKernelGGL(k_contact_hess<double>, dim3(n.cap), dim3(64), 0, s, G, n, act, X, P, body, g24, H432);
 }
 
 // incidence keys (body << 32 | contact << 1 | role) and body-pair keys (min(20) | max(20) | contact(24))
-__global__ void k_incidence(Geo G, int n, const int2* __restrict__ body, uint64_t* inc, int* n_inc, uint64_t* pairk,
+__global__ void k_incidence(Geo G, Cnt cn, const int2* __restrict__ body, uint64_t* inc, int* n_inc, uint64_t* pairk,
                             int* n_pair) {
+    const int n = cnt_of(cn);
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= n) return;
     int2 b = body[c];
@@ -345,26 +349,35 @@ __global__ void k_incidence(Geo G, int n, const int2* __restrict__ body, uint64_
     }
 }
 
-void launch_incidence(hipStream_t s, const Geo& G, int n, const int2* body, uint64_t* inc, int* n_inc, uint64_t* pairk,
+void launch_incidence(hipStream_t s, const Geo& G, Cnt n, const int2* body, uint64_t* inc, int* n_inc, uint64_t* pairk,
                       int* n_pair) {
-    if (n <= 0) return;
-    hipLaunchKernelGGL(k_incidence, dim3(cdiv(n, 256)), dim3(256), 0, s, G, n, body, inc, n_inc, pairk, n_pair);
+    if (n.cap <= 0) return;
+    hipLaunchKernelGGL(k_incidence, dim3(cdiv(n.cap, 256)), dim3(256), 0, s, G, n, body, inc, n_inc, pairk, n_pair);
 }
 
-__global__ void k_run_heads(int n, const uint64_t* __restrict__ k, int shift, int* head) {
+__global__ void k_run_heads(Cnt cn, const uint64_t* __restrict__ k, int shift, int* head) {
+    const int n = cnt_of(cn);
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     head[i] = (i == 0 || (k[i] >> shift) != (k[i - 1] >> shift)) ? 1 : 0;
 }
 
-void launch_run_heads(hipStream_t s, int n, const uint64_t* sorted, int shift, int* head) {
-    if (n <= 0) return;
-    hipLaunchKernelGGL(k_run_heads, dim3(cdiv(n, 256)), dim3(256), 0, s, n, sorted, shift, head);
+void launch_run_heads(hipStream_t s, Cnt n, const uint64_t* sorted, int shift, int* head) {
+    if (n.cap <= 0) return;
+    hipLaunchKernelGGL(k_run_heads, dim3(cdiv(n.cap, 256)), dim3(256), 0, s, n, sorted, shift, head);
 }
 
-__global__ void k_pair_unique(int n, const uint64_t* __restrict__ k, const int* __restrict__ head,
-                              const int* __restrict__ pos, int* pair_b, int* run_start, int* n_unique) {
+// n_unique receives the number of body pairs and n_ent twice that (the CSR entries); an empty
+// list writes zeros (the launch always has a workgroup)
+__global__ void k_pair_unique(Cnt cn, const uint64_t* __restrict__ k, const int* __restrict__ head,
+                              const int* __restrict__ pos, int* pair_b, int* run_start, int* n_unique, int* n_ent) {
+    const int n = cnt_of(cn);
     int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n == 0 && i == 0) {
+        run_start[0] = 0;
+        *n_unique = 0;
+        *n_ent = 0;
+    }
     if (i >= n) return;
     if (head[i]) {
         int u = pos[i];
@@ -376,22 +389,23 @@ __global__ void k_pair_unique(int n, const uint64_t* __restrict__ k, const int* 
         int total = pos[i] + head[i];
         run_start[total] = n;
         *n_unique = total;
+        *n_ent = 2 * total;
     }
 }
 
-void launch_pair_unique(hipStream_t s, int n, const uint64_t* sorted, const int* head, const int* pos, int* pair_b,
-                        int* run_start, int* n_unique) {
-    if (n <= 0) return;
-    hipLaunchKernelGGL(k_pair_unique, dim3(cdiv(n, 256)), dim3(256), 0, s, n, sorted, head, pos, pair_b, run_start,
-                       n_unique);
+void launch_pair_unique(hipStream_t s, Cnt n, const uint64_t* sorted, const int* head, const int* pos, int* pair_b,
+                        int* run_start, int* n_unique, int* n_ent) {
+    hipLaunchKernelGGL(k_pair_unique, dim3(std::max(1, cdiv(n.cap, 256))), dim3(256), 0, s, n, sorted, head, pos,
+                       pair_b, run_start, n_unique, n_ent);
 }
 
 // off-diagonal block (bmin, bmax) = sum over its contacts of H_AB (or H_AB^T when the contact's
 // first body is bmax), in sorted contact order
-__global__ void k_offblock_sum(int n_pairs, const int* __restrict__ pair_b, const int* __restrict__ run_start,
+__global__ void k_offblock_sum(Cnt cpairs, const int* __restrict__ pair_b, const int* __restrict__ run_start,
                                const uint64_t* __restrict__ k, const int2* __restrict__ body,
                                const double* __restrict__ H432, double* __restrict__ off, double* __restrict__ offT,
-                               int n_act, int* err) {
+                               Cnt cact, int* err) {
+    const int n_pairs = cnt_of(cpairs), n_act = cnt_of(cact);
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (long)n_pairs * 144) return;
     int u = (int)(t / 144), e = (int)(t % 144);
@@ -408,18 +422,18 @@ __global__ void k_offblock_sum(int n_pairs, const int* __restrict__ pair_b, cons
     offT[(size_t)u * 144 + C * 12 + R] = h;
 }
 
-void launch_offblock_sum(hipStream_t s, int n_pairs, const int* pair_b, const int* run_start, int n_keys,
+void launch_offblock_sum(hipStream_t s, Cnt n_pairs, const int* pair_b, const int* run_start,
                          const uint64_t* sorted, const int2* body, const double* H432, double* off, double* offT,
-                         int n_act, int* err) {
-    if (n_pairs <= 0) return;
-    long n = (long)n_pairs * 144;
+                         Cnt n_act, int* err) {
+    if (n_pairs.cap <= 0) return;
+    long n = (long)n_pairs.cap * 144;
     hipLaunchKernelGGL(k_offblock_sum, dim3(cdiv(n, 256)), dim3(256), 0, s, n_pairs, pair_b, run_start, sorted, body,
                        H432, off, offT, n_act, err);
 }
 
-__global__ void k_csr_keys(int n_pairs, const int* __restrict__ pair_b, uint64_t* keys, int* vals) {
+__global__ void k_csr_keys(Cnt cpairs, const int* __restrict__ pair_b, uint64_t* keys, int* vals) {
     int u = blockIdx.x * blockDim.x + threadIdx.x;
-    if (u >= n_pairs) return;
+    if (u >= cnt_of(cpairs)) return;
     uint64_t a = (uint64_t)pair_b[2 * u], b = (uint64_t)pair_b[2 * u + 1];
     keys[2 * u] = (a << 32) | b;
     vals[2 * u] = u << 1;
@@ -427,9 +441,9 @@ __global__ void k_csr_keys(int n_pairs, const int* __restrict__ pair_b, uint64_t
     vals[2 * u + 1] = (u << 1) | 1;
 }
 
-void launch_csr_keys(hipStream_t s, int n_pairs, const int* pair_b, uint64_t* keys, int* vals) {
-    if (n_pairs <= 0) return;
-    hipLaunchKernelGGL(k_csr_keys, dim3(cdiv(n_pairs, 256)), dim3(256), 0, s, n_pairs, pair_b, keys, vals);
+void launch_csr_keys(hipStream_t s, Cnt n_pairs, const int* pair_b, uint64_t* keys, int* vals) {
+    if (n_pairs.cap <= 0) return;
+    hipLaunchKernelGGL(k_csr_keys, dim3(cdiv(n_pairs.cap, 256)), dim3(256), 0, s, n_pairs, pair_b, keys, vals);
 }
 
 __device__ __forceinline__ int lower_bound64(const uint64_t* a, int n, uint64_t x) {
@@ -441,13 +455,13 @@ __device__ __forceinline__ int lower_bound64(const uint64_t* a, int n, uint64_t 
     return lo;
 }
 
-__global__ void k_row_ptr(int nb, int n_ent, const uint64_t* __restrict__ k, int* row_ptr) {
+__global__ void k_row_ptr(int nb, Cnt cent, const uint64_t* __restrict__ k, int* row_ptr) {
     int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b > nb) return;
-    row_ptr[b] = lower_bound64(k, n_ent, (uint64_t)b << 32);
+    row_ptr[b] = lower_bound64(k, cnt_of(cent), (uint64_t)b << 32);
 }
 
-void launch_row_ptr(hipStream_t s, int nb, int n_ent, const uint64_t* sorted, int* row_ptr) {
+void launch_row_ptr(hipStream_t s, int nb, Cnt n_ent, const uint64_t* sorted, int* row_ptr) {
     hipLaunchKernelGGL(k_row_ptr, dim3(cdiv(nb + 1, 256)), dim3(256), 0, s, nb, n_ent, sorted, row_ptr);
 }
 
@@ -457,15 +471,16 @@ void launch_row_ptr(hipStream_t s, int nb, int n_ent, const uint64_t* sorted, in
 // energy (energy_inertia, abd.py:53-64).  One wave per body.
 // ---------------------------------------------------------------------------------------------
 __global__ __launch_bounds__(64) void k_body_assemble(Geo G, AbdParams P, const double* __restrict__ q,
-                                                       const double* __restrict__ qtilde, int n_inc,
+                                                       const double* __restrict__ qtilde, Cnt cinc,
                                                        const uint64_t* __restrict__ inc,
                                                        const double* __restrict__ g24,
                                                        const double* __restrict__ H432,
                                                        const int* __restrict__ newton_active,
                                                        double* __restrict__ grad, double* __restrict__ diag,
                                                        double* __restrict__ pinv, double* __restrict__ e_body,
-                                                       int n_act) {
+                                                       Cnt cact) {
     __shared__ double A[81], V[81], H[144], W[12 * 24], gq[12], fac[12], qb[12], dq[12];
+    const int n_inc = cnt_of(cinc), n_act = cnt_of(cact);
     const int b = blockIdx.x;
     const int lane = threadIdx.x;
     const double pd = P.pinned_diag;
@@ -575,9 +590,9 @@ __global__ __launch_bounds__(64) void k_body_assemble(Geo G, AbdParams P, const 
 }
 
 void launch_body_assemble(hipStream_t s, const Geo& G, const AbdParams& P, const double* q, const double* qtilde,
-                          int n_inc, const uint64_t* inc_sorted, const int2* body, const double* g24,
+                          Cnt n_inc, const uint64_t* inc_sorted, const int2* body, const double* g24,
                           const double* H432, const int* newton_active, double* grad, double* diag, double* pinv,
-                          double* e_body, int n_act) {
+                          double* e_body, Cnt n_act) {
     hipLaunchKernelGGL(k_body_assemble, dim3(G.nb), dim3(64), 0, s, G, P, q, qtilde, n_inc, inc_sorted, g24, H432,
                        newton_active, grad, diag, pinv, e_body, n_act);
 }
@@ -934,11 +949,11 @@ __device__ __forceinline__ int pcg_entry(const PcgBufT<T>& B, int e, int nb, int
 // M_e = B_e P_col(e) for every CSR entry e (B_e = off or off^T of the pair), FP64 products rounded
 // to T; one thread per element.  Invalid entries get M = 0 and the error bit.
 template <typename T>
-__global__ void k_pcg_mblocks(int n_ent, const uint64_t* __restrict__ ent_key, const int* __restrict__ ent_val,
+__global__ void k_pcg_mblocks(Cnt cent, const uint64_t* __restrict__ ent_key, const int* __restrict__ ent_val,
                               const double* __restrict__ off, const double* __restrict__ offT,
                               const double* __restrict__ pinv, int nb, int n_pairs, T* __restrict__ mq, int* err) {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (long)n_ent * 144) return;
+    if (i >= (long)cnt_of(cent) * 144) return;
     const int e = (int)(i / 144), rc = (int)(i % 144), r = rc / 12, c = rc % 12;
     const int col = opaque((int)(ent_key[e] & 0xFFFFFFFFu));
     const int v = ent_val[e];
@@ -955,15 +970,15 @@ __global__ void k_pcg_mblocks(int n_ent, const uint64_t* __restrict__ ent_key, c
 }
 
 template <typename T>
-void launch_pcg_mblocks(hipStream_t s, int n_ent, const uint64_t* ent_key, const int* ent_val, const double* off,
+void launch_pcg_mblocks(hipStream_t s, Cnt n_ent, const uint64_t* ent_key, const int* ent_val, const double* off,
                         const double* offT, const double* pinv, int nb, int n_pairs, T* mq, int* err) {
-    if (n_ent <= 0) return;
-    hipLaunchKernelGGL(k_pcg_mblocks<T>, dim3(cdiv((long)n_ent * 144, 256)), dim3(256), 0, s, n_ent, ent_key, ent_val,
-                       off, offT, pinv, nb, n_pairs, mq, err);
+    if (n_ent.cap <= 0) return;
+    hipLaunchKernelGGL(k_pcg_mblocks<T>, dim3(cdiv((long)n_ent.cap * 144, 256)), dim3(256), 0, s, n_ent, ent_key,
+                       ent_val, off, offT, pinv, nb, n_pairs, mq, err);
 }
-template void launch_pcg_mblocks<double>(hipStream_t, int, const uint64_t*, const int*, const double*, const double*,
+template void launch_pcg_mblocks<double>(hipStream_t, Cnt, const uint64_t*, const int*, const double*, const double*,
                                          const double*, int, int, double*, int*);
-template void launch_pcg_mblocks<float>(hipStream_t, int, const uint64_t*, const int*, const double*, const double*,
+template void launch_pcg_mblocks<float>(hipStream_t, Cnt, const uint64_t*, const int*, const double*, const double*,
                                         const double*, int, int, float*, int*);
 
 
@@ -1335,9 +1350,10 @@ void launch_pcg_any(hipStream_t s, int ns, int ntiles, const int* scene_tile0, c
 // raw features per body: q - q~ (12), qdot (12), incident active contacts (1), summed contact
 // gradient (12), from the incidences of the last assembly
 __global__ void k_body_features(Geo G, const double* __restrict__ q, const double* __restrict__ qtilde,
-                                const double* __restrict__ qdot, int n_inc, const uint64_t* __restrict__ inc,
-                                const double* __restrict__ g24, int n_act, const double* __restrict__ pinv,
+                                const double* __restrict__ qdot, Cnt cinc, const uint64_t* __restrict__ inc,
+                                const double* __restrict__ g24, Cnt cact, const double* __restrict__ pinv,
                                 const double* __restrict__ grad, float* __restrict__ feat) {
+    const int n_inc = cnt_of(cinc), n_act = cnt_of(cact);
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= G.nb) return;
     float* f = feat + (size_t)ABD_N_FEATURES * b;
@@ -1370,7 +1386,7 @@ __global__ void k_body_features(Geo G, const double* __restrict__ q, const doubl
 }
 
 void launch_body_features(hipStream_t s, const Geo& G, const double* q, const double* qtilde, const double* qdot,
-                          int n_inc, const uint64_t* inc, const double* g24, int n_act, const double* pinv,
+                          Cnt n_inc, const uint64_t* inc, const double* g24, Cnt n_act, const double* pinv,
                           const double* grad, float* feat) {
     hipLaunchKernelGGL(k_body_features, dim3(cdiv(G.nb, 128)), dim3(128), 0, s, G, q, qtilde, qdot, n_inc, inc, g24,
                        n_act, pinv, grad, feat);
@@ -1597,10 +1613,10 @@ __device__ double accd_dev(int kind, const D3* x0, const D3* x1, double s, doubl
     return t;
 }
 
-__global__ void k_ccd(Geo G, int n, const uint64_t* __restrict__ keys, const double4* __restrict__ X0,
+__global__ void k_ccd(Geo G, Cnt cn, const uint64_t* __restrict__ keys, const double4* __restrict__ X0,
                       const double4* __restrict__ X1, AbdParams P, unsigned long long* amax_bits) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (i >= cnt_of(cn)) return;
     PairV p = decode(G, keys[i]);
     if (p.kind < 0) return;
     double toi;
@@ -1619,10 +1635,10 @@ __global__ void k_ccd(Geo G, int n, const uint64_t* __restrict__ keys, const dou
     if (toi < 1.0) atomicMin(&amax_bits[p.scene], (unsigned long long)__double_as_longlong(toi));
 }
 
-void launch_ccd(hipStream_t s, const Geo& G, int n, const uint64_t* keys, const double4* X0, const double4* X1,
+void launch_ccd(hipStream_t s, const Geo& G, Cnt n, const uint64_t* keys, const double4* X0, const double4* X1,
                 const AbdParams& P, unsigned long long* amax_bits) {
-    if (n <= 0) return;
-    hipLaunchKernelGGL(k_ccd, dim3(cdiv(n, 128)), dim3(128), 0, s, G, n, keys, X0, X1, P, amax_bits);
+    if (n.cap <= 0) return;
+    hipLaunchKernelGGL(k_ccd, dim3(cdiv(n.cap, 128)), dim3(128), 0, s, G, n, keys, X0, X1, P, amax_bits);
 }
 
 // energy_inertia (abd.py:53-64) at q - alpha_s du
@@ -1730,10 +1746,10 @@ void launch_predict_du(hipStream_t s, const Geo& G, int mode, double dt, const d
 }
 
 // smallest d^2 per body over the candidate pairs at X (ground pairs: the squared gap)
-__global__ void k_dmin(Geo G, int n, const uint64_t* __restrict__ keys, const double4* __restrict__ X, AbdParams P,
+__global__ void k_dmin(Geo G, Cnt cn, const uint64_t* __restrict__ keys, const double4* __restrict__ X, AbdParams P,
                        unsigned long long* dmin_bits) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (i >= cnt_of(cn)) return;
     PairV p = decode(G, keys[i]);
     if (p.kind < 0) return;
     const unsigned long long d2 = (unsigned long long)__double_as_longlong(cand_d2(G, p, X, P));
@@ -1741,19 +1757,22 @@ __global__ void k_dmin(Geo G, int n, const uint64_t* __restrict__ keys, const do
     if (p.kind != K_PG) atomicMin(&dmin_bits[pair_body(G, p, 3)], d2);
 }
 
-void launch_dmin(hipStream_t s, const Geo& G, int n, const uint64_t* keys, const double4* X, const AbdParams& P,
+void launch_dmin(hipStream_t s, const Geo& G, Cnt n, const uint64_t* keys, const double4* X, const AbdParams& P,
                  unsigned long long* dmin_bits) {
-    if (n <= 0) return;
-    hipLaunchKernelGGL(k_dmin, dim3(cdiv(n, 256)), dim3(256), 0, s, G, n, keys, X, P, dmin_bits);
+    if (n.cap <= 0) return;
+    hipLaunchKernelGGL(k_dmin, dim3(cdiv(n.cap, 256)), dim3(256), 0, s, G, n, keys, X, P, dmin_bits);
 }
 
 // Adaptive contact stiffness, per body (after Li et al. 2020, Incremental Potential Contact,
 // section 4.3, where one stiffness is raised for the whole scene): a body of an iterating scene
 // doubles its stiffness, up to k_max, when its closest contact is nearer than k_raise_gap * dhat and
 // nearer than after its previous iteration of the step.
-__global__ void k_stiffness_update(Geo G, AbdParams P, const int* __restrict__ active, SceneCtl c) {
+// skip (optional): a device flag; when set the kernel changes nothing (an iteration whose result the
+// host discards)
+__global__ void k_stiffness_update(Geo G, AbdParams P, const int* __restrict__ active, SceneCtl c,
+                                   const int* __restrict__ skip) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= G.nb) return;
+    if (b >= G.nb || (skip && *skip)) return;
     const int s = G.bscene[b];
     if (!active[s]) return;
     const double d = sqrt(__longlong_as_double((long long)c.dmin_bits[b]));
@@ -1764,8 +1783,9 @@ __global__ void k_stiffness_update(Geo G, AbdParams P, const int* __restrict__ a
     c.dmin_prev[b] = d;
 }
 
-void launch_stiffness_update(hipStream_t s, const Geo& G, const AbdParams& P, const int* active, SceneCtl c) {
-    hipLaunchKernelGGL(k_stiffness_update, dim3(cdiv(G.nb, 128)), dim3(128), 0, s, G, P, active, c);
+void launch_stiffness_update(hipStream_t s, const Geo& G, const AbdParams& P, const int* active, SceneCtl c,
+                             const int* skip) {
+    hipLaunchKernelGGL(k_stiffness_update, dim3(cdiv(G.nb, 128)), dim3(128), 0, s, G, P, active, c, skip);
 }
 
 // largest body stiffness of each scene (one workgroup per scene)
@@ -1785,17 +1805,18 @@ void launch_scene_kmax(hipStream_t s, const Geo& G, SceneCtl c) {
     hipLaunchKernelGGL(k_scene_kmax, dim3(G.ns), dim3(256), 0, s, G, c);
 }
 
-// add_du_kernel (abd.py:780-785): q -= alpha du
+// add_du_kernel (abd.py:780-785): q -= alpha du (nothing when the optional skip flag is set)
 __global__ void k_apply(Geo G, double* q, const double* __restrict__ du, const double* __restrict__ alpha,
-                        const int* __restrict__ active) {
+                        const int* __restrict__ active, const int* __restrict__ skip) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= 12 * G.nb) return;
+    if (i >= 12 * G.nb || (skip && *skip)) return;
     int s = G.bscene[i / 12];
     if (active[s]) q[i] -= alpha[s] * du[i];
 }
 
-void launch_apply(hipStream_t s, const Geo& G, double* q, const double* du, const double* alpha, const int* active) {
-    hipLaunchKernelGGL(k_apply, dim3(cdiv(12 * G.nb, 256)), dim3(256), 0, s, G, q, du, alpha, active);
+void launch_apply(hipStream_t s, const Geo& G, double* q, const double* du, const double* alpha, const int* active,
+                  const int* skip) {
+    hipLaunchKernelGGL(k_apply, dim3(cdiv(12 * G.nb, 256)), dim3(256), 0, s, G, q, du, alpha, active, skip);
 }
 
 // forward_states (abd.py:574-581)
@@ -1812,16 +1833,17 @@ void launch_forward(hipStream_t s, int nb, double dt, const double* q, double* q
 
 // abd.py:699: stop once the residual of the pre-update iterate is below tol
 // stop converged scenes (abd.py:699); the flags before the check go to active_before
-__global__ void k_newton_check(int ns, AbdParams P, SceneCtl c, int* active_before) {
+__global__ void k_newton_check(int ns, AbdParams P, SceneCtl c, int* active_before, const int* __restrict__ skip) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= ns) return;
+    if (s >= ns || (skip && *skip)) return;
     const int a = c.newton_active[s];
     active_before[s] = a;
     if (a && c.residual[s] < P.newton_tol) c.newton_active[s] = 0;
 }
 
-void launch_newton_check(hipStream_t s, int ns, const AbdParams& P, SceneCtl c, int* active_before) {
-    hipLaunchKernelGGL(k_newton_check, dim3(cdiv(ns, 128)), dim3(128), 0, s, ns, P, c, active_before);
+void launch_newton_check(hipStream_t s, int ns, const AbdParams& P, SceneCtl c, int* active_before,
+                         const int* skip) {
+    hipLaunchKernelGGL(k_newton_check, dim3(cdiv(ns, 128)), dim3(128), 0, s, ns, P, c, active_before, skip);
 }
 
 // get_contact_points (abd.py:759-766): midpoint of the closest points and the distance

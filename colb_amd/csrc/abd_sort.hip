@@ -1,6 +1,10 @@
 // abd_sort.hip — stable LSD radix sort of 64-bit keys (optionally carrying int values) and an
 // exclusive prefix sum of ints, for the candidate, incidence and body-pair lists.
 //
+// Every length is a Cnt: grids are sized by its host bound and the kernels read the actual length
+// on the device, so a sort or scan can follow the kernel that counted its input with no host
+// round trip in between.
+//
 // Radix sort: 4-bit digits over the bits the caller says may be set (a mask: one pass per 4-bit
 // window that holds a significant bit, windows placed greedily from the lowest such bit, so a key
 // made of a few packed fields costs the passes of its fields' widths only); up to SORT_SMALL_MAX
@@ -10,8 +14,10 @@
 // the equal digits of earlier threads and earlier items (a block-wide scan over a [digit][thread]
 // count matrix in LDS) and scatters it.  Each pass is stable, so the sort is stable.
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <stdint.h>
 
+#include "abd_cnt.hpp"
 #include "abd_device.hpp"
 
 namespace abd {
@@ -41,8 +47,9 @@ __device__ __forceinline__ int block_excl_scan256(int v, int* sh) {
     return incl - v;
 }
 
-__global__ __launch_bounds__(SORT_T) void k_scan_tile_sums(int n, const int* __restrict__ in, int* __restrict__ bsum) {
+__global__ __launch_bounds__(SORT_T) void k_scan_tile_sums(Cnt cn, const int* __restrict__ in, int* __restrict__ bsum) {
     __shared__ int sh[SORT_T];
+    const int n = cnt_of(cn);
     const int base = blockIdx.x * SORT_TILE + threadIdx.x * SORT_I;
     int s = 0;
     for (int i = 0; i < SORT_I; ++i)
@@ -70,8 +77,10 @@ __global__ __launch_bounds__(SORT_T) void k_scan_block_sums(int nb, int* bsum) {
 
 // bsum == nullptr: a single tile (n <= SORT_TILE), offset 0
 // `in` and `out` may be the same array (each element is read before the thread writes it)
-__global__ __launch_bounds__(SORT_T) void k_scan_apply(int n, const int* in, const int* __restrict__ bsum, int* out) {
+__global__ __launch_bounds__(SORT_T) void k_scan_apply(Cnt cn, const int* in, const int* __restrict__ bsum, int* out) {
     __shared__ int sh[SORT_T];
+    const int n = cnt_of(cn);
+    if (blockIdx.x * SORT_TILE >= n) return;  // whole workgroup: no barrier is skipped by part of it
     const int base = blockIdx.x * SORT_TILE + threadIdx.x * SORT_I;
     int v[SORT_I];
     int s = 0;
@@ -92,10 +101,11 @@ int scan_scratch_ints(long n) {
     return (int)(nb + 1 + (nb > SORT_TILE ? scan_scratch_ints(nb) : 0));
 }
 
-// exclusive scan of n ints; levels above the first are scanned recursively (in place)
-hipError_t excl_scan_int(hipStream_t s, const int* in, int* out, int n, int* bsum) {
-    if (n <= 0) return hipSuccess;
-    const int nb = cdivs(n, SORT_TILE);
+// exclusive scan of n ints; levels above the first are scanned recursively (in place).  Tile sums
+// of tiles past the actual length are zero, so the upper levels need only the host bound.
+hipError_t excl_scan_int(hipStream_t s, const int* in, int* out, Cnt n, int* bsum) {
+    if (n.cap <= 0) return hipSuccess;
+    const int nb = cdivs(n.cap, SORT_TILE);
     if (nb == 1) {
         hipLaunchKernelGGL(k_scan_apply, dim3(1), dim3(SORT_T), 0, s, n, in, (const int*)nullptr, out);
         return hipGetLastError();
@@ -104,7 +114,7 @@ hipError_t excl_scan_int(hipStream_t s, const int* in, int* out, int n, int* bsu
     if (nb <= SORT_TILE) {
         hipLaunchKernelGGL(k_scan_block_sums, dim3(1), dim3(SORT_T), 0, s, nb, bsum);
     } else {
-        hipError_t e = excl_scan_int(s, bsum, bsum, nb, bsum + nb + 1);
+        hipError_t e = excl_scan_int(s, bsum, bsum, host_cnt(nb), bsum + nb + 1);
         if (e != hipSuccess) return e;
     }
     hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(SORT_T), 0, s, n, in, bsum, out);
@@ -114,9 +124,13 @@ hipError_t excl_scan_int(hipStream_t s, const int* in, int* out, int n, int* bsu
 // --------------------------------------------------------------------------------------------
 // radix sort
 // --------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(SORT_T) void k_radix_hist(int n, const uint64_t* __restrict__ keys, int shift, int nblk,
+// lengths of at most SORT_SMALL_MAX are sorted by k_sort_small; the radix kernels then do nothing
+#define SORT_SMALL_MAX 8192
+__global__ __launch_bounds__(SORT_T) void k_radix_hist(Cnt cn, const uint64_t* __restrict__ keys, int shift, int nblk,
                                                         int* __restrict__ hist) {
     __shared__ int cnt[RADIX];
+    const int n = cnt_of(cn);
+    if (n <= SORT_SMALL_MAX) return;
     if (threadIdx.x < RADIX) cnt[threadIdx.x] = 0;
     __syncthreads();
     const int base = blockIdx.x * SORT_TILE + threadIdx.x * SORT_I;
@@ -126,12 +140,14 @@ __global__ __launch_bounds__(SORT_T) void k_radix_hist(int n, const uint64_t* __
     if (threadIdx.x < RADIX) hist[threadIdx.x * nblk + blockIdx.x] = cnt[threadIdx.x];
 }
 
-__global__ __launch_bounds__(SORT_T) void k_radix_scatter(int n, const uint64_t* __restrict__ kin,
+__global__ __launch_bounds__(SORT_T) void k_radix_scatter(Cnt cn, const uint64_t* __restrict__ kin,
                                                            const int* __restrict__ vin, uint64_t* __restrict__ kout,
                                                            int* __restrict__ vout, int shift, int nblk,
                                                            const int* __restrict__ hoff) {
     __shared__ int m[RADIX * SORT_T];
     __shared__ int sh[SORT_T];
+    const int n = cnt_of(cn);
+    if (n <= SORT_SMALL_MAX) return;
     const int t = threadIdx.x;
     const int base = blockIdx.x * SORT_TILE + t * SORT_I;
     uint64_t k[SORT_I];
@@ -180,12 +196,14 @@ __global__ __launch_bounds__(SORT_T) void k_radix_scatter(int n, const uint64_t*
 // #{j : k_j < k_i} + #{j < i : k_j == k_i} (stable).  Every workgroup holds all n keys in LDS and
 // ranks 16 of them, sixteen lanes per key, lane l counting over keys l, l + 16, ... (n / 16
 // comparisons per lane; at one step a wave reads 16 consecutive keys, the same for its 4 keys).
+// Lengths above SORT_SMALL_MAX leave it to the radix passes.
 #define SORT_SMALL_KEYS 16
-#define SORT_SMALL_MAX 4096
-__global__ __launch_bounds__(SORT_T) void k_sort_small(int n, const uint64_t* __restrict__ kin,
+__global__ __launch_bounds__(SORT_T) void k_sort_small(Cnt cn, const uint64_t* __restrict__ kin,
                                                         const int* __restrict__ vin, uint64_t* __restrict__ kout,
                                                         int* __restrict__ vout) {
     __shared__ uint64_t kb[SORT_SMALL_MAX];
+    const int n = cnt_of(cn);
+    if (n > SORT_SMALL_MAX || blockIdx.x * SORT_SMALL_KEYS >= n) return;  // whole workgroup
     const int t = threadIdx.x;
     for (int i = t; i < n; i += SORT_T) kb[i] = kin[i];
     __syncthreads();
@@ -223,30 +241,39 @@ static int radix_windows(uint64_t sig, int* shift) {
     return n;
 }
 
-hipError_t radix_sort64(hipStream_t s, const uint64_t* kin, uint64_t* kout, const int* vin, int* vout, int n,
+// copies the first n keys (and values) when no key bit can differ
+__global__ void k_copy_keys(Cnt cn, const uint64_t* __restrict__ kin, const int* __restrict__ vin,
+                            uint64_t* __restrict__ kout, int* __restrict__ vout) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= cnt_of(cn)) return;
+    kout[i] = kin[i];
+    if (vin) vout[i] = vin[i];
+}
+
+// n.cap <= SORT_SMALL_MAX: the counting-rank sort alone.  Larger bounds launch both forms; each
+// kernel reads the actual length and the form that does not apply returns at once.
+hipError_t radix_sort64(hipStream_t s, const uint64_t* kin, uint64_t* kout, const int* vin, int* vout, Cnt n,
                         uint64_t sig, void* scratch) {
-    if (n <= 0) return hipSuccess;
-    const int nblk = cdivs(n, SORT_TILE);
+    if (n.cap <= 0) return hipSuccess;
+    const int nblk = cdivs(n.cap, SORT_TILE);
     char* p = (char*)scratch;
     uint64_t* ktmp = (uint64_t*)p;
-    p += (size_t)n * 8;
+    p += (size_t)n.cap * 8;
     int* vtmp = (int*)p;
-    p += (size_t)n * 4;
+    p += (size_t)n.cap * 4;
     int* hist = (int*)p;
     int* hoff = hist + RADIX * nblk;
     int* bsum = hoff + RADIX * nblk;
     int shifts[64];
     const int passes = radix_windows(sig, shifts);
-    if (n <= SORT_SMALL_MAX) {
-        hipLaunchKernelGGL(k_sort_small, dim3(cdivs(n, SORT_SMALL_KEYS)), dim3(SORT_T), 0, s, n, kin, vin, kout,
+    if (passes == 0) {
+        hipLaunchKernelGGL(k_copy_keys, dim3(cdivs(n.cap, 256)), dim3(256), 0, s, n, kin, vin, kout,
                            vin ? vout : nullptr);
         return hipGetLastError();
     }
-    if (passes == 0) {
-        hipMemcpyAsync(kout, kin, (size_t)n * 8, hipMemcpyDeviceToDevice, s);
-        if (vin) hipMemcpyAsync(vout, vin, (size_t)n * 4, hipMemcpyDeviceToDevice, s);
-        return hipGetLastError();
-    }
+    hipLaunchKernelGGL(k_sort_small, dim3(cdivs(std::min(n.cap, SORT_SMALL_MAX), SORT_SMALL_KEYS)), dim3(SORT_T), 0,
+                       s, n, kin, vin, kout, vin ? vout : nullptr);
+    if (n.cap <= SORT_SMALL_MAX) return hipGetLastError();
     const uint64_t* ks = kin;
     const int* vs = vin;
     for (int pass = 0; pass < passes; ++pass) {
@@ -256,7 +283,9 @@ hipError_t radix_sort64(hipStream_t s, const uint64_t* kin, uint64_t* kout, cons
         int* vd = vin ? (to_out ? vout : vtmp) : nullptr;
         const int shift = shifts[pass];
         hipLaunchKernelGGL(k_radix_hist, dim3(nblk), dim3(SORT_T), 0, s, n, ks, shift, nblk, hist);
-        hipError_t e = excl_scan_int(s, hist, hoff, RADIX * nblk, bsum);
+        // the histogram scan sees stale entries when the radix form does not apply; its result is
+        // then unused
+        hipError_t e = excl_scan_int(s, hist, hoff, host_cnt(RADIX * nblk), bsum);
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL(k_radix_scatter, dim3(nblk), dim3(SORT_T), 0, s, n, ks, vs, kd, vd, shift, nblk, hoff);
         ks = kd;

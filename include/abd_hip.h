@@ -132,6 +132,9 @@ typedef struct AbdStepStats {
     int    traversals;         /* broad-phase BVH traversals during the step (the other detections of
                                   the step were answered by the candidate cache) */
     int    detections;         /* broad-phase detections during the step */
+    int    slow_iterations;    /* Newton iterations that needed a second host round trip (a line
+                                  search that halved, a path that left the candidate cache, or a
+                                  persistent PCG that gave up); the others take one */
 } AbdStepStats;
 
 typedef struct AbdCtx AbdCtx;
@@ -211,6 +214,9 @@ int abd_get_contact_points(AbdCtx* ctx, double* midpoints, double* dist, int cap
  * algorithmic bytes moved; *n = number of kernels. */
 int abd_get_kernel_times(const AbdCtx* ctx, char* names, int name_cap, double* ms, int* calls,
                          double* bytes, int cap, int* n);
+/* No counterpart in the reference (wp.ScopedTimer, abd.py:676, times the whole step): on = 0 off,
+ * 1 every timed stage, 2 the PCG solve launches only (two events per solve, no host
+ * synchronisation: light enough to leave on over a timed run).  Clears the accumulated times. */
 int abd_enable_timing(AbdCtx* ctx, int on);
 
 /* No counterpart in the reference (the tile autotuner of the batched configuration): assembles
