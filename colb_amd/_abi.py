@@ -87,6 +87,8 @@ class AbdStepStats(C.Structure):
         ("traversals", C.c_int),
         ("detections", C.c_int),
         ("slow_iterations", C.c_int),
+        ("graph_launches", C.c_int),
+        ("graph_captures", C.c_int),
     ]
 
     def as_dict(self):

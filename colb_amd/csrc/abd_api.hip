@@ -47,6 +47,9 @@ struct Timer {
     int* h_kslots = nullptr;   // pinned host copy
     int nk = 0;
     int last_idx = -1;         // record opened by the last tic (-1: that tic was not timed)
+    unsigned long long* pcg_ts = nullptr;  // PCG-only mode: launch timestamps of the current solve
+    double* pcg_acc = nullptr;             // PCG-only mode: summed ticks, bytes, launches
+    double clock_khz = 0.0;                // device wall-clock rate
     static constexpr int KSLOTS = 1 << 14;
     std::vector<Rec> open;
     std::vector<hipEvent_t> pool;
@@ -110,6 +113,15 @@ struct AbdCtx {
     bool cache_valid = false;
     int step_traversals = 0, step_detections = 0;
     int step_slow = 0;  // Newton iterations of the step that took a second host round trip
+    // HIP graph of the fast Newton iteration: captured the second time an iteration of the same key
+    // (graph_key) runs eagerly, then replayed by one hipGraphLaunch while the key stays
+    hipGraphExec_t gexec = nullptr;
+    uint64_t gkey = 0, gkey_seen = 0;
+    bool capturing = false;
+    bool graph_off = false;      // a capture failed on this context: eager launches from then on
+    uint64_t alloc_gen = 0;      // bumped by every device (re)allocation (captured arguments hold pointers)
+    int* h_now = nullptr;        // pinned copy of the scenes' Newton flags
+    int step_graph_launches = 0, step_captures = 0;
     // candidates: nc_bound bounds the current list's length (its kernels are sized by it), the
     // length itself is the device counter C_NCAND
     int cand_cap = 0, nc_bound = 0;
@@ -199,7 +211,7 @@ namespace {
 // candidates, C_NACT active contacts, C_NINC incidences, C_NPK contact body pairs, C_NUNIQ
 // distinct body pairs, C_NENT CSR entries), flags and the error word
 enum { C_COUNT = 0, C_OVERFLOW = 1, C_NINC = 2, C_NPK = 3, C_NUNIQ = 4, C_ANY = 5, C_PENDING = 6, C_NACT = 7, C_ERR = 8,
-       C_OUTSIDE = 9, C_NCAND = 10, C_NENT = 11, C_SKIP = 12, C_N = 13 };
+       C_OUTSIDE = 9, C_NCAND = 10, C_NENT = 11, C_SKIP = 12, C_NCACHE = 13, C_N = 14 };
 enum { R_RES = 0, R_ALPHA, R_ALPHA0, R_FLOOR, R_ACTIVE, R_CG, R_E1, R_NACT, R_NCAND, R_KC, R_RAISES, R_PALPHA, R_N };
 
 #define HIPCHK(expr)                                                                               \
@@ -219,6 +231,11 @@ enum { R_RES = 0, R_ALPHA, R_ALPHA0, R_FLOOR, R_ACTIVE, R_CG, R_E1, R_NACT, R_NC
 
 template <class T>
 int dalloc(AbdCtx* ctx, T** p, size_t n) {
+    if (ctx->capturing) {
+        ctx->err = "device allocation during graph capture";
+        return ABD_ERR_STATE;
+    }
+    ctx->alloc_gen += 1;
     n = std::max<size_t>(n, 1);
     HIPCHK(hipMalloc((void**)p, n * sizeof(T)));
     ctx->allocs.push_back((void*)*p);
@@ -228,6 +245,11 @@ int dalloc(AbdCtx* ctx, T** p, size_t n) {
 template <class T>
 int dfree(AbdCtx* ctx, T*& p) {
     if (p) {
+        if (ctx->capturing) {
+            ctx->err = "device free during graph capture";
+            return ABD_ERR_STATE;
+        }
+        ctx->alloc_gen += 1;
         auto it = std::find(ctx->allocs.begin(), ctx->allocs.end(), (void*)p);
         if (it != ctx->allocs.end()) ctx->allocs.erase(it);
         HIPCHK(hipFree(p));
@@ -236,8 +258,10 @@ int dfree(AbdCtx* ctx, T*& p) {
     return ABD_OK;
 }
 
+// events time every stage (mode 1), or only the per-iteration PCG launches in the PCG-only mode,
+// whose persistent solves stamp the device wall clock themselves (graph replays included)
 bool timed(const AbdCtx* ctx, const char* name) {
-    return ctx->T.on && (!ctx->T.pcg_only || std::strncmp(name, "pcg_", 4) == 0);
+    return ctx->T.on && (!ctx->T.pcg_only || std::strcmp(name, "pcg_iter") == 0);
 }
 
 void tic(AbdCtx* ctx, const char* name, double bytes = 0.0) {
@@ -260,7 +284,7 @@ void toc(AbdCtx* ctx) {
 int time_bytes(AbdCtx* ctx, const int* kmax, double fixed, double per_pair);
 
 void timer_collect(AbdCtx* ctx) {
-    if (!ctx->T.on) return;
+    if (!ctx->T.on || (ctx->T.open.empty() && ctx->T.nk == 0)) return;
     (void)hipStreamSynchronize(ctx->st);
     if (ctx->T.nk > 0) {
         (void)hipMemcpy(ctx->T.h_kslots, ctx->T.kslots, sizeof(int) * 2 * ctx->T.nk, hipMemcpyDeviceToHost);
@@ -346,7 +370,8 @@ __global__ void k_skip_flag(int* cnt, const unsigned* pcg_fail, int act_cap, int
                        cnt[C_NACT] > act_cap || cnt[C_NUNIQ] > pair_cap) ? 1 : 0;
 }
 // *n_out = pos[n - 1] + flag[n - 1] (the length of a compacted list), 0 for an empty input
-__global__ void k_scan_total(int n, const int* flag, const int* pos, int* n_out) {
+__global__ void k_scan_total(Cnt cn, const int* flag, const int* pos, int* n_out) {
+    const int n = cnt_of(cn);
     if (blockIdx.x == 0 && threadIdx.x == 0) *n_out = n > 0 ? pos[n - 1] + flag[n - 1] : 0;
 }
 __global__ void k_copy(int n, const double* a, double* b) {
@@ -407,6 +432,11 @@ int time_bytes(AbdCtx* ctx, const int* kmax, double fixed, double per_pair) {
 // --------------------------------------------------------------------------------------------
 int ensure_tmp(AbdCtx* ctx, size_t bytes) {
     if (bytes <= ctx->sort_bytes) return ABD_OK;
+    if (ctx->capturing) {
+        ctx->err = "scratch growth during graph capture";
+        return ABD_ERR_STATE;
+    }
+    ctx->alloc_gen += 1;
     HIPCHK(hipStreamSynchronize(ctx->st));
     CHK(dfree(ctx, ctx->sort_tmp));
     size_t nb = bytes + bytes / 4 + 4096;
@@ -438,9 +468,8 @@ int excl_scan(AbdCtx* ctx, const int* in, int* out, Cnt n) {
     return dbg(ctx, "excl_scan_int");
 }
 
-int sync_counters(AbdCtx* ctx) {
-    HIPCHK(hipMemcpyAsync(ctx->h_cnt, ctx->cnt, C_N * sizeof(int), hipMemcpyDeviceToHost, ctx->st));
-    HIPCHK(hipStreamSynchronize(ctx->st));
+// the device error word, after h_cnt came back
+int check_err_word(AbdCtx* ctx) {
     if (ctx->h_cnt[C_ERR]) {
         char buf[160];
         if (ctx->h_cnt[C_ERR] == ERR_PENETRATION)
@@ -451,6 +480,16 @@ int sync_counters(AbdCtx* ctx) {
         return ABD_ERR_STATE;
     }
     return ABD_OK;
+}
+
+int sync_counters(AbdCtx* ctx) {
+    if (ctx->capturing) {
+        ctx->err = "host round trip during graph capture";
+        return ABD_ERR_STATE;
+    }
+    HIPCHK(hipMemcpyAsync(ctx->h_cnt, ctx->cnt, C_N * sizeof(int), hipMemcpyDeviceToHost, ctx->st));
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    return check_err_word(ctx);
 }
 
 int bits_for(int n) {
@@ -634,6 +673,7 @@ int cached_broad_phase(AbdCtx* ctx, const double4* X0, const double4* X1, const 
         int n = 0;
         CHK(traverse(ctx, X0, X1, du, ctx->P.dhat + 2.0 * m, 1, true, &n));
         ctx->n_cache = n;
+        CHK(fill_i32(ctx, ctx->cnt + C_NCACHE, 1, n));
         tic(ctx, "candidate_sort");
         CHK(sort_keys(ctx, ctx->cache, ctx->cache_s, host_cnt(n), low_bits(48 + bits_for(ctx->ns + 1))));
         toc(ctx);
@@ -641,26 +681,28 @@ int cached_broad_phase(AbdCtx* ctx, const double4* X0, const double4* X1, const 
         HIPCHK(hipMemcpyAsync(ctx->Xc1, X1, ctx->nv * sizeof(double4), hipMemcpyDeviceToDevice, ctx->st));
         ctx->cache_valid = true;
     }
-    const int nc = ctx->n_cache;
-    CHK(grow_candidates(ctx, nc));
-    ctx->nc_bound = nc;
-    if (nc > 0) {
+    // the filter and its compaction are sized by the cache's capacity and read its length on the
+    // device, so their launches stay the same from one rebuild of the cache to the next
+    const Cnt cn{ctx->cache_cap, ctx->cnt + C_NCACHE};
+    CHK(grow_candidates(ctx, std::max(ctx->cache_cap, 1)));
+    ctx->nc_bound = ctx->cache_cap;
+    if (ctx->cache_cap > 0) {
         // algorithmic bytes: the cached key, four vertices at both ends, the flag
-        tic(ctx, "broad_phase_filter", (double)nc * (8 + 8 * 32 + 4));
-        launch_cache_filter(ctx->st, ctx->G, nc, ctx->cache_s, X0, X1, ctx->P, ctx->P.dhat, ctx->cache_flag);
+        tic(ctx, "broad_phase_filter", (double)ctx->n_cache * (8 + 8 * 32 + 4));
+        launch_cache_filter(ctx->st, ctx->G, cn, ctx->cache_s, X0, X1, ctx->P, ctx->P.dhat, ctx->cache_flag);
         CHK(dbg(ctx, "launch_cache_filter"));
-        CHK(excl_scan(ctx, ctx->cache_flag, ctx->cache_pos, host_cnt(nc)));
-        launch_compact(ctx->st, host_cnt(nc), ctx->cache_flag, ctx->cache_pos, ctx->cache_s, ctx->cand_s);
+        CHK(excl_scan(ctx, ctx->cache_flag, ctx->cache_pos, cn));
+        launch_compact(ctx->st, cn, ctx->cache_flag, ctx->cache_pos, ctx->cache_s, ctx->cand_s);
         CHK(dbg(ctx, "launch_compact"));
         toc(ctx);
     }
-    hipLaunchKernelGGL(k_scan_total, dim3(1), dim3(64), 0, ctx->st, nc, ctx->cache_flag, ctx->cache_pos,
+    hipLaunchKernelGGL(k_scan_total, dim3(1), dim3(64), 0, ctx->st, cn, ctx->cache_flag, ctx->cache_pos,
                        ctx->cnt + C_NCAND);
     CHK(dbg(ctx, "k_scan_total"));
     if (ctx->P.reserved0 & 4) {
         CHK(sync_counters(ctx));
         std::fprintf(stderr, "[abd] cached broad phase (%s): %d cached, %d candidates\n", du ? "swept" : "static",
-                     nc, ctx->h_cnt[C_NCAND]);
+                     ctx->n_cache, ctx->h_cnt[C_NCAND]);
     }
     return cand_ranges(ctx);
 }
@@ -1103,7 +1145,11 @@ int solve_persistent(AbdCtx* ctx, const PcgBufT<T>& B, bool* ok) {
         // algorithmic bytes of the launch: K iterations x [nnzb * (144 T + 4) + nb * (8 * 12 T + 144 T)],
         // nnzb = nb + 2 pairs
         const double t = sizeof(T);
-        CHK(time_bytes(ctx, ctx->cg_iters, (double)ctx->nb * (144 * t + 4 + 8 * 12 * t + 144 * t), 2.0 * (144 * t + 4)));
+        const double fixed = (double)ctx->nb * (144 * t + 4 + 8 * 12 * t + 144 * t), per_pair = 2.0 * (144 * t + 4);
+        if (B.ts)
+            launch_pcg_account(ctx->st, B.ts, ctx->T.pcg_acc, ctx->cg_iters, ctx->ns, ctx->cnt + C_NUNIQ, fixed, per_pair);
+        else
+            CHK(time_bytes(ctx, ctx->cg_iters, fixed, per_pair));
     }
     if (ctx->defer_pcg_check && !forced) {
         // the failure flag arrives with the iteration's report; the iteration is discarded and redone
@@ -1151,6 +1197,7 @@ void pcg_ctl(AbdCtx* ctx, PcgBufT<T>& B) {
     B.bar = ctx->pcg_ctl; B.done_cnt = (int*)(ctx->pcg_ctl + 1); B.fail = (int*)(ctx->pcg_ctl + 4);
     B.bnorm_out = ctx->bnorm; B.residual_out = ctx->C.residual;
     B.bref = nullptr;
+    B.ts = (ctx->T.on && ctx->T.pcg_only) ? ctx->T.pcg_ts : nullptr;
 }
 
 // the PCG proper: one cooperative launch where possible, else one launch per iteration
@@ -1415,9 +1462,9 @@ int log_step(AbdCtx* ctx) {
 }
 
 // The kernels after the line search (q -= alpha du, adaptive stiffness, the convergence test, the
-// report) and the one host round trip of a Newton iteration: report, convergence flags and counters
-// come back together.  skip (optional): a device flag under which nothing is committed.
-int commit_iteration(AbdCtx* ctx, const int* skip, std::vector<int>& now) {
+// report) and the copies of the report, the scenes' Newton flags, the PCG control words and the
+// counters into pinned host memory.  skip (optional): a device flag under which nothing is committed.
+int commit_launches(AbdCtx* ctx, const int* skip) {
     const int ns = ctx->ns;
     launch_apply(ctx->st, ctx->G, ctx->q, ctx->du, ctx->C.alpha, ctx->C.newton_active, skip);
     CHK(dbg(ctx, "launch_apply"));
@@ -1430,33 +1477,35 @@ int commit_iteration(AbdCtx* ctx, const int* skip, std::vector<int>& now) {
                        ctx->cand_lo, ctx->cand_hi, ctx->act_before, ctx->report);
     CHK(dbg(ctx, "k_report"));
     HIPCHK(hipMemcpyAsync(ctx->h_report, ctx->report, sizeof(double) * R_N * ns, hipMemcpyDeviceToHost, ctx->st));
-    HIPCHK(hipMemcpyAsync(now.data(), ctx->C.newton_active, sizeof(int) * ns, hipMemcpyDeviceToHost, ctx->st));
-    if (ctx->pcg_unchecked)
-        HIPCHK(hipMemcpyAsync(ctx->h_pcg_ctl, ctx->pcg_ctl, 8 * sizeof(unsigned), hipMemcpyDeviceToHost, ctx->st));
-    return sync_counters(ctx);
+    HIPCHK(hipMemcpyAsync(ctx->h_now, ctx->C.newton_active, sizeof(int) * ns, hipMemcpyDeviceToHost, ctx->st));
+    HIPCHK(hipMemcpyAsync(ctx->h_pcg_ctl, ctx->pcg_ctl, 8 * sizeof(unsigned), hipMemcpyDeviceToHost, ctx->st));
+    HIPCHK(hipMemcpyAsync(ctx->h_cnt, ctx->cnt, C_N * sizeof(int), hipMemcpyDeviceToHost, ctx->st));
+    return ABD_OK;
 }
 
-// One Newton iteration (abd.py:679-699).  fast: the persistent PCG's failure flag, the candidate
-// cache's containment test and the first line-search trial are not waited for; the iteration is
-// committed under the device flag C_SKIP and its report comes back in one round trip.  Only when the
-// flag is set does the host redo the part that needs it (the solve with one launch per PCG
-// iteration, a BVH traversal, or the remaining halvings) and commit again.
-int newton_iteration(AbdCtx* ctx, bool fast, std::vector<int>& now) {
-    if (!ctx->cand_valid) CHK(static_broad_phase(ctx));
-    CHK(fast ? assemble(ctx) : assemble_checked(ctx));
+// the iteration's one host round trip: wait for the copies of commit_launches
+int commit_wait(AbdCtx* ctx, std::vector<int>& now) {
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    std::copy(ctx->h_now, ctx->h_now + ctx->ns, now.begin());
+    return check_err_word(ctx);
+}
+
+int commit_iteration(AbdCtx* ctx, const int* skip, std::vector<int>& now) {
+    CHK(commit_launches(ctx, skip));
+    return commit_wait(ctx, now);
+}
+
+// Every launch of a fast Newton iteration up to its report copies, with no host round trip: the
+// assembly (lengths on the device), the solve (the persistent PCG's failure flag unread), the CCD
+// bound over the candidate cache (its containment test unread), the line search's first trial, the
+// skip flag, and the commit under it.
+int fast_launches(AbdCtx* ctx) {
+    CHK(assemble(ctx));
     ctx->pcg_unchecked = false;
-    ctx->defer_pcg_check = fast;
+    ctx->defer_pcg_check = true;
     const int rc = solve(ctx);
     ctx->defer_pcg_check = false;
     CHK(rc);
-    if (!fast) {
-        CHK(ccd_bound(ctx));
-        CHK(ls_begin(ctx));
-        CHK(ls_until_done(ctx, 1, 0));
-        CHK(commit_iteration(ctx, nullptr, now));  // applies the accepted step
-        ctx->cand_valid = true;
-        return ABD_OK;
-    }
     CHK(ccd_bound(ctx, true));
     CHK(ls_begin(ctx));
     CHK(ls_trial(ctx, 1));
@@ -1464,7 +1513,115 @@ int newton_iteration(AbdCtx* ctx, bool fast, std::vector<int>& now) {
                        ctx->pcg_unchecked ? (const unsigned*)(ctx->pcg_ctl + 4) : (const unsigned*)nullptr,
                        ctx->act_cap, ctx->pair_cap);
     CHK(dbg(ctx, "k_skip_flag"));
-    CHK(commit_iteration(ctx, ctx->cnt + C_SKIP, now));
+    return commit_launches(ctx, ctx->cnt + C_SKIP);
+}
+
+// Key of the fast iteration's launch sequence: every host value its launches bake in (buffer
+// pointers through the allocation generation, capacities and bounds that size grids, the
+// parameters passed by value, the PCG layout and form, the warm-start branch).
+uint64_t graph_key(const AbdCtx* ctx) {
+    uint64_t h = 1469598103934665603ull;
+    auto mix = [&](const void* p, size_t n) {
+        const unsigned char* b = (const unsigned char*)p;
+        for (size_t i = 0; i < n; ++i) h = (h ^ b[i]) * 1099511628211ull;
+    };
+    auto mixi = [&](uint64_t v) { mix(&v, sizeof v); };
+    mix(&ctx->P, sizeof ctx->P);
+    mixi(ctx->alloc_gen);
+    mixi((uint64_t)ctx->nc_bound); mixi((uint64_t)ctx->act_cap); mixi((uint64_t)ctx->pair_cap);
+    mixi((uint64_t)ctx->cand_cap); mixi((uint64_t)ctx->cache_cap);
+    mixi((uint64_t)ctx->sort_bytes); mixi((uint64_t)ctx->ntiles); mixi((uint64_t)ctx->bodies_per_tile);
+    mixi((uint64_t)ctx->pcg_multi); mixi((uint64_t)ctx->coop_ok);
+    mixi((uint64_t)(ctx->T.on && ctx->T.pcg_only)); mixi((uint64_t)(uintptr_t)ctx->T.pcg_ts);
+    mixi((uint64_t)(ctx->P.pcg_x0 == 1 && ctx->newton_it > 0));
+    return h | 1ull;  // never 0 (0 = no key)
+}
+
+// graph capture applies: the fast iteration, with the candidate cache valid (no traversal), the
+// persistent PCG (one launch), no per-stage timing, debug checks or learned warm start and
+// preconditioner, and not switched off (debug bit 6)
+bool graph_usable(AbdCtx* ctx) {
+    if (ctx->graph_off || (ctx->T.on && !ctx->T.pcg_only) || (ctx->P.reserved0 & (1 | 2 | 8 | 16 | 64))) return false;
+    if (!(ctx->P.cand_margin > 0.0) || !ctx->cache_valid || !ctx->cand_valid) return false;
+    if (ctx->mlp || ctx->pre_mlp || ctx->pcg_multi || !persist_ok(ctx)) return false;
+    return true;
+}
+
+// the fast iteration's launches as one graph launch: eager the first time a key is seen, captured
+// and instantiated the second time, replayed after that.  Host effects of the launch functions are
+// redone on a replay (the detection count, the unread PCG flag).
+int fast_graph(AbdCtx* ctx) {
+    const uint64_t key = graph_key(ctx);
+    if (ctx->gexec && key == ctx->gkey) {
+        HIPCHK(hipGraphLaunch(ctx->gexec, ctx->st));
+        ctx->step_graph_launches += 1;
+        ctx->step_detections += 1;
+        ctx->pcg_unchecked = true;
+        ctx->assembled = ctx->solved = true;
+        return ABD_OK;
+    }
+    if (key != ctx->gkey_seen) {
+        ctx->gkey_seen = key;
+        return fast_launches(ctx);
+    }
+    if (ctx->gexec) {
+        (void)hipGraphExecDestroy(ctx->gexec);
+        ctx->gexec = nullptr;
+        ctx->gkey = 0;
+    }
+    const int det = ctx->step_detections;
+    if (hipStreamBeginCapture(ctx->st, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        (void)hipGetLastError();
+        ctx->graph_off = true;
+        return fast_launches(ctx);
+    }
+    ctx->capturing = true;
+    const int rc = fast_launches(ctx);
+    ctx->capturing = false;
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(ctx->st, &g);
+    hipGraphExec_t x = nullptr;
+    hipError_t ei = hipErrorNotSupported;  // not instantiated
+    if (rc == ABD_OK && e == hipSuccess && g) ei = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    if (g) (void)hipGraphDestroy(g);
+    ctx->step_detections = det;
+    if (rc != ABD_OK || e != hipSuccess || ei != hipSuccess || !x) {
+        // the sequence could not be captured here: eager launches on this context from now on
+        (void)hipGetLastError();
+        if (x) (void)hipGraphExecDestroy(x);
+        ctx->graph_off = true;
+        ctx->err.clear();
+        return fast_launches(ctx);
+    }
+    ctx->gexec = x;
+    ctx->gkey = key;
+    ctx->step_captures += 1;
+    HIPCHK(hipGraphLaunch(ctx->gexec, ctx->st));
+    ctx->step_detections += 1;
+    return ABD_OK;
+}
+
+// One Newton iteration (abd.py:679-699).  fast: the persistent PCG's failure flag, the candidate
+// cache's containment test and the first line-search trial are not waited for; the iteration is
+// committed under the device flag C_SKIP and its report comes back in one round trip (the launches
+// replayed from a HIP graph when graph_usable).  Only when the flag is set does the host redo the
+// part that needs it (the solve with one launch per PCG iteration, a BVH traversal, the remaining
+// halvings, or the whole iteration after growing the assembly's buffers) and commit again.
+int newton_iteration(AbdCtx* ctx, bool fast, std::vector<int>& now) {
+    if (!ctx->cand_valid) CHK(static_broad_phase(ctx));
+    if (!fast) {
+        CHK(assemble_checked(ctx));
+        ctx->pcg_unchecked = false;
+        CHK(solve(ctx));
+        CHK(ccd_bound(ctx));
+        CHK(ls_begin(ctx));
+        CHK(ls_until_done(ctx, 1, 0));
+        CHK(commit_iteration(ctx, nullptr, now));  // applies the accepted step
+        ctx->cand_valid = true;
+        return ABD_OK;
+    }
+    CHK(graph_usable(ctx) ? fast_graph(ctx) : fast_launches(ctx));
+    CHK(commit_wait(ctx, now));
     if (ctx->h_cnt[C_SKIP]) {
         ctx->step_slow += 1;
         if (assembly_overflow(ctx)) {
@@ -1516,6 +1673,7 @@ int step_once(AbdCtx* ctx) {
     for (auto& s : ctx->stats) { s.alpha_min = 1.0; s.ccd_alpha_min = 1.0; }
     ctx->pcg_fallbacks = 0;
     ctx->step_traversals = ctx->step_detections = ctx->step_slow = 0;
+    ctx->step_graph_launches = ctx->step_captures = 0;
     hipLaunchKernelGGL(k_fill_u64, dim3(cdiv(ctx->nb, 128)), dim3(128), 0, ctx->st, ctx->nb,
                        (unsigned long long*)ctx->C.dmin_prev, (unsigned long long)0x7FF0000000000000ull);  // +inf
     CHK(fill_i32(ctx, ctx->C.k_raises, ns, 0));
@@ -1557,6 +1715,8 @@ int step_once(AbdCtx* ctx) {
             ctx->stats[s].traversals = ctx->step_traversals;
             ctx->stats[s].detections = ctx->step_detections;
             ctx->stats[s].slow_iterations = ctx->step_slow;
+            ctx->stats[s].graph_launches = ctx->step_graph_launches;
+            ctx->stats[s].graph_captures = ctx->step_captures;
             any |= now[s] != 0;
         }
         if (!any) break;
@@ -1944,7 +2104,8 @@ int abd_create(const AbdSceneDesc* d, const AbdParams* p, AbdCtx** out) {
 #undef U
     if (hipHostMalloc((void**)&ctx->h_pcg_ctl, 8 * sizeof(unsigned), hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc((void**)&ctx->h_cnt, C_N * sizeof(int), hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc((void**)&ctx->h_report, R_N * ns * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+        hipHostMalloc((void**)&ctx->h_report, R_N * ns * sizeof(double), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&ctx->h_now, ns * sizeof(int), hipHostMallocDefault) != hipSuccess) {
         ctx->err = "hipHostMalloc failed";
         return fail(ABD_ERR_HIP);
     }
@@ -1984,6 +2145,8 @@ void abd_destroy(AbdCtx* ctx) {
     if (ctx->h_cnt) (void)hipHostFree(ctx->h_cnt);
     if (ctx->h_pcg_ctl) (void)hipHostFree(ctx->h_pcg_ctl);
     if (ctx->h_report) (void)hipHostFree(ctx->h_report);
+    if (ctx->h_now) (void)hipHostFree(ctx->h_now);
+    if (ctx->gexec) (void)hipGraphExecDestroy(ctx->gexec);
     if (ctx->st) (void)hipStreamDestroy(ctx->st);
     delete ctx;
 }
@@ -2286,7 +2449,8 @@ int abd_get_contact_points(AbdCtx* ctx, double* midpoints, double* dist, int cap
     launch_cand_eval(ctx->st, ctx->G, host_cnt(nc), ctx->cand_s, ctx->X0, ctx->P, ctx->flag, nullptr);
     CHK(dbg(ctx, "launch_cand_eval"));
     CHK(excl_scan(ctx, ctx->flag, ctx->pos, host_cnt(nc)));
-    hipLaunchKernelGGL(k_scan_total, dim3(1), dim3(64), 0, ctx->st, nc, ctx->flag, ctx->pos, ctx->cnt + C_NACT);
+    hipLaunchKernelGGL(k_scan_total, dim3(1), dim3(64), 0, ctx->st, host_cnt(nc), ctx->flag, ctx->pos,
+                       ctx->cnt + C_NACT);
     CHK(sync_counters(ctx));
     int na = ctx->h_cnt[C_NACT];
     *n = na;
@@ -2485,13 +2649,40 @@ int abd_enable_timing(AbdCtx* ctx, int on) {
     timer_collect(ctx);  // records still open belong to the previous mode
     ctx->T.on = on != 0;
     ctx->T.pcg_only = on == 2;
+    if (on == 2) {
+        if (!ctx->T.pcg_ts) {
+            CHK(dalloc(ctx, &ctx->T.pcg_ts, 2));
+            CHK(dalloc(ctx, &ctx->T.pcg_acc, 4));
+            int dev = 0, khz = 0;
+            HIPCHK(hipGetDevice(&dev));
+            HIPCHK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+            ctx->T.clock_khz = khz > 0 ? (double)khz : 1.0;
+        }
+        hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, ctx->st, 1, ctx->T.pcg_ts, ~0ull);
+        hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, ctx->st, 1, ctx->T.pcg_ts + 1, 0ull);
+        hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, ctx->st, 4, (unsigned long long*)ctx->T.pcg_acc, 0ull);
+        HIPCHK(hipStreamSynchronize(ctx->st));
+    }
     ctx->T.ms.clear(); ctx->T.calls.clear(); ctx->T.bytes.clear(); ctx->T.order.clear();
     return ABD_OK;
 }
 
-int abd_get_kernel_times(const AbdCtx* ctx, char* names, int name_cap, double* ms, int* calls, double* bytes, int cap,
+int abd_get_kernel_times(const AbdCtx* cctx, char* names, int name_cap, double* ms, int* calls, double* bytes, int cap,
                          int* n) {
-    if (!ctx) return ABD_ERR_ARG;
+    if (!cctx) return ABD_ERR_ARG;
+    AbdCtx* ctx = const_cast<AbdCtx*>(cctx);
+    if (ctx->T.on && ctx->T.pcg_only && ctx->T.pcg_acc) {
+        // the persistent solves' device timestamps since timing was enabled
+        double acc[4] = {0, 0, 0, 0};
+        HIPCHK(hipStreamSynchronize(ctx->st));
+        HIPCHK(hipMemcpy(acc, ctx->T.pcg_acc, sizeof acc, hipMemcpyDeviceToHost));
+        if (acc[2] > 0.0) {
+            if (!ctx->T.calls.count("pcg_solve")) ctx->T.order.push_back("pcg_solve");
+            ctx->T.ms["pcg_solve"] = acc[0] / ctx->T.clock_khz;
+            ctx->T.calls["pcg_solve"] = (int)acc[2];
+            ctx->T.bytes["pcg_solve"] = acc[1];
+        }
+    }
     std::string all;
     int k = 0;
     for (auto& name : ctx->T.order) {

@@ -564,10 +564,10 @@ void launch_cache_contains(hipStream_t s, int nv, const double4* Xc0, const doub
 
 // flag[c] = 1 when cached pair c meets the exact candidate rule on X0 -> X1 (ground pairs: the
 // vertex comes within infl of the plane)
-__global__ void k_cache_filter(Geo G, int n, const uint64_t* __restrict__ keys, const double4* __restrict__ X0,
+__global__ void k_cache_filter(Geo G, Cnt cn, const uint64_t* __restrict__ keys, const double4* __restrict__ X0,
                                const double4* __restrict__ X1, AbdParams P, double infl, int* __restrict__ flag) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= n) return;
+    if (c >= cnt_of(cn)) return;
     const uint64_t key = keys[c];
     const int s = key_scene(key), kind = key_kind(key);
     const int il = opaque(key_il(key)), jl = opaque(key_jl(key));
@@ -593,10 +593,10 @@ __global__ void k_cache_filter(Geo G, int n, const uint64_t* __restrict__ keys, 
     flag[c] = ok ? 1 : 0;
 }
 
-void launch_cache_filter(hipStream_t s, const Geo& G, int n, const uint64_t* keys, const double4* X0,
+void launch_cache_filter(hipStream_t s, const Geo& G, Cnt n, const uint64_t* keys, const double4* X0,
                          const double4* X1, const AbdParams& P, double infl, int* flag) {
-    if (n <= 0) return;
-    hipLaunchKernelGGL(k_cache_filter, dim3(cdivb(n, 256)), dim3(256), 0, s, G, n, keys, X0, X1, P, infl, flag);
+    if (n.cap <= 0) return;
+    hipLaunchKernelGGL(k_cache_filter, dim3(cdivb(n.cap, 256)), dim3(256), 0, s, G, n, keys, X0, X1, P, infl, flag);
 }
 
 }  // namespace abd

@@ -126,6 +126,8 @@ struct PcgBufT {
     double* bnorm_out;
     double* residual_out;
     const double* bref;        // optional per-scene reference norm for the stopping test (warm start)
+    unsigned long long* ts;    // optional launch timestamps (device wall clock): [0] earliest workgroup
+                               // start, [1] latest workgroup end
 };
 using PcgBuf = PcgBufT<double>;
 using PcgBuf32 = PcgBufT<float>;
@@ -159,7 +161,7 @@ void launch_pair_queries(hipStream_t s, const Geo& G, const BodyMesh& M, int npa
                          uint64_t* cand, int cap, int* count, int* overflow, int box_only);
 void launch_cache_contains(hipStream_t s, int nv, const double4* Xc0, const double4* Xc1, const double4* X0,
                            const double4* X1, double m, int* outside);
-void launch_cache_filter(hipStream_t s, const Geo& G, int n, const uint64_t* keys, const double4* X0,
+void launch_cache_filter(hipStream_t s, const Geo& G, Cnt n, const uint64_t* keys, const double4* X0,
                          const double4* X1, const AbdParams& P, double infl, int* flag);
 void launch_ground_queries(hipStream_t s, const Geo& G, const double4* X0, const double4* X1, const AbdParams& P,
                            double infl, uint64_t* cand, int cap, int* count, int* overflow, int box_only);
@@ -228,6 +230,10 @@ hipError_t pcg_multi_max_blocks(int* out);
 template <typename T>
 void launch_pcg_mblocks(hipStream_t s, Cnt n_ent, const uint64_t* ent_key, const int* ent_val, const double* off,
                         const double* offT, const double* pinv, int nb, int n_pairs, T* mq, int* err);
+// adds one persistent solve to acc: [0] wall-clock ticks (ts[1] - ts[0]), [1] algorithmic bytes
+// K (fixed + per_pair P), [2] launches, with K = max(cg_iters[0..ns)) and P = *np; resets ts
+void launch_pcg_account(hipStream_t s, unsigned long long* ts, double* acc, const int* cg_iters, int ns,
+                        const int* np, double fixed, double per_pair);
 constexpr int PCG_TILES_PER_BLOCK = 4;  // waves per workgroup of k_pcg_persist (one tile each) and k_pcg_multi
 // mixed precision: FP64 -> FP32 copies of the PCG operands, FP32 -> FP64 of the solution
 void launch_to_f32(hipStream_t s, long n, const double* a, float* b);

@@ -994,6 +994,7 @@ __global__ __launch_bounds__(64 * PCG_WAVES) void k_pcg_persist(int ntiles, int 
     __shared__ int nbr_all[PCG_WAVES][TILE_BODIES + 1][PCG_NBR_CACHE];
     __shared__ T rb_all[PCG_WAVES][TILE_BODIES + 1][PCG_NBR_CACHE][12];
     const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    if (B.ts && threadIdx.x == 0) atomicMin(B.ts, (unsigned long long)wall_clock64());
     const int t = blockIdx.x * PCG_WAVES + wave;
     const bool tile = t < ntiles;  // waves past the last tile only take part in the barriers
     T(*vec)[12] = vec_all[wave];
@@ -1211,7 +1212,52 @@ __global__ __launch_bounds__(64 * PCG_WAVES) void k_pcg_persist(int ntiles, int 
             break;
     }
     if (on) B.x[idx] = x;
+    if (B.ts) {
+        __syncthreads();  // every wave of the workgroup has left the loop (its exits are uniform)
+        if (threadIdx.x == 0) atomicMax(B.ts + 1, (unsigned long long)wall_clock64());
+    }
 }
+
+// one persistent solve into the timing accumulator (one thread)
+__global__ void k_pcg_account(unsigned long long* ts, double* acc, const int* cg_iters, int ns, const int* np,
+                              double fixed, double per_pair) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    int K = 0;
+    for (int s = 0; s < ns; ++s) K = max(K, cg_iters[s]);
+    if (ts[1] > ts[0]) acc[0] += (double)(ts[1] - ts[0]);
+    acc[1] += (double)K * (fixed + per_pair * (double)(*np));
+    acc[2] += 1.0;
+    ts[0] = ~0ull;
+    ts[1] = 0ull;
+}
+
+void launch_pcg_account(hipStream_t s, unsigned long long* ts, double* acc, const int* cg_iters, int ns,
+                        const int* np, double fixed, double per_pair) {
+    hipLaunchKernelGGL(k_pcg_account, dim3(1), dim3(64), 0, s, ts, acc, cg_iters, ns, np, fixed, per_pair);
+}
+
+
+// A cooperative launch (all workgroups co-resident, which the grid barriers need).  Under stream
+// capture it is recorded as a kernel node with the cooperative launch attribute.
+#if !defined(ABD_EMULATION)
+static hipError_t launch_coop(const void* f, int blocks, int threads, void** args, hipStream_t s) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    hipError_t e = hipStreamIsCapturing(s, &cs);
+    if (e != hipSuccess) return e;
+    if (cs == hipStreamCaptureStatusNone) return hipLaunchCooperativeKernel(f, dim3(blocks), dim3(threads), args, 0, s);
+    hipLaunchAttribute attr = {};
+    attr.id = hipLaunchAttributeCooperative;
+    attr.val.cooperative = 1;
+    hipLaunchConfig_t cfg = {};
+    cfg.gridDim = dim3(blocks);
+    cfg.blockDim = dim3(threads);
+    cfg.dynamicSmemBytes = 0;
+    cfg.stream = s;
+    cfg.attrs = &attr;
+    cfg.numAttrs = 1;
+    return hipLaunchKernelExC(&cfg, f, args);
+}
+#endif
 
 template <typename T>
 hipError_t launch_pcg_persist(hipStream_t s, int ntiles, int ns, const int* tile_b0, const int* tile_b1,
@@ -1228,7 +1274,7 @@ hipError_t launch_pcg_persist(hipStream_t s, int ntiles, int ns, const int* tile
                        scene_tile0, B, tol, max_iter, nb, n_pairs);
     return hipSuccess;
 #else
-    return hipLaunchCooperativeKernel((const void*)k_pcg_persist<T>, dim3(blocks), dim3(64 * PCG_WAVES), args, 0, s);
+    return launch_coop((const void*)k_pcg_persist<T>, blocks, 64 * PCG_WAVES, args, s);
 #endif
 }
 // how many workgroups of k_pcg_persist<T> the device holds at once (a cooperative launch must fit)
@@ -1264,6 +1310,7 @@ __global__ __launch_bounds__(64 * PCG_WAVES) void k_pcg_multi(int ntiles, int ns
     __shared__ T vec_all[PCG_WAVES][TILE_BODIES + 1][12];
     const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
     const int gw = blockIdx.x * PCG_WAVES + wave, nw = gridDim.x * PCG_WAVES;
+    if (B.ts && threadIdx.x == 0) atomicMin(B.ts, (unsigned long long)wall_clock64());
     unsigned nbar = 0;
     for (int k = 0; k <= max_iter; ++k) {
         // slot k % 3 is next counted at stage k + 1; it was last read before the previous barrier
@@ -1273,6 +1320,10 @@ __global__ __launch_bounds__(64 * PCG_WAVES) void k_pcg_multi(int ntiles, int ns
         __builtin_amdgcn_s_waitcnt(0);
         if (!grid_barrier(B.bar, B.fail, ++nbar, gridDim.x)) break;
         if (k > 0 && aload(B.done_cnt + ((k - 1) % 3)) >= ns) break;
+    }
+    if (B.ts) {
+        __syncthreads();
+        if (threadIdx.x == 0) atomicMax(B.ts + 1, (unsigned long long)wall_clock64());
     }
 }
 
@@ -1289,7 +1340,7 @@ hipError_t launch_pcg_multi(hipStream_t s, int blocks, int ntiles, int ns, const
                        scene_tile0, B, tol, max_iter, nb, n_pairs);
     return hipSuccess;
 #else
-    return hipLaunchCooperativeKernel((const void*)k_pcg_multi<T>, dim3(blocks), dim3(64 * PCG_WAVES), args, 0, s);
+    return launch_coop((const void*)k_pcg_multi<T>, blocks, 64 * PCG_WAVES, args, s);
 #endif
 }
 

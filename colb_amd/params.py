@@ -47,7 +47,8 @@ class Params:
     debug: int = 0  # bit 0: synchronise and check after every launch; bit 1: also log them;
     #                 bit 2: print broad-phase list sizes; bit 3: one PCG launch per iteration;
     #                 bit 4: raise the persistent PCG's failure flag before its launch (fallback test);
-    #                 bit 5: the multi-tile persistent PCG even where one tile per wave fits
+    #                 bit 5: the multi-tile persistent PCG even where one tile per wave fits;
+    #                 bit 6: no HIP graph capture of the Newton iteration (eager launches)
 
     def to_c(self) -> AbdParams:
         p = AbdParams()

@@ -64,7 +64,7 @@ typedef struct AbdParams {
     int    reserved0;          /* debug bits (1: check every launch, 2: log launches, 4: print list
                                   sizes, 8: one PCG launch per iteration, 16: force the persistent
                                   PCG's fallback, 32: multi-tile persistent PCG where one tile per
-                                  wave fits) */
+                                  wave fits, 64: no HIP graph capture of the Newton iteration) */
     int    solver;             /* 0 = block-Jacobi PCG (abd.py:633-636, "cg"); 1 = direct: dense
                                   Cholesky per scene (the sparse direct branch, abd.py:637-654);
                                   2 = PCG, then the direct solve for every scene whose PCG ended
@@ -133,8 +133,11 @@ typedef struct AbdStepStats {
                                   the step were answered by the candidate cache) */
     int    detections;         /* broad-phase detections during the step */
     int    slow_iterations;    /* Newton iterations that needed a second host round trip (a line
-                                  search that halved, a path that left the candidate cache, or a
-                                  persistent PCG that gave up); the others take one */
+                                  search that halved, a path that left the candidate cache, a
+                                  persistent PCG that gave up, or lists that outgrew their
+                                  buffers); the others take one */
+    int    graph_launches;     /* Newton iterations launched as one replay of a captured HIP graph */
+    int    graph_captures;     /* HIP graph captures of the Newton iteration during the step */
 } AbdStepStats;
 
 typedef struct AbdCtx AbdCtx;
@@ -215,8 +218,11 @@ int abd_get_contact_points(AbdCtx* ctx, double* midpoints, double* dist, int cap
 int abd_get_kernel_times(const AbdCtx* ctx, char* names, int name_cap, double* ms, int* calls,
                          double* bytes, int cap, int* n);
 /* No counterpart in the reference (wp.ScopedTimer, abd.py:676, times the whole step): on = 0 off,
- * 1 every timed stage, 2 the PCG solve launches only (two events per solve, no host
- * synchronisation: light enough to leave on over a timed run).  Clears the accumulated times. */
+ * 1 every timed stage (HIP events), 2 the PCG solves only: each persistent solve stamps the device
+ * wall clock at its first workgroup's start and its last workgroup's end and a one-thread kernel adds
+ * the duration and its algorithmic bytes to a device accumulator (no events, no host
+ * synchronisation, also inside graph replays: light enough to leave on over a timed run).  Clears
+ * the accumulated times. */
 int abd_enable_timing(AbdCtx* ctx, int on);
 
 /* No counterpart in the reference (the tile autotuner of the batched configuration): assembles

@@ -473,3 +473,40 @@ def test_learned_predictor_trains_and_keeps_the_solution():
     assert runs["plain"][2] and runs["learned"][2]
     assert rel(runs["learned"][3], runs["plain"][3]) < 1e-6
     assert runs["learned"][1] > 0
+
+
+# ------------------------------------------------------------------------------------------------
+# the Newton iteration replayed from a captured HIP graph
+# ------------------------------------------------------------------------------------------------
+@pytest.mark.parametrize("case", ["chain3", "cubes_ground", "warm_chain3"])
+def test_graph_replay_matches_eager_launches(case):
+    """Every step of a run whose Newton iterations are replayed from a captured HIP graph equals,
+    bit for bit, the same run with eager launches (debug bit 6).  Replays alternate with the eager
+    work of each step (its static broad phase, the first sighting of a key, the redone parts of
+    slow iterations, the forward update), and each step is checked."""
+    if case == "chain3":
+        sc, p = S.chain_net(n=3), north_star(dhat=2e-3, k_contact=1e19, cg_tol=1e-12)
+    elif case == "cubes_ground":
+        sc, p = S.cubes_on_ground(lift=0.02), north_star(ground_enabled=True, dhat=0.05, k_contact=1e14, cg_tol=1e-12)
+    else:
+        sc, p = S.chain_net(n=3), north_star(dhat=2e-3, k_contact=1e19, cg_tol=1e-12, pcg_x0=1)
+    say("case", case)
+    g, e = solver(sc, p), solver(sc, p.replace(debug=64))
+    g.enable_timing(2)  # the PCG solves time themselves on the device, inside the replays too
+    launches = captures = iters = 0
+    for k in range(12):
+        sg, se = g.step()[0], e.step()[0]
+        launches += sg["graph_launches"]
+        captures += sg["graph_captures"]
+        iters += sg["newton_iters"]
+        assert se["graph_launches"] == 0 and se["graph_captures"] == 0
+        assert np.array_equal(g.get_state()[0], e.get_state()[0]), f"step {k}"
+        assert sg["newton_iters"] == se["newton_iters"] and sg["cg_iters"] == se["cg_iters"]
+        assert sg["residual_last"] == se["residual_last"]
+    t = g.kernel_times()["pcg_solve"]
+    say(f"  {launches} graph replays, {captures} captures over 12 steps; {t['calls']} timed solves, "
+        f"{t['ms'] / max(t['calls'], 1) * 1e3:.1f} us each")
+    assert launches > 0 and captures >= 1
+    assert t["calls"] >= iters and t["ms"] > 0 and t["bytes"] > 0
+    g.close()
+    e.close()
