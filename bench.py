@@ -285,6 +285,9 @@ def main():
     ap.add_argument("--train-iters", type=int, default=3000, help="Adam steps of the learned model")
     ap.add_argument("--pcg-layout", default=None, metavar="BPT[,multi]",
                     help="fix the PCG layout (bodies per tile, 1 = one launch per iteration) instead of autotuning")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="eager launches instead of replaying each Newton iteration from a captured HIP graph "
+                         "(the same kernels; for kernel tracers that do not follow graph launches)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run one 512 MiB device-to-device copy (reference kernel for --pmc byte counters)")
     args = ap.parse_args()
@@ -305,6 +308,9 @@ def main():
     sc, p, cfg = workload(args.workload, args.precision)
     p = p.replace(solver={"pcg": 0, "direct": 1, "fallback": 2}[args.solver])
     cfg["solver"] = args.solver
+    if args.no_graph:
+        p = p.replace(debug=p.debug | 64)
+    cfg["graph"] = not args.no_graph
     print(f"# workload {cfg['workload']}: {sc.n_bodies} bodies, {sc.n_verts} verts, {sc.n_tris} tris, "
           f"{sc.n_edges} edges", file=sys.stderr, flush=True)
     results, learned_info, mid_state, roof, pcg_us, first_solver = {}, None, None, None, None, None

@@ -712,7 +712,15 @@ __device__ __forceinline__ double scene_partial_p(const double* part, int ntiles
                                                   int lane) {
     const double* p = part + (size_t)(qty * 2 + par) * ntiles;
     double acc = 0.0;
-    for (int t = t0 + lane; t < t1; t += 64) acc += ldp<A>(p + t);
+    // four rounds of 64 tiles per batch, every load of a batch in flight at once, summed in tile order
+    for (int tb = t0 + lane; tb < t1; tb += 256) {
+        double v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = ldp<A>(p + min(tb + 64 * u, t1 - 1));
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (tb + 64 * u < t1) acc += v[u];
+    }
     return wave_sum(acc);
 }
 
@@ -1077,12 +1085,26 @@ __global__ __launch_bounds__(64 * PCG_WAVES) void k_pcg_persist(int ntiles, int 
         if (!done && mx > 0) fetch(0, k, rd);
         if (k > 0 && tile) {
             const int par = (k - 1) & 1;
-            // the three partial sums of the scene in one pass over its tiles
+            // the three partial sums of the scene in one pass over its tiles, four rounds of 64 tiles
+            // per batch with every load of the batch in flight at once (one round trip per 256 tiles
+            // instead of one per 64), summed in tile order as a one-round-at-a-time loop would
             double a_rho = 0.0, a_g = 0.0, a_d = 0.0;
-            for (int tt = t0 + lane; tt < t1; tt += 64) {
-                a_d += aload(B.part + (size_t)(0 * 2 + par) * ntiles + tt);
-                a_g += aload(B.part + (size_t)(1 * 2 + par) * ntiles + tt);
-                a_rho += aload(B.part + (size_t)(2 * 2 + par) * ntiles + tt);
+            for (int tb = t0 + lane; tb < t1; tb += 256) {
+                double pd[4], pg[4], pr[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int tt = min(tb + 64 * u, t1 - 1);  // past the scene: a repeat of its last tile, dropped
+                    pd[u] = aload(B.part + (size_t)(0 * 2 + par) * ntiles + tt);
+                    pg[u] = aload(B.part + (size_t)(1 * 2 + par) * ntiles + tt);
+                    pr[u] = aload(B.part + (size_t)(2 * 2 + par) * ntiles + tt);
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    if (tb + 64 * u < t1) {
+                        a_d += pd[u];
+                        a_g += pg[u];
+                        a_rho += pr[u];
+                    }
             }
             const double rho = wave_sum(a_rho), g = wave_sum(a_g), d = wave_sum(a_d);
             if (k == 1) {
