@@ -3,11 +3,13 @@ MI355X, on the 32x32 torus chain net (1024 links of 288 verts / 576 tris, four c
 FP64, dt 8 ms, dhat 1e-3, kappa 1e7, 16 Newton iterations, tolerance 1e-5).
 
 Each method (plain: the reference's Newton step; warm: every PCG after a step's first Newton
-iteration starts from the part of the previous direction its line search did not take; learned:
-warm plus the first iterate of every step moved toward a learned per-body prediction) runs the
-workload's steps from its initial state after a few untimed warmup steps; `value` is the first
-method's rate, `methods` holds every method's rate, Newton iterations, converged fraction and worst
-final residual.  The CPU baseline is the project's FP64
+iteration starts from the part of the previous direction its line search did not take; stall: a
+scene's Newton loop ends after a step whose line search accepted alpha < 1e-8, a step pinned by
+CCD, reported as stalled; learned: warm plus the first iterate of every step moved toward a learned
+per-body prediction) runs the workload's steps from its initial state after a few untimed warmup
+steps; `value` is the first method's rate, `methods` holds every method's rate, Newton iterations,
+converged and stalled fractions, worst final residual and, for every method after the first, how
+far its final state lies from the first's.  The CPU baseline is the project's FP64
 oracle (oracle/, "port") timed on this host on a bounded sample of the same workload (whole steps
 from the GPU's state at the middle of the timed run), single-threaded and with OpenMP.
 """
@@ -130,6 +132,9 @@ METHODS = {
     "plain": ({}, "the reference's Newton step: first iterate q_now (abd.py:674-699)"),
     "extrapolate": ({"predictor": 1}, "first iterate moved toward q_now + dt qdot (CCD-safe, energy-decreasing)"),
     "warm": ({"pcg_x0": 1}, "PCG from the untaken part (1 - alpha) du of the previous Newton direction"),
+    "stall": ({"stall_alpha": 1e-8},
+              "the reference's step, a scene stopped once its accepted line-search step falls below 1e-8 (CCD-pinned)"),
+    "warm_stall": ({"pcg_x0": 1, "stall_alpha": 1e-8}, "warm and stall together"),
     "learned": ({"predictor": 3, "pcg_x0": 1},
                 "warm, and the first iterate moved toward the learned per-body prediction of the step"),
 }
@@ -207,18 +212,21 @@ def run_method(name, sc, p, args, learned_model=None, timing=False):
         sts = solver.step()
         recs.append((max(x["newton_iters"] for x in sts), all(x["converged"] for x in sts),
                      max(x["residual_last"] for x in sts), max(x["cg_iters"] for x in sts),
-                     sts[0]["traversals"], sts[0]["detections"], sum(x["pcg_fallbacks"] for x in sts)))
+                     sts[0]["traversals"], sts[0]["detections"], sum(x["pcg_fallbacks"] for x in sts),
+                     np.mean([x["stalled"] for x in sts])))
         if k == args.steps // 2 - 1 and timing:
             tp = time.perf_counter()
             mid = solver.get_state()  # the CPU baseline's start state, copied outside the timed work
             paused += time.perf_counter() - tp
     t1 = time.perf_counter()
     wall_ms = (t1 - t0 - paused) * 1e3
+    final_q = solver.get_state()[0]  # after the clock stops: the trajectory comparison across methods
     newton = np.array([r[0] for r in recs])
     conv = np.array([r[1] for r in recs])
     out = {"description": desc, "sim_ms_per_wall_ms": round(args.steps * pm.dt * 1e3 / wall_ms, 6),
            "ms_per_step": round(wall_ms / args.steps, 3), "newton_iters_per_step": round(float(newton.mean()), 3),
            "converged_fraction": round(float(conv.mean()), 4),
+           "stalled_fraction": round(float(np.mean([r[7] for r in recs])), 4),
            "worst_final_residual": float(max(r[2] for r in recs)),
            "cg_iters_per_step": round(float(np.mean([r[3] for r in recs])), 1), "steps": args.steps,
            "traversals_per_step": round(float(np.mean([r[4] for r in recs])), 3),
@@ -228,7 +236,7 @@ def run_method(name, sc, p, args, learned_model=None, timing=False):
         out["pcg_times"] = solver.kernel_times()  # HIP events of the timed steps' PCG launches
         solver.enable_timing(0)
     print(f"# {name}: {out}", file=sys.stderr, flush=True)
-    return solver, out, mid
+    return solver, out, mid, final_q
 
 
 def kernel_roofline(solver, out, cfg, precision, steps):
@@ -275,8 +283,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=5, help="untimed steps that size the buffers before the run")
     ap.add_argument("--workload", default="chain_net_32x32", choices=["chain_net_32x32", "box_pile_4096",
                                                                    "hull_piles_256", "ref_chain_144", "cubes_8"])
-    ap.add_argument("--methods", default="plain,warm,learned",
-                    help="comma-separated: plain (the reference's step), extrapolate, warm, learned (trained first)")
+    ap.add_argument("--methods", default="plain,warm,stall,learned",
+                    help="comma-separated: plain (the reference's step), extrapolate, warm, stall, warm_stall, "
+                         "learned (trained first)")
     ap.add_argument("--solver", default="pcg", choices=["pcg", "direct", "fallback"],
                     help="linear solve: block-Jacobi PCG, dense Cholesky per scene, or PCG with the direct "
                          "solve for scenes whose PCG ends at its iteration cap")
@@ -329,8 +338,13 @@ def main():
         if m == "learned":
             model, learned_info = train_learned(sc, p, args)
             print(f"# learned: {learned_info}", file=sys.stderr, flush=True)
-        solver, out, mid = run_method(m, sc, p, args, model, timing=(m == methods[0]))
+        solver, out, mid, final_q = run_method(m, sc, p, args, model, timing=(m == methods[0]))
         results[m] = out
+        if m == methods[0]:
+            first_q = final_q
+        else:
+            # the same steps from the same initial state: how far this method's trajectory ends from the first's
+            out[f"final_q_rel_diff_vs_{methods[0]}"] = float(np.abs(final_q - first_q).max() / np.abs(first_q).max())
         if m == methods[0]:
             mid_state = mid
             roof, pcg_us = kernel_roofline(solver, out, cfg, args.precision, args.profile_steps)

@@ -47,6 +47,7 @@ class AbdParams(C.Structure):
         ("cand_margin", C.c_double),
         ("pcg_x0", C.c_int),
         ("reserved1", C.c_int),
+        ("stall_alpha", C.c_double),
     ]
 
 
@@ -89,6 +90,7 @@ class AbdStepStats(C.Structure):
         ("slow_iterations", C.c_int),
         ("graph_launches", C.c_int),
         ("graph_captures", C.c_int),
+        ("stalled", C.c_int),
     ]
 
     def as_dict(self):

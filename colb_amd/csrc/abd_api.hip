@@ -926,6 +926,7 @@ int check_params(AbdCtx* ctx, const AbdParams* p) {
     if (p->hess_proj != 0 && p->hess_proj != 1) return bad("hess_proj must be 0 or 1");
     if (p->pcg_x0 != 0 && p->pcg_x0 != 1) return bad("pcg_x0 must be 0 or 1");
     if (!finite(p->cand_margin)) return bad("cand_margin must be finite");
+    if (!(p->stall_alpha >= 0.0) || !(p->stall_alpha < 1.0)) return bad("stall_alpha must be in [0, 1)");
     if (!finite(p->k_max)) return bad("k_max must be finite");
     if (p->k_max > p->k_contact && (!(p->k_raise_gap > 0.0) || !(p->k_raise_gap < 1.0)))
         return bad("k_raise_gap must be in (0, 1)");
@@ -1706,7 +1707,10 @@ int step_once(AbdCtx* ctx) {
                 S.alpha_min = std::min(S.alpha_min, R[R_ALPHA]);
                 S.ccd_alpha_min = std::min(S.ccd_alpha_min, R[R_ALPHA0]);
                 S.energy = R[R_E1];
-                if (now[s] == 0) S.converged = 1;
+                if (now[s] == 0) {
+                    if (R[R_RES] < ctx->P.newton_tol) S.converged = 1;
+                    else S.stalled = 1;
+                }
             }
             ctx->stats[s].k_contact = R[R_KC];
             ctx->stats[s].k_raises = (int)R[R_RAISES];

@@ -1911,7 +1911,9 @@ __global__ void k_newton_check(int ns, AbdParams P, SceneCtl c, int* active_befo
     if (s >= ns || (skip && *skip)) return;
     const int a = c.newton_active[s];
     active_before[s] = a;
-    if (a && c.residual[s] < P.newton_tol) c.newton_active[s] = 0;
+    // converged (abd.py:699), or, with stall_alpha > 0, the step just taken was pinned below it
+    if (a && (c.residual[s] < P.newton_tol || (P.stall_alpha > 0.0 && c.alpha[s] < P.stall_alpha)))
+        c.newton_active[s] = 0;
 }
 
 void launch_newton_check(hipStream_t s, int ns, const AbdParams& P, SceneCtl c, int* active_before,

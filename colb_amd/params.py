@@ -44,6 +44,7 @@ class Params:
     k_raise_gap: float = 0.5  # raise the stiffness when a contact gets closer than this * dhat
     cand_margin: float = None  # candidate-cache margin (length); None: 4 dhat; <= 0: no cache
     pcg_x0: int = 0  # PCG initial guess: 0 = zero (the reference), 1 = the untaken part of the previous direction
+    stall_alpha: float = 0.0  # > 0: a scene stops iterating after an accepted step alpha below it (0: the reference)
     debug: int = 0  # bit 0: synchronise and check after every launch; bit 1: also log them;
     #                 bit 2: print broad-phase list sizes; bit 3: one PCG launch per iteration;
     #                 bit 4: raise the persistent PCG's failure flag before its launch (fallback test);
@@ -81,6 +82,7 @@ class Params:
         p.k_raise_gap = self.k_raise_gap
         p.cand_margin = 4.0 * self.dhat if self.cand_margin is None else self.cand_margin
         p.pcg_x0 = self.pcg_x0
+        p.stall_alpha = self.stall_alpha
         return p
 
     def replace(self, **kw):

@@ -97,6 +97,11 @@ typedef struct AbdParams {
                                   its residual is below ||grad|| (the stopping test stays relative
                                   to ||grad||).  A learned warm start (abd_set_warm_start) replaces it */
     int    reserved1;
+    double stall_alpha;        /* no counterpart in the reference (its loop runs to max_newton,
+                                  abd.py:699): > 0 stops a scene's Newton loop after an iteration whose
+                                  accepted line-search step alpha is below it (the CCD bound pinning
+                                  the iterate); the step is reported stalled, not converged.
+                                  0 = off (the reference) */
 } AbdParams;
 
 typedef struct AbdSceneDesc {
@@ -138,6 +143,7 @@ typedef struct AbdStepStats {
                                   buffers); the others take one */
     int    graph_launches;     /* Newton iterations launched as one replay of a captured HIP graph */
     int    graph_captures;     /* HIP graph captures of the Newton iteration during the step */
+    int    stalled;            /* the Newton loop stopped at an accepted step below stall_alpha */
 } AbdStepStats;
 
 typedef struct AbdCtx AbdCtx;

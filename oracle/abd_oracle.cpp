@@ -1222,6 +1222,7 @@ void step_once(Ctx& C) {
             st.alpha_min = std::min(st.alpha_min, alpha[s]);
             st.ccd_alpha_min = std::min(st.ccd_alpha_min, amax[s]);
             if (res[s] < C.P.newton_tol) { st.converged = 1; active[s] = 0; }
+            else if (C.P.stall_alpha > 0.0 && alpha[s] < C.P.stall_alpha) { st.stalled = 1; active[s] = 0; }
         }
     }
     for (int s = 0; s < C.ns; ++s) {

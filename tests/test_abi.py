@@ -69,7 +69,7 @@ INVALID = [("dt", 0.0), ("dt", -1e-3), ("dhat", 0.0), ("kappa", -1.0), ("k_conta
            ("ccd_gap_ratio", 1.0), ("ccd_max_iter", 0), ("ls_floor", 0.0), ("ls_floor", 1.0), ("pinned_diag", 0.0),
            ("dt", float("nan")), ("gravity", (0.0, float("inf"), 0.0)), ("predictor", 4), ("hess_proj", 2),
            ("solver", 3), ("precision", 2), ("direct_max_dofs", 1 << 20),
-           ("k_raise_gap", 1.0), ("pcg_x0", 2)]
+           ("k_raise_gap", 1.0), ("pcg_x0", 2), ("stall_alpha", -1e-3), ("stall_alpha", 1.0)]
 
 
 @pytest.mark.parametrize("field,value", INVALID)

@@ -171,6 +171,30 @@ def test_option_trajectories_match_oracle(O, case):
         assert raises > 0
 
 
+def test_stall_trajectory_matches_oracle(O):
+    """stall_alpha stops a scene after a CCD-pinned step: the GPU stops on the same iterations as
+    the oracle (pinned steps have alpha ~1e-11, far from the 1e-6 threshold) and ends in the same
+    state, on one scene and batched with a 3 x 3 net.  The first stall comes at step 15; the run
+    stops five steps later: a pinned contact (d ~1e-16) turns rounding differences of 1e-10 into
+    2e-7 at the stall, and some steps further on into a different CCD bound, a trait of the
+    scene that the plain method shares."""
+    p = north_star(dhat=1e-3, k_contact=1e20, stall_alpha=1e-6)
+    for ns in (1, 2):
+        sc = S.chain_net(n=4) if ns == 1 else S.concat_scenes([S.chain_net(n=4), S.chain_net(n=3)])
+        orc, gpu = O.Oracle(sc, p), solver(sc, p)
+        stalls = 0
+        for k in range(20):
+            so, sg = orc.step(), gpu.step()
+            err = rel(gpu.get_state()[0], orc.get_state()[0])
+            say(f"  scenes {ns} step {k} newton {[x['newton_iters'] for x in so]}/{[x['newton_iters'] for x in sg]} "
+                f"stalled {[x['stalled'] for x in so]}/{[x['stalled'] for x in sg]} err {err:.2e}")
+            for a, b in zip(so, sg):
+                assert (b["newton_iters"], b["stalled"], b["converged"]) == (a["newton_iters"], a["stalled"], a["converged"])
+                stalls += b["stalled"]
+            assert err < 1e-6
+        assert stalls > 0
+
+
 def test_stiffness_round_trip_and_reset():
     sc = cube_on_ground(0.5 * DHAT, n=2)
     p = ground_params()

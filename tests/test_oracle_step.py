@@ -7,6 +7,9 @@ contact stiffness:
   1e-2 alpha0), and at 0 when the floor trial's energy is infinite (through the ground plane);
 * predictor: on a free-falling body the predicted first iterate changes nothing but the iteration
   count (the incremental potential has one minimiser near q~), and it starts closer to it;
+* stall test (an option with no counterpart in the reference): a scene whose accepted step falls
+  below stall_alpha stops iterating, reported as stalled, and the state it ends the step in is the
+  state of the reference's loop cut at that iteration (max_newton = the iterations taken);
 * adaptive stiffness: a heavy cube on the ground whose contacts cannot carry its weight at the
   initial stiffness (the cubic barrier's force 6 k d (dhat^2 - d^2)^2 peaks at 1.717 k dhat^5) ends
   with a stiffness whose capacity holds the weight, with every contact outside k_raise_gap * dhat,
@@ -134,3 +137,29 @@ def test_set_params_resets_the_stiffness(oracle_mod):
     np.testing.assert_array_equal(orc.stiffness(), 3.0 * p.k_contact)
     orc.set_params(p)
     np.testing.assert_array_equal(orc.stiffness(), p.k_contact)
+
+
+def test_stall_equals_the_loop_cut_at_that_iteration(oracle_mod):
+    """A 4 x 4 chain net at d^ 1e-3, k 1e20 has CCD-pinned steps (alpha ~1e-11): with stall_alpha
+    1e-6 a step that takes one ends there.  Rerunning that step from the same state with the
+    reference's loop and max_newton = its iteration count gives the same state bit for bit."""
+    sc = S.chain_net(n=4)
+    p = north_star(dhat=1e-3, k_contact=1e20)
+    orc = oracle_mod.Oracle(sc, p.replace(stall_alpha=1e-6))
+    stalls = 0
+    for _ in range(30):
+        q, qn, qd = orc.get_state()
+        st = orc.step()[0]
+        if not st["stalled"]:
+            assert st["converged"] or st["newton_iters"] == p.max_newton
+            continue
+        stalls += 1
+        assert not st["converged"] and st["newton_iters"] < p.max_newton
+        assert st["alpha_min"] < 1e-6 and st["residual_last"] >= p.newton_tol
+        cut = oracle_mod.Oracle(sc, p.replace(max_newton=st["newton_iters"]))
+        cut.set_state(q, qn, qd)
+        sc_ = cut.step()[0]
+        assert sc_["newton_iters"] == st["newton_iters"] and not sc_["stalled"]
+        for a, b in zip(cut.get_state(), orc.get_state()):
+            np.testing.assert_array_equal(a, b)
+    assert stalls > 0
