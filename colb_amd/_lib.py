@@ -38,6 +38,7 @@ SIGNATURES = {
     "abd_get_contact_points": (C.c_int, [_V, _D, _D, C.c_int, _I]),
     "abd_get_kernel_times": (C.c_int, [_V, C.c_char_p, C.c_int, _D, _I, _D, C.c_int, _I]),
     "abd_enable_timing": (C.c_int, [_V, C.c_int]),
+    "abd_psd_project_test": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, _D, _D]),
     "abd_autotune": (C.c_int, [_V, C.c_int, _I, _I, _D]),
     "abd_set_pcg_layout": (C.c_int, [_V, C.c_int, C.c_int]),
     "abd_set_warm_start": (C.c_int, [_V, _V, _F, _F, _F, _F]),

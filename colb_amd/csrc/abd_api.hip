@@ -142,6 +142,7 @@ struct AbdCtx {
     double *off = nullptr, *offT = nullptr, *mq = nullptr;  // mq: 144 per CSR entry (persistent PCG)
     uint64_t *ent_k = nullptr, *ent_ks = nullptr;
     int *ent_v = nullptr, *ent_vs = nullptr, *row_ptr = nullptr;
+    int* inc_ptr = nullptr;  // each body's range of the sorted incidence keys (nb + 1)
     // PCG
     double *r = nullptr, *z = nullptr, *p0 = nullptr, *p1 = nullptr, *Ap = nullptr, *part = nullptr, *bnorm = nullptr;
     double *r1 = nullptr, *w1 = nullptr, *s0 = nullptr, *pcg_sc = nullptr;
@@ -786,6 +787,8 @@ int assemble(AbdCtx* ctx) {
     CHK(sort_keys(ctx, ctx->inc, ctx->inc_s, inc_cnt(ctx), (bbits << 32) | (cbits << 1) | 1ull));
     CHK(sort_keys(ctx, ctx->pk, ctx->pk_s, cpk, (bbits << 44) | (bbits << 24) | cbits));
     toc(ctx);
+    launch_row_ptr(ctx->st, ctx->nb, inc_cnt(ctx), ctx->inc_s, ctx->inc_ptr);
+    CHK(dbg(ctx, "launch_row_ptr"));
     launch_run_heads(ctx->st, cpk, ctx->pk_s, 24, ctx->head);
     CHK(dbg(ctx, "launch_run_heads"));
     CHK(excl_scan(ctx, ctx->head, ctx->hpos, cpk));
@@ -804,7 +807,7 @@ int assemble(AbdCtx* ctx) {
     launch_row_ptr(ctx->st, ctx->nb, ent_cnt(ctx), ctx->ent_ks, ctx->row_ptr);
     CHK(dbg(ctx, "launch_row_ptr"));
     tic(ctx, "body_assemble", (double)ctx->nb * (3 * 96 + 128 + 3 * 1152 + 96 + 8));
-    launch_body_assemble(ctx->st, ctx->G, ctx->P, ctx->q, ctx->qtilde, inc_cnt(ctx), ctx->inc_s, ctx->act_body,
+    launch_body_assemble(ctx->st, ctx->G, ctx->P, ctx->q, ctx->qtilde, ctx->inc_ptr, ctx->inc_s, ctx->act_body,
                          ctx->act_g, ctx->act_H, ctx->C.newton_active, ctx->grad, ctx->diag, ctx->pinv, ctx->e_body,
                          ca);
     CHK(dbg(ctx, "launch_body_assemble"));
@@ -2103,7 +2106,7 @@ int abd_create(const AbdSceneDesc* d, const AbdParams* p, AbdCtx** out) {
     A(C.newton_active, ns); A(C.residual, ns); A(C.alpha, ns); A(C.alpha0, ns); A(C.amax_bits, ns);
     A(C.ls_done, ns); A(C.ls_floor, ns); A(C.E0, ns); A(C.E1, ns);
     A(C.kb, nb); A(C.dmin_prev, nb); A(C.dmin_bits, nb); A(C.k_raises, ns); A(C.kc, ns); A(C.pred_alpha, ns);
-    A(nact_s, ns); A(act_before, ns); A(report, R_N * ns); A(cnt, C_N); A(row_ptr, nb + 1);
+    A(nact_s, ns); A(act_before, ns); A(report, R_N * ns); A(cnt, C_N); A(row_ptr, nb + 1); A(inc_ptr, nb + 1);
 #undef A
 #undef U
     if (hipHostMalloc((void**)&ctx->h_pcg_ctl, 8 * sizeof(unsigned), hipHostMallocDefault) != hipSuccess ||
@@ -2705,6 +2708,25 @@ int abd_get_kernel_times(const AbdCtx* cctx, char* names, int name_cap, double* 
         names[name_cap - 1] = 0;
     }
     return k > cap ? ABD_ERR_CAPACITY : ABD_OK;
+}
+
+int abd_psd_project_test(int m, int precision, int absolute, int n, const double* A, double* out) {
+    if (!A || !out || n < 1 || (m != 3 && m != 6 && m != 9 && m != 12) || (precision != 0 && precision != 1))
+        return ABD_ERR_ARG;
+    const size_t bytes = (size_t)n * m * m * sizeof(double);
+    double *dA = nullptr, *dO = nullptr;
+    hipError_t e = hipMalloc(&dA, bytes);
+    if (e == hipSuccess) e = hipMalloc(&dO, bytes);
+    if (e == hipSuccess) e = hipMemcpy(dA, A, bytes, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        launch_psd_test(nullptr, m, precision, absolute, n, dA, dO);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
+    if (e == hipSuccess) e = hipMemcpy(out, dO, bytes, hipMemcpyDeviceToHost);
+    (void)hipFree(dA);
+    (void)hipFree(dO);
+    return e == hipSuccess ? ABD_OK : ABD_ERR_HIP;
 }
 
 }  // extern "C"

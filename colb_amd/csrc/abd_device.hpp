@@ -335,91 +335,122 @@ __device__ inline bool wave_is_psd(const double* A, double* L, int m, int lane) 
     return true;
 }
 
-// PSD projection of the symmetric m x m matrix A (LDS, row-major, m <= 12) by one wave, through
-// the matrix sign function: A+ = (A + A S) / 2 with S = sign(A) (A S = |A|), the projection onto
-// the PSD cone that clamping the eigenvalues at zero gives (a unique matrix function, so it agrees
-// with the oracle's Jacobi-based psd_project to rounding).  S comes from Newton-Schulz iterations
-// on X0 = A / ||A||_F (eigenvalues in [-1, 1]; Higham, Functions of Matrices, 5.3): 24 quintic
-// steps X <- X (a + b X^2 + c X^4), (a, b, c) = (3.4445, -4.7750, 2.0315) (the quintic of Jordan et
-// al.'s Muon optimiser), which multiply small eigenvalues by 3.44 per step and keep the others in
-// [0.68, 1.21], then 6 cubic steps X <- X (3 - X^2) / 2, which converge quadratically to +-1.  An
-// eigenvalue below ~1e-14 ||A||_F may end with a partly converged sign; its contribution to A+ is
-// bounded by its own size, so the result is accurate to ~1e-14 of ||A||.  Every product is of two
-// polynomials in A, i.e. symmetric: each lane forms upper-triangle entries (lane, lane + 64) and
-// mirrors them.  V and the two static buffers are the iteration's LDS scratch.  absolute: A <- |A|
-// (eigenvalues mirrored rather than clamped).
-template <int M, typename HT>
-__device__ inline void sym_product(const HT* X, const HT* Y, HT* Z, const int* ui, const int* uj) {
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-        const int i = ui[h], j = uj[h];
-        if (i < 0) continue;
-        HT z = 0;
-#pragma unroll
-        for (int k = 0; k < M; ++k) z = fma(X[i * M + k], Y[j * M + k], z);  // Y symmetric: Y[k][j] = Y[j][k]
-        Z[i * M + j] = z;
-        Z[j * M + i] = z;
+// ---- 16 x 16 symmetric matrices on the matrix cores ---------------------------------------
+// One wave holds a symmetric matrix of order <= 16 (zero-padded) in the output layout of
+// v_mfma_{f64,f32}_16x16x4: lane l, slot u holds row R(l / 16, u), column l % 16, with
+// R(g, u) = g + 4 u for the FP64 instruction and 4 g + u for the FP32 one (both checked on the
+// hardware by the projection's test).  The instruction reads A(i, k) from lane i + 16 k and B(k, j)
+// from lane j + 16 k.  The product of two such matrices takes four instructions, the kb-th summing
+// over k = R(l / 16, kb): slot kb of lane l is X(R(l / 16, kb), l % 16) = X(l % 16, R(l / 16, kb)) by
+// symmetry, the instruction's A(l % 16, l / 16), and Y(R(l / 16, kb), l % 16) is its B(l / 16,
+// l % 16); the operands come straight from the registers, with no exchange through LDS.  (X Y is
+// formed as X^T Y, the same thing for a symmetric X.)
+template <typename HT>
+struct Sym16;
+template <>
+struct Sym16<double> {
+    typedef double V4 __attribute__((vector_size(4 * sizeof(double))));
+    static __device__ __forceinline__ V4 step(double a, double b, V4 c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
     }
-    __syncthreads();
-}
-
-// HT = float (the mixed-precision assembly): 11 quintic and 5 cubic steps, since eigenvalues below
-// ~1e-6 ||A|| are at the level of the FP32 rounding (more quintic steps would amplify the rounding
-// noise of the null space past the iteration's basin); the result is accurate to ~1e-6 of ||A||.
-template <int M, typename HT>
-__device__ inline void wave_psd_project_m(HT* A, HT* V, int lane, bool absolute, HT* T1, HT* T2) {
-    constexpr int m = M;
-    constexpr int mm = m * m;
-    constexpr int nup = m * (m + 1) / 2;  // <= 78
-    constexpr int n_quintic = sizeof(HT) == 8 ? 24 : 11, n_cubic = sizeof(HT) == 8 ? 6 : 5;
-    double f = 0;
-    for (int e = lane; e < mm; e += 64) f += (double)A[e] * (double)A[e];
-    const double fro = wave_sum(f);
-    if (fro == 0.0 || !(fro < INFINITY)) return;
-    int ui[2] = {-1, -1}, uj[2] = {0, 0};
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-        const int t = lane + 64 * h;
-        if (t < nup) {
-            int i = 0, rem = t;
-            while (rem >= m - i) { rem -= m - i; ++i; }
-            ui[h] = i;
-            uj[h] = i + rem;
-        }
+    static __device__ __forceinline__ int row(int g, int u) { return g + 4 * u; }
+};
+template <>
+struct Sym16<float> {
+    typedef float V4 __attribute__((vector_size(4 * sizeof(float))));
+    static __device__ __forceinline__ V4 step(float a, float b, V4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
     }
-    const double inv = rsq_nr(fro);
-    HT* X = V;
-    for (int e = lane; e < mm; e += 64) X[e] = (HT)((double)A[e] * inv);
-    __syncthreads();
-    for (int it = 0; it < n_quintic + n_cubic; ++it) {
-        const bool quintic = it < n_quintic;
-        sym_product<M>(X, X, T1, ui, uj);  // X^2
-        if (quintic) {
-            sym_product<M>(T1, T1, T2, ui, uj);  // X^4
-            for (int e = lane; e < mm; e += 64)
-                T2[e] = HT(2.0315) * T2[e] - HT(4.7750) * T1[e] + ((e / m == e % m) ? HT(3.4445) : HT(0));
-        } else {
-            for (int e = lane; e < mm; e += 64) T2[e] = ((e / m == e % m) ? HT(1.5) : HT(0)) - HT(0.5) * T1[e];
-        }
-        __syncthreads();
-        sym_product<M>(X, T2, T1, ui, uj);  // X p(X^2)
-        HT* t = X; X = T1; T1 = t;
-    }
-    sym_product<M>(A, X, T1, ui, uj);  // |A| = A S
-    for (int e = lane; e < mm; e += 64) A[e] = absolute ? T1[e] : HT(0.5) * (A[e] + T1[e]);
-    __syncthreads();
-}
+    static __device__ __forceinline__ int row(int g, int u) { return 4 * g + u; }
+};
 
 template <typename HT>
-__device__ inline void wave_psd_project(HT* A, HT* V, int m, int lane, bool absolute = false) {
-    __shared__ HT T1[144], T2[144];
-    switch (m) {
-        case 3: wave_psd_project_m<3>(A, V, lane, absolute, T1, T2); break;
-        case 6: wave_psd_project_m<6>(A, V, lane, absolute, T1, T2); break;
-        case 9: wave_psd_project_m<9>(A, V, lane, absolute, T1, T2); break;
-        case 12: wave_psd_project_m<12>(A, V, lane, absolute, T1, T2); break;
-        default: break;  // callers pass 3, 6, 9 or 12
+__device__ __forceinline__ typename Sym16<HT>::V4 sym16_mul(typename Sym16<HT>::V4 X, typename Sym16<HT>::V4 Y) {
+    typename Sym16<HT>::V4 acc = {0, 0, 0, 0};
+#pragma unroll
+    for (int kb = 0; kb < 4; ++kb) acc = Sym16<HT>::step(X[kb], Y[kb], acc);
+    return acc;
+}
+
+// (X + X^T) / 2 of a Sym16 matrix, through the 16 x 17 LDS scratch tr: exactly symmetric (each pair
+// of mirrored entries is the same sum).  A product of two commuting symmetric matrices is
+// symmetric only up to rounding, and the sign iteration amplifies an asymmetric error along
+// eigenvectors of small eigenvalues as it amplifies the eigenvalues themselves.
+template <typename HT>
+__device__ __forceinline__ typename Sym16<HT>::V4 sym16_symmetrize(typename Sym16<HT>::V4 X, HT* tr, int lane) {
+    const int col = lane % 16, g = lane / 16;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) tr[Sym16<HT>::row(g, u) * 17 + col] = X[u];
+    __syncthreads();
+    typename Sym16<HT>::V4 Y;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) Y[u] = HT(0.5) * (X[u] + tr[col * 17 + Sym16<HT>::row(g, u)]);
+    __syncthreads();
+    return Y;
+}
+
+// PSD projection of the symmetric m x m matrix A (LDS, row-major, m <= 12) by one wave (the whole
+// wave, uniformly), through the matrix sign function: A+ = (A + A S) / 2 with S = sign(A)
+// (A S = |A|), the projection onto the PSD cone that clamping the eigenvalues at zero gives (a
+// unique matrix function, so it agrees with the oracle's Jacobi-based psd_project to rounding).
+// S comes from Newton-Schulz iterations on X0 = A / ||A||_F (eigenvalues in [-1, 1]; Higham,
+// Functions of Matrices, 5.3): quintic steps X <- X (a + b X^2 + c X^4), (a, b, c) = (3.4445,
+// -4.7750, 2.0315) (the quintic of Jordan et al.'s Muon optimiser), which multiply small
+// eigenvalues by 3.44 per step and keep the others in [0.68, 1.21], then cubic steps
+// X <- X (3 - X^2) / 2, which converge quadratically to +-1.  FP64: 24 quintic and 6 cubic steps;
+// an eigenvalue below ~1e-14 ||A||_F may end with a partly converged sign, and its contribution to
+// A+ is bounded by its own size, so the result is accurate to ~1e-14 of ||A||.  FP32 (the
+// mixed-precision assembly): 11 and 5 steps, since eigenvalues below ~1e-6 ||A|| are at the level
+// of the FP32 rounding (more quintic steps would amplify the rounding noise of the null space past
+// the iteration's basin); accurate to ~1e-6 of ||A||.  The iteration runs in registers on the
+// matrix cores (Sym16), each iterate made exactly symmetric; the result is written back from the
+// upper triangle, mirrored.  absolute: A <- |A| (eigenvalues mirrored rather than clamped).
+template <typename HT>
+__device__ inline void wave_psd_project(HT* A, int m, int lane, bool absolute = false) {
+    typedef typename Sym16<HT>::V4 V4;
+    __shared__ HT tr[16 * 17];
+    constexpr int n_quintic = sizeof(HT) == 8 ? 24 : 11, n_cubic = sizeof(HT) == 8 ? 6 : 5;
+    const int col = lane % 16, g = lane / 16;
+    V4 a;
+    double f = 0;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const int row = Sym16<HT>::row(g, u);
+        a[u] = (row < m && col < m) ? A[row * m + col] : HT(0);
+        f += (double)a[u] * (double)a[u];
     }
+    const double fro = wave_sum(f);
+    if (fro == 0.0 || !(fro < INFINITY)) return;
+    const HT inv = (HT)rsq_nr(fro);
+    V4 X;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) X[u] = a[u] * inv;
+    for (int it = 0; it < n_quintic + n_cubic; ++it) {
+        const V4 X2 = sym16_mul<HT>(X, X);
+        V4 T;
+        if (it < n_quintic) {
+            const V4 X4 = sym16_mul<HT>(X2, X2);
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                T[u] = HT(2.0315) * X4[u] - HT(4.7750) * X2[u] + ((Sym16<HT>::row(g, u) == col) ? HT(3.4445) : HT(0));
+        } else {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) T[u] = ((Sym16<HT>::row(g, u) == col) ? HT(1.5) : HT(0)) - HT(0.5) * X2[u];
+        }
+        X = sym16_symmetrize<HT>(sym16_mul<HT>(X, T), tr, lane);
+    }
+    const V4 AS = sym16_mul<HT>(a, X);  // |A| = A S
+    __syncthreads();  // every lane has read A
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const int row = Sym16<HT>::row(g, u);
+        if (row < m && col < m && row <= col) {
+            const HT v = absolute ? AS[u] : HT(0.5) * (a[u] + AS[u]);
+            A[row * m + col] = v;
+            A[col * m + row] = v;
+        }
+    }
+    __syncthreads();
 }
 
 }  // namespace abd

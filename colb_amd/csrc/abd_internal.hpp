@@ -192,6 +192,7 @@ void launch_cand_eval(hipStream_t s, const Geo& G, Cnt n, const uint64_t* keys, 
 void launch_pair_d2(hipStream_t s, const Geo& G, int n, const uint64_t* keys, const double4* X, const AbdParams& P,
                     double* d2);
 void launch_compact(hipStream_t s, Cnt n, const int* flag, const int* pos, const uint64_t* keys, uint64_t* out);
+void launch_psd_test(hipStream_t s, int m, int precision, int absolute, int n, const double* A, double* out);
 void launch_contact_hess(hipStream_t s, const Geo& G, Cnt n, const uint64_t* act, const double4* X, const AbdParams& P,
                          int2* body, double* g24, double* H432);
 void launch_incidence(hipStream_t s, const Geo& G, Cnt n, const int2* body, uint64_t* inc, int* n_inc,
@@ -203,9 +204,9 @@ void launch_offblock_sum(hipStream_t s, Cnt n_pairs, const int* pair_b, const in
                          const uint64_t* sorted, const int2* body, const double* H432, double* off, double* offT,
                          Cnt n_act, int* err);
 void launch_csr_keys(hipStream_t s, Cnt n_pairs, const int* pair_b, uint64_t* keys, int* vals);
-void launch_row_ptr(hipStream_t s, int nb, Cnt n_ent, const uint64_t* sorted, int* row_ptr);
+void launch_row_ptr(hipStream_t s, int nb, Cnt n_keys, const uint64_t* sorted, int* row_ptr);
 void launch_body_assemble(hipStream_t s, const Geo& G, const AbdParams& P, const double* q, const double* qtilde,
-                          Cnt n_inc, const uint64_t* inc_sorted, const int2* body, const double* g24,
+                          const int* inc_ptr, const uint64_t* inc_sorted, const int2* body, const double* g24,
                           const double* H432, const int* newton_active, double* grad, double* diag, double* pinv,
                           double* e_body, Cnt n_act);
 void launch_scene_sum(hipStream_t s, const Geo& G, const double* e_body, const double* e_cand, const int* cand_lo,

@@ -228,7 +228,7 @@ __global__ __launch_bounds__(64) void k_contact_hess(Geo G, Cnt cn, const uint64
                                                       const double4* __restrict__ X, AbdParams P,
                                                       int2* __restrict__ body, double* __restrict__ g24,
                                                       double* __restrict__ H432) {
-    __shared__ HT A[144], V[144];
+    __shared__ HT A[144];
     __shared__ double gx[12], jv[4][4];
     __shared__ int side[4];
     const int c = blockIdx.x;
@@ -280,7 +280,7 @@ __global__ __launch_bounds__(64) void k_contact_hess(Geo G, Cnt cn, const uint64
         }
         if (lane < m) gx[lane] = kd * B1 * d2_grad(E, lane);
         __syncthreads();
-        wave_psd_project(A, V, m, lane, P.hess_proj == 1);
+        wave_psd_project(A, m, lane, P.hess_proj == 1);
     }
     // stencil point k -> vertex -> body side and Jacobian row [1, xr]
     const int bA = (int)G.rest[p.v[0]].w;
@@ -326,6 +326,27 @@ void launch_contact_hess(hipStream_t s, const Geo& G, Cnt n, const uint64_t* act
         hipLaunchKernelGGL(k_contact_hess<float>, dim3(n.cap), dim3(64), 0, s, G, n, act, X, P, body, g24, H432);
     else
         hipLaunchKernelGGL(k_contact_hess<double>, dim3(n.cap), dim3(64), 0, s, G, n, act, X, P, body, g24, H432);
+}
+
+// PSD projection of n symmetric m x m matrices (row-major FP64, rounded to HT), one wave each: the
+// routine of k_contact_hess and k_body_assemble on given matrices
+template <typename HT>
+__global__ __launch_bounds__(64) void k_psd_test(int m, int absolute, int n, const double* __restrict__ A,
+                                                 double* __restrict__ out) {
+    __shared__ HT S[144];
+    const int c = blockIdx.x, lane = threadIdx.x;
+    if (c >= n) return;
+    for (int e = lane; e < m * m; e += 64) S[e] = (HT)A[(size_t)c * m * m + e];
+    __syncthreads();
+    wave_psd_project(S, m, lane, absolute != 0);
+    for (int e = lane; e < m * m; e += 64) out[(size_t)c * m * m + e] = (double)S[e];
+}
+
+void launch_psd_test(hipStream_t s, int m, int precision, int absolute, int n, const double* A, double* out) {
+    if (precision == 1)
+        hipLaunchKernelGGL(k_psd_test<float>, dim3(n), dim3(64), 0, s, m, absolute, n, A, out);
+    else
+        hipLaunchKernelGGL(k_psd_test<double>, dim3(n), dim3(64), 0, s, m, absolute, n, A, out);
 }
 
 // incidence keys (body << 32 | contact << 1 | role) and body-pair keys (min(20) | max(20) | contact(24))
@@ -455,14 +476,32 @@ __device__ __forceinline__ int lower_bound64(const uint64_t* a, int n, uint64_t 
     return lo;
 }
 
+// row_ptr[b] (b = 0..nb) = the first key with row (bits 32+) >= b, i.e. lower_bound(b << 32) of the
+// sorted keys, from the run heads: one thread per key writes the rows its run starts (the rows
+// since the previous key's row, empty rows included), the last key the rows after it; rows past nb
+// (invalid keys) count as nb.  No search, one store per row.
 __global__ void k_row_ptr(int nb, Cnt cent, const uint64_t* __restrict__ k, int* row_ptr) {
-    int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b > nb) return;
-    row_ptr[b] = lower_bound64(k, cnt_of(cent), (uint64_t)b << 32);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = cnt_of(cent);
+    if (n == 0) {
+        if (i == 0)
+            for (int b = 0; b <= nb; ++b) row_ptr[b] = 0;
+        return;
+    }
+    if (i >= n) return;
+    auto row = [&](int j) {
+        const uint64_t hi = k[j] >> 32;
+        return hi < (uint64_t)nb ? (int)hi : nb;
+    };
+    const int r = row(i), prev = i == 0 ? -1 : row(i - 1);
+    for (int b = prev + 1; b <= r; ++b) row_ptr[b] = i;
+    if (i == n - 1)
+        for (int b = r + 1; b <= nb; ++b) row_ptr[b] = n;
 }
 
-void launch_row_ptr(hipStream_t s, int nb, Cnt n_ent, const uint64_t* sorted, int* row_ptr) {
-    hipLaunchKernelGGL(k_row_ptr, dim3(cdiv(nb + 1, 256)), dim3(256), 0, s, nb, n_ent, sorted, row_ptr);
+void launch_row_ptr(hipStream_t s, int nb, Cnt n_keys, const uint64_t* sorted, int* row_ptr) {
+    hipLaunchKernelGGL(k_row_ptr, dim3(cdiv(std::max(n_keys.cap, 1), 256)), dim3(256), 0, s, nb, n_keys, sorted,
+                       row_ptr);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -471,7 +510,8 @@ void launch_row_ptr(hipStream_t s, int nb, Cnt n_ent, const uint64_t* sorted, in
 // energy (energy_inertia, abd.py:53-64).  One wave per body.
 // ---------------------------------------------------------------------------------------------
 __global__ __launch_bounds__(64) void k_body_assemble(Geo G, AbdParams P, const double* __restrict__ q,
-                                                       const double* __restrict__ qtilde, Cnt cinc,
+                                                       const double* __restrict__ qtilde,
+                                                       const int* __restrict__ inc_ptr,
                                                        const uint64_t* __restrict__ inc,
                                                        const double* __restrict__ g24,
                                                        const double* __restrict__ H432,
@@ -480,7 +520,7 @@ __global__ __launch_bounds__(64) void k_body_assemble(Geo G, AbdParams P, const 
                                                        double* __restrict__ pinv, double* __restrict__ e_body,
                                                        Cnt cact) {
     __shared__ double A[81], V[81], H[144], W[12 * 24], gq[12], fac[12], qb[12], dq[12];
-    const int n_inc = cnt_of(cinc), n_act = cnt_of(cact);
+    const int n_act = cnt_of(cact);
     const int b = blockIdx.x;
     const int lane = threadIdx.x;
     const double pd = P.pinned_diag;
@@ -532,7 +572,7 @@ __global__ __launch_bounds__(64) void k_body_assemble(Geo G, AbdParams P, const 
     }
     __syncthreads();
     // near a rotation the ortho Hessian is already positive semidefinite: project only otherwise
-    if (!wave_is_psd(A, V, 9, lane)) wave_psd_project(A, V, 9, lane);
+    if (!wave_is_psd(A, V, 9, lane)) wave_psd_project(A, 9, lane);
     __syncthreads();
     for (int e = lane; e < 144; e += 64) {
         int R = e / 12, C = e % 12;
@@ -542,20 +582,50 @@ __global__ __launch_bounds__(64) void k_body_assemble(Geo G, AbdParams P, const 
         H[e] = h;
     }
     __syncthreads();
-    // contact incidences of this body, in sorted (contact) order
-    int lo = lower_bound64(inc, n_inc, (uint64_t)b << 32);
-    int hi = lower_bound64(inc, n_inc, (uint64_t)(b + 1) << 32);
-    for (int m = lo; m < hi; ++m) {
-        uint32_t w = (uint32_t)inc[m];
-        int c = opaque((int)(w >> 1)), role = opaque((int)(w & 1));
-        if (c >= n_act) {
-            if (lane == 0) atomicOr(G.err, ERR_INCIDENCE);
-            break;
+    // contact incidences of this body (inc_ptr: the body's range of the sorted incidence keys), in
+    // sorted (contact) order: each lane sums its entries e = lane, lane + 64, lane + 128 of H (and of
+    // the gradient) in registers, in incidence order, with the loads of four incidences in flight
+    const int lo = inc_ptr[b], hi = inc_ptr[b + 1];
+    double h0 = H[lane], h1 = H[lane + 64], h2 = lane < 16 ? H[lane + 128] : 0.0;
+    double gl = lane < 12 ? gq[lane] : 0.0;
+    bool bad = false;
+    for (int m0 = lo; m0 < hi && !bad; m0 += 4) {
+        int cc[4], rl[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const uint32_t w = (uint32_t)inc[min(m0 + u, hi - 1)];
+            cc[u] = opaque((int)(w >> 1));
+            rl[u] = opaque((int)(w & 1));
         }
-        const double* Hc = H432 + (size_t)c * 432 + role * 144;
-        for (int e = lane; e < 144; e += 64) H[e] += Hc[e];
-        if (lane < 12) gq[lane] += g24[(size_t)c * 24 + role * 12 + lane];
+        double v0[4], v1[4], v2[4], gv[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const bool ok = cc[u] >= 0 && cc[u] < n_act;  // (out-of-range contacts are not read)
+            const double* Hc = H432 + (size_t)(ok ? cc[u] : 0) * 432 + rl[u] * 144;
+            v0[u] = ok ? Hc[lane] : 0.0;
+            v1[u] = ok ? Hc[lane + 64] : 0.0;
+            v2[u] = (ok && lane < 16) ? Hc[lane + 128] : 0.0;
+            gv[u] = (ok && lane < 12) ? g24[(size_t)(ok ? cc[u] : 0) * 24 + rl[u] * 12 + lane] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (m0 + u >= hi || bad) break;
+            if (cc[u] < 0 || cc[u] >= n_act) {
+                if (lane == 0) atomicOr(G.err, ERR_INCIDENCE);
+                bad = true;
+                break;
+            }
+            h0 += v0[u];
+            h1 += v1[u];
+            h2 += v2[u];
+            gl += gv[u];
+        }
     }
+    __syncthreads();  // every lane has read H
+    H[lane] = h0;
+    H[lane + 64] = h1;
+    if (lane < 16) H[lane + 128] = h2;
+    if (lane < 12) gq[lane] = gl;
     __syncthreads();
     const bool active = newton_active[G.bscene[b]] != 0;
     if (lane < 12) grad[12 * b + lane] = active ? gq[lane] : 0.0;
@@ -590,10 +660,10 @@ __global__ __launch_bounds__(64) void k_body_assemble(Geo G, AbdParams P, const 
 }
 
 void launch_body_assemble(hipStream_t s, const Geo& G, const AbdParams& P, const double* q, const double* qtilde,
-                          Cnt n_inc, const uint64_t* inc_sorted, const int2* body, const double* g24,
+                          const int* inc_ptr, const uint64_t* inc_sorted, const int2* body, const double* g24,
                           const double* H432, const int* newton_active, double* grad, double* diag, double* pinv,
                           double* e_body, Cnt n_act) {
-    hipLaunchKernelGGL(k_body_assemble, dim3(G.nb), dim3(64), 0, s, G, P, q, qtilde, n_inc, inc_sorted, g24, H432,
+    hipLaunchKernelGGL(k_body_assemble, dim3(G.nb), dim3(64), 0, s, G, P, q, qtilde, inc_ptr, inc_sorted, g24, H432,
                        newton_active, grad, diag, pinv, e_body, n_act);
 }
 

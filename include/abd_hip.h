@@ -288,6 +288,11 @@ int abd_get_step_samples(const AbdCtx* ctx, float* feat, float* dq, int cap, int
 int abd_log_samples(AbdCtx* ctx, int on);
 /* Copies up to cap logged samples (feat: cap x 49, du: cap x 12); *n = number logged. */
 int abd_get_samples(const AbdCtx* ctx, float* feat, float* du, int cap, int* n);
+/* No counterpart in the reference (a check of the contact-Hessian projection, abd.py:247 projects
+ * with ipctk_wp): the PSD projection that the assembly applies, on n given symmetric m x m matrices
+ * (m = 3, 6, 9 or 12; A and out row-major, n x m x m), in FP64 (precision 0) or in the mixed
+ * assembly's FP32 (precision 1): out = (A + |A|) / 2, or |A| when absolute != 0.  Allocates. */
+int abd_psd_project_test(int m, int precision, int absolute, int n, const double* A, double* out);
 
 #ifdef __cplusplus
 }

@@ -60,6 +60,7 @@ struct EmuSched {
     std::function<void()> body;
     std::vector<double> xbuf;
     std::vector<float> mbuf;  // matrix-instruction operand exchange, 8 values per lane
+    std::vector<double> dbuf;  // the same for the FP64/FP32 16x16x4 instructions, 2 values per lane
     const void* main_bottom = nullptr;
     size_t main_size = 0;
 };
@@ -168,6 +169,37 @@ inline F4 __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(S4 a, S4 b, F4 c, int, int, 
     }
     emu_wave_sync();
     return d;
+}
+
+// v_mfma_f64_16x16x4_f64 / v_mfma_f32_16x16x4_f32 over the 64 lanes of the caller's wave: lane l
+// supplies A(l%16, l/16) and B(l/16, l%16) and receives, in slot u = 0..3, D(R, l%16) = C +
+// sum_k A(R, k) B(k, l%16) with R = l/16 + 4u (FP64; rows interleaved) or 4(l/16) + u (FP32), the
+// layouts measured on the hardware; the products are summed in k order, each rounded to T
+template <class T, class V4>
+inline V4 emu_mfma_16x16x4(T a, T b, V4 c) {
+    const unsigned lane = threadIdx.x % 64, w0 = threadIdx.x - lane;
+    double* db = emu_sched->dbuf.data();
+    db[2 * threadIdx.x] = (double)a;
+    db[2 * threadIdx.x + 1] = (double)b;
+    emu_wave_sync();
+    V4 d = c;
+    const int n = lane % 16;
+    for (int u = 0; u < 4; ++u) {
+        const int m = sizeof(T) == 8 ? (int)(lane / 16) + 4 * u : 4 * (int)(lane / 16) + u;
+        T acc = c[u];
+        for (int k = 0; k < 4; ++k) acc = std::fma((T)db[2 * (w0 + m + 16 * k)], (T)db[2 * (w0 + n + 16 * k) + 1], acc);
+        d[u] = acc;
+    }
+    emu_wave_sync();
+    return d;
+}
+template <class V4>
+inline V4 __builtin_amdgcn_mfma_f64_16x16x4f64(double a, double b, V4 c, int, int, int) {
+    return emu_mfma_16x16x4<double>(a, b, c);
+}
+template <class V4>
+inline V4 __builtin_amdgcn_mfma_f32_16x16x4f32(float a, float b, V4 c, int, int, int) {
+    return emu_mfma_16x16x4<float>(a, b, c);
 }
 
 inline int atomicAdd(int* p, int v) { return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST); }
@@ -293,6 +325,7 @@ void emu_launch(const char* name, K kernel, dim3 grid, dim3 block, Args... args)
     EmuSched S;
     S.xbuf.assign(nt, 0.0);
     S.mbuf.assign((size_t)nt * 8, 0.f);
+    S.dbuf.assign((size_t)nt * 2, 0.0);
     S.body = [&]() { kernel(args...); };
     EmuSched* prev = emu_sched;
     emu_sched = &S;

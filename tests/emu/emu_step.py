@@ -47,6 +47,19 @@ def main():
     qm, qo = gm.get_state()[0], om.get_state()[0]
     print("multi", np.abs(qm - qo).max() / np.abs(qo).max(), flush=True)
     gm.close()
+    # the assembly's PSD projection (matrix-core sign iteration) against eigendecompositions
+    from test_oracle_math import eig_projection, hip_psd_project, psd_cases
+    errs = []
+    for prec in (0, 1):
+        e = 0.0
+        for m in (3, 6, 9, 12):
+            A = psd_cases(m, seed=m, n=8)
+            for absolute in (False, True):
+                out = hip_psd_project(_lib.load(), A, prec, absolute)
+                e = max(e, max(np.abs(out[i] - eig_projection(A[i], absolute)).max() / np.linalg.norm(A[i])
+                               for i in range(len(A))))
+        errs.append(e)
+    print("psd", *errs, flush=True)
     learned(sc, p)
 
 

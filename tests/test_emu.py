@@ -2,8 +2,10 @@
 AddressSanitizer): the radix sort and scan unit driver, and two Newton steps of a two-cube scene
 through the C ABI compared with the oracle (indexing errors surface here as sanitizer reports,
 before any GPU run), then two steps with the learned step predictor (body features against the
-oracle's, the step against the oracle given the same prediction), and two steps with the
-multi-tile persistent PCG."""
+oracle's, the step against the oracle given the same prediction), two steps with the
+multi-tile persistent PCG, and the assembly's PSD projection on the matrix cores (the instruction
+emulated with the operand layout the kernels assume; test_gpu_features checks it on the hardware)
+against eigendecompositions."""
 import os
 import subprocess
 import sys
@@ -41,6 +43,10 @@ def test_emulated_step_matches_oracle(emu_lib):
     assert int(na_emu) == int(na_orc)
     mul = [line for line in out.stdout.splitlines() if line.startswith("multi")]
     assert mul and float(mul[-1].split()[1]) < 1e-6, out.stdout[-2000:]
+    psd = [line for line in out.stdout.splitlines() if line.startswith("psd")]
+    assert psd, out.stdout[-2000:]
+    _, e64, e32 = psd[-1].split()
+    assert float(e64) < 1e-13 and float(e32) < 2e-6
     lrn = [line for line in out.stdout.splitlines() if line.startswith("learned")]
     assert lrn, out.stdout[-2000:]
     _, ferr, serr, pa_emu, pa_orc = lrn[-1].split()

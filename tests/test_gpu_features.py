@@ -17,6 +17,7 @@ from colb_amd.params import north_star
 sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
 from test_oracle_ground import DHAT, cube_on_ground, ground_params  # noqa: E402
 from test_oracle_scene import fd_params, two_cube_scene  # noqa: E402
+from test_oracle_math import eig_projection, hip_psd_project, psd_cases  # noqa: E402
 
 pytestmark = pytest.mark.gpu
 
@@ -40,6 +41,25 @@ def solver(sc, p):
 def rel(a, b):
     a, b = np.asarray(a), np.asarray(b)
     return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+# ------------------------------------------------------------------------------------------------
+# the assembly's PSD projection (matrix-core sign iteration) against eigendecompositions
+# ------------------------------------------------------------------------------------------------
+@pytest.mark.parametrize("m", [3, 6, 9, 12])
+def test_psd_projection_matches_eigh_on_the_matrix_cores(m):
+    """FP64 within 1e-13 of ||A||_F, FP32 (the mixed assembly) within 2e-6; the result exactly
+    symmetric.  A wrong operand or output layout of the matrix instruction fails here."""
+    from colb_amd import _lib
+    A = psd_cases(m, seed=100 + m)
+    for prec, tol in ((0, 1e-13), (1, 2e-6)):
+        for absolute in (False, True):
+            out = hip_psd_project(_lib.load(), A, prec, absolute)
+            err = max(np.abs(out[i] - eig_projection(A[i], absolute)).max() / np.linalg.norm(A[i])
+                      for i in range(len(A)))
+            say(f"  m {m} precision {prec} absolute {absolute} err {err:.2e}")
+            assert err < tol
+            np.testing.assert_array_equal(out, out.transpose(0, 2, 1))
 
 
 # ------------------------------------------------------------------------------------------------

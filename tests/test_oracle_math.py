@@ -165,6 +165,43 @@ def test_segment_params_match_brute_force(oracle_mod):
 # ---------------------------------------------------------------------------------------------
 # PSD projection and the 12x12 Cholesky inverse
 # ---------------------------------------------------------------------------------------------
+def eig_projection(A, absolute=False):
+    """(A + |A|) / 2 (or |A|) of a symmetric matrix from its eigendecomposition."""
+    w, V = np.linalg.eigh(A)
+    return (V * (np.abs(w) if absolute else np.maximum(w, 0))) @ V.T
+
+
+def psd_cases(m, seed=0, n=24):
+    """Symmetric m x m test matrices for a PSD projection: eigenvalues spread over 12 decades, half
+    of them zero (a barrier Hessian's null space), definite and indefinite, at scales 1e-6 .. 1e12."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n):
+        Q, _ = np.linalg.qr(rng.normal(size=(m, m)))
+        w = rng.normal(size=m) * 10.0 ** rng.uniform(-12, 0, size=m)
+        if k % 3 == 0:
+            w[: m // 2] = 0.0
+        if k % 4 == 1:
+            w = np.abs(w)
+        if k % 4 == 2:
+            w = -np.abs(w)
+        A = (Q * w) @ Q.T * 10.0 ** rng.uniform(-6, 12)
+        out.append(0.5 * (A + A.T))
+    return np.ascontiguousarray(np.array(out))
+
+
+def hip_psd_project(lib, A, precision=0, absolute=False):
+    """The assembly's PSD projection (abd_psd_project_test) on matrices A[n, m, m]."""
+    import ctypes as C
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    out = np.zeros_like(A)
+    P = C.POINTER(C.c_double)
+    rc = lib.abd_psd_project_test(A.shape[1], precision, int(absolute), A.shape[0], A.ctypes.data_as(P),
+                                  out.ctypes.data_as(P))
+    assert rc == 0
+    return out
+
+
 @pytest.mark.parametrize("n", [3, 6, 9, 12])
 def test_psd_projection_matches_eigh(oracle_mod, n):
     rng = np.random.default_rng(n)
