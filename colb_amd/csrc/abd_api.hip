@@ -126,7 +126,10 @@ struct AbdCtx {
     // length itself is the device counter C_NCAND
     int cand_cap = 0, nc_bound = 0;
     uint64_t *cand = nullptr, *cand_s = nullptr, *act = nullptr;
-    int *flag = nullptr, *pos = nullptr, *cand_lo = nullptr, *cand_hi = nullptr;
+    int *flag = nullptr, *pos = nullptr, *cand_lo = nullptr, *cand_hi = nullptr;  // cand_hi = cand_lo + 1
+    // set while fast_launches runs: k_iter_reset has cleared the iteration's counters and flags, so
+    // the launch functions skip their own fills of them
+    bool iter_reset = false;
     double* e_cand = nullptr;
     bool cand_valid = false;
     // active contacts
@@ -315,6 +318,23 @@ void timer_collect(AbdCtx* ctx) {
 __global__ void k_fill_i32(int n, int* p, int v) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
+}
+// the fast Newton iteration's clears in one launch: the incidence and body-pair counts, the cache
+// containment flag, the line search's pending flag, the PCG control words and iteration counts, and
+// each scene's CCD bound (1.0)
+__global__ void k_iter_reset(int ns, int* cnt, unsigned long long* amax_bits, int* cg_iters, unsigned* pcg_ctl) {
+    const int t = threadIdx.x;
+    if (t == 0) {
+        cnt[C_NINC] = 0;
+        cnt[C_NPK] = 0;
+        cnt[C_OUTSIDE] = 0;
+        cnt[C_PENDING] = 0;
+    }
+    if (t < 8) pcg_ctl[t] = 0u;
+    for (int s = t; s < ns; s += blockDim.x) {
+        amax_bits[s] = 0x3FF0000000000000ull;
+        cg_iters[s] = 0;
+    }
 }
 __global__ void k_fill_u64(int n, unsigned long long* p, unsigned long long v) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -641,12 +661,11 @@ int traverse(AbdCtx* ctx, const double4* X0, const double4* X1, const double* du
 // the candidate list's length: bounded by nc_bound on the host, exact in C_NCAND on the device
 Cnt cand_cnt(const AbdCtx* ctx) { return Cnt{ctx->nc_bound, ctx->cnt + C_NCAND}; }
 
-// per-scene ranges of the sorted candidates
+// per-scene ranges of the sorted candidates: scene s holds [cand_lo[s], cand_lo[s + 1]) (cand_hi is
+// cand_lo + 1)
 int cand_ranges(AbdCtx* ctx) {
-    CHK(fill_i32(ctx, ctx->cand_lo, ctx->ns, 0));
-    CHK(fill_i32(ctx, ctx->cand_hi, ctx->ns, 0));
-    launch_cand_ranges(ctx->st, cand_cnt(ctx), ctx->cand_s, ctx->cand_lo, ctx->cand_hi);
-    CHK(dbg(ctx, "launch_cand_ranges"));
+    launch_row_ptr(ctx->st, ctx->ns, cand_cnt(ctx), ctx->cand_s, ctx->cand_lo, 48);
+    CHK(dbg(ctx, "launch_row_ptr(scenes)"));
     HIPCHK(hipGetLastError());
     return ABD_OK;
 }
@@ -661,7 +680,7 @@ int cached_broad_phase(AbdCtx* ctx, const double4* X0, const double4* X1, const 
     const double m = ctx->P.cand_margin;
     if (ctx->cache_valid) {
         tic(ctx, "cache_check", (double)ctx->nv * 128);
-        CHK(fill_i32(ctx, ctx->cnt + C_OUTSIDE, 1, 0));
+        if (!ctx->iter_reset) CHK(fill_i32(ctx, ctx->cnt + C_OUTSIDE, 1, 0));
         launch_cache_contains(ctx->st, ctx->nv, ctx->Xc0, ctx->Xc1, X0, X1, 0.5 * m, ctx->cnt + C_OUTSIDE);
         CHK(dbg(ctx, "launch_cache_contains"));
         toc(ctx);
@@ -776,7 +795,7 @@ int assemble(AbdCtx* ctx) {
     launch_contact_hess(ctx->st, ctx->G, ca, ctx->act, ctx->X0, ctx->P, ctx->act_body, ctx->act_g, ctx->act_H);
     CHK(dbg(ctx, "launch_contact_hess"));
     toc(ctx);
-    CHK(fill_i32(ctx, ctx->cnt + C_NINC, 2, 0));  // C_NINC, C_NPK
+    if (!ctx->iter_reset) CHK(fill_i32(ctx, ctx->cnt + C_NINC, 2, 0));  // C_NINC, C_NPK
     launch_incidence(ctx->st, ctx->G, ca, ctx->act_body, ctx->inc, ctx->cnt + C_NINC, ctx->pk, ctx->cnt + C_NPK);
     CHK(dbg(ctx, "launch_incidence"));
     HIPCHK(hipGetLastError());
@@ -1114,8 +1133,10 @@ int solve_persistent(AbdCtx* ctx, const PcgBufT<T>& B, bool* ok) {
     const bool fits = persist_fits<T>(ctx) && !(ctx->P.reserved0 & 32);
     const int mcap = fits ? 0 : multi_cap<T>(ctx);
     if (!fits && mcap <= 0) return ABD_OK;
-    CHK(fill_i32(ctx, ctx->cg_iters, ctx->ns, 0));
-    CHK(fill_i32(ctx, (int*)ctx->pcg_ctl, 8, 0));
+    if (!ctx->iter_reset) {
+        CHK(fill_i32(ctx, ctx->cg_iters, ctx->ns, 0));
+        CHK(fill_i32(ctx, (int*)ctx->pcg_ctl, 8, 0));
+    }
     if (!fits) CHK(fill_i32(ctx, ctx->cg_done, ctx->ns, 0));
     // debug bit 4: raise the failure flag before the launch, so every workgroup gives up at its first
     // barrier that does not fill at once and the solve is redone with one launch per iteration
@@ -1144,7 +1165,6 @@ int solve_persistent(AbdCtx* ctx, const PcgBufT<T>& B, bool* ok) {
         return ABD_OK;
     }
     CHK(dbg(ctx, "launch_pcg_persist"));
-    HIPCHK(hipMemcpyAsync(ctx->h_pcg_ctl, ctx->pcg_ctl, 8 * sizeof(unsigned), hipMemcpyDeviceToHost, ctx->st));
     {
         // algorithmic bytes of the launch: K iterations x [nnzb * (144 T + 4) + nb * (8 * 12 T + 144 T)],
         // nnzb = nb + 2 pairs
@@ -1162,6 +1182,7 @@ int solve_persistent(AbdCtx* ctx, const PcgBufT<T>& B, bool* ok) {
         *ok = true;
         return ABD_OK;
     }
+    HIPCHK(hipMemcpyAsync(ctx->h_pcg_ctl, ctx->pcg_ctl, 8 * sizeof(unsigned), hipMemcpyDeviceToHost, ctx->st));
     HIPCHK(hipStreamSynchronize(ctx->st));
     if (ctx->h_pcg_ctl[4]) {
         // a grid barrier did not fill: do not try again on this context (unless forced for a test)
@@ -1308,17 +1329,22 @@ int solve(AbdCtx* ctx) {
 // --------------------------------------------------------------------------------------------
 // CCD bound + backtracking line search (line_search_iterative, gauss_newton.py:98-115)
 // --------------------------------------------------------------------------------------------
-// speculative: see cached_broad_phase
-int ccd_bound(AbdCtx* ctx, bool speculative = false) {
-    launch_xform(ctx->st, ctx->G, ctx->q, nullptr, nullptr, ctx->X0);
-    CHK(dbg(ctx, "launch_xform"));
+// speculative: see cached_broad_phase.  x0_current: X0 already holds the vertices at q (the fast
+// iteration's assembly transformed them and q has not moved since)
+int ccd_bound(AbdCtx* ctx, bool speculative = false, bool x0_current = false) {
+    if (!x0_current) {
+        launch_xform(ctx->st, ctx->G, ctx->q, nullptr, nullptr, ctx->X0);
+        CHK(dbg(ctx, "launch_xform"));
+    }
     launch_xform(ctx->st, ctx->G, ctx->q, ctx->du, nullptr, ctx->X1);
     CHK(dbg(ctx, "launch_xform"));
-    CHK(fill_i32(ctx, ctx->cnt + C_OUTSIDE, 1, 0));
+    if (!ctx->iter_reset) CHK(fill_i32(ctx, ctx->cnt + C_OUTSIDE, 1, 0));
     CHK(broad_phase(ctx, ctx->X0, ctx->X1, ctx->du, speculative));
-    hipLaunchKernelGGL(k_fill_u64, dim3(cdiv(ctx->ns, 128)), dim3(128), 0, ctx->st, ctx->ns, ctx->C.amax_bits,
-                       (unsigned long long)0x3FF0000000000000ull);
-    CHK(dbg(ctx, "k_fill_u64"));
+    if (!ctx->iter_reset) {
+        hipLaunchKernelGGL(k_fill_u64, dim3(cdiv(ctx->ns, 128)), dim3(128), 0, ctx->st, ctx->ns, ctx->C.amax_bits,
+                           (unsigned long long)0x3FF0000000000000ull);
+        CHK(dbg(ctx, "k_fill_u64"));
+    }
     if (ctx->P.ccd_enabled) {
         tic(ctx, "ccd", (double)ctx->nc_bound * (8 + 8 * 32));
         launch_ccd(ctx->st, ctx->G, cand_cnt(ctx), ctx->cand_s, ctx->X0, ctx->X1, ctx->P, ctx->C.amax_bits);
@@ -1357,7 +1383,7 @@ int ls_trial(AbdCtx* ctx, int floor_accept) {
     CHK(dbg(ctx, "launch_body_energy"));
     launch_scene_sum(ctx->st, ctx->G, ctx->qtmp, ctx->e_cand, ctx->cand_lo, ctx->cand_hi, ctx->C.E1);
     CHK(dbg(ctx, "launch_scene_sum"));
-    CHK(fill_i32(ctx, ctx->cnt + C_PENDING, 1, 0));
+    if (!ctx->iter_reset) CHK(fill_i32(ctx, ctx->cnt + C_PENDING, 1, 0));
     launch_ls_decide(ctx->st, ctx->ns, ctx->P, ctx->C, floor_accept, ctx->cnt + C_PENDING);
     CHK(dbg(ctx, "launch_ls_decide"));
     HIPCHK(hipGetLastError());
@@ -1504,13 +1530,22 @@ int commit_iteration(AbdCtx* ctx, const int* skip, std::vector<int>& now) {
 // bound over the candidate cache (its containment test unread), the line search's first trial, the
 // skip flag, and the commit under it.
 int fast_launches(AbdCtx* ctx) {
+    // one kernel clears what the iteration's launches would clear one fill at a time
+    hipLaunchKernelGGL(k_iter_reset, dim3(1), dim3(256), 0, ctx->st, ctx->ns, ctx->cnt, ctx->C.amax_bits,
+                       ctx->cg_iters, ctx->pcg_ctl);
+    CHK(dbg(ctx, "k_iter_reset"));
+    struct Reset {
+        AbdCtx* c;
+        ~Reset() { c->iter_reset = false; }
+    } reset{ctx};
+    ctx->iter_reset = true;
     CHK(assemble(ctx));
     ctx->pcg_unchecked = false;
     ctx->defer_pcg_check = true;
     const int rc = solve(ctx);
     ctx->defer_pcg_check = false;
     CHK(rc);
-    CHK(ccd_bound(ctx, true));
+    CHK(ccd_bound(ctx, true, true));
     CHK(ls_begin(ctx));
     CHK(ls_trial(ctx, 1));
     hipLaunchKernelGGL(k_skip_flag, dim3(1), dim3(64), 0, ctx->st, ctx->cnt,
@@ -2093,7 +2128,8 @@ int abd_create(const AbdSceneDesc* d, const AbdParams* p, AbdCtx** out) {
     A(body_box, 6 * (size_t)nb);
     ctx->M = BodyMesh{ctx->body_v0, ctx->body_e0, ctx->body_t0, ctx->body_mesh, ctx->mesh_tn0, ctx->mesh_en0,
                       ctx->body_tw0, ctx->body_ew0, ctx->tri_nodes, ctx->edge_nodes, ctx->tri_world, ctx->edge_world};
-    A(cand_lo, ns); A(cand_hi, ns);
+    A(cand_lo, ns + 1);
+    ctx->cand_hi = ctx->cand_lo + 1;
     A(r, 12 * nb); A(z, 12 * nb); A(p0, 12 * nb); A(p1, 12 * nb); A(Ap, 12 * nb);
     A(r1, 12 * nb); A(w1, 12 * nb); A(s0, 12 * nb); A(pcg_sc, 4 * ns); A(pcg_ctl, 8);
     A(feat_raw, (size_t)ABD_N_FEATURES * nb); A(fmean, ABD_N_FEATURES); A(fistd, ABD_N_FEATURES); A(omean, 12);

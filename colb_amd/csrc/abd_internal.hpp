@@ -186,7 +186,6 @@ void launch_scene_norm(hipStream_t s, const Geo& G, const double* v, double* out
 void launch_warm_select(hipStream_t s, const Geo& G, const double* nb_b, const double* nb_r, const double* b,
                         double* rhs, double* x0, int* used);
 void launch_add_inplace(hipStream_t s, int n, double* y, const double* x);
-void launch_cand_ranges(hipStream_t s, Cnt n, const uint64_t* keys, int* lo, int* hi);
 void launch_cand_eval(hipStream_t s, const Geo& G, Cnt n, const uint64_t* keys, const double4* X, const AbdParams& P,
                       int* flag, double* energy);
 void launch_pair_d2(hipStream_t s, const Geo& G, int n, const uint64_t* keys, const double4* X, const AbdParams& P,
@@ -204,7 +203,7 @@ void launch_offblock_sum(hipStream_t s, Cnt n_pairs, const int* pair_b, const in
                          const uint64_t* sorted, const int2* body, const double* H432, double* off, double* offT,
                          Cnt n_act, int* err);
 void launch_csr_keys(hipStream_t s, Cnt n_pairs, const int* pair_b, uint64_t* keys, int* vals);
-void launch_row_ptr(hipStream_t s, int nb, Cnt n_keys, const uint64_t* sorted, int* row_ptr);
+void launch_row_ptr(hipStream_t s, int nb, Cnt n_keys, const uint64_t* sorted, int* row_ptr, int shift = 32);
 void launch_body_assemble(hipStream_t s, const Geo& G, const AbdParams& P, const double* q, const double* qtilde,
                           const int* inc_ptr, const uint64_t* inc_sorted, const int2* body, const double* g24,
                           const double* H432, const int* newton_active, double* grad, double* diag, double* pinv,

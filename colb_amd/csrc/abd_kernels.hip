@@ -69,23 +69,6 @@ void launch_xform(hipStream_t s, const Geo& G, const double* q, const double* du
 }
 
 // ---------------------------------------------------------------------------------------------
-// per-scene ranges of the sorted candidate keys
-// ---------------------------------------------------------------------------------------------
-__global__ void k_cand_ranges(Cnt cn, const uint64_t* __restrict__ k, int* lo, int* hi) {
-    const int n = cnt_of(cn);
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    int s = dkey_scene(k[i]);
-    if (i == 0 || dkey_scene(k[i - 1]) != s) lo[s] = i;
-    if (i == n - 1 || dkey_scene(k[i + 1]) != s) hi[s] = i + 1;
-}
-
-void launch_cand_ranges(hipStream_t s, Cnt n, const uint64_t* keys, int* lo, int* hi) {
-    if (n.cap <= 0) return;
-    hipLaunchKernelGGL(k_cand_ranges, dim3(cdiv(n.cap, 256)), dim3(256), 0, s, n, keys, lo, hi);
-}
-
-// ---------------------------------------------------------------------------------------------
 // narrow phase
 // ---------------------------------------------------------------------------------------------
 struct PairV {
@@ -476,11 +459,12 @@ __device__ __forceinline__ int lower_bound64(const uint64_t* a, int n, uint64_t 
     return lo;
 }
 
-// row_ptr[b] (b = 0..nb) = the first key with row (bits 32+) >= b, i.e. lower_bound(b << 32) of the
-// sorted keys, from the run heads: one thread per key writes the rows its run starts (the rows
-// since the previous key's row, empty rows included), the last key the rows after it; rows past nb
-// (invalid keys) count as nb.  No search, one store per row.
-__global__ void k_row_ptr(int nb, Cnt cent, const uint64_t* __restrict__ k, int* row_ptr) {
+// row_ptr[b] (b = 0..nb) = the first key with row (bits `shift`+) >= b, i.e. lower_bound(b << shift)
+// of the sorted keys, from the run heads: one thread per key writes the rows its run starts (the
+// rows since the previous key's row, empty rows included), the last key the rows after it; rows
+// past nb (invalid keys) count as nb.  No search, one store per row.  Rows: bodies of the CSR entry
+// and incidence keys (shift 32), scenes of the candidate keys (shift 48).
+__global__ void k_row_ptr(int nb, Cnt cent, const uint64_t* __restrict__ k, int shift, int* row_ptr) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int n = cnt_of(cent);
     if (n == 0) {
@@ -490,7 +474,7 @@ __global__ void k_row_ptr(int nb, Cnt cent, const uint64_t* __restrict__ k, int*
     }
     if (i >= n) return;
     auto row = [&](int j) {
-        const uint64_t hi = k[j] >> 32;
+        const uint64_t hi = k[j] >> shift;
         return hi < (uint64_t)nb ? (int)hi : nb;
     };
     const int r = row(i), prev = i == 0 ? -1 : row(i - 1);
@@ -499,9 +483,9 @@ __global__ void k_row_ptr(int nb, Cnt cent, const uint64_t* __restrict__ k, int*
         for (int b = r + 1; b <= nb; ++b) row_ptr[b] = n;
 }
 
-void launch_row_ptr(hipStream_t s, int nb, Cnt n_keys, const uint64_t* sorted, int* row_ptr) {
+void launch_row_ptr(hipStream_t s, int nb, Cnt n_keys, const uint64_t* sorted, int* row_ptr, int shift) {
     hipLaunchKernelGGL(k_row_ptr, dim3(cdiv(std::max(n_keys.cap, 1), 256)), dim3(256), 0, s, nb, n_keys, sorted,
-                       row_ptr);
+                       shift, row_ptr);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -986,10 +970,13 @@ __device__ bool grid_barrier(unsigned* bar, int* fail, unsigned k, unsigned nblo
         __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         const unsigned target = k * nblocks;
         int spins = 0;
-        // one agent-scope load per spin; the failure flag (set by a workgroup whose wait ran out)
-        // is read every 32 spins, so a failed solve still ends every workgroup within bounded
-        // time, while a barrier that fills costs no extra round trip.  A workgroup that passes a
-        // barrier another one gave up on stops at the next barrier, which cannot fill.
+        // one agent-scope load per spin, ~1500 cycles apart (s_sleep 24): polling the count more
+        // often delays the arrivals on it (8.66 us per chain-net PCG iteration at s_sleep 1, 8.12
+        // at 24, 8.27 at 32; issuing four polls at once was slower still).  The failure flag (set
+        // by a workgroup whose wait ran out) is read every 32 spins, so a failed solve still ends
+        // every workgroup within bounded time, while a barrier that fills costs no extra round
+        // trip.  A workgroup that passes a barrier another one gave up on stops at the next
+        // barrier, which cannot fill.
         while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
             if ((++spins & 31) == 0) {
                 if (__hip_atomic_load(fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) { ok = 0; break; }
@@ -999,7 +986,7 @@ __device__ bool grid_barrier(unsigned* bar, int* fail, unsigned k, unsigned nblo
                     break;
                 }
             }
-            __builtin_amdgcn_s_sleep(1);
+            __builtin_amdgcn_s_sleep(24);
         }
         ok_sh = ok;
     }
