@@ -193,7 +193,9 @@ struct AbdCtx {
     SceneCtl C;
     int* nact_s = nullptr;
     int* act_before = nullptr;
-    double* report = nullptr;
+    double* report = nullptr;  // head of the commit staging block: report (R_N x ns doubles), then
+                               // the Newton flags (ns ints), the counters (C_N) and the PCG words (8)
+    char* h_stage = nullptr;   // pinned copy of the staging block
     int* cnt = nullptr;       // device counters
     int* h_cnt = nullptr;     // pinned host mirror
     double* h_report = nullptr;
@@ -351,24 +353,6 @@ __global__ void k_nact_scene(int ns, const int* lo, const int* hi, const int* fl
     int a = lo[s], b = hi[s];
     out[s] = (b > a) ? (pos[b - 1] + flag[b - 1] - pos[a]) : 0;
 }
-__global__ void k_report(int ns, SceneCtl c, const int* cg, const int* nact, const int* lo, const int* hi,
-                         const int* active_before, double* rep) {
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= ns) return;
-    double* R = rep + (size_t)s * R_N;
-    R[R_RES] = c.residual[s];
-    R[R_ALPHA] = c.alpha[s];
-    R[R_ALPHA0] = c.alpha0[s];
-    R[R_FLOOR] = c.ls_floor[s];
-    R[R_ACTIVE] = active_before[s];
-    R[R_CG] = cg[s];
-    R[R_E1] = c.E1[s];
-    R[R_NACT] = nact[s];
-    R[R_NCAND] = hi[s] - lo[s];
-    R[R_KC] = c.kc[s];
-    R[R_RAISES] = c.k_raises[s];
-    R[R_PALPHA] = c.pred_alpha[s];
-}
 // timing slot: out[0] = max(a[0..n)) (1 without a), out[1] = *np; one wave
 __global__ void k_timing_slot(int n, const int* a, const int* np, int* out) {
     int m = a ? 0 : 1;
@@ -381,15 +365,80 @@ __global__ void k_timing_slot(int n, const int* a, const int* np, int* out) {
         out[1] = *np;
     }
 }
+// bytes of the commit staging block (report, Newton flags, counters, PCG words)
+inline size_t stage_bytes(int ns) { return sizeof(double) * R_N * ns + sizeof(int) * (ns + C_N + 8); }
+
 // C_SKIP: the speculative part of a Newton iteration is discarded when a scene's line search is still
 // halving (C_PENDING), the swept path left the cached candidate region (C_OUTSIDE), the persistent
 // PCG gave up at a grid barrier (*pcg_fail, when that solve's flag is unread) or the assembly's lists
 // outgrew their buffers
-__global__ void k_skip_flag(int* cnt, const unsigned* pcg_fail, int act_cap, int pair_cap) {
-    if (blockIdx.x == 0 && threadIdx.x == 0)
-        cnt[C_SKIP] = (cnt[C_PENDING] != 0 || cnt[C_OUTSIDE] != 0 || (pcg_fail && *pcg_fail != 0u) ||
-                       cnt[C_NACT] > act_cap || cnt[C_NUNIQ] > pair_cap) ? 1 : 0;
+__device__ __forceinline__ bool skip_test(const int* cnt, const unsigned* pcg_fail, int act_cap, int pair_cap) {
+    return cnt[C_PENDING] != 0 || cnt[C_OUTSIDE] != 0 || (pcg_fail && *pcg_fail != 0u) || cnt[C_NACT] > act_cap ||
+           cnt[C_NUNIQ] > pair_cap;
 }
+
+// add_du_kernel (abd.py:780-785): q -= alpha du, with the fast iteration's skip test (every thread
+// evaluates it; thread 0 stores it in C_SKIP for the kernels after and the host): nothing moves
+// when it holds
+__global__ void k_apply_tested(Geo G, double* q, const double* __restrict__ du, const double* __restrict__ alpha,
+                               const int* __restrict__ active, int* cnt, const unsigned* pcg_fail, int act_cap,
+                               int pair_cap) {
+    const bool skip = skip_test(cnt, pcg_fail, act_cap, pair_cap);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) cnt[C_SKIP] = skip ? 1 : 0;
+    if (i >= 12 * G.nb || skip) return;
+    const int s = G.bscene[i / 12];
+    if (active[s]) q[i] -= alpha[s] * du[i];
+}
+
+// The end of a Newton iteration, one workgroup per scene: the scene's largest contact stiffness
+// (k_scene_kmax), the convergence test (abd.py:699, or, with stall_alpha > 0, a step pinned below
+// it; nothing when the skip flag is set), the report,
+// and the staging block that one copy brings to the host (workgroup 0 adds the counters and the
+// PCG words, final by now)
+__global__ __launch_bounds__(256) void k_commit_tail(Geo G, AbdParams P, SceneCtl c, int* active_before,
+                                                     const int* skip, const int* cg, const int* nact, const int* lo,
+                                                     const int* hi, const int* cnt, const unsigned* pcg_ctl,
+                                                     double* stage) {
+    __shared__ double sh[4];
+    const int s = blockIdx.x, ns = G.ns;
+    double k = 0.0;
+    for (int b = G.scene_b0[s] + threadIdx.x; b < G.scene_b0[s + 1]; b += 256) k = fmax(k, c.kb[b]);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) k = fmax(k, __shfl_xor(k, o, 64));
+    if (threadIdx.x % 64 == 0) sh[threadIdx.x / 64] = k;
+    __syncthreads();
+    int* stage_i = (int*)(stage + (size_t)R_N * ns);
+    if (threadIdx.x == 0) {
+        const double kc = fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
+        c.kc[s] = kc;
+        if (!(skip && *skip)) {
+            const int a = c.newton_active[s];
+            active_before[s] = a;
+            if (a && (c.residual[s] < P.newton_tol || (P.stall_alpha > 0.0 && c.alpha[s] < P.stall_alpha)))
+                c.newton_active[s] = 0;
+        }
+        double* R = stage + (size_t)s * R_N;
+        R[R_RES] = c.residual[s];
+        R[R_ALPHA] = c.alpha[s];
+        R[R_ALPHA0] = c.alpha0[s];
+        R[R_FLOOR] = c.ls_floor[s];
+        R[R_ACTIVE] = active_before[s];
+        R[R_CG] = cg[s];
+        R[R_E1] = c.E1[s];
+        R[R_NACT] = nact[s];
+        R[R_NCAND] = hi[s] - lo[s];
+        R[R_KC] = kc;
+        R[R_RAISES] = c.k_raises[s];
+        R[R_PALPHA] = c.pred_alpha[s];
+        stage_i[s] = c.newton_active[s];
+    }
+    if (s == 0) {
+        for (int i = threadIdx.x; i < C_N; i += 256) stage_i[ns + i] = cnt[i];
+        if (threadIdx.x < 8) stage_i[ns + C_N + threadIdx.x] = (int)pcg_ctl[threadIdx.x];
+    }
+}
+
 // *n_out = pos[n - 1] + flag[n - 1] (the length of a compacted list), 0 for an empty input
 __global__ void k_scan_total(Cnt cn, const int* flag, const int* pos, int* n_out) {
     const int n = cnt_of(cn);
@@ -521,6 +570,21 @@ int bits_for(int n) {
 
 // mask of the low `bits` bits
 uint64_t low_bits(int bits) { return bits >= 64 ? ~0ull : ((1ull << bits) - 1); }
+
+// the bits a candidate key (make_key) can set: the scene index (none for one scene), the kind, and
+// the two scene-relative primitive ids up to the largest scene's vertex, edge and triangle counts;
+// the radix sort passes over these bits only
+uint64_t cand_key_sig(const AbdCtx* ctx) {
+    int mi = 1, mj = 1;
+    for (int s = 0; s < ctx->ns; ++s) {
+        const int nv = ctx->h_scene_v0[s + 1] - ctx->h_scene_v0[s], ne = ctx->h_scene_e0[s + 1] - ctx->h_scene_e0[s];
+        const int nt = ctx->h_scene_t0[s + 1] - ctx->h_scene_t0[s];
+        mi = std::max(mi, std::max(nv, ne));
+        mj = std::max(mj, std::max(nt, ne));
+    }
+    const uint64_t scene = ctx->ns > 1 ? low_bits(bits_for(ctx->ns)) << 48 : 0ull;
+    return scene | (3ull << 46) | (low_bits(bits_for(mi)) << 23) | low_bits(bits_for(mj));
+}
 
 // --------------------------------------------------------------------------------------------
 // capacities
@@ -695,7 +759,7 @@ int cached_broad_phase(AbdCtx* ctx, const double4* X0, const double4* X1, const 
         ctx->n_cache = n;
         CHK(fill_i32(ctx, ctx->cnt + C_NCACHE, 1, n));
         tic(ctx, "candidate_sort");
-        CHK(sort_keys(ctx, ctx->cache, ctx->cache_s, host_cnt(n), low_bits(48 + bits_for(ctx->ns + 1))));
+        CHK(sort_keys(ctx, ctx->cache, ctx->cache_s, host_cnt(n), cand_key_sig(ctx)));
         toc(ctx);
         HIPCHK(hipMemcpyAsync(ctx->Xc0, X0, ctx->nv * sizeof(double4), hipMemcpyDeviceToDevice, ctx->st));
         HIPCHK(hipMemcpyAsync(ctx->Xc1, X1, ctx->nv * sizeof(double4), hipMemcpyDeviceToDevice, ctx->st));
@@ -712,13 +776,12 @@ int cached_broad_phase(AbdCtx* ctx, const double4* X0, const double4* X1, const 
         launch_cache_filter(ctx->st, ctx->G, cn, ctx->cache_s, X0, X1, ctx->P, ctx->P.dhat, ctx->cache_flag);
         CHK(dbg(ctx, "launch_cache_filter"));
         CHK(excl_scan(ctx, ctx->cache_flag, ctx->cache_pos, cn));
-        launch_compact(ctx->st, cn, ctx->cache_flag, ctx->cache_pos, ctx->cache_s, ctx->cand_s);
+        launch_compact(ctx->st, cn, ctx->cache_flag, ctx->cache_pos, ctx->cache_s, ctx->cand_s, ctx->cnt + C_NCAND);
         CHK(dbg(ctx, "launch_compact"));
         toc(ctx);
+    } else {
+        CHK(fill_i32(ctx, ctx->cnt + C_NCAND, 1, 0));
     }
-    hipLaunchKernelGGL(k_scan_total, dim3(1), dim3(64), 0, ctx->st, cn, ctx->cache_flag, ctx->cache_pos,
-                       ctx->cnt + C_NCAND);
-    CHK(dbg(ctx, "k_scan_total"));
     if (ctx->P.reserved0 & 4) {
         CHK(sync_counters(ctx));
         std::fprintf(stderr, "[abd] cached broad phase (%s): %d cached, %d candidates\n", du ? "swept" : "static",
@@ -735,7 +798,7 @@ int broad_phase(AbdCtx* ctx, const double4* X0, const double4* X1, const double*
     ctx->nc_bound = n;
     CHK(fill_i32(ctx, ctx->cnt + C_NCAND, 1, n));
     tic(ctx, "candidate_sort");
-    CHK(sort_keys(ctx, ctx->cand, ctx->cand_s, host_cnt(n), low_bits(48 + bits_for(ctx->ns + 1))));
+    CHK(sort_keys(ctx, ctx->cand, ctx->cand_s, host_cnt(n), cand_key_sig(ctx)));
     toc(ctx);
     return cand_ranges(ctx);
 }
@@ -1363,10 +1426,9 @@ int ls_begin(AbdCtx* ctx) {
     launch_cand_eval(ctx->st, ctx->G, cand_cnt(ctx), ctx->cand_s, ctx->X0, ctx->P, nullptr, ctx->e_cand);
     CHK(dbg(ctx, "launch_cand_eval"));
     toc(ctx);
-    launch_scene_sum(ctx->st, ctx->G, ctx->e_body, ctx->e_cand, ctx->cand_lo, ctx->cand_hi, ctx->C.E0);
-    CHK(dbg(ctx, "launch_scene_sum"));
-    launch_ls_init(ctx->st, ctx->ns, ctx->P, ctx->C.newton_active, ctx->C);
-    CHK(dbg(ctx, "launch_ls_init"));
+    launch_scene_sum_ls(ctx->st, ctx->G, ctx->e_body, ctx->e_cand, ctx->cand_lo, ctx->cand_hi, ctx->C.E0, 1, ctx->P,
+                        ctx->C.newton_active, ctx->C, 0, nullptr);
+    CHK(dbg(ctx, "launch_scene_sum(ls_init)"));
     return ABD_OK;
 }
 
@@ -1381,11 +1443,10 @@ int ls_trial(AbdCtx* ctx, int floor_accept) {
     toc(ctx);
     launch_body_energy(ctx->st, ctx->G, ctx->P, ctx->q, ctx->du, ctx->C.alpha, ctx->qtilde, ctx->qtmp);
     CHK(dbg(ctx, "launch_body_energy"));
-    launch_scene_sum(ctx->st, ctx->G, ctx->qtmp, ctx->e_cand, ctx->cand_lo, ctx->cand_hi, ctx->C.E1);
-    CHK(dbg(ctx, "launch_scene_sum"));
     if (!ctx->iter_reset) CHK(fill_i32(ctx, ctx->cnt + C_PENDING, 1, 0));
-    launch_ls_decide(ctx->st, ctx->ns, ctx->P, ctx->C, floor_accept, ctx->cnt + C_PENDING);
-    CHK(dbg(ctx, "launch_ls_decide"));
+    launch_scene_sum_ls(ctx->st, ctx->G, ctx->qtmp, ctx->e_cand, ctx->cand_lo, ctx->cand_hi, ctx->C.E1, 2, ctx->P,
+                        nullptr, ctx->C, floor_accept, ctx->cnt + C_PENDING);
+    CHK(dbg(ctx, "launch_scene_sum(ls_decide)"));
     HIPCHK(hipGetLastError());
     return ABD_OK;
 }
@@ -1492,36 +1553,46 @@ int log_step(AbdCtx* ctx) {
 }
 
 // The kernels after the line search (q -= alpha du, adaptive stiffness, the convergence test, the
-// report) and the copies of the report, the scenes' Newton flags, the PCG control words and the
-// counters into pinned host memory.  skip (optional): a device flag under which nothing is committed.
-int commit_launches(AbdCtx* ctx, const int* skip) {
+// report) and one copy of the staging block (report, Newton flags, counters, PCG control words) into
+// pinned host memory.  tested: the fast iteration's skip test decides in k_apply_tested whether
+// anything is committed (C_SKIP).
+int commit_launches(AbdCtx* ctx, bool tested) {
     const int ns = ctx->ns;
-    launch_apply(ctx->st, ctx->G, ctx->q, ctx->du, ctx->C.alpha, ctx->C.newton_active, skip);
-    CHK(dbg(ctx, "launch_apply"));
+    const int* skip = tested ? ctx->cnt + C_SKIP : nullptr;
+    if (tested) {
+        hipLaunchKernelGGL(k_apply_tested, dim3(cdiv(12 * ctx->nb, 256)), dim3(256), 0, ctx->st, ctx->G, ctx->q,
+                           ctx->du, ctx->C.alpha, ctx->C.newton_active, ctx->cnt,
+                           ctx->pcg_unchecked ? (const unsigned*)(ctx->pcg_ctl + 4) : (const unsigned*)nullptr,
+                           ctx->act_cap, ctx->pair_cap);
+        CHK(dbg(ctx, "k_apply_tested"));
+    } else {
+        launch_apply(ctx->st, ctx->G, ctx->q, ctx->du, ctx->C.alpha, ctx->C.newton_active, nullptr);
+        CHK(dbg(ctx, "launch_apply"));
+    }
     CHK(adapt_stiffness(ctx, skip));
     // keep the pre-check flags for the report, then stop converged scenes (abd.py:699)
-    launch_newton_check(ctx->st, ns, ctx->P, ctx->C, ctx->act_before, skip);
-    CHK(dbg(ctx, "launch_newton_check"));
-    launch_scene_kmax(ctx->st, ctx->G, ctx->C);
-    hipLaunchKernelGGL(k_report, dim3(cdiv(ns, 128)), dim3(128), 0, ctx->st, ns, ctx->C, ctx->cg_iters, ctx->nact_s,
-                       ctx->cand_lo, ctx->cand_hi, ctx->act_before, ctx->report);
-    CHK(dbg(ctx, "k_report"));
-    HIPCHK(hipMemcpyAsync(ctx->h_report, ctx->report, sizeof(double) * R_N * ns, hipMemcpyDeviceToHost, ctx->st));
-    HIPCHK(hipMemcpyAsync(ctx->h_now, ctx->C.newton_active, sizeof(int) * ns, hipMemcpyDeviceToHost, ctx->st));
-    HIPCHK(hipMemcpyAsync(ctx->h_pcg_ctl, ctx->pcg_ctl, 8 * sizeof(unsigned), hipMemcpyDeviceToHost, ctx->st));
-    HIPCHK(hipMemcpyAsync(ctx->h_cnt, ctx->cnt, C_N * sizeof(int), hipMemcpyDeviceToHost, ctx->st));
+    hipLaunchKernelGGL(k_commit_tail, dim3(ns), dim3(256), 0, ctx->st, ctx->G, ctx->P, ctx->C, ctx->act_before, skip,
+                       ctx->cg_iters, ctx->nact_s, ctx->cand_lo, ctx->cand_hi, ctx->cnt, ctx->pcg_ctl, ctx->report);
+    CHK(dbg(ctx, "k_commit_tail"));
+    HIPCHK(hipMemcpyAsync(ctx->h_stage, ctx->report, stage_bytes(ns), hipMemcpyDeviceToHost, ctx->st));
     return ABD_OK;
 }
 
-// the iteration's one host round trip: wait for the copies of commit_launches
+// the iteration's one host round trip: wait for the copy of commit_launches and unpack it
 int commit_wait(AbdCtx* ctx, std::vector<int>& now) {
     HIPCHK(hipStreamSynchronize(ctx->st));
-    std::copy(ctx->h_now, ctx->h_now + ctx->ns, now.begin());
+    const int ns = ctx->ns;
+    const int* si = (const int*)(ctx->h_stage + sizeof(double) * R_N * ns);
+    std::memcpy(ctx->h_report, ctx->h_stage, sizeof(double) * R_N * ns);
+    std::memcpy(ctx->h_now, si, sizeof(int) * ns);
+    std::memcpy(ctx->h_cnt, si + ns, sizeof(int) * C_N);
+    std::memcpy(ctx->h_pcg_ctl, si + ns + C_N, sizeof(unsigned) * 8);
+    std::copy(ctx->h_now, ctx->h_now + ns, now.begin());
     return check_err_word(ctx);
 }
 
-int commit_iteration(AbdCtx* ctx, const int* skip, std::vector<int>& now) {
-    CHK(commit_launches(ctx, skip));
+int commit_iteration(AbdCtx* ctx, bool tested, std::vector<int>& now) {
+    CHK(commit_launches(ctx, tested));
     return commit_wait(ctx, now);
 }
 
@@ -1548,11 +1619,7 @@ int fast_launches(AbdCtx* ctx) {
     CHK(ccd_bound(ctx, true, true));
     CHK(ls_begin(ctx));
     CHK(ls_trial(ctx, 1));
-    hipLaunchKernelGGL(k_skip_flag, dim3(1), dim3(64), 0, ctx->st, ctx->cnt,
-                       ctx->pcg_unchecked ? (const unsigned*)(ctx->pcg_ctl + 4) : (const unsigned*)nullptr,
-                       ctx->act_cap, ctx->pair_cap);
-    CHK(dbg(ctx, "k_skip_flag"));
-    return commit_launches(ctx, ctx->cnt + C_SKIP);
+    return commit_launches(ctx, true);
 }
 
 // Key of the fast iteration's launch sequence: every host value its launches bake in (buffer
@@ -1655,7 +1722,7 @@ int newton_iteration(AbdCtx* ctx, bool fast, std::vector<int>& now) {
         CHK(ccd_bound(ctx));
         CHK(ls_begin(ctx));
         CHK(ls_until_done(ctx, 1, 0));
-        CHK(commit_iteration(ctx, nullptr, now));  // applies the accepted step
+        CHK(commit_iteration(ctx, false, now));  // applies the accepted step
         ctx->cand_valid = true;
         return ABD_OK;
     }
@@ -1692,7 +1759,7 @@ int newton_iteration(AbdCtx* ctx, bool fast, std::vector<int>& now) {
             CHK(ls_until_done(ctx, 1, 1));  // the halvings after the first trial
         }
         ctx->pcg_unchecked = false;
-        CHK(commit_iteration(ctx, nullptr, now));
+        CHK(commit_iteration(ctx, false, now));
     }
     ctx->pcg_unchecked = false;
     // the accepted iterate lies inside the swept boxes: the candidates stay valid
@@ -2142,13 +2209,14 @@ int abd_create(const AbdSceneDesc* d, const AbdParams* p, AbdCtx** out) {
     A(C.newton_active, ns); A(C.residual, ns); A(C.alpha, ns); A(C.alpha0, ns); A(C.amax_bits, ns);
     A(C.ls_done, ns); A(C.ls_floor, ns); A(C.E0, ns); A(C.E1, ns);
     A(C.kb, nb); A(C.dmin_prev, nb); A(C.dmin_bits, nb); A(C.k_raises, ns); A(C.kc, ns); A(C.pred_alpha, ns);
-    A(nact_s, ns); A(act_before, ns); A(report, R_N * ns); A(cnt, C_N); A(row_ptr, nb + 1); A(inc_ptr, nb + 1);
+    A(nact_s, ns); A(act_before, ns); A(report, R_N * ns + (ns + C_N + 8 + 1) / 2); A(cnt, C_N); A(row_ptr, nb + 1); A(inc_ptr, nb + 1);
 #undef A
 #undef U
     if (hipHostMalloc((void**)&ctx->h_pcg_ctl, 8 * sizeof(unsigned), hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc((void**)&ctx->h_cnt, C_N * sizeof(int), hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc((void**)&ctx->h_report, R_N * ns * sizeof(double), hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc((void**)&ctx->h_now, ns * sizeof(int), hipHostMallocDefault) != hipSuccess) {
+        hipHostMalloc((void**)&ctx->h_now, ns * sizeof(int), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&ctx->h_stage, stage_bytes(ns), hipHostMallocDefault) != hipSuccess) {
         ctx->err = "hipHostMalloc failed";
         return fail(ABD_ERR_HIP);
     }
@@ -2189,6 +2257,7 @@ void abd_destroy(AbdCtx* ctx) {
     if (ctx->h_pcg_ctl) (void)hipHostFree(ctx->h_pcg_ctl);
     if (ctx->h_report) (void)hipHostFree(ctx->h_report);
     if (ctx->h_now) (void)hipHostFree(ctx->h_now);
+    if (ctx->h_stage) (void)hipHostFree(ctx->h_stage);
     if (ctx->gexec) (void)hipGraphExecDestroy(ctx->gexec);
     if (ctx->st) (void)hipStreamDestroy(ctx->st);
     delete ctx;

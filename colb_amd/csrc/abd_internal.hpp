@@ -190,7 +190,8 @@ void launch_cand_eval(hipStream_t s, const Geo& G, Cnt n, const uint64_t* keys, 
                       int* flag, double* energy);
 void launch_pair_d2(hipStream_t s, const Geo& G, int n, const uint64_t* keys, const double4* X, const AbdParams& P,
                     double* d2);
-void launch_compact(hipStream_t s, Cnt n, const int* flag, const int* pos, const uint64_t* keys, uint64_t* out);
+void launch_compact(hipStream_t s, Cnt n, const int* flag, const int* pos, const uint64_t* keys, uint64_t* out,
+                    int* n_out = nullptr);
 void launch_psd_test(hipStream_t s, int m, int precision, int absolute, int n, const double* A, double* out);
 void launch_contact_hess(hipStream_t s, const Geo& G, Cnt n, const uint64_t* act, const double4* X, const AbdParams& P,
                          int2* body, double* g24, double* H432);
@@ -210,6 +211,9 @@ void launch_body_assemble(hipStream_t s, const Geo& G, const AbdParams& P, const
                           double* e_body, Cnt n_act);
 void launch_scene_sum(hipStream_t s, const Geo& G, const double* e_body, const double* e_cand, const int* cand_lo,
                       const int* cand_hi, double* out);
+void launch_scene_sum_ls(hipStream_t s, const Geo& G, const double* e_body, const double* e_cand, const int* lo,
+                         const int* hi, double* out, int ls, const AbdParams& P, const int* active, SceneCtl c,
+                         int floor_accept, int* pending);
 void launch_pcg_tile_sum(hipStream_t s, int ns, const int* scene_tile0, const double* part, int stride, int which,
                          double* out, int do_sqrt);
 template <typename T>
@@ -244,8 +248,6 @@ void launch_ccd(hipStream_t s, const Geo& G, Cnt n, const uint64_t* keys, const 
                 const AbdParams& P, unsigned long long* amax_bits);
 void launch_body_energy(hipStream_t s, const Geo& G, const AbdParams& P, const double* q, const double* du,
                         const double* alpha_scene, const double* qtilde, double* e_body);
-void launch_ls_init(hipStream_t s, int ns, const AbdParams& P, const int* newton_active, SceneCtl c);
-void launch_ls_decide(hipStream_t s, int ns, const AbdParams& P, SceneCtl c, int floor_accept, int* pending);
 void launch_predict_du(hipStream_t s, const Geo& G, int mode, double dt, const double* q, const double* qnow,
                        const double* qdot, const double* qtilde, double* du);
 void launch_prev_x0(hipStream_t s, const Geo& G, const double* alpha, const double* du, double* x0);
@@ -259,8 +261,6 @@ void launch_scene_kmax(hipStream_t s, const Geo& G, SceneCtl c);
 void launch_apply(hipStream_t s, const Geo& G, double* q, const double* du, const double* alpha, const int* active,
                   const int* skip);
 void launch_forward(hipStream_t s, int nb, double dt, const double* q, double* qnow, double* qdot);
-void launch_newton_check(hipStream_t s, int ns, const AbdParams& P, SceneCtl c, int* active_before,
-                         const int* skip);
 // direct solver (abd_direct.hip): per-scene dense Cholesky of the assembled system
 constexpr int DIRECT_TILE = 32;        // tile edge; a scene's matrix order is padded to a multiple
 constexpr int DIRECT_MAX_DOFS = 6144;  // largest padded order (the solve keeps the vector in LDS)

@@ -188,16 +188,20 @@ void launch_pair_d2(hipStream_t s, const Geo& G, int n, const uint64_t* keys, co
     hipLaunchKernelGGL(k_pair_d2, dim3(cdiv(n, 256)), dim3(256), 0, s, G, n, keys, X, P, d2);
 }
 
+// out[pos[i]] = keys[i] where flag[i]; n_out (optional): the compacted length pos[n-1] + flag[n-1]
+// (0 for an empty input)
 __global__ void k_compact(Cnt cn, const int* __restrict__ flag, const int* __restrict__ pos,
-                          const uint64_t* __restrict__ keys, uint64_t* __restrict__ out) {
+                          const uint64_t* __restrict__ keys, uint64_t* __restrict__ out, int* n_out) {
     const int n = cnt_of(cn);
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n && flag[i]) out[pos[i]] = keys[i];
+    if (n_out && i == max(n - 1, 0)) *n_out = n > 0 ? pos[n - 1] + flag[n - 1] : 0;
 }
 
-void launch_compact(hipStream_t s, Cnt n, const int* flag, const int* pos, const uint64_t* keys, uint64_t* out) {
+void launch_compact(hipStream_t s, Cnt n, const int* flag, const int* pos, const uint64_t* keys, uint64_t* out,
+                    int* n_out) {
     if (n.cap <= 0) return;
-    hipLaunchKernelGGL(k_compact, dim3(cdiv(n.cap, 256)), dim3(256), 0, s, n, flag, pos, keys, out);
+    hipLaunchKernelGGL(k_compact, dim3(cdiv(n.cap, 256)), dim3(256), 0, s, n, flag, pos, keys, out, n_out);
 }
 
 // contact_hessian_ee (abd.py:198-571), one wave per active contact: d^2 gradient/Hessian,
@@ -460,32 +464,17 @@ __device__ __forceinline__ int lower_bound64(const uint64_t* a, int n, uint64_t 
 }
 
 // row_ptr[b] (b = 0..nb) = the first key with row (bits `shift`+) >= b, i.e. lower_bound(b << shift)
-// of the sorted keys, from the run heads: one thread per key writes the rows its run starts (the
-// rows since the previous key's row, empty rows included), the last key the rows after it; rows
-// past nb (invalid keys) count as nb.  No search, one store per row.  Rows: bodies of the CSR entry
-// and incidence keys (shift 32), scenes of the candidate keys (shift 48).
+// of the sorted keys, one thread per row (log2 n dependent loads; filling the rows from the run
+// heads instead leaves one thread a serial loop over every empty row of a gap).  Rows: bodies of
+// the CSR entry and incidence keys (shift 32), scenes of the candidate keys (shift 48).
 __global__ void k_row_ptr(int nb, Cnt cent, const uint64_t* __restrict__ k, int shift, int* row_ptr) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int n = cnt_of(cent);
-    if (n == 0) {
-        if (i == 0)
-            for (int b = 0; b <= nb; ++b) row_ptr[b] = 0;
-        return;
-    }
-    if (i >= n) return;
-    auto row = [&](int j) {
-        const uint64_t hi = k[j] >> shift;
-        return hi < (uint64_t)nb ? (int)hi : nb;
-    };
-    const int r = row(i), prev = i == 0 ? -1 : row(i - 1);
-    for (int b = prev + 1; b <= r; ++b) row_ptr[b] = i;
-    if (i == n - 1)
-        for (int b = r + 1; b <= nb; ++b) row_ptr[b] = n;
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b > nb) return;
+    row_ptr[b] = lower_bound64(k, cnt_of(cent), (uint64_t)b << shift);
 }
 
 void launch_row_ptr(hipStream_t s, int nb, Cnt n_keys, const uint64_t* sorted, int* row_ptr, int shift) {
-    hipLaunchKernelGGL(k_row_ptr, dim3(cdiv(std::max(n_keys.cap, 1), 256)), dim3(256), 0, s, nb, n_keys, sorted,
-                       shift, row_ptr);
+    hipLaunchKernelGGL(k_row_ptr, dim3(cdiv(nb + 1, 256)), dim3(256), 0, s, nb, n_keys, sorted, shift, row_ptr);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -652,9 +641,16 @@ void launch_body_assemble(hipStream_t s, const Geo& G, const AbdParams& P, const
 }
 
 // per-scene sum of body energies and candidate energies (fixed tree order)
+__device__ void ls_init_scene(int s, const AbdParams& P, const int* active, const SceneCtl& c);
+__device__ void ls_decide_scene(int s, const AbdParams& P, const SceneCtl& c, int floor_accept, int* pending);
+
+// per-scene energy sum; ls (optional, after the sum, by the scene's thread 0): 1 = the line search's
+// start (k_ls_init), 2 = one trial's decision (k_ls_decide)
 __global__ __launch_bounds__(256) void k_scene_sum(Geo G, const double* __restrict__ e_body,
                                                     const double* __restrict__ e_cand, const int* __restrict__ lo,
-                                                    const int* __restrict__ hi, double* __restrict__ out) {
+                                                    const int* __restrict__ hi, double* __restrict__ out, int ls,
+                                                    AbdParams P, const int* active, SceneCtl c, int floor_accept,
+                                                    int* pending) {
     int s = blockIdx.x;
     double acc = 0.0;
     for (int b = G.scene_b0[s] + threadIdx.x; b < G.scene_b0[s + 1]; b += 256) acc += e_body[b];
@@ -667,12 +663,24 @@ __global__ __launch_bounds__(256) void k_scene_sum(Geo G, const double* __restri
         if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
         __syncthreads();
     }
-    if (threadIdx.x == 0) out[s] = sh[0];
+    if (threadIdx.x == 0) {
+        out[s] = sh[0];
+        if (ls == 1) ls_init_scene(s, P, active, c);
+        if (ls == 2) ls_decide_scene(s, P, c, floor_accept, pending);
+    }
 }
 
 void launch_scene_sum(hipStream_t s, const Geo& G, const double* e_body, const double* e_cand, const int* lo,
                       const int* hi, double* out) {
-    hipLaunchKernelGGL(k_scene_sum, dim3(G.ns), dim3(256), 0, s, G, e_body, e_cand, lo, hi, out);
+    hipLaunchKernelGGL(k_scene_sum, dim3(G.ns), dim3(256), 0, s, G, e_body, e_cand, lo, hi, out, 0, AbdParams{},
+                       (const int*)nullptr, SceneCtl{}, 0, (int*)nullptr);
+}
+
+void launch_scene_sum_ls(hipStream_t s, const Geo& G, const double* e_body, const double* e_cand, const int* lo,
+                         const int* hi, double* out, int ls, const AbdParams& P, const int* active, SceneCtl c,
+                         int floor_accept, int* pending) {
+    hipLaunchKernelGGL(k_scene_sum, dim3(G.ns), dim3(256), 0, s, G, e_body, e_cand, lo, hi, out, ls, P, active, c,
+                       floor_accept, pending);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1798,9 +1806,7 @@ void launch_body_energy(hipStream_t s, const Geo& G, const AbdParams& P, const d
 }
 
 // line_search_iterative (gauss_newton.py:98-115) per scene
-__global__ void k_ls_init(int ns, AbdParams P, const int* __restrict__ active, SceneCtl c) {
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= ns) return;
+__device__ void ls_init_scene(int s, const AbdParams& P, const int* active, const SceneCtl& c) {
     double amax = P.ccd_enabled ? fmin(1.0, __longlong_as_double((long long)c.amax_bits[s])) : 1.0;
     c.alpha0[s] = amax;
     c.alpha[s] = active[s] ? amax : 0.0;
@@ -1808,16 +1814,12 @@ __global__ void k_ls_init(int ns, AbdParams P, const int* __restrict__ active, S
     c.ls_floor[s] = 0;
 }
 
-void launch_ls_init(hipStream_t s, int ns, const AbdParams& P, const int* newton_active, SceneCtl c) {
-    hipLaunchKernelGGL(k_ls_init, dim3(cdiv(ns, 128)), dim3(128), 0, s, ns, P, newton_active, c);
-}
 
 // Accept the first halving step that lowers the energy.  At the floor without a decrease the
 // Newton line search keeps the floor step (gauss_newton.py:110-111) unless its energy is not finite
 // (a trial through the ground plane), the predictor's search takes no step (floor_accept 0).
-__global__ void k_ls_decide(int ns, AbdParams P, SceneCtl c, int floor_accept, int* pending) {
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= ns || c.ls_done[s]) return;
+__device__ void ls_decide_scene(int s, const AbdParams& P, const SceneCtl& c, int floor_accept, int* pending) {
+    if (c.ls_done[s]) return;
     if (c.E1[s] < c.E0[s]) {
         c.ls_done[s] = 1;
     } else if (c.alpha[s] * 0.5 < P.ls_floor * c.alpha0[s]) {
@@ -1833,9 +1835,6 @@ __global__ void k_ls_decide(int ns, AbdParams P, SceneCtl c, int floor_accept, i
     }
 }
 
-void launch_ls_decide(hipStream_t s, int ns, const AbdParams& P, SceneCtl c, int floor_accept, int* pending) {
-    hipLaunchKernelGGL(k_ls_decide, dim3(cdiv(ns, 128)), dim3(128), 0, s, ns, P, c, floor_accept, pending);
-}
 
 // Direction toward the learned predicted first iterate q_now + dq, dq = dt qdot + de-standardised
 // model output (the model predicts the step's correction over the extrapolation); pinned bodies and
@@ -1961,22 +1960,6 @@ void launch_forward(hipStream_t s, int nb, double dt, const double* q, double* q
     hipLaunchKernelGGL(k_forward, dim3(cdiv(12 * nb, 256)), dim3(256), 0, s, 12 * nb, dt, q, qnow, qdot);
 }
 
-// abd.py:699: stop once the residual of the pre-update iterate is below tol
-// stop converged scenes (abd.py:699); the flags before the check go to active_before
-__global__ void k_newton_check(int ns, AbdParams P, SceneCtl c, int* active_before, const int* __restrict__ skip) {
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= ns || (skip && *skip)) return;
-    const int a = c.newton_active[s];
-    active_before[s] = a;
-    // converged (abd.py:699), or, with stall_alpha > 0, the step just taken was pinned below it
-    if (a && (c.residual[s] < P.newton_tol || (P.stall_alpha > 0.0 && c.alpha[s] < P.stall_alpha)))
-        c.newton_active[s] = 0;
-}
-
-void launch_newton_check(hipStream_t s, int ns, const AbdParams& P, SceneCtl c, int* active_before,
-                         const int* skip) {
-    hipLaunchKernelGGL(k_newton_check, dim3(cdiv(ns, 128)), dim3(128), 0, s, ns, P, c, active_before, skip);
-}
 
 // get_contact_points (abd.py:759-766): midpoint of the closest points and the distance
 __global__ void k_contact_points(Geo G, int n, const uint64_t* __restrict__ keys, const double4* __restrict__ X,
