@@ -42,7 +42,7 @@ class Params:
     hess_proj: int = 0  # contact Hessian projection: 0 = clamp eigenvalues at 0, 1 = absolute values
     k_max: float = 0.0  # adaptive contact stiffness cap (<= k_contact: fixed stiffness)
     k_raise_gap: float = 0.5  # raise the stiffness when a contact gets closer than this * dhat
-    cand_margin: float = None  # candidate-cache margin (length); None: 4 dhat; <= 0: no cache
+    cand_margin: float = None  # candidate-cache margin (length); None: 8 dhat; <= 0: no cache
     pcg_x0: int = 0  # PCG initial guess: 0 = zero (the reference), 1 = the untaken part of the previous direction
     stall_alpha: float = 0.0  # > 0: a scene stops iterating after an accepted step alpha below it (0: the reference)
     debug: int = 0  # bit 0: synchronise and check after every launch; bit 1: also log them;
@@ -80,7 +80,7 @@ class Params:
         p.hess_proj = self.hess_proj
         p.k_max = self.k_max
         p.k_raise_gap = self.k_raise_gap
-        p.cand_margin = 4.0 * self.dhat if self.cand_margin is None else self.cand_margin
+        p.cand_margin = 8.0 * self.dhat if self.cand_margin is None else self.cand_margin
         p.pcg_x0 = self.pcg_x0
         p.stall_alpha = self.stall_alpha
         return p
