@@ -117,3 +117,18 @@ def test_profiler_npz_round_trip(tmp_path):
     q.load(str(path))
     np.testing.assert_array_equal(q.convergence, p.convergence)
     np.testing.assert_array_equal(q.alphas, p.alphas)
+
+
+@pytest.mark.parametrize("m,precision,n", [(5, 0, 1), (12, 2, 1), (3, 0, 0), (16, 1, 4)])
+def test_psd_projection_check_rejects_invalid_arguments(m, precision, n):
+    """abd_psd_project_test validates its arguments before it touches the device (no GPU call is
+    made): orders other than 3, 6, 9, 12, a precision other than 0 or 1, no matrices."""
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("HIP library not built")
+    import ctypes as C
+    import numpy as np
+    A = np.zeros(max(n, 1) * m * m)
+    out = np.zeros_like(A)
+    P = C.POINTER(C.c_double)
+    rc = _lib.load().abd_psd_project_test(m, precision, 0, n, A.ctypes.data_as(P), out.ctypes.data_as(P))
+    assert rc == -2  # ABD_ERR_ARG
