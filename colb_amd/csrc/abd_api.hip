@@ -866,8 +866,14 @@ int assemble(AbdCtx* ctx) {
     tic(ctx, "block_sort");
     // incidence keys body << 32 | contact << 1 | role; body-pair keys lo << 44 | hi << 24 | contact
     const uint64_t bbits = low_bits(bits_for(ctx->nb)), cbits = low_bits(bits_for(std::max(ctx->act_cap, 1)));
-    CHK(sort_keys(ctx, ctx->inc, ctx->inc_s, inc_cnt(ctx), (bbits << 32) | (cbits << 1) | 1ull));
-    CHK(sort_keys(ctx, ctx->pk, ctx->pk_s, cpk, (bbits << 44) | (bbits << 24) | cbits));
+    if (inc_cnt(ctx).cap <= SORT_SMALL_MAX_KEYS && cpk.cap <= SORT_SMALL_MAX_KEYS) {
+        // both lists fit the counting-rank sort: one launch for the two
+        HIPCHK(small_sort2(ctx->st, ctx->inc, ctx->inc_s, inc_cnt(ctx), ctx->pk, ctx->pk_s, cpk));
+        CHK(dbg(ctx, "small_sort2"));
+    } else {
+        CHK(sort_keys(ctx, ctx->inc, ctx->inc_s, inc_cnt(ctx), (bbits << 32) | (cbits << 1) | 1ull));
+        CHK(sort_keys(ctx, ctx->pk, ctx->pk_s, cpk, (bbits << 44) | (bbits << 24) | cbits));
+    }
     toc(ctx);
     launch_row_ptr(ctx->st, ctx->nb, inc_cnt(ctx), ctx->inc_s, ctx->inc_ptr);
     CHK(dbg(ctx, "launch_row_ptr"));

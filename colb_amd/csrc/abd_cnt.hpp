@@ -18,4 +18,7 @@ __device__ __forceinline__ int cnt_of(const Cnt& c) {
     return n < 0 ? 0 : (n > c.cap ? c.cap : n);
 }
 
+// the longest list the single-launch counting-rank sort takes (abd_sort.hip)
+constexpr int SORT_SMALL_MAX_KEYS = 8192;
+
 }  // namespace abd

@@ -147,6 +147,9 @@ size_t radix_scratch_bytes(long n);
 // sig: the bits that may be set in any key (stable sort over those bits)
 hipError_t radix_sort64(hipStream_t s, const uint64_t* kin, uint64_t* kout, const int* vin, int* vout, Cnt n,
                         uint64_t sig, void* scratch);
+// two lists of at most SORT_SMALL_MAX_KEYS keys, sorted by one launch of the counting-rank sort
+hipError_t small_sort2(hipStream_t s, const uint64_t* ka, uint64_t* oa, Cnt na, const uint64_t* kb, uint64_t* ob,
+                       Cnt nb);
 
 void launch_qtilde(hipStream_t s, int nb, const double* qnow, const double* qdot, double dt, const double* g, double* qtilde);
 void launch_xform(hipStream_t s, const Geo& G, const double* q, const double* du, const double* alpha_scene, double4* X);

@@ -126,6 +126,7 @@ hipError_t excl_scan_int(hipStream_t s, const int* in, int* out, Cnt n, int* bsu
 // --------------------------------------------------------------------------------------------
 // lengths of at most SORT_SMALL_MAX are sorted by k_sort_small; the radix kernels then do nothing
 #define SORT_SMALL_MAX 8192
+static_assert(SORT_SMALL_MAX == SORT_SMALL_MAX_KEYS, "the small-sort bound the assembly checks");
 __global__ __launch_bounds__(SORT_T) void k_radix_hist(Cnt cn, const uint64_t* __restrict__ keys, int shift, int nblk,
                                                         int* __restrict__ hist) {
     __shared__ int cnt[RADIX];
@@ -198,10 +199,8 @@ __global__ __launch_bounds__(SORT_T) void k_radix_scatter(Cnt cn, const uint64_t
 // comparisons per lane; at one step a wave reads 16 consecutive keys, the same for its 4 keys).
 // Lengths above SORT_SMALL_MAX leave it to the radix passes.
 #define SORT_SMALL_KEYS 16
-__global__ __launch_bounds__(SORT_T) void k_sort_small(Cnt cn, const uint64_t* __restrict__ kin,
-                                                        const int* __restrict__ vin, uint64_t* __restrict__ kout,
-                                                        int* __restrict__ vout) {
-    __shared__ uint64_t kb[SORT_SMALL_MAX];
+__device__ __forceinline__ void sort_small_block(Cnt cn, const uint64_t* __restrict__ kin, const int* __restrict__ vin,
+                                                 uint64_t* __restrict__ kout, int* __restrict__ vout, uint64_t* kb) {
     const int n = cnt_of(cn);
     if (n > SORT_SMALL_MAX || blockIdx.x * SORT_SMALL_KEYS >= n) return;  // whole workgroup
     const int t = threadIdx.x;
@@ -221,6 +220,34 @@ __global__ __launch_bounds__(SORT_T) void k_sort_small(Cnt cn, const uint64_t* _
         kout[rank] = k;
         if (vout) vout[rank] = vin[i];
     }
+}
+
+__global__ __launch_bounds__(SORT_T) void k_sort_small(Cnt cn, const uint64_t* __restrict__ kin,
+                                                        const int* __restrict__ vin, uint64_t* __restrict__ kout,
+                                                        int* __restrict__ vout) {
+    __shared__ uint64_t kb[SORT_SMALL_MAX];
+    sort_small_block(cn, kin, vin, kout, vout, kb);
+}
+
+// two independent key lists in one launch (blockIdx.y picks the list)
+__global__ __launch_bounds__(SORT_T) void k_sort_small2(Cnt na, const uint64_t* __restrict__ ka,
+                                                         uint64_t* __restrict__ oa, Cnt nb,
+                                                         const uint64_t* __restrict__ kb_in, uint64_t* __restrict__ ob) {
+    __shared__ uint64_t kb[SORT_SMALL_MAX];
+    if (blockIdx.y == 0)
+        sort_small_block(na, ka, nullptr, oa, nullptr, kb);
+    else
+        sort_small_block(nb, kb_in, nullptr, ob, nullptr, kb);
+}
+
+// two key lists of at most SORT_SMALL_MAX keys each (host bounds), sorted by one launch
+hipError_t small_sort2(hipStream_t s, const uint64_t* ka, uint64_t* oa, Cnt na, const uint64_t* kb, uint64_t* ob,
+                       Cnt nb) {
+    const int cap = std::max(na.cap, nb.cap);
+    if (cap <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_sort_small2, dim3(cdivs(cap, SORT_SMALL_KEYS), 2), dim3(SORT_T), 0, s, na, ka, oa, nb, kb,
+                       ob);
+    return hipGetLastError();
 }
 
 // scratch for radix_sort64: tmp keys (n), tmp vals (n, if values), hist 2*RADIX*nblk ints + scan

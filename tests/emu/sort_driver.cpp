@@ -1,5 +1,6 @@
 // Emulator driver for the radix sort and scan of colb_amd/csrc/abd_sort.hip (TEST ONLY).
 #include <algorithm>
+#include <array>
 #include <cstdio>
 #include <numeric>
 #include <random>
@@ -10,6 +11,8 @@ namespace abd {
 hipError_t excl_scan_int(hipStream_t s, const int* in, int* out, Cnt n, int* bsum);
 int scan_scratch_ints(long n);
 size_t radix_scratch_bytes(long n);
+hipError_t small_sort2(hipStream_t s, const uint64_t* ka, uint64_t* oa, Cnt na, const uint64_t* kb, uint64_t* ob,
+                       Cnt nb);
 hipError_t radix_sort64(hipStream_t s, const uint64_t* kin, uint64_t* kout, const int* vin, int* vout, Cnt n,
                         uint64_t sig, void* scratch);
 }  // namespace abd
@@ -106,6 +109,34 @@ int main() {
             fails += !ok;
             hipFree(dk); hipFree(dko); hipFree(dv); hipFree(dvo); hipFree(dn); hipFree(tmp);
         }
+    }
+    // two lists in one launch (device lengths below the host bounds, one list empty, one at the limit)
+    for (auto [ca, na, cb, nbk] : std::vector<std::array<int, 4>>{{4096, 3134, 2048, 1567}, {100, 0, 8192, 8192},
+                                                                  {8192, 8000, 64, 1}}) {
+        std::vector<uint64_t> ka(ca, ~0ull), kb(cb, ~0ull);
+        for (int i = 0; i < na; ++i) ka[i] = rng() & ((1ull << 40) - 1);
+        for (int i = 0; i < nbk; ++i) kb[i] = rng() & ((1ull << 52) - 1);
+        uint64_t *da, *dao, *db, *dbo;
+        int* dn;
+        hipMalloc(&da, ca * 8); hipMalloc(&dao, ca * 8); hipMalloc(&db, cb * 8); hipMalloc(&dbo, cb * 8);
+        hipMalloc(&dn, 8);
+        int nn[2] = {na, nbk};
+        hipMemcpy(da, ka.data(), ca * 8, hipMemcpyHostToDevice);
+        hipMemcpy(db, kb.data(), cb * 8, hipMemcpyHostToDevice);
+        hipMemcpy(dn, nn, 8, hipMemcpyHostToDevice);
+        abd::small_sort2(nullptr, da, dao, abd::Cnt{ca, dn}, db, dbo, abd::Cnt{cb, dn + 1});
+        std::vector<uint64_t> oa(ca), ob(cb);
+        hipMemcpy(oa.data(), dao, ca * 8, hipMemcpyDeviceToHost);
+        hipMemcpy(ob.data(), dbo, cb * 8, hipMemcpyDeviceToHost);
+        std::vector<uint64_t> ea(ka.begin(), ka.begin() + na), eb(kb.begin(), kb.begin() + nbk);
+        std::sort(ea.begin(), ea.end());
+        std::sort(eb.begin(), eb.end());
+        bool ok = true;
+        for (int i = 0; i < na; ++i) ok &= oa[i] == ea[i];
+        for (int i = 0; i < nbk; ++i) ok &= ob[i] == eb[i];
+        std::printf("two lists %d/%d, %d/%d %s\n", na, ca, nbk, cb, ok ? "ok" : "FAIL");
+        fails += !ok;
+        hipFree(da); hipFree(dao); hipFree(db); hipFree(dbo); hipFree(dn);
     }
     // multi-level scan: more than SORT_TILE^2 elements
     {
