@@ -53,7 +53,7 @@ def main():
     for prec in (0, 1):
         e = 0.0
         for m in (3, 6, 9, 12):
-            A = psd_cases(m, seed=m, n=8)
+            A = psd_cases(m, seed=m, n=3)
             for absolute in (False, True):
                 out = hip_psd_project(_lib.load(), A, prec, absolute)
                 e = max(e, max(np.abs(out[i] - eig_projection(A[i], absolute)).max() / np.linalg.norm(A[i])

@@ -147,7 +147,7 @@ def test_stall_equals_the_loop_cut_at_that_iteration(oracle_mod):
     p = north_star(dhat=1e-3, k_contact=1e20)
     orc = oracle_mod.Oracle(sc, p.replace(stall_alpha=1e-6))
     stalls = 0
-    for _ in range(30):
+    for _ in range(18):  # the first stall comes at step 15
         q, qn, qd = orc.get_state()
         st = orc.step()[0]
         if not st["stalled"]:
