@@ -507,6 +507,13 @@ __global__ __launch_bounds__(64) void k_body_assemble(Geo G, AbdParams P, const 
         if (lane == 0) e_body[b] = 0.0;
         return;
     }
+    if (!newton_active[G.bscene[b]]) {
+        // a scene whose Newton loop has stopped needs only a zero gradient from this assembly: its
+        // PCG ends at once on b = 0 and its line search takes no step, so its blocks, inverse and
+        // energy keep the values of its last iteration (in a batch most scenes stop early)
+        if (lane < 12) grad[12 * b + lane] = 0.0;
+        return;
+    }
     const double dt2 = P.dt * P.dt;
     const double kap = P.kappa;
     const double* M = G.mass4 + 16 * b;
