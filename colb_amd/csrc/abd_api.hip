@@ -745,7 +745,9 @@ int cached_broad_phase(AbdCtx* ctx, const double4* X0, const double4* X1, const 
     if (ctx->cache_valid) {
         tic(ctx, "cache_check", (double)ctx->nv * 128);
         if (!ctx->iter_reset) CHK(fill_i32(ctx, ctx->cnt + C_OUTSIDE, 1, 0));
-        launch_cache_contains(ctx->st, ctx->nv, ctx->Xc0, ctx->Xc1, X0, X1, 0.5 * m, ctx->cnt + C_OUTSIDE);
+        // batches skip the vertices of scenes whose Newton loop has stopped (one scene: nothing to skip)
+        launch_cache_contains(ctx->st, ctx->G, ctx->Xc0, ctx->Xc1, X0, X1, 0.5 * m,
+                              ctx->ns > 1 ? ctx->C.newton_active : nullptr, ctx->cnt + C_OUTSIDE);
         CHK(dbg(ctx, "launch_cache_contains"));
         toc(ctx);
         if (!speculative) {

@@ -541,11 +541,15 @@ void launch_ground_queries(hipStream_t s, const Geo& G, const double4* X0, const
 // box, the other half covering rounding) needs no traversal: the cached list, filtered by the exact
 // rule above, is its candidate list, in the same sorted order.
 // ---------------------------------------------------------------------------------------------
-__global__ void k_cache_contains(int nv, const double4* __restrict__ Xc0, const double4* __restrict__ Xc1,
+// active (optional, per scene): vertices of scenes whose Newton loop has stopped are skipped: their
+// path has not moved since it was last found inside (the accepted point of a checked path lies in
+// that path's box, and a stopped scene's direction is zero)
+__global__ void k_cache_contains(Geo G, const double4* __restrict__ Xc0, const double4* __restrict__ Xc1,
                                  const double4* __restrict__ X0, const double4* __restrict__ X1, double m,
-                                 int* __restrict__ outside) {
+                                 const int* __restrict__ active, int* __restrict__ outside) {
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= nv) return;
+    if (v >= G.nv) return;
+    if (active && !active[G.bscene[(int)G.rest[v].w]]) return;
     const double4 c0 = Xc0[v], c1 = Xc1[v], a = X0[v], b = X1[v];
     const double cl[3] = {fmin(c0.x, c1.x) - m, fmin(c0.y, c1.y) - m, fmin(c0.z, c1.z) - m};
     const double ch[3] = {fmax(c0.x, c1.x) + m, fmax(c0.y, c1.y) + m, fmax(c0.z, c1.z) + m};
@@ -556,10 +560,11 @@ __global__ void k_cache_contains(int nv, const double4* __restrict__ Xc0, const 
     if (!in) atomicOr(outside, 1);
 }
 
-void launch_cache_contains(hipStream_t s, int nv, const double4* Xc0, const double4* Xc1, const double4* X0,
-                           const double4* X1, double m, int* outside) {
-    if (nv <= 0) return;
-    hipLaunchKernelGGL(k_cache_contains, dim3(cdivb(nv, 256)), dim3(256), 0, s, nv, Xc0, Xc1, X0, X1, m, outside);
+void launch_cache_contains(hipStream_t s, const Geo& G, const double4* Xc0, const double4* Xc1, const double4* X0,
+                           const double4* X1, double m, const int* active, int* outside) {
+    if (G.nv <= 0) return;
+    hipLaunchKernelGGL(k_cache_contains, dim3(cdivb(G.nv, 256)), dim3(256), 0, s, G, Xc0, Xc1, X0, X1, m, active,
+                       outside);
 }
 
 // flag[c] = 1 when cached pair c meets the exact candidate rule on X0 -> X1 (ground pairs: the

@@ -162,8 +162,8 @@ void launch_body_pair_write(hipStream_t s, const Geo& G, const float* box, const
 void launch_pair_queries(hipStream_t s, const Geo& G, const BodyMesh& M, int npairs, int max_queries,
                          const int2* pairs, const double4* X0, const double4* X1, const float* bbox, double infl,
                          uint64_t* cand, int cap, int* count, int* overflow, int box_only);
-void launch_cache_contains(hipStream_t s, int nv, const double4* Xc0, const double4* Xc1, const double4* X0,
-                           const double4* X1, double m, int* outside);
+void launch_cache_contains(hipStream_t s, const Geo& G, const double4* Xc0, const double4* Xc1, const double4* X0,
+                           const double4* X1, double m, const int* active, int* outside);
 void launch_cache_filter(hipStream_t s, const Geo& G, Cnt n, const uint64_t* keys, const double4* X0,
                          const double4* X1, const AbdParams& P, double infl, int* flag);
 void launch_ground_queries(hipStream_t s, const Geo& G, const double4* X0, const double4* X1, const AbdParams& P,
