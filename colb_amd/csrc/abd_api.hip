@@ -1428,10 +1428,12 @@ int ccd_bound(AbdCtx* ctx, bool speculative = false, bool x0_current = false) {
 
 // the line search's start: E0 at q over the swept candidate list, alpha0 from the CCD bound
 int ls_begin(AbdCtx* ctx) {
-    launch_body_energy(ctx->st, ctx->G, ctx->P, ctx->q, nullptr, nullptr, ctx->qtilde, ctx->e_body);
+    // batches leave out the energies of scenes whose Newton loop has stopped (their search is decided)
+    const int* act = ctx->ns > 1 ? ctx->C.newton_active : nullptr;
+    launch_body_energy(ctx->st, ctx->G, ctx->P, ctx->q, nullptr, nullptr, ctx->qtilde, ctx->e_body, act);
     CHK(dbg(ctx, "launch_body_energy"));
     tic(ctx, "energy", (double)ctx->nc_bound * (8 + 4 * 32 + 8));
-    launch_cand_eval(ctx->st, ctx->G, cand_cnt(ctx), ctx->cand_s, ctx->X0, ctx->P, nullptr, ctx->e_cand);
+    launch_cand_eval(ctx->st, ctx->G, cand_cnt(ctx), ctx->cand_s, ctx->X0, ctx->P, nullptr, ctx->e_cand, act);
     CHK(dbg(ctx, "launch_cand_eval"));
     toc(ctx);
     launch_scene_sum_ls(ctx->st, ctx->G, ctx->e_body, ctx->e_cand, ctx->cand_lo, ctx->cand_hi, ctx->C.E0, 1, ctx->P,
@@ -1446,10 +1448,11 @@ int ls_trial(AbdCtx* ctx, int floor_accept) {
     launch_xform(ctx->st, ctx->G, ctx->q, ctx->du, ctx->C.alpha, ctx->Xt);
     CHK(dbg(ctx, "launch_xform"));
     tic(ctx, "energy", (double)ctx->nc_bound * (8 + 4 * 32 + 8));
-    launch_cand_eval(ctx->st, ctx->G, cand_cnt(ctx), ctx->cand_s, ctx->Xt, ctx->P, nullptr, ctx->e_cand);
+    const int* act = ctx->ns > 1 ? ctx->C.newton_active : nullptr;
+    launch_cand_eval(ctx->st, ctx->G, cand_cnt(ctx), ctx->cand_s, ctx->Xt, ctx->P, nullptr, ctx->e_cand, act);
     CHK(dbg(ctx, "launch_cand_eval"));
     toc(ctx);
-    launch_body_energy(ctx->st, ctx->G, ctx->P, ctx->q, ctx->du, ctx->C.alpha, ctx->qtilde, ctx->qtmp);
+    launch_body_energy(ctx->st, ctx->G, ctx->P, ctx->q, ctx->du, ctx->C.alpha, ctx->qtilde, ctx->qtmp, act);
     CHK(dbg(ctx, "launch_body_energy"));
     if (!ctx->iter_reset) CHK(fill_i32(ctx, ctx->cnt + C_PENDING, 1, 0));
     launch_scene_sum_ls(ctx->st, ctx->G, ctx->qtmp, ctx->e_cand, ctx->cand_lo, ctx->cand_hi, ctx->C.E1, 2, ctx->P,

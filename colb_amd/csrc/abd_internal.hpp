@@ -190,7 +190,7 @@ void launch_warm_select(hipStream_t s, const Geo& G, const double* nb_b, const d
                         double* rhs, double* x0, int* used);
 void launch_add_inplace(hipStream_t s, int n, double* y, const double* x);
 void launch_cand_eval(hipStream_t s, const Geo& G, Cnt n, const uint64_t* keys, const double4* X, const AbdParams& P,
-                      int* flag, double* energy);
+                      int* flag, double* energy, const int* active = nullptr);
 void launch_pair_d2(hipStream_t s, const Geo& G, int n, const uint64_t* keys, const double4* X, const AbdParams& P,
                     double* d2);
 void launch_compact(hipStream_t s, Cnt n, const int* flag, const int* pos, const uint64_t* keys, uint64_t* out,
@@ -250,7 +250,7 @@ void launch_pcg_any(hipStream_t s, int ns, int ntiles, const int* scene_tile0, c
 void launch_ccd(hipStream_t s, const Geo& G, Cnt n, const uint64_t* keys, const double4* X0, const double4* X1,
                 const AbdParams& P, unsigned long long* amax_bits);
 void launch_body_energy(hipStream_t s, const Geo& G, const AbdParams& P, const double* q, const double* du,
-                        const double* alpha_scene, const double* qtilde, double* e_body);
+                        const double* alpha_scene, const double* qtilde, double* e_body, const int* active = nullptr);
 void launch_predict_du(hipStream_t s, const Geo& G, int mode, double dt, const double* q, const double* qnow,
                        const double* qdot, const double* qtilde, double* du);
 void launch_prev_x0(hipStream_t s, const Geo& G, const double* alpha, const double* du, double* x0);

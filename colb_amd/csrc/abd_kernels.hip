@@ -145,12 +145,21 @@ __device__ __forceinline__ double cand_d2(const Geo& G, const PairV& p, const do
     return pair_d2(p.kind, x);
 }
 
-// energy_contact (abd.py:66-75): dt^2 k B(d^2) for active pairs (d < dhat)
+// energy_contact (abd.py:66-75): dt^2 k B(d^2) for active pairs (d < dhat).  active (optional, per
+// scene, energies only): candidates of scenes whose Newton loop has stopped get energy 0 (their
+// line search is decided already)
 __global__ void k_cand_eval(Geo G, Cnt cn, const uint64_t* __restrict__ keys, const double4* __restrict__ X, AbdParams P,
-                            int* __restrict__ flag, double* __restrict__ energy) {
+                            int* __restrict__ flag, double* __restrict__ energy, const int* __restrict__ active) {
     const int n = cnt_of(cn);
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    if (active) {
+        const int sc = dkey_scene(keys[i]);
+        if (sc >= 0 && sc < G.ns && !active[sc]) {
+            energy[i] = 0.0;
+            return;
+        }
+    }
     PairV p = decode(G, keys[i]);
     if (p.kind < 0) {
         if (flag) flag[i] = 0;
@@ -169,9 +178,10 @@ __global__ void k_cand_eval(Geo G, Cnt cn, const uint64_t* __restrict__ keys, co
 }
 
 void launch_cand_eval(hipStream_t s, const Geo& G, Cnt n, const uint64_t* keys, const double4* X, const AbdParams& P,
-                      int* flag, double* energy) {
+                      int* flag, double* energy, const int* active) {
     if (n.cap <= 0) return;
-    hipLaunchKernelGGL(k_cand_eval, dim3(cdiv(n.cap, 256)), dim3(256), 0, s, G, n, keys, X, P, flag, energy);
+    hipLaunchKernelGGL(k_cand_eval, dim3(cdiv(n.cap, 256)), dim3(256), 0, s, G, n, keys, X, P, flag, energy,
+                       flag ? (const int*)nullptr : active);
 }
 
 __global__ void k_pair_d2(Geo G, int n, const uint64_t* __restrict__ keys, const double4* __restrict__ X, AbdParams P,
@@ -1789,10 +1799,11 @@ void launch_ccd(hipStream_t s, const Geo& G, Cnt n, const uint64_t* keys, const 
 // energy_inertia (abd.py:53-64) at q - alpha_s du
 __global__ void k_body_energy(Geo G, AbdParams P, const double* __restrict__ q, const double* __restrict__ du,
                               const double* __restrict__ alpha, const double* __restrict__ qtilde,
-                              double* __restrict__ e_body) {
+                              double* __restrict__ e_body, const int* __restrict__ active) {
     int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= G.nb) return;
-    if (G.pinned[b]) { e_body[b] = 0.0; return; }
+    // pinned bodies, and (with `active`) bodies of scenes whose Newton loop has stopped: 0
+    if (G.pinned[b] || (active && !active[G.bscene[b]])) { e_body[b] = 0.0; return; }
     double a = alpha ? alpha[G.bscene[b]] : 0.0;
     double qb[12], dq[12];
     for (int k = 0; k < 12; ++k) {
@@ -1808,8 +1819,9 @@ __global__ void k_body_energy(Geo G, AbdParams P, const double* __restrict__ q, 
 }
 
 void launch_body_energy(hipStream_t s, const Geo& G, const AbdParams& P, const double* q, const double* du,
-                        const double* alpha, const double* qtilde, double* e_body) {
-    hipLaunchKernelGGL(k_body_energy, dim3(cdiv(G.nb, 128)), dim3(128), 0, s, G, P, q, du, alpha, qtilde, e_body);
+                        const double* alpha, const double* qtilde, double* e_body, const int* active) {
+    hipLaunchKernelGGL(k_body_energy, dim3(cdiv(G.nb, 128)), dim3(128), 0, s, G, P, q, du, alpha, qtilde, e_body,
+                       active);
 }
 
 // line_search_iterative (gauss_newton.py:98-115) per scene
