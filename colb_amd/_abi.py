@@ -91,6 +91,8 @@ class AbdStepStats(C.Structure):
         ("graph_launches", C.c_int),
         ("graph_captures", C.c_int),
         ("stalled", C.c_int),
+        ("ls_trials", C.c_int),
+        ("ls_round_trips", C.c_int),
     ]
 
     def as_dict(self):

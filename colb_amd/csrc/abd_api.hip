@@ -122,6 +122,7 @@ struct AbdCtx {
     uint64_t alloc_gen = 0;      // bumped by every device (re)allocation (captured arguments hold pointers)
     int* h_now = nullptr;        // pinned copy of the scenes' Newton flags
     int step_graph_launches = 0, step_captures = 0;
+    int step_ls_trials = 0, step_ls_syncs = 0;  // line-search trials launched, host round trips waiting for them
     // candidates: nc_bound bounds the current list's length (its kernels are sized by it), the
     // length itself is the device counter C_NCAND
     int cand_cap = 0, nc_bound = 0;
@@ -156,6 +157,7 @@ struct AbdCtx {
     bool pcg_unchecked = false;   // the last solve was persistent and its failure flag is not read yet
     bool force_iter_pcg = false;  // solve with one launch per iteration (redo after a failed persistent solve)
     int pcg_fallbacks = 0;        // solves of the current step rerun with one launch per iteration
+    int ls_trials = 2;            // line-search trials launched together before a host round trip
     int coop_blocks[2] = {-1, -1};  // co-resident k_pcg_persist workgroups (FP64, FP32); -1 = not queried
     int multi_blocks[2] = {-1, -1}; // co-resident k_pcg_multi workgroups (FP64, FP32); -1 = not queried
     bool pcg_multi = false;       // solve with one launch per iteration (autotuner choice)
@@ -595,7 +597,8 @@ int grow_candidates(AbdCtx* ctx, int cap) {
     CHK(dfree(ctx, ctx->cand)); CHK(dfree(ctx, ctx->cand_s)); CHK(dfree(ctx, ctx->act));
     CHK(dfree(ctx, ctx->flag)); CHK(dfree(ctx, ctx->pos)); CHK(dfree(ctx, ctx->e_cand));
     CHK(dalloc(ctx, &ctx->cand, cap)); CHK(dalloc(ctx, &ctx->cand_s, cap)); CHK(dalloc(ctx, &ctx->act, cap));
-    CHK(dalloc(ctx, &ctx->flag, cap)); CHK(dalloc(ctx, &ctx->pos, cap)); CHK(dalloc(ctx, &ctx->e_cand, cap));
+    CHK(dalloc(ctx, &ctx->flag, cap)); CHK(dalloc(ctx, &ctx->pos, cap));
+    CHK(dalloc(ctx, &ctx->e_cand, (size_t)LS_TRIALS_MAX * cap));  // one slice per line-search trial
     ctx->cand_cap = cap;
     return ABD_OK;
 }
@@ -1442,33 +1445,58 @@ int ls_begin(AbdCtx* ctx) {
     return ABD_OK;
 }
 
-// one trial of every undecided scene: E1 at q - alpha du, accept / halve / floor (C_PENDING: some
-// scene halved)
-int ls_trial(AbdCtx* ctx, int floor_accept) {
-    launch_xform(ctx->st, ctx->G, ctx->q, ctx->du, ctx->C.alpha, ctx->Xt);
+// `trials` consecutive trials of every undecided scene in one set of launches, the k-th at
+// q - 2^-k alpha du (the steps the halvings of one trial per launch would reach), decided in order:
+// accept / halve / floor (C_PENDING: some scene is still undecided after the last).  The decisions,
+// energies and steps are those of one trial per launch; trials past a scene's decisive one are
+// wasted work.  Xt holds the vertices of the k-th trial at Xt + k nv, so only with trials = 1 does
+// it hold every scene's accepted iterate afterwards (the adaptive stiffness reads it).
+int ls_trial(AbdCtx* ctx, int floor_accept, int trials = 1) {
+    if (trials < 1 || trials > LS_TRIALS_MAX) {
+        ctx->err = "line-search trials per launch out of range";
+        return ABD_ERR_ARG;
+    }
+    launch_xform(ctx->st, ctx->G, ctx->q, ctx->du, ctx->C.alpha, ctx->Xt, trials);
     CHK(dbg(ctx, "launch_xform"));
-    tic(ctx, "energy", (double)ctx->nc_bound * (8 + 4 * 32 + 8));
+    ctx->step_ls_trials += trials;
+    tic(ctx, "energy", (double)trials * ctx->nc_bound * (8 + 4 * 32 + 8));
     const int* act = ctx->ns > 1 ? ctx->C.newton_active : nullptr;
-    launch_cand_eval(ctx->st, ctx->G, cand_cnt(ctx), ctx->cand_s, ctx->Xt, ctx->P, nullptr, ctx->e_cand, act);
+    launch_cand_eval(ctx->st, ctx->G, cand_cnt(ctx), ctx->cand_s, ctx->Xt, ctx->P, nullptr, ctx->e_cand, act, trials,
+                     (size_t)ctx->cand_cap);
     CHK(dbg(ctx, "launch_cand_eval"));
     toc(ctx);
-    launch_body_energy(ctx->st, ctx->G, ctx->P, ctx->q, ctx->du, ctx->C.alpha, ctx->qtilde, ctx->qtmp, act);
+    // qtmp (12 nb doubles) holds the body energies of up to 12 trials
+    launch_body_energy(ctx->st, ctx->G, ctx->P, ctx->q, ctx->du, ctx->C.alpha, ctx->qtilde, ctx->qtmp, act, trials);
     CHK(dbg(ctx, "launch_body_energy"));
     if (!ctx->iter_reset) CHK(fill_i32(ctx, ctx->cnt + C_PENDING, 1, 0));
     launch_scene_sum_ls(ctx->st, ctx->G, ctx->qtmp, ctx->e_cand, ctx->cand_lo, ctx->cand_hi, ctx->C.E1, 2, ctx->P,
-                        nullptr, ctx->C, floor_accept, ctx->cnt + C_PENDING);
+                        nullptr, ctx->C, floor_accept, ctx->cnt + C_PENDING, trials, (size_t)ctx->cand_cap);
     CHK(dbg(ctx, "launch_scene_sum(ls_decide)"));
     HIPCHK(hipGetLastError());
     return ABD_OK;
 }
 
-// trials until every scene has decided (one host round trip each)
+// trials until every scene has decided, one host round trip per launch of trials.  The launches
+// after the first take ctx->ls_trials trials each (doubled after a launch that leaves a scene
+// undecided, lowered by one after a search that one launch finished), so a scene whose searches
+// need many halvings pays one round trip for them, and one that needs one pays for one trial.
+// With the adaptive stiffness on (it reads the accepted vertices in Xt) or debug bit 128, one trial
+// per launch.
 int ls_until_done(AbdCtx* ctx, int floor_accept, int first_trial) {
-    for (int trial = first_trial; trial < 64; ++trial) {
-        CHK(ls_trial(ctx, floor_accept));
+    const bool batch = !(ctx->P.k_max > ctx->P.k_contact) && !(ctx->P.reserved0 & 128);
+    int k = batch ? std::min(std::max(ctx->ls_trials, 1), LS_TRIALS_MAX) : 1;
+    int launches = 0;
+    for (int trial = first_trial; trial < 64;) {
+        const int n = std::min(k, 64 - trial);
+        CHK(ls_trial(ctx, floor_accept, n));
+        trial += n;
+        launches += 1;
+        ctx->step_ls_syncs += 1;
         CHK(sync_counters(ctx));
         if (!ctx->h_cnt[C_PENDING]) break;
+        if (batch) k = std::min(2 * k, LS_TRIALS_MAX);
     }
+    if (batch) ctx->ls_trials = launches > 1 ? k : std::max(1, ctx->ls_trials - 1);
     return ABD_OK;
 }
 
@@ -1791,6 +1819,7 @@ int step_once(AbdCtx* ctx) {
     ctx->pcg_fallbacks = 0;
     ctx->step_traversals = ctx->step_detections = ctx->step_slow = 0;
     ctx->step_graph_launches = ctx->step_captures = 0;
+    ctx->step_ls_trials = ctx->step_ls_syncs = 0;
     hipLaunchKernelGGL(k_fill_u64, dim3(cdiv(ctx->nb, 128)), dim3(128), 0, ctx->st, ctx->nb,
                        (unsigned long long*)ctx->C.dmin_prev, (unsigned long long)0x7FF0000000000000ull);  // +inf
     CHK(fill_i32(ctx, ctx->C.k_raises, ns, 0));
@@ -1837,6 +1866,8 @@ int step_once(AbdCtx* ctx) {
             ctx->stats[s].slow_iterations = ctx->step_slow;
             ctx->stats[s].graph_launches = ctx->step_graph_launches;
             ctx->stats[s].graph_captures = ctx->step_captures;
+            ctx->stats[s].ls_trials = ctx->step_ls_trials;
+            ctx->stats[s].ls_round_trips = ctx->step_ls_syncs;
             any |= now[s] != 0;
         }
         if (!any) break;
@@ -2175,7 +2206,7 @@ int abd_create(const AbdSceneDesc* d, const AbdParams* p, AbdCtx** out) {
     A(scene_t0, ns + 1); U(scene_t0, stt.data(), ns + 1);
     A(q, 12 * nb); A(qnow, 12 * nb); A(qdot, 12 * nb); A(qtilde, 12 * nb); A(qtmp, 12 * nb);
     A(du, 12 * nb); A(grad, 12 * nb); A(diag, 144 * nb); A(pinv, 144 * nb); A(e_body, nb);
-    A(X0, nv); A(X1, nv); A(Xt, nv); A(Xc0, nv); A(Xc1, nv);
+    A(X0, nv); A(X1, nv); A(Xt, (size_t)LS_TRIALS_MAX * nv); A(Xc0, nv); A(Xc1, nv);
     A(body_v0, nb + 1); U(body_v0, bv0.data(), nb + 1);
     A(body_e0, nb + 1); U(body_e0, be0.data(), nb + 1);
     A(body_t0, nb + 1); U(body_t0, bt0.data(), nb + 1);

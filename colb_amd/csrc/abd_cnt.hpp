@@ -20,5 +20,8 @@ __device__ __forceinline__ int cnt_of(const Cnt& c) {
 
 // the longest list the single-launch counting-rank sort takes (abd_sort.hip)
 constexpr int SORT_SMALL_MAX_KEYS = 8192;
+// line-search trials evaluated by one set of launches (the vertex and candidate-energy buffers of
+// the trials hold this many slices)
+constexpr int LS_TRIALS_MAX = 8;
 
 }  // namespace abd

@@ -152,7 +152,9 @@ hipError_t small_sort2(hipStream_t s, const uint64_t* ka, uint64_t* oa, Cnt na, 
                        Cnt nb);
 
 void launch_qtilde(hipStream_t s, int nb, const double* qnow, const double* qdot, double dt, const double* g, double* qtilde);
-void launch_xform(hipStream_t s, const Geo& G, const double* q, const double* du, const double* alpha_scene, double4* X);
+// trials > 1: X + k nv for k < trials holds the vertices at q - 2^-k alpha du (line-search trials)
+void launch_xform(hipStream_t s, const Geo& G, const double* q, const double* du, const double* alpha_scene, double4* X,
+                  int trials = 1);
 // abd_broad.hip
 void launch_body_boxes(hipStream_t s, const Geo& G, const BodyMesh& M, const double4* X0, const double4* X1,
                        double half_infl, float* box);
@@ -190,7 +192,7 @@ void launch_warm_select(hipStream_t s, const Geo& G, const double* nb_b, const d
                         double* rhs, double* x0, int* used);
 void launch_add_inplace(hipStream_t s, int n, double* y, const double* x);
 void launch_cand_eval(hipStream_t s, const Geo& G, Cnt n, const uint64_t* keys, const double4* X, const AbdParams& P,
-                      int* flag, double* energy, const int* active = nullptr);
+                      int* flag, double* energy, const int* active = nullptr, int trials = 1, size_t estride = 0);
 void launch_pair_d2(hipStream_t s, const Geo& G, int n, const uint64_t* keys, const double4* X, const AbdParams& P,
                     double* d2);
 void launch_compact(hipStream_t s, Cnt n, const int* flag, const int* pos, const uint64_t* keys, uint64_t* out,
@@ -216,7 +218,7 @@ void launch_scene_sum(hipStream_t s, const Geo& G, const double* e_body, const d
                       const int* cand_hi, double* out);
 void launch_scene_sum_ls(hipStream_t s, const Geo& G, const double* e_body, const double* e_cand, const int* lo,
                          const int* hi, double* out, int ls, const AbdParams& P, const int* active, SceneCtl c,
-                         int floor_accept, int* pending);
+                         int floor_accept, int* pending, int trials = 1, size_t estride = 0);
 void launch_pcg_tile_sum(hipStream_t s, int ns, const int* scene_tile0, const double* part, int stride, int which,
                          double* out, int do_sqrt);
 template <typename T>
@@ -250,7 +252,8 @@ void launch_pcg_any(hipStream_t s, int ns, int ntiles, const int* scene_tile0, c
 void launch_ccd(hipStream_t s, const Geo& G, Cnt n, const uint64_t* keys, const double4* X0, const double4* X1,
                 const AbdParams& P, unsigned long long* amax_bits);
 void launch_body_energy(hipStream_t s, const Geo& G, const AbdParams& P, const double* q, const double* du,
-                        const double* alpha_scene, const double* qtilde, double* e_body, const int* active = nullptr);
+                        const double* alpha_scene, const double* qtilde, double* e_body, const int* active = nullptr,
+                        int trials = 1);
 void launch_predict_du(hipStream_t s, const Geo& G, int mode, double dt, const double* q, const double* qnow,
                        const double* qdot, const double* qtilde, double* du);
 void launch_prev_x0(hipStream_t s, const Geo& G, const double* alpha, const double* du, double* x0);

@@ -48,15 +48,18 @@ void launch_qtilde(hipStream_t s, int nb, const double* qnow, const double* qdot
 }
 
 // compute_V (abd_simple.py:26-35) at q - alpha_s * du
+// blockIdx.y = k > 0 (line-search trials in one launch): the vertices at q - 2^-k alpha du into
+// X + k nv
 __global__ void k_xform(Geo G, const double* __restrict__ q, const double* __restrict__ du,
                         const double* __restrict__ alpha, double4* __restrict__ X) {
     int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= G.nv) return;
+    X += (size_t)blockIdx.y * G.nv;
     double4 r = G.rest[v];
     int b = (int)r.w;
     const double* qb = q + 12 * b;
     double a = 0.0;
-    if (du) a = alpha ? alpha[G.bscene[b]] : 1.0;
+    if (du) a = alpha ? ldexp(alpha[G.bscene[b]], -(int)blockIdx.y) : 1.0;
     double c[12];
 #pragma unroll
     for (int k = 0; k < 12; ++k) c[k] = du ? qb[k] - a * du[12 * b + k] : qb[k];
@@ -64,8 +67,9 @@ __global__ void k_xform(Geo G, const double* __restrict__ q, const double* __res
                         c[2] + r.x * c[5] + r.y * c[8] + r.z * c[11], 0.0);
 }
 
-void launch_xform(hipStream_t s, const Geo& G, const double* q, const double* du, const double* alpha, double4* X) {
-    hipLaunchKernelGGL(k_xform, dim3(cdiv(G.nv, 256)), dim3(256), 0, s, G, q, du, alpha, X);
+void launch_xform(hipStream_t s, const Geo& G, const double* q, const double* du, const double* alpha, double4* X,
+                  int trials) {
+    hipLaunchKernelGGL(k_xform, dim3(cdiv(G.nv, 256), trials), dim3(256), 0, s, G, q, du, alpha, X);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -148,11 +152,15 @@ __device__ __forceinline__ double cand_d2(const Geo& G, const PairV& p, const do
 // energy_contact (abd.py:66-75): dt^2 k B(d^2) for active pairs (d < dhat).  active (optional, per
 // scene, energies only): candidates of scenes whose Newton loop has stopped get energy 0 (their
 // line search is decided already)
+// blockIdx.y = k > 0 (line-search trials in one launch): vertices X + k nv, energies energy + k estride
 __global__ void k_cand_eval(Geo G, Cnt cn, const uint64_t* __restrict__ keys, const double4* __restrict__ X, AbdParams P,
-                            int* __restrict__ flag, double* __restrict__ energy, const int* __restrict__ active) {
+                            int* __restrict__ flag, double* __restrict__ energy, const int* __restrict__ active,
+                            size_t estride) {
     const int n = cnt_of(cn);
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    X += (size_t)blockIdx.y * G.nv;
+    if (energy) energy += (size_t)blockIdx.y * estride;
     if (active) {
         const int sc = dkey_scene(keys[i]);
         if (sc >= 0 && sc < G.ns && !active[sc]) {
@@ -178,10 +186,10 @@ __global__ void k_cand_eval(Geo G, Cnt cn, const uint64_t* __restrict__ keys, co
 }
 
 void launch_cand_eval(hipStream_t s, const Geo& G, Cnt n, const uint64_t* keys, const double4* X, const AbdParams& P,
-                      int* flag, double* energy, const int* active) {
+                      int* flag, double* energy, const int* active, int trials, size_t estride) {
     if (n.cap <= 0) return;
-    hipLaunchKernelGGL(k_cand_eval, dim3(cdiv(n.cap, 256)), dim3(256), 0, s, G, n, keys, X, P, flag, energy,
-                       flag ? (const int*)nullptr : active);
+    hipLaunchKernelGGL(k_cand_eval, dim3(cdiv(n.cap, 256), trials), dim3(256), 0, s, G, n, keys, X, P, flag, energy,
+                       flag ? (const int*)nullptr : active, estride);
 }
 
 __global__ void k_pair_d2(Geo G, int n, const uint64_t* __restrict__ keys, const double4* __restrict__ X, AbdParams P,
@@ -662,42 +670,56 @@ __device__ void ls_init_scene(int s, const AbdParams& P, const int* active, cons
 __device__ void ls_decide_scene(int s, const AbdParams& P, const SceneCtl& c, int floor_accept, int* pending);
 
 // per-scene energy sum; ls (optional, after the sum, by the scene's thread 0): 1 = the line search's
-// start (k_ls_init), 2 = one trial's decision (k_ls_decide)
+// start, 2 = the decisions of `trials` consecutive trials (the k-th at 2^-k alpha, its body energies
+// at e_body + k nb and candidate energies at e_cand + k estride), taken in order as one trial per
+// launch would take them: a scene decided before trial k (here or in an earlier launch) is left
+// alone, so its E1 stays the energy of its decisive trial.  `pending` reports the scenes still
+// undecided after the last trial.
 __global__ __launch_bounds__(256) void k_scene_sum(Geo G, const double* __restrict__ e_body,
                                                     const double* __restrict__ e_cand, const int* __restrict__ lo,
                                                     const int* __restrict__ hi, double* __restrict__ out, int ls,
                                                     AbdParams P, const int* active, SceneCtl c, int floor_accept,
-                                                    int* pending) {
-    int s = blockIdx.x;
-    double acc = 0.0;
-    for (int b = G.scene_b0[s] + threadIdx.x; b < G.scene_b0[s + 1]; b += 256) acc += e_body[b];
-    if (e_cand)
-        for (int i = lo[s] + threadIdx.x; i < hi[s]; i += 256) acc += e_cand[i];
+                                                    int* pending, int trials, size_t estride) {
+    const int s = blockIdx.x;
     __shared__ double sh[256];
-    sh[threadIdx.x] = acc;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __shared__ int more;  // halvings decided before the last trial (reported by the last one only)
+    const int nk = (ls == 2) ? trials : 1;
+    for (int k = 0; k < nk; ++k) {
+        // c.ls_done[s] is written by thread 0 before the barrier that ends the previous trial: the
+        // exit is uniform
+        if (ls == 2 && c.ls_done[s]) return;
+        const double* eb = e_body + (size_t)k * G.nb;
+        const double* ec = e_cand ? e_cand + (size_t)k * estride : nullptr;
+        double acc = 0.0;
+        for (int b = G.scene_b0[s] + threadIdx.x; b < G.scene_b0[s + 1]; b += 256) acc += eb[b];
+        if (ec)
+            for (int i = lo[s] + threadIdx.x; i < hi[s]; i += 256) acc += ec[i];
+        sh[threadIdx.x] = acc;
         __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        out[s] = sh[0];
-        if (ls == 1) ls_init_scene(s, P, active, c);
-        if (ls == 2) ls_decide_scene(s, P, c, floor_accept, pending);
+        for (int o = 128; o > 0; o >>= 1) {
+            if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            out[s] = sh[0];
+            if (ls == 1) ls_init_scene(s, P, active, c);
+            if (ls == 2) ls_decide_scene(s, P, c, floor_accept, k + 1 == nk ? pending : &more);
+        }
+        __syncthreads();
     }
 }
 
 void launch_scene_sum(hipStream_t s, const Geo& G, const double* e_body, const double* e_cand, const int* lo,
                       const int* hi, double* out) {
     hipLaunchKernelGGL(k_scene_sum, dim3(G.ns), dim3(256), 0, s, G, e_body, e_cand, lo, hi, out, 0, AbdParams{},
-                       (const int*)nullptr, SceneCtl{}, 0, (int*)nullptr);
+                       (const int*)nullptr, SceneCtl{}, 0, (int*)nullptr, 1, (size_t)0);
 }
 
 void launch_scene_sum_ls(hipStream_t s, const Geo& G, const double* e_body, const double* e_cand, const int* lo,
                          const int* hi, double* out, int ls, const AbdParams& P, const int* active, SceneCtl c,
-                         int floor_accept, int* pending) {
+                         int floor_accept, int* pending, int trials, size_t estride) {
     hipLaunchKernelGGL(k_scene_sum, dim3(G.ns), dim3(256), 0, s, G, e_body, e_cand, lo, hi, out, ls, P, active, c,
-                       floor_accept, pending);
+                       floor_accept, pending, trials, estride);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1802,9 +1824,10 @@ __global__ void k_body_energy(Geo G, AbdParams P, const double* __restrict__ q, 
                               double* __restrict__ e_body, const int* __restrict__ active) {
     int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= G.nb) return;
+    e_body += (size_t)blockIdx.y * G.nb;  // k = blockIdx.y: the trial at 2^-k alpha
     // pinned bodies, and (with `active`) bodies of scenes whose Newton loop has stopped: 0
     if (G.pinned[b] || (active && !active[G.bscene[b]])) { e_body[b] = 0.0; return; }
-    double a = alpha ? alpha[G.bscene[b]] : 0.0;
+    double a = alpha ? ldexp(alpha[G.bscene[b]], -(int)blockIdx.y) : 0.0;
     double qb[12], dq[12];
     for (int k = 0; k < 12; ++k) {
         qb[k] = du ? q[12 * b + k] - a * du[12 * b + k] : q[12 * b + k];
@@ -1819,9 +1842,9 @@ __global__ void k_body_energy(Geo G, AbdParams P, const double* __restrict__ q, 
 }
 
 void launch_body_energy(hipStream_t s, const Geo& G, const AbdParams& P, const double* q, const double* du,
-                        const double* alpha, const double* qtilde, double* e_body, const int* active) {
-    hipLaunchKernelGGL(k_body_energy, dim3(cdiv(G.nb, 128)), dim3(128), 0, s, G, P, q, du, alpha, qtilde, e_body,
-                       active);
+                        const double* alpha, const double* qtilde, double* e_body, const int* active, int trials) {
+    hipLaunchKernelGGL(k_body_energy, dim3(cdiv(G.nb, 128), trials), dim3(128), 0, s, G, P, q, du, alpha, qtilde,
+                       e_body, active);
 }
 
 // line_search_iterative (gauss_newton.py:98-115) per scene

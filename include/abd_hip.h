@@ -64,7 +64,8 @@ typedef struct AbdParams {
     int    reserved0;          /* debug bits (1: check every launch, 2: log launches, 4: print list
                                   sizes, 8: one PCG launch per iteration, 16: force the persistent
                                   PCG's fallback, 32: multi-tile persistent PCG where one tile per
-                                  wave fits, 64: no HIP graph capture of the Newton iteration) */
+                                  wave fits, 64: no HIP graph capture of the Newton iteration,
+                                  128: one line-search trial per host round trip) */
     int    solver;             /* 0 = block-Jacobi PCG (abd.py:633-636, "cg"); 1 = direct: dense
                                   Cholesky per scene (the sparse direct branch, abd.py:637-654);
                                   2 = PCG, then the direct solve for every scene whose PCG ended
@@ -144,6 +145,10 @@ typedef struct AbdStepStats {
     int    graph_launches;     /* Newton iterations launched as one replay of a captured HIP graph */
     int    graph_captures;     /* HIP graph captures of the Newton iteration during the step */
     int    stalled;            /* the Newton loop stopped at an accepted step below stall_alpha */
+    int    ls_trials;          /* line-search trials evaluated during the step (trials launched
+                                  together past a scene's decisive one included) */
+    int    ls_round_trips;     /* host round trips waiting for line-search trials (the first trial
+                                  of a fast Newton iteration comes back with its report) */
 } AbdStepStats;
 
 typedef struct AbdCtx AbdCtx;

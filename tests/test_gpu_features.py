@@ -554,3 +554,38 @@ def test_graph_replay_matches_eager_launches(case):
     assert t["calls"] >= iters and t["ms"] > 0 and t["bytes"] > 0
     g.close()
     e.close()
+
+
+@pytest.mark.parametrize("case", ["cubes_8", "cube_batch"])
+def test_line_search_trials_launched_together_give_the_same_bits(case):
+    """Line-search trials launched several at a time (the k-th at 2^-k of the step, decided in
+    order as one trial per launch decides them) give every step bit for bit as one trial per host
+    round trip (debug bit 128), with fewer round trips where the searches halve several times.  The
+    8-cube pile at d^ 1e-3, kappa 1e7 halves about four times per Newton iteration once it lands;
+    the batch mixes scenes whose searches end after different trials of one launch."""
+    if case == "cubes_8":
+        sc = S.cubes_on_ground()
+    else:
+        sc = S.concat_scenes([S.cubes_on_ground(seed=1), S.cubes_on_ground(seed=2, lift=0.02),
+                              S.cubes_on_ground(seed=3, n=1, lift=0.01)])
+    p = north_star(dt=8e-3, ground_enabled=True, dhat=1e-3, kappa=1e7, k_contact=1e20)
+    say("case", case)
+    a, b = solver(sc, p), solver(sc, p.replace(debug=128))
+    trips = {"together": 0, "single": 0}
+    trials = {"together": 0, "single": 0}
+    for k in range(40):
+        sa, sb = a.step(), b.step()
+        assert np.array_equal(a.get_state()[0], b.get_state()[0]), f"step {k}"
+        for xa, xb in zip(sa, sb):
+            for key in ("newton_iters", "cg_iters", "residual_last", "energy", "alpha_min", "ls_floor_hits"):
+                assert xa[key] == xb[key], (k, key)
+        trips["together"] += sa[0]["ls_round_trips"]
+        trips["single"] += sb[0]["ls_round_trips"]
+        trials["together"] += sa[0]["ls_trials"]
+        trials["single"] += sb[0]["ls_trials"]
+    say(f"  line-search round trips {trips}, trials {trials} over 40 steps ({len(sa)} scenes)")
+    assert trips["together"] <= trips["single"]
+    if case == "cubes_8":
+        assert trips["together"] < trips["single"]
+    a.close()
+    b.close()
