@@ -335,6 +335,104 @@ __device__ inline bool wave_is_psd(const double* A, double* L, int m, int lane) 
     return true;
 }
 
+// Eigen-decomposition S = V diag(w) V^T of the symmetric 3 x 3 S by cyclic Jacobi rotations
+// (Golub and Van Loan, Matrix Computations, 8.5.2), one thread; S is overwritten.
+__device__ inline void sym3_eig(double S[3][3], double V[3][3], double w[3]) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) V[i][j] = (i == j) ? 1.0 : 0.0;
+    for (int sweep = 0; sweep < 12; ++sweep) {
+        const double off = S[0][1] * S[0][1] + S[0][2] * S[0][2] + S[1][2] * S[1][2];
+        const double dg = S[0][0] * S[0][0] + S[1][1] * S[1][1] + S[2][2] * S[2][2];
+        if (!(off > 1e-32 * dg)) break;
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            const int p = (r == 2) ? 1 : 0, q = (r == 0) ? 1 : 2;
+            const double apq = S[p][q];
+            if (apq == 0.0) continue;
+            // the rotation that zeroes S[p][q]: t = tan, the root of t^2 + 2 theta t - 1 of smaller size
+            const double th = (S[q][q] - S[p][p]) / (2.0 * apq);
+            const double t = (th >= 0.0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
+            const double c = 1.0 / sqrt(t * t + 1.0), sn = t * c;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const double skp = S[k][p], skq = S[k][q];
+                S[k][p] = c * skp - sn * skq;
+                S[k][q] = sn * skp + c * skq;
+            }
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const double spk = S[p][k], sqk = S[q][k];
+                S[p][k] = c * spk - sn * sqk;
+                S[q][k] = sn * spk + c * sqk;
+            }
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const double vkp = V[k][p], vkq = V[k][q];
+                V[k][p] = c * vkp - sn * vkq;
+                V[k][q] = sn * vkp + c * vkq;
+            }
+        }
+    }
+    w[0] = S[0][0];
+    w[1] = S[1][1];
+    w[2] = S[2][2];
+}
+
+// The negative eigenpairs of the orthogonality Hessian k4 * h (ortho.py:19-41; k4 = 4 kappa) of the
+// rows a_0..a_2 of the 3 x 3 matrix A (a[3 i + r] = A(i, r)), from its analytic eigensystem.
+// Psi = kappa sum_ij (a_i.a_j - delta_ij)^2 = kappa sum_k (sigma_k^2 - 1)^2 depends on A only
+// through its singular values (A = U diag(sigma) V^T), so its Hessian's eigenvectors are those of
+// isotropic energies (Smith, de Goes and Kim, Analytic Eigensystems for Isotropic Distortion
+// Energies, 2019): u_k v_k^T with eigenvalue k4 (3 sigma_k^2 - 1), and for i < j
+// (u_i v_j^T + u_j v_i^T) / sqrt 2 with k4 (sigma_i^2 + sigma_i sigma_j + sigma_j^2 - 1) and
+// (u_i v_j^T - u_j v_i^T) / sqrt 2 with k4 (sigma_i^2 - sigma_i sigma_j + sigma_j^2 - 1).  V and
+// sigma^2 from the eigen-decomposition of A^T A, u_k = A v_k / sigma_k.  Writes the eigenvalues
+// below zero into lam and their unit eigenvectors (9 entries each, in the order of a) into Q, and
+// returns how many; -1 when A is near singular (sigma_min^2 <= 1e-12 sigma_max^2), where u_k is not
+// determined accurately.  One thread.
+__device__ inline int ortho_negative_modes(const double* a, double k4, double* Q, double* lam) {
+    double S[3][3], V[3][3], w[3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) S[r][c] = a[r] * a[c] + a[3 + r] * a[3 + c] + a[6 + r] * a[6 + c];
+    sym3_eig(S, V, w);
+    const double wm = fmax(w[0], fmax(w[1], w[2]));
+    if (!(w[0] > 1e-12 * wm && w[1] > 1e-12 * wm && w[2] > 1e-12 * wm)) return -1;
+    double sg[3], U[3][3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        sg[k] = sqrt(w[k]);
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+            U[i][k] = (a[3 * i] * V[0][k] + a[3 * i + 1] * V[1][k] + a[3 * i + 2] * V[2][k]) / sg[k];
+    }
+    int n = 0;
+    for (int k = 0; k < 3; ++k) {
+        const double l = k4 * (3.0 * w[k] - 1.0);
+        if (l < 0.0) {
+            lam[n] = l;
+            for (int e = 0; e < 9; ++e) Q[9 * n + e] = U[e / 3][k] * V[e % 3][k];
+            ++n;
+        }
+    }
+    const double h = 0.70710678118654752440;
+    for (int i = 0; i < 3; ++i)
+        for (int j = i + 1; j < 3; ++j)
+            for (int sgn = 1; sgn >= -1; sgn -= 2) {
+                const double l = k4 * (w[i] + sgn * sg[i] * sg[j] + w[j] - 1.0);
+                if (l < 0.0) {
+                    lam[n] = l;
+                    for (int e = 0; e < 9; ++e)
+                        Q[9 * n + e] = h * (U[e / 3][i] * V[e % 3][j] + sgn * U[e / 3][j] * V[e % 3][i]);
+                    ++n;
+                }
+            }
+    return n;
+}
+
 // ---- 16 x 16 symmetric matrices on the matrix cores ---------------------------------------
 // One wave holds a symmetric matrix of order <= 16 (zero-padded) in the output layout of
 // v_mfma_{f64,f32}_16x16x4: lane l, slot u holds row R(l / 16, u), column l % 16, with

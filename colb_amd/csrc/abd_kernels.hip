@@ -568,9 +568,23 @@ __global__ __launch_bounds__(64) void k_body_assemble(Geo G, AbdParams P, const 
         }
         A[e] = 4.0 * kap * h;
     }
+    // PSD projection: the Hessian minus its negative eigenpairs, which come in closed form from the
+    // singular values of the body's 3 x 3 block (ortho_negative_modes; mode m in V[9 m .. 9 m + 8]);
+    // near singular blocks take the matrix-sign projection (after a Cholesky test)
+    __shared__ double lam_neg[9];
+    __shared__ int n_neg;
+    if (lane == 0) n_neg = ortho_negative_modes(a, 4.0 * kap, V, lam_neg);
     __syncthreads();
-    // near a rotation the ortho Hessian is already positive semidefinite: project only otherwise
-    if (!wave_is_psd(A, V, 9, lane)) wave_psd_project(A, 9, lane);
+    if (n_neg < 0) {
+        if (!wave_is_psd(A, V, 9, lane)) wave_psd_project(A, 9, lane);
+    } else if (n_neg > 0) {
+        for (int e = lane; e < 81; e += 64) {
+            const int R = e / 9, C = e % 9;
+            double h = A[e];
+            for (int m = 0; m < n_neg; ++m) h -= lam_neg[m] * (V[9 * m + R] * V[9 * m + C]);  // symmetric bits
+            A[e] = h;
+        }
+    }
     __syncthreads();
     for (int e = lane; e < 144; e += 64) {
         int R = e / 12, C = e % 12;
